@@ -1,0 +1,946 @@
+// C ABI of libdreams_hip: context, primes, programs, batch plans, CRT, top-k.
+#include <algorithm>
+#include <chrono>
+#include <cstdarg>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <numeric>
+
+#include "drx_ctx.h"
+#include "drx_device.h"
+
+using namespace drx;
+
+namespace drx {
+
+int fail(drx_ctx* ctx, int code, const char* fmt, ...) {
+  if (ctx) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    ctx->err = buf;
+  }
+  return code;
+}
+
+#define HIPCHK DRX_HIPCHK
+
+// ---- host modular arithmetic (p < 2^31)
+
+bool is_prime_u32(uint32_t n) {
+  if (n < 2) return false;
+  for (uint32_t q : {2u, 3u, 5u, 7u, 11u, 13u}) {
+    if (n % q == 0) return n == q;
+  }
+  uint32_t d = n - 1;
+  int s = 0;
+  while ((d & 1) == 0) {
+    d >>= 1;
+    ++s;
+  }
+  for (uint32_t a : {2u, 3u, 5u, 7u}) {  // deterministic below 3.2e9
+    uint64_t x = powmod(a, d, n);
+    if (x == 1 || x == n - 1) continue;
+    bool comp = true;
+    for (int r = 1; r < s; ++r) {
+      x = x * x % n;
+      if (x == n - 1) {
+        comp = false;
+        break;
+      }
+    }
+    if (comp) return false;
+  }
+  return true;
+}
+
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int upload_programs(drx_ctx* ctx) {
+  if (!ctx->progs_dirty) return DRX_OK;
+  const size_t np = ctx->progs.size();
+  std::vector<DevProgram> dp(np);
+  for (size_t i = 0; i < np; ++i) dp[i] = ctx->progs[i].dev;
+  HIPCHK(ctx, ctx->d_code.ensure(std::max<size_t>(1, ctx->h_code.size()) * 4));
+  HIPCHK(ctx, ctx->d_cres.ensure(std::max<size_t>(1, ctx->h_cres.size()) * 8));
+  HIPCHK(ctx, ctx->d_cflt.ensure(std::max<size_t>(1, ctx->h_cflt.size()) * 8));
+  HIPCHK(ctx, ctx->d_progs.ensure(std::max<size_t>(1, np) * sizeof(DevProgram)));
+  if (!ctx->h_code.empty())
+    HIPCHK(ctx, hipMemcpy(ctx->d_code.p, ctx->h_code.data(), ctx->h_code.size() * 4, hipMemcpyHostToDevice));
+  if (!ctx->h_cres.empty())
+    HIPCHK(ctx, hipMemcpy(ctx->d_cres.p, ctx->h_cres.data(), ctx->h_cres.size() * 8, hipMemcpyHostToDevice));
+  if (!ctx->h_cflt.empty())
+    HIPCHK(ctx, hipMemcpy(ctx->d_cflt.p, ctx->h_cflt.data(), ctx->h_cflt.size() * 8, hipMemcpyHostToDevice));
+  if (np) HIPCHK(ctx, hipMemcpy(ctx->d_progs.p, dp.data(), np * sizeof(DevProgram), hipMemcpyHostToDevice));
+  ctx->progs_dirty = false;
+  return DRX_OK;
+}
+
+// p_j^{-1} mod p_i for the first k primes (lazily extended).
+int ensure_inv_table(drx_ctx* ctx, int k) {
+  if (k <= ctx->kinv) return DRX_OK;
+  std::vector<uint32_t> t((size_t)k * k, 0);
+  for (int j = 0; j < k; ++j)
+    for (int i = 0; i < k; ++i)
+      if (i != j) t[(size_t)j * k + i] = powmod(ctx->primes[j] % ctx->primes[i], ctx->primes[i] - 2, ctx->primes[i]);
+  HIPCHK(ctx, ctx->d_inv.ensure(t.size() * 4));
+  HIPCHK(ctx, hipMemcpy(ctx->d_inv.p, t.data(), t.size() * 4, hipMemcpyHostToDevice));
+  ctx->inv_table.swap(t);
+  ctx->kinv = k;
+  return DRX_OK;
+}
+
+
+int axis_tables(drx_ctx* ctx, const int64_t* q, int dim, int K, int64_t depth, double* axr, int64_t* axi,
+                double* axd) {
+  for (int ax = 0; ax < dim; ++ax) {
+    const __int128 sn = q[4 * ax], sd = q[4 * ax + 1], tn = q[4 * ax + 2], td = q[4 * ax + 3];
+    const __int128 c0 = sn * td, c1 = tn * sd, den = sd * td;
+    if (den == 0) return fail(ctx, DRX_EINVAL, "axis %d: zero denominator", ax);
+    const __int128 lim = (__int128)1 << 62;
+    const __int128 last = c0 + (__int128)depth * c1;
+    if (c0 > lim || c0 < -lim || c1 > lim || c1 < -lim || den > lim || den < -lim || last > lim || last < -lim)
+      return fail(ctx, DRX_ERANGE, "axis %d: shift/trajectory too large for int64", ax);
+    double* base = axr + (size_t)ax * 3 * K;
+    for (int k = 0; k < K; ++k) {
+      const uint32_t p = ctx->primes[k];
+      const uint32_t dr = mod_i128(den, p);
+      base[k] = balanced(mod_i128(c0, p), p);
+      base[K + k] = balanced(mod_i128(c1, p), p);
+      // inverse of the combined denominator; 0 when p divides it (Fermat)
+      base[2 * K + k] = balanced(dr == 1 ? 1u : powmod(dr, p - 2, p), p);
+    }
+    axi[2 * ax] = (int64_t)c0;
+    axi[2 * ax + 1] = (int64_t)c1;
+    axd[ax] = (double)(int64_t)den;
+  }
+  return DRX_OK;
+}
+
+int init_tables(drx_ctx* ctx, const drx_walk& w, int m, const uint32_t* init_res_pool, const double* init_f_pool,
+                double* res, double* f) {
+  const int K = w.K;
+  for (int e = 0; e < m * m; ++e) {
+    f[e] = 0.0;
+    for (int k = 0; k < K; ++k) res[(size_t)e * K + k] = 0.0;
+  }
+  f[m * m] = 0.0;  // log_scale
+  if (w.init_kind == DRX_INIT_GIVEN) {
+    for (int e = 0; e < m * m; ++e) {
+      for (int k = 0; k < K; ++k) {
+        const uint32_t v = init_res_pool[w.init_off + (int64_t)e * K + k];
+        if (v >= ctx->primes[k]) return fail(ctx, DRX_EINVAL, "initial residue not canonical");
+        res[(size_t)e * K + k] = (double)v;
+      }
+      f[e] = init_f_pool[w.init_f_off + e];
+    }
+    return DRX_OK;
+  }
+  for (int d = 0; d < m; ++d) {
+    for (int k = 0; k < K; ++k) res[(size_t)(d * m + d) * K + k] = 1.0;
+    f[d * m + d] = 1.0;
+  }
+  if (w.init_kind == DRX_INIT_PCF) {
+    for (int k = 0; k < K; ++k) res[(size_t)1 * K + k] = (double)mod_i64(w.a0, ctx->primes[k]);
+    f[1] = (double)w.a0;
+  }
+  return DRX_OK;
+}
+}  // namespace drx
+
+// ======================================================================= plan
+struct drx_plan {
+  drx_ctx* ctx = nullptr;
+  drx_batch b{};
+  drx_launch_cfg cfg{};
+  int n = 0, kmax = 0, ld = 0;
+  struct Group {
+    int m;
+    int task_begin, task_end;
+    int span_begin, span_end;
+    int threads;
+    size_t lds;
+  };
+  std::vector<Group> groups;
+  std::vector<int> walk_order;  // span index -> walk
+  DBuf d_walks, d_tasks, d_spans, d_res0, d_fst0, d_res, d_fst, d_axr, d_axi, d_axd, d_cp, d_mf;
+  DBuf d_fout, d_rout, d_pout, d_digits, d_limbs, d_neg, d_topk, d_rowk;
+  KArgs args{};
+  GarnerArgs gargs{};
+  int ntask = 0;
+  double residue_steps = 0, mm_total = 0, ma_total = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<drx_plan*> plain;  // PLAIN mode: one sub-plan per walk
+  ~drx_plan() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (DBuf* d : {&d_walks, &d_tasks, &d_spans, &d_res0, &d_fst0, &d_res, &d_fst, &d_axr, &d_axi, &d_axd,
+                    &d_cp, &d_mf, &d_fout, &d_rout, &d_pout, &d_digits, &d_limbs, &d_neg, &d_topk, &d_rowk})
+      d->release();
+    for (drx_plan* p : plain) delete p;
+  }
+};
+
+namespace {
+
+template <typename T>
+int upload(drx_ctx* ctx, DBuf& d, const std::vector<T>& h) {
+  HIPCHK(ctx, d.ensure(std::max<size_t>(1, h.size()) * sizeof(T)));
+  if (!h.empty()) HIPCHK(ctx, hipMemcpy(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return DRX_OK;
+}
+
+int launch_sequence(drx_plan* pl, bool timed) {
+  drx_ctx* ctx = pl->ctx;
+  hipStream_t s = ctx->stream;
+  hipEvent_t* ev = ctx->ev;
+  if (timed) HIPCHK(ctx, hipEventRecord(ev[0], s));
+  for (const auto& g : pl->groups) {
+    KArgs a = pl->args;
+    a.tasks += g.task_begin;
+    HIPCHK(ctx, launch_walk(g.m, a, g.task_end - g.task_begin, g.threads, g.lds, s));
+  }
+  if (timed) HIPCHK(ctx, hipEventRecord(ev[1], s));
+  for (const auto& g : pl->groups) {
+    KArgs a = pl->args;
+    a.spans += g.span_begin;
+    HIPCHK(ctx, launch_combine(g.m, a, g.span_end - g.span_begin, s));
+  }
+  if (timed) HIPCHK(ctx, hipEventRecord(ev[2], s));
+  for (const auto& g : pl->groups) {
+    KArgs a = pl->args;
+    a.spans += g.span_begin;
+    HIPCHK(ctx, launch_shadow(g.m, a, g.span_end - g.span_begin, 1, s));
+  }
+  if (timed) HIPCHK(ctx, hipEventRecord(ev[3], s));
+  HIPCHK(ctx, launch_finalize(pl->args, (int)pl->walk_order.size(), pl->b.full != nullptr, s));
+  if (pl->b.crt_limbs) HIPCHK(ctx, launch_garner(pl->gargs, s));
+  if (pl->b.top_k > 0)
+    HIPCHK(ctx, launch_topk(pl->d_fout.as<double>() + 3, 6, pl->n, pl->b.top_k, pl->d_topk.as<int32_t>(), s));
+  if (timed) HIPCHK(ctx, hipEventRecord(ev[4], s));
+  return DRX_OK;
+}
+
+// The walk kernel's task array is laid out per walk in step order; groups
+// hold one matrix order each; within a group walks run longest first.
+int plan_build(drx_ctx* ctx, drx_plan* pl) {
+  const drx_batch& b = pl->b;
+  const int n = b.n;
+  const int np = (int)ctx->progs.size();
+  const int K_ctx = (int)ctx->primes.size();
+  int kmax = 1;
+  for (int i = 0; i < n; ++i) {
+    const drx_walk& w = b.walks[i];
+    if (w.prog < 0 || w.prog >= np) return fail(ctx, DRX_EINVAL, "walk %d: bad program id %d", i, w.prog);
+    if (w.K < 1 || w.K > K_ctx)
+      return fail(ctx, DRX_EINVAL, "walk %d: K=%d outside [1, %d] (primes set)", i, w.K, K_ctx);
+    const HostProgram& hp = ctx->progs[w.prog];
+    if (w.K > (int)hp.dev.kcap)
+      return fail(ctx, DRX_EINVAL, "walk %d: K=%d exceeds the program's constant table (%u)", i, w.K, hp.dev.kcap);
+    if (w.depth < 0 || w.depth > (int64_t)1 << 30)
+      return fail(ctx, DRX_EINVAL, "walk %d: depth %lld outside [0, 2^30]", i, (long long)w.depth);
+    if (w.init_kind < DRX_INIT_PCF || w.init_kind > DRX_INIT_GIVEN)
+      return fail(ctx, DRX_EINVAL, "walk %d: bad init kind", i);
+    if (w.init_kind == DRX_INIT_PCF && hp.dev.m != 2)
+      return fail(ctx, DRX_EINVAL, "walk %d: PCF initial matrix needs a 2x2 program", i);
+    if (w.init_kind == DRX_INIT_GIVEN && (!b.init_res_pool || !b.init_f_pool))
+      return fail(ctx, DRX_EINVAL, "walk %d: DRX_INIT_GIVEN without init pools", i);
+    if (hp.dev.dim > 0 && !b.axis_pool) return fail(ctx, DRX_EINVAL, "walk %d: no axis pool", i);
+    if (pow2_ceil(w.K) > 256) return fail(ctx, DRX_ERANGE, "walk %d: K=%d above the 256 primes of one workgroup", i, w.K);
+    kmax = std::max(kmax, w.K);
+  }
+  if (b.res_stride < kmax) return fail(ctx, DRX_EINVAL, "res_stride %d < max K %d", b.res_stride, kmax);
+  pl->n = n;
+  pl->kmax = kmax;
+  pl->ld = b.res_stride;
+
+  const int chunk_cfg = pl->cfg.chunk_steps > 0 ? pl->cfg.chunk_steps : 4096;
+  const int thr_cfg = pl->cfg.threads > 0 ? pl->cfg.threads : 256;
+  if ((thr_cfg & (thr_cfg - 1)) || thr_cfg < 32 || thr_cfg > 256)
+    return fail(ctx, DRX_EINVAL, "threads=%d must be a power of two in [32, 256]", thr_cfg);
+
+  // order: by m, then cost descending
+  std::vector<int> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  auto cost = [&](int i) {
+    const drx_walk& w = b.walks[i];
+    return (double)w.depth * (double)w.K * (1.0 + ctx->progs[w.prog].pk.code.size());
+  };
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    const int mx = ctx->progs[b.walks[x].prog].dev.m, my = ctx->progs[b.walks[y].prog].dev.m;
+    if (mx != my) return mx < my;
+    return cost(x) > cost(y);
+  });
+
+  std::vector<DevWalk> hw(n);
+  std::vector<DevTask> ht;
+  std::vector<DevSpan> hs;
+  std::vector<double> res0, fst0, axr, axd;
+  std::vector<int64_t> axi;
+  uint64_t cp_total = 0, mf_total = 0;
+  pl->groups.clear();
+  pl->walk_order.clear();
+  double rsteps = 0, mmt = 0, mat = 0;
+  for (int oi = 0; oi < n; ++oi) {
+    const int i = order[oi];
+    const drx_walk& w = b.walks[i];
+    const HostProgram& hp = ctx->progs[w.prog];
+    const int m = (int)hp.dev.m, dim = (int)hp.dev.dim, K = w.K;
+    DevWalk& dw = hw[i];
+    dw.prog = w.prog;
+    dw.K = K;
+    dw.m = m;
+    dw.dim = dim;
+    dw.target = w.target;
+    dw.res_off = res0.size();
+    dw.fst_off = fst0.size();
+    dw.axr_off = axr.size();
+    dw.axi_off = axi.size();
+    dw.axd_off = axd.size();
+    // initial state and axes
+    const size_t r0 = res0.size();
+    res0.resize(r0 + (size_t)m * m * K, 0.0);
+    const size_t f0 = fst0.size();
+    fst0.resize(f0 + m * m + 1, 0.0);
+    int rc0 = init_tables(ctx, w, m, b.init_res_pool, b.init_f_pool, res0.data() + r0, fst0.data() + f0);
+    if (rc0) return rc0;
+    const size_t a0 = axr.size(), i0 = axi.size(), d0 = axd.size();
+    axr.resize(a0 + (size_t)dim * 3 * K);
+    axi.resize(i0 + (size_t)dim * 2);
+    axd.resize(d0 + (size_t)dim);
+    if (dim > 0) {
+      rc0 = axis_tables(ctx, b.axis_pool + w.axis_off, dim, K, w.depth, axr.data() + a0, axi.data() + i0,
+                        axd.data() + d0);
+      if (rc0) return fail(ctx, rc0, "walk %d: %s", i, std::string(ctx->err).c_str());
+    }
+    // tasks
+    const bool plain = pl->cfg.mode == DRX_MODE_PLAIN;
+    const int64_t chunk = plain ? std::max<int64_t>(w.depth, 1) : chunk_cfg;
+    if (pl->groups.empty() || pl->groups.back().m != m) {
+      drx_plan::Group g{};
+      g.m = m;
+      g.task_begin = g.task_end = (int)ht.size();
+      g.span_begin = g.span_end = (int)hs.size();
+      g.threads = plain ? (int)pow2_ceil(K) : std::max<int>(thr_cfg, pow2_ceil(K));
+      g.lds = 0;
+      pl->groups.push_back(g);
+    }
+    drx_plan::Group& g = pl->groups.back();
+    const int threads = std::max<int>(g.threads, pow2_ceil(K));
+    g.threads = threads;
+    DevSpan sp{};
+    sp.walk = i;
+    sp.out_row = i;
+    sp.task_begin = (int)ht.size();
+    for (int64_t s0 = 0; s0 < w.depth; s0 += chunk) {
+      DevTask t{};
+      t.walk = i;
+      t.s_begin = s0;
+      t.s_end = std::min<int64_t>(w.depth, s0 + chunk);
+      t.cp_off = cp_total;
+      t.mf_off = mf_total;
+      cp_total += (uint64_t)m * m * K;
+      mf_total += (uint64_t)(t.s_end - t.s_begin) * m * m;
+      ht.push_back(t);
+    }
+    sp.task_end = (int)ht.size();
+    hs.push_back(sp);
+    g.task_end = (int)ht.size();
+    g.span_end = (int)hs.size();
+    pl->walk_order.push_back(i);
+    rsteps += (double)w.depth * K;
+    mmt += (double)w.depth * K * hp.mm_ops;
+    mat += (double)w.depth * K * hp.ma_ops;
+  }
+  // LDS per group (max over its programs), checked against the device limit
+  size_t lds_limit = 65536;
+  {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) == hipSuccess && v > 0)
+      lds_limit = std::min<size_t>((size_t)v, 65536);
+  }
+  for (auto& g : pl->groups) {
+    size_t need = 0;
+    for (int si = g.span_begin; si < g.span_end; ++si) {
+      const DevWalk& dw = hw[hs[si].walk];
+      const DevProgram& dp = ctx->progs[dw.prog].dev;
+      const size_t KP = pow2_ceil(dw.K);
+      const size_t nl = dp.const_lds ? dp.nconst : 0;
+      const size_t bytes = 8 * (nl * KP + nl + (size_t)dp.nslots * g.threads + (size_t)g.threads * g.m * g.m);
+      need = std::max(need, bytes);
+    }
+    if (need > lds_limit)
+      return fail(ctx, DRX_ERANGE, "program needs %zu bytes of LDS (limit %zu); use fewer threads", need, lds_limit);
+    g.lds = need;
+  }
+  pl->ntask = (int)ht.size();
+  pl->residue_steps = rsteps;
+  pl->mm_total = mmt;
+  pl->ma_total = mat;
+
+  int rc;
+  if ((rc = upload(ctx, pl->d_walks, hw)) || (rc = upload(ctx, pl->d_tasks, ht)) || (rc = upload(ctx, pl->d_spans, hs)) ||
+      (rc = upload(ctx, pl->d_res0, res0)) || (rc = upload(ctx, pl->d_fst0, fst0)) || (rc = upload(ctx, pl->d_axr, axr)) ||
+      (rc = upload(ctx, pl->d_axi, axi)) || (rc = upload(ctx, pl->d_axd, axd)))
+    return rc;
+  HIPCHK(ctx, pl->d_res.ensure(std::max<size_t>(1, res0.size()) * 8));
+  HIPCHK(ctx, pl->d_fst.ensure(std::max<size_t>(1, fst0.size()) * 8));
+  HIPCHK(ctx, pl->d_cp.ensure(std::max<uint64_t>(1, cp_total) * 8));
+  HIPCHK(ctx, pl->d_mf.ensure(std::max<uint64_t>(1, mf_total) * 8));
+  HIPCHK(ctx, pl->d_fout.ensure((size_t)std::max(1, n) * 6 * 8));
+  HIPCHK(ctx, pl->d_rout.ensure((size_t)std::max(1, n) * 2 * pl->ld * 8));
+  if (b.full) HIPCHK(ctx, pl->d_pout.ensure((size_t)std::max(1, n) * 16 * pl->ld * 8));
+  if (b.top_k > 0) {
+    if (b.top_k > n) return fail(ctx, DRX_EINVAL, "top_k=%d > n=%d", b.top_k, n);
+    HIPCHK(ctx, pl->d_topk.ensure((size_t)b.top_k * 4));
+  }
+  if (b.crt_limbs) {
+    if ((rc = ensure_inv_table(ctx, kmax))) return rc;
+    std::vector<int32_t> rowk(2 * (size_t)n);
+    for (int i = 0; i < n; ++i) rowk[2 * i] = rowk[2 * i + 1] = b.walks[i].K;
+    if ((rc = upload(ctx, pl->d_rowk, rowk))) return rc;
+    HIPCHK(ctx, pl->d_digits.ensure((size_t)std::max(1, n) * 2 * pl->ld * 4));
+    HIPCHK(ctx, pl->d_limbs.ensure((size_t)std::max(1, n) * 2 * pl->ld * 4));
+    HIPCHK(ctx, pl->d_neg.ensure((size_t)std::max(1, n) * 2));
+  }
+  if ((rc = upload_programs(ctx))) return rc;
+
+  KArgs& a = pl->args;
+  a.tasks = pl->d_tasks.as<DevTask>();
+  a.walks = pl->d_walks.as<DevWalk>();
+  a.progs = ctx->d_progs.as<DevProgram>();
+  a.spans = pl->d_spans.as<DevSpan>();
+  a.code = ctx->d_code.as<uint32_t>();
+  a.cres = ctx->d_cres.as<double>();
+  a.cflt = ctx->d_cflt.as<double>();
+  a.prime_p = ctx->d_prime_p.as<double>();
+  a.prime_pinv = ctx->d_prime_pinv.as<double>();
+  a.prime_e = ctx->d_prime_e.as<uint32_t>();
+  a.axr = pl->d_axr.as<double>();
+  a.axi = pl->d_axi.as<int64_t>();
+  a.axd = pl->d_axd.as<double>();
+  a.res_in = pl->d_res0.as<double>();
+  a.fst_in = pl->d_fst0.as<double>();
+  a.res = pl->d_res.as<double>();
+  a.fst = pl->d_fst.as<double>();
+  a.cp = pl->d_cp.as<double>();
+  a.mf = pl->d_mf.as<double>();
+  a.fout = pl->d_fout.as<double>();
+  a.rout = pl->d_rout.as<int64_t>();
+  a.pout = b.full ? pl->d_pout.as<int64_t>() : nullptr;
+  a.fcompact = nullptr;
+  a.out_ld = (uint64_t)pl->ld;
+  a.with_fp = 1;
+  GarnerArgs& g = pl->gargs;
+  g.res = pl->d_rout.as<int64_t>();
+  g.res_stride = (uint64_t)pl->ld;
+  g.n = b.crt_limbs ? 2 * n : 0;
+  g.K = kmax;
+  g.row_k = b.crt_limbs ? pl->d_rowk.as<int32_t>() : nullptr;
+  g.prime_p = ctx->d_prime_p.as<double>();
+  g.prime_pinv = ctx->d_prime_pinv.as<double>();
+  g.prime_u = ctx->d_prime_u.as<uint32_t>();
+  g.inv_table = ctx->d_inv.as<uint32_t>();
+  g.inv_ld = (uint64_t)ctx->kinv;
+  g.digits = pl->d_digits.as<uint32_t>();
+  g.limbs = pl->d_limbs.as<uint32_t>();
+  g.neg = pl->d_neg.as<uint8_t>();
+
+  if (pl->cfg.use_graph) {
+    HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const int rc2 = launch_sequence(pl, false);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (rc2) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc2;
+    }
+    if (e != hipSuccess) return fail(ctx, DRX_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    pl->graph = graph;
+    HIPCHK(ctx, hipGraphInstantiate(&pl->gexec, graph, nullptr, nullptr, 0));
+  }
+  return DRX_OK;
+}
+
+int plan_fetch(drx_plan* pl) {
+  drx_ctx* ctx = pl->ctx;
+  const drx_batch& b = pl->b;
+  hipStream_t s = ctx->stream;
+  const int n = pl->n;
+  if (b.out) HIPCHK(ctx, hipMemcpyAsync(b.out, pl->d_fout.p, (size_t)n * 6 * 8, hipMemcpyDeviceToHost, s));
+  if (b.residues)
+    HIPCHK(ctx, hipMemcpyAsync(b.residues, pl->d_rout.p, (size_t)n * 2 * pl->ld * 8, hipMemcpyDeviceToHost, s));
+  if (b.full) HIPCHK(ctx, hipMemcpyAsync(b.full, pl->d_pout.p, (size_t)n * 16 * pl->ld * 8, hipMemcpyDeviceToHost, s));
+  if (b.crt_limbs) {
+    HIPCHK(ctx, hipMemcpyAsync(b.crt_limbs, pl->d_limbs.p, (size_t)n * 2 * pl->ld * 4, hipMemcpyDeviceToHost, s));
+    if (b.crt_neg) HIPCHK(ctx, hipMemcpyAsync(b.crt_neg, pl->d_neg.p, (size_t)n * 2, hipMemcpyDeviceToHost, s));
+  }
+  if (b.top_k > 0 && b.topk_idx)
+    HIPCHK(ctx, hipMemcpyAsync(b.topk_idx, pl->d_topk.p, (size_t)b.top_k * 4, hipMemcpyDeviceToHost, s));
+  return DRX_OK;
+}
+
+int plan_run_one(drx_plan* pl, bool timed) {
+  drx_ctx* ctx = pl->ctx;
+  if (pl->d_pout.p && pl->b.full)
+    HIPCHK(ctx, hipMemsetAsync(pl->d_pout.p, 0, (size_t)pl->n * 16 * pl->ld * 8, ctx->stream));
+  if (pl->b.crt_limbs)
+    HIPCHK(ctx, hipMemsetAsync(pl->d_limbs.p, 0, (size_t)pl->n * 2 * pl->ld * 4, ctx->stream));
+  if (pl->gexec) {
+    if (timed) HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    HIPCHK(ctx, hipGraphLaunch(pl->gexec, ctx->stream));
+    if (timed) HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+  } else {
+    int rc = launch_sequence(pl, timed);
+    if (rc) return rc;
+  }
+  return plan_fetch(pl);
+}
+
+}  // namespace
+
+// ==================================================================== C ABI
+extern "C" {
+
+int drx_abi_version(void) { return DRX_ABI_VERSION; }
+
+int drx_device_count(int* n) {
+  if (!n) return DRX_EINVAL;
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+  *n = c;
+  return DRX_OK;
+}
+
+int drx_ctx_create(int device, drx_ctx** out) {
+  if (!out) return DRX_EINVAL;
+  *out = nullptr;
+  int nd = 0;
+  if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0) return DRX_EHIP;
+  if (device < 0 || device >= nd) return DRX_EINVAL;
+  drx_ctx* ctx = new (std::nothrow) drx_ctx();
+  if (!ctx) return DRX_ENOMEM;
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return DRX_EHIP;
+  }
+  for (auto& e : ctx->ev) {
+    if (hipEventCreate(&e) != hipSuccess) {
+      drx_ctx_destroy(ctx);
+      return DRX_EHIP;
+    }
+  }
+  *out = ctx;
+  return DRX_OK;
+}
+
+void drx_ctx_destroy(drx_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (DBuf* d : {&ctx->d_prime_p, &ctx->d_prime_pinv, &ctx->d_prime_e, &ctx->d_prime_u, &ctx->d_inv, &ctx->d_code,
+                  &ctx->d_cres, &ctx->d_cflt, &ctx->d_progs, &ctx->d_tmp0, &ctx->d_tmp1, &ctx->d_tmp2, &ctx->d_tmp3})
+    d->release();
+  for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* drx_last_error(const drx_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int drx_set_primes(drx_ctx* ctx, const int64_t* primes, int K) {
+  if (!ctx) return DRX_EINVAL;
+  if (!primes || K < 1 || K > kMaxK) return fail(ctx, DRX_EINVAL, "K=%d outside [1, %d]", K, kMaxK);
+  std::vector<uint32_t> pr(K);
+  for (int i = 0; i < K; ++i) {
+    const int64_t p = primes[i];
+    if (p <= ((int64_t)1 << 30) || p >= ((int64_t)1 << 31) || !is_prime_u32((uint32_t)p))
+      return fail(ctx, DRX_EINVAL, "prime %d (%lld) is not a prime in (2^30, 2^31)", i, (long long)p);
+    for (int j = 0; j < i; ++j)
+      if (pr[j] == (uint32_t)p) return fail(ctx, DRX_EINVAL, "prime %lld repeated", (long long)p);
+    pr[i] = (uint32_t)p;
+  }
+  if (!ctx->primes.empty() && (size_t)K >= ctx->primes.size() &&
+      std::equal(ctx->primes.begin(), ctx->primes.end(), pr.begin()) && (size_t)K == ctx->primes.size())
+    return DRX_OK;
+  if (!ctx->progs.empty()) {
+    // programs hold residues for the old table; only an extension is allowed
+    if ((size_t)K < ctx->primes.size() || !std::equal(ctx->primes.begin(), ctx->primes.end(), pr.begin()))
+      return fail(ctx, DRX_ESTATE, "primes changed while programs are loaded; release programs first");
+  }
+  std::vector<double> pd(K), pinv(K);
+  std::vector<uint32_t> pe(K);
+  for (int i = 0; i < K; ++i) {
+    pd[i] = (double)pr[i];
+    pinv[i] = 1.0 / (double)pr[i];
+    pe[i] = pr[i] - 2;
+  }
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, ctx->d_prime_p.ensure(K * 8));
+  HIPCHK(ctx, ctx->d_prime_pinv.ensure(K * 8));
+  HIPCHK(ctx, ctx->d_prime_e.ensure(K * 4));
+  HIPCHK(ctx, ctx->d_prime_u.ensure(K * 4));
+  HIPCHK(ctx, hipMemcpy(ctx->d_prime_p.p, pd.data(), K * 8, hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemcpy(ctx->d_prime_pinv.p, pinv.data(), K * 8, hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemcpy(ctx->d_prime_e.p, pe.data(), K * 4, hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemcpy(ctx->d_prime_u.p, pr.data(), K * 4, hipMemcpyHostToDevice));
+  ctx->primes.swap(pr);
+  ctx->kinv = 0;
+  return DRX_OK;
+}
+
+int drx_program_upload(drx_ctx* ctx, const uint32_t* ops, int n_ops, int m, int dim, const int64_t* consts,
+                       const uint32_t* const_res, const double* const_f, int n_const, int kcap, int* prog_id) {
+  if (!ctx) return DRX_EINVAL;
+  if (!prog_id || (n_ops > 0 && !ops) || n_ops < 0 || n_const < 0)
+    return fail(ctx, DRX_EINVAL, "bad program arguments");
+  if (ctx->primes.empty()) return fail(ctx, DRX_ESTATE, "set primes before uploading programs");
+  if (n_const > 0 && !consts && !(const_res && const_f))
+    return fail(ctx, DRX_EINVAL, "constants missing (give consts or const_res + const_f)");
+  const int K = (int)ctx->primes.size();
+  const bool given = const_res != nullptr;
+  const int kc = given ? std::min(kcap, K) : K;
+  if (given && kcap < 1) return fail(ctx, DRX_EINVAL, "kcap must be >= 1");
+  std::vector<char> is_one(n_const, 0), is_zero(n_const, 0);
+  for (int i = 0; i < n_const; ++i) {
+    if (given) {
+      bool one = const_f[i] == 1.0, zero = const_f[i] == 0.0;
+      for (int k = 0; k < kc; ++k) {
+        one = one && const_res[(size_t)i * kcap + k] == 1u;
+        zero = zero && const_res[(size_t)i * kcap + k] == 0u;
+      }
+      is_one[i] = one;
+      is_zero[i] = zero;
+    } else {
+      is_one[i] = consts[i] == 1;
+      is_zero[i] = consts[i] == 0;
+    }
+  }
+  HostProgram hp;
+  const std::string err = pack_program(ops, n_ops, m, dim, n_const, is_one, is_zero, &hp.pk);
+  if (!err.empty()) return fail(ctx, DRX_EINVAL, "%s", err.c_str());
+  const PackedProgram& pk = hp.pk;
+  const int nc = (int)pk.const_slots.size();
+  DevProgram& d = hp.dev;
+  d.code_off = (uint32_t)ctx->h_code.size();
+  d.ncode = (uint32_t)(pk.code.size() / 2);
+  d.nconst = (uint32_t)nc;
+  d.kcap = (uint32_t)kc;
+  d.cres_off = ctx->h_cres.size();
+  d.cflt_off = ctx->h_cflt.size();
+  d.m = (uint32_t)m;
+  d.dim = (uint32_t)dim;
+  d.kinds = pk.kinds;
+  d.colred = pk.colred;
+  d.const_lds = (uint32_t)pk.const_lds;
+  d.nslots = (uint32_t)pk.nslots;
+  ctx->h_code.insert(ctx->h_code.end(), pk.code.begin(), pk.code.end());
+  if (ctx->h_code.size() % 2) ctx->h_code.push_back(0);  // keep 8-byte alignment
+  for (int s = 0; s < nc; ++s) {
+    const int ci = pk.const_slots[s];
+    for (int k = 0; k < kc; ++k) {
+      const uint32_t p = ctx->primes[k];
+      const uint32_t r = given ? const_res[(size_t)ci * kcap + k] : mod_i64(consts[ci], p);
+      if (r >= p) return fail(ctx, DRX_EINVAL, "constant %d: residue not canonical", ci);
+      ctx->h_cres.push_back(balanced(r, p));
+    }
+    ctx->h_cflt.push_back(given ? const_f[ci] : (double)consts[ci]);
+  }
+  // executed modular work per step: program + structured P*M
+  int mmo = 0, mao = 0;
+  for (size_t i = 0; i < pk.code.size(); i += 2) {
+    const uint32_t op = pk.code[i] & 255u;
+    if (op == D_MUL || op == D_SQR) mmo += 1;
+    else if (op == D_CUB) mmo += 2;
+    else if (op == D_INV) mmo += 60;
+    else if (op == D_ADD || op == D_SUB || op == D_NEG) mao += 1;
+  }
+  for (int j = 0; j < m; ++j) {
+    int terms = 0;
+    for (int k = 0; k < m; ++k) {
+      const uint32_t kd = (pk.kinds >> (2 * (k * m + j))) & 3u;
+      if (kd == E_GEN) mmo += m;
+      if (kd != E_ZERO) ++terms;
+    }
+    if (terms > 1) mao += m * (terms - 1);
+  }
+  hp.mm_ops = mmo;
+  hp.ma_ops = mao;
+  ctx->progs.push_back(std::move(hp));
+  ctx->progs_dirty = true;
+  *prog_id = (int)ctx->progs.size() - 1;
+  return DRX_OK;
+}
+
+int drx_program_info(drx_ctx* ctx, int prog_id, int* n_dev_ops, int* n_const_used, uint32_t* entry_kinds) {
+  if (!ctx) return DRX_EINVAL;
+  if (prog_id < 0 || prog_id >= (int)ctx->progs.size()) return fail(ctx, DRX_EINVAL, "bad program id");
+  const HostProgram& hp = ctx->progs[prog_id];
+  if (n_dev_ops) *n_dev_ops = (int)hp.dev.ncode;
+  if (n_const_used) *n_const_used = (int)hp.dev.nconst;
+  if (entry_kinds) *entry_kinds = hp.dev.kinds;
+  return DRX_OK;
+}
+
+int drx_program_pack(const uint32_t* ops, int n_ops, int m, int dim, const int64_t* consts, int n_const,
+                     uint32_t* words, int max_words, int* const_slots, int max_slots, int64_t* meta6, char* err,
+                     int err_len) {
+  if ((n_ops > 0 && !ops) || n_ops < 0 || n_const < 0 || (n_const > 0 && !consts) || !meta6) return DRX_EINVAL;
+  std::vector<char> is_one(n_const, 0), is_zero(n_const, 0);
+  for (int i = 0; i < n_const; ++i) {
+    is_one[i] = consts[i] == 1;
+    is_zero[i] = consts[i] == 0;
+  }
+  PackedProgram pk;
+  const std::string msg = pack_program(ops, n_ops, m, dim, n_const, is_one, is_zero, &pk);
+  if (!msg.empty()) {
+    if (err && err_len > 0) std::snprintf(err, (size_t)err_len, "%s", msg.c_str());
+    return DRX_EINVAL;
+  }
+  meta6[0] = (int64_t)pk.code.size();
+  meta6[1] = pk.kinds;
+  meta6[2] = pk.colred;
+  meta6[3] = pk.const_lds;
+  meta6[4] = pk.nslots;
+  meta6[5] = (int64_t)pk.const_slots.size();
+  if (words) std::copy(pk.code.begin(), pk.code.begin() + std::min<size_t>(pk.code.size(), max_words), words);
+  if (const_slots)
+    for (size_t i = 0; i < pk.const_slots.size() && (int)i < max_slots; ++i) const_slots[i] = pk.const_slots[i];
+  return DRX_OK;
+}
+
+int drx_program_release_all(drx_ctx* ctx) {
+  if (!ctx) return DRX_EINVAL;
+  ctx->progs.clear();
+  ctx->h_code.clear();
+  ctx->h_cres.clear();
+  ctx->h_cflt.clear();
+  ctx->progs_dirty = true;
+  return DRX_OK;
+}
+
+int drx_plan_create(drx_ctx* ctx, const drx_batch* b, const drx_launch_cfg* cfg, drx_plan** out) {
+  if (!ctx) return DRX_EINVAL;
+  if (!b || !out || b->n < 0 || (b->n > 0 && (!b->walks || !b->out || !b->residues)))
+    return fail(ctx, DRX_EINVAL, "bad batch arguments");
+  *out = nullptr;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  drx_plan* pl = new (std::nothrow) drx_plan();
+  if (!pl) return fail(ctx, DRX_ENOMEM, "out of host memory");
+  pl->ctx = ctx;
+  pl->b = *b;
+  if (cfg) pl->cfg = *cfg;
+  if (pl->cfg.mode == DRX_MODE_PLAIN) {
+    // one sub-plan per walk: each run launches and synchronises one walk
+    pl->n = b->n;
+    pl->ld = b->res_stride;
+    for (int i = 0; i < b->n; ++i) {
+      drx_batch sb = *b;
+      sb.walks = b->walks + i;
+      sb.n = 1;
+      sb.out = b->out + i;
+      sb.residues = b->residues + (size_t)i * 2 * b->res_stride;
+      sb.full = b->full ? b->full + (size_t)i * 16 * b->res_stride : nullptr;
+      sb.crt_limbs = b->crt_limbs ? b->crt_limbs + (size_t)i * 2 * b->res_stride : nullptr;
+      sb.crt_neg = b->crt_neg ? b->crt_neg + 2 * i : nullptr;
+      sb.top_k = 0;
+      sb.topk_idx = nullptr;
+      drx_plan* sp = new (std::nothrow) drx_plan();
+      if (!sp) {
+        delete pl;
+        return fail(ctx, DRX_ENOMEM, "out of host memory");
+      }
+      sp->ctx = ctx;
+      sp->b = sb;
+      sp->cfg = pl->cfg;
+      sp->cfg.use_graph = 0;
+      pl->plain.push_back(sp);
+      const int rc = plan_build(ctx, sp);
+      if (rc) {
+        delete pl;
+        return rc;
+      }
+      pl->residue_steps += sp->residue_steps;
+      pl->mm_total += sp->mm_total;
+      pl->ma_total += sp->ma_total;
+      pl->ntask += sp->ntask;
+    }
+    if (b->top_k > 0) {
+      if (b->top_k > b->n) {
+        delete pl;
+        return fail(ctx, DRX_EINVAL, "top_k=%d > n=%d", b->top_k, b->n);
+      }
+      if (ctx->d_tmp0.ensure((size_t)b->n * 8) != hipSuccess || ctx->d_tmp1.ensure((size_t)b->top_k * 4) != hipSuccess) {
+        delete pl;
+        return fail(ctx, DRX_ENOMEM, "device allocation failed");
+      }
+    }
+    *out = pl;
+    return DRX_OK;
+  }
+  const int rc = plan_build(ctx, pl);
+  if (rc) {
+    delete pl;
+    return rc;
+  }
+  *out = pl;
+  return DRX_OK;
+}
+
+int drx_plan_run(drx_plan* pl) {
+  if (!pl) return DRX_EINVAL;
+  drx_ctx* ctx = pl->ctx;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  Timing tm;
+  if (!pl->plain.empty() || pl->cfg.mode == DRX_MODE_PLAIN) {
+    const double t0 = now_s();
+    for (drx_plan* sp : pl->plain) {
+      int rc = plan_run_one(sp, true);
+      if (rc) return rc;
+      HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+      float a = 0, b2 = 0, c = 0, d = 0;
+      (void)hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]);
+      (void)hipEventElapsedTime(&b2, ctx->ev[1], ctx->ev[2]);
+      (void)hipEventElapsedTime(&c, ctx->ev[2], ctx->ev[3]);
+      (void)hipEventElapsedTime(&d, ctx->ev[3], ctx->ev[4]);
+      tm.walk_ms += a;
+      tm.combine_ms += b2;
+      tm.shadow_ms += c;
+      tm.tail_ms += d;
+      tm.walk_launches += (int)sp->groups.size();
+    }
+    if (pl->b.top_k > 0) {
+      // rank the per-walk deltas on the device like the batched path does
+      std::vector<double> dl(pl->n);
+      for (int i = 0; i < pl->n; ++i) dl[i] = pl->b.out[i].delta_float;
+      HIPCHK(ctx, hipMemcpyAsync(ctx->d_tmp0.p, dl.data(), dl.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+      HIPCHK(ctx, launch_topk(ctx->d_tmp0.as<double>(), 1, pl->n, pl->b.top_k, ctx->d_tmp1.as<int32_t>(), ctx->stream));
+      HIPCHK(ctx, hipMemcpyAsync(pl->b.topk_idx, ctx->d_tmp1.p, (size_t)pl->b.top_k * 4, hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    tm.total_ms = (now_s() - t0) * 1e3;
+    ctx->timing = tm;
+    return DRX_OK;
+  }
+  int rc = plan_run_one(pl, true);
+  if (rc) return rc;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  float a = 0, b2 = 0, c = 0, d = 0, tot = 0;
+  if (!pl->gexec) {
+    (void)hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]);
+    (void)hipEventElapsedTime(&b2, ctx->ev[1], ctx->ev[2]);
+    (void)hipEventElapsedTime(&c, ctx->ev[2], ctx->ev[3]);
+    (void)hipEventElapsedTime(&d, ctx->ev[3], ctx->ev[4]);
+  }
+  (void)hipEventElapsedTime(&tot, ctx->ev[0], ctx->ev[4]);
+  tm.walk_ms = a;
+  tm.combine_ms = b2;
+  tm.shadow_ms = c;
+  tm.tail_ms = d;
+  tm.total_ms = tot;
+  tm.walk_launches = (int)pl->groups.size();
+  ctx->timing = tm;
+  return DRX_OK;
+}
+
+int drx_plan_stats(const drx_plan* pl, double* s4) {
+  if (!pl || !s4) return DRX_EINVAL;
+  s4[0] = pl->residue_steps;
+  s4[1] = pl->mm_total;
+  s4[2] = pl->ma_total;
+  s4[3] = pl->ntask;
+  return DRX_OK;
+}
+
+void drx_plan_destroy(drx_plan* pl) { delete pl; }
+
+int drx_walk_batch(drx_ctx* ctx, const drx_batch* b, const drx_launch_cfg* cfg) {
+  drx_plan* pl = nullptr;
+  int rc = drx_plan_create(ctx, b, cfg, &pl);
+  if (rc) return rc;
+  rc = drx_plan_run(pl);
+  drx_plan_destroy(pl);
+  return rc;
+}
+
+int drx_crt(drx_ctx* ctx, const int64_t* residues, int n, int K, int stride, uint32_t* limbs, uint8_t* neg) {
+  if (!ctx) return DRX_EINVAL;
+  if (n < 0 || K < 1 || stride < K || (n > 0 && (!residues || !limbs || !neg)))
+    return fail(ctx, DRX_EINVAL, "bad CRT arguments");
+  if (K > (int)ctx->primes.size()) return fail(ctx, DRX_EINVAL, "K=%d exceeds the %zu primes set", K, ctx->primes.size());
+  if (n == 0) return DRX_OK;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < K; ++k) {
+      const int64_t r = residues[(size_t)i * stride + k];
+      if (r < 0 || r >= (int64_t)ctx->primes[k]) return fail(ctx, DRX_EINVAL, "residue row %d prime %d not canonical", i, k);
+    }
+  int rc = ensure_inv_table(ctx, K);
+  if (rc) return rc;
+  const size_t rows = (size_t)n * stride;
+  HIPCHK(ctx, ctx->d_tmp0.ensure(rows * 8));
+  HIPCHK(ctx, ctx->d_tmp1.ensure(rows * 4));
+  HIPCHK(ctx, ctx->d_tmp2.ensure(rows * 4));
+  HIPCHK(ctx, ctx->d_tmp3.ensure((size_t)n));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->d_tmp0.p, residues, rows * 8, hipMemcpyHostToDevice, ctx->stream));
+  GarnerArgs g{};
+  g.res = ctx->d_tmp0.as<int64_t>();
+  g.res_stride = (uint64_t)stride;
+  g.n = n;
+  g.K = K;
+  g.row_k = nullptr;
+  g.prime_p = ctx->d_prime_p.as<double>();
+  g.prime_pinv = ctx->d_prime_pinv.as<double>();
+  g.prime_u = ctx->d_prime_u.as<uint32_t>();
+  g.inv_table = ctx->d_inv.as<uint32_t>();
+  g.inv_ld = (uint64_t)ctx->kinv;
+  g.digits = ctx->d_tmp1.as<uint32_t>();
+  g.limbs = ctx->d_tmp2.as<uint32_t>();
+  g.neg = ctx->d_tmp3.as<uint8_t>();
+  HIPCHK(ctx, launch_garner(g, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(limbs, ctx->d_tmp2.p, rows * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(neg, ctx->d_tmp3.p, (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return DRX_OK;
+}
+
+int drx_topk(drx_ctx* ctx, const double* values, int n, int k, int32_t* idx) {
+  if (!ctx) return DRX_EINVAL;
+  if (n < 0 || k < 0 || k > n || (n > 0 && !values) || (k > 0 && !idx))
+    return fail(ctx, DRX_EINVAL, "bad top-k arguments");
+  if (k == 0) return DRX_OK;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, ctx->d_tmp0.ensure((size_t)n * 8));
+  HIPCHK(ctx, ctx->d_tmp1.ensure((size_t)k * 4));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->d_tmp0.p, values, (size_t)n * 8, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, launch_topk(ctx->d_tmp0.as<double>(), 1, n, k, ctx->d_tmp1.as<int32_t>(), ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(idx, ctx->d_tmp1.p, (size_t)k * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return DRX_OK;
+}
+
+int drx_last_timing(const drx_ctx* ctx, double* t6) {
+  if (!ctx || !t6) return DRX_EINVAL;
+  t6[0] = ctx->timing.walk_ms;
+  t6[1] = ctx->timing.combine_ms;
+  t6[2] = ctx->timing.shadow_ms;
+  t6[3] = ctx->timing.tail_ms;
+  t6[4] = ctx->timing.total_ms;
+  t6[5] = ctx->timing.walk_launches;
+  return DRX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,134 @@
+// Internal context / plan structures of libdreams_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/dreams_hip.h"
+#include "drx_kernels.h"
+#include "drx_pack.h"
+#include "drx_types.h"
+
+namespace drx {
+
+// Growable device buffer.
+struct DBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= cap) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = bytes < 256 ? 256 : bytes + bytes / 4;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+      p = nullptr;
+      want = bytes;
+      e = hipMalloc(&p, want);
+      if (e != hipSuccess) {
+        p = nullptr;
+        return e;
+      }
+    }
+    cap = want;
+    return hipSuccess;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  template <typename T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+struct HostProgram {
+  PackedProgram pk;
+  DevProgram dev;
+  // executed work per step, for the roofline: modular multiplies / adds
+  int mm_ops = 0, ma_ops = 0;
+};
+
+struct Timing {
+  double walk_ms = 0, combine_ms = 0, shadow_ms = 0, tail_ms = 0, total_ms = 0;
+  int walk_launches = 0;
+};
+
+}  // namespace drx
+
+struct drx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // primes
+  std::vector<uint32_t> primes;
+  drx::DBuf d_prime_p, d_prime_pinv, d_prime_e, d_prime_u;
+  std::vector<uint32_t> inv_table;  // [kinv][kinv]: inv_table[j*kinv+i] = p_j^{-1} mod p_i
+  int kinv = 0;
+  drx::DBuf d_inv;
+  // programs
+  std::vector<drx::HostProgram> progs;
+  std::vector<uint32_t> h_code;
+  std::vector<double> h_cres, h_cflt;
+  drx::DBuf d_code, d_cres, d_cflt, d_progs;
+  bool progs_dirty = true;
+  // scratch for standalone CRT / top-k
+  drx::DBuf d_tmp0, d_tmp1, d_tmp2, d_tmp3;
+  drx::Timing timing;
+  hipEvent_t ev[12] = {};
+  std::mutex mu;
+};
+
+// Shared host helpers (drx_api.cpp).
+namespace drx {
+int fail(drx_ctx* ctx, int code, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
+bool is_prime_u32(uint32_t n);
+double now_s();
+int upload_programs(drx_ctx* ctx);
+int ensure_inv_table(drx_ctx* ctx, int k);
+
+inline uint32_t mod_i64(int64_t c, uint32_t p) {
+  int64_t r = c % (int64_t)p;
+  return (uint32_t)(r < 0 ? r + p : r);
+}
+inline uint32_t mod_i128(__int128 c, uint32_t p) {
+  __int128 r = c % (__int128)p;
+  return (uint32_t)(r < 0 ? r + p : r);
+}
+inline uint32_t powmod(uint32_t a, uint64_t e, uint32_t p) {
+  uint64_t r = 1 % p, x = a % p;
+  while (e) {
+    if (e & 1) r = r * x % p;
+    x = x * x % p;
+    e >>= 1;
+  }
+  return (uint32_t)r;
+}
+inline double balanced(uint32_t r, uint32_t p) { return r > p / 2 ? (double)r - (double)p : (double)r; }
+inline uint32_t pow2_ceil(uint32_t k) {
+  uint32_t kp = 1;
+  while (kp < k) kp <<= 1;
+  return kp;
+}
+
+// Residue and FP64 tables of one walk's axes: axis i at step s is
+// (c0 + s*c1) * inv(den) with c0 = sn*td, c1 = tn*sd, den = sd*td.
+// axr gets [dim][3][K] balanced residues, axi [dim][2] int64, axd [dim].
+int axis_tables(drx_ctx* ctx, const int64_t* q, int dim, int K, int64_t depth, double* axr,
+                int64_t* axi, double* axd);
+// Initial matrix of a walk: res [m*m][K] (exact residues as doubles), f [m*m].
+int init_tables(drx_ctx* ctx, const drx_walk& w, int m, const uint32_t* init_res_pool,
+                const double* init_f_pool, double* res, double* f);
+}  // namespace drx
+
+#define DRX_HIPCHK(ctx, expr)                                                                   \
+  do {                                                                                          \
+    hipError_t _e = (expr);                                                                     \
+    if (_e != hipSuccess) return ::drx::fail((ctx), DRX_EHIP, "%s: %s", #expr, hipGetErrorString(_e)); \
+  } while (0)
+

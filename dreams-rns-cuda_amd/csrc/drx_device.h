@@ -1,0 +1,114 @@
+// Device-side arithmetic and bytecode encoding for the walk kernels (gfx950).
+//
+// Residue arithmetic runs on the FP64 pipes, which are full rate on CDNA4:
+// a residue is an integer held exactly in a double, kept in a "balanced",
+// lazily reduced range. For a prime p in [2^30, 2^31) with pinv = fl(1/p):
+//
+//   mm(a, b):  h = fl(a*b); l = fma(a, b, -h)        (a*b = h + l exactly)
+//              q = rint(h * pinv); r = fma(-q, p, h) + l
+//
+// r == a*b - q*p exactly whenever |a|, |b| <= 2^40 (then |h - q p| < 2^53 and
+// the fma is exact), and |r| <= p/2 + 1.5 * 2^-52 * |a b| < 2^31. Additions
+// are a single fadd with no reduction; the host-side packer tracks magnitude
+// bounds and inserts RED (in-place reduction) before any value could exceed
+// 2^40. Canonical residues in [0, p) are produced only at the very end.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace drx {
+
+// ----------------------------------------------------------------- encoding
+// A device instruction is two 32-bit words (read by one scalar load):
+//   w0: [7:0] op    [9:8] dst bank   [25:10] dst index
+//   w1: [2:0] a bank [15:3] a index  [18:16] b bank  [31:19] b index
+// Every instruction also leaves its result in the accumulator A, so a value
+// consumed only by the next instruction never touches a register bank.
+enum DevOp : uint32_t {
+  D_ADD = 1,   // a + b
+  D_SUB = 2,   // a - b
+  D_MUL = 3,   // a * b
+  D_NEG = 4,   // -a
+  D_SQR = 5,   // a^2
+  D_CUB = 6,   // a^3
+  D_INV = 7,   // a^(p-2) mod p          | float: 1/a if |a| > 1e-300 else 0
+  D_RED = 8,   // a mod p (balanced)     | float: a
+  D_MOV = 9,   // a
+};
+// Destination banks.
+enum DstBank : uint32_t { DB_R = 0, DB_E = 1, DB_S = 2, DB_NONE = 3 };
+// Source banks: R temps (VGPR), X axes 0..6 / per-step increments 8..14 /
+// step 15 (VGPR),
+// C constants (VGPR), L constants (LDS), S temps (LDS), A accumulator, Z zero.
+enum SrcBank : uint32_t { SB_R = 0, SB_X = 1, SB_C = 2, SB_L = 3, SB_S = 4, SB_A = 5, SB_Z = 6 };
+
+constexpr int kBank = 16;       // VGPR bank width (doubles)
+constexpr int kStepSlot = 15;   // X[15] holds the step
+constexpr int kIncSlot = 8;     // X[8 + i] holds axis i's per-step increment
+
+__host__ __device__ constexpr uint32_t enc_w0(uint32_t op, uint32_t dbank, uint32_t dst) {
+  return op | (dbank << 8) | (dst << 10);
+}
+__host__ __device__ constexpr uint32_t enc_w1(uint32_t abank, uint32_t a, uint32_t bbank, uint32_t b) {
+  return abank | (a << 3) | (bbank << 16) | (b << 19);
+}
+
+// Matrix entry kinds (2 bits per entry, entry e = row*m + col).
+enum EntryKind : uint32_t { E_ZERO = 0, E_ONE = 1, E_GEN = 2 };
+
+// ------------------------------------------------------------ modular FP64
+__device__ __forceinline__ double mm(double a, double b, double p, double pinv) {
+  const double h = a * b;
+  const double l = __builtin_fma(a, b, -h);
+  const double q = __builtin_rint(h * pinv);
+  const double r = __builtin_fma(-q, p, h);
+  return r + l;
+}
+
+// In-place balanced reduction, exact for |x| < 2^52.
+__device__ __forceinline__ double red(double x, double p, double pinv) {
+  const double q = __builtin_rint(x * pinv);
+  return __builtin_fma(-q, p, x);
+}
+
+// Canonical residue in [0, p) of an exact integer |x| < 2^52.
+__device__ __forceinline__ double canon(double x, double p, double pinv) {
+  double r = __builtin_fma(-__builtin_floor(x * pinv), p, x);
+  r = (r < 0.0) ? r + p : r;
+  r = (r >= p) ? r - p : r;
+  return r;
+}
+
+// a^(p-2) mod p by left-to-right square-and-multiply over the 31 bits of
+// e = p - 2 (bit 30 is always set for p in [2^30, 2^31)). Fermat gives
+// 0 for a == 0 mod p, the reference's INV(0) = 0.
+__device__ __forceinline__ double inv_fermat(double a, double p, double pinv, uint32_t e) {
+  double r = a;  // bit 30
+#pragma unroll 1
+  for (int bit = 29; bit >= 0; --bit) {
+    r = mm(r, r, p, pinv);
+    const double t = mm(r, a, p, pinv);
+    r = ((e >> bit) & 1u) ? t : r;
+  }
+  return r;
+}
+
+// --------------------------------------------------------------- float ops
+// Correctly rounded x^3: x^2 = h + l and h*x = c + e exactly (FMA), then
+// x^3 = c + (e + l*x), whose inner sum is ~2^-52 of c, so the final add
+// rounds the exact cube correctly except within ~2^-50 ulp of a tie.
+__device__ __forceinline__ double cube_cr(double x) {
+  const double h = x * x;
+  const double l = __builtin_fma(x, x, -h);
+  const double c = h * x;
+  const double e = __builtin_fma(h, x, -c);
+  const double t = __builtin_fma(l, x, e);
+  return c + t;
+}
+
+__device__ __forceinline__ double finv(double x) {
+  return (__builtin_fabs(x) > 1e-300) ? 1.0 / x : 0.0;
+}
+
+}  // namespace drx

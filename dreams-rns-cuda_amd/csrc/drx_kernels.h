@@ -1,0 +1,63 @@
+// Kernel argument blocks and launchers (internal to libdreams_hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "drx_types.h"
+
+namespace drx {
+
+// Pointers into the context's device pools for one batch launch.
+struct KArgs {
+  const DevTask* tasks;
+  const DevWalk* walks;
+  const DevProgram* progs;
+  const DevSpan* spans;
+  const uint32_t* code;
+  const double* cres;
+  const double* cflt;
+  const double* prime_p;
+  const double* prime_pinv;
+  const uint32_t* prime_e;
+  const double* axr;
+  const int64_t* axi;
+  const double* axd;
+  const double* res_in;  // walk state before this launch: residues
+  const double* fst_in;  // walk state before this launch: P_f and log_scale
+  double* res;        // walk state after: residues (may alias res_in)
+  double* fst;        // walk state after: P_f and log_scale (may alias fst_in)
+  double* cp;         // chunk products
+  double* mf;         // shadow matrices M_f(s)
+  double* fout;       // per walk: p_f, q_f, log_scale, delta, log_q, (pad)
+  int64_t* rout;      // [out_row][2][out_ld]: p residues then q residues
+  int64_t* pout;      // optional [out_row][16][out_ld] full state residues
+  double* fcompact;   // optional [out_row][6]: copy of fout of the finalized walks
+  uint64_t out_ld;
+  int with_fp;
+};
+
+struct GarnerArgs {
+  const int64_t* res;     // [n] rows of canonical residues, row stride res_stride
+  uint64_t res_stride;
+  int n;
+  int K;                  // primes per row unless row_k is given
+  const int32_t* row_k;   // optional per-row prime count
+  const double* prime_p;
+  const double* prime_pinv;
+  const uint32_t* prime_u;
+  const uint32_t* inv_table;  // inv_table[j * inv_ld + i] = p_j^{-1} mod p_i
+  uint64_t inv_ld;
+  uint32_t* digits;       // scratch [n][res_stride]
+  uint32_t* limbs;        // [n][res_stride] little-endian magnitude of the centred value
+  uint8_t* neg;           // [n] 1 if the centred value is negative
+};
+
+hipError_t launch_walk(int m, const KArgs& a, int ntask, int threads, size_t lds, hipStream_t s);
+hipError_t launch_combine(int m, const KArgs& a, int nspan, hipStream_t s);
+hipError_t launch_shadow(int m, const KArgs& a, int nspan, int finalize, hipStream_t s);
+hipError_t launch_finalize(const KArgs& a, int nspan, int full, hipStream_t s);
+hipError_t launch_garner(const GarnerArgs& g, hipStream_t s);
+hipError_t launch_topk(const double* d, int stride, int n, int k, int32_t* out, hipStream_t s);
+
+}  // namespace drx

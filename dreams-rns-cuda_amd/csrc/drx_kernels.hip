@@ -1,0 +1,516 @@
+// Walk kernels for gfx950: bytecode interpreter over residue channels, chunked
+// P*M products with an ordered LDS tree combine, FP64 shadow, finalisation,
+// Garner CRT and top-k.
+#include "drx_kernels.h"
+
+#include "drx_device.h"
+
+namespace drx {
+
+// ------------------------------------------------------------- interpreter
+// One instruction stream per workgroup (the whole block walks one program),
+// so fetch, decode and bank dispatch are wave-uniform scalar work. Banks are
+// 16-wide vectors of doubles indexed with uniform indices (VGPR index mode).
+typedef double v16d __attribute__((ext_vector_type(16)));
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// Entry bank: m*m doubles rounded up to a power of two.
+template <int M> struct EBank { typedef v16d type; };
+template <> struct EBank<2> { typedef v4d type; };
+
+struct Lane {
+  double p, pinv;
+  uint32_t pe;
+  uint32_t k;        // prime slot within the block's prime stripe
+  uint32_t tid, T;   // for LDS temp slots
+};
+
+template <bool FP>
+__device__ __forceinline__ double fetch(uint32_t bank, uint32_t i, const v16d& R, const v16d& X,
+                                        const v16d& Cv, double acc, const double* __restrict__ ldc,
+                                        uint32_t ldc_ld, const double* __restrict__ lds_s,
+                                        const Lane& ln) {
+  switch (bank) {
+    case SB_R: return R[i & 15u];
+    case SB_X: return X[i & 15u];
+    case SB_C: return Cv[i & 15u];
+    case SB_L: return FP ? ldc[i] : ldc[i * ldc_ld + ln.k];
+    case SB_S: return lds_s[i * ln.T + ln.tid];
+    case SB_A: return acc;
+    default: return 0.0;
+  }
+}
+
+template <bool FP, typename EV>
+__device__ __forceinline__ void run_program(const uint2* __restrict__ code, uint32_t n, v16d& R,
+                                            EV& E, const v16d& X, const v16d& Cv,
+                                            const double* __restrict__ ldc, uint32_t ldc_ld,
+                                            double* __restrict__ lds_s, const Lane& ln) {
+  double acc = 0.0;
+#pragma unroll 1
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint2 w = code[i];
+    const uint32_t op = w.x & 255u;
+    const double x = fetch<FP>(w.y & 7u, (w.y >> 3) & 8191u, R, X, Cv, acc, ldc, ldc_ld, lds_s, ln);
+    double r;
+    if (op == D_MUL || op == D_ADD || op == D_SUB) {
+      const double y = fetch<FP>((w.y >> 16) & 7u, w.y >> 19, R, X, Cv, acc, ldc, ldc_ld, lds_s, ln);
+      if (op == D_MUL) r = FP ? x * y : mm(x, y, ln.p, ln.pinv);
+      else r = (op == D_ADD) ? x + y : x - y;
+    } else {
+      switch (op) {
+        case D_NEG: r = -x; break;
+        case D_SQR: r = FP ? x * x : mm(x, x, ln.p, ln.pinv); break;
+        case D_CUB: r = FP ? cube_cr(x) : mm(mm(x, x, ln.p, ln.pinv), x, ln.p, ln.pinv); break;
+        case D_INV: r = FP ? finv(x) : inv_fermat(x, ln.p, ln.pinv, ln.pe); break;
+        case D_RED: r = FP ? x : red(x, ln.p, ln.pinv); break;
+        default: r = x; break;  // D_MOV
+      }
+    }
+    const uint32_t db = (w.x >> 8) & 3u, d = w.x >> 10;
+    if (db == DB_R) R[d & 15u] = r;
+    else if (db == DB_E) E[d & (sizeof(EV) / sizeof(double) - 1)] = r;
+    else if (db == DB_S) lds_s[d * ln.T + ln.tid] = r;
+    acc = r;
+  }
+}
+
+// P <- P * M where M's entries sit in E[0 .. M*M) and `kinds` marks the
+// statically zero / one entries (uniform branches).
+template <int M, typename EV>
+__device__ __forceinline__ void step_matmul(double (&P)[M * M], const EV& E, uint32_t kinds,
+                                            uint32_t colred, double p, double pinv) {
+  double C[M * M];
+#pragma unroll
+  for (int i = 0; i < M; ++i) {
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < M; ++k) {
+        const uint32_t kd = (kinds >> (2 * (k * M + j))) & 3u;
+        if (kd == E_GEN) acc += mm(P[i * M + k], E[k * M + j], p, pinv);
+        else if (kd == E_ONE) acc += P[i * M + k];
+      }
+      if ((colred >> j) & 1u) acc = red(acc, p, pinv);
+      C[i * M + j] = acc;
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < M * M; ++e) P[e] = C[e];
+}
+
+// P <- P * Q, both general.
+template <int M>
+__device__ __forceinline__ void gen_matmul(double (&P)[M * M], const double (&Q)[M * M], double p,
+                                           double pinv) {
+  double C[M * M];
+#pragma unroll
+  for (int i = 0; i < M; ++i)
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      double acc = mm(P[i * M], Q[j], p, pinv);
+#pragma unroll
+      for (int k = 1; k < M; ++k) acc += mm(P[i * M + k], Q[k * M + j], p, pinv);
+      C[i * M + j] = acc;
+    }
+#pragma unroll
+  for (int e = 0; e < M * M; ++e) P[e] = C[e];
+}
+
+__device__ __forceinline__ uint32_t pow2_at_least(uint32_t k) {
+  uint32_t kp = 1;
+  while (kp < k) kp <<= 1;
+  return kp;
+}
+
+// ----------------------------------------------------------- walk kernel
+// One workgroup per task (a step range of one walk). Lane (c, k): sub-chunk
+// c of the range, prime k. Each lane multiplies M(s) over its sub-chunk; the
+// C sub-chunk products are then combined in order by a log2(C)-level tree in
+// LDS (the walk product is associative). Afterwards every lane evaluates the
+// FP64 matrices M_f(s) for the shadow, strided over the whole range.
+//
+// LDS: [nconst][KP] constant residues + [nconst] FP64 constants (bank L),
+// [nslots][T] temps (bank S), then the tree buffer [C][M*M][KP].
+template <int M>
+__global__ void __launch_bounds__(256) k_walk_chunk(const KArgs a) {
+  extern __shared__ double lds[];
+  const DevTask t = a.tasks[blockIdx.x];
+  const DevWalk w = a.walks[t.walk];
+  const DevProgram pg = a.progs[w.prog];
+  const uint32_t T = blockDim.x;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t K = (uint32_t)w.K;
+  const uint32_t KP = pow2_at_least(K);
+  const uint32_t C = T / KP;
+  const uint32_t k = tid % KP;
+  const uint32_t c = tid / KP;
+  const bool active = k < K;
+  const uint32_t kk = active ? k : 0u;
+  Lane ln;
+  ln.p = a.prime_p[kk];
+  ln.pinv = a.prime_pinv[kk];
+  ln.pe = a.prime_e[kk];
+  ln.k = k;
+  ln.tid = tid;
+  ln.T = T;
+  const double p = ln.p, pinv = ln.pinv;
+  const uint2* __restrict__ code = reinterpret_cast<const uint2*>(a.code + pg.code_off);
+  const uint32_t dim = pg.dim;
+  const uint32_t nl = pg.const_lds ? pg.nconst : 0u;
+
+  double* ldc = lds;
+  double* ldcf = ldc + (uint64_t)nl * KP;
+  double* lds_s = ldcf + nl;
+  double* tree = lds_s + (uint64_t)pg.nslots * T;
+  for (uint32_t i = tid; i < nl * KP; i += T) {
+    const uint32_t ci = i / KP, ck = i % KP;
+    ldc[i] = (ck < K) ? a.cres[pg.cres_off + (uint64_t)ci * pg.kcap + ck] : 0.0;
+  }
+  for (uint32_t i = tid; i < nl; i += T) ldcf[i] = a.cflt[pg.cflt_off + i];
+  __syncthreads();
+
+  v16d R = 0.0, X = 0.0, Cv = 0.0;
+  typename EBank<M>::type E = 0.0;
+
+  // ---- residue pass
+  const int64_t L = t.s_end - t.s_begin;
+  const int64_t len = (L + C - 1) / C;
+  int64_t sb = t.s_begin + (int64_t)c * len;
+  if (sb > t.s_end || !active) sb = t.s_end;
+  int64_t se = sb + len;
+  if (se > t.s_end) se = t.s_end;
+
+  if (!pg.const_lds) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+      if ((uint32_t)i < pg.nconst) Cv[i] = a.cres[pg.cres_off + (uint64_t)i * pg.kcap + kk];
+  }
+#pragma unroll
+  for (int ax = 0; ax < kMaxDim; ++ax) {
+    if ((uint32_t)ax < dim) {
+      const double* q = a.axr + w.axr_off + (uint64_t)ax * 3 * K;
+      const double c0 = q[kk], c1 = q[K + kk], c2 = q[2 * K + kk];
+      const double x0 = red(c0 + mm((double)sb, c1, p, pinv), p, pinv);
+      X[ax] = mm(x0, c2, p, pinv);
+      X[kIncSlot + ax] = mm(c1, c2, p, pinv);
+    }
+  }
+
+  double P[M * M];
+#pragma unroll
+  for (int e = 0; e < M * M; ++e) P[e] = (e % (M + 1) == 0) ? 1.0 : 0.0;
+
+  for (int64_t s = sb; s < se; ++s) {
+    X[kStepSlot] = (double)s;
+    run_program<false>(code, pg.ncode, R, E, X, Cv, ldc, KP, lds_s, ln);
+    step_matmul<M>(P, E, pg.kinds, pg.colred, p, pinv);
+#pragma unroll
+    for (int ax = 0; ax < kMaxDim; ++ax)
+      if ((uint32_t)ax < dim) X[ax] = red(X[ax] + X[kIncSlot + ax], p, pinv);
+  }
+
+  // ---- ordered tree combine of the C sub-chunk products
+#pragma unroll
+  for (int e = 0; e < M * M; ++e) tree[((uint64_t)c * M * M + e) * KP + k] = P[e];
+  for (uint32_t stride = 1; stride < C; stride <<= 1) {
+    __syncthreads();
+    if ((c % (2 * stride)) == 0 && c + stride < C) {
+      double Q[M * M];
+#pragma unroll
+      for (int e = 0; e < M * M; ++e) Q[e] = tree[((uint64_t)(c + stride) * M * M + e) * KP + k];
+      gen_matmul<M>(P, Q, p, pinv);
+#pragma unroll
+      for (int e = 0; e < M * M; ++e) tree[((uint64_t)c * M * M + e) * KP + k] = P[e];
+    }
+  }
+  if (c == 0 && active) {
+#pragma unroll
+    for (int e = 0; e < M * M; ++e) a.cp[t.cp_off + (uint64_t)e * K + k] = P[e];
+  }
+
+  // ---- FP64 shadow matrices M_f(s), one step per lane
+  if (!a.with_fp) return;
+  if (!pg.const_lds) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+      if ((uint32_t)i < pg.nconst) Cv[i] = a.cflt[pg.cflt_off + i];
+  }
+  for (int64_t s = t.s_begin + tid; s < t.s_end; s += T) {
+#pragma unroll
+    for (int ax = 0; ax < kMaxDim; ++ax) {
+      if ((uint32_t)ax < dim) {
+        const int64_t* q = a.axi + w.axi_off + 2 * ax;
+        X[ax] = (double)(q[0] + s * q[1]) / a.axd[w.axd_off + ax];
+      }
+    }
+    X[kStepSlot] = (double)s;
+    run_program<true>(code, pg.ncode, R, E, X, Cv, ldcf, 0, lds_s, ln);
+    double* out = a.mf + t.mf_off + (uint64_t)(s - t.s_begin) * M * M;
+#pragma unroll
+    for (int e = 0; e < M * M; ++e) {
+      const uint32_t kd = (pg.kinds >> (2 * e)) & 3u;
+      out[e] = (kd == E_GEN) ? E[e] : (kd == E_ONE ? 1.0 : 0.0);
+    }
+  }
+}
+
+// --------------------------------------------------------- ordered combine
+// state <- state * cp[t0] * cp[t1] * ... over the walk's tasks (step order).
+template <int M>
+__global__ void k_combine(const KArgs a) {
+  const DevSpan sp = a.spans[blockIdx.x];
+  const DevWalk w = a.walks[sp.walk];
+  const uint32_t K = (uint32_t)w.K;
+  for (uint32_t k = threadIdx.x; k < K; k += blockDim.x) {
+    const double p = a.prime_p[k], pinv = a.prime_pinv[k];
+    double S[M * M];
+#pragma unroll
+    for (int e = 0; e < M * M; ++e) S[e] = a.res_in[w.res_off + (uint64_t)e * K + k];
+    for (int ti = sp.task_begin; ti < sp.task_end; ++ti) {
+      const uint64_t off = a.tasks[ti].cp_off;
+      double Q[M * M];
+#pragma unroll
+      for (int e = 0; e < M * M; ++e) Q[e] = a.cp[off + (uint64_t)e * K + k];
+      gen_matmul<M>(S, Q, p, pinv);
+    }
+#pragma unroll
+    for (int e = 0; e < M * M; ++e) a.res[w.res_off + (uint64_t)e * K + k] = S[e];
+  }
+}
+
+// ------------------------------------------------------------ FP64 shadow
+// One lane per walk: P_f <- P_f @ M_f(s) for every step in order, each entry
+// a sequential FMA chain acc = a0*b0; acc = fma(ak, bk, acc); then
+// mx = max|P_f| (NaN propagates) and, if mx > 1e10, P_f /= mx and
+// log_scale += log(mx). Finishes with delta_float against the walk target.
+template <int M>
+__global__ void k_shadow(const KArgs a, int nspan, int finalize) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nspan) return;
+  const DevSpan sp = a.spans[i];
+  const DevWalk w = a.walks[sp.walk];
+  const double* st_in = a.fst_in + w.fst_off;
+  double* st = a.fst + w.fst_off;
+  double P[M * M];
+#pragma unroll
+  for (int e = 0; e < M * M; ++e) P[e] = st_in[e];
+  double ls = st_in[M * M];
+  for (int ti = sp.task_begin; ti < sp.task_end; ++ti) {
+    const DevTask t = a.tasks[ti];
+    const double* mf = a.mf + t.mf_off;
+    for (int64_t s = t.s_begin; s < t.s_end; ++s, mf += M * M) {
+      double Mf[M * M];
+#pragma unroll
+      for (int e = 0; e < M * M; ++e) Mf[e] = mf[e];
+      double C[M * M];
+#pragma unroll
+      for (int r = 0; r < M; ++r)
+#pragma unroll
+        for (int cc = 0; cc < M; ++cc) {
+          double acc = P[r * M] * Mf[cc];
+#pragma unroll
+          for (int q = 1; q < M; ++q) acc = __builtin_fma(P[r * M + q], Mf[q * M + cc], acc);
+          C[r * M + cc] = acc;
+        }
+      double mx = 0.0;
+      bool nan = false;
+#pragma unroll
+      for (int e = 0; e < M * M; ++e) {
+        const double v = __builtin_fabs(C[e]);
+        nan |= (v != v);
+        mx = (v > mx) ? v : mx;
+      }
+      if (!nan && mx > 1e10) {
+#pragma unroll
+        for (int e = 0; e < M * M; ++e) C[e] = C[e] / mx;
+        ls += log(mx);
+      }
+#pragma unroll
+      for (int e = 0; e < M * M; ++e) P[e] = C[e];
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < M * M; ++e) st[e] = P[e];
+  st[M * M] = ls;
+  if (finalize) {
+    // delta_float (runner.py:86-100)
+    const double pf = P[M - 1], qf = P[M * M - 1];
+    double delta, logq;
+    if (!isfinite(pf) || !isfinite(qf) || __builtin_fabs(qf) < 1e-300) {
+      delta = -1e10;
+      logq = 0.0;
+    } else {
+      const double err = __builtin_fabs(pf / qf - w.target);
+      logq = ls + log(__builtin_fabs(qf));
+      if (err < 1e-300) delta = 100.0;
+      else if (logq <= 0.0) delta = -1e10;
+      else delta = -(1.0 + log(err) / logq);
+    }
+    double* o = a.fout + (uint64_t)sp.walk * 6;
+    o[0] = pf;
+    o[1] = qf;
+    o[2] = ls;
+    o[3] = delta;
+    o[4] = logq;
+  }
+}
+
+// --------------------------------------------------------------- finalize
+// Canonical residues of the last column (p = P[0][m-1], q = P[m-1][m-1])
+// and, when requested, of the whole state matrix.
+__global__ void k_finalize(const KArgs a, int full) {
+  const DevSpan sp = a.spans[blockIdx.x];
+  const DevWalk w = a.walks[sp.walk];
+  const uint32_t K = (uint32_t)w.K, m = (uint32_t)w.m;
+  const double* S = a.res + w.res_off;
+  int64_t* ro = a.rout + (uint64_t)sp.out_row * 2 * a.out_ld;
+  if (a.fcompact && threadIdx.x < 6)
+    a.fcompact[(uint64_t)sp.out_row * 6 + threadIdx.x] = a.fout[(uint64_t)sp.walk * 6 + threadIdx.x];
+  for (uint32_t k = threadIdx.x; k < K; k += blockDim.x) {
+    const double p = a.prime_p[k], pinv = a.prime_pinv[k];
+    ro[k] = (int64_t)canon(S[(uint64_t)(m - 1) * K + k], p, pinv);
+    ro[a.out_ld + k] = (int64_t)canon(S[(uint64_t)(m * m - 1) * K + k], p, pinv);
+    if (full) {
+      int64_t* po = a.pout + (uint64_t)sp.out_row * 16 * a.out_ld;
+      for (uint32_t e = 0; e < m * m; ++e)
+        po[(uint64_t)e * a.out_ld + k] = (int64_t)canon(S[(uint64_t)e * K + k], p, pinv);
+    }
+  }
+}
+
+// -------------------------------------------------------------- Garner CRT
+// One thread per (walk, value): mixed-radix digits v_i of x in [0, M), the
+// sign of the centred value (x > (M-1)/2 <=> digits compare above
+// (p_i - 1)/2 from the top), the digits of M - x when negative, and the
+// binary limbs by Horner x = (...(v_{K-1} p_{K-2} + v_{K-2}) ...) p_0 + v_0.
+__global__ void k_garner(const GarnerArgs g) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= g.n) return;
+  const int K = g.row_k ? g.row_k[id] : g.K;
+  const int64_t* r = g.res + (uint64_t)id * g.res_stride;
+  uint32_t* v = g.digits + (uint64_t)id * g.res_stride;
+  uint32_t* limb = g.limbs + (uint64_t)id * g.res_stride;
+  for (int i = 0; i < K; ++i) {
+    const double p = g.prime_p[i], pinv = g.prime_pinv[i];
+    double t = canon((double)r[i], p, pinv);
+    for (int j = 0; j < i; ++j) {
+      // t = (t - v_j) * p_j^{-1} mod p_i
+      t = canon(mm(t - (double)v[j], (double)g.inv_table[(uint64_t)j * g.inv_ld + i], p, pinv), p, pinv);
+    }
+    v[i] = (uint32_t)t;
+  }
+  // sign: compare digits with (p_i - 1) / 2 from the most significant end
+  int neg = 0;
+  for (int i = K - 1; i >= 0; --i) {
+    const uint32_t h = (g.prime_u[i] - 1u) >> 1;
+    if (v[i] != h) {
+      neg = v[i] > h;
+      break;
+    }
+  }
+  if (neg) {  // digits of M - x = (M - 1 - x) + 1
+    uint32_t carry = 1;
+    for (int i = 0; i < K; ++i) {
+      uint32_t d = g.prime_u[i] - 1u - v[i] + carry;
+      carry = (d == g.prime_u[i]) ? 1u : 0u;
+      v[i] = carry ? 0u : d;
+    }
+  }
+  // Horner into little-endian 32-bit limbs
+  for (int i = 0; i < K; ++i) limb[i] = 0;
+  int n = 0;
+  for (int i = K - 1; i >= 0; --i) {
+    const uint64_t pm = (i == K - 1) ? 0ull : (uint64_t)g.prime_u[i];
+    uint64_t carry = v[i];
+    if (i == K - 1) {
+      limb[0] = v[i];
+      n = 1;
+      continue;
+    }
+    for (int l = 0; l < n; ++l) {
+      const uint64_t z = (uint64_t)limb[l] * pm + carry;
+      limb[l] = (uint32_t)z;
+      carry = z >> 32;
+    }
+    if (carry && n < K) limb[n++] = (uint32_t)carry;
+  }
+  g.neg[id] = (uint8_t)neg;
+}
+
+// ------------------------------------------------------------------ top-k
+// rank(i) = #{j : d_j > d_i or (d_j == d_i and j < i)}, NaN ranked last;
+// element i is written at out[rank] when rank < k. LDS-tiled O(n^2 / T).
+__device__ __forceinline__ bool better(double dj, int j, double di, int i) {
+  const bool nj = dj != dj, ni = di != di;
+  if (nj != ni) return ni;            // non-NaN beats NaN
+  if (!nj && dj != di) return dj > di;
+  return j < i;
+}
+
+__global__ void k_topk(const double* __restrict__ d, int stride, int n, int k,
+                       int32_t* __restrict__ out) {
+  __shared__ double tile[256];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double di = (i < n) ? d[(uint64_t)i * stride] : 0.0;
+  int rank = 0;
+  for (int base = 0; base < n; base += 256) {
+    __syncthreads();
+    if (base + (int)threadIdx.x < n) tile[threadIdx.x] = d[(uint64_t)(base + threadIdx.x) * stride];
+    __syncthreads();
+    const int lim = (n - base < 256) ? n - base : 256;
+    for (int j = 0; j < lim; ++j) rank += better(tile[j], base + j, di, i) ? 1 : 0;
+  }
+  if (i < n && rank < k) out[rank] = i;
+}
+
+// ---------------------------------------------------------------- launchers
+hipError_t launch_walk(int m, const KArgs& a, int ntask, int threads, size_t lds, hipStream_t s) {
+  if (ntask <= 0) return hipSuccess;
+  if (m == 2) hipLaunchKernelGGL(k_walk_chunk<2>, dim3(ntask), dim3(threads), lds, s, a);
+  else if (m == 3) hipLaunchKernelGGL(k_walk_chunk<3>, dim3(ntask), dim3(threads), lds, s, a);
+  else if (m == 4) hipLaunchKernelGGL(k_walk_chunk<4>, dim3(ntask), dim3(threads), lds, s, a);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_combine(int m, const KArgs& a, int nspan, hipStream_t s) {
+  if (nspan <= 0) return hipSuccess;
+  if (m == 2) hipLaunchKernelGGL(k_combine<2>, dim3(nspan), dim3(64), 0, s, a);
+  else if (m == 3) hipLaunchKernelGGL(k_combine<3>, dim3(nspan), dim3(64), 0, s, a);
+  else if (m == 4) hipLaunchKernelGGL(k_combine<4>, dim3(nspan), dim3(64), 0, s, a);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_shadow(int m, const KArgs& a, int nspan, int finalize, hipStream_t s) {
+  if (nspan <= 0) return hipSuccess;
+  const int tb = 64, nb = (nspan + tb - 1) / tb;
+  if (m == 2) hipLaunchKernelGGL(k_shadow<2>, dim3(nb), dim3(tb), 0, s, a, nspan, finalize);
+  else if (m == 3) hipLaunchKernelGGL(k_shadow<3>, dim3(nb), dim3(tb), 0, s, a, nspan, finalize);
+  else if (m == 4) hipLaunchKernelGGL(k_shadow<4>, dim3(nb), dim3(tb), 0, s, a, nspan, finalize);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize(const KArgs& a, int nspan, int full, hipStream_t s) {
+  if (nspan <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_finalize, dim3(nspan), dim3(64), 0, s, a, full);
+  return hipGetLastError();
+}
+
+hipError_t launch_garner(const GarnerArgs& g, hipStream_t s) {
+  if (g.n <= 0) return hipSuccess;
+  const int tb = 64;
+  hipLaunchKernelGGL(k_garner, dim3((g.n + tb - 1) / tb), dim3(tb), 0, s, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_topk(const double* d, int stride, int n, int k, int32_t* out, hipStream_t s) {
+  if (n <= 0 || k <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_topk, dim3((n + 255) / 256), dim3(256), 0, s, d, stride, n, k, out);
+  return hipGetLastError();
+}
+
+}  // namespace drx

@@ -1,0 +1,61 @@
+// Plain-data records shared by the host packer and the kernels.
+#pragma once
+
+#include <stdint.h>
+
+namespace drx {
+
+constexpr int kMaxM = 4;          // matrix order handled by the kernels
+constexpr int kMaxK = 1024;       // primes in one context
+constexpr int kMaxDim = 7;        // axes per program (X bank: axes, increments, step)
+constexpr int kMaxConst = 1023;   // constants per program (10-bit field)
+constexpr int kMaxCode = 1 << 16; // device words per program
+
+// One packed program (device copy; offsets index the context pools).
+struct DevProgram {
+  uint32_t code_off;    // first word in the code pool (two words per instruction)
+  uint32_t ncode;       // instructions
+  uint32_t nconst;      // constants referenced by the packed code
+  uint32_t kcap;        // primes covered by the constant residue table
+  uint64_t cres_off;    // [nconst][kcap] balanced residues (double)
+  uint64_t cflt_off;    // [nconst] FP64 values
+  uint32_t m, dim;
+  uint32_t kinds;       // 2 bits per entry: EntryKind
+  uint32_t colred;      // bit j: reduce column j after P*M
+  uint32_t const_lds;   // 1: constants live in LDS (bank L), 0: in the VGPR bank C
+  uint32_t nslots;      // LDS temp slots (bank S)
+};
+
+// One walk's resident state and parameters (device copy).
+struct DevWalk {
+  int32_t prog;
+  int32_t K;
+  int32_t m;
+  int32_t dim;
+  uint64_t res_off;     // P residues [m*m][K] (double, balanced)
+  uint64_t fst_off;     // P_f [m*m] then log_scale (double)
+  uint64_t axr_off;     // axis residues [dim][3][K]: c0, c1, c2 (double, balanced)
+  uint64_t axi_off;     // axis integers [dim][2]: c0, c1 (int64) for the FP64 axis value
+  uint64_t axd_off;     // axis denominators [dim] (double)
+  double target;        // L for delta_float
+};
+
+// A contiguous step range of one walk processed by one workgroup.
+struct DevTask {
+  int32_t walk;
+  int32_t pad;
+  int64_t s_begin;
+  int64_t s_end;
+  uint64_t mf_off;      // M_f of step s at mf[mf_off + (s - s_begin) * m * m]
+  uint64_t cp_off;      // chunk product [m*m][K] in the chunk-product pool
+};
+
+// Per-walk span of tasks for the ordered combine / shadow passes.
+struct DevSpan {
+  int32_t walk;
+  int32_t task_begin;   // tasks of this walk are contiguous and in step order
+  int32_t task_end;
+  int32_t out_row;      // row of this walk in the finalize outputs
+};
+
+}  // namespace drx

@@ -1,0 +1,200 @@
+/* dreams_hip.h -- C ABI of libdreams_hip.so, the MI355X (gfx950) walk engine.
+ *
+ * The reference (VesterlundCoder/Dreams-RNS-CUDA) exposes this path only as
+ * Python functions; it has no FFI. Each entry point below is what a ctypes
+ * binding of those functions needs, and names the reference interface it
+ * replaces. All functions return DRX_OK (0) or a negative DRX_E* code;
+ * drx_last_error() then holds a message. No C++ exception or abort crosses
+ * the ABI. Host buffers are caller-owned (C-contiguous); device memory is
+ * owned by the context. A context must be used by one thread at a time; the
+ * server functions are internally synchronised.
+ */
+#ifndef DREAMS_HIP_H
+#define DREAMS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DRX_ABI_VERSION 1
+
+enum {
+  DRX_OK = 0,
+  DRX_EINVAL = -1,   /* bad argument (the reference raises ValueError/IndexError) */
+  DRX_EHIP = -2,     /* HIP runtime error */
+  DRX_ENOMEM = -3,   /* device or host allocation failed */
+  DRX_ERANGE = -4,   /* value outside what the engine supports (e.g. m > 4) */
+  DRX_ESTATE = -5,   /* call out of sequence (e.g. no primes set) */
+  DRX_EAGAIN = -6    /* server: nothing ready / queue full */
+};
+
+typedef struct drx_ctx drx_ctx;
+
+/* ---- context --------------------------------------------------------- */
+int drx_abi_version(void);
+int drx_device_count(int* n);
+int drx_ctx_create(int device, drx_ctx** out);
+void drx_ctx_destroy(drx_ctx* ctx);
+const char* drx_last_error(const drx_ctx* ctx);
+
+/* Prime table shared by every walk of the context; a walk with K primes uses
+ * the first K. Replaces generate_rns_primes (python/dreams_rns/runner.py:33-52),
+ * whose output the caller passes here (odd primes in (2^30, 2^31)). */
+int drx_set_primes(drx_ctx* ctx, const int64_t* primes, int K);
+
+/* ---- programs ----------------------------------------------------------
+ * Upload one compiled CmfProgram (python/dreams_rns/compiler.py:44-89):
+ * `ops` holds n_ops instructions as (op, arg0, arg1, arg2) uint32 quadruples,
+ * exactly Instruction.pack (compiler.py:39-41). Constants are given either as
+ * int64 values (`consts`, reduced here) or, for constants that do not fit,
+ * pre-reduced: `const_res` [n_const][kcap] canonical residues for the first
+ * kcap primes plus `const_f` [n_const] FP64 values (Python float(c)).
+ * Replaces _precompute_const_residues (runner.py:182-195). */
+int drx_program_upload(drx_ctx* ctx, const uint32_t* ops, int n_ops, int m, int dim,
+                       const int64_t* consts, const uint32_t* const_res, const double* const_f,
+                       int n_const, int kcap, int* prog_id);
+int drx_program_info(drx_ctx* ctx, int prog_id, int* n_dev_ops, int* n_const_used,
+                     uint32_t* entry_kinds);
+int drx_program_release_all(drx_ctx* ctx);
+
+/* Pack a program without a device (inspection and host-side testing of the
+ * packer): writes up to max_words device words (two per instruction) and the
+ * packing metadata: meta[0] words, [1] entry kinds (2 bits per entry),
+ * [2] columns reduced after P*M, [3] constants in LDS, [4] LDS temp slots,
+ * [5] constants used; const_slots[i] = program constant index of slot i
+ * (up to max_slots entries). Returns DRX_EINVAL with the packer's message in
+ * err (err_len bytes) for programs the engine cannot run. */
+int drx_program_pack(const uint32_t* ops, int n_ops, int m, int dim, const int64_t* consts, int n_const,
+                     uint32_t* words, int max_words, int* const_slots, int max_slots, int64_t* meta6,
+                     char* err, int err_len);
+
+/* ---- walks ------------------------------------------------------------- */
+enum { DRX_INIT_PCF = 0, DRX_INIT_IDENTITY = 1, DRX_INIT_GIVEN = 2 };
+enum { DRX_MODE_BATCHED = 0, DRX_MODE_PLAIN = 1, DRX_MODE_PERSISTENT = 2 };
+
+/* One walk: P = P0 * M(0) * M(1) * ... * M(depth-1) mod each of K primes,
+ * plus the FP64 shadow. Axis i at step s is (sn_i + s * tn_i / td_i ...) given as
+ * four int64 per axis in the axis pool at axis_off: sn, sd, tn, td, i.e. the
+ * value (sn*td + s*tn*sd) / (sd*td) (python/dreams_rns/cmf_walk.py:206-224).
+ * A PCF walk (runner.py:338-413) is sn = shift_val, tn = direction,
+ * sd = td = 1, init_kind DRX_INIT_PCF with a0. */
+typedef struct drx_walk {
+  int32_t prog;
+  int32_t K;
+  int64_t depth;
+  int32_t init_kind;
+  int32_t reserved;
+  int64_t a0;          /* DRX_INIT_PCF: A = [[1, a0], [0, 1]] */
+  int64_t init_off;    /* DRX_INIT_GIVEN: first of m*m*K residues ([m*m][K]) in init_res_pool */
+  int64_t init_f_off;  /* DRX_INIT_GIVEN: first of m*m FP64 entries in init_f_pool */
+  int64_t axis_off;    /* index of sn of axis 0 in the axis pool */
+  double target;       /* L for delta_float */
+} drx_walk;
+
+typedef struct drx_walk_out {
+  double p_float;
+  double q_float;
+  double log_scale;
+  double delta_float;
+  double log_q;
+  double reserved;
+} drx_walk_out;
+
+/* Batch I/O. Optional outputs may be NULL. */
+typedef struct drx_batch {
+  const drx_walk* walks;
+  int n;
+  const int64_t* axis_pool;
+  const uint32_t* init_res_pool;  /* DRX_INIT_GIVEN: canonical residues, see drx_walk */
+  const double* init_f_pool;      /* DRX_INIT_GIVEN: FP64 entries, see drx_walk */
+  drx_walk_out* out;              /* [n] */
+  int64_t* residues;              /* [n][2][res_stride]: p then q residues (canonical) */
+  int res_stride;                 /* >= max K */
+  int64_t* full;                  /* optional [n][16][res_stride] whole final P (row-major m*m) */
+  uint32_t* crt_limbs;            /* optional [n][2][res_stride] |centred p|, |centred q| limbs */
+  uint8_t* crt_neg;               /* optional [n][2] sign of centred p, q */
+  int top_k;                      /* 0: none */
+  int32_t* topk_idx;              /* [top_k] walk indices by delta_float desc, ties low index */
+} drx_batch;
+
+typedef struct drx_launch_cfg {
+  int mode;         /* DRX_MODE_* */
+  int threads;      /* workgroup size, power of two in [32, 256]; 0 = tuned */
+  int chunk_steps;  /* max steps of one walk per workgroup task; 0 = tuned */
+  int use_graph;    /* capture the launch sequence in a hipGraph and replay it */
+} drx_launch_cfg;
+
+/* Run a batch of walks to completion. Replaces run_pcf_walk (runner.py:338),
+ * run_cmf_walk (cmf_walk.py:363), run_pcf_walk_batch_gpu / run_cmf_walk_batch_gpu
+ * (gpu_walk.py:279, 395) and, with crt_limbs, crt_reconstruct + centered
+ * (runner.py:57-81) on the p and q of every walk. */
+int drx_walk_batch(drx_ctx* ctx, const drx_batch* b, const drx_launch_cfg* cfg);
+
+/* The same in two phases: drx_plan_create packs and uploads everything
+ * (inputs become resident in HBM) and, with use_graph, captures the launch
+ * sequence; drx_plan_run executes it (re-runnable: every run starts from the
+ * walks' initial matrices) and copies the outputs into the drx_batch buffers
+ * given at creation, which must stay valid. */
+typedef struct drx_plan drx_plan;
+int drx_plan_create(drx_ctx* ctx, const drx_batch* b, const drx_launch_cfg* cfg, drx_plan** out);
+int drx_plan_run(drx_plan* plan);
+/* Work of the plan: residue-steps (sum depth*K) and the modular operation
+ * counts summed over residue-steps: [0] residue-steps, [1] modular multiplies,
+ * [2] modular additions, [3] walk-kernel tasks. */
+int drx_plan_stats(const drx_plan* plan, double* s4);
+void drx_plan_destroy(drx_plan* plan);
+
+/* Garner CRT of n rows of K canonical residues (row stride `stride`) over the
+ * first K context primes: magnitude limbs (little-endian uint32, K per row)
+ * and sign of centered(x, M). Replaces crt_reconstruct + centered
+ * (runner.py:57-81). */
+int drx_crt(drx_ctx* ctx, const int64_t* residues, int n, int K, int stride, uint32_t* limbs,
+            uint8_t* neg);
+
+/* Indices of the k largest values (ties: lower index first; NaN last).
+ * No reference counterpart in Python; the reference CUDA sketch is
+ * k_topk_select (src/cuda/persistent_kernel.cu:796-841). */
+int drx_topk(drx_ctx* ctx, const double* values, int n, int k, int32_t* idx);
+
+/* Device time of the last drx_walk_batch, from HIP events on the library's
+ * stream: [0] walk kernels, [1] combine, [2] shadow, [3] finalize + CRT + top-k,
+ * [4] whole batch (first to last launch); milliseconds. [5] walk-kernel
+ * launches. */
+int drx_last_timing(const drx_ctx* ctx, double* t6);
+
+/* ---- continuous-batching server ---------------------------------------
+ * Requests are admitted into device-resident slots as memory allows; every
+ * round advances each resident walk by up to `round_steps` steps, so walks
+ * submitted while others run join at the next round boundary. */
+typedef struct drx_server_cfg {
+  int max_slots;        /* resident walks */
+  int round_steps;      /* steps per walk per round */
+  int threads;          /* workgroup size, 0 = tuned */
+  int64_t mem_budget;   /* bytes of device state for slots (0 = 1 GiB) */
+} drx_server_cfg;
+
+typedef struct drx_server drx_server;
+
+typedef struct drx_server_result {
+  uint64_t ticket;
+  drx_walk_out out;
+  int32_t K;
+  int32_t status;          /* 0 ok, < 0 error */
+  double t_submit, t_admit, t_done;  /* seconds, steady clock */
+} drx_server_result;
+
+int drx_server_create(drx_ctx* ctx, const drx_server_cfg* cfg, drx_server** out);
+/* Copies the walk (and its axis values) into the server queue. */
+int drx_server_submit(drx_server* s, const drx_walk* w, const int64_t* axis_vals, uint64_t* ticket);
+/* Up to max finished results; residues[i] gets 2*K int64 (p then q) at stride. */
+int drx_server_poll(drx_server* s, drx_server_result* res, int64_t* residues, int stride, int max,
+                    int* n_done);
+int drx_server_pending(drx_server* s, int* n_queued, int* n_resident);
+void drx_server_destroy(drx_server* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DREAMS_HIP_H */

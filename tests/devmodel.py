@@ -1,0 +1,122 @@
+"""CPU model of the packed device code (test helper).
+
+Executes the words produced by the packer (drx_program_pack) with the bank /
+accumulator semantics of the device interpreter, using exact integers mod p
+for the residue pass and Python floats for the FP64 pass, then the per-step
+product P <- P * M exactly like the walk kernel's entry-kind matmul. Agreement
+with the oracle validates the packer's dead-code removal, operand routing and
+register allocation on the CPU.
+"""
+
+from __future__ import annotations
+
+from typing import List, Sequence
+
+D_ADD, D_SUB, D_MUL, D_NEG, D_SQR, D_CUB, D_INV, D_RED, D_MOV = range(1, 10)
+DB_R, DB_E, DB_S, DB_NONE = range(4)
+SB_R, SB_X, SB_C, SB_L, SB_S, SB_A, SB_Z = range(7)
+STEP_SLOT, INC_SLOT = 15, 8
+
+
+def decode(words):
+    out = []
+    for i in range(0, len(words), 2):
+        w0, w1 = int(words[i]), int(words[i + 1])
+        out.append((w0 & 255, (w0 >> 8) & 3, w0 >> 10, w1 & 7, (w1 >> 3) & 8191, (w1 >> 16) & 7, w1 >> 19))
+    return out
+
+
+def _cube(x: float) -> float:
+    from fractions import Fraction
+
+    return float(Fraction(x) ** 3)
+
+
+def run_program(code, fp: bool, X: List, consts: Sequence, p: int = 0, nslots: int = 0):
+    """One step of the interpreter; returns the entry bank E (16 slots)."""
+    R = [0] * 16 if not fp else [0.0] * 16
+    S = [0] * max(1, nslots) if not fp else [0.0] * max(1, nslots)
+    E = [0] * 16 if not fp else [0.0] * 16
+    acc = 0 if not fp else 0.0
+    zero = 0 if not fp else 0.0
+
+    def fetch(bank, i):
+        if bank == SB_R:
+            return R[i & 15]
+        if bank == SB_X:
+            return X[i & 15]
+        if bank in (SB_C, SB_L):
+            return consts[i]
+        if bank == SB_S:
+            return S[i]
+        if bank == SB_A:
+            return acc
+        return zero
+
+    for op, db, d, ab, a, bb, b in code:
+        x = fetch(ab, a)
+        y = fetch(bb, b)
+        if fp:
+            r = {D_ADD: lambda: x + y, D_SUB: lambda: x - y, D_MUL: lambda: x * y, D_NEG: lambda: -x,
+                 D_SQR: lambda: x * x, D_CUB: lambda: _cube(x),
+                 D_INV: lambda: (1.0 / x if abs(x) > 1e-300 else 0.0), D_RED: lambda: x, D_MOV: lambda: x}[op]()
+        else:
+            r = {D_ADD: lambda: (x + y) % p, D_SUB: lambda: (x - y) % p, D_MUL: lambda: x * y % p,
+                 D_NEG: lambda: -x % p, D_SQR: lambda: x * x % p, D_CUB: lambda: x * x % p * x % p,
+                 D_INV: lambda: pow(x, p - 2, p), D_RED: lambda: x % p, D_MOV: lambda: x % p}[op]()
+        if db == DB_R:
+            R[d & 15] = r
+        elif db == DB_E:
+            E[d & 15] = r
+        elif db == DB_S:
+            S[d] = r
+        acc = r
+    return E
+
+
+def walk(program, words, meta, const_slots, primes, axes, depth, init_res, init_f):
+    """Residues (m, m, K) and M_f list per step of a walk through the model."""
+    m, dim = program.m, program.dim
+    code = decode(words)
+    kinds = meta["kinds"]
+    K = len(primes)
+    P = [[[int(init_res[i * m + j][k]) for j in range(m)] for i in range(m)] for k in range(K)]
+    for k, p in enumerate(primes):
+        p = int(p)
+        consts = [int(program.constants[ci]) % p for ci in const_slots]
+        for s in range(depth):
+            X = [0] * 16
+            for ax in range(dim):
+                sn, sd, tn, td = axes[ax]
+                num, den = sn * td + s * tn * sd, sd * td
+                X[ax] = num % p * pow(den % p, p - 2, p) % p if den != 1 else num % p
+            X[STEP_SLOT] = s % p
+            E = run_program(code, False, X, consts, p, meta["nslots"])
+            C = [[0] * m for _ in range(m)]
+            for i in range(m):
+                for j in range(m):
+                    acc = 0
+                    for t in range(m):
+                        kd = (kinds >> (2 * (t * m + j))) & 3
+                        if kd == 2:
+                            acc += P[k][i][t] * E[t * m + j]
+                        elif kd == 1:
+                            acc += P[k][i][t]
+                    C[i][j] = acc % p
+            P[k] = C
+    res = [[[P[k][i][j] for k in range(K)] for j in range(m)] for i in range(m)]
+    mfs = []
+    fconsts = [float(int(program.constants[ci])) for ci in const_slots]
+    for s in range(depth):
+        X = [0.0] * 16
+        for ax in range(dim):
+            sn, sd, tn, td = axes[ax]
+            X[ax] = float(sn * td + s * tn * sd) / float(sd * td)
+        X[STEP_SLOT] = float(s)
+        E = run_program(code, True, X, fconsts, 0, meta["nslots"])
+        mf = []
+        for e in range(m * m):
+            kd = (kinds >> (2 * e)) & 3
+            mf.append(E[e] if kd == 2 else (1.0 if kd == 1 else 0.0))
+        mfs.append(mf)
+    return res, mfs

@@ -1,0 +1,155 @@
+"""GPU parity: the HIP path (through the C ABI) against the reference's golden
+vectors and the oracle, bit-exact for residues and CRT integers; the FP64
+shadow bit-exact where the reference's floating-point operations are exact
+or correctly rounded, else within the contract tolerance: delta_float within
+1e-12 relative."""
+
+import math
+
+import numpy as np
+import pytest
+
+from conftest import program_from_json
+
+import dreams_rns as D
+from dreams_rns import synth
+from dreams_rns.engine import BatchPlan, cmf_spec, pcf_spec
+from oracle import cref, exact
+from oracle import walk as ow
+
+pytestmark = pytest.mark.gpu
+REL = 1e-12
+
+
+def _hex(x):
+    return float(x).hex()
+
+
+def _close(a, b, rel=REL):
+    a, b = float(a), float(b)
+    return a == b or abs(a - b) <= rel * max(abs(a), abs(b))
+
+
+def test_pcf_golden_vectors(gpu_ctx, golden):
+    for w in golden["run_pcf_walk"]:
+        p = program_from_json(w["program"])
+        got = D.run_pcf_walk(p, w["a0"], w["depth"], w["K"], w["shift_val"])
+        o = w["out"]
+        print("pcf", w["a"], "|", w["b"], w["depth"], w["K"], flush=True)
+        assert got["p_residues"].tolist() == o["p_residues"]
+        assert got["q_residues"].tolist() == o["q_residues"]
+        assert got["primes"].tolist() == o["primes"]
+        assert got["p_residues"].dtype == np.int64 and isinstance(got["p_float"], np.float64)
+        assert _hex(got["p_float"]) == o["p_float"] and _hex(got["q_float"]) == o["q_float"]
+        assert _close(got["log_scale"], float.fromhex(o["log_scale"]))
+
+
+def test_cmf_golden_vectors(gpu_ctx, golden):
+    for c in golden["run_cmf_walk"]:
+        p = program_from_json(c["program"])
+        shift = [tuple(v) for v in c["shift"]]
+        traj = None if c["traj"] is None else [tuple(v) for v in c["traj"]]
+        got = D.run_cmf_walk(p, c["depth"], c["K"], shift, traj)
+        o = c["out"]
+        print("cmf m=%d ops=%d" % (c["m"], len(c["program"]["ops"])), flush=True)
+        assert got["p_residues"].tolist() == o["p_residues"]
+        assert got["q_residues"].tolist() == o["q_residues"]
+        for key in ("p_float", "q_float", "log_scale"):
+            assert _close(got[key], float.fromhex(o[key])), key
+
+
+def test_verify_pcf_readme_case(gpu_ctx, golden):
+    g = golden["verify_pcf"][0]
+    got = D.verify_pcf(*g["args"])
+    ref = g["out"]
+    assert got["p_bits"] == ref["p_bits"] == 992
+    assert _close(got["delta_float"], float.fromhex(ref["delta_float"]))
+    assert _close(got["delta_exact"], float.fromhex(ref["delta_exact"]))
+    assert _close(got["est_float"], float.fromhex(ref["est_float"]))
+    assert _close(got["log_q"], float.fromhex(ref["log_q"]))
+
+
+def test_crt_matches_textbook_garner(gpu_ctx):
+    rng = np.random.default_rng(7)
+    for K in (1, 2, 8, 32, 64, 128):
+        pr = [int(p) for p in D.generate_rns_primes(K)]
+        for _ in range(5):
+            res = [int(rng.integers(0, p)) for p in pr]
+            x, M = D.crt_reconstruct(res, pr)
+            assert (x, M) == ow.crt_reconstruct(res, pr) == exact.garner(res, pr)
+        # round trip and signed values around the centre
+        M = math.prod(pr)
+        for v in (0, 1, M // 2, M // 2 + 1, M - 1, 12345678901234567890 % M):
+            x, _ = D.crt_reconstruct([v % p for p in pr], pr)
+            assert x == v
+            assert D.centered(x, M) == (v - M if v > M // 2 else v)
+
+
+def test_batch_pcf2_subset_against_c_oracle(gpu_ctx):
+    cfg = synth.pcf2(n_walks=48)
+    progs = [D.compile_pcf(w["a"], w["b"]) for w in cfg["walks"]]
+    a0s = [D.pcf_initial_values(w["a"]) for w in cfg["walks"]]
+    specs = [pcf_spec(p, a0, 2000, 32, 1, target=w["L"]) for p, a0, w in zip(progs, a0s, cfg["walks"])]
+    with BatchPlan(specs, crt=True, top_k=16) as plan:
+        res = plan.run()
+    primes = D.generate_rns_primes(32)
+    deltas = []
+    for i, (p, a0, w) in enumerate(zip(progs, a0s, cfg["walks"])):
+        rr, Pf, ls = cref.pcf_walk(p, a0, 2000, primes)
+        d = res.walk_dict(i)
+        assert d["p_residues"].tolist() == rr[0, 1].tolist(), i
+        assert d["q_residues"].tolist() == rr[1, 1].tolist(), i
+        assert _hex(d["p_float"]) == _hex(Pf[0, 1]) and _hex(d["q_float"]) == _hex(Pf[1, 1]), i
+        assert _close(d["log_scale"], ls), i
+        pb, qb = res.crt_values(i)
+        xp, M = ow.crt_reconstruct(rr[0, 1].tolist(), primes.tolist())
+        xq, _ = ow.crt_reconstruct(rr[1, 1].tolist(), primes.tolist())
+        assert (pb, qb) == (ow.centered(xp, M), ow.centered(xq, M)), i
+        dl, lq = ow.compute_dreams_delta_float(Pf[0, 1], Pf[1, 1], ls, w["L"])
+        assert _close(res.out[i]["delta_float"], dl), i
+        deltas.append(dl)
+    order = sorted(range(len(deltas)), key=lambda j: (-deltas[j], j))[:16]
+    assert res.topk.tolist() == order
+
+
+def test_plain_mode_matches_batched(gpu_ctx):
+    cfg = synth.pcf2(n_walks=6)
+    specs = [pcf_spec(D.compile_pcf(w["a"], w["b"]), D.pcf_initial_values(w["a"]), 700, 32)
+             for w in cfg["walks"]]
+    a = BatchPlan(specs, mode="plain").run()
+    b = BatchPlan(specs, mode="batched", chunk_steps=256, threads=128).run()
+    c = BatchPlan(specs, mode="batched", use_graph=True).run()
+    for r in (b, c):
+        assert np.array_equal(a.residues, r.residues)
+        assert [_hex(v) for v in a.out["p_float"]] == [_hex(v) for v in r.out["p_float"]]
+        assert [_hex(v) for v in a.out["log_scale"]] == [_hex(v) for v in r.out["log_scale"]]
+
+
+def test_inv_of_zero_is_zero(gpu_ctx):
+    # b(n) = -1/(n+1) at shift -3: n + 1 = 0 at step 2 (INV(0) = 0 in residues, 0.0 in floats)
+    p = D.compile_pcf("3", "-1/(n+1)")
+    got = D.run_pcf_walk(p, 3, 60, 8, -3)
+    ref = ow.run_pcf_walk(p, 3, 60, 8, -3)
+    assert got["p_residues"].tolist() == ref["p_residues"].tolist()
+    assert _hex(got["q_float"]) == _hex(ref["q_float"])
+
+
+def test_cmf3_shapes_against_c_oracle(gpu_ctx):
+    cf = lambda mat, m, dim, axes, dirs: D.compile_cmf_from_dict(mat, m, dim, axes, dirs)
+    rng = np.random.default_rng(99)
+    specs, refs = [], []
+    primes = D.generate_rns_primes(64)
+    for m in (3, 3, 4):
+        spec = synth.cmf_walk_spec(rng, m, cf, 100, 250, True) if m == 3 else synth.cmf_walk_spec(rng, m, cf, 10, 400, False)
+        p = spec["program"]
+        specs.append(cmf_spec(p, 150, 64, spec["shift"]))
+        axes = [(s, 1, d, 1) for s, d in zip(spec["shift"], spec["directions"])]
+        refs.append(cref.walk(p, 150, primes, axes))
+    with BatchPlan(specs, full=True) as plan:
+        res = plan.run()
+    for i, (rr, Pf, ls) in enumerate(refs):
+        m = specs[i].program.m
+        assert np.array_equal(res.full_matrix(i, m), rr), i
+        d = res.walk_dict(i)
+        assert _close(d["p_float"], Pf[0, m - 1]) and _close(d["q_float"], Pf[m - 1, m - 1]), i
+        assert _close(d["log_scale"], ls), i
