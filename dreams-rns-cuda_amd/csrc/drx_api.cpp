@@ -171,7 +171,7 @@ struct drx_plan {
   std::vector<Group> groups;
   std::vector<int> walk_order;  // span index -> walk
   DBuf d_walks, d_tasks, d_spans, d_res0, d_fst0, d_res, d_fst, d_axr, d_axi, d_axd, d_cp, d_mf;
-  DBuf d_fout, d_rout, d_pout, d_digits, d_limbs, d_neg, d_topk, d_rowk;
+  DBuf d_fout, d_rout, d_pout, d_pfull, d_digits, d_limbs, d_neg, d_topk, d_rowk;
   KArgs args{};
   GarnerArgs gargs{};
   int ntask = 0;
@@ -183,7 +183,7 @@ struct drx_plan {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
     for (DBuf* d : {&d_walks, &d_tasks, &d_spans, &d_res0, &d_fst0, &d_res, &d_fst, &d_axr, &d_axi, &d_axd,
-                    &d_cp, &d_mf, &d_fout, &d_rout, &d_pout, &d_digits, &d_limbs, &d_neg, &d_topk, &d_rowk})
+                    &d_cp, &d_mf, &d_fout, &d_rout, &d_pout, &d_pfull, &d_digits, &d_limbs, &d_neg, &d_topk, &d_rowk})
       d->release();
     for (drx_plan* p : plain) delete p;
   }
@@ -398,6 +398,7 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
   HIPCHK(ctx, pl->d_fout.ensure((size_t)std::max(1, n) * 6 * 8));
   HIPCHK(ctx, pl->d_rout.ensure((size_t)std::max(1, n) * 2 * pl->ld * 8));
   if (b.full) HIPCHK(ctx, pl->d_pout.ensure((size_t)std::max(1, n) * 16 * pl->ld * 8));
+  if (b.full_f) HIPCHK(ctx, pl->d_pfull.ensure((size_t)std::max(1, n) * 16 * 8));
   if (b.top_k > 0) {
     if (b.top_k > n) return fail(ctx, DRX_EINVAL, "top_k=%d > n=%d", b.top_k, n);
     HIPCHK(ctx, pl->d_topk.ensure((size_t)b.top_k * 4));
@@ -437,6 +438,7 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
   a.rout = pl->d_rout.as<int64_t>();
   a.pout = b.full ? pl->d_pout.as<int64_t>() : nullptr;
   a.fcompact = nullptr;
+  a.ffull = b.full_f ? pl->d_pfull.as<double>() : nullptr;
   a.out_ld = (uint64_t)pl->ld;
   a.with_fp = 1;
   GarnerArgs& g = pl->gargs;
@@ -479,6 +481,7 @@ int plan_fetch(drx_plan* pl) {
   if (b.residues)
     HIPCHK(ctx, hipMemcpyAsync(b.residues, pl->d_rout.p, (size_t)n * 2 * pl->ld * 8, hipMemcpyDeviceToHost, s));
   if (b.full) HIPCHK(ctx, hipMemcpyAsync(b.full, pl->d_pout.p, (size_t)n * 16 * pl->ld * 8, hipMemcpyDeviceToHost, s));
+  if (b.full_f) HIPCHK(ctx, hipMemcpyAsync(b.full_f, pl->d_pfull.p, (size_t)n * 16 * 8, hipMemcpyDeviceToHost, s));
   if (b.crt_limbs) {
     HIPCHK(ctx, hipMemcpyAsync(b.crt_limbs, pl->d_limbs.p, (size_t)n * 2 * pl->ld * 4, hipMemcpyDeviceToHost, s));
     if (b.crt_neg) HIPCHK(ctx, hipMemcpyAsync(b.crt_neg, pl->d_neg.p, (size_t)n * 2, hipMemcpyDeviceToHost, s));
@@ -751,6 +754,7 @@ int drx_plan_create(drx_ctx* ctx, const drx_batch* b, const drx_launch_cfg* cfg,
       sb.out = b->out + i;
       sb.residues = b->residues + (size_t)i * 2 * b->res_stride;
       sb.full = b->full ? b->full + (size_t)i * 16 * b->res_stride : nullptr;
+      sb.full_f = b->full_f ? b->full_f + (size_t)i * 16 : nullptr;
       sb.crt_limbs = b->crt_limbs ? b->crt_limbs + (size_t)i * 2 * b->res_stride : nullptr;
       sb.crt_neg = b->crt_neg ? b->crt_neg + 2 * i : nullptr;
       sb.top_k = 0;

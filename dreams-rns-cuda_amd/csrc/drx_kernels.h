@@ -33,6 +33,7 @@ struct KArgs {
   int64_t* rout;      // [out_row][2][out_ld]: p residues then q residues
   int64_t* pout;      // optional [out_row][16][out_ld] full state residues
   double* fcompact;   // optional [out_row][6]: copy of fout of the finalized walks
+  double* ffull;      // optional [out_row][16]: final P_f
   uint64_t out_ld;
   int with_fp;
 };

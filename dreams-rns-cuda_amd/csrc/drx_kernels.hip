@@ -322,10 +322,16 @@ __global__ void k_shadow(const KArgs a, int nspan, int finalize) {
         nan |= (v != v);
         mx = (v > mx) ? v : mx;
       }
-      if (!nan && mx > 1e10) {
+      // The rescale is taken by most lanes at most steps; a wave-uniform
+      // branch with per-lane selects keeps the control flow uniform (a
+      // divergent branch here was miscompiled for M = 3: loop-carried
+      // registers were clobbered by the next step's loads).
+      const bool resc = !nan && mx > 1e10;
+      if (__builtin_amdgcn_ballot_w64(resc) != 0) {
+        const double l = log(resc ? mx : 1.0);  // log(1) == 0 exactly
 #pragma unroll
-        for (int e = 0; e < M * M; ++e) C[e] = C[e] / mx;
-        ls += log(mx);
+        for (int e = 0; e < M * M; ++e) C[e] = resc ? C[e] / mx : C[e];
+        ls = resc ? ls + l : ls;
       }
 #pragma unroll
       for (int e = 0; e < M * M; ++e) P[e] = C[e];
@@ -368,6 +374,8 @@ __global__ void k_finalize(const KArgs a, int full) {
   int64_t* ro = a.rout + (uint64_t)sp.out_row * 2 * a.out_ld;
   if (a.fcompact && threadIdx.x < 6)
     a.fcompact[(uint64_t)sp.out_row * 6 + threadIdx.x] = a.fout[(uint64_t)sp.walk * 6 + threadIdx.x];
+  if (a.ffull && threadIdx.x < 16)
+    a.ffull[(uint64_t)sp.out_row * 16 + threadIdx.x] = threadIdx.x < m * m ? a.fst[w.fst_off + threadIdx.x] : 0.0;
   for (uint32_t k = threadIdx.x; k < K; k += blockDim.x) {
     const double p = a.prime_p[k], pinv = a.prime_pinv[k];
     ro[k] = (int64_t)canon(S[(uint64_t)(m - 1) * K + k], p, pinv);

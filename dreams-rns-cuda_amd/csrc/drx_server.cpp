@@ -209,6 +209,7 @@ int round_once(drx_server* s, std::vector<int>& finished) {
   a.rout = s->d_rout.as<int64_t>();
   a.pout = nullptr;
   a.fcompact = s->d_fcomp.as<double>();
+  a.ffull = nullptr;
   a.out_ld = (uint64_t)s->kslot;
   a.with_fp = 1;
   for (const G& g : groups) {

@@ -53,7 +53,7 @@ class Batch(C.Structure):
     _fields_ = [
         ("walks", C.c_void_p), ("n", C.c_int), ("axis_pool", C.c_void_p),
         ("init_res_pool", C.c_void_p), ("init_f_pool", C.c_void_p), ("out", C.c_void_p),
-        ("residues", C.c_void_p), ("res_stride", C.c_int), ("full", C.c_void_p),
+        ("residues", C.c_void_p), ("res_stride", C.c_int), ("full", C.c_void_p), ("full_f", C.c_void_p),
         ("crt_limbs", C.c_void_p), ("crt_neg", C.c_void_p), ("top_k", C.c_int),
         ("topk_idx", C.c_void_p),
     ]

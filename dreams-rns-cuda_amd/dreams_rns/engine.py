@@ -95,6 +95,7 @@ class BatchResult:
     limbs: Optional[np.ndarray] = None   # (n, 2, stride) uint32
     neg: Optional[np.ndarray] = None     # (n, 2) uint8
     full: Optional[np.ndarray] = None    # (n, 16, stride) int64
+    full_f: Optional[np.ndarray] = None  # (n, 16) float64: final shadow matrix
     topk: Optional[np.ndarray] = None    # (top_k,) int32
 
     def walk_dict(self, i: int) -> Dict[str, Any]:
@@ -126,6 +127,11 @@ class BatchResult:
             raise RuntimeError("plan was built without full=True")
         K = int(self.Ks[i])
         return self.full[i, : m * m, :K].reshape(m, m, K).copy()
+
+    def full_shadow(self, i: int, m: int) -> np.ndarray:
+        if self.full_f is None:
+            raise RuntimeError("plan was built without full=True")
+        return self.full_f[i, : m * m].reshape(m, m).copy()
 
 
 class BatchPlan:
@@ -198,6 +204,7 @@ class BatchPlan:
             limbs=np.zeros((n, 2, stride), dtype=np.uint32) if crt else None,
             neg=np.zeros((n, 2), dtype=np.uint8) if crt else None,
             full=np.zeros((n, 16, stride), dtype=np.int64) if full else None,
+            full_f=np.zeros((n, 16), dtype=np.float64) if full else None,
             topk=np.zeros(top_k, dtype=np.int32) if top_k else None,
         )
         r = self.result
@@ -211,6 +218,7 @@ class BatchPlan:
         b.residues = r.residues.ctypes.data
         b.res_stride = stride
         b.full = r.full.ctypes.data if full else None
+        b.full_f = r.full_f.ctypes.data if full else None
         b.crt_limbs = r.limbs.ctypes.data if crt else None
         b.crt_neg = r.neg.ctypes.data if crt else None
         b.top_k = int(top_k)

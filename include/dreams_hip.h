@@ -113,6 +113,7 @@ typedef struct drx_batch {
   int64_t* residues;              /* [n][2][res_stride]: p then q residues (canonical) */
   int res_stride;                 /* >= max K */
   int64_t* full;                  /* optional [n][16][res_stride] whole final P (row-major m*m) */
+  double* full_f;                 /* optional [n][16] whole final FP64 shadow P_f (row-major m*m) */
   uint32_t* crt_limbs;            /* optional [n][2][res_stride] |centred p|, |centred q| limbs */
   uint8_t* crt_neg;               /* optional [n][2] sign of centred p, q */
   int top_k;                      /* 0: none */

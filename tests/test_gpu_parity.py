@@ -44,6 +44,25 @@ def test_pcf_golden_vectors(gpu_ctx, golden):
         assert _close(got["log_scale"], float.fromhex(o["log_scale"]))
 
 
+def test_cmf_rational_axes_whole_shadow(gpu_ctx, golden):
+    # 3x3 CMF with rational shift and trajectory: the whole shadow matrix,
+    # step by step, against the C restatement (P_f after 1 step is M_f(0))
+    c = golden["run_cmf_walk"][1]
+    p = program_from_json(c["program"])
+    shift = [tuple(v) for v in c["shift"]]
+    traj = [tuple(v) for v in c["traj"]]
+    axes = [(s[0], s[1], t[0], t[1]) for s, t in zip(shift, traj)]
+    for depth in (1, 2, 3, 5, 40):
+        with BatchPlan([cmf_spec(p, depth, 8, shift, traj)], full=True) as plan:
+            res = plan.run()
+        rr, Pf, ls = cref.walk(p, depth, D.generate_rns_primes(8), axes)
+        got = res.full_shadow(0, 3)
+        print("depth", depth, "\ngpu\n", got, "\noracle\n", Pf, flush=True)
+        assert np.array_equal(res.full_matrix(0, 3), rr)
+        for a, b in zip(got.reshape(-1), Pf.reshape(-1)):
+            assert _close(a, b), (depth, a, b)
+
+
 def test_cmf_golden_vectors(gpu_ctx, golden):
     for c in golden["run_cmf_walk"]:
         p = program_from_json(c["program"])
