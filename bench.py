@@ -1,0 +1,259 @@
+"""Benchmark of the MI355X walk engine on its headline workload.
+
+One step = one pass of the whole pcf2 batch through the GPU path: 4096 seeded
+PCFs (deg a <= 3, deg b <= 6, coefficients in [-20, 20]), depth 2000, K = 32
+primes, FP64 shadow with delta_float against a seeded target per walk, GPU
+Garner CRT of p and q for every walk and top-64 by delta_float. Inputs are
+packed and resident in HBM before the timed region; each step re-runs the
+batch from the initial matrices and copies all results to the host.
+
+Printed (one JSON line): residue-steps/s of the batched path (value), the
+plain one-walk-per-launch GPU path and the CPU reference restatement on
+bounded samples, p50/p99 latency of the batch and of a short served stream,
+and the roofline of the dominant kernel (the walk kernel).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import multiprocessing as mp
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (os.path.join(ROOT, "dreams-rns-cuda_amd"), ROOT):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+METRIC = "residue-steps/sec batched+served vs plain (bit-exact CRT); p50/p99 latency"
+PEAK_LANE_OPS = 256 * 64 * 2.4e9  # CU x lanes x clock (rocminfo: 256 CUs, 2400 MHz)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+# ------------------------------------------------------------- work model
+def program_work(prog) -> float:
+    """Integer-VALU ops per residue-step by the fixed convention
+    W = 6*MM + 2*MA with MM = #MUL + #POW2 + 2*#POW3 + 56*#INV + m^3 and
+    MA = #ADD + #SUB + #NEG + m^2 (m - 1), counted on the walk's bytecode."""
+    cnt = {}
+    for ins in prog.opcodes:
+        cnt[int(ins.op)] = cnt.get(int(ins.op), 0) + 1
+    m = prog.m
+    mm = cnt.get(6, 0) + cnt.get(8, 0) + 2 * cnt.get(9, 0) + 56 * cnt.get(10, 0) + m ** 3
+    ma = cnt.get(4, 0) + cnt.get(5, 0) + cnt.get(7, 0) + m * m * (m - 1)
+    return 6.0 * mm + 2.0 * ma
+
+
+# ------------------------------------------------------------ CPU baseline
+def _cpu_walk(args):
+    import dreams_rns as D
+    from oracle import walk as ow
+
+    a, b, depth, K = args
+    prog = D.compile_pcf(a, b)
+    res = ow.run_pcf_walk(prog, D.pcf_initial_values(a), depth, K)
+    primes = [int(p) for p in res["primes"]]
+    for key in ("p_residues", "q_residues"):
+        x, M = ow.crt_reconstruct([int(v) for v in res[key]], primes)
+        ow.centered(x, M)
+    return depth * K
+
+
+def cpu_baseline(walks, depth, K, n_one, n_all, cores):
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    os.environ.setdefault("OPENBLAS_NUM_THREADS", "1")
+    jobs = [(w["a"], w["b"], depth, K) for w in walks]
+    t0 = time.perf_counter()
+    done = sum(_cpu_walk(j) for j in jobs[:n_one])
+    one = done / (time.perf_counter() - t0)
+    ctx = mp.get_context("fork")
+    with ctx.Pool(cores) as pool:
+        pool.map(_cpu_walk, jobs[:cores])  # warm the workers
+        t0 = time.perf_counter()
+        done = sum(pool.map(_cpu_walk, jobs[:n_all], chunksize=1))
+        allc = done / (time.perf_counter() - t0)
+    return one, allc
+
+
+# ------------------------------------------------------------------ served
+def served_run(specs_pool, rate, duration, round_steps, seed=1005):
+    """Poisson arrivals of walks drawn from ``specs_pool`` at ``rate`` req/s
+    for ``duration`` s into the continuous-batching server."""
+    from dreams_rns.server import WalkServer
+
+    rng = np.random.default_rng(seed)
+    n = int(rng.poisson(rate * duration))
+    times = np.sort(rng.uniform(0.0, duration, size=n))
+    picks = rng.integers(0, len(specs_pool), size=n)
+    lat, rsteps = [], 0
+    with WalkServer(round_steps=round_steps) as srv:
+        t0 = time.perf_counter()
+        tickets = {}
+        for t, i in zip(times, picks):
+            now = time.perf_counter() - t0
+            if t > now:
+                time.sleep(t - now)
+            tickets[srv.submit(specs_pool[i])] = specs_pool[i]
+            for r in srv.poll():
+                lat.append(r["latency_s"])
+        for r in srv.drain():
+            lat.append(r["latency_s"])
+        wall = time.perf_counter() - t0
+        rsteps = sum(s.depth * s.K for s in tickets.values())
+    lat = np.array(lat)
+    return {"rate_req_s": rate, "requests": int(n), "residue_steps_per_s": rsteps / wall,
+            "p50_ms": float(np.percentile(lat, 50) * 1e3) if n else None,
+            "p99_ms": float(np.percentile(lat, 99) * 1e3) if n else None, "wall_s": wall}
+
+
+# -------------------------------------------------------------------- main
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--walks", type=int, default=4096)
+    ap.add_argument("--plain-sample", type=int, default=32)
+    ap.add_argument("--cpu-one", type=int, default=8)
+    ap.add_argument("--cpu-all", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=1000.0)
+    ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--graph", action="store_true", help="replay the batch as a hipGraph")
+    ap.add_argument("--profile", action="store_true",
+                    help="for counter collection: run the batch --steps times and a 512 MiB device copy "
+                         "(the calibration kernel for FETCH_SIZE / WRITE_SIZE), print nothing else")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(walks shard trivially across GPUs, but only the one-GPU path is built)", file=sys.stderr)
+        sys.exit(2)
+
+    import dreams_rns as D
+    from dreams_rns import synth
+    from dreams_rns.engine import BatchPlan, pcf_spec
+
+    t0 = time.perf_counter()
+    cfg = synth.pcf2(n_walks=args.walks)
+    depth, K, top_k = cfg["depth"], cfg["K"], cfg["top_k"]
+    progs = [D.compile_pcf(w["a"], w["b"]) for w in cfg["walks"]]
+    specs = [pcf_spec(p, D.pcf_initial_values(w["a"]), depth, K, 1, target=w["L"])
+             for p, w in zip(progs, cfg["walks"])]
+    work = sum(program_work(p) * depth * K for p in progs)
+    rsteps = float(len(specs) * depth * K)
+    log(f"compiled {len(progs)} PCFs in {time.perf_counter() - t0:.1f}s")
+
+    plan = BatchPlan(specs, crt=True, top_k=top_k, threads=args.threads, chunk_steps=args.chunk,
+                     use_graph=args.graph)
+    ctx = plan.ctx
+    if args.profile:
+        for _ in range(args.steps):
+            plan.run()
+        plan.close()
+        import torch
+
+        src = torch.ones(512 << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("profile run done: batch x%d, copy 512 MiB" % args.steps)
+        return
+    for _ in range(args.warmup):
+        plan.run()
+    walk_ms, step_ms = [], []
+    for i in range(args.steps):
+        t = time.perf_counter()
+        plan.run()
+        step_ms.append((time.perf_counter() - t) * 1e3)
+        tm = ctx.last_timing()
+        walk_ms.append(tm[0])
+        log(f"step {i}: {step_ms[-1]:.2f} ms (walk kernel {tm[0]:.2f} ms, combine {tm[1]:.2f}, "
+            f"shadow {tm[2]:.2f}, tail {tm[3]:.2f})")
+    res = plan.result
+    ms = float(np.mean(step_ms))
+    value = rsteps / (ms * 1e-3)
+    walk_avg = float(np.mean(walk_ms)) if not args.graph else float("nan")
+    # bit-exactness spot check of the timed output against the C restatement
+    from oracle import cref
+
+    primes = D.generate_rns_primes(K)
+    check = [0, len(specs) // 2, len(specs) - 1]
+    for i in check:
+        rr, _, _ = cref.pcf_walk(specs[i].program, specs[i].a0, depth, primes)
+        assert res.residues[i, 0, :K].tolist() == rr[0, 1].tolist(), f"walk {i} p residues differ"
+        assert res.residues[i, 1, :K].tolist() == rr[1, 1].tolist(), f"walk {i} q residues differ"
+    log(f"spot-checked walks {check} against the C restatement: bit-exact")
+    plan.close()
+
+    # plain comparator: one walk per launch sequence, synchronous
+    n_plain = min(args.plain_sample, len(specs))
+    pplan = BatchPlan(specs[:n_plain], mode="plain", crt=True)
+    pplan.run()
+    t = time.perf_counter()
+    pplan.run()
+    plain_s = time.perf_counter() - t
+    plain = n_plain * depth * K / plain_s
+    pplan.close()
+    log(f"plain: {n_plain} walks in {plain_s * 1e3:.1f} ms -> {plain:.3e} residue-steps/s")
+
+    served = None
+    if not args.no_serve:
+        served = served_run(specs[:512], args.serve_rate, args.serve_seconds, round_steps=1000)
+        log(f"served: {served}")
+
+    cpu = None
+    if not args.no_cpu:
+        cores = min(16, os.cpu_count() or 1)
+        one, allc = cpu_baseline(cfg["walks"], depth, K, args.cpu_one, args.cpu_all, cores)
+        cpu = {"value": allc, "unit": "residue-steps/s", "cores": cores, "kind": "port",
+               "one_core": one,
+               "sample": f"{args.cpu_all} pcf2 walks (all cores) and {args.cpu_one} (one core) at depth "
+                         f"{depth}, K={K}: numpy restatement of run_pcf_walk + crt_reconstruct + centered"}
+        log(f"cpu: one core {one:.3e}, {cores} cores {allc:.3e} residue-steps/s")
+
+    achieved = work / (walk_avg * 1e-3) if walk_avg == walk_avg else None
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "residue-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"pcf2: {len(specs)} seeded PCFs, deg a<=3, deg b<=6, coef [-20,20], "
+                               f"depth {depth}, K={K}, FP64 shadow + delta_float, GPU Garner CRT, top-{top_k}",
+                   "walks": len(specs), "depth": depth, "K": K, "top_k": top_k,
+                   "threads": args.threads or 256, "chunk_steps": args.chunk or 4096, "graph": bool(args.graph)},
+        "latency_batch_ms": {"p50": float(np.percentile(step_ms, 50)), "p99": float(np.percentile(step_ms, 99))},
+        "plain": {"value": plain, "sample_walks": n_plain, "x": value / plain},
+        "served": served,
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12 if achieved else None,
+                     "peak": PEAK_LANE_OPS / 1e12, "unit": "TOP/s",
+                     "frac": achieved / PEAK_LANE_OPS if achieved else None, "traffic": None,
+                     "kernel": "k_walk_chunk<2>", "kernel_ms": walk_avg,
+                     "work_per_launch": work},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        line["x_cpu"] = value / cpu["value"]
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -163,6 +163,7 @@ struct drx_plan {
   int n = 0, kmax = 0, ld = 0;
   struct Group {
     int m;
+    int fd;
     int task_begin, task_end;
     int span_begin, span_end;
     int threads;
@@ -206,7 +207,7 @@ int launch_sequence(drx_plan* pl, bool timed) {
   for (const auto& g : pl->groups) {
     KArgs a = pl->args;
     a.tasks += g.task_begin;
-    HIPCHK(ctx, launch_walk(g.m, a, g.task_end - g.task_begin, g.threads, g.lds, s));
+    HIPCHK(ctx, launch_walk(g.m, g.fd, a, g.task_end - g.task_begin, g.threads, g.lds, s));
   }
   if (timed) HIPCHK(ctx, hipEventRecord(ev[1], s));
   for (const auto& g : pl->groups) {
@@ -274,9 +275,14 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
     const drx_walk& w = b.walks[i];
     return (double)w.depth * (double)w.K * (1.0 + ctx->progs[w.prog].pk.code.size());
   };
+  const bool allow_fd = !pl->cfg.interp_only;
+  auto gkey = [&](int i) {
+    const HostProgram& hp = ctx->progs[b.walks[i].prog];
+    return (int)hp.dev.m * 2 + ((allow_fd && hp.fd) ? 1 : 0);
+  };
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-    const int mx = ctx->progs[b.walks[x].prog].dev.m, my = ctx->progs[b.walks[y].prog].dev.m;
-    if (mx != my) return mx < my;
+    const int kx = gkey(x), ky = gkey(y);
+    if (kx != ky) return kx < ky;
     return cost(x) > cost(y);
   });
 
@@ -324,9 +330,11 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
     // tasks
     const bool plain = pl->cfg.mode == DRX_MODE_PLAIN;
     const int64_t chunk = plain ? std::max<int64_t>(w.depth, 1) : chunk_cfg;
-    if (pl->groups.empty() || pl->groups.back().m != m) {
+    const int fdw = (allow_fd && hp.fd) ? 1 : 0;
+    if (pl->groups.empty() || pl->groups.back().m != m || pl->groups.back().fd != fdw) {
       drx_plan::Group g{};
       g.m = m;
+      g.fd = fdw;
       g.task_begin = g.task_end = (int)ht.size();
       g.span_begin = g.span_end = (int)hs.size();
       g.threads = plain ? (int)pow2_ceil(K) : std::max<int>(thr_cfg, pow2_ceil(K));
@@ -573,6 +581,7 @@ int drx_set_primes(drx_ctx* ctx, const int64_t* primes, int K) {
       if (pr[j] == (uint32_t)p) return fail(ctx, DRX_EINVAL, "prime %lld repeated", (long long)p);
     pr[i] = (uint32_t)p;
   }
+  std::lock_guard<std::mutex> lock(ctx->mu);
   if (!ctx->primes.empty() && (size_t)K >= ctx->primes.size() &&
       std::equal(ctx->primes.begin(), ctx->primes.end(), pr.begin()) && (size_t)K == ctx->primes.size())
     return DRX_OK;
@@ -607,6 +616,7 @@ int drx_program_upload(drx_ctx* ctx, const uint32_t* ops, int n_ops, int m, int 
   if (!ctx) return DRX_EINVAL;
   if (!prog_id || (n_ops > 0 && !ops) || n_ops < 0 || n_const < 0)
     return fail(ctx, DRX_EINVAL, "bad program arguments");
+  std::lock_guard<std::mutex> lock(ctx->mu);
   if (ctx->primes.empty()) return fail(ctx, DRX_ESTATE, "set primes before uploading programs");
   if (n_const > 0 && !consts && !(const_res && const_f))
     return fail(ctx, DRX_EINVAL, "constants missing (give consts or const_res + const_f)");
@@ -647,6 +657,10 @@ int drx_program_upload(drx_ctx* ctx, const uint32_t* ops, int n_ops, int m, int 
   d.colred = pk.colred;
   d.const_lds = (uint32_t)pk.const_lds;
   d.nslots = (uint32_t)pk.nslots;
+  d.fd_ok = (uint32_t)pk.fd_ok;
+  d.fd_degree = (uint32_t)pk.fd_degree;
+  d.fd_deg = pk.fd_deg;
+  hp.fd = pk.fd_ok && ((m == 2 && pk.fd_degree <= kFdMax2) || (m == 3 && pk.fd_degree <= kFdMax3));
   ctx->h_code.insert(ctx->h_code.end(), pk.code.begin(), pk.code.end());
   if (ctx->h_code.size() % 2) ctx->h_code.push_back(0);  // keep 8-byte alignment
   for (int s = 0; s < nc; ++s) {
@@ -677,6 +691,23 @@ int drx_program_upload(drx_ctx* ctx, const uint32_t* ops, int n_ops, int m, int 
     }
     if (terms > 1) mao += m * (terms - 1);
   }
+  if (hp.fd) {
+    // per step: the entry tables advance (one add per difference), the
+    // interpreted program runs only when the tables are seeded
+    mmo = 0;
+    mao = 0;
+    for (int e = 0; e < m * m; ++e)
+      if (((pk.kinds >> (2 * e)) & 3u) == E_GEN) mao += (int)((pk.fd_deg >> (4 * e)) & 15u);
+    for (int j = 0; j < m; ++j) {
+      int terms = 0;
+      for (int k = 0; k < m; ++k) {
+        const uint32_t kd = (pk.kinds >> (2 * (k * m + j))) & 3u;
+        if (kd == E_GEN) mmo += m;
+        if (kd != E_ZERO) ++terms;
+      }
+      if (terms > 1) mao += m * (terms - 1);
+    }
+  }
   hp.mm_ops = mmo;
   hp.ma_ops = mao;
   ctx->progs.push_back(std::move(hp));
@@ -696,9 +727,9 @@ int drx_program_info(drx_ctx* ctx, int prog_id, int* n_dev_ops, int* n_const_use
 }
 
 int drx_program_pack(const uint32_t* ops, int n_ops, int m, int dim, const int64_t* consts, int n_const,
-                     uint32_t* words, int max_words, int* const_slots, int max_slots, int64_t* meta6, char* err,
+                     uint32_t* words, int max_words, int* const_slots, int max_slots, int64_t* meta8, char* err,
                      int err_len) {
-  if ((n_ops > 0 && !ops) || n_ops < 0 || n_const < 0 || (n_const > 0 && !consts) || !meta6) return DRX_EINVAL;
+  if ((n_ops > 0 && !ops) || n_ops < 0 || n_const < 0 || (n_const > 0 && !consts) || !meta8) return DRX_EINVAL;
   std::vector<char> is_one(n_const, 0), is_zero(n_const, 0);
   for (int i = 0; i < n_const; ++i) {
     is_one[i] = consts[i] == 1;
@@ -710,12 +741,14 @@ int drx_program_pack(const uint32_t* ops, int n_ops, int m, int dim, const int64
     if (err && err_len > 0) std::snprintf(err, (size_t)err_len, "%s", msg.c_str());
     return DRX_EINVAL;
   }
-  meta6[0] = (int64_t)pk.code.size();
-  meta6[1] = pk.kinds;
-  meta6[2] = pk.colred;
-  meta6[3] = pk.const_lds;
-  meta6[4] = pk.nslots;
-  meta6[5] = (int64_t)pk.const_slots.size();
+  meta8[0] = (int64_t)pk.code.size();
+  meta8[1] = pk.kinds;
+  meta8[2] = pk.colred;
+  meta8[3] = pk.const_lds;
+  meta8[4] = pk.nslots;
+  meta8[5] = (int64_t)pk.const_slots.size();
+  meta8[6] = pk.fd_ok;
+  meta8[7] = pk.fd_degree;
   if (words) std::copy(pk.code.begin(), pk.code.begin() + std::min<size_t>(pk.code.size(), max_words), words);
   if (const_slots)
     for (size_t i = 0; i < pk.const_slots.size() && (int)i < max_slots; ++i) const_slots[i] = pk.const_slots[i];
@@ -724,6 +757,7 @@ int drx_program_pack(const uint32_t* ops, int n_ops, int m, int dim, const int64
 
 int drx_program_release_all(drx_ctx* ctx) {
   if (!ctx) return DRX_EINVAL;
+  std::lock_guard<std::mutex> lock(ctx->mu);
   ctx->progs.clear();
   ctx->h_code.clear();
   ctx->h_cres.clear();
@@ -737,6 +771,7 @@ int drx_plan_create(drx_ctx* ctx, const drx_batch* b, const drx_launch_cfg* cfg,
   if (!b || !out || b->n < 0 || (b->n > 0 && (!b->walks || !b->out || !b->residues)))
     return fail(ctx, DRX_EINVAL, "bad batch arguments");
   *out = nullptr;
+  std::lock_guard<std::mutex> lock(ctx->mu);
   HIPCHK(ctx, hipSetDevice(ctx->device));
   drx_plan* pl = new (std::nothrow) drx_plan();
   if (!pl) return fail(ctx, DRX_ENOMEM, "out of host memory");

@@ -54,7 +54,11 @@ struct GarnerArgs {
   uint8_t* neg;           // [n] 1 if the centred value is negative
 };
 
-hipError_t launch_walk(int m, const KArgs& a, int ntask, int threads, size_t lds, hipStream_t s);
+// Forward-difference table depth of the polynomial walk kernel per matrix order.
+constexpr int kFdMax2 = 6;
+constexpr int kFdMax3 = 3;
+
+hipError_t launch_walk(int m, int fd, const KArgs& a, int ntask, int threads, size_t lds, hipStream_t s);
 hipError_t launch_combine(int m, const KArgs& a, int nspan, hipStream_t s);
 hipError_t launch_shadow(int m, const KArgs& a, int nspan, int finalize, hipStream_t s);
 hipError_t launch_finalize(const KArgs& a, int nspan, int full, hipStream_t s);

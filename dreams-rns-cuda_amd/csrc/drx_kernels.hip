@@ -124,7 +124,7 @@ __device__ __forceinline__ uint32_t pow2_at_least(uint32_t k) {
   return kp;
 }
 
-// ----------------------------------------------------------- walk kernel
+// ----------------------------------------------------------- walk kernels
 // One workgroup per task (a step range of one walk). Lane (c, k): sub-chunk
 // c of the range, prime k. Each lane multiplies M(s) over its sub-chunk; the
 // C sub-chunk products are then combined in order by a log2(C)-level tree in
@@ -133,85 +133,95 @@ __device__ __forceinline__ uint32_t pow2_at_least(uint32_t k) {
 //
 // LDS: [nconst][KP] constant residues + [nconst] FP64 constants (bank L),
 // [nslots][T] temps (bank S), then the tree buffer [C][M*M][KP].
-template <int M>
-__global__ void __launch_bounds__(256) k_walk_chunk(const KArgs a) {
-  extern __shared__ double lds[];
-  const DevTask t = a.tasks[blockIdx.x];
-  const DevWalk w = a.walks[t.walk];
-  const DevProgram pg = a.progs[w.prog];
-  const uint32_t T = blockDim.x;
-  const uint32_t tid = threadIdx.x;
-  const uint32_t K = (uint32_t)w.K;
-  const uint32_t KP = pow2_at_least(K);
-  const uint32_t C = T / KP;
-  const uint32_t k = tid % KP;
-  const uint32_t c = tid / KP;
-  const bool active = k < K;
-  const uint32_t kk = active ? k : 0u;
+
+struct WalkSetup {
+  DevTask t;
+  DevWalk w;
+  DevProgram pg;
   Lane ln;
-  ln.p = a.prime_p[kk];
-  ln.pinv = a.prime_pinv[kk];
-  ln.pe = a.prime_e[kk];
-  ln.k = k;
-  ln.tid = tid;
-  ln.T = T;
-  const double p = ln.p, pinv = ln.pinv;
-  const uint2* __restrict__ code = reinterpret_cast<const uint2*>(a.code + pg.code_off);
-  const uint32_t dim = pg.dim;
-  const uint32_t nl = pg.const_lds ? pg.nconst : 0u;
+  uint32_t K, KP, C, c, k;
+  bool active;
+  int64_t sb, se;
+  const uint2* code;
+  double *ldc, *ldcf, *lds_s, *tree;
+};
 
-  double* ldc = lds;
-  double* ldcf = ldc + (uint64_t)nl * KP;
-  double* lds_s = ldcf + nl;
-  double* tree = lds_s + (uint64_t)pg.nslots * T;
-  for (uint32_t i = tid; i < nl * KP; i += T) {
-    const uint32_t ci = i / KP, ck = i % KP;
-    ldc[i] = (ck < K) ? a.cres[pg.cres_off + (uint64_t)ci * pg.kcap + ck] : 0.0;
+template <int M>
+__device__ __forceinline__ WalkSetup walk_setup(const KArgs& a, double* lds) {
+  WalkSetup S;
+  S.t = a.tasks[blockIdx.x];
+  S.w = a.walks[S.t.walk];
+  S.pg = a.progs[S.w.prog];
+  const uint32_t T = blockDim.x, tid = threadIdx.x;
+  S.K = (uint32_t)S.w.K;
+  S.KP = pow2_at_least(S.K);
+  S.C = T / S.KP;
+  S.k = tid % S.KP;
+  S.c = tid / S.KP;
+  S.active = S.k < S.K;
+  const uint32_t kk = S.active ? S.k : 0u;
+  S.ln.p = a.prime_p[kk];
+  S.ln.pinv = a.prime_pinv[kk];
+  S.ln.pe = a.prime_e[kk];
+  S.ln.k = S.k;
+  S.ln.tid = tid;
+  S.ln.T = T;
+  S.code = reinterpret_cast<const uint2*>(a.code + S.pg.code_off);
+  const uint32_t nl = S.pg.const_lds ? S.pg.nconst : 0u;
+  S.ldc = lds;
+  S.ldcf = S.ldc + (uint64_t)nl * S.KP;
+  S.lds_s = S.ldcf + nl;
+  S.tree = S.lds_s + (uint64_t)S.pg.nslots * T;
+  for (uint32_t i = tid; i < nl * S.KP; i += T) {
+    const uint32_t ci = i / S.KP, ck = i % S.KP;
+    S.ldc[i] = (ck < S.K) ? a.cres[S.pg.cres_off + (uint64_t)ci * S.pg.kcap + ck] : 0.0;
   }
-  for (uint32_t i = tid; i < nl; i += T) ldcf[i] = a.cflt[pg.cflt_off + i];
+  for (uint32_t i = tid; i < nl; i += T) S.ldcf[i] = a.cflt[S.pg.cflt_off + i];
   __syncthreads();
+  const int64_t L = S.t.s_end - S.t.s_begin;
+  const int64_t len = (L + S.C - 1) / S.C;
+  S.sb = S.t.s_begin + (int64_t)S.c * len;
+  if (S.sb > S.t.s_end || !S.active) S.sb = S.t.s_end;
+  S.se = S.sb + len;
+  if (S.se > S.t.s_end) S.se = S.t.s_end;
+  return S;
+}
 
-  v16d R = 0.0, X = 0.0, Cv = 0.0;
-  typename EBank<M>::type E = 0.0;
-
-  // ---- residue pass
-  const int64_t L = t.s_end - t.s_begin;
-  const int64_t len = (L + C - 1) / C;
-  int64_t sb = t.s_begin + (int64_t)c * len;
-  if (sb > t.s_end || !active) sb = t.s_end;
-  int64_t se = sb + len;
-  if (se > t.s_end) se = t.s_end;
-
-  if (!pg.const_lds) {
+// Residue constants into the VGPR bank; axis values at step s0 and per-step
+// increments into the X bank.
+__device__ __forceinline__ void load_rns_inputs(const KArgs& a, const WalkSetup& S, int64_t s0, v16d& X,
+                                                v16d& Cv) {
+  const double p = S.ln.p, pinv = S.ln.pinv;
+  const uint32_t kk = S.active ? S.k : 0u;
+  if (!S.pg.const_lds) {
 #pragma unroll
     for (int i = 0; i < 16; ++i)
-      if ((uint32_t)i < pg.nconst) Cv[i] = a.cres[pg.cres_off + (uint64_t)i * pg.kcap + kk];
+      if ((uint32_t)i < S.pg.nconst) Cv[i] = a.cres[S.pg.cres_off + (uint64_t)i * S.pg.kcap + kk];
   }
 #pragma unroll
   for (int ax = 0; ax < kMaxDim; ++ax) {
-    if ((uint32_t)ax < dim) {
-      const double* q = a.axr + w.axr_off + (uint64_t)ax * 3 * K;
-      const double c0 = q[kk], c1 = q[K + kk], c2 = q[2 * K + kk];
-      const double x0 = red(c0 + mm((double)sb, c1, p, pinv), p, pinv);
+    if ((uint32_t)ax < S.pg.dim) {
+      const double* q = a.axr + S.w.axr_off + (uint64_t)ax * 3 * S.K;
+      const double c0 = q[kk], c1 = q[S.K + kk], c2 = q[2 * S.K + kk];
+      const double x0 = red(c0 + mm((double)s0, c1, p, pinv), p, pinv);
       X[ax] = mm(x0, c2, p, pinv);
       X[kIncSlot + ax] = mm(c1, c2, p, pinv);
     }
   }
+}
 
-  double P[M * M];
+__device__ __forceinline__ void advance_axes(v16d& X, uint32_t dim, double p, double pinv) {
 #pragma unroll
-  for (int e = 0; e < M * M; ++e) P[e] = (e % (M + 1) == 0) ? 1.0 : 0.0;
+  for (int ax = 0; ax < kMaxDim; ++ax)
+    if ((uint32_t)ax < dim) X[ax] = red(X[ax] + X[kIncSlot + ax], p, pinv);
+}
 
-  for (int64_t s = sb; s < se; ++s) {
-    X[kStepSlot] = (double)s;
-    run_program<false>(code, pg.ncode, R, E, X, Cv, ldc, KP, lds_s, ln);
-    step_matmul<M>(P, E, pg.kinds, pg.colred, p, pinv);
-#pragma unroll
-    for (int ax = 0; ax < kMaxDim; ++ax)
-      if ((uint32_t)ax < dim) X[ax] = red(X[ax] + X[kIncSlot + ax], p, pinv);
-  }
-
-  // ---- ordered tree combine of the C sub-chunk products
+// Ordered tree combine of the C sub-chunk products; lane (0, k) ends with the
+// product over the whole task and writes it to the chunk-product pool.
+template <int M>
+__device__ __forceinline__ void tree_combine(const KArgs& a, const WalkSetup& S, double (&P)[M * M]) {
+  const uint32_t c = S.c, k = S.k, KP = S.KP, C = S.C;
+  double* tree = S.tree;
 #pragma unroll
   for (int e = 0; e < M * M; ++e) tree[((uint64_t)c * M * M + e) * KP + k] = P[e];
   for (uint32_t stride = 1; stride < C; stride <<= 1) {
@@ -220,40 +230,154 @@ __global__ void __launch_bounds__(256) k_walk_chunk(const KArgs a) {
       double Q[M * M];
 #pragma unroll
       for (int e = 0; e < M * M; ++e) Q[e] = tree[((uint64_t)(c + stride) * M * M + e) * KP + k];
-      gen_matmul<M>(P, Q, p, pinv);
+      gen_matmul<M>(P, Q, S.ln.p, S.ln.pinv);
 #pragma unroll
       for (int e = 0; e < M * M; ++e) tree[((uint64_t)c * M * M + e) * KP + k] = P[e];
     }
   }
-  if (c == 0 && active) {
+  if (c == 0 && S.active) {
 #pragma unroll
-    for (int e = 0; e < M * M; ++e) a.cp[t.cp_off + (uint64_t)e * K + k] = P[e];
+    for (int e = 0; e < M * M; ++e) a.cp[S.t.cp_off + (uint64_t)e * S.K + k] = P[e];
   }
+}
 
-  // ---- FP64 shadow matrices M_f(s), one step per lane
-  if (!a.with_fp) return;
-  if (!pg.const_lds) {
+// FP64 shadow matrices M_f(s) for every step of the task, one step per lane.
+template <int M>
+__device__ __forceinline__ void fp_pass(const KArgs& a, const WalkSetup& S) {
+  const uint32_t T = blockDim.x, tid = threadIdx.x;
+  typename EBank<M>::type E = 0.0;
+  v16d R = 0.0, X = 0.0, Cv = 0.0;
+  if (!S.pg.const_lds) {
 #pragma unroll
     for (int i = 0; i < 16; ++i)
-      if ((uint32_t)i < pg.nconst) Cv[i] = a.cflt[pg.cflt_off + i];
+      if ((uint32_t)i < S.pg.nconst) Cv[i] = a.cflt[S.pg.cflt_off + i];
   }
-  for (int64_t s = t.s_begin + tid; s < t.s_end; s += T) {
+  for (int64_t s = S.t.s_begin + tid; s < S.t.s_end; s += T) {
 #pragma unroll
     for (int ax = 0; ax < kMaxDim; ++ax) {
-      if ((uint32_t)ax < dim) {
-        const int64_t* q = a.axi + w.axi_off + 2 * ax;
-        X[ax] = (double)(q[0] + s * q[1]) / a.axd[w.axd_off + ax];
+      if ((uint32_t)ax < S.pg.dim) {
+        const int64_t* q = a.axi + S.w.axi_off + 2 * ax;
+        X[ax] = (double)(q[0] + s * q[1]) / a.axd[S.w.axd_off + ax];
       }
     }
     X[kStepSlot] = (double)s;
-    run_program<true>(code, pg.ncode, R, E, X, Cv, ldcf, 0, lds_s, ln);
-    double* out = a.mf + t.mf_off + (uint64_t)(s - t.s_begin) * M * M;
+    run_program<true>(S.code, S.pg.ncode, R, E, X, Cv, S.ldcf, 0, S.lds_s, S.ln);
+    double* out = a.mf + S.t.mf_off + (uint64_t)(s - S.t.s_begin) * M * M;
 #pragma unroll
     for (int e = 0; e < M * M; ++e) {
-      const uint32_t kd = (pg.kinds >> (2 * e)) & 3u;
+      const uint32_t kd = (S.pg.kinds >> (2 * e)) & 3u;
       out[e] = (kd == E_GEN) ? E[e] : (kd == E_ONE ? 1.0 : 0.0);
     }
   }
+}
+
+// Interpreting walk: the program runs once per step and residue channel.
+template <int M>
+__global__ void __launch_bounds__(256) k_walk_chunk(const KArgs a) {
+  extern __shared__ double lds[];
+  const WalkSetup S = walk_setup<M>(a, lds);
+  const double p = S.ln.p, pinv = S.ln.pinv;
+  v16d R = 0.0, X = 0.0, Cv = 0.0;
+  typename EBank<M>::type E = 0.0;
+  load_rns_inputs(a, S, S.sb, X, Cv);
+  double P[M * M];
+#pragma unroll
+  for (int e = 0; e < M * M; ++e) P[e] = (e % (M + 1) == 0) ? 1.0 : 0.0;
+  for (int64_t s = S.sb; s < S.se; ++s) {
+    X[kStepSlot] = (double)s;
+    run_program<false>(S.code, S.pg.ncode, R, E, X, Cv, S.ldc, S.KP, S.lds_s, S.ln);
+    step_matmul<M>(P, E, S.pg.kinds, S.pg.colred, p, pinv);
+    advance_axes(X, S.pg.dim, p, pinv);
+  }
+  tree_combine<M>(a, S, P);
+  if (a.with_fp) fp_pass<M>(a, S);
+}
+
+// Forward-difference walk for programs whose general entries are all
+// polynomials in the step (no INV on their paths). For an entry of degree
+// d, the table D[e][0..d] holds its forward differences; one step is
+// D[i] += D[i+1] for i < d (exact modular arithmetic, so the entry values are
+// those the program would compute). The interpreter runs only to seed the
+// tables: d + 1 evaluations at the start of each sub-chunk. Tables are
+// reduced every fourth step, which keeps every entry below C(4 + d, d) * 2^31
+// (< 2^41 for d <= DMAX).
+template <int M, int DMAX>
+__global__ void __launch_bounds__(256) k_walk_fd(const KArgs a) {
+  extern __shared__ double lds[];
+  const WalkSetup S = walk_setup<M>(a, lds);
+  const double p = S.ln.p, pinv = S.ln.pinv;
+  const uint32_t kinds = S.pg.kinds;
+  const uint64_t degs = S.pg.fd_deg;
+  double D[M * M][DMAX + 1];
+  {
+    v16d R = 0.0, X = 0.0, Cv = 0.0;
+    typename EBank<M>::type E = 0.0;
+    load_rns_inputs(a, S, S.sb, X, Cv);
+#pragma unroll
+    for (int j = 0; j <= DMAX; ++j) {
+      if ((uint32_t)j <= S.pg.fd_degree) {
+        X[kStepSlot] = (double)(S.sb + j);
+        run_program<false>(S.code, S.pg.ncode, R, E, X, Cv, S.ldc, S.KP, S.lds_s, S.ln);
+        advance_axes(X, S.pg.dim, p, pinv);
+      }
+#pragma unroll
+      for (int e = 0; e < M * M; ++e) D[e][j] = ((uint32_t)j <= S.pg.fd_degree) ? E[e] : 0.0;
+    }
+  }
+  // Newton forward differences: D[e][j] <- Delta^j f(sb)
+#pragma unroll
+  for (int i = 1; i <= DMAX; ++i)
+#pragma unroll
+    for (int j = DMAX; j >= i; --j)
+#pragma unroll
+      for (int e = 0; e < M * M; ++e) D[e][j] = D[e][j] - D[e][j - 1];
+#pragma unroll
+  for (int e = 0; e < M * M; ++e)
+#pragma unroll
+    for (int j = 0; j <= DMAX; ++j) D[e][j] = red(D[e][j], p, pinv);
+
+  double P[M * M];
+#pragma unroll
+  for (int e = 0; e < M * M; ++e) P[e] = (e % (M + 1) == 0) ? 1.0 : 0.0;
+  int64_t s = S.sb;
+  uint32_t phase = 0;
+  for (; s < S.se; ++s) {
+    double Cm[M * M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+#pragma unroll
+      for (int j = 0; j < M; ++j) {
+        double acc = 0.0;
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+          const uint32_t kd = (kinds >> (2 * (q * M + j))) & 3u;
+          if (kd == E_GEN) acc += mm(P[i * M + q], D[q * M + j][0], p, pinv);
+          else if (kd == E_ONE) acc += P[i * M + q];
+        }
+        if ((S.pg.colred >> j) & 1u) acc = red(acc, p, pinv);
+        Cm[i * M + j] = acc;
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < M * M; ++e) P[e] = Cm[e];
+    // advance the tables of the general entries
+    phase = (phase + 1) & 3u;
+#pragma unroll
+    for (int e = 0; e < M * M; ++e) {
+      if (((kinds >> (2 * e)) & 3u) != E_GEN) continue;
+      const uint32_t d = (uint32_t)(degs >> (4 * e)) & 15u;
+#pragma unroll
+      for (int i = 0; i < DMAX; ++i)
+        if ((uint32_t)i < d) D[e][i] += D[e][i + 1];
+      if (phase == 0) {
+#pragma unroll
+        for (int i = 0; i < DMAX; ++i)
+          if ((uint32_t)i < d) D[e][i] = red(D[e][i], p, pinv);
+      }
+    }
+  }
+  tree_combine<M>(a, S, P);
+  if (a.with_fp) fp_pass<M>(a, S);
 }
 
 // --------------------------------------------------------- ordered combine
@@ -297,6 +421,10 @@ __global__ void k_shadow(const KArgs a, int nspan, int finalize) {
 #pragma unroll
   for (int e = 0; e < M * M; ++e) P[e] = st_in[e];
   double ls = st_in[M * M];
+  // The loop body is one branch-free block: the rescale is a select, and
+  // log(mx) of step s is added while step s + 1 is multiplied (it only feeds
+  // log_scale, in the same order as the reference adds it).
+  double pend = 1.0;  // mx awaiting its log; log(1) == 0 exactly
   for (int ti = sp.task_begin; ti < sp.task_end; ++ti) {
     const DevTask t = a.tasks[ti];
     const double* mf = a.mf + t.mf_off;
@@ -314,6 +442,7 @@ __global__ void k_shadow(const KArgs a, int nspan, int finalize) {
           for (int q = 1; q < M; ++q) acc = __builtin_fma(P[r * M + q], Mf[q * M + cc], acc);
           C[r * M + cc] = acc;
         }
+      ls += log(pend);
       double mx = 0.0;
       bool nan = false;
 #pragma unroll
@@ -322,21 +451,13 @@ __global__ void k_shadow(const KArgs a, int nspan, int finalize) {
         nan |= (v != v);
         mx = (v > mx) ? v : mx;
       }
-      // The rescale is taken by most lanes at most steps; a wave-uniform
-      // branch with per-lane selects keeps the control flow uniform (a
-      // divergent branch here was miscompiled for M = 3: loop-carried
-      // registers were clobbered by the next step's loads).
       const bool resc = !nan && mx > 1e10;
-      if (__builtin_amdgcn_ballot_w64(resc) != 0) {
-        const double l = log(resc ? mx : 1.0);  // log(1) == 0 exactly
+      pend = resc ? mx : 1.0;
 #pragma unroll
-        for (int e = 0; e < M * M; ++e) C[e] = resc ? C[e] / mx : C[e];
-        ls = resc ? ls + l : ls;
-      }
-#pragma unroll
-      for (int e = 0; e < M * M; ++e) P[e] = C[e];
+      for (int e = 0; e < M * M; ++e) P[e] = resc ? C[e] / mx : C[e];
     }
   }
+  ls += log(pend);
 #pragma unroll
   for (int e = 0; e < M * M; ++e) st[e] = P[e];
   st[M * M] = ls;
@@ -474,8 +595,14 @@ __global__ void k_topk(const double* __restrict__ d, int stride, int n, int k,
 }
 
 // ---------------------------------------------------------------- launchers
-hipError_t launch_walk(int m, const KArgs& a, int ntask, int threads, size_t lds, hipStream_t s) {
+hipError_t launch_walk(int m, int fd, const KArgs& a, int ntask, int threads, size_t lds, hipStream_t s) {
   if (ntask <= 0) return hipSuccess;
+  if (fd) {
+    if (m == 2) hipLaunchKernelGGL((k_walk_fd<2, kFdMax2>), dim3(ntask), dim3(threads), lds, s, a);
+    else if (m == 3) hipLaunchKernelGGL((k_walk_fd<3, kFdMax3>), dim3(ntask), dim3(threads), lds, s, a);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+  }
   if (m == 2) hipLaunchKernelGGL(k_walk_chunk<2>, dim3(ntask), dim3(threads), lds, s, a);
   else if (m == 3) hipLaunchKernelGGL(k_walk_chunk<3>, dim3(ntask), dim3(threads), lds, s, a);
   else if (m == 4) hipLaunchKernelGGL(k_walk_chunk<4>, dim3(ntask), dim3(threads), lds, s, a);

@@ -168,6 +168,51 @@ std::string pack_program(const uint32_t* ops, int n_ops, int m, int dim, int n_c
       stack.push_back(vals[v].b);
     }
   }
+  // polynomial degree of every needed value in the step (-1: not a polynomial)
+  {
+    std::vector<int> deg(nv, 0);
+    for (int v = 0; v < nv; ++v) {
+      const Val& x = vals[v];
+      switch (x.kind) {
+        case V_ZERO: deg[v] = 0; break;
+        case V_C: deg[v] = 0; break;
+        case V_X: deg[v] = 1; break;
+        case V_N: deg[v] = 1; break;
+        default: {
+          const int da = deg[x.a], db = deg[x.b];
+          if (x.op == D_INV || da < 0 || (db < 0 && (x.op == D_ADD || x.op == D_SUB || x.op == D_MUL))) {
+            deg[v] = -1;
+          } else if (x.op == D_ADD || x.op == D_SUB) {
+            deg[v] = std::max(da, db);
+          } else if (x.op == D_MUL) {
+            deg[v] = da + db;
+          } else if (x.op == D_SQR) {
+            deg[v] = 2 * da;
+          } else if (x.op == D_CUB) {
+            deg[v] = 3 * da;
+          } else {
+            deg[v] = da;  // NEG
+          }
+          if (deg[v] > 15) deg[v] = -1;  // beyond any table the kernels keep
+        }
+      }
+    }
+    int ok = 1, dmax = 0;
+    uint64_t packed = 0;
+    for (int e = 0; e < m * m; ++e) {
+      if (((pp.kinds >> (2 * e)) & 3u) != E_GEN) continue;
+      const int d = deg[entry[e]];
+      if (d < 0) {
+        ok = 0;
+        break;
+      }
+      dmax = std::max(dmax, d);
+      packed |= (uint64_t)d << (4 * e);
+    }
+    pp.fd_ok = ok;
+    pp.fd_degree = ok ? dmax : 0;
+    pp.fd_deg = ok ? packed : 0;
+  }
   for (int j = 0; j < m; ++j) {
     int nz = 0, ones = 0;
     for (int k = 0; k < m; ++k) {

@@ -17,6 +17,12 @@ struct PackedProgram {
   int const_lds = 0;                 // constants in LDS instead of the VGPR bank
   int nslots = 0;                    // LDS temp slots
   int n_src_ops = 0;                 // reference instructions before END
+  // Degree in the step of every general entry when the program has no INV
+  // on any path to an entry (then every entry is a polynomial in the step):
+  // 4 bits per entry; fd_degree = max over entries; fd_ok = 0 otherwise.
+  int fd_ok = 0;
+  int fd_degree = 0;
+  uint64_t fd_deg = 0;
   // per-instruction statistics of the reference program (for the roofline)
   int n_mul = 0, n_sqr = 0, n_cub = 0, n_inv = 0, n_add = 0, n_sub = 0, n_neg = 0;
 };

@@ -123,17 +123,19 @@ int round_once(drx_server* s, std::vector<int>& finished) {
   hipStream_t st = ctx->stream;
   finished.clear();
   struct TaskRef {
-    int slot, m;
+    int slot, m, fd;
   };
   std::vector<TaskRef> refs;
   for (int i = 0; i < s->nslots; ++i)
-    if (s->slots[i].used) refs.push_back({i, s->slots[i].m});
+    if (s->slots[i].used) refs.push_back({i, s->slots[i].m, ctx->progs[s->slots[i].w.prog].fd});
   if (refs.empty()) return DRX_OK;
-  std::stable_sort(refs.begin(), refs.end(), [](const TaskRef& a, const TaskRef& b) { return a.m < b.m; });
+  std::stable_sort(refs.begin(), refs.end(), [](const TaskRef& a, const TaskRef& b) {
+    return a.m != b.m ? a.m < b.m : a.fd < b.fd;
+  });
   std::vector<DevTask> ht;
   std::vector<DevSpan> hs, hf;
   struct G {
-    int m, t0, t1, threads;
+    int m, fd, t0, t1, threads;
     size_t lds;
   };
   std::vector<G> groups;
@@ -149,7 +151,8 @@ int round_once(drx_server* s, std::vector<int>& finished) {
     t.cp_off = (uint64_t)ht.size() * slot_res(s);
     t.mf_off = mf_total;
     mf_total += (uint64_t)(t.s_end - t.s_begin) * r.m * r.m;
-    if (groups.empty() || groups.back().m != r.m) groups.push_back({r.m, (int)ht.size(), (int)ht.size(), 0, 0});
+    if (groups.empty() || groups.back().m != r.m || groups.back().fd != r.fd)
+      groups.push_back({r.m, r.fd, (int)ht.size(), (int)ht.size(), 0, 0});
     G& g = groups.back();
     const int threads = std::max<int>(s->cfg.threads > 0 ? s->cfg.threads : 256, (int)pow2_ceil(dw.K));
     g.threads = std::max(g.threads, threads);
@@ -215,7 +218,7 @@ int round_once(drx_server* s, std::vector<int>& finished) {
   for (const G& g : groups) {
     KArgs ga = a;
     ga.tasks += g.t0;
-    DRX_HIPCHK(ctx, launch_walk(g.m, ga, g.t1 - g.t0, g.threads, g.lds, st));
+    DRX_HIPCHK(ctx, launch_walk(g.m, g.fd, ga, g.t1 - g.t0, g.threads, g.lds, st));
   }
   for (const G& g : groups) {
     KArgs ga = a;
@@ -429,6 +432,12 @@ int drx_server_pending(drx_server* s, int* n_queued, int* n_resident) {
     for (const SrvSlot& sl : s->slots) r += sl.used;
     *n_resident = r;
   }
+  return DRX_OK;
+}
+
+int drx_server_capacity(drx_server* s, int* n_slots) {
+  if (!s || !n_slots) return DRX_EINVAL;
+  *n_slots = s->nslots;
   return DRX_OK;
 }
 

@@ -24,6 +24,9 @@ struct DevProgram {
   uint32_t colred;      // bit j: reduce column j after P*M
   uint32_t const_lds;   // 1: constants live in LDS (bank L), 0: in the VGPR bank C
   uint32_t nslots;      // LDS temp slots (bank S)
+  uint32_t fd_ok;       // every general entry is a polynomial in the step
+  uint32_t fd_degree;   // max degree over entries
+  uint64_t fd_deg;      // 4 bits per entry: its degree
 };
 
 // One walk's resident state and parameters (device copy).

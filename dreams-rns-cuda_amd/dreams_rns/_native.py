@@ -60,7 +60,8 @@ class Batch(C.Structure):
 
 
 class LaunchCfg(C.Structure):
-    _fields_ = [("mode", C.c_int), ("threads", C.c_int), ("chunk_steps", C.c_int), ("use_graph", C.c_int)]
+    _fields_ = [("mode", C.c_int), ("threads", C.c_int), ("chunk_steps", C.c_int), ("use_graph", C.c_int),
+                ("interp_only", C.c_int)]
 
 
 class ServerCfg(C.Structure):
@@ -101,6 +102,7 @@ SIGNATURES = {
     "drx_server_poll": (C.c_int, [C.c_void_p, C.POINTER(ServerResult), C.c_void_p, C.c_int, C.c_int,
                                   C.POINTER(C.c_int)]),
     "drx_server_pending": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "drx_server_capacity": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "drx_server_destroy": (None, [C.c_void_p]),
 }
 
@@ -254,7 +256,7 @@ def pack_program(program):
     cap = 4 * len(program.opcodes) + 64
     words = np.zeros(cap, dtype=np.uint32)
     slots = np.zeros(max(1, len(program.constants)), dtype=np.int32)
-    meta = np.zeros(6, dtype=np.int64)
+    meta = np.zeros(8, dtype=np.int64)
     err = C.create_string_buffer(256)
     rc = lib.drx_program_pack(ops.ctypes.data, len(program.opcodes), program.m, program.dim, consts.ctypes.data,
                               len(program.constants), words.ctypes.data, cap, slots.ctypes.data, len(slots),
@@ -264,5 +266,5 @@ def pack_program(program):
     nw = int(meta[0])
     return words[:nw].copy(), [int(s) for s in slots[: int(meta[5])]], {
         "kinds": int(meta[1]), "colred": int(meta[2]), "const_lds": int(meta[3]),
-        "nslots": int(meta[4]), "nconst": int(meta[5])}
+        "nslots": int(meta[4]), "nconst": int(meta[5]), "fd_ok": int(meta[6]), "fd_degree": int(meta[7])}
 

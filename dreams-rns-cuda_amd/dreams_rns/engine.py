@@ -141,7 +141,7 @@ class BatchPlan:
 
     def __init__(self, specs: Sequence[WalkSpec], ctx: Optional[N.Context] = None, mode: str = "batched",
                  threads: int = 0, chunk_steps: int = 0, use_graph: bool = False, crt: bool = False,
-                 full: bool = False, top_k: int = 0):
+                 full: bool = False, top_k: int = 0, interp_only: bool = False):
         self.ctx = ctx or N.context()
         n = len(specs)
         if n == 0:
@@ -223,7 +223,8 @@ class BatchPlan:
         b.crt_neg = r.neg.ctypes.data if crt else None
         b.top_k = int(top_k)
         b.topk_idx = r.topk.ctypes.data if top_k else None
-        cfg = N.LaunchCfg(self.MODES[mode], int(threads), int(chunk_steps), 1 if use_graph else 0)
+        cfg = N.LaunchCfg(self.MODES[mode], int(threads), int(chunk_steps), 1 if use_graph else 0,
+                          1 if interp_only else 0)
         self._b, self._cfg = b, cfg
         h = C.c_void_p()
         ctx._check(ctx.lib.drx_plan_create(ctx.h, C.byref(b), C.byref(cfg), C.byref(h)))

@@ -63,11 +63,12 @@ int drx_program_release_all(drx_ctx* ctx);
  * packer): writes up to max_words device words (two per instruction) and the
  * packing metadata: meta[0] words, [1] entry kinds (2 bits per entry),
  * [2] columns reduced after P*M, [3] constants in LDS, [4] LDS temp slots,
- * [5] constants used; const_slots[i] = program constant index of slot i
+ * [5] constants used, [6] all entries polynomial (forward differences
+ * apply), [7] their maximal degree; const_slots[i] = program constant index of slot i
  * (up to max_slots entries). Returns DRX_EINVAL with the packer's message in
  * err (err_len bytes) for programs the engine cannot run. */
 int drx_program_pack(const uint32_t* ops, int n_ops, int m, int dim, const int64_t* consts, int n_const,
-                     uint32_t* words, int max_words, int* const_slots, int max_slots, int64_t* meta6,
+                     uint32_t* words, int max_words, int* const_slots, int max_slots, int64_t* meta8,
                      char* err, int err_len);
 
 /* ---- walks ------------------------------------------------------------- */
@@ -125,6 +126,7 @@ typedef struct drx_launch_cfg {
   int threads;      /* workgroup size, power of two in [32, 256]; 0 = tuned */
   int chunk_steps;  /* max steps of one walk per workgroup task; 0 = tuned */
   int use_graph;    /* capture the launch sequence in a hipGraph and replay it */
+  int interp_only;  /* 1: interpret every step even for polynomial programs (no forward differences) */
 } drx_launch_cfg;
 
 /* Run a batch of walks to completion. Replaces run_pcf_walk (runner.py:338),
@@ -193,6 +195,8 @@ int drx_server_submit(drx_server* s, const drx_walk* w, const int64_t* axis_vals
 int drx_server_poll(drx_server* s, drx_server_result* res, int64_t* residues, int stride, int max,
                     int* n_done);
 int drx_server_pending(drx_server* s, int* n_queued, int* n_resident);
+/* Resident slots the memory budget (and max_slots) allow. */
+int drx_server_capacity(drx_server* s, int* n_slots);
 void drx_server_destroy(drx_server* s);
 
 #ifdef __cplusplus

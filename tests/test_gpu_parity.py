@@ -172,3 +172,31 @@ def test_cmf3_shapes_against_c_oracle(gpu_ctx):
         d = res.walk_dict(i)
         assert _close(d["p_float"], Pf[0, m - 1]) and _close(d["q_float"], Pf[m - 1, m - 1]), i
         assert _close(d["log_scale"], ls), i
+
+
+def test_forward_difference_path_equals_interpreter(gpu_ctx):
+    # polynomial programs run on the forward-difference kernel; forcing the
+    # interpreter must give the same residues and shadow, bit for bit
+    cfg = synth.pcf2(n_walks=40, seed=77)
+    specs = [pcf_spec(D.compile_pcf(w["a"], w["b"]), D.pcf_initial_values(w["a"]), d, K, 1, target=w["L"])
+             for w, d, K in zip(cfg["walks"], [1, 2, 3, 5, 7, 64, 333, 2000] * 5, [8, 16, 32, 64, 31] * 8)]
+    cf = lambda mat, m, dim, axes, dirs: D.compile_cmf_from_dict(mat, m, dim, axes, dirs)
+    rng = np.random.default_rng(3)
+    for _ in range(6):  # polynomial 3x3 CMFs (degree <= 3)
+        mat = {(i, j): synth._mpoly_str(rng, ("x", "y", "z"), 1, 2, -9, 9) for i in range(3) for j in range(3)}
+        p = cf(mat, 3, 3, ["x", "y", "z"], [1, -1, 2])
+        specs.append(cmf_spec(p, int(rng.integers(10, 900)), 32, [(1, 3), 2, 0], [(1, 2), 1, -1]))
+    a = BatchPlan(specs, full=True, crt=True).run()
+    b = BatchPlan(specs, full=True, crt=True, interp_only=True).run()
+    assert np.array_equal(a.full, b.full)
+    assert np.array_equal(a.full_f, b.full_f)
+    assert np.array_equal(a.limbs, b.limbs) and np.array_equal(a.neg, b.neg)
+    primes = D.generate_rns_primes(64)
+    for i in (0, 5, 7, 39, 40, 45):
+        s = specs[i]
+        rr, Pf, ls = cref.walk(s.program, s.depth, primes[: s.K], list(s.axes),
+                               *((np.array([[1] * s.K, [s.a0 % int(q) for q in primes[: s.K]], [0] * s.K,
+                                            [1] * s.K]), np.array([1.0, float(s.a0), 0.0, 1.0]))
+                                 if s.init == "pcf" else (None, None)))
+        m = s.program.m
+        assert np.array_equal(a.full_matrix(i, m), rr), i
