@@ -122,8 +122,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--walks", type=int, default=4096)
     ap.add_argument("--plain-sample", type=int, default=32)
-    ap.add_argument("--cpu-one", type=int, default=8)
-    ap.add_argument("--cpu-all", type=int, default=64)
+    ap.add_argument("--cpu-one", type=int, default=32)
+    ap.add_argument("--cpu-all", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-rate", type=float, default=1000.0)
@@ -161,13 +161,8 @@ def main():
         for _ in range(args.steps):
             plan.run()
         plan.close()
-        import torch
-
-        src = torch.ones(512 << 20, dtype=torch.uint8, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log("profile run done: batch x%d, copy 512 MiB" % args.steps)
+        ms = ctx.copy_calibrate(512 << 20, reps=1)
+        log("profile run done: batch x%d, copy 512 MiB (%.3f ms)" % (args.steps, ms))
         return
     for _ in range(args.warmup):
         plan.run()

@@ -68,10 +68,10 @@ int upload_programs(drx_ctx* ctx) {
   const size_t np = ctx->progs.size();
   std::vector<DevProgram> dp(np);
   for (size_t i = 0; i < np; ++i) dp[i] = ctx->progs[i].dev;
-  HIPCHK(ctx, ctx->d_code.ensure(std::max<size_t>(1, ctx->h_code.size()) * 4));
-  HIPCHK(ctx, ctx->d_cres.ensure(std::max<size_t>(1, ctx->h_cres.size()) * 8));
-  HIPCHK(ctx, ctx->d_cflt.ensure(std::max<size_t>(1, ctx->h_cflt.size()) * 8));
-  HIPCHK(ctx, ctx->d_progs.ensure(std::max<size_t>(1, np) * sizeof(DevProgram)));
+  HIPCHK(ctx, ctx->d_code.ensure_keep(std::max<size_t>(1, ctx->h_code.size()) * 4, ctx->retired));
+  HIPCHK(ctx, ctx->d_cres.ensure_keep(std::max<size_t>(1, ctx->h_cres.size()) * 8, ctx->retired));
+  HIPCHK(ctx, ctx->d_cflt.ensure_keep(std::max<size_t>(1, ctx->h_cflt.size()) * 8, ctx->retired));
+  HIPCHK(ctx, ctx->d_progs.ensure_keep(std::max<size_t>(1, np) * sizeof(DevProgram), ctx->retired));
   if (!ctx->h_code.empty())
     HIPCHK(ctx, hipMemcpy(ctx->d_code.p, ctx->h_code.data(), ctx->h_code.size() * 4, hipMemcpyHostToDevice));
   if (!ctx->h_cres.empty())
@@ -84,16 +84,36 @@ int upload_programs(drx_ctx* ctx) {
 }
 
 // p_j^{-1} mod p_i for the first k primes (lazily extended).
+// Inverse of a mod m for coprime a, m (extended Euclid).
+static uint32_t inv_mod(uint32_t a, uint32_t m) {
+  int64_t r0 = m, r1 = a % m, t0 = 0, t1 = 1;
+  while (r1) {
+    const int64_t q = r0 / r1, r2 = r0 - q * r1, t2 = t0 - q * t1;
+    r0 = r1;
+    r1 = r2;
+    t0 = t1;
+    t1 = t2;
+  }
+  return (uint32_t)(t0 < 0 ? t0 + m : t0);
+}
+
+// Garner's table inv[j][i] = p_j^{-1} mod p_i, built once for every prime
+// of the context so the device copy never moves while plans hold it.
 int ensure_inv_table(drx_ctx* ctx, int k) {
   if (k <= ctx->kinv) return DRX_OK;
-  std::vector<uint32_t> t((size_t)k * k, 0);
-  for (int j = 0; j < k; ++j)
-    for (int i = 0; i < k; ++i)
-      if (i != j) t[(size_t)j * k + i] = powmod(ctx->primes[j] % ctx->primes[i], ctx->primes[i] - 2, ctx->primes[i]);
+  const int kall = (int)ctx->primes.size();
+  if (k > kall) return fail(ctx, DRX_EINVAL, "K=%d exceeds the %d primes set", k, kall);
+  std::vector<uint32_t> t((size_t)kall * kall, 0);
+  for (int j = 0; j < kall; ++j)
+    for (int i = 0; i < kall; ++i)
+      if (i != j) t[(size_t)j * kall + i] = inv_mod(ctx->primes[j] % ctx->primes[i], ctx->primes[i]);
+  if (ctx->d_inv.p) ctx->retired.push_back(ctx->d_inv.p);
+  ctx->d_inv.p = nullptr;
+  ctx->d_inv.cap = 0;
   HIPCHK(ctx, ctx->d_inv.ensure(t.size() * 4));
   HIPCHK(ctx, hipMemcpy(ctx->d_inv.p, t.data(), t.size() * 4, hipMemcpyHostToDevice));
   ctx->inv_table.swap(t);
-  ctx->kinv = k;
+  ctx->kinv = kall;
   return DRX_OK;
 }
 
@@ -160,6 +180,7 @@ struct drx_plan {
   drx_ctx* ctx = nullptr;
   drx_batch b{};
   drx_launch_cfg cfg{};
+  uint64_t epoch = 0;  // context epoch the plan was built in
   int n = 0, kmax = 0, ld = 0;
   struct Group {
     int m;
@@ -172,7 +193,7 @@ struct drx_plan {
   std::vector<Group> groups;
   std::vector<int> walk_order;  // span index -> walk
   DBuf d_walks, d_tasks, d_spans, d_res0, d_fst0, d_res, d_fst, d_axr, d_axi, d_axd, d_cp, d_mf;
-  DBuf d_fout, d_rout, d_pout, d_pfull, d_digits, d_limbs, d_neg, d_topk, d_rowk;
+  DBuf d_fout, d_rout, d_pout, d_pfull, d_limbs, d_neg, d_topk, d_topk_ws, d_rowk;
   KArgs args{};
   GarnerArgs gargs{};
   int ntask = 0;
@@ -184,7 +205,7 @@ struct drx_plan {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
     for (DBuf* d : {&d_walks, &d_tasks, &d_spans, &d_res0, &d_fst0, &d_res, &d_fst, &d_axr, &d_axi, &d_axd,
-                    &d_cp, &d_mf, &d_fout, &d_rout, &d_pout, &d_pfull, &d_digits, &d_limbs, &d_neg, &d_topk, &d_rowk})
+                    &d_cp, &d_mf, &d_fout, &d_rout, &d_pout, &d_pfull, &d_limbs, &d_neg, &d_topk, &d_topk_ws, &d_rowk})
       d->release();
     for (drx_plan* p : plain) delete p;
   }
@@ -225,7 +246,8 @@ int launch_sequence(drx_plan* pl, bool timed) {
   HIPCHK(ctx, launch_finalize(pl->args, (int)pl->walk_order.size(), pl->b.full != nullptr, s));
   if (pl->b.crt_limbs) HIPCHK(ctx, launch_garner(pl->gargs, s));
   if (pl->b.top_k > 0)
-    HIPCHK(ctx, launch_topk(pl->d_fout.as<double>() + 3, 6, pl->n, pl->b.top_k, pl->d_topk.as<int32_t>(), s));
+    HIPCHK(ctx, launch_topk(pl->d_fout.as<double>() + 3, 6, pl->n, pl->b.top_k, pl->d_topk.as<int32_t>(),
+                            pl->d_topk_ws.p, s));
   if (timed) HIPCHK(ctx, hipEventRecord(ev[4], s));
   return DRX_OK;
 }
@@ -410,13 +432,13 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
   if (b.top_k > 0) {
     if (b.top_k > n) return fail(ctx, DRX_EINVAL, "top_k=%d > n=%d", b.top_k, n);
     HIPCHK(ctx, pl->d_topk.ensure((size_t)b.top_k * 4));
+    HIPCHK(ctx, pl->d_topk_ws.ensure(std::max<size_t>(16, topk_workspace_bytes(n, b.top_k))));
   }
   if (b.crt_limbs) {
     if ((rc = ensure_inv_table(ctx, kmax))) return rc;
     std::vector<int32_t> rowk(2 * (size_t)n);
     for (int i = 0; i < n; ++i) rowk[2 * i] = rowk[2 * i + 1] = b.walks[i].K;
     if ((rc = upload(ctx, pl->d_rowk, rowk))) return rc;
-    HIPCHK(ctx, pl->d_digits.ensure((size_t)std::max(1, n) * 2 * pl->ld * 4));
     HIPCHK(ctx, pl->d_limbs.ensure((size_t)std::max(1, n) * 2 * pl->ld * 4));
     HIPCHK(ctx, pl->d_neg.ensure((size_t)std::max(1, n) * 2));
   }
@@ -460,7 +482,6 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
   g.prime_u = ctx->d_prime_u.as<uint32_t>();
   g.inv_table = ctx->d_inv.as<uint32_t>();
   g.inv_ld = (uint64_t)ctx->kinv;
-  g.digits = pl->d_digits.as<uint32_t>();
   g.limbs = pl->d_limbs.as<uint32_t>();
   g.neg = pl->d_neg.as<uint8_t>();
 
@@ -561,6 +582,7 @@ void drx_ctx_destroy(drx_ctx* ctx) {
   for (DBuf* d : {&ctx->d_prime_p, &ctx->d_prime_pinv, &ctx->d_prime_e, &ctx->d_prime_u, &ctx->d_inv, &ctx->d_code,
                   &ctx->d_cres, &ctx->d_cflt, &ctx->d_progs, &ctx->d_tmp0, &ctx->d_tmp1, &ctx->d_tmp2, &ctx->d_tmp3})
     d->release();
+  for (void* p : ctx->retired) (void)hipFree(p);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -598,10 +620,13 @@ int drx_set_primes(drx_ctx* ctx, const int64_t* primes, int K) {
     pe[i] = pr[i] - 2;
   }
   HIPCHK(ctx, hipSetDevice(ctx->device));
-  HIPCHK(ctx, ctx->d_prime_p.ensure(K * 8));
-  HIPCHK(ctx, ctx->d_prime_pinv.ensure(K * 8));
-  HIPCHK(ctx, ctx->d_prime_e.ensure(K * 4));
-  HIPCHK(ctx, ctx->d_prime_u.ensure(K * 4));
+  // an extension keeps every earlier plan valid; any other change retires them
+  const bool extends = pr.size() >= ctx->primes.size() && std::equal(ctx->primes.begin(), ctx->primes.end(), pr.begin());
+  if (!extends) ++ctx->epoch;
+  HIPCHK(ctx, ctx->d_prime_p.ensure_keep(K * 8, ctx->retired));
+  HIPCHK(ctx, ctx->d_prime_pinv.ensure_keep(K * 8, ctx->retired));
+  HIPCHK(ctx, ctx->d_prime_e.ensure_keep(K * 4, ctx->retired));
+  HIPCHK(ctx, ctx->d_prime_u.ensure_keep(K * 4, ctx->retired));
   HIPCHK(ctx, hipMemcpy(ctx->d_prime_p.p, pd.data(), K * 8, hipMemcpyHostToDevice));
   HIPCHK(ctx, hipMemcpy(ctx->d_prime_pinv.p, pinv.data(), K * 8, hipMemcpyHostToDevice));
   HIPCHK(ctx, hipMemcpy(ctx->d_prime_e.p, pe.data(), K * 4, hipMemcpyHostToDevice));
@@ -763,6 +788,7 @@ int drx_program_release_all(drx_ctx* ctx) {
   ctx->h_cres.clear();
   ctx->h_cflt.clear();
   ctx->progs_dirty = true;
+  ++ctx->epoch;
   return DRX_OK;
 }
 
@@ -776,6 +802,7 @@ int drx_plan_create(drx_ctx* ctx, const drx_batch* b, const drx_launch_cfg* cfg,
   drx_plan* pl = new (std::nothrow) drx_plan();
   if (!pl) return fail(ctx, DRX_ENOMEM, "out of host memory");
   pl->ctx = ctx;
+  pl->epoch = ctx->epoch;
   pl->b = *b;
   if (cfg) pl->cfg = *cfg;
   if (pl->cfg.mode == DRX_MODE_PLAIN) {
@@ -840,6 +867,8 @@ int drx_plan_run(drx_plan* pl) {
   if (!pl) return DRX_EINVAL;
   drx_ctx* ctx = pl->ctx;
   std::lock_guard<std::mutex> lock(ctx->mu);
+  if (pl->epoch != ctx->epoch)
+    return fail(ctx, DRX_ESTATE, "the context's primes or programs changed since this plan was built; build it again");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   Timing tm;
   if (!pl->plain.empty() || pl->cfg.mode == DRX_MODE_PLAIN) {
@@ -864,7 +893,9 @@ int drx_plan_run(drx_plan* pl) {
       std::vector<double> dl(pl->n);
       for (int i = 0; i < pl->n; ++i) dl[i] = pl->b.out[i].delta_float;
       HIPCHK(ctx, hipMemcpyAsync(ctx->d_tmp0.p, dl.data(), dl.size() * 8, hipMemcpyHostToDevice, ctx->stream));
-      HIPCHK(ctx, launch_topk(ctx->d_tmp0.as<double>(), 1, pl->n, pl->b.top_k, ctx->d_tmp1.as<int32_t>(), ctx->stream));
+      HIPCHK(ctx, ctx->d_tmp2.ensure(std::max<size_t>(16, topk_workspace_bytes(pl->n, pl->b.top_k))));
+      HIPCHK(ctx, launch_topk(ctx->d_tmp0.as<double>(), 1, pl->n, pl->b.top_k, ctx->d_tmp1.as<int32_t>(),
+                              ctx->d_tmp2.p, ctx->stream));
       HIPCHK(ctx, hipMemcpyAsync(pl->b.topk_idx, ctx->d_tmp1.p, (size_t)pl->b.top_k * 4, hipMemcpyDeviceToHost, ctx->stream));
       HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     }
@@ -930,7 +961,6 @@ int drx_crt(drx_ctx* ctx, const int64_t* residues, int n, int K, int stride, uin
   if (rc) return rc;
   const size_t rows = (size_t)n * stride;
   HIPCHK(ctx, ctx->d_tmp0.ensure(rows * 8));
-  HIPCHK(ctx, ctx->d_tmp1.ensure(rows * 4));
   HIPCHK(ctx, ctx->d_tmp2.ensure(rows * 4));
   HIPCHK(ctx, ctx->d_tmp3.ensure((size_t)n));
   HIPCHK(ctx, hipMemcpyAsync(ctx->d_tmp0.p, residues, rows * 8, hipMemcpyHostToDevice, ctx->stream));
@@ -945,7 +975,6 @@ int drx_crt(drx_ctx* ctx, const int64_t* residues, int n, int K, int stride, uin
   g.prime_u = ctx->d_prime_u.as<uint32_t>();
   g.inv_table = ctx->d_inv.as<uint32_t>();
   g.inv_ld = (uint64_t)ctx->kinv;
-  g.digits = ctx->d_tmp1.as<uint32_t>();
   g.limbs = ctx->d_tmp2.as<uint32_t>();
   g.neg = ctx->d_tmp3.as<uint8_t>();
   HIPCHK(ctx, launch_garner(g, ctx->stream));
@@ -965,9 +994,32 @@ int drx_topk(drx_ctx* ctx, const double* values, int n, int k, int32_t* idx) {
   HIPCHK(ctx, ctx->d_tmp0.ensure((size_t)n * 8));
   HIPCHK(ctx, ctx->d_tmp1.ensure((size_t)k * 4));
   HIPCHK(ctx, hipMemcpyAsync(ctx->d_tmp0.p, values, (size_t)n * 8, hipMemcpyHostToDevice, ctx->stream));
-  HIPCHK(ctx, launch_topk(ctx->d_tmp0.as<double>(), 1, n, k, ctx->d_tmp1.as<int32_t>(), ctx->stream));
+  HIPCHK(ctx, ctx->d_tmp2.ensure(std::max<size_t>(16, topk_workspace_bytes(n, k))));
+  HIPCHK(ctx, launch_topk(ctx->d_tmp0.as<double>(), 1, n, k, ctx->d_tmp1.as<int32_t>(), ctx->d_tmp2.p, ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(idx, ctx->d_tmp1.p, (size_t)k * 4, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return DRX_OK;
+}
+
+int drx_copy_calibrate(drx_ctx* ctx, uint64_t bytes, int reps, double* ms) {
+  if (!ctx || !ms || reps <= 0 || bytes < 16 || bytes % 16) return DRX_EINVAL;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  void *src = nullptr, *dst = nullptr;
+  hipError_t e = hipMalloc(&src, bytes);
+  if (e == hipSuccess) e = hipMalloc(&dst, bytes);
+  if (e == hipSuccess) e = hipMemsetAsync(src, 0x5a, bytes, ctx->stream);
+  if (e == hipSuccess) e = launch_copy(src, dst, bytes / 16, ctx->stream);  // warm
+  if (e == hipSuccess) e = hipEventRecord(ctx->ev[10], ctx->stream);
+  for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_copy(src, dst, bytes / 16, ctx->stream);
+  if (e == hipSuccess) e = hipEventRecord(ctx->ev[11], ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  float t = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, ctx->ev[10], ctx->ev[11]);
+  if (src) (void)hipFree(src);
+  if (dst) (void)hipFree(dst);
+  HIPCHK(ctx, e);
+  *ms = (double)t / reps;
   return DRX_OK;
 }
 

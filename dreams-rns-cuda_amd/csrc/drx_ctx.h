@@ -38,6 +38,15 @@ struct DBuf {
     cap = want;
     return hipSuccess;
   }
+  // Like ensure, but an outgrown buffer is handed to `retired` instead of
+  // freed: launch arguments captured earlier (plans, graphs) stay valid.
+  hipError_t ensure_keep(size_t bytes, std::vector<void*>& retired) {
+    if (bytes <= cap) return hipSuccess;
+    if (p) retired.push_back(p);
+    p = nullptr;
+    cap = 0;
+    return ensure(bytes);
+  }
   void release() {
     if (p) (void)hipFree(p);
     p = nullptr;
@@ -82,6 +91,13 @@ struct drx_ctx {
   drx::DBuf d_tmp0, d_tmp1, d_tmp2, d_tmp3;
   drx::Timing timing;
   hipEvent_t ev[12] = {};
+  // Bumped when the prime table changes other than by extension or the
+  // programs are released: plans and server requests of an older epoch
+  // refer to tables that no longer exist and are refused.
+  uint64_t epoch = 0;
+  // Outgrown shared device buffers (primes, programs, Garner table), kept
+  // until the context ends because plans built earlier still point at them.
+  std::vector<void*> retired;
   std::mutex mu;
 };
 

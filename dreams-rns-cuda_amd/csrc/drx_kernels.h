@@ -49,7 +49,6 @@ struct GarnerArgs {
   const uint32_t* prime_u;
   const uint32_t* inv_table;  // inv_table[j * inv_ld + i] = p_j^{-1} mod p_i
   uint64_t inv_ld;
-  uint32_t* digits;       // scratch [n][res_stride]
   uint32_t* limbs;        // [n][res_stride] little-endian magnitude of the centred value
   uint8_t* neg;           // [n] 1 if the centred value is negative
 };
@@ -63,6 +62,14 @@ hipError_t launch_combine(int m, const KArgs& a, int nspan, hipStream_t s);
 hipError_t launch_shadow(int m, const KArgs& a, int nspan, int finalize, hipStream_t s);
 hipError_t launch_finalize(const KArgs& a, int nspan, int full, hipStream_t s);
 hipError_t launch_garner(const GarnerArgs& g, hipStream_t s);
-hipError_t launch_topk(const double* d, int stride, int n, int k, int32_t* out, hipStream_t s);
+// Top-k walk indices by value (larger first, NaN last, ties to the lower
+// index). `ws` holds topk_workspace_bytes(n, k) bytes (unused when 0).
+struct TopCand {
+  uint64_t key;
+  uint32_t idx, pad;
+};
+size_t topk_workspace_bytes(int n, int k);
+hipError_t launch_topk(const double* d, int stride, int n, int k, int32_t* out, void* ws, hipStream_t s);
+hipError_t launch_copy(const void* src, void* dst, size_t n16, hipStream_t s);
 
 }  // namespace drx

@@ -514,63 +514,154 @@ __global__ void k_finalize(const KArgs a, int full) {
 // sign of the centred value (x > (M-1)/2 <=> digits compare above
 // (p_i - 1)/2 from the top), the digits of M - x when negative, and the
 // binary limbs by Horner x = (...(v_{K-1} p_{K-2} + v_{K-2}) ...) p_0 + v_0.
-__global__ void k_garner(const GarnerArgs g) {
-  const int id = blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= g.n) return;
-  const int K = g.row_k ? g.row_k[id] : g.K;
-  const int64_t* r = g.res + (uint64_t)id * g.res_stride;
-  uint32_t* v = g.digits + (uint64_t)id * g.res_stride;
-  uint32_t* limb = g.limbs + (uint64_t)id * g.res_stride;
-  for (int i = 0; i < K; ++i) {
-    const double p = g.prime_p[i], pinv = g.prime_pinv[i];
-    double t = canon((double)r[i], p, pinv);
-    for (int j = 0; j < i; ++j) {
-      // t = (t - v_j) * p_j^{-1} mod p_i
-      t = canon(mm(t - (double)v[j], (double)g.inv_table[(uint64_t)j * g.inv_ld + i], p, pinv), p, pinv);
-    }
-    v[i] = (uint32_t)t;
+// One wavefront per row. Digit / limb t of the row lives in lane t % 64,
+// register t / 64 (J registers, K <= 64 J). Mixed-radix digits by the
+// recurrence t_i <- (t_i - v_j) p_j^{-1} mod p_i, v_j broadcast from its lane;
+// then the sign against (p_i - 1) / 2 from the top digit, the negation
+// M - x for the upper half, and Horner x = v_0 + p_0 (v_1 + p_1 (...)) on
+// 32-bit limbs with a ballot carry-lookahead across lanes.
+template <int J>
+__global__ void __launch_bounds__(256) k_garner_wave(const GarnerArgs g) {
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= g.n) return;  // the whole wavefront leaves together
+  const int K = g.row_k ? g.row_k[row] : g.K;
+  const int64_t* r = g.res + (uint64_t)row * g.res_stride;
+  double t[J], pd[J], pv[J];
+  uint32_t pu[J], v[J];
+#pragma unroll
+  for (int jj = 0; jj < J; ++jj) {
+    const int i = jj * 64 + lane;
+    const bool in = i < K;
+    pd[jj] = in ? g.prime_p[i] : 3.0;
+    pv[jj] = in ? g.prime_pinv[i] : 1.0 / 3.0;
+    pu[jj] = in ? g.prime_u[i] : 3u;
+    t[jj] = in ? (double)r[i] : 0.0;
   }
-  // sign: compare digits with (p_i - 1) / 2 from the most significant end
-  int neg = 0;
-  for (int i = K - 1; i >= 0; --i) {
-    const uint32_t h = (g.prime_u[i] - 1u) >> 1;
-    if (v[i] != h) {
-      neg = v[i] > h;
-      break;
+  // mixed-radix digits: after step j, t_j is final
+#pragma unroll
+  for (int jr = 0; jr < J; ++jr) {
+    const int jbase = jr * 64;
+    const int lim = (K - jbase < 64) ? K - jbase : 64;
+    for (int jl = 0; jl < lim; ++jl) {
+      const int j = jbase + jl;
+      const double vj = (double)__builtin_amdgcn_readlane((int)t[jr], jl);
+      const uint32_t* inv = g.inv_table + (uint64_t)j * g.inv_ld;
+#pragma unroll
+      for (int jj = jr; jj < J; ++jj) {
+        const int i = jj * 64 + lane;
+        const bool upd = i > j && i < K;
+        const double w = upd ? (double)inv[i] : 0.0;
+        const double x = canon(mm(t[jj] - vj, w, pd[jj], pv[jj]), pd[jj], pv[jj]);
+        t[jj] = upd ? x : t[jj];
+      }
     }
   }
-  if (neg) {  // digits of M - x = (M - 1 - x) + 1
-    uint32_t carry = 1;
-    for (int i = 0; i < K; ++i) {
-      uint32_t d = g.prime_u[i] - 1u - v[i] + carry;
-      carry = (d == g.prime_u[i]) ? 1u : 0u;
-      v[i] = carry ? 0u : d;
+#pragma unroll
+  for (int jj = 0; jj < J; ++jj) v[jj] = (jj * 64 + lane < K) ? (uint32_t)t[jj] : 0u;
+
+  // sign: the most significant digit that differs from (p_i - 1) / 2
+  int neg = 0, found = 0;
+#pragma unroll
+  for (int jj = J - 1; jj >= 0; --jj) {
+    const uint32_t h = (pu[jj] - 1u) >> 1;
+    const uint64_t m = __ballot((jj * 64 + lane < K) && v[jj] != h);
+    if (!found && m) {
+      const int top = 63 - __clzll((long long)m);
+      neg = __builtin_amdgcn_readlane(v[jj] > h ? 1 : 0, top);
+      found = 1;
     }
   }
-  // Horner into little-endian 32-bit limbs
-  for (int i = 0; i < K; ++i) limb[i] = 0;
-  int n = 0;
-  for (int i = K - 1; i >= 0; --i) {
-    const uint64_t pm = (i == K - 1) ? 0ull : (uint64_t)g.prime_u[i];
-    uint64_t carry = v[i];
-    if (i == K - 1) {
-      limb[0] = v[i];
-      n = 1;
-      continue;
+  if (neg) {  // digits of M - x = (M - 1 - x) + 1: the +1 ripples through zero digits
+    int below_nz = 0;
+#pragma unroll
+    for (int jj = 0; jj < J; ++jj) {
+      const bool in = jj * 64 + lane < K;
+      const uint64_t nz = __ballot(in && v[jj] != 0u);
+      const bool carry = !below_nz && (nz & ((1ull << lane) - 1ull)) == 0ull;
+      const uint32_t d = carry ? (v[jj] == 0u ? 0u : pu[jj] - v[jj]) : pu[jj] - 1u - v[jj];
+      v[jj] = in ? d : 0u;
+      below_nz = below_nz || nz != 0ull;
     }
-    for (int l = 0; l < n; ++l) {
-      const uint64_t z = (uint64_t)limb[l] * pm + carry;
-      limb[l] = (uint32_t)z;
-      carry = z >> 32;
-    }
-    if (carry && n < K) limb[n++] = (uint32_t)carry;
   }
-  g.neg[id] = (uint8_t)neg;
+
+  // Horner from the top digit into little-endian 32-bit limbs
+  uint32_t L[J];
+#pragma unroll
+  for (int jj = 0; jj < J; ++jj) L[jj] = 0u;
+  {
+    const int top = K - 1;
+    uint32_t vt = 0u;
+#pragma unroll
+    for (int jj = 0; jj < J; ++jj)
+      if (jj == top / 64) vt = (uint32_t)__builtin_amdgcn_readlane((int)v[jj], top % 64);
+    L[0] = (lane == 0) ? vt : 0u;
+  }
+#pragma unroll
+  for (int jr = J - 1; jr >= 0; --jr) {
+    const int ib = jr * 64;
+    const int hi_l = (K - 2 - ib < 63) ? K - 2 - ib : 63;
+    for (int jl = hi_l; jl >= 0; --jl) {
+      const uint32_t p = (uint32_t)__builtin_amdgcn_readlane((int)pu[jr], jl);
+      const uint32_t vi = (uint32_t)__builtin_amdgcn_readlane((int)v[jr], jl);
+      uint32_t lo[J], hi[J], c[J];
+#pragma unroll
+      for (int jj = 0; jj < J; ++jj) {
+        const uint64_t z = (uint64_t)L[jj] * p;
+        lo[jj] = (uint32_t)z;
+        hi[jj] = (uint32_t)(z >> 32);
+      }
+      // limb t gets lo_t + hi_{t-1} (+ v_i at t = 0): below 2^33
+#pragma unroll
+      for (int jj = 0; jj < J; ++jj) {
+        uint32_t hp = (uint32_t)__shfl_up((int)hi[jj], 1);
+        const uint32_t h0 = (jj == 0) ? vi : (uint32_t)__builtin_amdgcn_readlane((int)hi[jj > 0 ? jj - 1 : 0], 63);
+        hp = (lane == 0) ? h0 : hp;
+        const uint64_t X = (uint64_t)lo[jj] + hp;
+        L[jj] = (uint32_t)X;
+        c[jj] = (uint32_t)(X >> 32);
+      }
+      // then + c_{t-1}, with the 1-bit ripple resolved by carry-lookahead
+      uint64_t Y[J], gm[J], pm[J];
+#pragma unroll
+      for (int jj = 0; jj < J; ++jj) {
+        uint32_t cp = (uint32_t)__shfl_up((int)c[jj], 1);
+        const uint32_t c0 = (jj == 0) ? 0u : (uint32_t)__builtin_amdgcn_readlane((int)c[jj > 0 ? jj - 1 : 0], 63);
+        cp = (lane == 0) ? c0 : cp;
+        Y[jj] = (uint64_t)L[jj] + cp;
+        gm[jj] = __ballot((Y[jj] >> 32) != 0ull);
+        pm[jj] = __ballot(Y[jj] == 0xFFFFFFFFull);
+      }
+      uint64_t cin = 0ull;
+#pragma unroll
+      for (int jj = 0; jj < J; ++jj) {
+        const uint64_t A = gm[jj] | pm[jj], B = gm[jj];
+        const uint64_t S1 = A + B;
+        const uint64_t S = S1 + cin;
+        const uint64_t CI = S ^ A ^ B;  // carry into each lane's limb
+        cin = (S1 < A || S < S1) ? 1ull : 0ull;
+        L[jj] = (uint32_t)(Y[jj] + ((CI >> lane) & 1ull));
+      }
+    }
+  }
+  uint32_t* limb = g.limbs + (uint64_t)row * g.res_stride;
+#pragma unroll
+  for (int jj = 0; jj < J; ++jj)
+    if (jj * 64 + lane < K) limb[jj * 64 + lane] = L[jj];
+  if (lane == 0) g.neg[row] = (uint8_t)neg;
 }
 
 // ------------------------------------------------------------------ top-k
-// rank(i) = #{j : d_j > d_i or (d_j == d_i and j < i)}, NaN ranked last;
-// element i is written at out[rank] when rank < k. LDS-tiled O(n^2 / T).
+// Order: larger delta first, NaN last, ties to the lower index. A double
+// maps to a 64-bit key whose unsigned order is the numeric order (-0 and +0
+// equal, NaN below everything); (key desc, index asc) is then a strict total
+// order on the walks.
+__device__ __forceinline__ uint64_t order_key(double d) {
+  if (d != d) return 0ull;
+  const uint64_t b = (d == 0.0) ? 0ull : (uint64_t)__double_as_longlong(d);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
 __device__ __forceinline__ bool better(double dj, int j, double di, int i) {
   const bool nj = dj != dj, ni = di != di;
   if (nj != ni) return ni;            // non-NaN beats NaN
@@ -578,8 +669,71 @@ __device__ __forceinline__ bool better(double dj, int j, double di, int i) {
   return j < i;
 }
 
-__global__ void k_topk(const double* __restrict__ d, int stride, int n, int k,
-                       int32_t* __restrict__ out) {
+// Tournament round: each block sorts kTopB candidates in LDS (bitonic, in
+// the order above) and keeps its first kk. Padding candidates carry key 0 and
+// index 0xFFFFFFFF, after every real one. The last round (one block) writes
+// the winners' indices.
+constexpr int kTopB = 2048;
+
+template <bool FROM_VALUES>
+__global__ void __launch_bounds__(256) k_topk_round(const double* __restrict__ d, int stride,
+                                                    const TopCand* __restrict__ cin, int n, int kk,
+                                                    TopCand* __restrict__ cout, int32_t* __restrict__ out) {
+  __shared__ uint64_t sk[kTopB];
+  __shared__ uint32_t si[kTopB];
+  const int base = blockIdx.x * kTopB;
+  for (int t = threadIdx.x; t < kTopB; t += 256) {
+    const int i = base + t;
+    uint64_t key = 0ull;
+    uint32_t idx = 0xFFFFFFFFu;
+    if (i < n) {
+      if (FROM_VALUES) {
+        key = order_key(d[(uint64_t)i * stride]);
+        idx = (uint32_t)i;
+      } else {
+        key = cin[i].key;
+        idx = cin[i].idx;
+      }
+    }
+    sk[t] = key;
+    si[t] = idx;
+  }
+  __syncthreads();
+  for (int size = 2; size <= kTopB; size <<= 1) {
+    for (int half = size >> 1; half > 0; half >>= 1) {
+      for (int t = threadIdx.x; t < kTopB / 2; t += 256) {
+        const int lo = 2 * t - (t & (half - 1));
+        const int hi = lo + half;
+        const uint64_t ka = sk[lo], kb = sk[hi];
+        const uint32_t ia = si[lo], ib = si[hi];
+        const bool b_first = kb > ka || (kb == ka && ib < ia);
+        if (b_first == ((lo & size) == 0)) {
+          sk[lo] = kb;
+          sk[hi] = ka;
+          si[lo] = ib;
+          si[hi] = ia;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (gridDim.x == 1 && out) {
+    for (int t = threadIdx.x; t < kk; t += 256) out[t] = (int32_t)si[t];
+  } else {
+    for (int t = threadIdx.x; t < kk; t += 256) {
+      TopCand c;
+      c.key = sk[t];
+      c.idx = si[t];
+      c.pad = 0u;
+      cout[(uint64_t)blockIdx.x * kk + t] = c;
+    }
+  }
+}
+
+// k above the tournament's reach: rank(i) = #{j before i}, element i written
+// at out[rank] when rank < k. LDS-tiled O(n^2 / T).
+__global__ void k_topk_rank(const double* __restrict__ d, int stride, int n, int k,
+                            int32_t* __restrict__ out) {
   __shared__ double tile[256];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const double di = (i < n) ? d[(uint64_t)i * stride] : 0.0;
@@ -637,14 +791,69 @@ hipError_t launch_finalize(const KArgs& a, int nspan, int full, hipStream_t s) {
 
 hipError_t launch_garner(const GarnerArgs& g, hipStream_t s) {
   if (g.n <= 0) return hipSuccess;
-  const int tb = 64;
-  hipLaunchKernelGGL(k_garner, dim3((g.n + tb - 1) / tb), dim3(tb), 0, s, g);
+  const dim3 grid((g.n + 3) / 4), block(256);
+  if (g.K <= 64) hipLaunchKernelGGL(k_garner_wave<1>, grid, block, 0, s, g);
+  else if (g.K <= 128) hipLaunchKernelGGL(k_garner_wave<2>, grid, block, 0, s, g);
+  else if (g.K <= 256) hipLaunchKernelGGL(k_garner_wave<4>, grid, block, 0, s, g);
+  else if (g.K <= kMaxK) hipLaunchKernelGGL(k_garner_wave<kMaxK / 64>, grid, block, 0, s, g);
+  else return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
-hipError_t launch_topk(const double* d, int stride, int n, int k, int32_t* out, hipStream_t s) {
+// Plain 16-byte-vector copy: the calibration kernel for HBM byte counters
+// (it moves exactly 2 x n16 x 16 bytes) and the achievable-bandwidth probe.
+__global__ void __launch_bounds__(256) k_copy_calib(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                    size_t n16) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_copy(const void* src, void* dst, size_t n16, hipStream_t s) {
+  if (n16 == 0) return hipSuccess;
+  const size_t tb = 256, want = (n16 + tb - 1) / tb;
+  const unsigned nb = (unsigned)(want < 8192 ? want : 8192);
+  hipLaunchKernelGGL(k_copy_calib, dim3(nb), dim3(tb), 0, s, (const uint4*)src, (uint4*)dst, n16);
+  return hipGetLastError();
+}
+
+static int topk_keep(int k) { return k < kTopB / 2 ? k : kTopB / 2; }
+
+size_t topk_workspace_bytes(int n, int k) {
+  if (n <= kTopB || k > kTopB / 2) return 0;
+  const size_t nb = ((size_t)n + kTopB - 1) / kTopB;
+  return 2 * nb * (size_t)topk_keep(k) * sizeof(TopCand);
+}
+
+hipError_t launch_topk(const double* d, int stride, int n, int k, int32_t* out, void* ws, hipStream_t s) {
   if (n <= 0 || k <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_topk, dim3((n + 255) / 256), dim3(256), 0, s, d, stride, n, k, out);
+  if (k > n) return hipErrorInvalidValue;
+  if (k > kTopB / 2) {
+    hipLaunchKernelGGL(k_topk_rank, dim3((n + 255) / 256), dim3(256), 0, s, d, stride, n, k, out);
+    return hipGetLastError();
+  }
+  const int kk = topk_keep(k);
+  int cur = n;
+  int nb = (cur + kTopB - 1) / kTopB;
+  if (nb == 1) {
+    hipLaunchKernelGGL(k_topk_round<true>, dim3(1), dim3(256), 0, s, d, stride, nullptr, cur, k, nullptr, out);
+    return hipGetLastError();
+  }
+  if (!ws) return hipErrorInvalidValue;
+  TopCand* buf[2] = {(TopCand*)ws, (TopCand*)ws + (size_t)nb * kk};
+  hipLaunchKernelGGL(k_topk_round<true>, dim3(nb), dim3(256), 0, s, d, stride, nullptr, cur, kk, buf[0], nullptr);
+  cur = nb * kk;
+  int src = 0;
+  for (;;) {
+    nb = (cur + kTopB - 1) / kTopB;
+    if (nb == 1) {
+      hipLaunchKernelGGL(k_topk_round<false>, dim3(1), dim3(256), 0, s, nullptr, 0, buf[src], cur, k, nullptr, out);
+      break;
+    }
+    hipLaunchKernelGGL(k_topk_round<false>, dim3(nb), dim3(256), 0, s, nullptr, 0, buf[src], cur, kk, buf[1 - src],
+                       nullptr);
+    cur = nb * kk;
+    src = 1 - src;
+  }
   return hipGetLastError();
 }
 

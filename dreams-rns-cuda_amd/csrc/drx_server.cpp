@@ -24,6 +24,7 @@ namespace {
 
 struct SrvReq {
   uint64_t ticket;
+  uint64_t epoch;  // context epoch the program id belongs to
   drx_walk w;
   std::vector<int64_t> axis;
   double t_submit;
@@ -32,6 +33,7 @@ struct SrvReq {
 struct SrvSlot {
   bool used = false;
   uint64_t ticket = 0;
+  uint64_t epoch = 0;
   drx_walk w{};
   int m = 0;
   int64_t step = 0;
@@ -78,6 +80,7 @@ size_t slot_axr(const drx_server* s) { return (size_t)kMaxDim * 3 * s->kslot; }
 int admit(drx_server* s, int slot, SrvReq& rq) {
   drx_ctx* ctx = s->ctx;
   const drx_walk& w = rq.w;
+  if (rq.epoch != ctx->epoch) return fail(ctx, DRX_ESTATE, "programs were released after this request was submitted");
   const HostProgram& hp = ctx->progs[w.prog];
   const int m = (int)hp.dev.m, dim = (int)hp.dev.dim, K = w.K;
   std::vector<double> res((size_t)m * m * K), f(m * m + 1), axr((size_t)dim * 3 * K), axd(dim);
@@ -114,6 +117,7 @@ int admit(drx_server* s, int slot, SrvReq& rq) {
   sl.step = 0;
   sl.t_submit = rq.t_submit;
   sl.t_admit = now_s();
+  sl.epoch = rq.epoch;
   return DRX_OK;
 }
 
@@ -125,6 +129,9 @@ int round_once(drx_server* s, std::vector<int>& finished) {
   struct TaskRef {
     int slot, m, fd;
   };
+  for (int i = 0; i < s->nslots; ++i)
+    if (s->slots[i].used && s->slots[i].epoch != ctx->epoch)
+      return fail(ctx, DRX_ESTATE, "programs were released while requests were resident");
   std::vector<TaskRef> refs;
   for (int i = 0; i < s->nslots; ++i)
     if (s->slots[i].used) refs.push_back({i, s->slots[i].m, ctx->progs[s->slots[i].w.prog].fd});
@@ -391,6 +398,7 @@ int drx_server_submit(drx_server* s, const drx_walk* w, const int64_t* axis_vals
     if (w->init_kind == DRX_INIT_PCF && dp.m != 2) return fail(ctx, DRX_EINVAL, "PCF initial matrix needs m=2");
     if (dp.dim > 0 && !axis_vals) return fail(ctx, DRX_EINVAL, "axis values missing");
     rq.axis.assign(axis_vals, axis_vals + 4 * dp.dim);
+    rq.epoch = ctx->epoch;
   }
   rq.w = *w;
   rq.t_submit = now_s();

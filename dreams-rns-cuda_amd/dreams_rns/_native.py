@@ -96,6 +96,7 @@ SIGNATURES = {
     "drx_plan_destroy": (None, [C.c_void_p]),
     "drx_crt": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "drx_topk": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "drx_copy_calibrate": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p]),
     "drx_last_timing": (C.c_int, [C.c_void_p, C.c_void_p]),
     "drx_server_create": (C.c_int, [C.c_void_p, C.POINTER(ServerCfg), C.POINTER(C.c_void_p)]),
     "drx_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Walk), C.c_void_p, C.POINTER(C.c_uint64)]),
@@ -223,6 +224,28 @@ class Context:
         t = np.zeros(6, dtype=np.float64)
         self._check(self.lib.drx_last_timing(self.h, _ptr(t)))
         return t
+
+    def release_programs(self) -> None:
+        """Drop every uploaded program; plans built before are refused."""
+        with self._lock:
+            self._check(self.lib.drx_program_release_all(self.h))
+            self._programs.clear()
+
+    def topk(self, values, k: int) -> np.ndarray:
+        """Indices of the ``k`` largest ``values`` on the device (NaN last,
+        ties to the lower index), as the batched path ranks delta_float."""
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        idx = np.zeros(max(1, int(k)), dtype=np.int32)
+        self._check(self.lib.drx_topk(self.h, _ptr(v), len(v), int(k), _ptr(idx)))
+        return idx[: int(k)]
+
+    def copy_calibrate(self, nbytes: int, reps: int = 10) -> float:
+        """Mean device ms of a plain copy of ``nbytes`` (reads and writes
+        exactly ``nbytes`` each): byte-counter calibration and achievable
+        copy bandwidth."""
+        ms = C.c_double(0.0)
+        self._check(self.lib.drx_copy_calibrate(self.h, int(nbytes), int(reps), C.byref(ms)))
+        return ms.value
 
 
 _ctx: Optional[Context] = None

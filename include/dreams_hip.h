@@ -41,7 +41,9 @@ const char* drx_last_error(const drx_ctx* ctx);
 
 /* Prime table shared by every walk of the context; a walk with K primes uses
  * the first K. Replaces generate_rns_primes (python/dreams_rns/runner.py:33-52),
- * whose output the caller passes here (odd primes in (2^30, 2^31)). */
+ * whose output the caller passes here (odd primes in (2^30, 2^31)). Extending
+ * the table keeps existing plans valid; any other change (allowed only while
+ * no programs are loaded) makes them fail with DRX_ESTATE. */
 int drx_set_primes(drx_ctx* ctx, const int64_t* primes, int K);
 
 /* ---- programs ----------------------------------------------------------
@@ -57,6 +59,8 @@ int drx_program_upload(drx_ctx* ctx, const uint32_t* ops, int n_ops, int m, int 
                        int n_const, int kcap, int* prog_id);
 int drx_program_info(drx_ctx* ctx, int prog_id, int* n_dev_ops, int* n_const_used,
                      uint32_t* entry_kinds);
+/* Drop every program. Plans built before, and server requests submitted
+ * before, are refused afterwards (DRX_ESTATE). */
 int drx_program_release_all(drx_ctx* ctx);
 
 /* Pack a program without a device (inspection and host-side testing of the
@@ -160,6 +164,13 @@ int drx_crt(drx_ctx* ctx, const int64_t* residues, int n, int K, int stride, uin
  * No reference counterpart in Python; the reference CUDA sketch is
  * k_topk_select (src/cuda/persistent_kernel.cu:796-841). */
 int drx_topk(drx_ctx* ctx, const double* values, int n, int k, int32_t* idx);
+
+/* Plain device copy of `bytes` (a multiple of 16) between two fresh buffers,
+ * `reps` timed launches after one warm-up; *ms = mean device time per copy.
+ * No counterpart in the reference: it is the calibration kernel for HBM byte
+ * counters (each launch reads and writes exactly `bytes`) and the achievable
+ * copy bandwidth the walk kernels are compared with. Allocates and frees. */
+int drx_copy_calibrate(drx_ctx* ctx, uint64_t bytes, int reps, double* ms);
 
 /* Device time of the last drx_walk_batch, from HIP events on the library's
  * stream: [0] walk kernels, [1] combine, [2] shadow, [3] finalize + CRT + top-k,

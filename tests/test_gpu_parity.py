@@ -90,7 +90,7 @@ def test_verify_pcf_readme_case(gpu_ctx, golden):
 
 def test_crt_matches_textbook_garner(gpu_ctx):
     rng = np.random.default_rng(7)
-    for K in (1, 2, 8, 32, 64, 128):
+    for K in (1, 2, 8, 32, 63, 64, 65, 128, 129, 200, 256, 300):
         pr = [int(p) for p in D.generate_rns_primes(K)]
         for _ in range(5):
             res = [int(rng.integers(0, p)) for p in pr]
@@ -200,3 +200,44 @@ def test_forward_difference_path_equals_interpreter(gpu_ctx):
                                  if s.init == "pcf" else (None, None)))
         m = s.program.m
         assert np.array_equal(a.full_matrix(i, m), rr), i
+
+
+def _topk_ref(v, k):
+    key = [(1, 0.0, i) if math.isnan(x) else (0, -x, i) for i, x in enumerate(v)]
+    return [t[2] for t in sorted(key)[:k]]
+
+
+def test_topk_tournament_and_rank_paths(gpu_ctx):
+    # several tournament rounds, NaN, +-0 and ties; k above the tournament's
+    # reach goes through the rank kernel
+    rng = np.random.default_rng(21)
+    for n, k in ((1, 1), (5, 5), (2048, 64), (2049, 1), (4096, 64), (4096, 1024), (70000, 1024),
+                 (300000, 64), (300000, 128), (5000, 1025), (3000, 3000)):
+        v = rng.normal(size=n).round(2)          # many exact ties
+        v[rng.integers(0, n, size=max(1, n // 50))] = np.nan
+        v[rng.integers(0, n, size=max(1, n // 100))] = -0.0
+        v[rng.integers(0, n, size=max(1, n // 100))] = 0.0
+        if n > 10:
+            v[3] = np.inf
+            v[7] = -np.inf
+        got = gpu_ctx.topk(v, k).tolist()
+        print("topk", n, k, flush=True)
+        assert got == _topk_ref(v.tolist(), k), (n, k)
+
+
+def test_plan_built_before_release_is_refused(gpu_ctx):
+    cfg = synth.pcf2(n_walks=3, seed=31)
+    specs = [pcf_spec(D.compile_pcf(w["a"], w["b"]), D.pcf_initial_values(w["a"]), 50, 8) for w in cfg["walks"]]
+    plan = BatchPlan(specs)
+    first = plan.run().residues.copy()  # run() refills the plan's result arrays
+    # a later plan with new programs grows the shared tables: the first plan still runs
+    more = synth.pcf2(n_walks=40, seed=32)
+    BatchPlan([pcf_spec(D.compile_pcf(w["a"], w["b"]), D.pcf_initial_values(w["a"]), 20, 8)
+               for w in more["walks"]]).run()
+    plan.result.residues[:] = 0
+    assert np.array_equal(plan.run().residues, first)
+    gpu_ctx.release_programs()
+    with pytest.raises(Exception, match="changed since this plan was built"):
+        plan.run()
+    plan.close()
+    assert np.array_equal(BatchPlan(specs).run().residues, first)
