@@ -52,6 +52,41 @@ def program_work(prog) -> float:
     return 6.0 * mm + 2.0 * ma
 
 
+PCF_KINDS = 0 | (2 << 2) | (1 << 4) | (2 << 6)  # companion [[0, b], [1, a]]: ZERO, GEN, ONE, GEN
+
+
+def walk_kernel_name(progs) -> str:
+    """The walk kernel the library picks for these programs (the dominant
+    kernel of the batch when all of them agree)."""
+    from dreams_rns._native import pack_program
+
+    names = set()
+    for p in progs:
+        meta = pack_program(p)[2]
+        if meta["fd_ok"] and p.m == 2 and meta["fd_degree"] <= 6:
+            names.add("k_walk_fd_pcf" if meta["kinds"] == PCF_KINDS else "k_walk_fd<2>")
+        elif meta["fd_ok"] and p.m == 3 and meta["fd_degree"] <= 3:
+            names.add("k_walk_fd<3>")
+        else:
+            names.add(f"k_walk_chunk<{p.m}>")
+    return names.pop() if len(names) == 1 else "+".join(sorted(names))
+
+
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    counter summary (FETCH_SIZE / WRITE_SIZE scaled by the copy calibration),
+    or None when the summary is for another kernel."""
+    path = os.path.join(ROOT, "profiles", "walk_kernel_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("kernel") != kernel or t.get("workload") != workload:
+        return None
+    return float(t["traffic_bytes"])
+
+
 # ------------------------------------------------------------ CPU baseline
 def _cpu_walk(args):
     import dreams_rns as D
@@ -173,8 +208,8 @@ def main():
         step_ms.append((time.perf_counter() - t) * 1e3)
         tm = ctx.last_timing()
         walk_ms.append(tm[0])
-        log(f"step {i}: {step_ms[-1]:.2f} ms (walk kernel {tm[0]:.2f} ms, combine {tm[1]:.2f}, "
-            f"shadow {tm[2]:.2f}, tail {tm[3]:.2f})")
+        log(f"step {i}: {step_ms[-1]:.2f} ms (walk kernel {tm[0]:.3f} ms, combine {tm[1]:.3f}, "
+            f"shadow wait {tm[2]:.3f}, tail {tm[3]:.3f}, device {tm[4]:.3f})")
     res = plan.result
     ms = float(np.mean(step_ms))
     value = rsteps / (ms * 1e-3)
@@ -218,6 +253,8 @@ def main():
         log(f"cpu: one core {one:.3e}, {cores} cores {allc:.3e} residue-steps/s")
 
     achieved = work / (walk_avg * 1e-3) if walk_avg == walk_avg else None
+    kname = walk_kernel_name(progs)
+    traffic = measured_traffic(kname, "pcf2")
     line = {
         "metric": METRIC,
         "value": value,
@@ -240,8 +277,8 @@ def main():
         "served": served,
         "roofline": {"bound": "valu", "achieved": achieved / 1e12 if achieved else None,
                      "peak": PEAK_LANE_OPS / 1e12, "unit": "TOP/s",
-                     "frac": achieved / PEAK_LANE_OPS if achieved else None, "traffic": None,
-                     "kernel": "k_walk_chunk<2>", "kernel_ms": walk_avg,
+                     "frac": achieved / PEAK_LANE_OPS if achieved else None, "traffic": traffic,
+                     "kernel": kname, "kernel_ms": walk_avg,
                      "work_per_launch": work},
         "cpu_baseline": cpu,
     }

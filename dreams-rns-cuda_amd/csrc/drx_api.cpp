@@ -84,6 +84,79 @@ int upload_programs(drx_ctx* ctx) {
 }
 
 // p_j^{-1} mod p_i for the first k primes (lazily extended).
+int walk_kind(const HostProgram& hp, bool interp_only) {
+  if (interp_only || !hp.fd) return WALK_INTERP;
+  return pcf_shape(hp) >= 0 ? WALK_FD_PCF : WALK_FD;
+}
+
+int pcf_shape(const HostProgram& hp) {
+  // companion PCF [[0, b], [1, a]]: entry kinds ZERO, GEN, ONE, GEN
+  const uint32_t pcf_kinds = E_ZERO | (E_GEN << 2) | (E_ONE << 4) | (E_GEN << 6);
+  if (!hp.fd || hp.dev.m != 2 || hp.dev.kinds != pcf_kinds) return -1;
+  const int db = (int)((hp.dev.fd_deg >> 4) & 15u), da = (int)((hp.dev.fd_deg >> 12) & 15u);
+  if (db > kFdMax2 || da > kFdMax2) return -1;
+  return 8 * db + da;
+}
+
+size_t walk_lds_bytes(const DevProgram& pg, int kind, int m, uint32_t K, int threads) {
+  const size_t tree = (size_t)threads * m * m;  // [C][m*m][KP] with C * KP = threads
+  if (kind != WALK_INTERP) return 8 * tree;
+  const size_t KP = pow2_ceil(K);
+  const size_t nl = pg.const_lds ? pg.nconst : 0;
+  return 8 * (nl * KP + nl + (size_t)pg.nslots * threads + tree);
+}
+
+size_t seed_lds_bytes(const DevProgram& pg, int seed_threads) {
+  const size_t nl = pg.const_lds ? pg.nconst : 0;
+  return 8 * (nl * (size_t)seed_threads + nl + (size_t)pg.nslots * seed_threads);
+}
+
+size_t mf_lds_bytes(const DevProgram& pg) {
+  const size_t nl = pg.const_lds ? pg.nconst : 0;
+  return 8 * (nl + (size_t)pg.nslots * kMfBlock);
+}
+
+int launch_stages(drx_ctx* ctx, const KArgs& a, const std::vector<Stage>& stages, const DevSpan* seed_spans,
+                  int finalize, bool timed) {
+  hipStream_t s1 = ctx->stream, s2 = ctx->stream2;
+  hipEvent_t* ev = ctx->ev;
+  HIPCHK(ctx, hipEventRecord(ev[kEvFork], s1));
+  HIPCHK(ctx, hipStreamWaitEvent(s2, ev[kEvFork], 0));
+  for (const Stage& g : stages) {
+    KArgs ma = a;
+    ma.ranges += g.range_begin;
+    HIPCHK(ctx, launch_mf(g.m, ma, g.range_end - g.range_begin, g.mf_lds, s2));
+  }
+  for (const Stage& g : stages) {
+    KArgs sa = a;
+    sa.spans += g.span_begin;
+    HIPCHK(ctx, launch_shadow(g.m, sa, g.span_end - g.span_begin, finalize, s2));
+  }
+  HIPCHK(ctx, hipEventRecord(ev[kEvJoin], s2));
+  for (const Stage& g : stages) {
+    if (g.kind == WALK_INTERP || g.seed_end <= g.seed_begin) continue;
+    KArgs fa = a;
+    fa.spans = seed_spans + g.seed_begin;
+    HIPCHK(ctx, launch_fd_seed(g.m, fa, g.seed_end - g.seed_begin, g.seed_threads, g.seed_lds, s1));
+  }
+  if (timed) HIPCHK(ctx, hipEventRecord(ev[0], s1));
+  for (const Stage& g : stages) {
+    KArgs wa = a;
+    wa.tasks += g.task_begin;
+    HIPCHK(ctx, launch_walk(g.m, g.kind, wa, g.task_end - g.task_begin, g.threads, g.lds, s1));
+  }
+  if (timed) HIPCHK(ctx, hipEventRecord(ev[1], s1));
+  for (const Stage& g : stages) {
+    KArgs ca = a;
+    ca.spans += g.span_begin;
+    HIPCHK(ctx, launch_combine(g.m, ca, g.span_end - g.span_begin, s1));
+  }
+  if (timed) HIPCHK(ctx, hipEventRecord(ev[2], s1));
+  HIPCHK(ctx, hipStreamWaitEvent(s1, ev[kEvJoin], 0));
+  if (timed) HIPCHK(ctx, hipEventRecord(ev[3], s1));
+  return DRX_OK;
+}
+
 // Inverse of a mod m for coprime a, m (extended Euclid).
 static uint32_t inv_mod(uint32_t a, uint32_t m) {
   int64_t r0 = m, r1 = a % m, t0 = 0, t1 = 1;
@@ -151,7 +224,8 @@ int init_tables(drx_ctx* ctx, const drx_walk& w, int m, const uint32_t* init_res
     f[e] = 0.0;
     for (int k = 0; k < K; ++k) res[(size_t)e * K + k] = 0.0;
   }
-  f[m * m] = 0.0;  // log_scale
+  f[m * m] = 1.0;      // log_scale = ln(1.0) + 0 * ln 2
+  f[m * m + 1] = 0.0;
   if (w.init_kind == DRX_INIT_GIVEN) {
     for (int e = 0; e < m * m; ++e) {
       for (int k = 0; k < K; ++k) {
@@ -182,17 +256,9 @@ struct drx_plan {
   drx_launch_cfg cfg{};
   uint64_t epoch = 0;  // context epoch the plan was built in
   int n = 0, kmax = 0, ld = 0;
-  struct Group {
-    int m;
-    int fd;
-    int task_begin, task_end;
-    int span_begin, span_end;
-    int threads;
-    size_t lds;
-  };
-  std::vector<Group> groups;
+  std::vector<Stage> stages;
   std::vector<int> walk_order;  // span index -> walk
-  DBuf d_walks, d_tasks, d_spans, d_res0, d_fst0, d_res, d_fst, d_axr, d_axi, d_axd, d_cp, d_mf;
+  DBuf d_walks, d_tasks, d_spans, d_ranges, d_res0, d_fst0, d_res, d_fst, d_axr, d_axi, d_axd, d_cp, d_mf, d_fdt;
   DBuf d_fout, d_rout, d_pout, d_pfull, d_limbs, d_neg, d_topk, d_topk_ws, d_rowk;
   KArgs args{};
   GarnerArgs gargs{};
@@ -204,8 +270,9 @@ struct drx_plan {
   ~drx_plan() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
-    for (DBuf* d : {&d_walks, &d_tasks, &d_spans, &d_res0, &d_fst0, &d_res, &d_fst, &d_axr, &d_axi, &d_axd,
-                    &d_cp, &d_mf, &d_fout, &d_rout, &d_pout, &d_pfull, &d_limbs, &d_neg, &d_topk, &d_topk_ws, &d_rowk})
+    for (DBuf* d : {&d_walks, &d_tasks, &d_spans, &d_ranges, &d_res0, &d_fst0, &d_res, &d_fst, &d_axr, &d_axi,
+                    &d_axd, &d_cp, &d_mf, &d_fdt, &d_fout, &d_rout, &d_pout, &d_pfull, &d_limbs, &d_neg, &d_topk,
+                    &d_topk_ws, &d_rowk})
       d->release();
     for (drx_plan* p : plain) delete p;
   }
@@ -223,37 +290,20 @@ int upload(drx_ctx* ctx, DBuf& d, const std::vector<T>& h) {
 int launch_sequence(drx_plan* pl, bool timed) {
   drx_ctx* ctx = pl->ctx;
   hipStream_t s = ctx->stream;
-  hipEvent_t* ev = ctx->ev;
-  if (timed) HIPCHK(ctx, hipEventRecord(ev[0], s));
-  for (const auto& g : pl->groups) {
-    KArgs a = pl->args;
-    a.tasks += g.task_begin;
-    HIPCHK(ctx, launch_walk(g.m, g.fd, a, g.task_end - g.task_begin, g.threads, g.lds, s));
-  }
-  if (timed) HIPCHK(ctx, hipEventRecord(ev[1], s));
-  for (const auto& g : pl->groups) {
-    KArgs a = pl->args;
-    a.spans += g.span_begin;
-    HIPCHK(ctx, launch_combine(g.m, a, g.span_end - g.span_begin, s));
-  }
-  if (timed) HIPCHK(ctx, hipEventRecord(ev[2], s));
-  for (const auto& g : pl->groups) {
-    KArgs a = pl->args;
-    a.spans += g.span_begin;
-    HIPCHK(ctx, launch_shadow(g.m, a, g.span_end - g.span_begin, 1, s));
-  }
-  if (timed) HIPCHK(ctx, hipEventRecord(ev[3], s));
+  int rc = launch_stages(ctx, pl->args, pl->stages, pl->args.spans, 1, timed);
+  if (rc) return rc;
   HIPCHK(ctx, launch_finalize(pl->args, (int)pl->walk_order.size(), pl->b.full != nullptr, s));
   if (pl->b.crt_limbs) HIPCHK(ctx, launch_garner(pl->gargs, s));
   if (pl->b.top_k > 0)
     HIPCHK(ctx, launch_topk(pl->d_fout.as<double>() + 3, 6, pl->n, pl->b.top_k, pl->d_topk.as<int32_t>(),
                             pl->d_topk_ws.p, s));
-  if (timed) HIPCHK(ctx, hipEventRecord(ev[4], s));
+  if (timed) HIPCHK(ctx, hipEventRecord(ctx->ev[4], s));
   return DRX_OK;
 }
 
-// The walk kernel's task array is laid out per walk in step order; groups
-// hold one matrix order each; within a group walks run longest first.
+// The walk kernel's task array is laid out per walk in step order; stages
+// hold one matrix order and walk-kernel family each; within a stage walks run
+// longest first.
 int plan_build(drx_ctx* ctx, drx_plan* pl) {
   const drx_batch& b = pl->b;
   const int n = b.n;
@@ -290,31 +340,34 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
   if ((thr_cfg & (thr_cfg - 1)) || thr_cfg < 32 || thr_cfg > 256)
     return fail(ctx, DRX_EINVAL, "threads=%d must be a power of two in [32, 256]", thr_cfg);
 
-  // order: by m, then cost descending
+  // order: by matrix order and walk-kernel family, then cost descending
+  // (longest first), then the companion-PCF shape (equal shapes adjacent)
   std::vector<int> order(n);
   std::iota(order.begin(), order.end(), 0);
   auto cost = [&](int i) {
     const drx_walk& w = b.walks[i];
     return (double)w.depth * (double)w.K * (1.0 + ctx->progs[w.prog].pk.code.size());
   };
-  const bool allow_fd = !pl->cfg.interp_only;
-  auto gkey = [&](int i) {
-    const HostProgram& hp = ctx->progs[b.walks[i].prog];
-    return (int)hp.dev.m * 2 + ((allow_fd && hp.fd) ? 1 : 0);
-  };
+  const bool interp_only = pl->cfg.interp_only != 0;
+  auto kind_of = [&](int i) { return walk_kind(ctx->progs[b.walks[i].prog], interp_only); };
+  auto gkey = [&](int i) { return (int)ctx->progs[b.walks[i].prog].dev.m * 4 + kind_of(i); };
+  auto shape_of = [&](int i) { return kind_of(i) == WALK_FD_PCF ? pcf_shape(ctx->progs[b.walks[i].prog]) : 0; };
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
     const int kx = gkey(x), ky = gkey(y);
     if (kx != ky) return kx < ky;
-    return cost(x) > cost(y);
+    const double cx = cost(x), cy = cost(y);
+    if (cx != cy) return cx > cy;
+    return shape_of(x) < shape_of(y);
   });
 
   std::vector<DevWalk> hw(n);
   std::vector<DevTask> ht;
   std::vector<DevSpan> hs;
+  std::vector<DevRange> hr;
   std::vector<double> res0, fst0, axr, axd;
   std::vector<int64_t> axi;
-  uint64_t cp_total = 0, mf_total = 0;
-  pl->groups.clear();
+  uint64_t cp_total = 0, mf_total = 0, fdt_total = 0;
+  pl->stages.clear();
   pl->walk_order.clear();
   double rsteps = 0, mmt = 0, mat = 0;
   for (int oi = 0; oi < n; ++oi) {
@@ -322,6 +375,7 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
     const drx_walk& w = b.walks[i];
     const HostProgram& hp = ctx->progs[w.prog];
     const int m = (int)hp.dev.m, dim = (int)hp.dev.dim, K = w.K;
+    const int kind = kind_of(i);
     DevWalk& dw = hw[i];
     dw.prog = w.prog;
     dw.K = K;
@@ -333,11 +387,13 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
     dw.axr_off = axr.size();
     dw.axi_off = axi.size();
     dw.axd_off = axd.size();
+    dw.fdt_off = fdt_total;
+    if (kind != WALK_INTERP) fdt_total += (uint64_t)m * m * (fd_dmax(m) + 1) * K;
     // initial state and axes
     const size_t r0 = res0.size();
     res0.resize(r0 + (size_t)m * m * K, 0.0);
     const size_t f0 = fst0.size();
-    fst0.resize(f0 + m * m + 1, 0.0);
+    fst0.resize(f0 + fst_size(m), 0.0);
     int rc0 = init_tables(ctx, w, m, b.init_res_pool, b.init_f_pool, res0.data() + r0, fst0.data() + f0);
     if (rc0) return rc0;
     const size_t a0 = axr.size(), i0 = axi.size(), d0 = axd.size();
@@ -352,64 +408,72 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
     // tasks
     const bool plain = pl->cfg.mode == DRX_MODE_PLAIN;
     const int64_t chunk = plain ? std::max<int64_t>(w.depth, 1) : chunk_cfg;
-    const int fdw = (allow_fd && hp.fd) ? 1 : 0;
-    if (pl->groups.empty() || pl->groups.back().m != m || pl->groups.back().fd != fdw) {
-      drx_plan::Group g{};
+    if (pl->stages.empty() || pl->stages.back().m != m || pl->stages.back().kind != kind) {
+      Stage g;
       g.m = m;
-      g.fd = fdw;
+      g.kind = kind;
       g.task_begin = g.task_end = (int)ht.size();
       g.span_begin = g.span_end = (int)hs.size();
+      g.seed_begin = g.seed_end = (int)hs.size();
+      g.range_begin = g.range_end = (int)hr.size();
       g.threads = plain ? (int)pow2_ceil(K) : std::max<int>(thr_cfg, pow2_ceil(K));
-      g.lds = 0;
-      pl->groups.push_back(g);
+      pl->stages.push_back(g);
     }
-    drx_plan::Group& g = pl->groups.back();
-    const int threads = std::max<int>(g.threads, pow2_ceil(K));
-    g.threads = threads;
+    Stage& g = pl->stages.back();
+    g.threads = std::max<int>(g.threads, pow2_ceil(K));
+    g.seed_threads = std::max<int>(g.seed_threads, pow2_ceil(K));
     DevSpan sp{};
     sp.walk = i;
     sp.out_row = i;
     sp.task_begin = (int)ht.size();
+    const int shape = shape_of(i);
     for (int64_t s0 = 0; s0 < w.depth; s0 += chunk) {
       DevTask t{};
       t.walk = i;
+      t.shape = shape;
       t.s_begin = s0;
       t.s_end = std::min<int64_t>(w.depth, s0 + chunk);
       t.cp_off = cp_total;
       t.mf_off = mf_total;
       cp_total += (uint64_t)m * m * K;
       mf_total += (uint64_t)(t.s_end - t.s_begin) * m * m;
+      for (int64_t r = t.s_begin; r < t.s_end; r += kMfBlock) {
+        DevRange rg{};
+        rg.task = (int)ht.size();
+        rg.s_begin = r;
+        rg.s_end = std::min<int64_t>(t.s_end, r + kMfBlock);
+        hr.push_back(rg);
+      }
       ht.push_back(t);
     }
     sp.task_end = (int)ht.size();
     hs.push_back(sp);
     g.task_end = (int)ht.size();
-    g.span_end = (int)hs.size();
+    g.span_end = g.seed_end = (int)hs.size();
+    g.range_end = (int)hr.size();
     pl->walk_order.push_back(i);
     rsteps += (double)w.depth * K;
     mmt += (double)w.depth * K * hp.mm_ops;
     mat += (double)w.depth * K * hp.ma_ops;
   }
-  // LDS per group (max over its programs), checked against the device limit
+  // LDS per stage (max over its programs), checked against the device limit
   size_t lds_limit = 65536;
   {
     int v = 0;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) == hipSuccess && v > 0)
       lds_limit = std::min<size_t>((size_t)v, 65536);
   }
-  for (auto& g : pl->groups) {
-    size_t need = 0;
+  for (auto& g : pl->stages) {
     for (int si = g.span_begin; si < g.span_end; ++si) {
       const DevWalk& dw = hw[hs[si].walk];
       const DevProgram& dp = ctx->progs[dw.prog].dev;
-      const size_t KP = pow2_ceil(dw.K);
-      const size_t nl = dp.const_lds ? dp.nconst : 0;
-      const size_t bytes = 8 * (nl * KP + nl + (size_t)dp.nslots * g.threads + (size_t)g.threads * g.m * g.m);
-      need = std::max(need, bytes);
+      g.lds = std::max(g.lds, walk_lds_bytes(dp, g.kind, g.m, (uint32_t)dw.K, g.threads));
+      g.mf_lds = std::max(g.mf_lds, mf_lds_bytes(dp));
+      if (g.kind != WALK_INTERP) g.seed_lds = std::max(g.seed_lds, seed_lds_bytes(dp, g.seed_threads));
     }
+    const size_t need = std::max(g.lds, std::max(g.mf_lds, g.seed_lds));
     if (need > lds_limit)
       return fail(ctx, DRX_ERANGE, "program needs %zu bytes of LDS (limit %zu); use fewer threads", need, lds_limit);
-    g.lds = need;
   }
   pl->ntask = (int)ht.size();
   pl->residue_steps = rsteps;
@@ -418,13 +482,15 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
 
   int rc;
   if ((rc = upload(ctx, pl->d_walks, hw)) || (rc = upload(ctx, pl->d_tasks, ht)) || (rc = upload(ctx, pl->d_spans, hs)) ||
+      (rc = upload(ctx, pl->d_ranges, hr)) ||
       (rc = upload(ctx, pl->d_res0, res0)) || (rc = upload(ctx, pl->d_fst0, fst0)) || (rc = upload(ctx, pl->d_axr, axr)) ||
       (rc = upload(ctx, pl->d_axi, axi)) || (rc = upload(ctx, pl->d_axd, axd)))
     return rc;
   HIPCHK(ctx, pl->d_res.ensure(std::max<size_t>(1, res0.size()) * 8));
   HIPCHK(ctx, pl->d_fst.ensure(std::max<size_t>(1, fst0.size()) * 8));
   HIPCHK(ctx, pl->d_cp.ensure(std::max<uint64_t>(1, cp_total) * 8));
-  HIPCHK(ctx, pl->d_mf.ensure(std::max<uint64_t>(1, mf_total) * 8));
+  HIPCHK(ctx, pl->d_mf.ensure(std::max<uint64_t>(16, mf_total) * 8));
+  HIPCHK(ctx, pl->d_fdt.ensure(std::max<uint64_t>(1, fdt_total) * 8));
   HIPCHK(ctx, pl->d_fout.ensure((size_t)std::max(1, n) * 6 * 8));
   HIPCHK(ctx, pl->d_rout.ensure((size_t)std::max(1, n) * 2 * pl->ld * 8));
   if (b.full) HIPCHK(ctx, pl->d_pout.ensure((size_t)std::max(1, n) * 16 * pl->ld * 8));
@@ -449,6 +515,9 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
   a.walks = pl->d_walks.as<DevWalk>();
   a.progs = ctx->d_progs.as<DevProgram>();
   a.spans = pl->d_spans.as<DevSpan>();
+  a.ranges = pl->d_ranges.as<DevRange>();
+  a.invs = ctx->d_invs.as<double>();
+  a.fdt = pl->d_fdt.as<double>();
   a.code = ctx->d_code.as<uint32_t>();
   a.cres = ctx->d_cres.as<double>();
   a.cflt = ctx->d_cflt.as<double>();
@@ -470,7 +539,6 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
   a.fcompact = nullptr;
   a.ffull = b.full_f ? pl->d_pfull.as<double>() : nullptr;
   a.out_ld = (uint64_t)pl->ld;
-  a.with_fp = 1;
   GarnerArgs& g = pl->gargs;
   g.res = pl->d_rout.as<int64_t>();
   g.res_stride = (uint64_t)pl->ld;
@@ -561,7 +629,9 @@ int drx_ctx_create(int device, drx_ctx** out) {
   drx_ctx* ctx = new (std::nothrow) drx_ctx();
   if (!ctx) return DRX_ENOMEM;
   ctx->device = device;
-  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess) {
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return DRX_EHIP;
   }
@@ -579,13 +649,15 @@ void drx_ctx_destroy(drx_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  for (DBuf* d : {&ctx->d_prime_p, &ctx->d_prime_pinv, &ctx->d_prime_e, &ctx->d_prime_u, &ctx->d_inv, &ctx->d_code,
+  if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
+  for (DBuf* d : {&ctx->d_prime_p, &ctx->d_prime_pinv, &ctx->d_prime_e, &ctx->d_prime_u, &ctx->d_invs, &ctx->d_inv, &ctx->d_code,
                   &ctx->d_cres, &ctx->d_cflt, &ctx->d_progs, &ctx->d_tmp0, &ctx->d_tmp1, &ctx->d_tmp2, &ctx->d_tmp3})
     d->release();
   for (void* p : ctx->retired) (void)hipFree(p);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
   delete ctx;
 }
 
@@ -631,6 +703,13 @@ int drx_set_primes(drx_ctx* ctx, const int64_t* primes, int K) {
   HIPCHK(ctx, hipMemcpy(ctx->d_prime_pinv.p, pinv.data(), K * 8, hipMemcpyHostToDevice));
   HIPCHK(ctx, hipMemcpy(ctx->d_prime_e.p, pe.data(), K * 4, hipMemcpyHostToDevice));
   HIPCHK(ctx, hipMemcpy(ctx->d_prime_u.p, pr.data(), K * 4, hipMemcpyHostToDevice));
+  {
+    std::vector<double> invs((size_t)K * (kFdInv + 1), 0.0);
+    for (int i = 0; i < K; ++i)
+      for (int t = 1; t <= kFdInv; ++t) invs[(size_t)i * (kFdInv + 1) + t] = balanced(inv_mod((uint32_t)t, pr[i]), pr[i]);
+    HIPCHK(ctx, ctx->d_invs.ensure_keep(invs.size() * 8, ctx->retired));
+    HIPCHK(ctx, hipMemcpy(ctx->d_invs.p, invs.data(), invs.size() * 8, hipMemcpyHostToDevice));
+  }
   ctx->primes.swap(pr);
   ctx->kinv = 0;
   return DRX_OK;
@@ -886,7 +965,7 @@ int drx_plan_run(drx_plan* pl) {
       tm.combine_ms += b2;
       tm.shadow_ms += c;
       tm.tail_ms += d;
-      tm.walk_launches += (int)sp->groups.size();
+      tm.walk_launches += (int)sp->stages.size();
     }
     if (pl->b.top_k > 0) {
       // rank the per-walk deltas on the device like the batched path does
@@ -919,7 +998,7 @@ int drx_plan_run(drx_plan* pl) {
   tm.shadow_ms = c;
   tm.tail_ms = d;
   tm.total_ms = tot;
-  tm.walk_launches = (int)pl->groups.size();
+  tm.walk_launches = (int)pl->stages.size();
   ctx->timing = tm;
   return DRX_OK;
 }

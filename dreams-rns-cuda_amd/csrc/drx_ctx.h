@@ -73,11 +73,13 @@ struct Timing {
 
 struct drx_ctx {
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;   // residue walk, combine, tails
+  hipStream_t stream2 = nullptr;  // FP64 shadow chain, forked from and joined back into `stream`
   std::string err;
   // primes
   std::vector<uint32_t> primes;
   drx::DBuf d_prime_p, d_prime_pinv, d_prime_e, d_prime_u;
+  drx::DBuf d_invs;  // [K][kFdInv + 1] balanced 1/t mod p_k (forward-difference table jumps)
   std::vector<uint32_t> inv_table;  // [kinv][kinv]: inv_table[j*kinv+i] = p_j^{-1} mod p_i
   int kinv = 0;
   drx::DBuf d_inv;
@@ -103,6 +105,37 @@ struct drx_ctx {
 
 // Shared host helpers (drx_api.cpp).
 namespace drx {
+
+// Events of the context: [0..4] timing marks of the last timed launch
+// sequence, [6] fork to stream2, [7] join back, [10..11] copy calibration.
+enum { kEvFork = 6, kEvJoin = 7 };
+
+// One group of a launch: walks of one matrix order and walk-kernel family.
+struct Stage {
+  int m = 0, kind = 0;             // WalkKind
+  int task_begin = 0, task_end = 0;
+  int span_begin = 0, span_end = 0;
+  int seed_begin = 0, seed_end = 0;    // spans to seed (forward-difference stages)
+  int range_begin = 0, range_end = 0;  // M_f blocks
+  int threads = 0, seed_threads = 0;
+  size_t lds = 0, seed_lds = 0, mf_lds = 0;
+};
+
+// Launch the walk pipeline of `stages` with the argument block `a`: on
+// stream2 the FP64 M_f pass and the shadow (with delta_float when
+// `finalize`), on stream the Newton-table seeds (spans from `seed_spans`),
+// the walk kernels and the ordered combine; stream then waits for stream2.
+// With `timed`, events 0..3 mark walk start, walk end, combine end, join.
+int launch_stages(drx_ctx* ctx, const KArgs& a, const std::vector<Stage>& stages, const DevSpan* seed_spans,
+                  int finalize, bool timed);
+// LDS bytes of the kernels of one walk (program `pg`, K primes) in a stage.
+size_t walk_lds_bytes(const DevProgram& pg, int kind, int m, uint32_t K, int threads);
+size_t seed_lds_bytes(const DevProgram& pg, int seed_threads);
+size_t mf_lds_bytes(const DevProgram& pg);
+// Walk-kernel family of program `hp` (forward differences unless interp_only).
+int walk_kind(const HostProgram& hp, bool interp_only);
+// Task shape for WALK_FD_PCF: 8 * deg b + deg a.
+int pcf_shape(const HostProgram& hp);
 int fail(drx_ctx* ctx, int code, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
 bool is_prime_u32(uint32_t n);
 double now_s();

@@ -14,29 +14,36 @@ struct KArgs {
   const DevWalk* walks;
   const DevProgram* progs;
   const DevSpan* spans;
+  const DevRange* ranges;  // M_f blocks
   const uint32_t* code;
   const double* cres;
   const double* cflt;
   const double* prime_p;
   const double* prime_pinv;
   const uint32_t* prime_e;
+  const double* invs;     // [K][kFdInv + 1]: 1/t mod p_k (balanced), t >= 1
   const double* axr;
   const int64_t* axi;
   const double* axd;
   const double* res_in;  // walk state before this launch: residues
-  const double* fst_in;  // walk state before this launch: P_f and log_scale
+  const double* fst_in;  // walk state before this launch: FP64 shadow state
   double* res;        // walk state after: residues (may alias res_in)
-  double* fst;        // walk state after: P_f and log_scale (may alias fst_in)
+  double* fst;        // walk state after: FP64 shadow state (may alias fst_in)
   double* cp;         // chunk products
   double* mf;         // shadow matrices M_f(s)
+  double* fdt;        // forward-difference Newton tables at step 0
   double* fout;       // per walk: p_f, q_f, log_scale, delta, log_q, (pad)
   int64_t* rout;      // [out_row][2][out_ld]: p residues then q residues
   int64_t* pout;      // optional [out_row][16][out_ld] full state residues
   double* fcompact;   // optional [out_row][6]: copy of fout of the finalized walks
   double* ffull;      // optional [out_row][16]: final P_f
   uint64_t out_ld;
-  int with_fp;
 };
+
+// Walk kernel families.
+enum WalkKind { WALK_INTERP = 0, WALK_FD = 1, WALK_FD_PCF = 2 };
+// Steps per M_f block (one lane per step).
+constexpr int kMfBlock = 256;
 
 struct GarnerArgs {
   const int64_t* res;     // [n] rows of canonical residues, row stride res_stride
@@ -53,11 +60,12 @@ struct GarnerArgs {
   uint8_t* neg;           // [n] 1 if the centred value is negative
 };
 
-// Forward-difference table depth of the polynomial walk kernel per matrix order.
-constexpr int kFdMax2 = 6;
-constexpr int kFdMax3 = 3;
 
-hipError_t launch_walk(int m, int fd, const KArgs& a, int ntask, int threads, size_t lds, hipStream_t s);
+hipError_t launch_walk(int m, int kind, const KArgs& a, int ntask, int threads, size_t lds, hipStream_t s);
+// Newton tables of forward-difference walks: one block of `threads` (= KP) lanes per span.
+hipError_t launch_fd_seed(int m, const KArgs& a, int nspan, int threads, size_t lds, hipStream_t s);
+// FP64 M_f(s) over `nrange` blocks of a.ranges.
+hipError_t launch_mf(int m, const KArgs& a, int nrange, size_t lds, hipStream_t s);
 hipError_t launch_combine(int m, const KArgs& a, int nspan, hipStream_t s);
 hipError_t launch_shadow(int m, const KArgs& a, int nspan, int finalize, hipStream_t s);
 hipError_t launch_finalize(const KArgs& a, int nspan, int full, hipStream_t s);

@@ -128,26 +128,51 @@ __device__ __forceinline__ uint32_t pow2_at_least(uint32_t k) {
 // One workgroup per task (a step range of one walk). Lane (c, k): sub-chunk
 // c of the range, prime k. Each lane multiplies M(s) over its sub-chunk; the
 // C sub-chunk products are then combined in order by a log2(C)-level tree in
-// LDS (the walk product is associative). Afterwards every lane evaluates the
-// FP64 matrices M_f(s) for the shadow, strided over the whole range.
+// LDS (the walk product is associative).
 //
-// LDS: [nconst][KP] constant residues + [nconst] FP64 constants (bank L),
-// [nslots][T] temps (bank S), then the tree buffer [C][M*M][KP].
+// Interpreting kernels keep the program's constants ([nconst][KP] residues
+// and [nconst] FP64 values, bank L) and temps ([nslots][T], bank S) in LDS;
+// the walk kernels put the tree buffer [C][M*M][KP] after them.
+
+struct ProgLds {
+  double *ldc, *ldcf, *lds_s, *end;
+};
+
+// Constants of `pg` into LDS (residues for K primes in a KP-wide stripe when
+// `residues`, FP64 values always); the caller synchronises.
+__device__ __forceinline__ ProgLds prog_lds(const KArgs& a, const DevProgram& pg, uint32_t K, uint32_t KP,
+                                            double* lds, bool residues) {
+  const uint32_t T = blockDim.x, tid = threadIdx.x;
+  const uint32_t nl = pg.const_lds ? pg.nconst : 0u;
+  ProgLds L;
+  L.ldc = lds;
+  L.ldcf = L.ldc + (residues ? (uint64_t)nl * KP : 0ull);
+  L.lds_s = L.ldcf + nl;
+  L.end = L.lds_s + (uint64_t)pg.nslots * T;
+  if (residues) {
+    for (uint32_t i = tid; i < nl * KP; i += T) {
+      const uint32_t ci = i / KP, ck = i % KP;
+      L.ldc[i] = (ck < K) ? a.cres[pg.cres_off + (uint64_t)ci * pg.kcap + ck] : 0.0;
+    }
+  }
+  for (uint32_t i = tid; i < nl; i += T) L.ldcf[i] = a.cflt[pg.cflt_off + i];
+  return L;
+}
 
 struct WalkSetup {
   DevTask t;
   DevWalk w;
   DevProgram pg;
   Lane ln;
-  uint32_t K, KP, C, c, k;
+  uint32_t K, KP, C, c, k, kk;
   bool active;
   int64_t sb, se;
   const uint2* code;
-  double *ldc, *ldcf, *lds_s, *tree;
+  double *ldc, *lds_s, *tree;
 };
 
-template <int M>
-__device__ __forceinline__ WalkSetup walk_setup(const KArgs& a, double* lds) {
+// Task, walk and lane geometry; with `interp`, the program constants in LDS.
+__device__ __forceinline__ WalkSetup walk_setup(const KArgs& a, double* lds, bool interp) {
   WalkSetup S;
   S.t = a.tasks[blockIdx.x];
   S.w = a.walks[S.t.walk];
@@ -159,24 +184,23 @@ __device__ __forceinline__ WalkSetup walk_setup(const KArgs& a, double* lds) {
   S.k = tid % S.KP;
   S.c = tid / S.KP;
   S.active = S.k < S.K;
-  const uint32_t kk = S.active ? S.k : 0u;
-  S.ln.p = a.prime_p[kk];
-  S.ln.pinv = a.prime_pinv[kk];
-  S.ln.pe = a.prime_e[kk];
+  S.kk = S.active ? S.k : 0u;
+  S.ln.p = a.prime_p[S.kk];
+  S.ln.pinv = a.prime_pinv[S.kk];
+  S.ln.pe = a.prime_e[S.kk];
   S.ln.k = S.k;
   S.ln.tid = tid;
   S.ln.T = T;
   S.code = reinterpret_cast<const uint2*>(a.code + S.pg.code_off);
-  const uint32_t nl = S.pg.const_lds ? S.pg.nconst : 0u;
-  S.ldc = lds;
-  S.ldcf = S.ldc + (uint64_t)nl * S.KP;
-  S.lds_s = S.ldcf + nl;
-  S.tree = S.lds_s + (uint64_t)S.pg.nslots * T;
-  for (uint32_t i = tid; i < nl * S.KP; i += T) {
-    const uint32_t ci = i / S.KP, ck = i % S.KP;
-    S.ldc[i] = (ck < S.K) ? a.cres[S.pg.cres_off + (uint64_t)ci * S.pg.kcap + ck] : 0.0;
+  if (interp) {
+    const ProgLds L = prog_lds(a, S.pg, S.K, S.KP, lds, true);
+    S.ldc = L.ldc;
+    S.lds_s = L.lds_s;
+    S.tree = L.end;
+  } else {
+    S.ldc = S.lds_s = nullptr;
+    S.tree = lds;
   }
-  for (uint32_t i = tid; i < nl; i += T) S.ldcf[i] = a.cflt[S.pg.cflt_off + i];
   __syncthreads();
   const int64_t L = S.t.s_end - S.t.s_begin;
   const int64_t len = (L + S.C - 1) / S.C;
@@ -189,20 +213,20 @@ __device__ __forceinline__ WalkSetup walk_setup(const KArgs& a, double* lds) {
 
 // Residue constants into the VGPR bank; axis values at step s0 and per-step
 // increments into the X bank.
-__device__ __forceinline__ void load_rns_inputs(const KArgs& a, const WalkSetup& S, int64_t s0, v16d& X,
+__device__ __forceinline__ void load_rns_inputs(const KArgs& a, const DevWalk& w, const DevProgram& pg,
+                                                uint32_t K, uint32_t kk, const Lane& ln, int64_t s0, v16d& X,
                                                 v16d& Cv) {
-  const double p = S.ln.p, pinv = S.ln.pinv;
-  const uint32_t kk = S.active ? S.k : 0u;
-  if (!S.pg.const_lds) {
+  const double p = ln.p, pinv = ln.pinv;
+  if (!pg.const_lds) {
 #pragma unroll
     for (int i = 0; i < 16; ++i)
-      if ((uint32_t)i < S.pg.nconst) Cv[i] = a.cres[S.pg.cres_off + (uint64_t)i * S.pg.kcap + kk];
+      if ((uint32_t)i < pg.nconst) Cv[i] = a.cres[pg.cres_off + (uint64_t)i * pg.kcap + kk];
   }
 #pragma unroll
   for (int ax = 0; ax < kMaxDim; ++ax) {
-    if ((uint32_t)ax < S.pg.dim) {
-      const double* q = a.axr + S.w.axr_off + (uint64_t)ax * 3 * S.K;
-      const double c0 = q[kk], c1 = q[S.K + kk], c2 = q[2 * S.K + kk];
+    if ((uint32_t)ax < pg.dim) {
+      const double* q = a.axr + w.axr_off + (uint64_t)ax * 3 * K;
+      const double c0 = q[kk], c1 = q[K + kk], c2 = q[2 * K + kk];
       const double x0 = red(c0 + mm((double)s0, c1, p, pinv), p, pinv);
       X[ax] = mm(x0, c2, p, pinv);
       X[kIncSlot + ax] = mm(c1, c2, p, pinv);
@@ -241,45 +265,15 @@ __device__ __forceinline__ void tree_combine(const KArgs& a, const WalkSetup& S,
   }
 }
 
-// FP64 shadow matrices M_f(s) for every step of the task, one step per lane.
-template <int M>
-__device__ __forceinline__ void fp_pass(const KArgs& a, const WalkSetup& S) {
-  const uint32_t T = blockDim.x, tid = threadIdx.x;
-  typename EBank<M>::type E = 0.0;
-  v16d R = 0.0, X = 0.0, Cv = 0.0;
-  if (!S.pg.const_lds) {
-#pragma unroll
-    for (int i = 0; i < 16; ++i)
-      if ((uint32_t)i < S.pg.nconst) Cv[i] = a.cflt[S.pg.cflt_off + i];
-  }
-  for (int64_t s = S.t.s_begin + tid; s < S.t.s_end; s += T) {
-#pragma unroll
-    for (int ax = 0; ax < kMaxDim; ++ax) {
-      if ((uint32_t)ax < S.pg.dim) {
-        const int64_t* q = a.axi + S.w.axi_off + 2 * ax;
-        X[ax] = (double)(q[0] + s * q[1]) / a.axd[S.w.axd_off + ax];
-      }
-    }
-    X[kStepSlot] = (double)s;
-    run_program<true>(S.code, S.pg.ncode, R, E, X, Cv, S.ldcf, 0, S.lds_s, S.ln);
-    double* out = a.mf + S.t.mf_off + (uint64_t)(s - S.t.s_begin) * M * M;
-#pragma unroll
-    for (int e = 0; e < M * M; ++e) {
-      const uint32_t kd = (S.pg.kinds >> (2 * e)) & 3u;
-      out[e] = (kd == E_GEN) ? E[e] : (kd == E_ONE ? 1.0 : 0.0);
-    }
-  }
-}
-
 // Interpreting walk: the program runs once per step and residue channel.
 template <int M>
 __global__ void __launch_bounds__(256) k_walk_chunk(const KArgs a) {
   extern __shared__ double lds[];
-  const WalkSetup S = walk_setup<M>(a, lds);
+  const WalkSetup S = walk_setup(a, lds, true);
   const double p = S.ln.p, pinv = S.ln.pinv;
   v16d R = 0.0, X = 0.0, Cv = 0.0;
   typename EBank<M>::type E = 0.0;
-  load_rns_inputs(a, S, S.sb, X, Cv);
+  load_rns_inputs(a, S.w, S.pg, S.K, S.kk, S.ln, S.sb, X, Cv);
   double P[M * M];
 #pragma unroll
   for (int e = 0; e < M * M; ++e) P[e] = (e % (M + 1) == 0) ? 1.0 : 0.0;
@@ -290,94 +284,212 @@ __global__ void __launch_bounds__(256) k_walk_chunk(const KArgs a) {
     advance_axes(X, S.pg.dim, p, pinv);
   }
   tree_combine<M>(a, S, P);
-  if (a.with_fp) fp_pass<M>(a, S);
 }
 
-// Forward-difference walk for programs whose general entries are all
-// polynomials in the step (no INV on their paths). For an entry of degree
-// d, the table D[e][0..d] holds its forward differences; one step is
-// D[i] += D[i+1] for i < d (exact modular arithmetic, so the entry values are
-// those the program would compute). The interpreter runs only to seed the
-// tables: d + 1 evaluations at the start of each sub-chunk. Tables are
-// reduced every fourth step, which keeps every entry below C(4 + d, d) * 2^31
-// (< 2^41 for d <= DMAX).
-template <int M, int DMAX>
-__global__ void __launch_bounds__(256) k_walk_fd(const KArgs a) {
+// ------------------------------------------------ forward-difference walks
+// For a program whose general entries are all polynomials in the step (no
+// INV on their paths), entry e of degree d has the Newton table
+// N_j = Delta^j f_e(s), j <= d, and one step is N_j += N_{j+1} (j < d):
+// exact modular additions, so the entry values are those the program would
+// compute. k_fd_seed interprets the program at steps 0..d once per walk and
+// prime; every sub-chunk lane then moves the table to its own first step sb
+// with Delta^j f(sb) = sum_t C(sb, t) N_{j+t} and runs its steps with no
+// interpreter at all.
+
+// One workgroup of KP lanes per walk (span); lane k = prime k.
+template <int M>
+__global__ void __launch_bounds__(256) k_fd_seed(const KArgs a) {
+  constexpr int D = fd_dmax(M);
   extern __shared__ double lds[];
-  const WalkSetup S = walk_setup<M>(a, lds);
+  const DevSpan sp = a.spans[blockIdx.x];
+  const DevWalk w = a.walks[sp.walk];
+  const DevProgram pg = a.progs[w.prog];
+  const uint32_t K = (uint32_t)w.K, KP = blockDim.x, k = threadIdx.x;
+  const bool active = k < K;
+  const uint32_t kk = active ? k : 0u;
+  Lane ln;
+  ln.p = a.prime_p[kk];
+  ln.pinv = a.prime_pinv[kk];
+  ln.pe = a.prime_e[kk];
+  ln.k = k;
+  ln.tid = threadIdx.x;
+  ln.T = blockDim.x;
+  const ProgLds L = prog_lds(a, pg, K, KP, lds, true);
+  __syncthreads();
+  const uint2* code = reinterpret_cast<const uint2*>(a.code + pg.code_off);
+  v16d R = 0.0, X = 0.0, Cv = 0.0;
+  typename EBank<M>::type E = 0.0;
+  load_rns_inputs(a, w, pg, K, kk, ln, 0, X, Cv);
+  // all D + 1 points: the differences above an entry's degree then come out
+  // as exact zeros, which the full-depth kernel relies on
+  double N[M * M][D + 1];
+#pragma unroll
+  for (int j = 0; j <= D; ++j) {
+    X[kStepSlot] = (double)j;
+    run_program<false>(code, pg.ncode, R, E, X, Cv, L.ldc, KP, L.lds_s, ln);
+    advance_axes(X, pg.dim, ln.p, ln.pinv);
+#pragma unroll
+    for (int e = 0; e < M * M; ++e) N[e][j] = E[e];
+  }
+  // values at 0..D -> Newton forward differences at 0
+#pragma unroll
+  for (int i = 1; i <= D; ++i)
+#pragma unroll
+    for (int j = D; j >= i; --j)
+#pragma unroll
+      for (int e = 0; e < M * M; ++e) N[e][j] = N[e][j] - N[e][j - 1];
+  if (active) {
+    double* out = a.fdt + w.fdt_off;
+#pragma unroll
+    for (int e = 0; e < M * M; ++e)
+#pragma unroll
+      for (int j = 0; j <= D; ++j) out[((uint64_t)e * (D + 1) + j) * K + k] = red(N[e][j], ln.p, ln.pinv);
+  }
+}
+
+// Newton table at step 0 -> at step s0: G_j = sum_{t <= D-j} C(s0, t) N_{j+t},
+// with C(s0, t) = C(s0, t-1) (s0 - t + 1) / t mod p. Inputs and outputs are
+// balanced (below p/2 in magnitude).
+template <int D>
+__device__ __forceinline__ void fd_jump(double (&N)[D + 1], int64_t s0, const KArgs& a, uint32_t kk, double p,
+                                        double pinv) {
+  double B[D + 1];
+  B[0] = 1.0;
+#pragma unroll
+  for (int t = 1; t <= D; ++t)
+    B[t] = mm(mm(B[t - 1], (double)(s0 - t + 1), p, pinv), a.invs[(uint64_t)kk * (kFdInv + 1) + t], p, pinv);
+  double G[D + 1];
+#pragma unroll
+  for (int j = 0; j <= D; ++j) {
+    double acc = N[j];
+#pragma unroll
+    for (int t = 1; t + j <= D; ++t) acc += mm(B[t], N[j + t], p, pinv);
+    G[j] = red(acc, p, pinv);
+  }
+#pragma unroll
+  for (int j = 0; j <= D; ++j) N[j] = G[j];
+}
+
+// Steps between table reductions: the tables start below 2^30 and grow by at
+// most C(U + d, d), which keeps every multiplier below 2^40 (d <= 4: U = 8,
+// C(12, 4) = 495; d <= 6: U = 4, C(10, 6) = 210).
+__host__ __device__ constexpr int fd_unroll(int d) { return d <= 4 ? 8 : 4; }
+
+template <int D>
+__device__ __forceinline__ void fd_advance(double (&N)[D + 1]) {
+#pragma unroll
+  for (int i = 0; i < D; ++i) N[i] += N[i + 1];
+}
+
+template <int D>
+__device__ __forceinline__ void fd_reduce(double (&N)[D + 1], double p, double pinv) {
+#pragma unroll
+  for (int i = 0; i < D; ++i) N[i] = red(N[i], p, pinv);  // N[D] is constant and already reduced
+}
+
+// Companion PCF M(n) = [[0, b(n)], [1, a(n)]] with deg b = DB, deg a = DA:
+// one step maps row (x, y) to (y, x b + y a). Entries of P stay below 2^33.
+template <int DB, int DA>
+__device__ __forceinline__ void pcf_fd_body(const KArgs& a, const WalkSetup& S) {
+  constexpr int U = fd_unroll(DB > DA ? DB : DA);
+  constexpr int D = kFdMax2;
+  const double p = S.ln.p, pinv = S.ln.pinv;
+  const double* tb = a.fdt + S.w.fdt_off + (uint64_t)1 * (D + 1) * S.K + S.kk;  // entry (0, 1) = b
+  const double* ta = a.fdt + S.w.fdt_off + (uint64_t)3 * (D + 1) * S.K + S.kk;  // entry (1, 1) = a
+  double Bt[DB + 1], At[DA + 1];
+#pragma unroll
+  for (int j = 0; j <= DB; ++j) Bt[j] = tb[(uint64_t)j * S.K];
+#pragma unroll
+  for (int j = 0; j <= DA; ++j) At[j] = ta[(uint64_t)j * S.K];
+  fd_jump<DB>(Bt, S.sb, a, S.kk, p, pinv);
+  fd_jump<DA>(At, S.sb, a, S.kk, p, pinv);
+  double x0 = 1.0, y0 = 0.0, x1 = 0.0, y1 = 1.0;
+  int64_t n = S.se - S.sb;
+  for (; n >= U; n -= U) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const double nx0 = y0, nx1 = y1;
+      y0 = mm(x0, Bt[0], p, pinv) + mm(y0, At[0], p, pinv);
+      y1 = mm(x1, Bt[0], p, pinv) + mm(y1, At[0], p, pinv);
+      x0 = nx0;
+      x1 = nx1;
+      fd_advance<DB>(Bt);
+      fd_advance<DA>(At);
+    }
+    fd_reduce<DB>(Bt, p, pinv);
+    fd_reduce<DA>(At, p, pinv);
+  }
+  for (; n > 0; --n) {
+    const double nx0 = y0, nx1 = y1;
+    y0 = mm(x0, Bt[0], p, pinv) + mm(y0, At[0], p, pinv);
+    y1 = mm(x1, Bt[0], p, pinv) + mm(y1, At[0], p, pinv);
+    x0 = nx0;
+    x1 = nx1;
+    fd_advance<DB>(Bt);
+    fd_advance<DA>(At);
+  }
+  double P[4] = {x0, y0, x1, y1};
+  tree_combine<2>(a, S, P);
+}
+
+#define DRX_PCF_CASE(DB, DA) \
+  case 8 * DB + DA:          \
+    pcf_fd_body<DB, DA>(a, S); \
+    break;
+#define DRX_PCF_ROW(DB)                                                                                  \
+  DRX_PCF_CASE(DB, 0) DRX_PCF_CASE(DB, 1) DRX_PCF_CASE(DB, 2) DRX_PCF_CASE(DB, 3) DRX_PCF_CASE(DB, 4) \
+  DRX_PCF_CASE(DB, 5) DRX_PCF_CASE(DB, 6)
+
+// Companion-PCF walks; the task's shape (8 deg b + deg a) picks a fully
+// unrolled body (the shape is uniform over the workgroup).
+__global__ void __launch_bounds__(256) k_walk_fd_pcf(const KArgs a) {
+  extern __shared__ double lds[];
+  const WalkSetup S = walk_setup(a, lds, false);
+  switch (S.t.shape) {
+    DRX_PCF_ROW(0) DRX_PCF_ROW(1) DRX_PCF_ROW(2) DRX_PCF_ROW(3) DRX_PCF_ROW(4) DRX_PCF_ROW(5) DRX_PCF_ROW(6)
+    default: break;
+  }
+}
+#undef DRX_PCF_ROW
+#undef DRX_PCF_CASE
+
+// Any other polynomial program: entry kinds are runtime (uniform) values and
+// every general entry's table is advanced at the full depth D (the Newton
+// differences above an entry's degree are zero).
+template <int M>
+__global__ void __launch_bounds__(256) k_walk_fd(const KArgs a) {
+  constexpr int D = fd_dmax(M);
+  constexpr int U = fd_unroll(D);
+  extern __shared__ double lds[];
+  const WalkSetup S = walk_setup(a, lds, false);
   const double p = S.ln.p, pinv = S.ln.pinv;
   const uint32_t kinds = S.pg.kinds;
-  const uint64_t degs = S.pg.fd_deg;
-  double D[M * M][DMAX + 1];
-  {
-    v16d R = 0.0, X = 0.0, Cv = 0.0;
-    typename EBank<M>::type E = 0.0;
-    load_rns_inputs(a, S, S.sb, X, Cv);
+  double N[M * M][D + 1];
 #pragma unroll
-    for (int j = 0; j <= DMAX; ++j) {
-      if ((uint32_t)j <= S.pg.fd_degree) {
-        X[kStepSlot] = (double)(S.sb + j);
-        run_program<false>(S.code, S.pg.ncode, R, E, X, Cv, S.ldc, S.KP, S.lds_s, S.ln);
-        advance_axes(X, S.pg.dim, p, pinv);
-      }
+  for (int e = 0; e < M * M; ++e) {
+    const double* t = a.fdt + S.w.fdt_off + (uint64_t)e * (D + 1) * S.K + S.kk;
 #pragma unroll
-      for (int e = 0; e < M * M; ++e) D[e][j] = ((uint32_t)j <= S.pg.fd_degree) ? E[e] : 0.0;
-    }
+    for (int j = 0; j <= D; ++j) N[e][j] = t[(uint64_t)j * S.K];
+    fd_jump<D>(N[e], S.sb, a, S.kk, p, pinv);
   }
-  // Newton forward differences: D[e][j] <- Delta^j f(sb)
-#pragma unroll
-  for (int i = 1; i <= DMAX; ++i)
-#pragma unroll
-    for (int j = DMAX; j >= i; --j)
-#pragma unroll
-      for (int e = 0; e < M * M; ++e) D[e][j] = D[e][j] - D[e][j - 1];
-#pragma unroll
-  for (int e = 0; e < M * M; ++e)
-#pragma unroll
-    for (int j = 0; j <= DMAX; ++j) D[e][j] = red(D[e][j], p, pinv);
-
   double P[M * M];
 #pragma unroll
   for (int e = 0; e < M * M; ++e) P[e] = (e % (M + 1) == 0) ? 1.0 : 0.0;
-  int64_t s = S.sb;
   uint32_t phase = 0;
-  for (; s < S.se; ++s) {
-    double Cm[M * M];
+  for (int64_t s = S.sb; s < S.se; ++s) {
+    double E[M * M];
 #pragma unroll
-    for (int i = 0; i < M; ++i) {
+    for (int e = 0; e < M * M; ++e) E[e] = N[e][0];
+    step_matmul<M>(P, E, kinds, S.pg.colred, p, pinv);
 #pragma unroll
-      for (int j = 0; j < M; ++j) {
-        double acc = 0.0;
+    for (int e = 0; e < M * M; ++e) fd_advance<D>(N[e]);
+    if (++phase == (uint32_t)U) {
+      phase = 0;
 #pragma unroll
-        for (int q = 0; q < M; ++q) {
-          const uint32_t kd = (kinds >> (2 * (q * M + j))) & 3u;
-          if (kd == E_GEN) acc += mm(P[i * M + q], D[q * M + j][0], p, pinv);
-          else if (kd == E_ONE) acc += P[i * M + q];
-        }
-        if ((S.pg.colred >> j) & 1u) acc = red(acc, p, pinv);
-        Cm[i * M + j] = acc;
-      }
-    }
-#pragma unroll
-    for (int e = 0; e < M * M; ++e) P[e] = Cm[e];
-    // advance the tables of the general entries
-    phase = (phase + 1) & 3u;
-#pragma unroll
-    for (int e = 0; e < M * M; ++e) {
-      if (((kinds >> (2 * e)) & 3u) != E_GEN) continue;
-      const uint32_t d = (uint32_t)(degs >> (4 * e)) & 15u;
-#pragma unroll
-      for (int i = 0; i < DMAX; ++i)
-        if ((uint32_t)i < d) D[e][i] += D[e][i + 1];
-      if (phase == 0) {
-#pragma unroll
-        for (int i = 0; i < DMAX; ++i)
-          if ((uint32_t)i < d) D[e][i] = red(D[e][i], p, pinv);
-      }
+      for (int e = 0; e < M * M; ++e) fd_reduce<D>(N[e], p, pinv);
     }
   }
   tree_combine<M>(a, S, P);
-  if (a.with_fp) fp_pass<M>(a, S);
 }
 
 // --------------------------------------------------------- ordered combine
@@ -404,83 +516,187 @@ __global__ void k_combine(const KArgs a) {
   }
 }
 
-// ------------------------------------------------------------ FP64 shadow
-// One lane per walk: P_f <- P_f @ M_f(s) for every step in order, each entry
-// a sequential FMA chain acc = a0*b0; acc = fma(ak, bk, acc); then
-// mx = max|P_f| (NaN propagates) and, if mx > 1e10, P_f /= mx and
-// log_scale += log(mx). Finishes with delta_float against the walk target.
+// ------------------------------------------------------- FP64 M_f(s) pass
+// The program in FP64 for a block of consecutive steps of one task, one lane
+// per step (runner.py:263-316 / cmf_walk.py:261-313): M_f(s) into the
+// shadow pool. Every lane runs the program (lanes past the range repeat the
+// last step) so the interpreter never diverges.
 template <int M>
-__global__ void k_shadow(const KArgs a, int nspan, int finalize) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nspan) return;
-  const DevSpan sp = a.spans[i];
-  const DevWalk w = a.walks[sp.walk];
-  const double* st_in = a.fst_in + w.fst_off;
-  double* st = a.fst + w.fst_off;
-  double P[M * M];
+__global__ void __launch_bounds__(256) k_mf(const KArgs a) {
+  extern __shared__ double lds[];
+  const DevRange r = a.ranges[blockIdx.x];
+  const DevTask t = a.tasks[r.task];
+  const DevWalk w = a.walks[t.walk];
+  const DevProgram pg = a.progs[w.prog];
+  const ProgLds L = prog_lds(a, pg, 0, 0, lds, false);
+  __syncthreads();
+  Lane ln;
+  ln.p = ln.pinv = 0.0;
+  ln.pe = 0;
+  ln.k = 0;
+  ln.tid = threadIdx.x;
+  ln.T = blockDim.x;
+  const uint2* code = reinterpret_cast<const uint2*>(a.code + pg.code_off);
+  typename EBank<M>::type E = 0.0;
+  v16d R = 0.0, X = 0.0, Cv = 0.0;
+  if (!pg.const_lds) {
 #pragma unroll
-  for (int e = 0; e < M * M; ++e) P[e] = st_in[e];
-  double ls = st_in[M * M];
-  // The loop body is one branch-free block: the rescale is a select, and
-  // log(mx) of step s is added while step s + 1 is multiplied (it only feeds
-  // log_scale, in the same order as the reference adds it).
-  double pend = 1.0;  // mx awaiting its log; log(1) == 0 exactly
-  for (int ti = sp.task_begin; ti < sp.task_end; ++ti) {
-    const DevTask t = a.tasks[ti];
-    const double* mf = a.mf + t.mf_off;
-    for (int64_t s = t.s_begin; s < t.s_end; ++s, mf += M * M) {
-      double Mf[M * M];
+    for (int i = 0; i < 16; ++i)
+      if ((uint32_t)i < pg.nconst) Cv[i] = a.cflt[pg.cflt_off + i];
+  }
+  const int64_t s_want = r.s_begin + threadIdx.x;
+  const int64_t s = s_want < r.s_end ? s_want : r.s_end - 1;
 #pragma unroll
-      for (int e = 0; e < M * M; ++e) Mf[e] = mf[e];
-      double C[M * M];
-#pragma unroll
-      for (int r = 0; r < M; ++r)
-#pragma unroll
-        for (int cc = 0; cc < M; ++cc) {
-          double acc = P[r * M] * Mf[cc];
-#pragma unroll
-          for (int q = 1; q < M; ++q) acc = __builtin_fma(P[r * M + q], Mf[q * M + cc], acc);
-          C[r * M + cc] = acc;
-        }
-      ls += log(pend);
-      double mx = 0.0;
-      bool nan = false;
-#pragma unroll
-      for (int e = 0; e < M * M; ++e) {
-        const double v = __builtin_fabs(C[e]);
-        nan |= (v != v);
-        mx = (v > mx) ? v : mx;
-      }
-      const bool resc = !nan && mx > 1e10;
-      pend = resc ? mx : 1.0;
-#pragma unroll
-      for (int e = 0; e < M * M; ++e) P[e] = resc ? C[e] / mx : C[e];
+  for (int ax = 0; ax < kMaxDim; ++ax) {
+    if ((uint32_t)ax < pg.dim) {
+      const int64_t* q = a.axi + w.axi_off + 2 * ax;
+      X[ax] = (double)(q[0] + s * q[1]) / a.axd[w.axd_off + ax];
     }
   }
-  ls += log(pend);
+  X[kStepSlot] = (double)s;
+  run_program<true>(code, pg.ncode, R, E, X, Cv, L.ldcf, 0, L.lds_s, ln);
+  if (s_want < r.s_end) {
+    double* out = a.mf + t.mf_off + (uint64_t)(s - t.s_begin) * M * M;
 #pragma unroll
-  for (int e = 0; e < M * M; ++e) st[e] = P[e];
-  st[M * M] = ls;
-  if (finalize) {
-    // delta_float (runner.py:86-100)
-    const double pf = P[M - 1], qf = P[M * M - 1];
-    double delta, logq;
-    if (!isfinite(pf) || !isfinite(qf) || __builtin_fabs(qf) < 1e-300) {
-      delta = -1e10;
-      logq = 0.0;
-    } else {
-      const double err = __builtin_fabs(pf / qf - w.target);
-      logq = ls + log(__builtin_fabs(qf));
-      if (err < 1e-300) delta = 100.0;
-      else if (logq <= 0.0) delta = -1e10;
-      else delta = -(1.0 + log(err) / logq);
+    for (int e = 0; e < M * M; ++e) {
+      const uint32_t kd = (pg.kinds >> (2 * e)) & 3u;
+      out[e] = (kd == E_GEN) ? E[e] : (kd == E_ONE ? 1.0 : 0.0);
     }
-    double* o = a.fout + (uint64_t)sp.walk * 6;
-    o[0] = pf;
-    o[1] = qf;
-    o[2] = ls;
-    o[3] = delta;
-    o[4] = logq;
+  }
+}
+
+// ------------------------------------------------------------ FP64 shadow
+// P_f <- P_f @ M_f(s) for every step in order (runner.py:388-405), each entry
+// a sequential FMA chain acc = a0*b0; acc = fma(ak, bk, acc); then
+// mx = max|P_f| (NaN propagates) and, if mx > 1e10, P_f /= mx and log_scale
+// grows by ln mx. G lanes per walk, lane (i, j) holding entry P_f[i][j]: rows
+// come from the other lanes of the group (DPP quad permutes for 2x2,
+// ds_bpermute otherwise) and the maximum is a butterfly over the group.
+// log_scale is carried as a mantissa product and a binary exponent, both
+// exact in step order, and ln is taken once when it is read.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+// quad_perm selectors (lane l of a quad reads lane sel[l])
+constexpr int kQuadRow0 = 0xA0;  // [0, 0, 2, 2]: P[i][0] of the lane's row
+constexpr int kQuadRow1 = 0xF5;  // [1, 1, 3, 3]: P[i][1]
+constexpr int kQuadX1 = 0xB1;    // [1, 0, 3, 2]: partner at xor 1
+constexpr int kQuadX2 = 0x4E;    // [2, 3, 0, 1]: partner at xor 2
+
+template <int M>
+__device__ __forceinline__ double group_row(double v, int gbase, int i, int q) {
+  if (M == 2) return q == 0 ? dpp_f64<kQuadRow0>(v) : dpp_f64<kQuadRow1>(v);
+  return __shfl(v, gbase + i * M + q, 64);
+}
+
+template <int M>
+__device__ __forceinline__ double group_max(double v) {
+  if (M == 2) {
+    v = fmax(v, dpp_f64<kQuadX1>(v));
+    return fmax(v, dpp_f64<kQuadX2>(v));
+  }
+#pragma unroll
+  for (int d = 1; d < 16; d <<= 1) v = fmax(v, __shfl_xor(v, d, 64));
+  return v;
+}
+
+constexpr int kShadowPf = 4;  // M_f loads issued this many steps ahead
+
+template <int M>
+__global__ void __launch_bounds__(256) k_shadow(const KArgs a, int nspan, int finalize) {
+  constexpr int G = (M == 2) ? 4 : 16;
+  const int lg = threadIdx.x % G;
+  const int gid = (int)((blockIdx.x * blockDim.x + threadIdx.x) / G);
+  const bool live = gid < nspan;
+  const DevSpan sp = a.spans[live ? gid : nspan - 1];
+  const DevWalk w = a.walks[sp.walk];
+  const bool ent = lg < M * M;
+  const int ei = ent ? lg / M : 0, ej = ent ? lg % M : 0;
+  const int gbase = (int)(threadIdx.x & 63) - lg;
+  const double* st_in = a.fst_in + w.fst_off;
+  double Pm = ent ? st_in[lg] : 0.0;
+  double mant = st_in[M * M], ex = st_in[M * M + 1];
+  int64_t nsteps = 0;
+  const double* mf = a.mf;
+  if (sp.task_end > sp.task_begin) {
+    const DevTask t0 = a.tasks[sp.task_begin], t1 = a.tasks[sp.task_end - 1];
+    mf = a.mf + t0.mf_off;
+    nsteps = t1.s_end - t0.s_begin;  // the walk's tasks are contiguous in the M_f pool
+  }
+  const int64_t last = nsteps > 0 ? nsteps - 1 : 0;
+  double ring[kShadowPf][M];
+#pragma unroll
+  for (int u = 0; u < kShadowPf; ++u) {
+    const int64_t sl = u < last ? u : last;
+#pragma unroll
+    for (int q = 0; q < M; ++q) ring[u][q] = mf[(uint64_t)sl * M * M + q * M + ej];
+  }
+  for (int64_t s = 0; s < nsteps; s += kShadowPf) {
+#pragma unroll
+    for (int u = 0; u < kShadowPf; ++u) {
+      double cur[M];
+#pragma unroll
+      for (int q = 0; q < M; ++q) cur[q] = ring[u][q];
+      {
+        const int64_t sn = s + u + kShadowPf;
+        const int64_t sl = sn < last ? sn : last;
+#pragma unroll
+        for (int q = 0; q < M; ++q) ring[u][q] = mf[(uint64_t)sl * M * M + q * M + ej];
+      }
+      double acc = group_row<M>(Pm, gbase, ei, 0) * cur[0];
+#pragma unroll
+      for (int q = 1; q < M; ++q) acc = __builtin_fma(group_row<M>(Pm, gbase, ei, q), cur[q], acc);
+      const double v = ent ? __builtin_fabs(acc) : 0.0;
+      const double nanv = (ent && acc != acc) ? 1.0 : 0.0;
+      const double mx = group_max<M>(v);
+      const double anynan = group_max<M>(nanv);
+      const bool resc = anynan == 0.0 && mx > 1e10;
+      const bool in = s + u < nsteps;
+      const double np = resc ? acc / mx : acc;
+      Pm = in ? (ent ? np : 0.0) : Pm;
+      int e1 = 0;
+      const double m1 = __builtin_frexp(mx, &e1);
+      const bool take = in && resc;
+      double nm = mant * (take ? m1 : 1.0);
+      int e2 = 0;
+      nm = __builtin_frexp(nm, &e2);
+      mant = nm;
+      ex += (take ? (double)e1 : 0.0) + (double)e2;
+    }
+  }
+  double* st = a.fst + w.fst_off;
+  if (live && ent) st[lg] = Pm;
+  if (live && lg == 0) {
+    st[M * M] = mant;
+    st[M * M + 1] = ex;
+  }
+  if (finalize) {
+    // delta_float (runner.py:86-100) on lane 0 of the group
+    const double pf = __shfl(Pm, gbase + M - 1, 64), qf = __shfl(Pm, gbase + M * M - 1, 64);
+    if (live && lg == 0) {
+      const double ls = log(mant) + ex * 0.69314718055994530942;
+      double delta, logq;
+      if (!isfinite(pf) || !isfinite(qf) || __builtin_fabs(qf) < 1e-300) {
+        delta = -1e10;
+        logq = 0.0;
+      } else {
+        const double err = __builtin_fabs(pf / qf - w.target);
+        logq = ls + log(__builtin_fabs(qf));
+        if (err < 1e-300) delta = 100.0;
+        else if (logq <= 0.0) delta = -1e10;
+        else delta = -(1.0 + log(err) / logq);
+      }
+      double* o = a.fout + (uint64_t)sp.walk * 6;
+      o[0] = pf;
+      o[1] = qf;
+      o[2] = ls;
+      o[3] = delta;
+      o[4] = logq;
+    }
   }
 }
 
@@ -749,17 +965,38 @@ __global__ void k_topk_rank(const double* __restrict__ d, int stride, int n, int
 }
 
 // ---------------------------------------------------------------- launchers
-hipError_t launch_walk(int m, int fd, const KArgs& a, int ntask, int threads, size_t lds, hipStream_t s) {
+hipError_t launch_walk(int m, int kind, const KArgs& a, int ntask, int threads, size_t lds, hipStream_t s) {
   if (ntask <= 0) return hipSuccess;
-  if (fd) {
-    if (m == 2) hipLaunchKernelGGL((k_walk_fd<2, kFdMax2>), dim3(ntask), dim3(threads), lds, s, a);
-    else if (m == 3) hipLaunchKernelGGL((k_walk_fd<3, kFdMax3>), dim3(ntask), dim3(threads), lds, s, a);
+  const dim3 grid(ntask), block(threads);
+  if (kind == WALK_FD_PCF) {
+    if (m != 2) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_walk_fd_pcf, grid, block, lds, s, a);
+  } else if (kind == WALK_FD) {
+    if (m == 2) hipLaunchKernelGGL(k_walk_fd<2>, grid, block, lds, s, a);
+    else if (m == 3) hipLaunchKernelGGL(k_walk_fd<3>, grid, block, lds, s, a);
     else return hipErrorInvalidValue;
-    return hipGetLastError();
+  } else {
+    if (m == 2) hipLaunchKernelGGL(k_walk_chunk<2>, grid, block, lds, s, a);
+    else if (m == 3) hipLaunchKernelGGL(k_walk_chunk<3>, grid, block, lds, s, a);
+    else if (m == 4) hipLaunchKernelGGL(k_walk_chunk<4>, grid, block, lds, s, a);
+    else return hipErrorInvalidValue;
   }
-  if (m == 2) hipLaunchKernelGGL(k_walk_chunk<2>, dim3(ntask), dim3(threads), lds, s, a);
-  else if (m == 3) hipLaunchKernelGGL(k_walk_chunk<3>, dim3(ntask), dim3(threads), lds, s, a);
-  else if (m == 4) hipLaunchKernelGGL(k_walk_chunk<4>, dim3(ntask), dim3(threads), lds, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_fd_seed(int m, const KArgs& a, int nspan, int threads, size_t lds, hipStream_t s) {
+  if (nspan <= 0) return hipSuccess;
+  if (m == 2) hipLaunchKernelGGL(k_fd_seed<2>, dim3(nspan), dim3(threads), lds, s, a);
+  else if (m == 3) hipLaunchKernelGGL(k_fd_seed<3>, dim3(nspan), dim3(threads), lds, s, a);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_mf(int m, const KArgs& a, int nrange, size_t lds, hipStream_t s) {
+  if (nrange <= 0) return hipSuccess;
+  if (m == 2) hipLaunchKernelGGL(k_mf<2>, dim3(nrange), dim3(kMfBlock), lds, s, a);
+  else if (m == 3) hipLaunchKernelGGL(k_mf<3>, dim3(nrange), dim3(kMfBlock), lds, s, a);
+  else if (m == 4) hipLaunchKernelGGL(k_mf<4>, dim3(nrange), dim3(kMfBlock), lds, s, a);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }
@@ -775,7 +1012,8 @@ hipError_t launch_combine(int m, const KArgs& a, int nspan, hipStream_t s) {
 
 hipError_t launch_shadow(int m, const KArgs& a, int nspan, int finalize, hipStream_t s) {
   if (nspan <= 0) return hipSuccess;
-  const int tb = 64, nb = (nspan + tb - 1) / tb;
+  const int g = (m == 2) ? 4 : 16;
+  const int tb = 256, nb = (int)(((int64_t)nspan * g + tb - 1) / tb);
   if (m == 2) hipLaunchKernelGGL(k_shadow<2>, dim3(nb), dim3(tb), 0, s, a, nspan, finalize);
   else if (m == 3) hipLaunchKernelGGL(k_shadow<3>, dim3(nb), dim3(tb), 0, s, a, nspan, finalize);
   else if (m == 4) hipLaunchKernelGGL(k_shadow<4>, dim3(nb), dim3(tb), 0, s, a, nspan, finalize);

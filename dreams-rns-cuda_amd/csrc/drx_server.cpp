@@ -62,11 +62,12 @@ struct drx_server {
   std::vector<SrvSlot> slots;
   std::deque<SrvDone> done;
   uint64_t next_ticket = 1;
-  DBuf d_walks, d_tasks, d_spans, d_fspans, d_res, d_fst, d_axr, d_axi, d_axd, d_cp, d_mf, d_fout, d_rout, d_fcomp;
+  DBuf d_walks, d_tasks, d_spans, d_fspans, d_sspans, d_ranges, d_res, d_fst, d_axr, d_axi, d_axd, d_cp, d_mf, d_fdt,
+      d_fout, d_rout, d_fcomp;
   std::vector<DevWalk> h_walks;
   ~drx_server() {
-    for (DBuf* d : {&d_walks, &d_tasks, &d_spans, &d_fspans, &d_res, &d_fst, &d_axr, &d_axi, &d_axd, &d_cp, &d_mf,
-                    &d_fout, &d_rout, &d_fcomp})
+    for (DBuf* d : {&d_walks, &d_tasks, &d_spans, &d_fspans, &d_sspans, &d_ranges, &d_res, &d_fst, &d_axr, &d_axi,
+                    &d_axd, &d_cp, &d_mf, &d_fdt, &d_fout, &d_rout, &d_fcomp})
       d->release();
   }
 };
@@ -75,6 +76,11 @@ namespace {
 
 size_t slot_res(const drx_server* s) { return (size_t)16 * s->kslot; }
 size_t slot_axr(const drx_server* s) { return (size_t)kMaxDim * 3 * s->kslot; }
+// Newton tables: the larger of 2x2 (4 entries x 7) and 3x3 (9 entries x 4) per prime
+size_t slot_fdt(const drx_server* s) {
+  const size_t a = 4 * (kFdMax2 + 1), b = 9 * (kFdMax3 + 1);
+  return (a > b ? a : b) * s->kslot;
+}
 
 // Host tables for a newly admitted walk, copied into its slot.
 int admit(drx_server* s, int slot, SrvReq& rq) {
@@ -83,7 +89,7 @@ int admit(drx_server* s, int slot, SrvReq& rq) {
   if (rq.epoch != ctx->epoch) return fail(ctx, DRX_ESTATE, "programs were released after this request was submitted");
   const HostProgram& hp = ctx->progs[w.prog];
   const int m = (int)hp.dev.m, dim = (int)hp.dev.dim, K = w.K;
-  std::vector<double> res((size_t)m * m * K), f(m * m + 1), axr((size_t)dim * 3 * K), axd(dim);
+  std::vector<double> res((size_t)m * m * K), f(fst_size(m)), axr((size_t)dim * 3 * K), axd(dim);
   std::vector<int64_t> axi((size_t)dim * 2);
   int rc = init_tables(ctx, w, m, nullptr, nullptr, res.data(), f.data());
   if (rc) return rc;
@@ -95,7 +101,8 @@ int admit(drx_server* s, int slot, SrvReq& rq) {
   dw.m = m;
   dw.dim = dim;
   dw.res_off = (uint64_t)slot * slot_res(s);
-  dw.fst_off = (uint64_t)slot * 17;
+  dw.fst_off = (uint64_t)slot * kFstMax;
+  dw.fdt_off = (uint64_t)slot * slot_fdt(s);
   dw.axr_off = (uint64_t)slot * slot_axr(s);
   dw.axi_off = (uint64_t)slot * kMaxDim * 2;
   dw.axd_off = (uint64_t)slot * kMaxDim;
@@ -126,26 +133,29 @@ int round_once(drx_server* s, std::vector<int>& finished) {
   drx_ctx* ctx = s->ctx;
   hipStream_t st = ctx->stream;
   finished.clear();
-  struct TaskRef {
-    int slot, m, fd;
-  };
   for (int i = 0; i < s->nslots; ++i)
     if (s->slots[i].used && s->slots[i].epoch != ctx->epoch)
       return fail(ctx, DRX_ESTATE, "programs were released while requests were resident");
+  struct TaskRef {
+    int slot, m, kind, shape;
+  };
   std::vector<TaskRef> refs;
-  for (int i = 0; i < s->nslots; ++i)
-    if (s->slots[i].used) refs.push_back({i, s->slots[i].m, ctx->progs[s->slots[i].w.prog].fd});
+  for (int i = 0; i < s->nslots; ++i) {
+    if (!s->slots[i].used) continue;
+    const HostProgram& hp = ctx->progs[s->slots[i].w.prog];
+    const int kind = walk_kind(hp, false);
+    refs.push_back({i, s->slots[i].m, kind, kind == WALK_FD_PCF ? pcf_shape(hp) : 0});
+  }
   if (refs.empty()) return DRX_OK;
   std::stable_sort(refs.begin(), refs.end(), [](const TaskRef& a, const TaskRef& b) {
-    return a.m != b.m ? a.m < b.m : a.fd < b.fd;
+    if (a.m != b.m) return a.m < b.m;
+    if (a.kind != b.kind) return a.kind < b.kind;
+    return a.shape < b.shape;
   });
   std::vector<DevTask> ht;
-  std::vector<DevSpan> hs, hf;
-  struct G {
-    int m, fd, t0, t1, threads;
-    size_t lds;
-  };
-  std::vector<G> groups;
+  std::vector<DevSpan> hs, hf, hseed;
+  std::vector<DevRange> hr;
+  std::vector<Stage> stages;
   uint64_t mf_total = 0;
   for (const TaskRef& r : refs) {
     SrvSlot& sl = s->slots[r.slot];
@@ -153,46 +163,79 @@ int round_once(drx_server* s, std::vector<int>& finished) {
     const DevProgram& dp = ctx->progs[dw.prog].dev;
     DevTask t{};
     t.walk = r.slot;
+    t.shape = r.shape;
     t.s_begin = sl.step;
     t.s_end = std::min<int64_t>(sl.w.depth, sl.step + s->cfg.round_steps);
     t.cp_off = (uint64_t)ht.size() * slot_res(s);
     t.mf_off = mf_total;
     mf_total += (uint64_t)(t.s_end - t.s_begin) * r.m * r.m;
-    if (groups.empty() || groups.back().m != r.m || groups.back().fd != r.fd)
-      groups.push_back({r.m, r.fd, (int)ht.size(), (int)ht.size(), 0, 0});
-    G& g = groups.back();
+    if (stages.empty() || stages.back().m != r.m || stages.back().kind != r.kind) {
+      Stage g;
+      g.m = r.m;
+      g.kind = r.kind;
+      g.task_begin = g.task_end = g.span_begin = g.span_end = (int)ht.size();
+      g.seed_begin = g.seed_end = (int)hseed.size();
+      g.range_begin = g.range_end = (int)hr.size();
+      stages.push_back(g);
+    }
+    Stage& g = stages.back();
     const int threads = std::max<int>(s->cfg.threads > 0 ? s->cfg.threads : 256, (int)pow2_ceil(dw.K));
     g.threads = std::max(g.threads, threads);
-    const size_t KP = pow2_ceil(dw.K);
-    const size_t nl = dp.const_lds ? dp.nconst : 0;
-    g.lds = std::max(g.lds, 8 * (nl * KP + nl + (size_t)dp.nslots * g.threads + (size_t)g.threads * r.m * r.m));
+    g.seed_threads = std::max<int>(g.seed_threads, (int)pow2_ceil(dw.K));
     DevSpan sp{};
     sp.walk = r.slot;
     sp.task_begin = (int)ht.size();
     sp.task_end = (int)ht.size() + 1;
     sp.out_row = r.slot;
+    for (int64_t q = t.s_begin; q < t.s_end; q += kMfBlock) {
+      DevRange rg{};
+      rg.task = (int)ht.size();
+      rg.s_begin = q;
+      rg.s_end = std::min<int64_t>(t.s_end, q + kMfBlock);
+      hr.push_back(rg);
+    }
     ht.push_back(t);
-    g.t1 = (int)ht.size();
+    g.task_end = g.span_end = (int)ht.size();
+    g.range_end = (int)hr.size();
     hs.push_back(sp);
+    if (r.kind != WALK_INTERP && sl.step == 0) {  // Newton tables once, on the walk's first round
+      hseed.push_back(sp);
+      g.seed_end = (int)hseed.size();
+    }
     if (t.s_end >= sl.w.depth) {
       sp.out_row = (int)hf.size();
       hf.push_back(sp);
       finished.push_back(r.slot);
     }
   }
-  for (const G& g : groups)
-    if (g.lds > 65536) return fail(ctx, DRX_ERANGE, "server round needs %zu bytes of LDS", g.lds);
+  for (Stage& g : stages) {
+    for (int ti = g.task_begin; ti < g.task_end; ++ti) {
+      const DevWalk& dw = s->h_walks[ht[ti].walk];
+      const DevProgram& dp = ctx->progs[dw.prog].dev;
+      g.lds = std::max(g.lds, walk_lds_bytes(dp, g.kind, g.m, (uint32_t)dw.K, g.threads));
+      g.mf_lds = std::max(g.mf_lds, mf_lds_bytes(dp));
+      if (g.kind != WALK_INTERP) g.seed_lds = std::max(g.seed_lds, seed_lds_bytes(dp, g.seed_threads));
+    }
+    if (std::max(g.lds, std::max(g.mf_lds, g.seed_lds)) > 65536)
+      return fail(ctx, DRX_ERANGE, "server round needs more than 64 KiB of LDS");
+  }
   DRX_HIPCHK(ctx, s->d_tasks.ensure(ht.size() * sizeof(DevTask)));
   DRX_HIPCHK(ctx, s->d_spans.ensure(hs.size() * sizeof(DevSpan)));
   DRX_HIPCHK(ctx, s->d_fspans.ensure(std::max<size_t>(1, hf.size()) * sizeof(DevSpan)));
+  DRX_HIPCHK(ctx, s->d_sspans.ensure(std::max<size_t>(1, hseed.size()) * sizeof(DevSpan)));
+  DRX_HIPCHK(ctx, s->d_ranges.ensure(std::max<size_t>(1, hr.size()) * sizeof(DevRange)));
   DRX_HIPCHK(ctx, s->d_cp.ensure(ht.size() * slot_res(s) * 8));
-  DRX_HIPCHK(ctx, s->d_mf.ensure(std::max<uint64_t>(1, mf_total) * 8));
+  DRX_HIPCHK(ctx, s->d_mf.ensure(std::max<uint64_t>(16, mf_total) * 8));
   DRX_HIPCHK(ctx, s->d_walks.ensure(s->h_walks.size() * sizeof(DevWalk)));
   DRX_HIPCHK(ctx, hipMemcpyAsync(s->d_walks.p, s->h_walks.data(), s->h_walks.size() * sizeof(DevWalk), hipMemcpyHostToDevice, st));
   DRX_HIPCHK(ctx, hipMemcpyAsync(s->d_tasks.p, ht.data(), ht.size() * sizeof(DevTask), hipMemcpyHostToDevice, st));
   DRX_HIPCHK(ctx, hipMemcpyAsync(s->d_spans.p, hs.data(), hs.size() * sizeof(DevSpan), hipMemcpyHostToDevice, st));
   if (!hf.empty())
     DRX_HIPCHK(ctx, hipMemcpyAsync(s->d_fspans.p, hf.data(), hf.size() * sizeof(DevSpan), hipMemcpyHostToDevice, st));
+  if (!hseed.empty())
+    DRX_HIPCHK(ctx, hipMemcpyAsync(s->d_sspans.p, hseed.data(), hseed.size() * sizeof(DevSpan), hipMemcpyHostToDevice, st));
+  if (!hr.empty())
+    DRX_HIPCHK(ctx, hipMemcpyAsync(s->d_ranges.p, hr.data(), hr.size() * sizeof(DevRange), hipMemcpyHostToDevice, st));
   int rc = upload_programs(ctx);
   if (rc) return rc;
   KArgs a{};
@@ -200,12 +243,14 @@ int round_once(drx_server* s, std::vector<int>& finished) {
   a.walks = s->d_walks.as<DevWalk>();
   a.progs = ctx->d_progs.as<DevProgram>();
   a.spans = s->d_spans.as<DevSpan>();
+  a.ranges = s->d_ranges.as<DevRange>();
   a.code = ctx->d_code.as<uint32_t>();
   a.cres = ctx->d_cres.as<double>();
   a.cflt = ctx->d_cflt.as<double>();
   a.prime_p = ctx->d_prime_p.as<double>();
   a.prime_pinv = ctx->d_prime_pinv.as<double>();
   a.prime_e = ctx->d_prime_e.as<uint32_t>();
+  a.invs = ctx->d_invs.as<double>();
   a.axr = s->d_axr.as<double>();
   a.axi = s->d_axi.as<int64_t>();
   a.axd = s->d_axd.as<double>();
@@ -215,24 +260,15 @@ int round_once(drx_server* s, std::vector<int>& finished) {
   a.fst = s->d_fst.as<double>();
   a.cp = s->d_cp.as<double>();
   a.mf = s->d_mf.as<double>();
+  a.fdt = s->d_fdt.as<double>();
   a.fout = s->d_fout.as<double>();
   a.rout = s->d_rout.as<int64_t>();
   a.pout = nullptr;
   a.fcompact = s->d_fcomp.as<double>();
   a.ffull = nullptr;
   a.out_ld = (uint64_t)s->kslot;
-  a.with_fp = 1;
-  for (const G& g : groups) {
-    KArgs ga = a;
-    ga.tasks += g.t0;
-    DRX_HIPCHK(ctx, launch_walk(g.m, g.fd, ga, g.t1 - g.t0, g.threads, g.lds, st));
-  }
-  for (const G& g : groups) {
-    KArgs ga = a;
-    ga.spans += g.t0;  // one span per task
-    DRX_HIPCHK(ctx, launch_combine(g.m, ga, g.t1 - g.t0, st));
-    DRX_HIPCHK(ctx, launch_shadow(g.m, ga, g.t1 - g.t0, 1, st));
-  }
+  rc = launch_stages(ctx, a, stages, s->d_sspans.as<DevSpan>(), 1, false);
+  if (rc) return rc;
   if (!hf.empty()) {
     KArgs fa = a;
     fa.spans = s->d_fspans.as<DevSpan>();
@@ -360,8 +396,10 @@ int drx_server_create(drx_ctx* ctx, const drx_server_cfg* cfg, drx_server** out)
   s->kslot = (int)ctx->primes.size();
   const int64_t budget = cfg->mem_budget > 0 ? cfg->mem_budget : ((int64_t)1 << 30);
   // resident state + per-round scratch of one slot
-  const int64_t per_slot = (int64_t)8 * (slot_res(s) * 2 + 17 + slot_axr(s) + kMaxDim * 3 + 6 + 2 * s->kslot) +
-                           (int64_t)8 * 16 * cfg->round_steps + (int64_t)sizeof(DevWalk) + 2 * sizeof(DevTask);
+  const int64_t per_slot =
+      (int64_t)8 * (slot_res(s) * 2 + kFstMax + slot_axr(s) + slot_fdt(s) + kMaxDim * 3 + 6 + 2 * s->kslot) +
+      (int64_t)8 * 16 * cfg->round_steps + (int64_t)sizeof(DevWalk) + 2 * sizeof(DevTask) + 3 * sizeof(DevSpan) +
+      (int64_t)sizeof(DevRange) * ((cfg->round_steps + kMfBlock - 1) / kMfBlock);
   int64_t cap = budget / per_slot;
   if (cfg->max_slots > 0) cap = std::min<int64_t>(cap, cfg->max_slots);
   if (cap < 1) {
@@ -372,7 +410,8 @@ int drx_server_create(drx_ctx* ctx, const drx_server_cfg* cfg, drx_server** out)
   s->slots.resize(s->nslots);
   s->h_walks.resize(s->nslots);
   const size_t ns = (size_t)s->nslots;
-  if (s->d_res.ensure(ns * slot_res(s) * 8) || s->d_fst.ensure(ns * 17 * 8) || s->d_axr.ensure(ns * slot_axr(s) * 8) ||
+  if (s->d_res.ensure(ns * slot_res(s) * 8) || s->d_fst.ensure(ns * kFstMax * 8) || s->d_axr.ensure(ns * slot_axr(s) * 8) ||
+      s->d_fdt.ensure(ns * slot_fdt(s) * 8) ||
       s->d_axi.ensure(ns * kMaxDim * 2 * 8) || s->d_axd.ensure(ns * kMaxDim * 8) || s->d_fout.ensure(ns * 6 * 8) ||
       s->d_rout.ensure(ns * 2 * s->kslot * 8) || s->d_fcomp.ensure(ns * 6 * 8)) {
     delete s;

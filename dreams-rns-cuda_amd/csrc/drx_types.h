@@ -1,6 +1,7 @@
 // Plain-data records shared by the host packer and the kernels.
 #pragma once
 
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace drx {
@@ -10,6 +11,18 @@ constexpr int kMaxK = 1024;       // primes in one context
 constexpr int kMaxDim = 7;        // axes per program (X bank: axes, increments, step)
 constexpr int kMaxConst = 1023;   // constants per program (10-bit field)
 constexpr int kMaxCode = 1 << 16; // device words per program
+constexpr int kFdMax2 = 6;        // forward-difference degree limit, 2x2 programs
+constexpr int kFdMax3 = 3;        // forward-difference degree limit, 3x3 programs
+constexpr int kFdInv = 6;         // small inverses 1/t mod p kept per prime (t <= kFdInv)
+
+// FP64 shadow state per walk: P_f [m*m], then log_scale as a mantissa
+// product in [0.5, 1) and a power-of-two exponent (log_scale = ln(mant) +
+// exp * ln 2), so it accumulates exactly in step order without a log per step.
+__host__ __device__ constexpr int fst_size(int m) { return m * m + 2; }
+constexpr int kFstMax = 18;       // fst_size(kMaxM)
+
+// Forward-difference table depth (entries per degree column) for order m.
+__host__ __device__ constexpr int fd_dmax(int m) { return m == 2 ? kFdMax2 : kFdMax3; }
 
 // One packed program (device copy; offsets index the context pools).
 struct DevProgram {
@@ -36,7 +49,8 @@ struct DevWalk {
   int32_t m;
   int32_t dim;
   uint64_t res_off;     // P residues [m*m][K] (double, balanced)
-  uint64_t fst_off;     // P_f [m*m] then log_scale (double)
+  uint64_t fst_off;     // FP64 shadow state, fst_size(m) doubles
+  uint64_t fdt_off;     // forward-difference walks: Newton tables at step 0, [m*m][fd_dmax(m)+1][K]
   uint64_t axr_off;     // axis residues [dim][3][K]: c0, c1, c2 (double, balanced)
   uint64_t axi_off;     // axis integers [dim][2]: c0, c1 (int64) for the FP64 axis value
   uint64_t axd_off;     // axis denominators [dim] (double)
@@ -46,7 +60,7 @@ struct DevWalk {
 // A contiguous step range of one walk processed by one workgroup.
 struct DevTask {
   int32_t walk;
-  int32_t pad;
+  int32_t shape;        // companion-PCF forward-difference walks: 8 * deg b + deg a, else 0
   int64_t s_begin;
   int64_t s_end;
   uint64_t mf_off;      // M_f of step s at mf[mf_off + (s - s_begin) * m * m]
@@ -59,6 +73,14 @@ struct DevSpan {
   int32_t task_begin;   // tasks of this walk are contiguous and in step order
   int32_t task_end;
   int32_t out_row;      // row of this walk in the finalize outputs
+};
+
+// A block of consecutive steps of one task for the FP64 M_f(s) evaluation.
+struct DevRange {
+  int32_t task;
+  int32_t pad;
+  int64_t s_begin;
+  int64_t s_end;
 };
 
 }  // namespace drx

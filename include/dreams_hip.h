@@ -173,9 +173,10 @@ int drx_topk(drx_ctx* ctx, const double* values, int n, int k, int32_t* idx);
 int drx_copy_calibrate(drx_ctx* ctx, uint64_t bytes, int reps, double* ms);
 
 /* Device time of the last drx_walk_batch, from HIP events on the library's
- * stream: [0] walk kernels, [1] combine, [2] shadow, [3] finalize + CRT + top-k,
- * [4] whole batch (first to last launch); milliseconds. [5] walk-kernel
- * launches. */
+ * stream: [0] walk kernels, [1] combine, [2] wait for the FP64 shadow chain
+ * (which runs on a second stream beside the walk), [3] finalize + CRT +
+ * top-k, [4] whole batch from the first walk kernel; milliseconds. [5]
+ * walk-kernel launches. */
 int drx_last_timing(const drx_ctx* ctx, double* t6);
 
 /* ---- continuous-batching server ---------------------------------------

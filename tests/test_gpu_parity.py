@@ -241,3 +241,45 @@ def test_plan_built_before_release_is_refused(gpu_ctx):
         plan.run()
     plan.close()
     assert np.array_equal(BatchPlan(specs).run().residues, first)
+
+
+def _poly(rng, deg, lo=-20, hi=20, avoid01=False):
+    """Polynomial string in n of exact degree `deg` (leading coefficient != 0;
+    a constant avoids 0 and 1 when `avoid01`, so the entry stays general)."""
+    while True:
+        c = [int(v) for v in rng.integers(lo, hi + 1, size=deg + 1)]
+        if c[-1] != 0 and not (avoid01 and deg == 0 and c[0] in (0, 1)):
+            break
+    return " + ".join(f"({v})*n**{i}" for i, v in enumerate(c))
+
+
+def test_forward_difference_every_pcf_shape(gpu_ctx):
+    # every (deg b, deg a) body of the companion-PCF kernel, with depths that
+    # leave sub-chunks empty, short and unrolled-plus-remainder
+    rng = np.random.default_rng(404)
+    specs = []
+    depths = [0, 1, 2, 7, 9, 31, 250, 1001]
+    for db in range(7):
+        for da in range(7):
+            a, b = _poly(rng, da, avoid01=True), _poly(rng, db, avoid01=True)
+            p = D.compile_pcf(a, b)
+            specs.append(pcf_spec(p, D.pcf_initial_values(a), depths[(db * 7 + da) % len(depths)],
+                                  int(rng.choice([8, 31, 32, 64])), int(rng.integers(-3, 4)), target=0.25))
+    # a 2x2 polynomial program that is not a companion PCF (generic kernel)
+    cf = lambda mat, m, dim, axes, dirs: D.compile_cmf_from_dict(mat, m, dim, axes, dirs)
+    mat = {(0, 0): "x**2 + 3", (0, 1): "2*x - 1", (1, 0): "x**3 - x", (1, 1): "5"}
+    specs.append(cmf_spec(cf(mat, 2, 1, ["x"], [1]), 777, 32, [2]))
+    a = BatchPlan(specs, full=True, crt=True).run()
+    b = BatchPlan(specs, full=True, crt=True, interp_only=True).run()
+    assert np.array_equal(a.full, b.full)
+    assert np.array_equal(a.full_f, b.full_f)
+    assert np.array_equal(a.limbs, b.limbs) and np.array_equal(a.neg, b.neg)
+    primes = D.generate_rns_primes(64)
+    for i in range(0, len(specs), 5):
+        s = specs[i]
+        init = None
+        if s.init == "pcf":
+            init = (np.array([[1] * s.K, [s.a0 % int(q) for q in primes[: s.K]], [0] * s.K, [1] * s.K]),
+                    np.array([1.0, float(s.a0), 0.0, 1.0]))
+        rr, Pf, ls = cref.walk(s.program, s.depth, primes[: s.K], list(s.axes), *(init or (None, None)))
+        assert np.array_equal(a.full_matrix(i, s.program.m), rr), i
