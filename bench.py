@@ -52,39 +52,42 @@ def program_work(prog) -> float:
     return 6.0 * mm + 2.0 * ma
 
 
-PCF_KINDS = 0 | (2 << 2) | (1 << 4) | (2 << 6)  # companion [[0, b], [1, a]]: ZERO, GEN, ONE, GEN
-
-
 def walk_kernel_name(progs) -> str:
-    """The walk kernel the library picks for these programs (the dominant
-    kernel of the batch when all of them agree)."""
+    """The walk kernel the library picks for most of these programs, with the
+    others and their walk counts."""
+    from collections import Counter
+
     from dreams_rns._native import pack_program
 
-    names = set()
+    names = Counter()
     for p in progs:
         meta = pack_program(p)[2]
         if meta["fd_ok"] and p.m == 2 and meta["fd_degree"] <= 6:
-            names.add("k_walk_fd_pcf" if meta["kinds"] == PCF_KINDS else "k_walk_fd<2>")
+            k = meta["kinds"]
+            pcf = (k & 3) == 0 and ((k >> 4) & 3) == 1  # entry (0,0) zero, (1,0) one
+            names["k_walk_fd_pcf" if pcf else "k_walk_fd<2>"] += 1
         elif meta["fd_ok"] and p.m == 3 and meta["fd_degree"] <= 3:
-            names.add("k_walk_fd<3>")
+            names["k_walk_fd<3>"] += 1
         else:
-            names.add(f"k_walk_chunk<{p.m}>")
-    return names.pop() if len(names) == 1 else "+".join(sorted(names))
+            names[f"k_walk_chunk<{p.m}>"] += 1
+    (top, _), rest = names.most_common(1)[0], names.most_common()[1:]
+    return top + "".join(f" (+{k} for {v} walks)" for k, v in rest)
 
 
-def measured_traffic(kernel: str, workload: str):
-    """HBM bytes per launch of `kernel` on `workload` from the committed
-    counter summary (FETCH_SIZE / WRITE_SIZE scaled by the copy calibration),
-    or None when the summary is for another kernel."""
-    path = os.path.join(ROOT, "profiles", "walk_kernel_traffic.json")
+def measured_counters(kernel: str, workload: str):
+    """Counter summary of `kernel` on `workload` committed beside the rocprofv3
+    summaries: HBM bytes per launch (FETCH_SIZE / WRITE_SIZE scaled by the
+    copy calibration) and VALU instructions per launch, or None when the
+    summary is for another kernel."""
+    path = os.path.join(ROOT, "profiles", "walk_kernel_counters.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    if t.get("kernel") != kernel or t.get("workload") != workload:
+    if t.get("kernel") != kernel.split(" ")[0] or t.get("workload") != workload:
         return None
-    return float(t["traffic_bytes"])
+    return t
 
 
 # ------------------------------------------------------------ CPU baseline
@@ -254,7 +257,13 @@ def main():
 
     achieved = work / (walk_avg * 1e-3) if walk_avg == walk_avg else None
     kname = walk_kernel_name(progs)
-    traffic = measured_traffic(kname, "pcf2")
+    cnt = measured_counters(kname, "pcf2")
+    traffic = float(cnt["traffic_bytes"]) if cnt else None
+    executed = None
+    if cnt and walk_avg == walk_avg:
+        ex = float(cnt["valu_insts"]) * 64 / (walk_avg * 1e-3)
+        executed = {"achieved": ex / 1e12, "frac": ex / PEAK_LANE_OPS,
+                    "valu_lane_ops_per_launch": float(cnt["valu_insts"]) * 64}
     line = {
         "metric": METRIC,
         "value": value,
@@ -279,7 +288,10 @@ def main():
                      "peak": PEAK_LANE_OPS / 1e12, "unit": "TOP/s",
                      "frac": achieved / PEAK_LANE_OPS if achieved else None, "traffic": traffic,
                      "kernel": kname, "kernel_ms": walk_avg,
-                     "work_per_launch": work},
+                     "work_per_launch": work,
+                     "work_model": "W = 6*MM + 2*MA lane-ops per residue-step on each walk's bytecode "
+                                   "(per-step program evaluation; forward differences skip it, so frac can pass 1)",
+                     "executed": executed},
         "cpu_baseline": cpu,
     }
     if cpu:

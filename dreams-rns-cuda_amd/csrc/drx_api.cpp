@@ -68,12 +68,17 @@ int upload_programs(drx_ctx* ctx) {
   const size_t np = ctx->progs.size();
   std::vector<DevProgram> dp(np);
   for (size_t i = 0; i < np; ++i) dp[i] = ctx->progs[i].dev;
-  HIPCHK(ctx, ctx->d_code.ensure_keep(std::max<size_t>(1, ctx->h_code.size()) * 4, ctx->retired));
+  // padded past the end: the interpreters read code in kCodeFetch-instruction pieces
+  const size_t code_words = ctx->h_code.size() + 2 * kCodeFetch;
+  HIPCHK(ctx, ctx->d_code.ensure_keep(code_words * 4, ctx->retired));
   HIPCHK(ctx, ctx->d_cres.ensure_keep(std::max<size_t>(1, ctx->h_cres.size()) * 8, ctx->retired));
   HIPCHK(ctx, ctx->d_cflt.ensure_keep(std::max<size_t>(1, ctx->h_cflt.size()) * 8, ctx->retired));
   HIPCHK(ctx, ctx->d_progs.ensure_keep(std::max<size_t>(1, np) * sizeof(DevProgram), ctx->retired));
-  if (!ctx->h_code.empty())
-    HIPCHK(ctx, hipMemcpy(ctx->d_code.p, ctx->h_code.data(), ctx->h_code.size() * 4, hipMemcpyHostToDevice));
+  {
+    std::vector<uint32_t> padded(code_words, 0u);
+    std::copy(ctx->h_code.begin(), ctx->h_code.end(), padded.begin());
+    HIPCHK(ctx, hipMemcpy(ctx->d_code.p, padded.data(), code_words * 4, hipMemcpyHostToDevice));
+  }
   if (!ctx->h_cres.empty())
     HIPCHK(ctx, hipMemcpy(ctx->d_cres.p, ctx->h_cres.data(), ctx->h_cres.size() * 8, hipMemcpyHostToDevice));
   if (!ctx->h_cflt.empty())
@@ -90,10 +95,13 @@ int walk_kind(const HostProgram& hp, bool interp_only) {
 }
 
 int pcf_shape(const HostProgram& hp) {
-  // companion PCF [[0, b], [1, a]]: entry kinds ZERO, GEN, ONE, GEN
-  const uint32_t pcf_kinds = E_ZERO | (E_GEN << 2) | (E_ONE << 4) | (E_GEN << 6);
-  if (!hp.fd || hp.dev.m != 2 || hp.dev.kinds != pcf_kinds) return -1;
-  const int db = (int)((hp.dev.fd_deg >> 4) & 15u), da = (int)((hp.dev.fd_deg >> 12) & 15u);
+  // companion PCF [[0, b], [1, a]]: entry (0, 0) zero, (1, 0) one; b and a of
+  // any kind (a static 0 or 1 is a table of degree 0)
+  if (!hp.fd || hp.dev.m != 2) return -1;
+  const uint32_t k = hp.dev.kinds;
+  if ((k & 3u) != E_ZERO || ((k >> 4) & 3u) != E_ONE) return -1;
+  const int db = ((k >> 2) & 3u) == E_GEN ? (int)((hp.dev.fd_deg >> 4) & 15u) : 0;
+  const int da = ((k >> 6) & 3u) == E_GEN ? (int)((hp.dev.fd_deg >> 12) & 15u) : 0;
   if (db > kFdMax2 || da > kFdMax2) return -1;
   return 8 * db + da;
 }
@@ -106,39 +114,78 @@ size_t walk_lds_bytes(const DevProgram& pg, int kind, int m, uint32_t K, int thr
   return 8 * (nl * KP + nl + (size_t)pg.nslots * threads + tree);
 }
 
-size_t seed_lds_bytes(const DevProgram& pg, int seed_threads) {
+size_t seed_lds_bytes(const DevProgram& pg, int kp, int seed_threads) {
   const size_t nl = pg.const_lds ? pg.nconst : 0;
-  return 8 * (nl * (size_t)seed_threads + nl + (size_t)pg.nslots * seed_threads);
+  return 8 * (nl * (size_t)kp + nl + (size_t)pg.nslots * seed_threads);
 }
 
 size_t mf_lds_bytes(const DevProgram& pg) {
-  const size_t nl = pg.const_lds ? pg.nconst : 0;
-  return 8 * (nl + (size_t)pg.nslots * kMfBlock);
+  return 8 * ((size_t)pg.nconst + (size_t)pg.nslots * kMfBlock);  // every FP64 constant, then temps
+}
+
+void shadow_windows(int64_t max_steps, int64_t* win, int* nwin) {
+  int64_t w = (max_steps + kMaxWin - 1) / kMaxWin;
+  w = (w + kMfRange - 1) / kMfRange * kMfRange;
+  if (w < 2 * kMfRange) w = 2 * kMfRange;
+  int n = (int)((max_steps + w - 1) / w);
+  *win = w;
+  *nwin = n < 1 ? 1 : n;
+}
+
+void order_ranges(std::vector<Stage>& stages, std::vector<DevRange>& ranges, int64_t win, int nwin) {
+  for (Stage& g : stages) {
+    auto first = ranges.begin() + g.range_begin, last = ranges.begin() + g.range_end;
+    std::stable_sort(first, last, [&](const DevRange& x, const DevRange& y) { return x.s_begin / win < y.s_begin / win; });
+    g.win_range.assign(nwin + 1, g.range_end);
+    g.win_range[0] = g.range_begin;
+    int c = 0;
+    for (int r = g.range_begin; r < g.range_end; ++r) {
+      const int rc = (int)std::min<int64_t>(ranges[r].s_begin / win, nwin - 1);
+      while (c < rc) g.win_range[++c] = r;
+    }
+    while (c < nwin) g.win_range[++c] = g.range_end;
+  }
 }
 
 int launch_stages(drx_ctx* ctx, const KArgs& a, const std::vector<Stage>& stages, const DevSpan* seed_spans,
-                  int finalize, bool timed) {
-  hipStream_t s1 = ctx->stream, s2 = ctx->stream2;
+                  int finalize, bool timed, int64_t win, int nwin) {
+  hipStream_t s1 = ctx->stream, s2 = ctx->stream2, s3 = ctx->stream3;
   hipEvent_t* ev = ctx->ev;
+  if (nwin < 1 || nwin > kMaxWin) return fail(ctx, DRX_EINVAL, "bad shadow window count %d", nwin);
   HIPCHK(ctx, hipEventRecord(ev[kEvFork], s1));
   HIPCHK(ctx, hipStreamWaitEvent(s2, ev[kEvFork], 0));
-  for (const Stage& g : stages) {
-    KArgs ma = a;
-    ma.ranges += g.range_begin;
-    HIPCHK(ctx, launch_mf(g.m, ma, g.range_end - g.range_begin, g.mf_lds, s2));
-  }
-  for (const Stage& g : stages) {
-    KArgs sa = a;
-    sa.spans += g.span_begin;
-    HIPCHK(ctx, launch_shadow(g.m, sa, g.span_end - g.span_begin, finalize, s2));
-  }
-  HIPCHK(ctx, hipEventRecord(ev[kEvJoin], s2));
+  HIPCHK(ctx, hipStreamWaitEvent(s3, ev[kEvFork], 0));
+  // Enqueue order follows the critical paths: the seeds and the first M_f
+  // window start together, the walk is queued before the later windows.
   for (const Stage& g : stages) {
     if (g.kind == WALK_INTERP || g.seed_end <= g.seed_begin) continue;
     KArgs fa = a;
     fa.spans = seed_spans + g.seed_begin;
-    HIPCHK(ctx, launch_fd_seed(g.m, fa, g.seed_end - g.seed_begin, g.seed_threads, g.seed_lds, s1));
+    HIPCHK(ctx, launch_fd_seed(g.m, fa, g.seed_end - g.seed_begin, g.seed_kp, fd_seed_threads(g.m, g.seed_kp),
+                               g.seed_lds, s1));
   }
+  auto mf_window = [&](int c) -> int {
+    for (const Stage& g : stages) {
+      const int r0 = g.win_range[c], r1 = g.win_range[c + 1];
+      KArgs ma = a;
+      ma.ranges += r0;
+      HIPCHK(ctx, launch_mf(g.m, ma, r1 - r0, g.mf_lds, s2));
+    }
+    HIPCHK(ctx, hipEventRecord(ev[kEvWin + c], s2));
+    return DRX_OK;
+  };
+  auto shadow_window = [&](int c) -> int {
+    HIPCHK(ctx, hipStreamWaitEvent(s3, ev[kEvWin + c], 0));
+    for (const Stage& g : stages) {
+      KArgs sa = a;
+      sa.spans += g.span_begin;
+      HIPCHK(ctx, launch_shadow(g.m, sa, g.span_end - g.span_begin, (finalize && c == nwin - 1) ? 1 : 0,
+                                (int64_t)c * win, c == nwin - 1 ? INT64_MAX : (int64_t)(c + 1) * win, s3));
+    }
+    return DRX_OK;
+  };
+  int rc = mf_window(0);
+  if (rc) return rc;
   if (timed) HIPCHK(ctx, hipEventRecord(ev[0], s1));
   for (const Stage& g : stages) {
     KArgs wa = a;
@@ -146,14 +193,24 @@ int launch_stages(drx_ctx* ctx, const KArgs& a, const std::vector<Stage>& stages
     HIPCHK(ctx, launch_walk(g.m, g.kind, wa, g.task_end - g.task_begin, g.threads, g.lds, s1));
   }
   if (timed) HIPCHK(ctx, hipEventRecord(ev[1], s1));
+  for (int c = 0; c < nwin; ++c) {
+    if (c + 1 < nwin && (rc = mf_window(c + 1))) return rc;
+    if ((rc = shadow_window(c))) return rc;
+  }
+  // stream2 ends before stream3's last wait, so joining stream3 joins both
+  HIPCHK(ctx, hipEventRecord(ev[kEvJoin], s3));
   for (const Stage& g : stages) {
     KArgs ca = a;
     ca.spans += g.span_begin;
     HIPCHK(ctx, launch_combine(g.m, ca, g.span_end - g.span_begin, s1));
   }
   if (timed) HIPCHK(ctx, hipEventRecord(ev[2], s1));
-  HIPCHK(ctx, hipStreamWaitEvent(s1, ev[kEvJoin], 0));
-  if (timed) HIPCHK(ctx, hipEventRecord(ev[3], s1));
+  return DRX_OK;
+}
+
+int join_stages(drx_ctx* ctx, bool timed) {
+  HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev[kEvJoin], 0));
+  if (timed) HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
   return DRX_OK;
 }
 
@@ -257,6 +314,8 @@ struct drx_plan {
   uint64_t epoch = 0;  // context epoch the plan was built in
   int n = 0, kmax = 0, ld = 0;
   std::vector<Stage> stages;
+  int64_t win = 1;  // FP64 chain window (steps) and count
+  int nwin = 1;
   std::vector<int> walk_order;  // span index -> walk
   DBuf d_walks, d_tasks, d_spans, d_ranges, d_res0, d_fst0, d_res, d_fst, d_axr, d_axi, d_axd, d_cp, d_mf, d_fdt;
   DBuf d_fout, d_rout, d_pout, d_pfull, d_limbs, d_neg, d_topk, d_topk_ws, d_rowk;
@@ -290,10 +349,16 @@ int upload(drx_ctx* ctx, DBuf& d, const std::vector<T>& h) {
 int launch_sequence(drx_plan* pl, bool timed) {
   drx_ctx* ctx = pl->ctx;
   hipStream_t s = ctx->stream;
-  int rc = launch_stages(ctx, pl->args, pl->stages, pl->args.spans, 1, timed);
+  int rc = launch_stages(ctx, pl->args, pl->stages, pl->args.spans, 1, timed, pl->win, pl->nwin);
   if (rc) return rc;
-  HIPCHK(ctx, launch_finalize(pl->args, (int)pl->walk_order.size(), pl->b.full != nullptr, s));
-  if (pl->b.crt_limbs) HIPCHK(ctx, launch_garner(pl->gargs, s));
+  // residue outputs and CRT need only the combined residues: they run while
+  // the shadow chain is still going; the final P_f copy needs the shadow
+  const bool ffull = pl->args.ffull != nullptr;
+  if (!ffull) HIPCHK(ctx, launch_finalize(pl->args, (int)pl->walk_order.size(), pl->b.full != nullptr, s));
+  if (!ffull && pl->b.crt_limbs) HIPCHK(ctx, launch_garner(pl->gargs, s));
+  if ((rc = join_stages(ctx, timed))) return rc;
+  if (ffull) HIPCHK(ctx, launch_finalize(pl->args, (int)pl->walk_order.size(), pl->b.full != nullptr, s));
+  if (ffull && pl->b.crt_limbs) HIPCHK(ctx, launch_garner(pl->gargs, s));
   if (pl->b.top_k > 0)
     HIPCHK(ctx, launch_topk(pl->d_fout.as<double>() + 3, 6, pl->n, pl->b.top_k, pl->d_topk.as<int32_t>(),
                             pl->d_topk_ws.p, s));
@@ -421,12 +486,13 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
     }
     Stage& g = pl->stages.back();
     g.threads = std::max<int>(g.threads, pow2_ceil(K));
-    g.seed_threads = std::max<int>(g.seed_threads, pow2_ceil(K));
+    g.seed_kp = std::max<int>(g.seed_kp, pow2_ceil(K));
     DevSpan sp{};
     sp.walk = i;
     sp.out_row = i;
     sp.task_begin = (int)ht.size();
     const int shape = shape_of(i);
+    mf_total = (mf_total + 1) & ~(uint64_t)1;  // 16-byte aligned M_f per walk (its tasks stay contiguous)
     for (int64_t s0 = 0; s0 < w.depth; s0 += chunk) {
       DevTask t{};
       t.walk = i;
@@ -437,11 +503,11 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
       t.mf_off = mf_total;
       cp_total += (uint64_t)m * m * K;
       mf_total += (uint64_t)(t.s_end - t.s_begin) * m * m;
-      for (int64_t r = t.s_begin; r < t.s_end; r += kMfBlock) {
+      for (int64_t r = t.s_begin; r < t.s_end; r += kMfRange) {
         DevRange rg{};
         rg.task = (int)ht.size();
         rg.s_begin = r;
-        rg.s_end = std::min<int64_t>(t.s_end, r + kMfBlock);
+        rg.s_end = std::min<int64_t>(t.s_end, r + kMfRange);
         hr.push_back(rg);
       }
       ht.push_back(t);
@@ -469,11 +535,18 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
       const DevProgram& dp = ctx->progs[dw.prog].dev;
       g.lds = std::max(g.lds, walk_lds_bytes(dp, g.kind, g.m, (uint32_t)dw.K, g.threads));
       g.mf_lds = std::max(g.mf_lds, mf_lds_bytes(dp));
-      if (g.kind != WALK_INTERP) g.seed_lds = std::max(g.seed_lds, seed_lds_bytes(dp, g.seed_threads));
+      if (g.kind != WALK_INTERP)
+        g.seed_lds = std::max(g.seed_lds, seed_lds_bytes(dp, g.seed_kp, fd_seed_threads(g.m, g.seed_kp)));
     }
     const size_t need = std::max(g.lds, std::max(g.mf_lds, g.seed_lds));
     if (need > lds_limit)
       return fail(ctx, DRX_ERANGE, "program needs %zu bytes of LDS (limit %zu); use fewer threads", need, lds_limit);
+  }
+  {
+    int64_t maxd = 0;
+    for (int i = 0; i < n; ++i) maxd = std::max<int64_t>(maxd, b.walks[i].depth);
+    shadow_windows(maxd, &pl->win, &pl->nwin);
+    order_ranges(pl->stages, hr, pl->win, pl->nwin);
   }
   pl->ntask = (int)ht.size();
   pl->residue_steps = rsteps;
@@ -592,8 +665,6 @@ int plan_run_one(drx_plan* pl, bool timed) {
   drx_ctx* ctx = pl->ctx;
   if (pl->d_pout.p && pl->b.full)
     HIPCHK(ctx, hipMemsetAsync(pl->d_pout.p, 0, (size_t)pl->n * 16 * pl->ld * 8, ctx->stream));
-  if (pl->b.crt_limbs)
-    HIPCHK(ctx, hipMemsetAsync(pl->d_limbs.p, 0, (size_t)pl->n * 2 * pl->ld * 4, ctx->stream));
   if (pl->gexec) {
     if (timed) HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     HIPCHK(ctx, hipGraphLaunch(pl->gexec, ctx->stream));
@@ -630,8 +701,10 @@ int drx_ctx_create(int device, drx_ctx** out) {
   if (!ctx) return DRX_ENOMEM;
   ctx->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess) {
+      hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->stream3, hipStreamNonBlocking) != hipSuccess) {
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
     delete ctx;
     return DRX_EHIP;
   }
@@ -650,6 +723,7 @@ void drx_ctx_destroy(drx_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
+  if (ctx->stream3) (void)hipStreamSynchronize(ctx->stream3);
   for (DBuf* d : {&ctx->d_prime_p, &ctx->d_prime_pinv, &ctx->d_prime_e, &ctx->d_prime_u, &ctx->d_invs, &ctx->d_inv, &ctx->d_code,
                   &ctx->d_cres, &ctx->d_cflt, &ctx->d_progs, &ctx->d_tmp0, &ctx->d_tmp1, &ctx->d_tmp2, &ctx->d_tmp3})
     d->release();
@@ -658,6 +732,7 @@ void drx_ctx_destroy(drx_ctx* ctx) {
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
+  if (ctx->stream3) (void)hipStreamDestroy(ctx->stream3);
   delete ctx;
 }
 
@@ -1078,6 +1153,17 @@ int drx_topk(drx_ctx* ctx, const double* values, int n, int k, int32_t* idx) {
   HIPCHK(ctx, hipMemcpyAsync(idx, ctx->d_tmp1.p, (size_t)k * 4, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return DRX_OK;
+}
+
+int drx_host_alloc(uint64_t bytes, void** out) {
+  if (!out) return DRX_EINVAL;
+  *out = nullptr;
+  if (bytes == 0) bytes = 1;
+  return hipHostMalloc(out, bytes, hipHostMallocDefault) == hipSuccess ? DRX_OK : DRX_ENOMEM;
+}
+
+void drx_host_free(void* p) {
+  if (p) (void)hipHostFree(p);
 }
 
 int drx_copy_calibrate(drx_ctx* ctx, uint64_t bytes, int reps, double* ms) {

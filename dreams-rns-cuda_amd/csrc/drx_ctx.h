@@ -74,7 +74,8 @@ struct Timing {
 struct drx_ctx {
   int device = 0;
   hipStream_t stream = nullptr;   // residue walk, combine, tails
-  hipStream_t stream2 = nullptr;  // FP64 shadow chain, forked from and joined back into `stream`
+  hipStream_t stream2 = nullptr;  // FP64 M_f pass, forked from `stream`
+  hipStream_t stream3 = nullptr;  // FP64 shadow, window by window behind stream2, joined into `stream`
   std::string err;
   // primes
   std::vector<uint32_t> primes;
@@ -92,7 +93,7 @@ struct drx_ctx {
   // scratch for standalone CRT / top-k
   drx::DBuf d_tmp0, d_tmp1, d_tmp2, d_tmp3;
   drx::Timing timing;
-  hipEvent_t ev[12] = {};
+  hipEvent_t ev[24] = {};
   // Bumped when the prime table changes other than by extension or the
   // programs are released: plans and server requests of an older epoch
   // refer to tables that no longer exist and are refused.
@@ -107,8 +108,9 @@ struct drx_ctx {
 namespace drx {
 
 // Events of the context: [0..4] timing marks of the last timed launch
-// sequence, [6] fork to stream2, [7] join back, [10..11] copy calibration.
-enum { kEvFork = 6, kEvJoin = 7 };
+// sequence, [6] fork, [7] join back, [10..11] copy calibration, [12..] one per
+// M_f window (the shadow of window c waits for it).
+enum { kEvFork = 6, kEvJoin = 7, kEvWin = 12, kMaxWin = 8 };
 
 // One group of a launch: walks of one matrix order and walk-kernel family.
 struct Stage {
@@ -116,21 +118,32 @@ struct Stage {
   int task_begin = 0, task_end = 0;
   int span_begin = 0, span_end = 0;
   int seed_begin = 0, seed_end = 0;    // spans to seed (forward-difference stages)
-  int range_begin = 0, range_end = 0;  // M_f blocks
-  int threads = 0, seed_threads = 0;
+  int range_begin = 0, range_end = 0;  // M_f blocks, ordered by window
+  std::vector<int> win_range;          // [nwin + 1]: ranges of window c are win_range[c] .. win_range[c + 1]
+  int threads = 0, seed_kp = 0;    // walk workgroup; seed primes per point group (pow2 >= every K)
   size_t lds = 0, seed_lds = 0, mf_lds = 0;
 };
 
 // Launch the walk pipeline of `stages` with the argument block `a`: on
 // stream2 the FP64 M_f pass and the shadow (with delta_float when
 // `finalize`), on stream the Newton-table seeds (spans from `seed_spans`),
-// the walk kernels and the ordered combine; stream then waits for stream2.
-// With `timed`, events 0..3 mark walk start, walk end, combine end, join.
+// the walk kernels and the ordered combine. join_stages makes stream wait for
+// stream2 (work on stream before it overlaps the shadow chain). With
+// `timed`, events 0..3 mark walk start, walk end, combine end, join.
+// The FP64 chain runs in `nwin` windows of `win` steps: the M_f pass of
+// window c on stream2, the shadow of window c on stream3 behind it.
 int launch_stages(drx_ctx* ctx, const KArgs& a, const std::vector<Stage>& stages, const DevSpan* seed_spans,
-                  int finalize, bool timed);
+                  int finalize, bool timed, int64_t win, int nwin);
+int join_stages(drx_ctx* ctx, bool timed);
+// Window length (a multiple of kMfRange) and count for walks of up to
+// `max_steps` steps: at most kMaxWin windows of at least two M_f ranges.
+void shadow_windows(int64_t max_steps, int64_t* win, int* nwin);
+// Orders the M_f ranges of every stage by window (a range belongs to the
+// window of its first step) and fills Stage::win_range.
+void order_ranges(std::vector<Stage>& stages, std::vector<DevRange>& ranges, int64_t win, int nwin);
 // LDS bytes of the kernels of one walk (program `pg`, K primes) in a stage.
 size_t walk_lds_bytes(const DevProgram& pg, int kind, int m, uint32_t K, int threads);
-size_t seed_lds_bytes(const DevProgram& pg, int seed_threads);
+size_t seed_lds_bytes(const DevProgram& pg, int kp, int seed_threads);
 size_t mf_lds_bytes(const DevProgram& pg);
 // Walk-kernel family of program `hp` (forward differences unless interp_only).
 int walk_kind(const HostProgram& hp, bool interp_only);

@@ -40,10 +40,17 @@ struct KArgs {
   uint64_t out_ld;
 };
 
+// Instructions per scalar code load: the interpreters read the program in
+// 64-byte pieces instead of waiting for one 8-byte scalar load per
+// instruction; the code pool is padded by kCodeFetch instructions.
+constexpr int kCodeFetch = 8;
+
 // Walk kernel families.
 enum WalkKind { WALK_INTERP = 0, WALK_FD = 1, WALK_FD_PCF = 2 };
-// Steps per M_f block (one lane per step).
+// M_f pass: one lane per step, kMfBlock lanes per block; a range (one block)
+// covers kMfRange consecutive steps of one task.
 constexpr int kMfBlock = 256;
+constexpr int kMfRange = kMfBlock;
 
 struct GarnerArgs {
   const int64_t* res;     // [n] rows of canonical residues, row stride res_stride
@@ -56,18 +63,29 @@ struct GarnerArgs {
   const uint32_t* prime_u;
   const uint32_t* inv_table;  // inv_table[j * inv_ld + i] = p_j^{-1} mod p_i
   uint64_t inv_ld;
-  uint32_t* limbs;        // [n][res_stride] little-endian magnitude of the centred value
+  uint32_t* limbs;        // [n][res_stride] little-endian magnitude of the centred value (zero past K)
   uint8_t* neg;           // [n] 1 if the centred value is negative
 };
 
 
 hipError_t launch_walk(int m, int kind, const KArgs& a, int ntask, int threads, size_t lds, hipStream_t s);
-// Newton tables of forward-difference walks: one block of `threads` (= KP) lanes per span.
-hipError_t launch_fd_seed(int m, const KArgs& a, int nspan, int threads, size_t lds, hipStream_t s);
+// Newton tables of forward-difference walks: one block per span of `threads`
+// lanes, `kp` (power of two >= every K) per point group; threads % kp == 0.
+hipError_t launch_fd_seed(int m, const KArgs& a, int nspan, int kp, int threads, size_t lds, hipStream_t s);
+// Seed block for primes padded to kp: point groups of kp lanes, at most 512 lanes.
+inline int fd_seed_threads(int m, int kp) {
+  const int pts = fd_dmax(m) + 1;
+  int j = 512 / kp;
+  if (j < 1) j = 1;
+  if (j > pts) j = pts;
+  return j * kp;
+}
 // FP64 M_f(s) over `nrange` blocks of a.ranges.
 hipError_t launch_mf(int m, const KArgs& a, int nrange, size_t lds, hipStream_t s);
 hipError_t launch_combine(int m, const KArgs& a, int nspan, hipStream_t s);
-hipError_t launch_shadow(int m, const KArgs& a, int nspan, int finalize, hipStream_t s);
+// FP64 shadow over steps [w0, w1) of every span's walk (w0 == 0 starts from
+// fst_in, later windows continue from fst); delta_float when `finalize`.
+hipError_t launch_shadow(int m, const KArgs& a, int nspan, int finalize, int64_t w0, int64_t w1, hipStream_t s);
 hipError_t launch_finalize(const KArgs& a, int nspan, int full, hipStream_t s);
 hipError_t launch_garner(const GarnerArgs& g, hipStream_t s);
 // Top-k walk indices by value (larger first, NaN last, ties to the lower

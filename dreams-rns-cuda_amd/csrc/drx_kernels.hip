@@ -25,53 +25,67 @@ struct Lane {
   uint32_t tid, T;   // for LDS temp slots
 };
 
+// Operand of a residue instruction (bank L: [nconst][ldc_ld] residues).
 template <bool FP>
 __device__ __forceinline__ double fetch(uint32_t bank, uint32_t i, const v16d& R, const v16d& X,
                                         const v16d& Cv, double acc, const double* __restrict__ ldc,
                                         uint32_t ldc_ld, const double* __restrict__ lds_s,
                                         const Lane& ln) {
+  static_assert(!FP, "the FP64 pass has its own operand fetch");
   switch (bank) {
     case SB_R: return R[i & 15u];
     case SB_X: return X[i & 15u];
     case SB_C: return Cv[i & 15u];
-    case SB_L: return FP ? ldc[i] : ldc[i * ldc_ld + ln.k];
+    case SB_L: return ldc[i * ldc_ld + ln.k];
     case SB_S: return lds_s[i * ln.T + ln.tid];
     case SB_A: return acc;
     default: return 0.0;
   }
 }
 
-template <bool FP, typename EV>
-__device__ __forceinline__ void run_program(const uint2* __restrict__ code, uint32_t n, v16d& R,
-                                            EV& E, const v16d& X, const v16d& Cv,
-                                            const double* __restrict__ ldc, uint32_t ldc_ld,
-                                            double* __restrict__ lds_s, const Lane& ln) {
+// One residue instruction.
+template <typename EV>
+__device__ __forceinline__ void exec_res(uint2 w, double& acc, v16d& R, EV& E, const v16d& X, const v16d& Cv,
+                                         const double* __restrict__ ldc, uint32_t ldc_ld,
+                                         double* __restrict__ lds_s, const Lane& ln) {
+  const uint32_t op = w.x & 255u;
+  const double x = fetch<false>(w.y & 7u, (w.y >> 3) & 8191u, R, X, Cv, acc, ldc, ldc_ld, lds_s, ln);
+  double r;
+  if (op == D_MUL || op == D_ADD || op == D_SUB) {
+    const double y = fetch<false>((w.y >> 16) & 7u, w.y >> 19, R, X, Cv, acc, ldc, ldc_ld, lds_s, ln);
+    if (op == D_MUL) r = mm(x, y, ln.p, ln.pinv);
+    else r = (op == D_ADD) ? x + y : x - y;
+  } else {
+    switch (op) {
+      case D_NEG: r = -x; break;
+      case D_SQR: r = mm(x, x, ln.p, ln.pinv); break;
+      case D_CUB: r = mm(mm(x, x, ln.p, ln.pinv), x, ln.p, ln.pinv); break;
+      case D_INV: r = inv_fermat(x, ln.p, ln.pinv, ln.pe); break;
+      case D_RED: r = red(x, ln.p, ln.pinv); break;
+      default: r = x; break;  // D_MOV
+    }
+  }
+  const uint32_t db = (w.x >> 8) & 3u, d = w.x >> 10;
+  if (db == DB_R) R[d & 15u] = r;
+  else if (db == DB_E) E[d & (sizeof(EV) / sizeof(double) - 1)] = r;
+  else if (db == DB_S) lds_s[d * ln.T + ln.tid] = r;
+  acc = r;
+}
+
+// The packed program over residues of one prime per lane.
+template <typename EV>
+__device__ __forceinline__ void run_program(const uint2* __restrict__ code, uint32_t n, v16d& R, EV& E,
+                                            const v16d& X, const v16d& Cv, const double* __restrict__ ldc,
+                                            uint32_t ldc_ld, double* __restrict__ lds_s, const Lane& ln) {
   double acc = 0.0;
 #pragma unroll 1
-  for (uint32_t i = 0; i < n; ++i) {
-    const uint2 w = code[i];
-    const uint32_t op = w.x & 255u;
-    const double x = fetch<FP>(w.y & 7u, (w.y >> 3) & 8191u, R, X, Cv, acc, ldc, ldc_ld, lds_s, ln);
-    double r;
-    if (op == D_MUL || op == D_ADD || op == D_SUB) {
-      const double y = fetch<FP>((w.y >> 16) & 7u, w.y >> 19, R, X, Cv, acc, ldc, ldc_ld, lds_s, ln);
-      if (op == D_MUL) r = FP ? x * y : mm(x, y, ln.p, ln.pinv);
-      else r = (op == D_ADD) ? x + y : x - y;
-    } else {
-      switch (op) {
-        case D_NEG: r = -x; break;
-        case D_SQR: r = FP ? x * x : mm(x, x, ln.p, ln.pinv); break;
-        case D_CUB: r = FP ? cube_cr(x) : mm(mm(x, x, ln.p, ln.pinv), x, ln.p, ln.pinv); break;
-        case D_INV: r = FP ? finv(x) : inv_fermat(x, ln.p, ln.pinv, ln.pe); break;
-        case D_RED: r = FP ? x : red(x, ln.p, ln.pinv); break;
-        default: r = x; break;  // D_MOV
-      }
-    }
-    const uint32_t db = (w.x >> 8) & 3u, d = w.x >> 10;
-    if (db == DB_R) R[d & 15u] = r;
-    else if (db == DB_E) E[d & (sizeof(EV) / sizeof(double) - 1)] = r;
-    else if (db == DB_S) lds_s[d * ln.T + ln.tid] = r;
-    acc = r;
+  for (uint32_t pc = 0; pc < n; pc += kCodeFetch) {
+    uint2 w[kCodeFetch];
+#pragma unroll
+    for (int j = 0; j < kCodeFetch; ++j) w[j] = code[pc + j];
+#pragma unroll
+    for (int j = 0; j < kCodeFetch; ++j)
+      if (pc + j < n) exec_res(w[j], acc, R, E, X, Cv, ldc, ldc_ld, lds_s, ln);
   }
 }
 
@@ -279,7 +293,7 @@ __global__ void __launch_bounds__(256) k_walk_chunk(const KArgs a) {
   for (int e = 0; e < M * M; ++e) P[e] = (e % (M + 1) == 0) ? 1.0 : 0.0;
   for (int64_t s = S.sb; s < S.se; ++s) {
     X[kStepSlot] = (double)s;
-    run_program<false>(S.code, S.pg.ncode, R, E, X, Cv, S.ldc, S.KP, S.lds_s, S.ln);
+    run_program(S.code, S.pg.ncode, R, E, X, Cv, S.ldc, S.KP, S.lds_s, S.ln);
     step_matmul<M>(P, E, S.pg.kinds, S.pg.colred, p, pinv);
     advance_axes(X, S.pg.dim, p, pinv);
   }
@@ -296,15 +310,20 @@ __global__ void __launch_bounds__(256) k_walk_chunk(const KArgs a) {
 // with Delta^j f(sb) = sum_t C(sb, t) N_{j+t} and runs its steps with no
 // interpreter at all.
 
-// One workgroup of KP lanes per walk (span); lane k = prime k.
+// One workgroup per walk (span): lane (g, k) = point group g, prime k, with
+// KP = `kp` lanes per group. Phase 1 interprets the program at steps
+// g, g + J, ... (J = blockDim / KP) and stores the raw entry values in the
+// walk's table; phase 2 turns each prime's D + 1 values into Newton forward
+// differences in place.
 template <int M>
-__global__ void __launch_bounds__(256) k_fd_seed(const KArgs a) {
+__global__ void __launch_bounds__(512) k_fd_seed(const KArgs a, uint32_t kp) {
   constexpr int D = fd_dmax(M);
   extern __shared__ double lds[];
   const DevSpan sp = a.spans[blockIdx.x];
   const DevWalk w = a.walks[sp.walk];
   const DevProgram pg = a.progs[w.prog];
-  const uint32_t K = (uint32_t)w.K, KP = blockDim.x, k = threadIdx.x;
+  const uint32_t K = (uint32_t)w.K, KP = kp, J = blockDim.x / kp;
+  const uint32_t k = threadIdx.x % KP, g = threadIdx.x / KP;
   const bool active = k < K;
   const uint32_t kk = active ? k : 0u;
   Lane ln;
@@ -317,33 +336,34 @@ __global__ void __launch_bounds__(256) k_fd_seed(const KArgs a) {
   const ProgLds L = prog_lds(a, pg, K, KP, lds, true);
   __syncthreads();
   const uint2* code = reinterpret_cast<const uint2*>(a.code + pg.code_off);
-  v16d R = 0.0, X = 0.0, Cv = 0.0;
-  typename EBank<M>::type E = 0.0;
-  load_rns_inputs(a, w, pg, K, kk, ln, 0, X, Cv);
-  // all D + 1 points: the differences above an entry's degree then come out
-  // as exact zeros, which the full-depth kernel relies on
-  double N[M * M][D + 1];
-#pragma unroll
-  for (int j = 0; j <= D; ++j) {
+  double* tab = a.fdt + w.fdt_off;
+  for (uint32_t j = g; j <= (uint32_t)D; j += J) {
+    v16d R = 0.0, X = 0.0, Cv = 0.0;
+    typename EBank<M>::type E = 0.0;
+    load_rns_inputs(a, w, pg, K, kk, ln, (int64_t)j, X, Cv);
     X[kStepSlot] = (double)j;
-    run_program<false>(code, pg.ncode, R, E, X, Cv, L.ldc, KP, L.lds_s, ln);
-    advance_axes(X, pg.dim, ln.p, ln.pinv);
+    run_program(code, pg.ncode, R, E, X, Cv, L.ldc, KP, L.lds_s, ln);
+    if (active) {
 #pragma unroll
-    for (int e = 0; e < M * M; ++e) N[e][j] = E[e];
+      for (int e = 0; e < M * M; ++e) tab[((uint64_t)e * (D + 1) + j) * K + k] = E[e];
+    }
   }
-  // values at 0..D -> Newton forward differences at 0
+  __syncthreads();
+  // all D + 1 points: the differences above an entry's degree come out as
+  // exact zeros, which the full-depth kernel relies on
+  if (g == 0 && active) {
 #pragma unroll
-  for (int i = 1; i <= D; ++i)
+    for (int e = 0; e < M * M; ++e) {
+      double N[D + 1];
 #pragma unroll
-    for (int j = D; j >= i; --j)
+      for (int j = 0; j <= D; ++j) N[j] = tab[((uint64_t)e * (D + 1) + j) * K + k];
 #pragma unroll
-      for (int e = 0; e < M * M; ++e) N[e][j] = N[e][j] - N[e][j - 1];
-  if (active) {
-    double* out = a.fdt + w.fdt_off;
+      for (int i = 1; i <= D; ++i)
 #pragma unroll
-    for (int e = 0; e < M * M; ++e)
+        for (int j = D; j >= i; --j) N[j] = N[j] - N[j - 1];
 #pragma unroll
-      for (int j = 0; j <= D; ++j) out[((uint64_t)e * (D + 1) + j) * K + k] = red(N[e][j], ln.p, ln.pinv);
+      for (int j = 0; j <= D; ++j) tab[((uint64_t)e * (D + 1) + j) * K + k] = red(N[j], ln.p, ln.pinv);
+    }
   }
 }
 
@@ -387,8 +407,9 @@ __device__ __forceinline__ void fd_reduce(double (&N)[D + 1], double p, double p
   for (int i = 0; i < D; ++i) N[i] = red(N[i], p, pinv);  // N[D] is constant and already reduced
 }
 
-// Companion PCF M(n) = [[0, b(n)], [1, a(n)]] with deg b = DB, deg a = DA:
-// one step maps row (x, y) to (y, x b + y a). Entries of P stay below 2^33.
+// Companion PCF M(n) = [[0, b(n)], [1, a(n)]] with deg b = DB, deg a = DA
+// (a static 0 or 1 entry is a degree-0 table): one step maps row (x, y) to
+// (y, x b + y a). Entries of P stay below 2^33.
 template <int DB, int DA>
 __device__ __forceinline__ void pcf_fd_body(const KArgs& a, const WalkSetup& S) {
   constexpr int U = fd_unroll(DB > DA ? DB : DA);
@@ -397,10 +418,11 @@ __device__ __forceinline__ void pcf_fd_body(const KArgs& a, const WalkSetup& S) 
   const double* tb = a.fdt + S.w.fdt_off + (uint64_t)1 * (D + 1) * S.K + S.kk;  // entry (0, 1) = b
   const double* ta = a.fdt + S.w.fdt_off + (uint64_t)3 * (D + 1) * S.K + S.kk;  // entry (1, 1) = a
   double Bt[DB + 1], At[DA + 1];
+  const uint32_t kb = (S.pg.kinds >> 2) & 3u, ka = (S.pg.kinds >> 6) & 3u;  // uniform
 #pragma unroll
-  for (int j = 0; j <= DB; ++j) Bt[j] = tb[(uint64_t)j * S.K];
+  for (int j = 0; j <= DB; ++j) Bt[j] = (kb == E_GEN) ? tb[(uint64_t)j * S.K] : (j == 0 && kb == E_ONE ? 1.0 : 0.0);
 #pragma unroll
-  for (int j = 0; j <= DA; ++j) At[j] = ta[(uint64_t)j * S.K];
+  for (int j = 0; j <= DA; ++j) At[j] = (ka == E_GEN) ? ta[(uint64_t)j * S.K] : (j == 0 && ka == E_ONE ? 1.0 : 0.0);
   fd_jump<DB>(Bt, S.sb, a, S.kk, p, pinv);
   fd_jump<DA>(At, S.sb, a, S.kk, p, pinv);
   double x0 = 1.0, y0 = 0.0, x1 = 0.0, y1 = 1.0;
@@ -517,10 +539,73 @@ __global__ void k_combine(const KArgs a) {
 }
 
 // ------------------------------------------------------- FP64 M_f(s) pass
-// The program in FP64 for a block of consecutive steps of one task, one lane
-// per step (runner.py:263-316 / cmf_walk.py:261-313): M_f(s) into the
-// shadow pool. Every lane runs the program (lanes past the range repeat the
-// last step) so the interpreter never diverges.
+// The program in FP64 (runner.py:263-316 / cmf_walk.py:261-313), one step
+// per lane; all lanes of a block share the walk, so decode is wave-uniform.
+// Constants (banks C and L) are read from an LDS copy of the FP64 constant
+// table: a uniform constant bank would sit in SGPRs and be copied to VGPRs on
+// every dynamically indexed read.
+__device__ __forceinline__ double fetch_fp(uint32_t bank, uint32_t i, const v16d& R, const v16d& X, double acc,
+                                           const double* __restrict__ ldc, const double* __restrict__ lds_s,
+                                           uint32_t tid, uint32_t T) {
+  switch (bank) {
+    case SB_R: return R[i & 15u];
+    case SB_X: return X[i & 15u];
+    case SB_C: return ldc[i & 15u];
+    case SB_L: return ldc[i];
+    case SB_S: return lds_s[(uint64_t)i * T + tid];
+    case SB_A: return acc;
+    default: return 0.0;
+  }
+}
+
+template <typename EV>
+__device__ __forceinline__ void exec_fp(uint2 w, double& acc, v16d& R, EV& E, const v16d& X,
+                                        const double* __restrict__ ldc, double* __restrict__ lds_s, uint32_t tid,
+                                        uint32_t T) {
+  {
+    const uint32_t op = w.x & 255u;
+    const double x = fetch_fp(w.y & 7u, (w.y >> 3) & 8191u, R, X, acc, ldc, lds_s, tid, T);
+    double r;
+    if (op == D_MUL || op == D_ADD || op == D_SUB) {
+      const double y = fetch_fp((w.y >> 16) & 7u, w.y >> 19, R, X, acc, ldc, lds_s, tid, T);
+      r = (op == D_MUL) ? x * y : ((op == D_ADD) ? x + y : x - y);
+    } else {
+      switch (op) {
+        case D_NEG: r = -x; break;
+        case D_SQR: r = x * x; break;
+        case D_CUB: r = cube_cr(x); break;
+        case D_INV: r = finv(x); break;
+        default: r = x; break;  // D_RED (no-op in FP64), D_MOV
+      }
+    }
+    const uint32_t db = (w.x >> 8) & 3u, d = w.x >> 10;
+    if (db == DB_R) R[d & 15u] = r;
+    else if (db == DB_E) E[d & (sizeof(EV) / sizeof(double) - 1)] = r;
+    else if (db == DB_S) lds_s[(uint64_t)d * T + tid] = r;
+    acc = r;
+  }
+}
+
+template <typename EV>
+__device__ __forceinline__ void run_program_fp(const uint2* __restrict__ code, uint32_t n, v16d& R, EV& E,
+                                               const v16d& X, const double* __restrict__ ldc,
+                                               double* __restrict__ lds_s, uint32_t tid, uint32_t T) {
+  double acc = 0.0;
+#pragma unroll 1
+  for (uint32_t pc = 0; pc < n; pc += kCodeFetch) {
+    uint2 w[kCodeFetch];
+#pragma unroll
+    for (int j = 0; j < kCodeFetch; ++j) w[j] = code[pc + j];
+#pragma unroll
+    for (int j = 0; j < kCodeFetch; ++j)
+      if (pc + j < n) exec_fp(w[j], acc, R, E, X, ldc, lds_s, tid, T);
+  }
+}
+
+// One block per range of kMfBlock consecutive steps of one task, one lane per
+// step; lanes past the range repeat its last step so the interpreter never
+// diverges. M_f(s) is stored column-major: the shadow lane of column j reads
+// M_f[0..M)[j] in one piece.
 template <int M>
 __global__ void __launch_bounds__(256) k_mf(const KArgs a) {
   extern __shared__ double lds[];
@@ -528,39 +613,31 @@ __global__ void __launch_bounds__(256) k_mf(const KArgs a) {
   const DevTask t = a.tasks[r.task];
   const DevWalk w = a.walks[t.walk];
   const DevProgram pg = a.progs[w.prog];
-  const ProgLds L = prog_lds(a, pg, 0, 0, lds, false);
+  const uint32_t T = blockDim.x, tid = threadIdx.x;
+  double* ldcf = lds;
+  double* lds_s = ldcf + pg.nconst;
+  for (uint32_t i = tid; i < pg.nconst; i += T) ldcf[i] = a.cflt[pg.cflt_off + i];
   __syncthreads();
-  Lane ln;
-  ln.p = ln.pinv = 0.0;
-  ln.pe = 0;
-  ln.k = 0;
-  ln.tid = threadIdx.x;
-  ln.T = blockDim.x;
   const uint2* code = reinterpret_cast<const uint2*>(a.code + pg.code_off);
   typename EBank<M>::type E = 0.0;
-  v16d R = 0.0, X = 0.0, Cv = 0.0;
-  if (!pg.const_lds) {
-#pragma unroll
-    for (int i = 0; i < 16; ++i)
-      if ((uint32_t)i < pg.nconst) Cv[i] = a.cflt[pg.cflt_off + i];
-  }
-  const int64_t s_want = r.s_begin + threadIdx.x;
-  const int64_t s = s_want < r.s_end ? s_want : r.s_end - 1;
+  v16d R = 0.0, X = 0.0;
+  const int64_t want = r.s_begin + tid;
+  const int64_t s = want < r.s_end ? want : r.s_end - 1;
 #pragma unroll
   for (int ax = 0; ax < kMaxDim; ++ax) {
     if ((uint32_t)ax < pg.dim) {
-      const int64_t* q = a.axi + w.axi_off + 2 * ax;
-      X[ax] = (double)(q[0] + s * q[1]) / a.axd[w.axd_off + ax];
+      const int64_t* c = a.axi + w.axi_off + 2 * ax;
+      X[ax] = (double)(c[0] + s * c[1]) / a.axd[w.axd_off + ax];
     }
   }
   X[kStepSlot] = (double)s;
-  run_program<true>(code, pg.ncode, R, E, X, Cv, L.ldcf, 0, L.lds_s, ln);
-  if (s_want < r.s_end) {
+  run_program_fp(code, pg.ncode, R, E, X, ldcf, lds_s, tid, T);
+  if (want < r.s_end) {
     double* out = a.mf + t.mf_off + (uint64_t)(s - t.s_begin) * M * M;
 #pragma unroll
     for (int e = 0; e < M * M; ++e) {
       const uint32_t kd = (pg.kinds >> (2 * e)) & 3u;
-      out[e] = (kd == E_GEN) ? E[e] : (kd == E_ONE ? 1.0 : 0.0);
+      out[(e % M) * M + e / M] = (kd == E_GEN) ? E[e] : (kd == E_ONE ? 1.0 : 0.0);
     }
   }
 }
@@ -604,11 +681,82 @@ __device__ __forceinline__ double group_max(double v) {
   return v;
 }
 
-constexpr int kShadowPf = 4;  // M_f loads issued this many steps ahead
-
 template <int M>
-__global__ void __launch_bounds__(256) k_shadow(const KArgs a, int nspan, int finalize) {
+__device__ __forceinline__ int group_or(int v) {
+  if (M == 2) {
+    v |= __builtin_amdgcn_mov_dpp(v, kQuadX1, 0xF, 0xF, false);
+    return v | __builtin_amdgcn_mov_dpp(v, kQuadX2, 0xF, 0xF, false);
+  }
+#pragma unroll
+  for (int d = 1; d < 16; d <<= 1) v |= __shfl_xor(v, d, 64);
+  return v;
+}
+
+// Shadow state of one lane: its entry of P_f, and log_scale as the mantissa
+// product `mant` times 2^ex (renormalised once per tile; the scaling by a
+// power of two is exact, so the product's roundings are those of mant * m1).
+struct ShadowLane {
+  double Pm, mant;
+  int ex;
+};
+
+template <int M, bool MASK>
+__device__ __forceinline__ void shadow_step(ShadowLane& L, const double (&col)[M], int gbase, int ei, bool ent,
+                                            bool in) {
+  double acc = group_row<M>(L.Pm, gbase, ei, 0) * col[0];
+#pragma unroll
+  for (int q = 1; q < M; ++q) acc = __builtin_fma(group_row<M>(L.Pm, gbase, ei, q), col[q], acc);
+  const double mx = group_max<M>(ent ? __builtin_fabs(acc) : 0.0);
+  const int anynan = group_or<M>((ent && acc != acc) ? 1 : 0);
+  const bool resc = anynan == 0 && mx > 1e10;
+  const double np = ent ? (resc ? acc / mx : acc) : 0.0;
+  int e1 = 0;
+  const double m1 = __builtin_frexp(mx, &e1);
+  const bool take = MASK ? (resc && in) : resc;
+  L.Pm = MASK ? (in ? np : L.Pm) : np;
+  L.mant *= take ? m1 : 1.0;
+  L.ex += take ? e1 : 0;
+}
+
+constexpr int kShadowTile = 16;  // M_f steps per register tile
+
+// M_f column of lane column j for steps s0 .. s0 + T (clamped to `last`)
+template <int M>
+__device__ __forceinline__ void shadow_tile(double (&t)[kShadowTile][M], const double* __restrict__ mf, int64_t s0,
+                                            int64_t last, int ej) {
+#pragma unroll
+  for (int u = 0; u < kShadowTile; ++u) {
+    const int64_t sl = s0 + u < last ? s0 + u : last;
+    const double* c = mf + (uint64_t)sl * M * M + (uint64_t)ej * M;
+    if (M == 2) {
+      const double2 v = *reinterpret_cast<const double2*>(c);
+      t[u][0] = v.x;
+      t[u][M - 1] = v.y;
+    } else {
+#pragma unroll
+      for (int q = 0; q < M; ++q) t[u][q] = c[q];
+    }
+  }
+}
+
+template <int M, bool MASK>
+__device__ __forceinline__ void shadow_run_tile(ShadowLane& L, const double (&t)[kShadowTile][M], int64_t s0,
+                                                int64_t nsteps, int gbase, int ei, bool ent) {
+#pragma unroll
+  for (int u = 0; u < kShadowTile; ++u) {
+    double col[M];
+#pragma unroll
+    for (int q = 0; q < M; ++q) col[q] = t[u][q];
+    shadow_step<M, MASK>(L, col, gbase, ei, ent, s0 + u < nsteps);
+  }
+}
+
+// Steps [w0, w1) of every walk (counted from its first task), continuing
+// from the state of the previous window (the initial state when w0 == 0).
+template <int M>
+__global__ void __launch_bounds__(256) k_shadow(const KArgs a, int nspan, int finalize, int64_t w0, int64_t w1) {
   constexpr int G = (M == 2) ? 4 : 16;
+  constexpr int T = kShadowTile;
   const int lg = threadIdx.x % G;
   const int gid = (int)((blockIdx.x * blockDim.x + threadIdx.x) / G);
   const bool live = gid < nspan;
@@ -617,57 +765,49 @@ __global__ void __launch_bounds__(256) k_shadow(const KArgs a, int nspan, int fi
   const bool ent = lg < M * M;
   const int ei = ent ? lg / M : 0, ej = ent ? lg % M : 0;
   const int gbase = (int)(threadIdx.x & 63) - lg;
-  const double* st_in = a.fst_in + w.fst_off;
-  double Pm = ent ? st_in[lg] : 0.0;
-  double mant = st_in[M * M], ex = st_in[M * M + 1];
+  const double* st_in = (w0 == 0 ? a.fst_in : a.fst) + w.fst_off;
+  ShadowLane L;
+  L.Pm = ent ? st_in[lg] : 0.0;
+  {
+    const double m0 = st_in[M * M];
+    int e0 = 0;
+    L.mant = __builtin_frexp(m0, &e0);
+    L.ex = e0 + (int)st_in[M * M + 1];
+  }
   int64_t nsteps = 0;
   const double* mf = a.mf;
   if (sp.task_end > sp.task_begin) {
     const DevTask t0 = a.tasks[sp.task_begin], t1 = a.tasks[sp.task_end - 1];
-    mf = a.mf + t0.mf_off;
-    nsteps = t1.s_end - t0.s_begin;  // the walk's tasks are contiguous in the M_f pool
-  }
-  const int64_t last = nsteps > 0 ? nsteps - 1 : 0;
-  double ring[kShadowPf][M];
-#pragma unroll
-  for (int u = 0; u < kShadowPf; ++u) {
-    const int64_t sl = u < last ? u : last;
-#pragma unroll
-    for (int q = 0; q < M; ++q) ring[u][q] = mf[(uint64_t)sl * M * M + q * M + ej];
-  }
-  for (int64_t s = 0; s < nsteps; s += kShadowPf) {
-#pragma unroll
-    for (int u = 0; u < kShadowPf; ++u) {
-      double cur[M];
-#pragma unroll
-      for (int q = 0; q < M; ++q) cur[q] = ring[u][q];
-      {
-        const int64_t sn = s + u + kShadowPf;
-        const int64_t sl = sn < last ? sn : last;
-#pragma unroll
-        for (int q = 0; q < M; ++q) ring[u][q] = mf[(uint64_t)sl * M * M + q * M + ej];
-      }
-      double acc = group_row<M>(Pm, gbase, ei, 0) * cur[0];
-#pragma unroll
-      for (int q = 1; q < M; ++q) acc = __builtin_fma(group_row<M>(Pm, gbase, ei, q), cur[q], acc);
-      const double v = ent ? __builtin_fabs(acc) : 0.0;
-      const double nanv = (ent && acc != acc) ? 1.0 : 0.0;
-      const double mx = group_max<M>(v);
-      const double anynan = group_max<M>(nanv);
-      const bool resc = anynan == 0.0 && mx > 1e10;
-      const bool in = s + u < nsteps;
-      const double np = resc ? acc / mx : acc;
-      Pm = in ? (ent ? np : 0.0) : Pm;
-      int e1 = 0;
-      const double m1 = __builtin_frexp(mx, &e1);
-      const bool take = in && resc;
-      double nm = mant * (take ? m1 : 1.0);
-      int e2 = 0;
-      nm = __builtin_frexp(nm, &e2);
-      mant = nm;
-      ex += (take ? (double)e1 : 0.0) + (double)e2;
+    const int64_t total = t1.s_end - t0.s_begin;  // the walk's tasks are contiguous in the M_f pool
+    const int64_t end = total < w1 ? total : w1;
+    if (end > w0) {
+      mf = a.mf + t0.mf_off + (uint64_t)w0 * M * M;
+      nsteps = end - w0;
     }
   }
+  const int64_t last = nsteps > 0 ? nsteps - 1 : 0;
+  // two register tiles: the loads of one are in flight while the other is used
+  double ta[T][M], tb[T][M];
+  shadow_tile<M>(ta, mf, 0, last, ej);
+  int64_t s = 0;
+  for (; s + 2 * T <= nsteps; s += 2 * T) {
+    shadow_tile<M>(tb, mf, s + T, last, ej);
+    shadow_run_tile<M, false>(L, ta, s, nsteps, gbase, ei, ent);
+    shadow_tile<M>(ta, mf, s + 2 * T, last, ej);
+    shadow_run_tile<M, false>(L, tb, s + T, nsteps, gbase, ei, ent);
+    int e2 = 0;
+    L.mant = __builtin_frexp(L.mant, &e2);
+    L.ex += e2;
+  }
+  if (s < nsteps) {  // last partial pair of tiles, step by step masked
+    shadow_tile<M>(tb, mf, s + T, last, ej);
+    shadow_run_tile<M, true>(L, ta, s, nsteps, gbase, ei, ent);
+    shadow_run_tile<M, true>(L, tb, s + T, nsteps, gbase, ei, ent);
+  }
+  int e3 = 0;
+  L.mant = __builtin_frexp(L.mant, &e3);
+  L.ex += e3;
+  const double Pm = L.Pm, mant = L.mant, ex = (double)L.ex;
   double* st = a.fst + w.fst_off;
   if (live && ent) st[lg] = Pm;
   if (live && lg == 0) {
@@ -860,10 +1000,14 @@ __global__ void __launch_bounds__(256) k_garner_wave(const GarnerArgs g) {
       }
     }
   }
+  // the row's limbs, zeros up to the row stride (so no clearing pass is needed)
   uint32_t* limb = g.limbs + (uint64_t)row * g.res_stride;
 #pragma unroll
-  for (int jj = 0; jj < J; ++jj)
-    if (jj * 64 + lane < K) limb[jj * 64 + lane] = L[jj];
+  for (int jj = 0; jj < J; ++jj) {
+    const uint64_t t = (uint64_t)jj * 64 + lane;
+    if (t < g.res_stride) limb[t] = (t < (uint64_t)K) ? L[jj] : 0u;
+  }
+  for (uint64_t t = (uint64_t)J * 64 + lane; t < g.res_stride; t += 64) limb[t] = 0u;
   if (lane == 0) g.neg[row] = (uint8_t)neg;
 }
 
@@ -885,20 +1029,20 @@ __device__ __forceinline__ bool better(double dj, int j, double di, int i) {
   return j < i;
 }
 
-// Tournament round: each block sorts kTopB candidates in LDS (bitonic, in
-// the order above) and keeps its first kk. Padding candidates carry key 0 and
+// Tournament round: each block sorts B candidates in LDS (bitonic, in the
+// order above) and keeps its first kk. Padding candidates carry key 0 and
 // index 0xFFFFFFFF, after every real one. The last round (one block) writes
 // the winners' indices.
-constexpr int kTopB = 2048;
+constexpr int kTopB = 2048;  // largest block
 
-template <bool FROM_VALUES>
+template <int B, bool FROM_VALUES>
 __global__ void __launch_bounds__(256) k_topk_round(const double* __restrict__ d, int stride,
                                                     const TopCand* __restrict__ cin, int n, int kk,
                                                     TopCand* __restrict__ cout, int32_t* __restrict__ out) {
-  __shared__ uint64_t sk[kTopB];
-  __shared__ uint32_t si[kTopB];
-  const int base = blockIdx.x * kTopB;
-  for (int t = threadIdx.x; t < kTopB; t += 256) {
+  __shared__ uint64_t sk[B];
+  __shared__ uint32_t si[B];
+  const int base = blockIdx.x * B;
+  for (int t = threadIdx.x; t < B; t += blockDim.x) {
     const int i = base + t;
     uint64_t key = 0ull;
     uint32_t idx = 0xFFFFFFFFu;
@@ -915,9 +1059,9 @@ __global__ void __launch_bounds__(256) k_topk_round(const double* __restrict__ d
     si[t] = idx;
   }
   __syncthreads();
-  for (int size = 2; size <= kTopB; size <<= 1) {
+  for (int size = 2; size <= B; size <<= 1) {
     for (int half = size >> 1; half > 0; half >>= 1) {
-      for (int t = threadIdx.x; t < kTopB / 2; t += 256) {
+      for (int t = threadIdx.x; t < B / 2; t += blockDim.x) {
         const int lo = 2 * t - (t & (half - 1));
         const int hi = lo + half;
         const uint64_t ka = sk[lo], kb = sk[hi];
@@ -934,9 +1078,9 @@ __global__ void __launch_bounds__(256) k_topk_round(const double* __restrict__ d
     }
   }
   if (gridDim.x == 1 && out) {
-    for (int t = threadIdx.x; t < kk; t += 256) out[t] = (int32_t)si[t];
+    for (int t = threadIdx.x; t < kk; t += blockDim.x) out[t] = (int32_t)si[t];
   } else {
-    for (int t = threadIdx.x; t < kk; t += 256) {
+    for (int t = threadIdx.x; t < kk; t += blockDim.x) {
       TopCand c;
       c.key = sk[t];
       c.idx = si[t];
@@ -984,10 +1128,11 @@ hipError_t launch_walk(int m, int kind, const KArgs& a, int ntask, int threads, 
   return hipGetLastError();
 }
 
-hipError_t launch_fd_seed(int m, const KArgs& a, int nspan, int threads, size_t lds, hipStream_t s) {
+hipError_t launch_fd_seed(int m, const KArgs& a, int nspan, int kp, int threads, size_t lds, hipStream_t s) {
   if (nspan <= 0) return hipSuccess;
-  if (m == 2) hipLaunchKernelGGL(k_fd_seed<2>, dim3(nspan), dim3(threads), lds, s, a);
-  else if (m == 3) hipLaunchKernelGGL(k_fd_seed<3>, dim3(nspan), dim3(threads), lds, s, a);
+  if (kp <= 0 || threads % kp != 0 || threads > 512) return hipErrorInvalidValue;
+  if (m == 2) hipLaunchKernelGGL(k_fd_seed<2>, dim3(nspan), dim3(threads), lds, s, a, (uint32_t)kp);
+  else if (m == 3) hipLaunchKernelGGL(k_fd_seed<3>, dim3(nspan), dim3(threads), lds, s, a, (uint32_t)kp);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }
@@ -1010,13 +1155,13 @@ hipError_t launch_combine(int m, const KArgs& a, int nspan, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_shadow(int m, const KArgs& a, int nspan, int finalize, hipStream_t s) {
+hipError_t launch_shadow(int m, const KArgs& a, int nspan, int finalize, int64_t w0, int64_t w1, hipStream_t s) {
   if (nspan <= 0) return hipSuccess;
   const int g = (m == 2) ? 4 : 16;
   const int tb = 256, nb = (int)(((int64_t)nspan * g + tb - 1) / tb);
-  if (m == 2) hipLaunchKernelGGL(k_shadow<2>, dim3(nb), dim3(tb), 0, s, a, nspan, finalize);
-  else if (m == 3) hipLaunchKernelGGL(k_shadow<3>, dim3(nb), dim3(tb), 0, s, a, nspan, finalize);
-  else if (m == 4) hipLaunchKernelGGL(k_shadow<4>, dim3(nb), dim3(tb), 0, s, a, nspan, finalize);
+  if (m == 2) hipLaunchKernelGGL(k_shadow<2>, dim3(nb), dim3(tb), 0, s, a, nspan, finalize, w0, w1);
+  else if (m == 3) hipLaunchKernelGGL(k_shadow<3>, dim3(nb), dim3(tb), 0, s, a, nspan, finalize, w0, w1);
+  else if (m == 4) hipLaunchKernelGGL(k_shadow<4>, dim3(nb), dim3(tb), 0, s, a, nspan, finalize, w0, w1);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }
@@ -1054,12 +1199,39 @@ hipError_t launch_copy(const void* src, void* dst, size_t n16, hipStream_t s) {
   return hipGetLastError();
 }
 
-static int topk_keep(int k) { return k < kTopB / 2 ? k : kTopB / 2; }
+// Block of the intermediate rounds: room for 4k candidates (a 4x reduction
+// per round), at least 256.
+static int topk_block(int k) {
+  int b = 256;
+  while (b < 4 * k && b < kTopB) b <<= 1;
+  return b;
+}
 
 size_t topk_workspace_bytes(int n, int k) {
   if (n <= kTopB || k > kTopB / 2) return 0;
-  const size_t nb = ((size_t)n + kTopB - 1) / kTopB;
-  return 2 * nb * (size_t)topk_keep(k) * sizeof(TopCand);
+  const size_t nb = ((size_t)n + topk_block(k) - 1) / topk_block(k);
+  return 2 * nb * (size_t)k * sizeof(TopCand);
+}
+
+template <bool FROM_VALUES>
+static void topk_launch(int B, int nb, const double* d, int stride, const TopCand* cin, int n, int kk, TopCand* cout,
+                        int32_t* out, hipStream_t s) {
+  const dim3 g(nb), t(256);
+  switch (B) {
+    case 64: hipLaunchKernelGGL((k_topk_round<64, FROM_VALUES>), g, t, 0, s, d, stride, cin, n, kk, cout, out); break;
+    case 128: hipLaunchKernelGGL((k_topk_round<128, FROM_VALUES>), g, t, 0, s, d, stride, cin, n, kk, cout, out); break;
+    case 256: hipLaunchKernelGGL((k_topk_round<256, FROM_VALUES>), g, t, 0, s, d, stride, cin, n, kk, cout, out); break;
+    case 512: hipLaunchKernelGGL((k_topk_round<512, FROM_VALUES>), g, t, 0, s, d, stride, cin, n, kk, cout, out); break;
+    case 1024: hipLaunchKernelGGL((k_topk_round<1024, FROM_VALUES>), g, t, 0, s, d, stride, cin, n, kk, cout, out); break;
+    default: hipLaunchKernelGGL((k_topk_round<kTopB, FROM_VALUES>), g, t, 0, s, d, stride, cin, n, kk, cout, out); break;
+  }
+}
+
+// Smallest block of at least `cur` (>= 64) for the last round.
+static int topk_final_block(int cur) {
+  int b = 64;
+  while (b < cur) b <<= 1;
+  return b;
 }
 
 hipError_t launch_topk(const double* d, int stride, int n, int k, int32_t* out, void* ws, hipStream_t s) {
@@ -1069,29 +1241,23 @@ hipError_t launch_topk(const double* d, int stride, int n, int k, int32_t* out, 
     hipLaunchKernelGGL(k_topk_rank, dim3((n + 255) / 256), dim3(256), 0, s, d, stride, n, k, out);
     return hipGetLastError();
   }
-  const int kk = topk_keep(k);
-  int cur = n;
-  int nb = (cur + kTopB - 1) / kTopB;
-  if (nb == 1) {
-    hipLaunchKernelGGL(k_topk_round<true>, dim3(1), dim3(256), 0, s, d, stride, nullptr, cur, k, nullptr, out);
+  if (n <= kTopB) {
+    topk_launch<true>(topk_final_block(n), 1, d, stride, nullptr, n, k, nullptr, out, s);
     return hipGetLastError();
   }
   if (!ws) return hipErrorInvalidValue;
-  TopCand* buf[2] = {(TopCand*)ws, (TopCand*)ws + (size_t)nb * kk};
-  hipLaunchKernelGGL(k_topk_round<true>, dim3(nb), dim3(256), 0, s, d, stride, nullptr, cur, kk, buf[0], nullptr);
-  cur = nb * kk;
-  int src = 0;
-  for (;;) {
-    nb = (cur + kTopB - 1) / kTopB;
-    if (nb == 1) {
-      hipLaunchKernelGGL(k_topk_round<false>, dim3(1), dim3(256), 0, s, nullptr, 0, buf[src], cur, k, nullptr, out);
-      break;
-    }
-    hipLaunchKernelGGL(k_topk_round<false>, dim3(nb), dim3(256), 0, s, nullptr, 0, buf[src], cur, kk, buf[1 - src],
-                       nullptr);
-    cur = nb * kk;
+  const int B = topk_block(k);
+  int nb = (n + B - 1) / B;
+  TopCand* buf[2] = {(TopCand*)ws, (TopCand*)ws + (size_t)nb * k};
+  topk_launch<true>(B, nb, d, stride, nullptr, n, k, buf[0], nullptr, s);
+  int cur = nb * k, src = 0;
+  while (cur > kTopB) {
+    nb = (cur + B - 1) / B;
+    topk_launch<false>(B, nb, nullptr, 0, buf[src], cur, k, buf[1 - src], nullptr, s);
+    cur = nb * k;
     src = 1 - src;
   }
+  topk_launch<false>(topk_final_block(cur), 1, nullptr, 0, buf[src], cur, k, nullptr, out, s);
   return hipGetLastError();
 }
 

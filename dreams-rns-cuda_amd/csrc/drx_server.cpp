@@ -167,6 +167,7 @@ int round_once(drx_server* s, std::vector<int>& finished) {
     t.s_begin = sl.step;
     t.s_end = std::min<int64_t>(sl.w.depth, sl.step + s->cfg.round_steps);
     t.cp_off = (uint64_t)ht.size() * slot_res(s);
+    mf_total = (mf_total + 1) & ~(uint64_t)1;  // 16-byte aligned M_f
     t.mf_off = mf_total;
     mf_total += (uint64_t)(t.s_end - t.s_begin) * r.m * r.m;
     if (stages.empty() || stages.back().m != r.m || stages.back().kind != r.kind) {
@@ -181,17 +182,17 @@ int round_once(drx_server* s, std::vector<int>& finished) {
     Stage& g = stages.back();
     const int threads = std::max<int>(s->cfg.threads > 0 ? s->cfg.threads : 256, (int)pow2_ceil(dw.K));
     g.threads = std::max(g.threads, threads);
-    g.seed_threads = std::max<int>(g.seed_threads, (int)pow2_ceil(dw.K));
+    g.seed_kp = std::max<int>(g.seed_kp, (int)pow2_ceil(dw.K));
     DevSpan sp{};
     sp.walk = r.slot;
     sp.task_begin = (int)ht.size();
     sp.task_end = (int)ht.size() + 1;
     sp.out_row = r.slot;
-    for (int64_t q = t.s_begin; q < t.s_end; q += kMfBlock) {
+    for (int64_t q = t.s_begin; q < t.s_end; q += kMfRange) {
       DevRange rg{};
       rg.task = (int)ht.size();
       rg.s_begin = q;
-      rg.s_end = std::min<int64_t>(t.s_end, q + kMfBlock);
+      rg.s_end = std::min<int64_t>(t.s_end, q + kMfRange);
       hr.push_back(rg);
     }
     ht.push_back(t);
@@ -214,7 +215,8 @@ int round_once(drx_server* s, std::vector<int>& finished) {
       const DevProgram& dp = ctx->progs[dw.prog].dev;
       g.lds = std::max(g.lds, walk_lds_bytes(dp, g.kind, g.m, (uint32_t)dw.K, g.threads));
       g.mf_lds = std::max(g.mf_lds, mf_lds_bytes(dp));
-      if (g.kind != WALK_INTERP) g.seed_lds = std::max(g.seed_lds, seed_lds_bytes(dp, g.seed_threads));
+      if (g.kind != WALK_INTERP)
+        g.seed_lds = std::max(g.seed_lds, seed_lds_bytes(dp, g.seed_kp, fd_seed_threads(g.m, g.seed_kp)));
     }
     if (std::max(g.lds, std::max(g.mf_lds, g.seed_lds)) > 65536)
       return fail(ctx, DRX_ERANGE, "server round needs more than 64 KiB of LDS");
@@ -267,7 +269,9 @@ int round_once(drx_server* s, std::vector<int>& finished) {
   a.fcompact = s->d_fcomp.as<double>();
   a.ffull = nullptr;
   a.out_ld = (uint64_t)s->kslot;
-  rc = launch_stages(ctx, a, stages, s->d_sspans.as<DevSpan>(), 1, false);
+  for (Stage& g : stages) g.win_range = {g.range_begin, g.range_end};  // one window per round
+  rc = launch_stages(ctx, a, stages, s->d_sspans.as<DevSpan>(), 1, false, INT64_MAX, 1);
+  if (!rc) rc = join_stages(ctx, false);
   if (rc) return rc;
   if (!hf.empty()) {
     KArgs fa = a;
@@ -399,7 +403,7 @@ int drx_server_create(drx_ctx* ctx, const drx_server_cfg* cfg, drx_server** out)
   const int64_t per_slot =
       (int64_t)8 * (slot_res(s) * 2 + kFstMax + slot_axr(s) + slot_fdt(s) + kMaxDim * 3 + 6 + 2 * s->kslot) +
       (int64_t)8 * 16 * cfg->round_steps + (int64_t)sizeof(DevWalk) + 2 * sizeof(DevTask) + 3 * sizeof(DevSpan) +
-      (int64_t)sizeof(DevRange) * ((cfg->round_steps + kMfBlock - 1) / kMfBlock);
+      (int64_t)sizeof(DevRange) * ((cfg->round_steps + kMfRange - 1) / kMfRange);
   int64_t cap = budget / per_slot;
   if (cfg->max_slots > 0) cap = std::min<int64_t>(cap, cfg->max_slots);
   if (cap < 1) {

@@ -9,6 +9,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 import threading
+import weakref
 from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
@@ -97,6 +98,8 @@ SIGNATURES = {
     "drx_crt": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "drx_topk": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     "drx_copy_calibrate": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p]),
+    "drx_host_alloc": (C.c_int, [C.c_uint64, C.c_void_p]),
+    "drx_host_free": (None, [C.c_void_p]),
     "drx_last_timing": (C.c_int, [C.c_void_p, C.c_void_p]),
     "drx_server_create": (C.c_int, [C.c_void_p, C.POINTER(ServerCfg), C.POINTER(C.c_void_p)]),
     "drx_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Walk), C.c_void_p, C.POINTER(C.c_uint64)]),
@@ -266,6 +269,22 @@ def device_count() -> int:
     n = C.c_int(0)
     lib.drx_device_count(C.byref(n))
     return n.value
+
+
+def pinned_zeros(shape, dtype) -> np.ndarray:
+    """Zero-filled array in page-locked host memory (freed with the array)."""
+    lib = load_library()
+    dt = np.dtype(dtype)
+    count = int(np.prod(shape, dtype=np.int64)) if len(shape) else 1
+    nbytes = max(1, count * dt.itemsize)
+    ptr = C.c_void_p()
+    if lib.drx_host_alloc(nbytes, C.byref(ptr)) != 0 or not ptr.value:
+        raise MemoryError(f"page-locked allocation of {nbytes} bytes failed")
+    buf = (C.c_char * nbytes).from_address(ptr.value)
+    weakref.finalize(buf, lib.drx_host_free, ptr.value)
+    arr = np.frombuffer(buf, dtype=np.uint8, count=count * dt.itemsize).view(dt).reshape(shape)
+    arr[...] = 0
+    return arr
 
 
 def pack_program(program):

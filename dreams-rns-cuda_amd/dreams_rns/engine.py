@@ -196,16 +196,18 @@ class BatchPlan:
         self._axis = np.array(axis if axis else [0], dtype=np.int64)
         self._init_res = np.concatenate(init_res) if init_res else np.zeros(1, np.uint32)
         self._init_f = np.array(init_f if init_f else [0.0], dtype=np.float64)
+        # outputs live in page-locked memory: the copies back run at link speed
+        z = N.pinned_zeros
         self.result = BatchResult(
-            residues=np.zeros((n, 2, stride), dtype=np.int64),
-            out=np.zeros(n, dtype=N.OUT_DTYPE),
+            residues=z((n, 2, stride), np.int64),
+            out=z((n,), N.OUT_DTYPE),
             primes=ctx.primes[:kmax].copy(),
             Ks=walks["K"].copy(),
-            limbs=np.zeros((n, 2, stride), dtype=np.uint32) if crt else None,
-            neg=np.zeros((n, 2), dtype=np.uint8) if crt else None,
-            full=np.zeros((n, 16, stride), dtype=np.int64) if full else None,
-            full_f=np.zeros((n, 16), dtype=np.float64) if full else None,
-            topk=np.zeros(top_k, dtype=np.int32) if top_k else None,
+            limbs=z((n, 2, stride), np.uint32) if crt else None,
+            neg=z((n, 2), np.uint8) if crt else None,
+            full=z((n, 16, stride), np.int64) if full else None,
+            full_f=z((n, 16), np.float64) if full else None,
+            topk=z((top_k,), np.int32) if top_k else None,
         )
         r = self.result
         b = N.Batch()

@@ -165,6 +165,12 @@ int drx_crt(drx_ctx* ctx, const int64_t* residues, int n, int K, int stride, uin
  * k_topk_select (src/cuda/persistent_kernel.cu:796-841). */
 int drx_topk(drx_ctx* ctx, const double* values, int n, int k, int32_t* idx);
 
+/* Page-locked host memory for batch outputs (and inputs): device-to-host
+ * copies into it run at link speed with no staging. No counterpart in the
+ * reference (its results are fresh numpy arrays). */
+int drx_host_alloc(uint64_t bytes, void** out);
+void drx_host_free(void* p);
+
 /* Plain device copy of `bytes` (a multiple of 16) between two fresh buffers,
  * `reps` timed launches after one warm-up; *ms = mean device time per copy.
  * No counterpart in the reference: it is the calibration kernel for HBM byte

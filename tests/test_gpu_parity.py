@@ -265,6 +265,10 @@ def test_forward_difference_every_pcf_shape(gpu_ctx):
             p = D.compile_pcf(a, b)
             specs.append(pcf_spec(p, D.pcf_initial_values(a), depths[(db * 7 + da) % len(depths)],
                                   int(rng.choice([8, 31, 32, 64])), int(rng.integers(-3, 4)), target=0.25))
+    # companion PCFs with a static 0 or 1 entry (degree-0 tables)
+    for a, b in (("0", "n**2 + 1"), ("n + 1", "1"), ("1", "0"), ("0", "1"), ("2*n - 3", "-1")):
+        p = D.compile_pcf(a, b)
+        specs.append(pcf_spec(p, D.pcf_initial_values(a), 333, 32, 2, target=0.5))
     # a 2x2 polynomial program that is not a companion PCF (generic kernel)
     cf = lambda mat, m, dim, axes, dirs: D.compile_cmf_from_dict(mat, m, dim, axes, dirs)
     mat = {(0, 0): "x**2 + 3", (0, 1): "2*x - 1", (1, 0): "x**3 - x", (1, 1): "5"}
