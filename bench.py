@@ -152,6 +152,69 @@ def served_run(specs_pool, rate, duration, round_steps, seed=1005):
             "p99_ms": float(np.percentile(lat, 99) * 1e3) if n else None, "wall_s": wall}
 
 
+# ------------------------------------------------------- other workloads
+def other_workload(args):
+    """cmf3: 1024 3x3 CMFs with INV, depth 10000, K = 64; mix4: 16384 uneven
+    walks (2x2 / 3x3 / 4x4, depth 500..50000, K 16..128), top-128. One JSON
+    line in the same form (no CPU or served legs)."""
+    import dreams_rns as D
+    from dreams_rns import synth
+    from dreams_rns.engine import BatchPlan, cmf_spec, pcf_spec
+
+    cf = lambda mat, m, dim, axes, dirs: D.compile_cmf_from_dict(mat, m, dim, axes, dirs)
+    t0 = time.perf_counter()
+    if args.workload == "cmf3":
+        n = args.walks if args.walks != 4096 else 1024
+        cfg = synth.cmf3(cf, n_walks=n)
+        specs = [cmf_spec(w["program"], cfg["depth"], cfg["K"], w["shift"], target=0.25) for w in cfg["walks"]]
+        top_k = 64
+        desc = f"cmf3: {n} 3x3 CMFs (100-250 ops with INV), depth {cfg['depth']}, K={cfg['K']}"
+    else:
+        n = args.walks if args.walks != 4096 else 16384
+        cfg = synth.mix4(cf, n_walks=n)
+        specs = []
+        for w in cfg["walks"]:
+            if w["kind"] == "pcf":
+                p = D.compile_pcf(w["a"], w["b"])
+                specs.append(pcf_spec(p, D.pcf_initial_values(w["a"]), w["depth"], w["K"], 1, target=w["L"]))
+            else:
+                specs.append(cmf_spec(w["program"], w["depth"], w["K"], w["shift"], target=w["L"]))
+        top_k = cfg["top_k"]
+        desc = f"mix4: {n} uneven walks (2x2/3x3/4x4, depth 500..50000, K 16..128)"
+    work = sum(program_work(s.program) * s.depth * s.K for s in specs)
+    rsteps = float(sum(s.depth * s.K for s in specs))
+    log(f"generated {len(specs)} walks in {time.perf_counter() - t0:.1f}s")
+    plan = BatchPlan(specs, crt=True, top_k=top_k, threads=args.threads, chunk_steps=args.chunk,
+                     use_graph=args.graph)
+    ctx = plan.ctx
+    for _ in range(args.warmup):
+        plan.run()
+    step_ms, walk_ms = [], []
+    for i in range(args.steps):
+        t = time.perf_counter()
+        plan.run()
+        step_ms.append((time.perf_counter() - t) * 1e3)
+        tm = ctx.last_timing()
+        walk_ms.append(tm[0])
+        log(f"step {i}: {step_ms[-1]:.2f} ms (walk kernels {tm[0]:.3f} ms, device {tm[4]:.3f})")
+    plan.close()
+    ms = float(np.mean(step_ms))
+    walk_avg = float(np.mean(walk_ms)) if not args.graph else float("nan")
+    achieved = work / (walk_avg * 1e-3) if walk_avg == walk_avg else None
+    print(json.dumps({
+        "metric": METRIC, "value": rsteps / (ms * 1e-3), "unit": "residue-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": desc, "walks": len(specs), "top_k": top_k, "graph": bool(args.graph)},
+        "latency_batch_ms": {"p50": float(np.percentile(step_ms, 50)), "p99": float(np.percentile(step_ms, 99))},
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12 if achieved else None,
+                     "peak": PEAK_LANE_OPS / 1e12, "unit": "TOP/s",
+                     "frac": achieved / PEAK_LANE_OPS if achieved else None, "traffic": None,
+                     "kernel": walk_kernel_name([s.program for s in specs]), "kernel_ms": walk_avg,
+                     "work_per_launch": work},
+    }))
+
+
 # -------------------------------------------------------------------- main
 def main():
     ap = argparse.ArgumentParser()
@@ -169,6 +232,9 @@ def main():
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--graph", action="store_true", help="replay the batch as a hipGraph")
+    ap.add_argument("--workload", choices=("pcf2", "cmf3", "mix4"), default="pcf2",
+                    help="pcf2 (the headline line); cmf3 / mix4 measure the other batch shapes "
+                         "(no CPU or served legs)")
     ap.add_argument("--profile", action="store_true",
                     help="for counter collection: run the batch --steps times and a 512 MiB device copy "
                          "(the calibration kernel for FETCH_SIZE / WRITE_SIZE), print nothing else")
@@ -183,6 +249,8 @@ def main():
     from dreams_rns.engine import BatchPlan, pcf_spec
 
     t0 = time.perf_counter()
+    if args.workload != "pcf2":
+        return other_workload(args)
     cfg = synth.pcf2(n_walks=args.walks)
     depth, K, top_k = cfg["depth"], cfg["K"], cfg["top_k"]
     progs = [D.compile_pcf(w["a"], w["b"]) for w in cfg["walks"]]
