@@ -184,6 +184,14 @@ def other_workload(args):
     work = sum(program_work(s.program) * s.depth * s.K for s in specs)
     rsteps = float(sum(s.depth * s.K for s in specs))
     log(f"generated {len(specs)} walks in {time.perf_counter() - t0:.1f}s")
+    tuned = None
+    if not (args.threads or args.chunk):
+        from dreams_rns.engine import autotune
+
+        t = time.perf_counter()
+        tuned = autotune(specs, budget_s=60.0)
+        args.threads, args.chunk = tuned["threads"], tuned["chunk_steps"]
+        log(f"autotuned in {time.perf_counter() - t:.1f}s: threads {args.threads}, chunk {args.chunk}")
     plan = BatchPlan(specs, crt=True, top_k=top_k, threads=args.threads, chunk_steps=args.chunk,
                      use_graph=args.graph)
     ctx = plan.ctx
@@ -205,7 +213,8 @@ def other_workload(args):
         "metric": METRIC, "value": rsteps / (ms * 1e-3), "unit": "residue-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": desc, "walks": len(specs), "top_k": top_k, "graph": bool(args.graph)},
+        "config": {"workload": desc, "walks": len(specs), "top_k": top_k, "graph": bool(args.graph),
+                   "threads": args.threads, "chunk_steps": args.chunk, "autotuned": tuned is not None},
         "latency_batch_ms": {"p50": float(np.percentile(step_ms, 50)), "p99": float(np.percentile(step_ms, 99))},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12 if achieved else None,
                      "peak": PEAK_LANE_OPS / 1e12, "unit": "TOP/s",
@@ -260,6 +269,14 @@ def main():
     rsteps = float(len(specs) * depth * K)
     log(f"compiled {len(progs)} PCFs in {time.perf_counter() - t0:.1f}s")
 
+    tuned = None
+    if not (args.threads or args.chunk):
+        from dreams_rns.engine import autotune
+
+        t = time.perf_counter()
+        tuned = autotune(specs)
+        args.threads, args.chunk = tuned["threads"], tuned["chunk_steps"]
+        log(f"autotuned in {time.perf_counter() - t:.1f}s: threads {args.threads}, chunk {args.chunk}")
     plan = BatchPlan(specs, crt=True, top_k=top_k, threads=args.threads, chunk_steps=args.chunk,
                      use_graph=args.graph)
     ctx = plan.ctx
@@ -348,7 +365,8 @@ def main():
         "config": {"workload": f"pcf2: {len(specs)} seeded PCFs, deg a<=3, deg b<=6, coef [-20,20], "
                                f"depth {depth}, K={K}, FP64 shadow + delta_float, GPU Garner CRT, top-{top_k}",
                    "walks": len(specs), "depth": depth, "K": K, "top_k": top_k,
-                   "threads": args.threads or 256, "chunk_steps": args.chunk or 4096, "graph": bool(args.graph)},
+                   "threads": args.threads or 256, "chunk_steps": args.chunk or 4096, "graph": bool(args.graph),
+                   "autotuned": tuned is not None},
         "latency_batch_ms": {"p50": float(np.percentile(step_ms, 50)), "p99": float(np.percentile(step_ms, 99))},
         "plain": {"value": plain, "sample_walks": n_plain, "x": value / plain},
         "served": served,

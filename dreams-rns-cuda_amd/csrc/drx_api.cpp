@@ -1,10 +1,11 @@
 // C ABI of libdreams_hip: context, primes, programs, batch plans, CRT, top-k.
 #include <algorithm>
 #include <chrono>
-#include <cstdarg>
 #include <cmath>
+#include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <numeric>
 
@@ -1075,6 +1076,107 @@ int drx_plan_run(drx_plan* pl) {
   tm.total_ms = tot;
   tm.walk_launches = (int)pl->stages.size();
   ctx->timing = tm;
+  return DRX_OK;
+}
+
+namespace {
+
+// The walks of `b` at indices 0, stride, 2 stride, ... as a batch writing
+// into host sinks owned by the caller (residues and the FP64 outputs; no CRT
+// or top-k).
+struct TuneSample {
+  std::vector<drx_walk> walks;
+  std::vector<drx_walk_out> out;
+  std::vector<int64_t> res;
+  drx_batch batch{};
+  TuneSample(const drx_batch* b, int want) {
+    const int stride = (b->n + want - 1) / want;
+    int kmax = 1;
+    for (int i = 0; i < b->n; i += stride) {
+      walks.push_back(b->walks[i]);
+      kmax = std::max(kmax, b->walks[i].K);
+    }
+    out.resize(walks.size());
+    res.resize(walks.size() * 2 * (size_t)kmax);
+    batch.walks = walks.data();
+    batch.n = (int)walks.size();
+    batch.axis_pool = b->axis_pool;
+    batch.init_res_pool = b->init_res_pool;
+    batch.init_f_pool = b->init_f_pool;
+    batch.out = out.data();
+    batch.residues = res.data();
+    batch.res_stride = kmax;
+  }
+};
+
+// Wall time of one plan run (seconds), best of `reps` after a warm-up.
+int time_plan(drx_ctx* ctx, const drx_batch* b, const drx_launch_cfg* c, int reps, double* sec) {
+  drx_plan* pl = nullptr;
+  int rc = drx_plan_create(ctx, b, c, &pl);
+  if (rc) return rc;
+  double best = 1e300;
+  for (int r = 0; r <= reps && rc == 0; ++r) {
+    const double t0 = now_s();
+    rc = drx_plan_run(pl);
+    if (rc == 0 && r > 0) best = std::min(best, now_s() - t0);
+  }
+  drx_plan_destroy(pl);
+  *sec = best;
+  return rc;
+}
+
+}  // namespace
+
+int drx_autotune(drx_ctx* ctx, const drx_batch* b, const drx_launch_cfg* base, int max_walks, double budget_s,
+                 drx_launch_cfg* best, double* best_ms) {
+  if (!ctx || !b || !best || !best_ms || b->n < 1 || !b->walks) return DRX_EINVAL;
+  drx_launch_cfg cfg0{};
+  if (base) cfg0 = *base;
+  cfg0.mode = DRX_MODE_BATCHED;
+  cfg0.use_graph = 0;
+  const int thr_c[] = {64, 128, 256};
+  const int chunk_c[] = {512, 1024, 2048, 4096, 8192, 16384};
+  const int reps = 2, ncand = 18;
+  const double t_start = now_s();
+  // sample size: the whole batch (up to max_walks) unless one run of the
+  // default shape takes more than its share of the budget, then halve
+  int want = max_walks > 0 ? std::min(max_walks, b->n) : b->n;
+  std::unique_ptr<TuneSample> smp;
+  for (;;) {
+    smp.reset(new TuneSample(b, want));
+    double t = 0;
+    int rc = time_plan(ctx, &smp->batch, &cfg0, 1, &t);
+    if (rc) return rc;
+    if (t * (reps + 1) * ncand <= budget_s || want <= 256) break;
+    want /= 2;
+  }
+  int kmax = 1;
+  for (const drx_walk& w : smp->walks) kmax = std::max(kmax, w.K);
+  const int kp = (int)pow2_ceil((uint32_t)kmax);
+  double best_t = 1e300;
+  drx_launch_cfg best_cfg = cfg0;
+  for (int thr : thr_c) {
+    if (thr < kp) continue;  // a workgroup holds at least one sub-chunk of every prime
+    for (int ch : chunk_c) {
+      if (best_t < 1e300 && now_s() - t_start > budget_s) break;
+      drx_launch_cfg c = cfg0;
+      c.threads = thr;
+      c.chunk_steps = ch;
+      double t = 0;
+      const int rc = time_plan(ctx, &smp->batch, &c, reps, &t);
+      if (rc == DRX_ERANGE) continue;  // LDS limit for this shape
+      if (rc) return rc;
+      if (t < best_t) {
+        best_t = t;
+        best_cfg = c;
+      }
+    }
+  }
+  if (best_t >= 1e300) return fail(ctx, DRX_ERANGE, "no launch configuration fits these walks");
+  best_cfg.use_graph = base ? base->use_graph : 0;
+  best_cfg.mode = base ? base->mode : DRX_MODE_BATCHED;
+  *best = best_cfg;
+  *best_ms = best_t * 1e3;
   return DRX_OK;
 }
 

@@ -98,6 +98,7 @@ SIGNATURES = {
     "drx_crt": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "drx_topk": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     "drx_copy_calibrate": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p]),
+    "drx_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_void_p]),
     "drx_host_alloc": (C.c_int, [C.c_uint64, C.c_void_p]),
     "drx_host_free": (None, [C.c_void_p]),
     "drx_last_timing": (C.c_int, [C.c_void_p, C.c_void_p]),

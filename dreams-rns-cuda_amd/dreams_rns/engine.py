@@ -134,6 +134,101 @@ class BatchResult:
         return self.full_f[i, : m * m].reshape(m, m).copy()
 
 
+def _pack_walks(specs: Sequence[WalkSpec], ctx: N.Context):
+    """drx_walk records and the axis / initial-state pools of a batch."""
+    kmax = max(int(s.K) for s in specs)
+    ctx.ensure_primes(kmax)
+    walks = np.zeros(len(specs), dtype=N.WALK_DTYPE)
+    axis: List[int] = []
+    init_res: List[np.ndarray] = []
+    init_f: List[float] = []
+    res_off = 0
+    for i, s in enumerate(specs):
+        w = walks[i]
+        w["prog"] = ctx.program_id(s.program)
+        w["K"] = s.K
+        w["depth"] = s.depth
+        w["target"] = s.target
+        w["axis_off"] = len(axis)
+        for q in s.axes:
+            axis.extend(int(v) for v in q)
+        m = s.program.m
+        if s.init == "pcf" and INT64_MIN <= int(s.a0) <= INT64_MAX:
+            w["init_kind"] = N.DRX_INIT_PCF
+            w["a0"] = int(s.a0)
+        elif s.init == "identity":
+            w["init_kind"] = N.DRX_INIT_IDENTITY
+        else:
+            if s.init == "pcf":  # a0 outside int64: give the residues
+                pr = [int(p) for p in ctx.primes[: s.K]]
+                r = np.zeros((4, s.K), dtype=np.uint32)
+                r[0] = 1
+                r[1] = [int(s.a0) % p for p in pr]
+                r[3] = 1
+                f = np.array([1.0, float(int(s.a0)), 0.0, 1.0])
+            elif s.init == "given":
+                r = np.ascontiguousarray(s.init_res, dtype=np.uint32).reshape(m * m, s.K)
+                f = np.ascontiguousarray(s.init_f, dtype=np.float64).reshape(m * m)
+            else:
+                raise ValueError(f"unknown init kind {s.init!r}")
+            w["init_kind"] = N.DRX_INIT_GIVEN
+            w["init_off"] = res_off
+            w["init_f_off"] = len(init_f)
+            init_res.append(r.reshape(-1))
+            init_f.extend(float(v) for v in f)
+            res_off += r.size
+    for v in axis:
+        if not INT64_MIN <= v <= INT64_MAX:
+            raise OverflowError("shift/trajectory value does not fit in int64")
+    axis_a = np.array(axis if axis else [0], dtype=np.int64)
+    init_res_a = np.concatenate(init_res) if init_res else np.zeros(1, np.uint32)
+    init_f_a = np.array(init_f if init_f else [0.0], dtype=np.float64)
+    return walks, axis_a, init_res_a, init_f_a
+
+
+def batch_signature(specs: Sequence[WalkSpec]) -> tuple:
+    """The shape mix of a batch, for the autotuner's cache: counts of
+    (matrix order, K, forward-difference or not, depth octave), relative to
+    the batch size in 1/16ths."""
+    from collections import Counter
+
+    c = Counter()
+    for s in specs:
+        meta = N.pack_program(s.program)[2]
+        c[(s.program.m, int(s.K), int(meta["fd_ok"]), max(0, int(s.depth)).bit_length())] += 1
+    n = len(specs)
+    return tuple(sorted((k, round(16 * v / n)) for k, v in c.items())) + (("n", n.bit_length()),)
+
+
+_TUNED: Dict[tuple, Dict[str, float]] = {}
+
+
+def autotune(specs: Sequence[WalkSpec], ctx: Optional[N.Context] = None, max_walks: int = 16384,
+             budget_s: float = 20.0, interp_only: bool = False) -> Dict[str, float]:
+    """Workgroup size and chunk length for this batch (drx_autotune: whole
+    pipeline runs on a sample of at most ``max_walks`` walks, shrunk to fit
+    ``budget_s``), cached per batch signature."""
+    ctx = ctx or N.context()
+    key = (batch_signature(specs), bool(interp_only))
+    if key in _TUNED:
+        return _TUNED[key]
+    walks, axis, init_res, init_f = _pack_walks(specs, ctx)
+    b = N.Batch()
+    b.walks = walks.ctypes.data
+    b.n = len(specs)
+    b.axis_pool = axis.ctypes.data
+    b.init_res_pool = init_res.ctypes.data
+    b.init_f_pool = init_f.ctypes.data
+    base = N.LaunchCfg(N.MODE_BATCHED, 0, 0, 0, 1 if interp_only else 0)
+    best = N.LaunchCfg()
+    ms = C.c_double(0.0)
+    ctx._check(ctx.lib.drx_autotune(ctx.h, C.byref(b), C.byref(base), int(max_walks), float(budget_s),
+                                    C.byref(best), C.byref(ms)))
+    out = {"threads": int(best.threads), "chunk_steps": int(best.chunk_steps), "sample_walk_ms": ms.value}
+    _TUNED[key] = out
+    return out
+
+
 class BatchPlan:
     """Packed, device-resident batch; see the module docstring."""
 
@@ -141,61 +236,20 @@ class BatchPlan:
 
     def __init__(self, specs: Sequence[WalkSpec], ctx: Optional[N.Context] = None, mode: str = "batched",
                  threads: int = 0, chunk_steps: int = 0, use_graph: bool = False, crt: bool = False,
-                 full: bool = False, top_k: int = 0, interp_only: bool = False):
+                 full: bool = False, top_k: int = 0, interp_only: bool = False, tune: bool = False):
         self.ctx = ctx or N.context()
         n = len(specs)
         if n == 0:
             raise ValueError("empty batch")
         kmax = max(int(s.K) for s in specs)
-        self.ctx.ensure_primes(kmax)
         ctx = self.ctx
-        stride = kmax
-        walks = np.zeros(n, dtype=N.WALK_DTYPE)
-        axis: List[int] = []
-        init_res: List[np.ndarray] = []
-        init_f: List[float] = []
-        res_off = 0
-        for i, s in enumerate(specs):
-            w = walks[i]
-            w["prog"] = ctx.program_id(s.program)
-            w["K"] = s.K
-            w["depth"] = s.depth
-            w["target"] = s.target
-            w["axis_off"] = len(axis)
-            for q in s.axes:
-                axis.extend(int(v) for v in q)
-            m = s.program.m
-            if s.init == "pcf" and INT64_MIN <= int(s.a0) <= INT64_MAX:
-                w["init_kind"] = N.DRX_INIT_PCF
-                w["a0"] = int(s.a0)
-            elif s.init == "identity":
-                w["init_kind"] = N.DRX_INIT_IDENTITY
-            else:
-                if s.init == "pcf":  # a0 outside int64: give the residues
-                    pr = [int(p) for p in ctx.primes[: s.K]]
-                    r = np.zeros((4, s.K), dtype=np.uint32)
-                    r[0] = 1
-                    r[1] = [int(s.a0) % p for p in pr]
-                    r[3] = 1
-                    f = np.array([1.0, float(int(s.a0)), 0.0, 1.0])
-                elif s.init == "given":
-                    r = np.ascontiguousarray(s.init_res, dtype=np.uint32).reshape(m * m, s.K)
-                    f = np.ascontiguousarray(s.init_f, dtype=np.float64).reshape(m * m)
-                else:
-                    raise ValueError(f"unknown init kind {s.init!r}")
-                w["init_kind"] = N.DRX_INIT_GIVEN
-                w["init_off"] = res_off
-                w["init_f_off"] = len(init_f)
-                init_res.append(r.reshape(-1))
-                init_f.extend(float(v) for v in f)
-                res_off += r.size
-        for v in axis:
-            if not INT64_MIN <= v <= INT64_MAX:
-                raise OverflowError("shift/trajectory value does not fit in int64")
+        walks, self._axis, self._init_res, self._init_f = _pack_walks(specs, ctx)
         self._walks = walks
-        self._axis = np.array(axis if axis else [0], dtype=np.int64)
-        self._init_res = np.concatenate(init_res) if init_res else np.zeros(1, np.uint32)
-        self._init_f = np.array(init_f if init_f else [0.0], dtype=np.float64)
+        stride = kmax
+        if tune and mode != "plain" and not (threads or chunk_steps):
+            t = autotune(specs, ctx=ctx, interp_only=interp_only)
+            threads, chunk_steps = t["threads"], t["chunk_steps"]
+        self.threads, self.chunk_steps = int(threads), int(chunk_steps)
         # outputs live in page-locked memory: the copies back run at link speed
         z = N.pinned_zeros
         self.result = BatchResult(

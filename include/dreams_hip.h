@@ -153,6 +153,20 @@ int drx_plan_run(drx_plan* plan);
 int drx_plan_stats(const drx_plan* plan, double* s4);
 void drx_plan_destroy(drx_plan* plan);
 
+/* Autotuner: picks the workgroup size (threads; with K primes per sub-chunk
+ * this sets the sub-chunks, i.e. step ranges, per workgroup) and the chunk
+ * length (steps per workgroup task, i.e. tasks per walk) for this batch.
+ * Candidates run the whole pipeline on a sample of the walks (every n/s-th
+ * walk, keeping the mix of shapes): the batch itself up to max_walks walks,
+ * halved until one run fits its share of budget_s. Each is timed by the wall
+ * time of a plan run (best of two after a warm-up); candidates stop after
+ * budget_s seconds (at least one is always measured). *best gets base with
+ * threads and chunk_steps set; *best_ms its measured time. No counterpart in the reference: its CUDA
+ * launches use a fixed BLOCK_SIZE of 256 (include/dreams/config.h:95,
+ * src/cuda/persistent_kernel.cu:878-893). */
+int drx_autotune(drx_ctx* ctx, const drx_batch* b, const drx_launch_cfg* base, int max_walks, double budget_s,
+                 drx_launch_cfg* best, double* best_ms);
+
 /* Garner CRT of n rows of K canonical residues (row stride `stride`) over the
  * first K context primes: magnitude limbs (little-endian uint32, K per row)
  * and sign of centered(x, M). Replaces crt_reconstruct + centered

@@ -287,3 +287,24 @@ def test_forward_difference_every_pcf_shape(gpu_ctx):
                     np.array([1.0, float(s.a0), 0.0, 1.0]))
         rr, Pf, ls = cref.walk(s.program, s.depth, primes[: s.K], list(s.axes), *(init or (None, None)))
         assert np.array_equal(a.full_matrix(i, s.program.m), rr), i
+
+
+def test_autotune_choice_is_valid_and_results_unchanged(gpu_ctx):
+    from dreams_rns.engine import autotune
+
+    cfg = synth.pcf2(n_walks=300, seed=55)
+    specs = [pcf_spec(D.compile_pcf(w["a"], w["b"]), D.pcf_initial_values(w["a"]), 1500, 32, 1, target=w["L"])
+             for w in cfg["walks"]]
+    t = autotune(specs, max_walks=128, budget_s=10.0)
+    assert t["threads"] in (64, 128, 256) and t["chunk_steps"] in (512, 1024, 2048, 4096, 8192, 16384)
+    assert t["sample_walk_ms"] > 0
+    assert autotune(specs) is t  # cached by the batch signature
+    with BatchPlan(specs, crt=True, top_k=8) as p0:
+        r0 = p0.run()
+        ref = (r0.residues.copy(), r0.limbs.copy(), r0.out["log_scale"].copy(), r0.topk.copy())
+    with BatchPlan(specs, crt=True, top_k=8, tune=True) as p1:
+        assert (p1.threads, p1.chunk_steps) == (t["threads"], t["chunk_steps"])
+        r1 = p1.run()
+        assert np.array_equal(r1.residues, ref[0]) and np.array_equal(r1.limbs, ref[1])
+        assert [v.hex() for v in r1.out["log_scale"].tolist()] == [v.hex() for v in ref[2].tolist()]
+        assert np.array_equal(r1.topk, ref[3])
