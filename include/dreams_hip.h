@@ -160,8 +160,10 @@ void drx_plan_destroy(drx_plan* plan);
  * walk, keeping the mix of shapes): the batch itself up to max_walks walks,
  * halved until one run fits its share of budget_s. Each is timed by the wall
  * time of a plan run (best of two after a warm-up); candidates stop after
- * budget_s seconds (at least one is always measured). *best gets base with
- * threads and chunk_steps set; *best_ms its measured time. No counterpart in the reference: its CUDA
+ * budget_s seconds (at least one is always measured); the default shape
+ * (256 threads, 4096 steps) is kept unless a candidate beats it by more than
+ * 3%. *best gets base with threads and chunk_steps set; *best_ms its
+ * measured time. No counterpart in the reference: its CUDA
  * launches use a fixed BLOCK_SIZE of 256 (include/dreams/config.h:95,
  * src/cuda/persistent_kernel.cu:878-893). */
 int drx_autotune(drx_ctx* ctx, const drx_batch* b, const drx_launch_cfg* base, int max_walks, double budget_s,
