@@ -111,17 +111,17 @@ size_t walk_lds_bytes(const DevProgram& pg, int kind, int m, uint32_t K, int thr
   const size_t tree = (size_t)threads * m * m;  // [C][m*m][KP] with C * KP = threads
   if (kind != WALK_INTERP) return 8 * tree;
   const size_t KP = pow2_ceil(K);
-  const size_t nl = pg.const_lds ? pg.nconst : 0;
-  return 8 * (nl * KP + nl + (size_t)pg.nslots * threads + tree);
+  const size_t nl = pg.nconst > (uint32_t)kBank ? pg.nconst - kBank : 0;
+  return 8 * std::max(nl * KP + (size_t)pg.nslots * threads, tree);  // the tree overlays constants and temps
 }
 
 size_t seed_lds_bytes(const DevProgram& pg, int kp, int seed_threads) {
-  const size_t nl = pg.const_lds ? pg.nconst : 0;
-  return 8 * (nl * (size_t)kp + nl + (size_t)pg.nslots * seed_threads);
+  const size_t nl = pg.nconst > (uint32_t)kBank ? pg.nconst - kBank : 0;
+  return 8 * (nl * (size_t)kp + (size_t)pg.nslots * seed_threads);
 }
 
 size_t mf_lds_bytes(const DevProgram& pg) {
-  return 8 * ((size_t)pg.nconst + (size_t)pg.nslots * kMfBlock);  // every FP64 constant, then temps
+  return 8 * ((size_t)pg.nconst + (size_t)pg.fnslots * kMfBlock);  // every FP64 constant, then temps
 }
 
 void shadow_windows(int64_t max_steps, int64_t* win, int* nwin) {
@@ -824,9 +824,53 @@ int drx_program_upload(drx_ctx* ctx, const uint32_t* ops, int n_ops, int m, int 
   if (!err.empty()) return fail(ctx, DRX_EINVAL, "%s", err.c_str());
   const PackedProgram& pk = hp.pk;
   const int nc = (int)pk.const_slots.size();
+  // every constant the code may reference: the given ones, then the folded
+  // ones computed from them with the operation the code would have applied
+  // (residues mod each covered prime, FP64 with the same IEEE operation)
+  const int n_all = n_const + (int)pk.derived.size();
+  std::vector<uint32_t> cr((size_t)n_all * kc);
+  std::vector<double> cf(n_all);
+  for (int ci = 0; ci < n_const; ++ci) {
+    for (int k = 0; k < kc; ++k) {
+      const uint32_t p = ctx->primes[k];
+      const uint32_t r = given ? const_res[(size_t)ci * kcap + k] : mod_i64(consts[ci], p);
+      if (r >= p) return fail(ctx, DRX_EINVAL, "constant %d: residue not canonical", ci);
+      cr[(size_t)ci * kc + k] = r;
+    }
+    cf[ci] = given ? const_f[ci] : (double)consts[ci];
+  }
+  for (size_t j = 0; j < pk.derived.size(); ++j) {
+    const DerivedConst& dc = pk.derived[j];
+    const int ci = n_const + (int)j;
+    const double fa = cf[dc.a], fb = dc.b >= 0 ? cf[dc.b] : 0.0;
+    switch (dc.op) {
+      case D_ADD: cf[ci] = fa + fb; break;
+      case D_SUB: cf[ci] = fa - fb; break;
+      case D_MUL: cf[ci] = fa * fb; break;
+      case D_NEG: cf[ci] = -fa; break;
+      case D_SQR: cf[ci] = fa * fa; break;
+      default: cf[ci] = std::fabs(fa) > 1e-300 ? 1.0 / fa : 0.0; break;  // D_INV
+    }
+    for (int k = 0; k < kc; ++k) {
+      const uint64_t p = ctx->primes[k];
+      const uint64_t ra = cr[(size_t)dc.a * kc + k], rb = dc.b >= 0 ? cr[(size_t)dc.b * kc + k] : 0;
+      uint64_t r;
+      switch (dc.op) {
+        case D_ADD: r = (ra + rb) % p; break;
+        case D_SUB: r = (ra + p - rb) % p; break;
+        case D_MUL: r = ra * rb % p; break;
+        case D_NEG: r = (p - ra) % p; break;
+        case D_SQR: r = ra * ra % p; break;
+        default: r = inv_mod((uint32_t)ra, (uint32_t)p); break;  // D_INV, 0 -> 0
+      }
+      cr[(size_t)ci * kc + k] = (uint32_t)r;
+    }
+  }
   DevProgram& d = hp.dev;
   d.code_off = (uint32_t)ctx->h_code.size();
   d.ncode = (uint32_t)(pk.code.size() / 2);
+  d.fcode_off = (uint32_t)(ctx->h_code.size() + pk.code.size());
+  d.nfcode = (uint32_t)(pk.fcode.size() / 2);
   d.nconst = (uint32_t)nc;
   d.kcap = (uint32_t)kc;
   d.cres_off = ctx->h_cres.size();
@@ -837,21 +881,19 @@ int drx_program_upload(drx_ctx* ctx, const uint32_t* ops, int n_ops, int m, int 
   d.colred = pk.colred;
   d.const_lds = (uint32_t)pk.const_lds;
   d.nslots = (uint32_t)pk.nslots;
+  d.fnslots = (uint32_t)pk.fnslots;
+  d.pad = 0;
   d.fd_ok = (uint32_t)pk.fd_ok;
   d.fd_degree = (uint32_t)pk.fd_degree;
   d.fd_deg = pk.fd_deg;
   hp.fd = pk.fd_ok && ((m == 2 && pk.fd_degree <= kFdMax2) || (m == 3 && pk.fd_degree <= kFdMax3));
   ctx->h_code.insert(ctx->h_code.end(), pk.code.begin(), pk.code.end());
+  ctx->h_code.insert(ctx->h_code.end(), pk.fcode.begin(), pk.fcode.end());
   if (ctx->h_code.size() % 2) ctx->h_code.push_back(0);  // keep 8-byte alignment
   for (int s = 0; s < nc; ++s) {
     const int ci = pk.const_slots[s];
-    for (int k = 0; k < kc; ++k) {
-      const uint32_t p = ctx->primes[k];
-      const uint32_t r = given ? const_res[(size_t)ci * kcap + k] : mod_i64(consts[ci], p);
-      if (r >= p) return fail(ctx, DRX_EINVAL, "constant %d: residue not canonical", ci);
-      ctx->h_cres.push_back(balanced(r, p));
-    }
-    ctx->h_cflt.push_back(given ? const_f[ci] : (double)consts[ci]);
+    for (int k = 0; k < kc; ++k) ctx->h_cres.push_back(balanced(cr[(size_t)ci * kc + k], ctx->primes[k]));
+    ctx->h_cflt.push_back(cf[ci]);
   }
   // executed modular work per step: program + structured P*M
   int mmo = 0, mao = 0;
@@ -860,6 +902,7 @@ int drx_program_upload(drx_ctx* ctx, const uint32_t* ops, int n_ops, int m, int 
     if (op == D_MUL || op == D_SQR) mmo += 1;
     else if (op == D_CUB) mmo += 2;
     else if (op == D_INV) mmo += 60;
+    else if (op == D_NZ1 || op == D_ZSEL) mmo += 1;  // one reduction and a select
     else if (op == D_ADD || op == D_SUB || op == D_NEG) mao += 1;
   }
   for (int j = 0; j < m; ++j) {
@@ -907,9 +950,10 @@ int drx_program_info(drx_ctx* ctx, int prog_id, int* n_dev_ops, int* n_const_use
 }
 
 int drx_program_pack(const uint32_t* ops, int n_ops, int m, int dim, const int64_t* consts, int n_const,
-                     uint32_t* words, int max_words, int* const_slots, int max_slots, int64_t* meta8, char* err,
-                     int err_len) {
-  if ((n_ops > 0 && !ops) || n_ops < 0 || n_const < 0 || (n_const > 0 && !consts) || !meta8) return DRX_EINVAL;
+                     uint32_t* words, int max_words, int* const_slots, int max_slots, int32_t* derived,
+                     int max_derived, int64_t* meta, int max_meta, char* err, int err_len) {
+  if ((n_ops > 0 && !ops) || n_ops < 0 || n_const < 0 || (n_const > 0 && !consts) || !meta || max_meta < 0)
+    return DRX_EINVAL;
   std::vector<char> is_one(n_const, 0), is_zero(n_const, 0);
   for (int i = 0; i < n_const; ++i) {
     is_one[i] = consts[i] == 1;
@@ -921,17 +965,24 @@ int drx_program_pack(const uint32_t* ops, int n_ops, int m, int dim, const int64
     if (err && err_len > 0) std::snprintf(err, (size_t)err_len, "%s", msg.c_str());
     return DRX_EINVAL;
   }
-  meta8[0] = (int64_t)pk.code.size();
-  meta8[1] = pk.kinds;
-  meta8[2] = pk.colred;
-  meta8[3] = pk.const_lds;
-  meta8[4] = pk.nslots;
-  meta8[5] = (int64_t)pk.const_slots.size();
-  meta8[6] = pk.fd_ok;
-  meta8[7] = pk.fd_degree;
-  if (words) std::copy(pk.code.begin(), pk.code.begin() + std::min<size_t>(pk.code.size(), max_words), words);
+  const int64_t mv[DRX_PACK_META] = {(int64_t)pk.code.size(), pk.kinds, pk.colred, pk.const_lds, pk.nslots,
+                                 (int64_t)pk.const_slots.size(), pk.fd_ok, pk.fd_degree,
+                                 (int64_t)pk.fcode.size(), pk.fnslots, (int64_t)pk.derived.size(),
+                                 pk.n_inv_batched, pk.n_src_ops};
+  for (int i = 0; i < std::min(max_meta, DRX_PACK_META); ++i) meta[i] = mv[i];
+  if (words) {
+    std::vector<uint32_t> all(pk.code);
+    all.insert(all.end(), pk.fcode.begin(), pk.fcode.end());
+    std::copy(all.begin(), all.begin() + std::min<size_t>(all.size(), std::max(0, max_words)), words);
+  }
   if (const_slots)
     for (size_t i = 0; i < pk.const_slots.size() && (int)i < max_slots; ++i) const_slots[i] = pk.const_slots[i];
+  if (derived)
+    for (size_t i = 0; i < pk.derived.size() && (int)i < max_derived; ++i) {
+      derived[3 * i] = (int32_t)pk.derived[i].op;
+      derived[3 * i + 1] = pk.derived[i].a;
+      derived[3 * i + 2] = pk.derived[i].b;
+    }
   return DRX_OK;
 }
 

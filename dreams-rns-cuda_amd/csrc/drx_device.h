@@ -35,6 +35,8 @@ enum DevOp : uint32_t {
   D_INV = 7,   // a^(p-2) mod p          | float: 1/a if |a| > 1e-300 else 0
   D_RED = 8,   // a mod p (balanced)     | float: a
   D_MOV = 9,   // a
+  D_NZ1 = 10,  // a, or 1 when a = 0 mod p (residue code only)
+  D_ZSEL = 11, // a, or 0 when b = 0 mod p (residue code only)
 };
 // Destination banks.
 enum DstBank : uint32_t { DB_R = 0, DB_E = 1, DB_S = 2, DB_NONE = 3 };

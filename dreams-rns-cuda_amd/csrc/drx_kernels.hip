@@ -51,9 +51,10 @@ __device__ __forceinline__ void exec_res(uint2 w, double& acc, v16d& R, EV& E, c
   const uint32_t op = w.x & 255u;
   const double x = fetch<false>(w.y & 7u, (w.y >> 3) & 8191u, R, X, Cv, acc, ldc, ldc_ld, lds_s, ln);
   double r;
-  if (op == D_MUL || op == D_ADD || op == D_SUB) {
+  if (op == D_MUL || op == D_ADD || op == D_SUB || op == D_ZSEL) {
     const double y = fetch<false>((w.y >> 16) & 7u, w.y >> 19, R, X, Cv, acc, ldc, ldc_ld, lds_s, ln);
     if (op == D_MUL) r = mm(x, y, ln.p, ln.pinv);
+    else if (op == D_ZSEL) r = (red(y, ln.p, ln.pinv) == 0.0) ? 0.0 : x;
     else r = (op == D_ADD) ? x + y : x - y;
   } else {
     switch (op) {
@@ -62,6 +63,7 @@ __device__ __forceinline__ void exec_res(uint2 w, double& acc, v16d& R, EV& E, c
       case D_CUB: r = mm(mm(x, x, ln.p, ln.pinv), x, ln.p, ln.pinv); break;
       case D_INV: r = inv_fermat(x, ln.p, ln.pinv, ln.pe); break;
       case D_RED: r = red(x, ln.p, ln.pinv); break;
+      case D_NZ1: r = (red(x, ln.p, ln.pinv) == 0.0) ? 1.0 : x; break;
       default: r = x; break;  // D_MOV
     }
   }
@@ -144,32 +146,29 @@ __device__ __forceinline__ uint32_t pow2_at_least(uint32_t k) {
 // C sub-chunk products are then combined in order by a log2(C)-level tree in
 // LDS (the walk product is associative).
 //
-// Interpreting kernels keep the program's constants ([nconst][KP] residues
-// and [nconst] FP64 values, bank L) and temps ([nslots][T], bank S) in LDS;
-// the walk kernels put the tree buffer [C][M*M][KP] after them.
+// Interpreting kernels keep the first kBank constant slots in the VGPR bank
+// C, the others in LDS ([nconst - kBank][KP] residues, bank L), and temps in
+// LDS ([nslots][T], bank S); the tree buffer [C][M*M][KP] reuses that LDS
+// once the steps are done.
 
 struct ProgLds {
-  double *ldc, *ldcf, *lds_s, *end;
+  double *ldc, *lds_s;
 };
 
-// Constants of `pg` into LDS (residues for K primes in a KP-wide stripe when
-// `residues`, FP64 values always); the caller synchronises.
+// Constants beyond the first kBank of `pg` into LDS, as residues for K
+// primes in a KP-wide stripe (bank L index i = constant slot kBank + i); the
+// caller synchronises.
 __device__ __forceinline__ ProgLds prog_lds(const KArgs& a, const DevProgram& pg, uint32_t K, uint32_t KP,
-                                            double* lds, bool residues) {
+                                            double* lds) {
   const uint32_t T = blockDim.x, tid = threadIdx.x;
-  const uint32_t nl = pg.const_lds ? pg.nconst : 0u;
+  const uint32_t nl = pg.nconst > (uint32_t)kBank ? pg.nconst - kBank : 0u;
   ProgLds L;
   L.ldc = lds;
-  L.ldcf = L.ldc + (residues ? (uint64_t)nl * KP : 0ull);
-  L.lds_s = L.ldcf + nl;
-  L.end = L.lds_s + (uint64_t)pg.nslots * T;
-  if (residues) {
-    for (uint32_t i = tid; i < nl * KP; i += T) {
-      const uint32_t ci = i / KP, ck = i % KP;
-      L.ldc[i] = (ck < K) ? a.cres[pg.cres_off + (uint64_t)ci * pg.kcap + ck] : 0.0;
-    }
+  L.lds_s = L.ldc + (uint64_t)nl * KP;
+  for (uint32_t i = tid; i < nl * KP; i += T) {
+    const uint32_t ci = i / KP, ck = i % KP;
+    L.ldc[i] = (ck < K) ? a.cres[pg.cres_off + (uint64_t)(kBank + ci) * pg.kcap + ck] : 0.0;
   }
-  for (uint32_t i = tid; i < nl; i += T) L.ldcf[i] = a.cflt[pg.cflt_off + i];
   return L;
 }
 
@@ -206,14 +205,15 @@ __device__ __forceinline__ WalkSetup walk_setup(const KArgs& a, double* lds, boo
   S.ln.tid = tid;
   S.ln.T = T;
   S.code = reinterpret_cast<const uint2*>(a.code + S.pg.code_off);
+  // the tree buffer overlays the constants and temps, which are dead once
+  // the steps are done (tree_combine synchronises before its first write)
+  S.tree = lds;
   if (interp) {
-    const ProgLds L = prog_lds(a, S.pg, S.K, S.KP, lds, true);
+    const ProgLds L = prog_lds(a, S.pg, S.K, S.KP, lds);
     S.ldc = L.ldc;
     S.lds_s = L.lds_s;
-    S.tree = L.end;
   } else {
     S.ldc = S.lds_s = nullptr;
-    S.tree = lds;
   }
   __syncthreads();
   const int64_t L = S.t.s_end - S.t.s_begin;
@@ -231,11 +231,9 @@ __device__ __forceinline__ void load_rns_inputs(const KArgs& a, const DevWalk& w
                                                 uint32_t K, uint32_t kk, const Lane& ln, int64_t s0, v16d& X,
                                                 v16d& Cv) {
   const double p = ln.p, pinv = ln.pinv;
-  if (!pg.const_lds) {
 #pragma unroll
-    for (int i = 0; i < 16; ++i)
-      if ((uint32_t)i < pg.nconst) Cv[i] = a.cres[pg.cres_off + (uint64_t)i * pg.kcap + kk];
-  }
+  for (int i = 0; i < kBank; ++i)
+    if ((uint32_t)i < pg.nconst) Cv[i] = a.cres[pg.cres_off + (uint64_t)i * pg.kcap + kk];
 #pragma unroll
   for (int ax = 0; ax < kMaxDim; ++ax) {
     if ((uint32_t)ax < pg.dim) {
@@ -260,6 +258,7 @@ template <int M>
 __device__ __forceinline__ void tree_combine(const KArgs& a, const WalkSetup& S, double (&P)[M * M]) {
   const uint32_t c = S.c, k = S.k, KP = S.KP, C = S.C;
   double* tree = S.tree;
+  __syncthreads();
 #pragma unroll
   for (int e = 0; e < M * M; ++e) tree[((uint64_t)c * M * M + e) * KP + k] = P[e];
   for (uint32_t stride = 1; stride < C; stride <<= 1) {
@@ -333,7 +332,7 @@ __global__ void __launch_bounds__(512) k_fd_seed(const KArgs a, uint32_t kp) {
   ln.k = k;
   ln.tid = threadIdx.x;
   ln.T = blockDim.x;
-  const ProgLds L = prog_lds(a, pg, K, KP, lds, true);
+  const ProgLds L = prog_lds(a, pg, K, KP, lds);
   __syncthreads();
   const uint2* code = reinterpret_cast<const uint2*>(a.code + pg.code_off);
   double* tab = a.fdt + w.fdt_off;
@@ -551,7 +550,7 @@ __device__ __forceinline__ double fetch_fp(uint32_t bank, uint32_t i, const v16d
     case SB_R: return R[i & 15u];
     case SB_X: return X[i & 15u];
     case SB_C: return ldc[i & 15u];
-    case SB_L: return ldc[i];
+    case SB_L: return ldc[kBank + i];
     case SB_S: return lds_s[(uint64_t)i * T + tid];
     case SB_A: return acc;
     default: return 0.0;
@@ -618,7 +617,7 @@ __global__ void __launch_bounds__(256) k_mf(const KArgs a) {
   double* lds_s = ldcf + pg.nconst;
   for (uint32_t i = tid; i < pg.nconst; i += T) ldcf[i] = a.cflt[pg.cflt_off + i];
   __syncthreads();
-  const uint2* code = reinterpret_cast<const uint2*>(a.code + pg.code_off);
+  const uint2* code = reinterpret_cast<const uint2*>(a.code + pg.fcode_off);
   typename EBank<M>::type E = 0.0;
   v16d R = 0.0, X = 0.0;
   const int64_t want = r.s_begin + tid;
@@ -631,7 +630,7 @@ __global__ void __launch_bounds__(256) k_mf(const KArgs a) {
     }
   }
   X[kStepSlot] = (double)s;
-  run_program_fp(code, pg.ncode, R, E, X, ldcf, lds_s, tid, T);
+  run_program_fp(code, pg.nfcode, R, E, X, ldcf, lds_s, tid, T);
   if (want < r.s_end) {
     double* out = a.mf + t.mf_off + (uint64_t)(s - t.s_begin) * M * M;
 #pragma unroll

@@ -4,21 +4,32 @@
 // every instruction writes a fresh register of a 512-entry file that is
 // zeroed at the start of every step (runner.py:217), and STORE copies a
 // register into a matrix entry. Packing:
-//   1. rename registers to values (reads of never-written registers read 0);
-//   2. keep only instructions reachable from the final STORE of each entry;
-//   3. classify entries as zero / one / general (static per program);
-//   4. insert RED where a sum could exceed 2^40 (the exactness bound of the
+//   1. rename registers to values (reads of never-written registers read 0),
+//      one value per distinct operation on the same operands;
+//   2. fold operations on constants into derived constants (their residues
+//      and FP64 values are computed at upload with the same operations);
+//   3. keep only values reachable from a general entry; classify entries as
+//      zero / one / general; find the entries' degrees in the step;
+//   4. FP64 code: the remaining operations in the reference's order;
+//   5. residue code: step-dependent INVs whose arguments do not depend on
+//      another INV share one inversion (Montgomery's trick: prefix products,
+//      one INV, back-substitution; a zero argument is replaced by 1 and its
+//      result masked to 0, so INV(0) = 0 as in the reference); RED is
+//      inserted where a sum could exceed 2^40 (the exactness bound of the
 //      FP64 residue multiply), tracking magnitude bounds per value;
-//   5. route operands: axes / step / constants read their banks directly,
-//      a value used by the very next instruction is read from the
-//      accumulator, everything else gets a register by linear scan (16 VGPR
-//      slots, then LDS slots).
-// Instruction order is preserved, so the FP64 shadow evaluates exactly the
-// reference's operations in the reference's order.
+//   6. route operands: axes / step / constants read their banks directly, a
+//      value used by the very next instruction is read from the accumulator,
+//      everything else gets a register by linear scan (16 VGPR slots, then
+//      LDS slots). Both codes share one constant-slot numbering.
+// Reordering independent operations changes no result: every value is the
+// same operation on the same operand values, in FP64 as in residues.
 #include "drx_pack.h"
 
 #include <algorithm>
+#include <array>
 #include <cstdio>
+#include <map>
+#include <queue>
 
 #include "drx_device.h"
 #include "drx_types.h"
@@ -51,6 +62,358 @@ struct Ins {            // device instruction before register assignment
 constexpr double kBoundLimit = 1099511627776.0;  // 2^40
 constexpr double kMulBound = 2147483648.0;       // 2^31
 
+bool binary_op(uint32_t op) { return op == D_ADD || op == D_SUB || op == D_MUL || op == D_ZSEL; }
+
+// A code list over a value graph, with its entry routing.
+struct CodeList {
+  std::vector<Ins> code;
+  std::vector<int> entry_direct;  // value -> entry its producer writes (-1: none)
+};
+
+// Instructions for the values in `order` (RED inserted when `with_red`),
+// then trailing moves for entries whose value cannot be written into E by
+// its producer: inputs, values shared by several entries or used later.
+CodeList build_code(std::vector<Val>& vals, const std::vector<int>& order, const std::vector<int>& entry,
+                    uint32_t kinds, int m, bool with_red) {
+  CodeList cl;
+  std::vector<double> bound(vals.size(), 0.0);
+  auto src_bound = [&](int v) -> double {
+    switch (vals[v].kind) {
+      case V_ZERO: return 0.0;
+      case V_X: return kMulBound;
+      case V_N: return kMulBound;
+      case V_C: return kMulBound / 2;
+      default: return bound[v];
+    }
+  };
+  for (int v : order) {
+    const Val x = vals[v];
+    const bool unary = !binary_op(x.op);
+    double bd;
+    switch (x.op) {
+      case D_ADD:
+      case D_SUB: bd = src_bound(x.a) + src_bound(x.b); break;
+      case D_NEG:
+      case D_NZ1:
+      case D_MOV: bd = src_bound(x.a); break;
+      case D_ZSEL: bd = src_bound(x.a); break;
+      default: bd = kMulBound; break;
+    }
+    if (with_red && bd > kBoundLimit) {
+      // the sum goes to the accumulator only; RED(A) produces value v
+      const int tmp = (int)vals.size();
+      vals.push_back(x);
+      bound.push_back(bd);
+      cl.code.push_back(Ins{x.op, x.a, unary ? -1 : x.b, tmp});
+      Val r{V_OP};
+      r.op = D_RED;
+      r.a = tmp;
+      vals[v] = r;
+      bound[v] = kMulBound;
+      cl.code.push_back(Ins{D_RED, tmp, -1, v});
+    } else {
+      bound[v] = bd;
+      cl.code.push_back(Ins{x.op, x.a, unary ? -1 : x.b, v});
+    }
+  }
+  const int nvals = (int)vals.size();
+  std::vector<int> n_entry_of(nvals, 0), uses(nvals, 0);
+  for (int e = 0; e < m * m; ++e)
+    if (((kinds >> (2 * e)) & 3u) == E_GEN) ++n_entry_of[entry[e]];
+  for (const Ins& in : cl.code) {
+    if (in.a >= 0) ++uses[in.a];
+    if (in.b >= 0) ++uses[in.b];
+  }
+  cl.entry_direct.assign(nvals, -1);
+  for (int e = 0; e < m * m; ++e) {
+    if (((kinds >> (2 * e)) & 3u) != E_GEN) continue;
+    const int v = entry[e];
+    if (vals[v].kind == V_OP && n_entry_of[v] == 1 && uses[v] == 0 && cl.entry_direct[v] < 0) {
+      cl.entry_direct[v] = e;
+    } else {
+      cl.code.push_back(Ins{D_MOV, v, -1, -1, e});
+      ++uses[v];
+    }
+  }
+  return cl;
+}
+
+// Constant slots: constants of the residue code by use count (ties: first
+// use), then those only the FP64 code reads.
+void assign_const_slots(const CodeList& rcl, const std::vector<Val>& rvals, const CodeList& fcl,
+                        const std::vector<Val>& fvals, int n_const_all, std::vector<int>& cslot,
+                        std::vector<int>& slots) {
+  std::vector<int> uses(n_const_all, 0), first(n_const_all, -1);
+  int pos = 0;
+  for (const Ins& in : rcl.code) {
+    for (int u : {in.a, in.b}) {
+      if (u < 0 || rvals[u].kind != V_C) continue;
+      const int ci = rvals[u].idx;
+      ++uses[ci];
+      if (first[ci] < 0) first[ci] = pos++;
+    }
+  }
+  std::vector<int> order;
+  for (int ci = 0; ci < n_const_all; ++ci)
+    if (first[ci] >= 0) order.push_back(ci);
+  std::sort(order.begin(), order.end(),
+            [&](int x, int y) { return uses[x] != uses[y] ? uses[x] > uses[y] : first[x] < first[y]; });
+  for (const Ins& in : fcl.code) {
+    for (int u : {in.a, in.b}) {
+      if (u < 0 || fvals[u].kind != V_C) continue;
+      const int ci = fvals[u].idx;
+      if (first[ci] < 0) {
+        first[ci] = pos++;
+        order.push_back(ci);
+      }
+    }
+  }
+  cslot.assign(n_const_all, -1);
+  slots = order;
+  for (size_t i = 0; i < order.size(); ++i) cslot[order[i]] = (int)i;
+}
+
+// Residue graph: the values, with every needed INV whose argument does not
+// depend on another INV replaced by its share of one inversion (Montgomery's
+// trick): z_j = NZ1(a_j), c_j = z_0 ... z_j, t = INV(c_{k-1}); going down,
+// w_j = t c_{j-1} and t <- t z_j; INV(a_j) = ZSEL(w_j, a_j). With
+// `recompute_z`, the back-substitution recomputes z_j instead of keeping it
+// live across the inversion. prio: the position in the reference's order
+// (new values: just after their last argument).
+struct ResGraph {
+  std::vector<Val> vals;
+  std::vector<double> prio;
+  int n_batched = 0;
+};
+
+ResGraph residue_graph(const std::vector<Val>& vals, const std::vector<char>& needed, bool recompute_z) {
+  ResGraph g;
+  g.vals = vals;
+  const int nv = (int)vals.size();
+  g.prio.resize(nv);
+  for (int v = 0; v < nv; ++v) g.prio[v] = v;
+  std::vector<char> dep_inv(nv, 0);  // value depends on an INV result
+  for (int v = 0; v < nv; ++v) {
+    const Val& x = vals[v];
+    if (x.kind != V_OP) continue;
+    dep_inv[v] = x.op == D_INV || dep_inv[x.a] || (binary_op(x.op) && dep_inv[x.b]);
+  }
+  std::vector<int> inv_vals;
+  for (int v = 0; v < nv; ++v)
+    if (needed[v] && vals[v].kind == V_OP && vals[v].op == D_INV && !dep_inv[vals[v].a]) inv_vals.push_back(v);
+  const int k = (int)inv_vals.size();
+  if (k < 2) return g;
+  auto add = [&](uint32_t op, int a, int b, double pr) {
+    Val x{V_OP};
+    x.op = op;
+    x.a = a;
+    x.b = b;
+    g.vals.push_back(x);
+    g.prio.push_back(pr);
+    return (int)g.vals.size() - 1;
+  };
+  std::vector<int> arg(k), z(k), c(k);
+  double last = 0;
+  for (int j = 0; j < k; ++j) {
+    arg[j] = vals[inv_vals[j]].a;
+    const double pr = g.prio[arg[j]] + 0.5;
+    last = std::max(last, pr);
+    z[j] = add(D_NZ1, arg[j], 0, pr);
+    c[j] = j == 0 ? z[0] : add(D_MUL, c[j - 1], z[j], pr);
+  }
+  int t = add(D_INV, c[k - 1], 0, last);
+  std::vector<int> w(k);
+  for (int j = k - 1; j >= 1; --j) {
+    w[j] = add(D_MUL, t, c[j - 1], last);
+    const int zj = recompute_z ? add(D_NZ1, arg[j], 0, last) : z[j];
+    t = add(D_MUL, t, zj, last);
+  }
+  w[0] = t;
+  for (int j = 0; j < k; ++j) {
+    Val r{V_OP};
+    r.op = D_ZSEL;
+    r.a = w[j];
+    r.b = arg[j];
+    g.vals[inv_vals[j]] = r;
+  }
+  g.n_batched = k;
+  return g;
+}
+
+constexpr int kPolicies = 4;
+
+// Evaluation order of the values reachable from `roots` (operands first):
+// 0: depth-first from the entries in order; 1: depth-first, the operand
+// needing more registers first (Sethi-Ullman labels); 2: depth-first from the
+// entries in reverse order; 3: list scheduling by prio.
+std::vector<int> schedule(const std::vector<Val>& vals, const std::vector<double>& prio,
+                          const std::vector<int>& roots, int policy) {
+  const int n = (int)vals.size();
+  auto dfs = [&](const std::vector<int>& rts, const std::vector<int>* label) {
+    std::vector<int> order;
+    std::vector<char> state(n, 0);  // 0 new, 1 open, 2 done
+    std::vector<std::pair<int, int>> st;
+    for (int r : rts) {
+      st.push_back({r, 0});
+      while (!st.empty()) {
+        auto& top = st.back();
+        const int v = top.first;
+        if (vals[v].kind != V_OP || state[v] == 2) {
+          st.pop_back();
+          continue;
+        }
+        state[v] = 1;
+        int first = vals[v].a, second = binary_op(vals[v].op) ? vals[v].b : -1;
+        if (label && second >= 0 && (*label)[second] > (*label)[first]) std::swap(first, second);
+        if (top.second == 0) {
+          top.second = 1;
+          st.push_back({first, 0});
+        } else if (top.second == 1) {
+          top.second = 2;
+          if (second >= 0) st.push_back({second, 0});
+        } else {
+          state[v] = 2;
+          order.push_back(v);
+          st.pop_back();
+        }
+      }
+    }
+    return order;
+  };
+  std::vector<int> base = dfs(roots, nullptr);
+  if (policy == 0) return base;
+  if (policy == 2) {
+    std::vector<int> rev(roots.rbegin(), roots.rend());
+    return dfs(rev, nullptr);
+  }
+  if (policy == 1) {
+    std::vector<int> label(n, 0);
+    for (int v : base) {
+      const int la = label[vals[v].a], lb = binary_op(vals[v].op) ? label[vals[v].b] : 0;
+      label[v] = la == lb ? la + 1 : std::max(la, lb);
+    }
+    return dfs(roots, &label);
+  }
+  // list scheduling over the reachable values
+  std::vector<int> pending(n, 0);
+  std::vector<std::vector<int>> users(n);
+  for (int v : base) {
+    for (int u : {vals[v].a, binary_op(vals[v].op) ? vals[v].b : -1}) {
+      if (u < 0 || vals[u].kind != V_OP) continue;
+      ++pending[v];
+      users[u].push_back(v);
+    }
+  }
+  auto cmp = [&](int x, int y) { return prio[x] != prio[y] ? prio[x] > prio[y] : x > y; };
+  std::priority_queue<int, std::vector<int>, decltype(cmp)> ready(cmp);
+  for (int v : base)
+    if (pending[v] == 0) ready.push(v);
+  std::vector<int> order;
+  while (!ready.empty()) {
+    const int v = ready.top();
+    ready.pop();
+    order.push_back(v);
+    for (int u : users[v])
+      if (--pending[u] == 0) ready.push(u);
+  }
+  return order;
+}
+
+// Operand routing and register allocation of one code list.
+std::string encode(const CodeList& cl, const std::vector<Val>& vals, const std::vector<int>& cslot,
+                   std::vector<uint32_t>& words, int& nslots) {
+  const std::vector<Ins>& code = cl.code;
+  const int nc = (int)code.size(), nvals = (int)vals.size();
+  std::vector<int> def_pos(nvals, -1), last_use(nvals, -1);
+  std::vector<char> only_next(nvals, 1);
+  for (int i = 0; i < nc; ++i)
+    if (code[i].def >= 0) def_pos[code[i].def] = i;
+  for (int i = 0; i < nc; ++i) {
+    for (int u : {code[i].a, code[i].b}) {
+      if (u < 0 || vals[u].kind != V_OP) continue;
+      last_use[u] = std::max(last_use[u], i);
+      if (i != def_pos[u] + 1) only_next[u] = 0;
+    }
+  }
+  // linear scan: slot 0..15 = VGPR bank R, 16.. = LDS bank S
+  std::vector<int> slot(nvals, -1);
+  std::vector<int> free_r, free_s;
+  for (int s = kBank - 1; s >= 0; --s) free_r.push_back(s);
+  int n_s = 0;
+  std::vector<std::vector<int>> dying(nc);
+  for (int v = 0; v < nvals; ++v)
+    if (last_use[v] >= 0 && def_pos[v] >= 0 && !(only_next[v] && cl.entry_direct[v] < 0))
+      dying[last_use[v]].push_back(v);
+  words.clear();
+  words.reserve(2 * nc);
+  for (int i = 0; i < nc; ++i) {
+    const Ins& in = code[i];
+    uint32_t sb[2] = {SB_Z, SB_Z}, si[2] = {0, 0};
+    const int opnd[2] = {in.a, in.b};
+    for (int q = 0; q < 2; ++q) {
+      const int u = opnd[q];
+      if (u < 0) continue;
+      const Val& x = vals[u];
+      if (x.kind == V_ZERO) {
+        sb[q] = SB_Z;
+      } else if (x.kind == V_X) {
+        sb[q] = SB_X;
+        si[q] = (uint32_t)x.idx;
+      } else if (x.kind == V_N) {
+        sb[q] = SB_X;
+        si[q] = kStepSlot;
+      } else if (x.kind == V_C) {
+        const int cs = cslot[x.idx];
+        sb[q] = cs < kBank ? SB_C : SB_L;
+        si[q] = (uint32_t)(cs < kBank ? cs : cs - kBank);
+      } else if (def_pos[u] == i - 1) {
+        sb[q] = SB_A;
+      } else {
+        const int s = slot[u];
+        if (s < 0) return "internal: operand without a register";
+        sb[q] = s < kBank ? SB_R : SB_S;
+        si[q] = s < kBank ? (uint32_t)s : (uint32_t)(s - kBank);
+      }
+    }
+    // free registers of values dying here (before assigning the destination)
+    for (int v : dying[i]) {
+      const int s = slot[v];
+      if (s < 0) continue;
+      if (s < kBank) free_r.push_back(s);
+      else free_s.push_back(s);
+    }
+    uint32_t db = DB_NONE, di = 0;
+    if (in.entry >= 0) {
+      db = DB_E;
+      di = (uint32_t)in.entry;
+    } else if (in.def >= 0) {
+      const int v = in.def;
+      if (cl.entry_direct[v] >= 0) {
+        db = DB_E;
+        di = (uint32_t)cl.entry_direct[v];
+      } else if (last_use[v] >= 0 && !only_next[v]) {
+        int s;
+        if (!free_r.empty()) {
+          s = free_r.back();
+          free_r.pop_back();
+        } else if (!free_s.empty()) {
+          s = free_s.back();
+          free_s.pop_back();
+        } else {
+          s = kBank + n_s++;
+        }
+        slot[v] = s;
+        db = s < kBank ? DB_R : DB_S;
+        di = s < kBank ? (uint32_t)s : (uint32_t)(s - kBank);
+      }
+    }
+    words.push_back(enc_w0(in.op, db, di));
+    words.push_back(enc_w1(sb[0], si[0], sb[1], si[1]));
+  }
+  nslots = n_s;
+  return std::string();
+}
+
 }  // namespace
 
 std::string pack_program(const uint32_t* ops, int n_ops, int m, int dim, int n_const,
@@ -67,6 +430,19 @@ std::string pack_program(const uint32_t* ops, int n_ops, int m, int dim, int n_c
   }
   std::vector<Val> vals;
   vals.push_back(Val{V_ZERO});
+  // value numbering: an operation on the same operand values is the same
+  // value (exact in residues and in FP64, where the operation is
+  // deterministic and ADD / MUL are commutative bit for bit)
+  std::map<std::array<int64_t, 4>, int> seen;
+  auto intern = [&](Val x) -> int {
+    if (x.kind == V_OP && (x.op == D_ADD || x.op == D_MUL) && x.a > x.b) std::swap(x.a, x.b);
+    const std::array<int64_t, 4> key = {x.kind, x.kind == V_OP ? (int64_t)x.op : x.idx, x.a, x.b};
+    auto it = seen.find(key);
+    if (it != seen.end()) return it->second;
+    vals.push_back(x);
+    seen.emplace(key, (int)vals.size() - 1);
+    return (int)vals.size() - 1;
+  };
   std::vector<int> reg(512, 0);
   std::vector<int> entry(m * m, 0);
   int n_src = 0;
@@ -83,18 +459,15 @@ std::string pack_program(const uint32_t* ops, int n_ops, int m, int dim, int n_c
         break;
       case R_LOAD_X:
         if ((int)a0 >= dim || !chk_reg(a1)) goto bad_operand;
-        vals.push_back(Val{V_X, (int)a0});
-        reg[a1] = (int)vals.size() - 1;
+        reg[a1] = intern(Val{V_X, (int)a0});
         break;
       case R_LOAD_C:
         if ((int)a0 >= n_const || !chk_reg(a1)) goto bad_operand;
-        vals.push_back(Val{V_C, (int)a0});
-        reg[a1] = (int)vals.size() - 1;
+        reg[a1] = intern(Val{V_C, (int)a0});
         break;
       case R_LOAD_N:
         if (!chk_reg(a0)) goto bad_operand;
-        vals.push_back(Val{V_N});
-        reg[a0] = (int)vals.size() - 1;
+        reg[a0] = intern(Val{V_N});
         break;
       case R_ADD:
       case R_SUB:
@@ -104,8 +477,7 @@ std::string pack_program(const uint32_t* ops, int n_ops, int m, int dim, int n_c
         v.op = op == R_ADD ? D_ADD : (op == R_SUB ? D_SUB : D_MUL);
         v.a = reg[a0];
         v.b = reg[a1];
-        vals.push_back(v);
-        reg[a2] = (int)vals.size() - 1;
+        reg[a2] = intern(v);
         if (op == R_ADD) ++pp.n_add;
         else if (op == R_SUB) ++pp.n_sub;
         else ++pp.n_mul;
@@ -120,8 +492,7 @@ std::string pack_program(const uint32_t* ops, int n_ops, int m, int dim, int n_c
         v.op = op == R_NEG ? D_NEG : (op == R_POW2 ? D_SQR : (op == R_POW3 ? D_CUB : D_INV));
         v.a = reg[a0];
         v.b = 0;
-        vals.push_back(v);
-        reg[a1] = (int)vals.size() - 1;
+        reg[a1] = intern(v);
         if (op == R_NEG) ++pp.n_neg;
         else if (op == R_POW2) ++pp.n_sqr;
         else if (op == R_POW3) ++pp.n_cub;
@@ -143,18 +514,34 @@ std::string pack_program(const uint32_t* ops, int n_ops, int m, int dim, int n_c
   }
   pp.n_src_ops = n_src;
 
+  // ---- constant folding (values are in topological order)
+  int n_const_all = n_const;
+  for (size_t v = 1; v < vals.size(); ++v) {
+    Val& x = vals[v];
+    if (x.kind != V_OP) continue;
+    const bool unary = !binary_op(x.op);
+    const bool foldable_op = x.op == D_ADD || x.op == D_SUB || x.op == D_MUL || x.op == D_NEG || x.op == D_SQR ||
+                             x.op == D_INV;
+    if (!foldable_op || vals[x.a].kind != V_C || (!unary && vals[x.b].kind != V_C)) continue;
+    pp.derived.push_back(DerivedConst{x.op, vals[x.a].idx, unary ? -1 : vals[x.b].idx});
+    const Val c{V_C, n_const_all++};
+    x = c;
+  }
+  if (n_const_all > kMaxConst) return "too many constants after folding";
+  std::vector<char> is_one(const_is_one), is_zero(const_is_zero);
+  is_one.resize(n_const_all, 0);
+  is_zero.resize(n_const_all, 0);
+
   // ---- entry kinds and liveness
   const int nv = (int)vals.size();
   std::vector<char> needed(nv, 0);
   std::vector<int> stack;
-  auto is_const_val = [&](int v, const std::vector<char>& tab) {
-    return vals[v].kind == V_C && tab[vals[v].idx];
-  };
+  auto is_const_val = [&](int v, const std::vector<char>& tab) { return vals[v].kind == V_C && tab[vals[v].idx]; };
   for (int e = 0; e < m * m; ++e) {
     const int v = entry[e];
     uint32_t kind = E_GEN;
-    if (vals[v].kind == V_ZERO || is_const_val(v, const_is_zero)) kind = E_ZERO;
-    else if (is_const_val(v, const_is_one)) kind = E_ONE;
+    if (vals[v].kind == V_ZERO || is_const_val(v, is_zero)) kind = E_ZERO;
+    else if (is_const_val(v, is_one)) kind = E_ONE;
     pp.kinds |= kind << (2 * e);
     if (kind == E_GEN) stack.push_back(v);
   }
@@ -180,7 +567,7 @@ std::string pack_program(const uint32_t* ops, int n_ops, int m, int dim, int n_c
         case V_N: deg[v] = 1; break;
         default: {
           const int da = deg[x.a], db = deg[x.b];
-          if (x.op == D_INV || da < 0 || (db < 0 && (x.op == D_ADD || x.op == D_SUB || x.op == D_MUL))) {
+          if (x.op == D_INV || da < 0 || (db < 0 && binary_op(x.op))) {
             deg[v] = -1;
           } else if (x.op == D_ADD || x.op == D_SUB) {
             deg[v] = std::max(da, db);
@@ -223,175 +610,71 @@ std::string pack_program(const uint32_t* ops, int n_ops, int m, int dim, int n_c
     if (ones > 0 && nz >= 2) pp.colred |= 1u << j;
   }
 
-  // ---- emit (with RED for magnitude control), values renumbered by Ins
-  std::vector<double> bound(nv, 0.0);
-  std::vector<Ins> code;
-  auto src_bound = [&](int v) -> double {
-    switch (vals[v].kind) {
-      case V_ZERO: return 0.0;
-      case V_X: return kMulBound;
-      case V_N: return kMulBound;
-      case V_C: return kMulBound / 2;
-      default: return bound[v];
-    }
-  };
-  for (int v = 0; v < nv; ++v) {
-    if (!needed[v] || vals[v].kind != V_OP) continue;
-    const Val x = vals[v];
-    double bd;
-    switch (x.op) {
-      case D_ADD:
-      case D_SUB: bd = src_bound(x.a) + src_bound(x.b); break;
-      case D_NEG: bd = src_bound(x.a); break;
-      default: bd = kMulBound; break;
-    }
-    const bool unary = !(x.op == D_ADD || x.op == D_SUB || x.op == D_MUL);
-    if (bd > kBoundLimit) {
-      // the sum goes to the accumulator only; RED(A) produces value v
-      const int tmp = (int)vals.size();
-      vals.push_back(x);
-      bound.push_back(bd);
-      needed.push_back(1);
-      code.push_back(Ins{x.op, x.a, unary ? -1 : x.b, tmp});
-      Val r{V_OP};
-      r.op = D_RED;
-      r.a = tmp;
-      vals[v] = r;
-      bound[v] = kMulBound;
-      code.push_back(Ins{D_RED, tmp, -1, v});
-    } else {
-      bound[v] = bd;
-      code.push_back(Ins{x.op, x.a, unary ? -1 : x.b, v});
-    }
-  }
-  // trailing moves for entries whose value cannot be written into E by its
-  // producer: inputs, values shared by several entries or used later
-  const int nvals = (int)vals.size();
-  std::vector<int> producer(nvals, -1);
-  for (int i = 0; i < (int)code.size(); ++i) producer[code[i].def] = i;
-  std::vector<int> n_entry_of(nvals, 0);
+  std::vector<int> roots;
   for (int e = 0; e < m * m; ++e)
-    if (((pp.kinds >> (2 * e)) & 3u) == E_GEN) ++n_entry_of[entry[e]];
-  std::vector<int> uses(nvals, 0);
-  for (const Ins& in : code) {
-    if (in.a >= 0) ++uses[in.a];
-    if (in.b >= 0) ++uses[in.b];
-  }
-  std::vector<int> entry_direct(nvals, -1);  // value -> entry its producer writes
-  for (int e = 0; e < m * m; ++e) {
-    if (((pp.kinds >> (2 * e)) & 3u) != E_GEN) continue;
-    const int v = entry[e];
-    if (vals[v].kind == V_OP && n_entry_of[v] == 1 && uses[v] == 0) {
-      entry_direct[v] = e;
-    } else {
-      code.push_back(Ins{D_MOV, v, -1, -1, e});
-      ++uses[v];
-    }
-  }
+    if (((pp.kinds >> (2 * e)) & 3u) == E_GEN) roots.push_back(entry[e]);
+  std::vector<int> dummy_cslot(n_const_all, 0);
 
-  // ---- operand routing and register allocation
-  const int nc = (int)code.size();
-  std::vector<int> def_pos(nvals, -1), last_use(nvals, -1);
-  std::vector<char> only_next(nvals, 1);
-  for (int i = 0; i < nc; ++i) {
-    if (code[i].def >= 0) def_pos[code[i].def] = i;
-  }
-  for (int i = 0; i < nc; ++i) {
-    for (int u : {code[i].a, code[i].b}) {
-      if (u < 0 || vals[u].kind != V_OP) continue;
-      last_use[u] = std::max(last_use[u], i);
-      if (i != def_pos[u] + 1) only_next[u] = 0;
-    }
-  }
-  // constant slots in order of first use
-  std::vector<int> cslot(n_const, -1);
-  for (int i = 0; i < nc; ++i) {
-    for (int u : {code[i].a, code[i].b}) {
-      if (u >= 0 && vals[u].kind == V_C && cslot[vals[u].idx] < 0) {
-        cslot[vals[u].idx] = (int)pp.const_slots.size();
-        pp.const_slots.push_back(vals[u].idx);
+  // ---- FP64 code: the same operations as the reference (one reciprocal per
+  // INV), in the candidate order needing the fewest LDS temp slots
+  CodeList fcl;
+  std::vector<Val> fvals;
+  {
+    std::vector<double> prio(nv);
+    for (int v = 0; v < nv; ++v) prio[v] = v;
+    int best = -1;
+    for (int policy = 0; policy < kPolicies; ++policy) {
+      std::vector<Val> cv = vals;
+      CodeList cl = build_code(cv, schedule(cv, prio, roots, policy), entry, pp.kinds, m, false);
+      std::vector<uint32_t> scratch_words;
+      int ns = 0;
+      const std::string err = encode(cl, cv, dummy_cslot, scratch_words, ns);
+      if (!err.empty()) return err;
+      if (best < 0 || ns < best) {
+        best = ns;
+        fcl = std::move(cl);
+        fvals = std::move(cv);
       }
     }
   }
+
+  // ---- residue code: shared inversion, then the evaluation order (of a few
+  // candidates) that needs the fewest LDS temp slots, then the shortest code
+  CodeList rcl;
+  std::vector<Val> rvals;
+  int best_slots = -1;
+  size_t best_len = 0;
+  for (int recompute = 0; recompute < 2; ++recompute) {
+    const ResGraph g = residue_graph(vals, needed, recompute != 0);
+    if (g.n_batched == 0 && recompute) break;  // nothing batched: one graph only
+    for (int policy = 0; policy < kPolicies; ++policy) {
+      std::vector<Val> cv = g.vals;
+      const std::vector<int> order = schedule(cv, g.prio, roots, policy);
+      CodeList cl = build_code(cv, order, entry, pp.kinds, m, true);
+      std::vector<uint32_t> scratch_words;
+      int ns = 0;
+      const std::string err = encode(cl, cv, dummy_cslot, scratch_words, ns);
+      if (!err.empty()) return err;
+      if (best_slots < 0 || ns < best_slots || (ns == best_slots && cl.code.size() < best_len)) {
+        best_slots = ns;
+        best_len = cl.code.size();
+        rcl = std::move(cl);
+        rvals = std::move(cv);
+        pp.n_inv_batched = g.n_batched;
+      }
+    }
+  }
+
+  // ---- constant slots (the residue code's most used first: slots 0..15 sit
+  // in VGPRs, the rest in LDS), then routing and allocation of both codes
+  std::vector<int> cslot;
+  assign_const_slots(rcl, rvals, fcl, fvals, n_const_all, cslot, pp.const_slots);
   pp.const_lds = pp.const_slots.size() > (size_t)kBank ? 1 : 0;
-  if (pp.const_slots.size() > 8191) return "too many constants";
-
-  // linear scan: slot 0..15 = VGPR bank R, 16.. = LDS bank S
-  std::vector<int> slot(nvals, -1);
-  std::vector<int> free_r, free_s;
-  for (int s = kBank - 1; s >= 0; --s) free_r.push_back(s);
-  int n_s = 0;
-  std::vector<std::vector<int>> dying(nc);
-  for (int v = 0; v < nvals; ++v)
-    if (last_use[v] >= 0 && def_pos[v] >= 0 && !(only_next[v] && entry_direct[v] < 0))
-      dying[last_use[v]].push_back(v);
-  std::vector<uint32_t> words;
-  words.reserve(2 * nc);
-  for (int i = 0; i < nc; ++i) {
-    const Ins& in = code[i];
-    uint32_t sb[2] = {SB_Z, SB_Z}, si[2] = {0, 0};
-    const int opnd[2] = {in.a, in.b};
-    for (int q = 0; q < 2; ++q) {
-      const int u = opnd[q];
-      if (u < 0) continue;
-      const Val& x = vals[u];
-      if (x.kind == V_ZERO) {
-        sb[q] = SB_Z;
-      } else if (x.kind == V_X) {
-        sb[q] = SB_X;
-        si[q] = (uint32_t)x.idx;
-      } else if (x.kind == V_N) {
-        sb[q] = SB_X;
-        si[q] = kStepSlot;
-      } else if (x.kind == V_C) {
-        sb[q] = pp.const_lds ? SB_L : SB_C;
-        si[q] = (uint32_t)cslot[x.idx];
-      } else if (def_pos[u] == i - 1) {
-        sb[q] = SB_A;
-      } else {
-        const int s = slot[u];
-        if (s < 0) return "internal: operand without a register";
-        sb[q] = s < kBank ? SB_R : SB_S;
-        si[q] = s < kBank ? (uint32_t)s : (uint32_t)(s - kBank);
-      }
-    }
-    // free registers of values dying here (before assigning the destination)
-    for (int v : dying[i]) {
-      const int s = slot[v];
-      if (s < 0) continue;
-      if (s < kBank) free_r.push_back(s);
-      else free_s.push_back(s);
-    }
-    uint32_t db = DB_NONE, di = 0;
-    if (in.entry >= 0) {
-      db = DB_E;
-      di = (uint32_t)in.entry;
-    } else if (in.def >= 0) {
-      const int v = in.def;
-      if (entry_direct[v] >= 0) {
-        db = DB_E;
-        di = (uint32_t)entry_direct[v];
-      } else if (last_use[v] >= 0 && !only_next[v]) {
-        int s;
-        if (!free_r.empty()) {
-          s = free_r.back();
-          free_r.pop_back();
-        } else if (!free_s.empty()) {
-          s = free_s.back();
-          free_s.pop_back();
-        } else {
-          s = kBank + n_s++;
-        }
-        slot[v] = s;
-        db = s < kBank ? DB_R : DB_S;
-        di = s < kBank ? (uint32_t)s : (uint32_t)(s - kBank);
-      }
-    }
-    words.push_back(enc_w0(in.op, db, di));
-    words.push_back(enc_w1(sb[0], si[0], sb[1], si[1]));
-  }
-  pp.nslots = n_s;
-  pp.code.swap(words);
+  if (pp.const_slots.size() > 8191 + kBank) return "too many constants";
+  std::string err = encode(rcl, rvals, cslot, pp.code, pp.nslots);
+  if (!err.empty()) return err;
+  err = encode(fcl, fvals, cslot, pp.fcode, pp.fnslots);
+  if (!err.empty()) return err;
   *out = std::move(pp);
   return std::string();
 }

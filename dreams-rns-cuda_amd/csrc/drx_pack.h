@@ -8,15 +8,32 @@
 
 namespace drx {
 
+// A constant computed from other constants at upload time (constant
+// subexpression folding): value = op(a, b) with a, b constant indices
+// (b = -1 for unary ops); its index is n_const + its position in `derived`.
+struct DerivedConst {
+  uint32_t op;  // D_ADD, D_SUB, D_MUL, D_NEG, D_SQR or D_INV
+  int a, b;
+};
+
 struct PackedProgram {
   int m = 0, dim = 0;
-  std::vector<uint32_t> code;        // two words per device instruction
-  std::vector<int> const_slots;      // device constant slot -> program constant index
+  // Residue code: two words per device instruction. Step-dependent INVs
+  // whose arguments do not depend on another INV share one inversion
+  // (Montgomery's trick with zero arguments masked, so INV(0) stays 0).
+  std::vector<uint32_t> code;
+  // FP64 code (the M_f pass): the reference's operations, one reciprocal per
+  // INV, so the shadow's bits follow the reference.
+  std::vector<uint32_t> fcode;
+  std::vector<int> const_slots;      // device constant slot -> constant index (>= n_const: derived)
+  std::vector<DerivedConst> derived;
   uint32_t kinds = 0;                // EntryKind per entry (2 bits)
   uint32_t colred = 0;               // columns reduced after P*M
-  int const_lds = 0;                 // constants in LDS instead of the VGPR bank
-  int nslots = 0;                    // LDS temp slots
+  int const_lds = 0;                 // constant slots beyond the 16 of the VGPR bank live in LDS
+  int nslots = 0;                    // LDS temp slots of the residue code
+  int fnslots = 0;                   // LDS temp slots of the FP64 code
   int n_src_ops = 0;                 // reference instructions before END
+  int n_inv_batched = 0;             // INVs served by the shared inversion
   // Degree in the step of every general entry when the program has no INV
   // on any path to an entry (then every entry is a polynomial in the step):
   // 4 bits per entry; fd_degree = max over entries; fd_ok = 0 otherwise.

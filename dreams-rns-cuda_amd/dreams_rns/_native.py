@@ -16,7 +16,8 @@ import numpy as np
 
 from ._build import LIB_PATH
 
-ABI_VERSION = 1
+ABI_VERSION = 2
+PACK_META = 13  # values drx_program_pack reports
 DRX_INIT_PCF, DRX_INIT_IDENTITY, DRX_INIT_GIVEN = 0, 1, 2
 MODE_BATCHED, MODE_PLAIN, MODE_PERSISTENT = 0, 1, 2
 
@@ -89,7 +90,8 @@ SIGNATURES = {
                                    C.POINTER(C.c_uint32)]),
     "drx_program_release_all": (C.c_int, [C.c_void_p]),
     "drx_program_pack": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
-                                   C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_char_p, C.c_int]),
+                                   C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                   C.c_char_p, C.c_int]),
     "drx_walk_batch": (C.c_int, [C.c_void_p, C.POINTER(Batch), C.POINTER(LaunchCfg)]),
     "drx_plan_create": (C.c_int, [C.c_void_p, C.POINTER(Batch), C.POINTER(LaunchCfg), C.POINTER(C.c_void_p)]),
     "drx_plan_run": (C.c_int, [C.c_void_p]),
@@ -290,24 +292,32 @@ def pinned_zeros(shape, dtype) -> np.ndarray:
 
 def pack_program(program):
     """Packed device code of ``program`` (no device needed): (words uint32,
-    const_slots int list, meta dict). Constants must fit in int64."""
+    const_slots int list, meta dict). ``words`` holds the residue code and
+    then the FP64 code (``meta["words"]`` / ``meta["fwords"]`` of them);
+    ``meta["derived"]`` lists the folded constants as (op, a, b) triples.
+    Constants must fit in int64."""
     lib = load_library()
     ops = np.ascontiguousarray(
         np.array([(int(i.op), i.arg0, i.arg1, i.arg2) for i in program.opcodes] or [(12, 0, 0, 0)],
                  dtype=np.uint32))
     consts = np.array([int(c) for c in program.constants] or [0], dtype=np.int64)
-    cap = 4 * len(program.opcodes) + 64
+    n = len(program.opcodes)
+    cap = 16 * n + 256
     words = np.zeros(cap, dtype=np.uint32)
-    slots = np.zeros(max(1, len(program.constants)), dtype=np.int32)
-    meta = np.zeros(8, dtype=np.int64)
+    slots = np.zeros(n + len(program.constants) + 1, dtype=np.int32)
+    derived = np.zeros((n + 1, 3), dtype=np.int32)
+    meta = np.zeros(PACK_META, dtype=np.int64)
     err = C.create_string_buffer(256)
-    rc = lib.drx_program_pack(ops.ctypes.data, len(program.opcodes), program.m, program.dim, consts.ctypes.data,
+    rc = lib.drx_program_pack(ops.ctypes.data, n, program.m, program.dim, consts.ctypes.data,
                               len(program.constants), words.ctypes.data, cap, slots.ctypes.data, len(slots),
-                              meta.ctypes.data, err, 256)
+                              derived.ctypes.data, len(derived), meta.ctypes.data, PACK_META, err, 256)
     if rc != 0:
         raise ValueError(err.value.decode())
-    nw = int(meta[0])
-    return words[:nw].copy(), [int(s) for s in slots[: int(meta[5])]], {
-        "kinds": int(meta[1]), "colred": int(meta[2]), "const_lds": int(meta[3]),
-        "nslots": int(meta[4]), "nconst": int(meta[5]), "fd_ok": int(meta[6]), "fd_degree": int(meta[7])}
-
+    nw, nfw = int(meta[0]), int(meta[8])
+    if nw + nfw > cap:
+        raise RuntimeError("packed code larger than the buffer")
+    return words[: nw + nfw].copy(), [int(s) for s in slots[: int(meta[5])]], {
+        "words": nw, "kinds": int(meta[1]), "colred": int(meta[2]), "const_lds": int(meta[3]),
+        "nslots": int(meta[4]), "nconst": int(meta[5]), "fd_ok": int(meta[6]), "fd_degree": int(meta[7]),
+        "fwords": nfw, "fnslots": int(meta[9]), "inv_batched": int(meta[11]), "src_ops": int(meta[12]),
+        "derived": [tuple(int(v) for v in derived[i]) for i in range(int(meta[10]))]}

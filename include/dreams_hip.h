@@ -18,7 +18,7 @@
 extern "C" {
 #endif
 
-#define DRX_ABI_VERSION 1
+#define DRX_ABI_VERSION 2
 
 enum {
   DRX_OK = 0,
@@ -64,16 +64,24 @@ int drx_program_info(drx_ctx* ctx, int prog_id, int* n_dev_ops, int* n_const_use
 int drx_program_release_all(drx_ctx* ctx);
 
 /* Pack a program without a device (inspection and host-side testing of the
- * packer): writes up to max_words device words (two per instruction) and the
- * packing metadata: meta[0] words, [1] entry kinds (2 bits per entry),
- * [2] columns reduced after P*M, [3] constants in LDS, [4] LDS temp slots,
- * [5] constants used, [6] all entries polynomial (forward differences
- * apply), [7] their maximal degree; const_slots[i] = program constant index of slot i
- * (up to max_slots entries). Returns DRX_EINVAL with the packer's message in
- * err (err_len bytes) for programs the engine cannot run. */
+ * packer). words receives the residue code followed by the FP64 code (two
+ * words per instruction each; up to max_words in total); const_slots[i] =
+ * constant index of device slot i (an index >= n_const names a folded
+ * constant); derived receives (op, a, b) per folded constant, its value being
+ * op applied to constants a and b (b = -1 for a unary op). meta receives up to
+ * max_meta of DRX_PACK_META values: [0] residue-code words, [1] entry kinds
+ * (2 bits per entry), [2] columns reduced after P*M, [3] constants in LDS,
+ * [4] LDS temp slots of the residue code, [5] constant slots, [6] all entries
+ * polynomial (forward differences apply), [7] their maximal degree,
+ * [8] FP64-code words, [9] LDS temp slots of the FP64 code, [10] folded
+ * constants, [11] INVs sharing one inversion, [12] reference instructions.
+ * Returns DRX_EINVAL with the packer's message in err (err_len bytes) for
+ * programs the engine cannot run. No counterpart in the reference: its
+ * bytecode is interpreted as written (python/dreams_rns/runner.py:217-316). */
+#define DRX_PACK_META 13
 int drx_program_pack(const uint32_t* ops, int n_ops, int m, int dim, const int64_t* consts, int n_const,
-                     uint32_t* words, int max_words, int* const_slots, int max_slots, int64_t* meta8,
-                     char* err, int err_len);
+                     uint32_t* words, int max_words, int* const_slots, int max_slots, int32_t* derived,
+                     int max_derived, int64_t* meta, int max_meta, char* err, int err_len);
 
 /* ---- walks ------------------------------------------------------------- */
 enum { DRX_INIT_PCF = 0, DRX_INIT_IDENTITY = 1, DRX_INIT_GIVEN = 2 };

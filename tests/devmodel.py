@@ -2,17 +2,19 @@
 
 Executes the words produced by the packer (drx_program_pack) with the bank /
 accumulator semantics of the device interpreter, using exact integers mod p
-for the residue pass and Python floats for the FP64 pass, then the per-step
-product P <- P * M exactly like the walk kernel's entry-kind matmul. Agreement
-with the oracle validates the packer's dead-code removal, operand routing and
-register allocation on the CPU.
+for the residue code and Python floats for the FP64 code, then the per-step
+product P <- P * M exactly like the walk kernel's entry-kind matmul. Folded
+constants are computed from the program's constants the way the upload does.
+Agreement with the oracle validates the packer's constant folding, shared
+inversion, dead-code removal, operand routing and register allocation on the
+CPU.
 """
 
 from __future__ import annotations
 
 from typing import List, Sequence
 
-D_ADD, D_SUB, D_MUL, D_NEG, D_SQR, D_CUB, D_INV, D_RED, D_MOV = range(1, 10)
+D_ADD, D_SUB, D_MUL, D_NEG, D_SQR, D_CUB, D_INV, D_RED, D_MOV, D_NZ1, D_ZSEL = range(1, 12)
 DB_R, DB_E, DB_S, DB_NONE = range(4)
 SB_R, SB_X, SB_C, SB_L, SB_S, SB_A, SB_Z = range(7)
 STEP_SLOT, INC_SLOT = 15, 8
@@ -32,6 +34,13 @@ def _cube(x: float) -> float:
     return float(Fraction(x) ** 3)
 
 
+LAZY_LIMIT = 1 << 40
+
+
+def _bal(v: int, p: int) -> int:
+    return (v + p // 2) % p - p // 2
+
+
 def run_program(code, fp: bool, X: List, consts: Sequence, p: int = 0, nslots: int = 0):
     """One step of the interpreter; returns the entry bank E (16 slots)."""
     R = [0] * 16 if not fp else [0.0] * 16
@@ -45,8 +54,10 @@ def run_program(code, fp: bool, X: List, consts: Sequence, p: int = 0, nslots: i
             return R[i & 15]
         if bank == SB_X:
             return X[i & 15]
-        if bank in (SB_C, SB_L):
-            return consts[i]
+        if bank == SB_C:
+            return consts[i & 15]
+        if bank == SB_L:
+            return consts[16 + i]
         if bank == SB_S:
             return S[i]
         if bank == SB_A:
@@ -57,13 +68,21 @@ def run_program(code, fp: bool, X: List, consts: Sequence, p: int = 0, nslots: i
         x = fetch(ab, a)
         y = fetch(bb, b)
         if fp:
+            assert op not in (D_NZ1, D_ZSEL), "residue-only instruction in the FP64 code"
             r = {D_ADD: lambda: x + y, D_SUB: lambda: x - y, D_MUL: lambda: x * y, D_NEG: lambda: -x,
                  D_SQR: lambda: x * x, D_CUB: lambda: _cube(x),
                  D_INV: lambda: (1.0 / x if abs(x) > 1e-300 else 0.0), D_RED: lambda: x, D_MOV: lambda: x}[op]()
         else:
-            r = {D_ADD: lambda: (x + y) % p, D_SUB: lambda: (x - y) % p, D_MUL: lambda: x * y % p,
-                 D_NEG: lambda: -x % p, D_SQR: lambda: x * x % p, D_CUB: lambda: x * x % p * x % p,
-                 D_INV: lambda: pow(x, p - 2, p), D_RED: lambda: x % p, D_MOV: lambda: x % p}[op]()
+            # lazy residues as on the device: sums stay unreduced, products and
+            # reductions are balanced; a multiply operand beyond 2^40 would
+            # break the device's exact FP64 product, so the packer must have
+            # inserted RED before that
+            if op in (D_MUL, D_SQR, D_CUB, D_INV, D_RED):
+                assert abs(x) <= LAZY_LIMIT and (op != D_MUL or abs(y) <= LAZY_LIMIT), "operand beyond 2^40"
+            r = {D_ADD: lambda: x + y, D_SUB: lambda: x - y, D_MUL: lambda: _bal(x * y, p),
+                 D_NEG: lambda: -x, D_SQR: lambda: _bal(x * x, p), D_CUB: lambda: _bal(x * x * x, p),
+                 D_INV: lambda: _bal(pow(x % p, p - 2, p), p), D_RED: lambda: _bal(x, p), D_MOV: lambda: x,
+                 D_NZ1: lambda: x if x % p else 1, D_ZSEL: lambda: 0 if y % p == 0 else x}[op]()
         if db == DB_R:
             R[d & 15] = r
         elif db == DB_E:
@@ -74,16 +93,37 @@ def run_program(code, fp: bool, X: List, consts: Sequence, p: int = 0, nslots: i
     return E
 
 
+def all_constants(program, meta, p=None):
+    """The program's constants followed by the folded ones: exact residues
+    mod p, or FP64 values when p is None."""
+    if p is None:
+        vals = [float(int(c)) for c in program.constants]
+    else:
+        vals = [int(c) % p for c in program.constants]
+    for op, a, b in meta["derived"]:
+        x, y = vals[a], (vals[b] if b >= 0 else None)
+        if p is None:
+            v = {D_ADD: lambda: x + y, D_SUB: lambda: x - y, D_MUL: lambda: x * y, D_NEG: lambda: -x,
+                 D_SQR: lambda: x * x, D_INV: lambda: (1.0 / x if abs(x) > 1e-300 else 0.0)}[op]()
+        else:
+            v = {D_ADD: lambda: (x + y) % p, D_SUB: lambda: (x - y) % p, D_MUL: lambda: x * y % p,
+                 D_NEG: lambda: -x % p, D_SQR: lambda: x * x % p, D_INV: lambda: pow(x, p - 2, p)}[op]()
+        vals.append(v)
+    return vals
+
+
 def walk(program, words, meta, const_slots, primes, axes, depth, init_res, init_f):
     """Residues (m, m, K) and M_f list per step of a walk through the model."""
     m, dim = program.m, program.dim
-    code = decode(words)
+    code = decode(words[: meta["words"]])
+    fcode = decode(words[meta["words"]:])
     kinds = meta["kinds"]
     K = len(primes)
     P = [[[int(init_res[i * m + j][k]) for j in range(m)] for i in range(m)] for k in range(K)]
     for k, p in enumerate(primes):
         p = int(p)
-        consts = [int(program.constants[ci]) % p for ci in const_slots]
+        cp = all_constants(program, meta, p)
+        consts = [cp[ci] for ci in const_slots]
         for s in range(depth):
             X = [0] * 16
             for ax in range(dim):
@@ -92,6 +132,7 @@ def walk(program, words, meta, const_slots, primes, axes, depth, init_res, init_
                 X[ax] = num % p * pow(den % p, p - 2, p) % p if den != 1 else num % p
             X[STEP_SLOT] = s % p
             E = run_program(code, False, X, consts, p, meta["nslots"])
+            assert all(abs(v) <= LAZY_LIMIT for v in E), "entry beyond 2^40"
             C = [[0] * m for _ in range(m)]
             for i in range(m):
                 for j in range(m):
@@ -106,14 +147,15 @@ def walk(program, words, meta, const_slots, primes, axes, depth, init_res, init_
             P[k] = C
     res = [[[P[k][i][j] for k in range(K)] for j in range(m)] for i in range(m)]
     mfs = []
-    fconsts = [float(int(program.constants[ci])) for ci in const_slots]
+    cf = all_constants(program, meta)
+    fconsts = [cf[ci] for ci in const_slots]
     for s in range(depth):
         X = [0.0] * 16
         for ax in range(dim):
             sn, sd, tn, td = axes[ax]
             X[ax] = float(sn * td + s * tn * sd) / float(sd * td)
         X[STEP_SLOT] = float(s)
-        E = run_program(code, True, X, fconsts, 0, meta["nslots"])
+        E = run_program(fcode, True, X, fconsts, 0, meta["fnslots"])
         mf = []
         for e in range(m * m):
             kd = (kinds >> (2 * e)) & 3

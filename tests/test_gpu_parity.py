@@ -153,6 +153,47 @@ def test_inv_of_zero_is_zero(gpu_ctx):
     assert _hex(got["q_float"]) == _hex(ref["q_float"])
 
 
+def test_shared_inversion_with_zero_arguments(gpu_ctx):
+    # several step-dependent INVs share one inversion on the device; their
+    # arguments vanish at steps 2 (x - 2), 7 (y + 3) and 4 (x*y + 24): INV(0)
+    # must stay 0 for that argument only. The constant 1/3 is folded.
+    cf = lambda mat, m, dim, axes, dirs: D.compile_cmf_from_dict(mat, m, dim, axes, dirs)
+    mat = {(0, 0): "1/(x - 2)", (0, 1): "y/3 + 1/(y + 3)", (0, 2): "1",
+           (1, 0): "x/(x*y + 24)", (1, 1): "0", (1, 2): "x - y",
+           (2, 0): "1/(x - 2) + 1/(y + 3)", (2, 1): "1", (2, 2): "x*y/(x + 100)"}
+    p = cf(mat, 3, 2, ["x", "y"], [1, 1])
+    words, _, meta = D._native.pack_program(p)
+    assert meta["inv_batched"] >= 3
+    primes = D.generate_rns_primes(64)
+    specs = [cmf_spec(p, depth, 64, [0, -10]) for depth in (3, 8, 300)]
+    with BatchPlan(specs, full=True) as plan:
+        res = plan.run()
+    for i, s in enumerate(specs):
+        rr, Pf, ls = cref.walk(p, s.depth, primes, [(0, 1, 1, 1), (-10, 1, 1, 1)])
+        assert np.array_equal(res.full_matrix(i, 3), rr), i
+        d = res.walk_dict(i)
+        assert _close(d["p_float"], Pf[0, 2]) and _close(d["q_float"], Pf[2, 2]), i
+        assert _close(d["log_scale"], ls), i
+
+
+def test_folded_constant_inverse_runs_forward_differences(gpu_ctx):
+    # a(n) = n/2 + 1/3, b(n) = -n^2/5: INVs of constants only, so the
+    # program is a polynomial in n after folding and takes the
+    # forward-difference kernel; it must equal the interpreter and the oracle
+    p = D.compile_pcf("n/2 + 1/3", "-n**2/5")
+    meta = D._native.pack_program(p)[2]
+    assert meta["fd_ok"] == 1 and meta["fd_degree"] == 2 and meta["derived"]
+    specs = [pcf_spec(p, D.pcf_initial_values("n/2 + 1/3"), d, 32) for d in (1, 9, 777)]
+    a = BatchPlan(specs, full=True).run()
+    b = BatchPlan(specs, full=True, interp_only=True).run()
+    assert np.array_equal(a.full, b.full) and np.array_equal(a.full_f, b.full_f)
+    ref = ow.run_pcf_walk(p, specs[2].a0, 777, 32, 1)
+    d = a.walk_dict(2)
+    assert d["p_residues"].tolist() == ref["p_residues"].tolist()
+    assert d["q_residues"].tolist() == ref["q_residues"].tolist()
+    assert _hex(d["p_float"]) == _hex(ref["p_float"]) and _hex(d["q_float"]) == _hex(ref["q_float"])
+
+
 def test_cmf3_shapes_against_c_oracle(gpu_ctx):
     cf = lambda mat, m, dim, axes, dirs: D.compile_cmf_from_dict(mat, m, dim, axes, dirs)
     rng = np.random.default_rng(99)

@@ -92,25 +92,33 @@ def test_long_sums_insert_reduction():
 
 
 def test_many_live_values_spill_to_lds_slots():
-    # 24 values live at once force LDS temp slots
+    # 24 values, each used by a forward sum (entry (0,1)) and by a backward
+    # product (entry (1,1)): all are live at once in any evaluation order, so
+    # both codes need LDS temp slots
     ops = [(1, 0, 0)]
     for i in range(1, 25):
-        ops.append((8, 0, i))          # r_i = x^2 (24 live values)
+        ops.append((4, 0, 0, i) if i == 1 else (4, i - 1, 0, i))   # r_i = i * x, distinct values
     acc = 1
     for i in range(2, 25):
         ops.append((4, acc, i, 100 + i))
         acc = 100 + i
+    ops.append((11, acc, 0, 1))
+    acc = 24
+    for i in range(23, 0, -1):
+        ops.append((6, acc, i, 200 + i))
+        acc = 200 + i
     ops.append((11, acc, 1, 1))
     p = _prog(2, 1, ops, [])
     _, _, meta = N.pack_program(p)
-    assert meta["nslots"] > 0
+    assert meta["nslots"] > 0 and meta["fnslots"] > 0
     _check(p, [(2, 1, 3, 1)], 3)
 
 
 def test_many_constants_go_to_lds():
+    # x * c0 * c1 * ... * c39: 40 constants referenced by step-dependent work
     consts = list(range(3, 3 + 40))
-    ops = [(2, 0, 0)]
-    acc = 0
+    ops = [(1, 0, 500), (2, 0, 0), (6, 500, 0, 501)]
+    acc = 501
     for i in range(1, 40):
         ops.append((2, i, 200 + i))
         ops.append((6, acc, 200 + i, 300 + i))
@@ -131,3 +139,58 @@ def test_packer_rejects_bad_programs():
         N.pack_program(_prog(5, 1, [], []))                 # m > 4
     with pytest.raises(ValueError):
         N.pack_program(_prog(2, 1, [(2, 5, 0)], [1]))       # constant index
+
+
+def test_constant_subexpressions_fold():
+    # c0 * c1, INV(c2), -(c0 + c1)^2 and the INV of the zero constant c3 are
+    # step-independent: folded into constants computed at upload; INV(c2)
+    # alone no longer blocks forward differences
+    ops = [(2, 0, 0), (2, 1, 1), (2, 2, 2), (2, 3, 3), (1, 0, 4),
+           (6, 0, 1, 10), (10, 2, 11), (4, 0, 1, 12), (8, 12, 13), (7, 13, 14), (10, 3, 15),
+           (6, 4, 10, 20), (4, 20, 11, 21), (11, 21, 0, 1),          # (0,1) = x c0 c1 + 1/c2
+           (4, 14, 15, 22), (6, 22, 4, 23), (11, 23, 1, 1),          # (1,1) = x (-(c0+c1)^2 + INV(0))
+           (11, 11, 1, 0)]                                            # (1,0) = 1/c2
+    p = _prog(2, 1, ops, [5, -7, 3, 0])
+    meta = _check(p, [(2, 3, 5, 7)], 4)
+    ops_folded = {d[0] for d in meta["derived"]}
+    assert {devmodel.D_MUL, devmodel.D_INV, devmodel.D_ADD, devmodel.D_SQR, devmodel.D_NEG} <= ops_folded
+    assert meta["fd_ok"] == 1 and meta["fd_degree"] == 1
+    words, _, meta = N.pack_program(p)
+    ops_left = {int(w) & 255 for w in words[: meta["words"]: 2]}
+    assert devmodel.D_INV not in ops_left
+
+
+def _inv_program(nested):
+    # 2x2 program with four step-dependent INVs: 1/(x - 2), 1/(x - 5),
+    # 1/(x^2 + 1), 1/(3x); with `nested`, the last one is 1/(1/(x-2) + x)
+    ops = [(1, 0, 0), (2, 0, 1), (2, 1, 2), (2, 2, 3), (2, 3, 4),
+           (5, 0, 1, 10), (10, 10, 11),                    # 1/(x-2)
+           (5, 0, 2, 12), (10, 12, 13),                    # 1/(x-5)
+           (8, 0, 14), (4, 14, 3, 15), (10, 15, 16)]       # 1/(x^2+1)
+    if nested:
+        ops += [(4, 11, 0, 17), (10, 17, 18)]              # 1/(1/(x-2) + x)
+    else:
+        ops += [(6, 0, 4, 17), (10, 17, 18)]               # 1/(3x)
+    ops += [(6, 11, 13, 20), (11, 20, 0, 0), (11, 16, 0, 1), (4, 18, 13, 21), (11, 21, 1, 0),
+            (5, 11, 16, 22), (11, 22, 1, 1)]
+    return _prog(2, 1, ops, [2, 5, 1, 3])
+
+
+def test_independent_inversions_share_one():
+    # steps 0..6 pass x = 2 and x = 5 (zero arguments: INV(0) = 0) and x = 0
+    # (1/(3x) of zero) through the shared inversion
+    p = _inv_program(False)
+    meta = _check(p, [(0, 1, 1, 1)], 7)
+    assert meta["inv_batched"] == 4
+    words, _, meta = N.pack_program(p)
+    code = devmodel.decode(words[: meta["words"]])
+    assert sum(1 for c in code if c[0] == devmodel.D_INV) == 1
+    assert sum(1 for c in devmodel.decode(words[meta["words"]:]) if c[0] == devmodel.D_INV) == 4
+
+
+def test_dependent_inversion_stays_separate():
+    p = _inv_program(True)
+    meta = _check(p, [(0, 1, 1, 1)], 7)
+    assert meta["inv_batched"] == 3
+    words, _, meta = N.pack_program(p)
+    assert sum(1 for c in devmodel.decode(words[: meta["words"]]) if c[0] == devmodel.D_INV) == 2
