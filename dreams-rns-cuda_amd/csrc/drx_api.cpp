@@ -107,17 +107,24 @@ int pcf_shape(const HostProgram& hp) {
   return 8 * db + da;
 }
 
+// Constant slots of the residue code beyond those held in the VGPR bank C.
+static size_t lds_const_rows(const DevProgram& pg) {
+  const uint32_t nc = (uint32_t)kResStep - pg.dim;
+  return pg.nconst > nc ? pg.nconst - nc : 0;
+}
+
 size_t walk_lds_bytes(const DevProgram& pg, int kind, int m, uint32_t K, int threads) {
   const size_t tree = (size_t)threads * m * m;  // [C][m*m][KP] with C * KP = threads
   if (kind != WALK_INTERP) return 8 * tree;
   const size_t KP = pow2_ceil(K);
-  const size_t nl = pg.nconst > (uint32_t)kBank ? pg.nconst - kBank : 0;
-  return 8 * std::max(nl * KP + (size_t)pg.nslots * threads, tree);  // the tree overlays constants and temps
+  const size_t nl = lds_const_rows(pg);
+  // constant rows, spill rows and entry rows; the tree overlays them
+  return 8 * std::max(nl * KP + ((size_t)pg.nslots + (size_t)m * m) * threads, tree);
 }
 
 size_t seed_lds_bytes(const DevProgram& pg, int kp, int seed_threads) {
-  const size_t nl = pg.nconst > (uint32_t)kBank ? pg.nconst - kBank : 0;
-  return 8 * (nl * (size_t)kp + (size_t)pg.nslots * seed_threads);
+  const size_t nl = lds_const_rows(pg);
+  return 8 * (nl * (size_t)kp + ((size_t)pg.nslots + (size_t)pg.m * pg.m) * seed_threads);
 }
 
 size_t mf_lds_bytes(const DevProgram& pg) {
@@ -531,10 +538,23 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
       lds_limit = std::min<size_t>((size_t)v, 65536);
   }
   for (auto& g : pl->stages) {
+    // the walk kernel's LDS grows with the workgroup: halve it (never below
+    // the widest prime stripe) until the stage fits; the sub-chunk split
+    // changes, the product does not
+    for (;;) {
+      size_t kp_max = 1;
+      g.lds = 0;
+      for (int si = g.span_begin; si < g.span_end; ++si) {
+        const DevWalk& dw = hw[hs[si].walk];
+        g.lds = std::max(g.lds, walk_lds_bytes(ctx->progs[dw.prog].dev, g.kind, g.m, (uint32_t)dw.K, g.threads));
+        kp_max = std::max<size_t>(kp_max, pow2_ceil((uint32_t)dw.K));
+      }
+      if (g.lds <= lds_limit || (size_t)g.threads / 2 < kp_max) break;
+      g.threads /= 2;
+    }
     for (int si = g.span_begin; si < g.span_end; ++si) {
       const DevWalk& dw = hw[hs[si].walk];
       const DevProgram& dp = ctx->progs[dw.prog].dev;
-      g.lds = std::max(g.lds, walk_lds_bytes(dp, g.kind, g.m, (uint32_t)dw.K, g.threads));
       g.mf_lds = std::max(g.mf_lds, mf_lds_bytes(dp));
       if (g.kind != WALK_INTERP)
         g.seed_lds = std::max(g.seed_lds, seed_lds_bytes(dp, g.seed_kp, fd_seed_threads(g.m, g.seed_kp)));
@@ -868,8 +888,8 @@ int drx_program_upload(drx_ctx* ctx, const uint32_t* ops, int n_ops, int m, int 
   }
   DevProgram& d = hp.dev;
   d.code_off = (uint32_t)ctx->h_code.size();
-  d.ncode = (uint32_t)(pk.code.size() / 2);
-  d.fcode_off = (uint32_t)(ctx->h_code.size() + pk.code.size());
+  d.ncode = (uint32_t)pk.code.size();
+  d.fcode_off = (uint32_t)((ctx->h_code.size() + pk.code.size() + 1) & ~(size_t)1);  // 8-byte aligned pairs
   d.nfcode = (uint32_t)(pk.fcode.size() / 2);
   d.nconst = (uint32_t)nc;
   d.kcap = (uint32_t)kc;
@@ -888,6 +908,7 @@ int drx_program_upload(drx_ctx* ctx, const uint32_t* ops, int n_ops, int m, int 
   d.fd_deg = pk.fd_deg;
   hp.fd = pk.fd_ok && ((m == 2 && pk.fd_degree <= kFdMax2) || (m == 3 && pk.fd_degree <= kFdMax3));
   ctx->h_code.insert(ctx->h_code.end(), pk.code.begin(), pk.code.end());
+  if (ctx->h_code.size() % 2) ctx->h_code.push_back(0);
   ctx->h_code.insert(ctx->h_code.end(), pk.fcode.begin(), pk.fcode.end());
   if (ctx->h_code.size() % 2) ctx->h_code.push_back(0);  // keep 8-byte alignment
   for (int s = 0; s < nc; ++s) {
@@ -897,13 +918,12 @@ int drx_program_upload(drx_ctx* ctx, const uint32_t* ops, int n_ops, int m, int 
   }
   // executed modular work per step: program + structured P*M
   int mmo = 0, mao = 0;
-  for (size_t i = 0; i < pk.code.size(); i += 2) {
-    const uint32_t op = pk.code[i] & 255u;
-    if (op == D_MUL || op == D_SQR) mmo += 1;
-    else if (op == D_CUB) mmo += 2;
-    else if (op == D_INV) mmo += 60;
-    else if (op == D_NZ1 || op == D_ZSEL) mmo += 1;  // one reduction and a select
-    else if (op == D_ADD || op == D_SUB || op == D_NEG) mao += 1;
+  for (const uint32_t w : pk.code) {
+    const uint32_t op = w & 15u;
+    if (op == RO_MUL) mmo += 1;
+    else if (op == RO_INV) mmo += 60;
+    else if (op == RO_NZ1 || op == RO_ZSEL || op == RO_RED) mmo += 1;  // one reduction and a select
+    else if (op == RO_LIN) mao += 1;
   }
   for (int j = 0; j < m; ++j) {
     int terms = 0;
@@ -972,6 +992,7 @@ int drx_program_pack(const uint32_t* ops, int n_ops, int m, int dim, const int64
   for (int i = 0; i < std::min(max_meta, DRX_PACK_META); ++i) meta[i] = mv[i];
   if (words) {
     std::vector<uint32_t> all(pk.code);
+    if (all.size() % 2) all.push_back(0);  // the FP64 code starts on an even word, as in the code pool
     all.insert(all.end(), pk.fcode.begin(), pk.fcode.end());
     std::copy(all.begin(), all.begin() + std::min<size_t>(all.size(), std::max(0, max_words)), words);
   }

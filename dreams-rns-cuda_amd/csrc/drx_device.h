@@ -19,8 +19,44 @@
 
 namespace drx {
 
-// ----------------------------------------------------------------- encoding
-// A device instruction is two 32-bit words (read by one scalar load):
+// ------------------------------------------------------- residue encoding
+// The residue code is one 32-bit word per instruction over two VGPR banks
+// per lane: R (16 temps) and C (C[15] = step, C[14 - i] = axis i,
+// C[0 .. 15 - dim) = the first constant slots). Operand x always comes from
+// R and operand y from R or C, and every result goes to R, so an instruction
+// costs two indexed register reads, one indexed write and one or two uniform
+// branches. A result that is a matrix entry is also stored to the lane's LDS
+// row of that entry.
+//   [3:0] op   [7:4] x   [8] y from C (else R)   [12:9] y   [16:13] dst
+//   [17] store to entry row [21:18]   [25:22] LIN signs (sx [23:22], sy [25:24])
+// LDM / STM / LDC carry an LDS row in [31:18] instead (and no entry store).
+// LIN computes sy * y + sx * x with each sign coded 0 -> 0, 1 -> +1, 2 -> -1
+// (ADD, SUB, reversed SUB, NEG, MOV, constant loads and zero).
+enum ResOp : uint32_t {
+  RO_MUL = 0,   // x * y
+  RO_LIN = 1,   // sy * y + sx * x
+  RO_INV = 2,   // x^(p-2)
+  RO_RED = 3,   // x mod p (balanced)
+  RO_NZ1 = 4,   // x, or 1 when x = 0 mod p
+  RO_ZSEL = 5,  // x, or 0 when y = 0 mod p
+  RO_LDM = 6,   // the lane's spill row
+  RO_STM = 7,   // spill row <- x (destination = x's own register)
+  RO_LDC = 8,   // LDS constant row (constant slot 15 - dim + row)
+};
+__host__ __device__ constexpr uint32_t enc_res(uint32_t op, uint32_t x, uint32_t y_c, uint32_t y, uint32_t d,
+                                               uint32_t e_store, uint32_t e, uint32_t sgn) {
+  return op | (x << 4) | (y_c << 8) | (y << 9) | (d << 13) | (e_store << 17) | (e << 18) | (sgn << 22);
+}
+__host__ __device__ constexpr uint32_t enc_row(uint32_t op, uint32_t x, uint32_t d, uint32_t row) {
+  return op | (x << 4) | (d << 13) | (row << 18);
+}
+constexpr uint32_t kLinZero = 0, kLinPlus = 1, kLinMinus = 2;
+__host__ __device__ constexpr uint32_t lin_signs(uint32_t sx, uint32_t sy) { return sx | (sy << 2); }
+constexpr int kResStep = 15;     // C slot of the step; axis i at C[14 - i]
+constexpr uint32_t kMaxRow = (1u << 14) - 1;
+
+// -------------------------------------------------------- FP64 encoding
+// The FP64 code (the M_f pass) is two 32-bit words per instruction:
 //   w0: [7:0] op    [9:8] dst bank   [25:10] dst index
 //   w1: [2:0] a bank [15:3] a index  [18:16] b bank  [31:19] b index
 // Every instruction also leaves its result in the accumulator A, so a value

@@ -25,69 +25,61 @@ struct Lane {
   uint32_t tid, T;   // for LDS temp slots
 };
 
-// Operand of a residue instruction (bank L: [nconst][ldc_ld] residues).
-template <bool FP>
-__device__ __forceinline__ double fetch(uint32_t bank, uint32_t i, const v16d& R, const v16d& X,
-                                        const v16d& Cv, double acc, const double* __restrict__ ldc,
-                                        uint32_t ldc_ld, const double* __restrict__ lds_s,
-                                        const Lane& ln) {
-  static_assert(!FP, "the FP64 pass has its own operand fetch");
-  switch (bank) {
-    case SB_R: return R[i & 15u];
-    case SB_X: return X[i & 15u];
-    case SB_C: return Cv[i & 15u];
-    case SB_L: return ldc[i * ldc_ld + ln.k];
-    case SB_S: return lds_s[i * ln.T + ln.tid];
-    case SB_A: return acc;
-    default: return 0.0;
-  }
+// Per-lane sign of a LIN operand: code 0 -> 0, 1 -> +1, 2 -> -1 (uniform).
+__device__ __forceinline__ double lin_sign(uint32_t code) {
+  return code == kLinZero ? 0.0 : (code == kLinPlus ? 1.0 : -1.0);
 }
 
-// One residue instruction.
-template <typename EV>
-__device__ __forceinline__ void exec_res(uint2 w, double& acc, v16d& R, EV& E, const v16d& X, const v16d& Cv,
-                                         const double* __restrict__ ldc, uint32_t ldc_ld,
-                                         double* __restrict__ lds_s, const Lane& ln) {
-  const uint32_t op = w.x & 255u;
-  const double x = fetch<false>(w.y & 7u, (w.y >> 3) & 8191u, R, X, Cv, acc, ldc, ldc_ld, lds_s, ln);
+// One residue instruction (see the encoding in drx_device.h). Bank C holds
+// the step, the axes and the first constant slots; `lc` is the LDS table of
+// further constant slots ([rows][KP], by prime), `lm` the lanes' spill rows
+// and `le` their entry rows ([rows][T], by thread).
+__device__ __forceinline__ void exec_res(uint32_t w, v16d& R, const v16d& C, const double* __restrict__ lc,
+                                         uint32_t KP, double* __restrict__ lm, double* __restrict__ le,
+                                         const Lane& ln) {
+  const uint32_t op = w & 15u;
+  const double x = R[(w >> 4) & 15u];
   double r;
-  if (op == D_MUL || op == D_ADD || op == D_SUB || op == D_ZSEL) {
-    const double y = fetch<false>((w.y >> 16) & 7u, w.y >> 19, R, X, Cv, acc, ldc, ldc_ld, lds_s, ln);
-    if (op == D_MUL) r = mm(x, y, ln.p, ln.pinv);
-    else if (op == D_ZSEL) r = (red(y, ln.p, ln.pinv) == 0.0) ? 0.0 : x;
-    else r = (op == D_ADD) ? x + y : x - y;
+  if (op <= RO_LIN) {
+    const uint32_t yi = (w >> 9) & 15u;
+    const double y = (w & 0x100u) ? C[yi] : R[yi];
+    if (op == RO_MUL) r = mm(x, y, ln.p, ln.pinv);
+    else r = __builtin_fma(y, lin_sign((w >> 24) & 3u), x * lin_sign((w >> 22) & 3u));
   } else {
+    const uint32_t row = w >> 18;
     switch (op) {
-      case D_NEG: r = -x; break;
-      case D_SQR: r = mm(x, x, ln.p, ln.pinv); break;
-      case D_CUB: r = mm(mm(x, x, ln.p, ln.pinv), x, ln.p, ln.pinv); break;
-      case D_INV: r = inv_fermat(x, ln.p, ln.pinv, ln.pe); break;
-      case D_RED: r = red(x, ln.p, ln.pinv); break;
-      case D_NZ1: r = (red(x, ln.p, ln.pinv) == 0.0) ? 1.0 : x; break;
-      default: r = x; break;  // D_MOV
+      case RO_INV: r = inv_fermat(x, ln.p, ln.pinv, ln.pe); break;
+      case RO_RED: r = red(x, ln.p, ln.pinv); break;
+      case RO_NZ1: r = (red(x, ln.p, ln.pinv) == 0.0) ? 1.0 : x; break;
+      case RO_ZSEL: {
+        const uint32_t yi = (w >> 9) & 15u;
+        const double y = (w & 0x100u) ? C[yi] : R[yi];
+        r = (red(y, ln.p, ln.pinv) == 0.0) ? 0.0 : x;
+        break;
+      }
+      case RO_LDM: r = lm[row * ln.T + ln.tid]; break;
+      case RO_STM:
+        lm[row * ln.T + ln.tid] = x;
+        r = x;
+        break;
+      default: r = lc[row * KP + ln.k]; break;  // RO_LDC
     }
   }
-  const uint32_t db = (w.x >> 8) & 3u, d = w.x >> 10;
-  if (db == DB_R) R[d & 15u] = r;
-  else if (db == DB_E) E[d & (sizeof(EV) / sizeof(double) - 1)] = r;
-  else if (db == DB_S) lds_s[d * ln.T + ln.tid] = r;
-  acc = r;
+  R[(w >> 13) & 15u] = r;
+  if (w & 0x20000u) le[((w >> 18) & 15u) * ln.T + ln.tid] = r;
 }
 
-// The packed program over residues of one prime per lane.
-template <typename EV>
-__device__ __forceinline__ void run_program(const uint2* __restrict__ code, uint32_t n, v16d& R, EV& E,
-                                            const v16d& X, const v16d& Cv, const double* __restrict__ ldc,
-                                            uint32_t ldc_ld, double* __restrict__ lds_s, const Lane& ln) {
-  double acc = 0.0;
+// The packed residue program over one prime per lane; the next instruction
+// word is loaded while the current one executes (the code pool is padded).
+__device__ __forceinline__ void run_program(const uint32_t* __restrict__ code, uint32_t n, v16d& R, const v16d& C,
+                                            const double* __restrict__ lc, uint32_t KP, double* __restrict__ lm,
+                                            double* __restrict__ le, const Lane& ln) {
+  uint32_t wn = code[0];
 #pragma unroll 1
-  for (uint32_t pc = 0; pc < n; pc += kCodeFetch) {
-    uint2 w[kCodeFetch];
-#pragma unroll
-    for (int j = 0; j < kCodeFetch; ++j) w[j] = code[pc + j];
-#pragma unroll
-    for (int j = 0; j < kCodeFetch; ++j)
-      if (pc + j < n) exec_res(w[j], acc, R, E, X, Cv, ldc, ldc_ld, lds_s, ln);
+  for (uint32_t pc = 0; pc < n; ++pc) {
+    const uint32_t w = wn;
+    wn = code[pc + 1];
+    exec_res(w, R, C, lc, KP, lm, le, ln);
   }
 }
 
@@ -146,28 +138,31 @@ __device__ __forceinline__ uint32_t pow2_at_least(uint32_t k) {
 // C sub-chunk products are then combined in order by a log2(C)-level tree in
 // LDS (the walk product is associative).
 //
-// Interpreting kernels keep the first kBank constant slots in the VGPR bank
-// C, the others in LDS ([nconst - kBank][KP] residues, bank L), and temps in
-// LDS ([nslots][T], bank S); the tree buffer [C][M*M][KP] reuses that LDS
-// once the steps are done.
+// Interpreting kernels keep the step, the axes and the first 15 - dim
+// constant slots in the VGPR bank C, further constant slots in LDS
+// ([rows][KP] residues, by prime), spilled temps and the matrix entries in
+// LDS ([rows][T] and [M*M][T], by thread); the tree buffer [C][M*M][KP]
+// reuses that LDS once the steps are done.
 
 struct ProgLds {
-  double *ldc, *lds_s;
+  double *lc, *lm, *le;
 };
 
-// Constants beyond the first kBank of `pg` into LDS, as residues for K
-// primes in a KP-wide stripe (bank L index i = constant slot kBank + i); the
-// caller synchronises.
+// Constant slots beyond those of bank C into LDS, as residues for K primes in
+// a KP-wide stripe (row i = constant slot 15 - dim + i); the caller
+// synchronises.
 __device__ __forceinline__ ProgLds prog_lds(const KArgs& a, const DevProgram& pg, uint32_t K, uint32_t KP,
                                             double* lds) {
   const uint32_t T = blockDim.x, tid = threadIdx.x;
-  const uint32_t nl = pg.nconst > (uint32_t)kBank ? pg.nconst - kBank : 0u;
+  const uint32_t nc = (uint32_t)kResStep - pg.dim;
+  const uint32_t nl = pg.nconst > nc ? pg.nconst - nc : 0u;
   ProgLds L;
-  L.ldc = lds;
-  L.lds_s = L.ldc + (uint64_t)nl * KP;
+  L.lc = lds;
+  L.lm = L.lc + (uint64_t)nl * KP;
+  L.le = L.lm + (uint64_t)pg.nslots * T;
   for (uint32_t i = tid; i < nl * KP; i += T) {
     const uint32_t ci = i / KP, ck = i % KP;
-    L.ldc[i] = (ck < K) ? a.cres[pg.cres_off + (uint64_t)(kBank + ci) * pg.kcap + ck] : 0.0;
+    L.lc[i] = (ck < K) ? a.cres[pg.cres_off + (uint64_t)(nc + ci) * pg.kcap + ck] : 0.0;
   }
   return L;
 }
@@ -180,8 +175,8 @@ struct WalkSetup {
   uint32_t K, KP, C, c, k, kk;
   bool active;
   int64_t sb, se;
-  const uint2* code;
-  double *ldc, *lds_s, *tree;
+  const uint32_t* code;
+  double *lc, *lm, *le, *tree;
 };
 
 // Task, walk and lane geometry; with `interp`, the program constants in LDS.
@@ -204,16 +199,17 @@ __device__ __forceinline__ WalkSetup walk_setup(const KArgs& a, double* lds, boo
   S.ln.k = S.k;
   S.ln.tid = tid;
   S.ln.T = T;
-  S.code = reinterpret_cast<const uint2*>(a.code + S.pg.code_off);
+  S.code = a.code + S.pg.code_off;
   // the tree buffer overlays the constants and temps, which are dead once
   // the steps are done (tree_combine synchronises before its first write)
   S.tree = lds;
   if (interp) {
     const ProgLds L = prog_lds(a, S.pg, S.K, S.KP, lds);
-    S.ldc = L.ldc;
-    S.lds_s = L.lds_s;
+    S.lc = L.lc;
+    S.lm = L.lm;
+    S.le = L.le;
   } else {
-    S.ldc = S.lds_s = nullptr;
+    S.lc = S.lm = S.le = nullptr;
   }
   __syncthreads();
   const int64_t L = S.t.s_end - S.t.s_begin;
@@ -225,31 +221,37 @@ __device__ __forceinline__ WalkSetup walk_setup(const KArgs& a, double* lds, boo
   return S;
 }
 
-// Residue constants into the VGPR bank; axis values at step s0 and per-step
-// increments into the X bank.
+// Bank C: the first constant slots, the axis values at step s0 (C[14 - i])
+// and the step s0 (C[15]); the axes' per-step increments into inc.
 __device__ __forceinline__ void load_rns_inputs(const KArgs& a, const DevWalk& w, const DevProgram& pg,
-                                                uint32_t K, uint32_t kk, const Lane& ln, int64_t s0, v16d& X,
-                                                v16d& Cv) {
+                                                uint32_t K, uint32_t kk, const Lane& ln, int64_t s0, v16d& C,
+                                                double (&inc)[kMaxDim]) {
   const double p = ln.p, pinv = ln.pinv;
+  const uint32_t nc = (uint32_t)kResStep - pg.dim;
 #pragma unroll
-  for (int i = 0; i < kBank; ++i)
-    if ((uint32_t)i < pg.nconst) Cv[i] = a.cres[pg.cres_off + (uint64_t)i * pg.kcap + kk];
+  for (int i = 0; i < kResStep; ++i)
+    if ((uint32_t)i < nc && (uint32_t)i < pg.nconst) C[i] = a.cres[pg.cres_off + (uint64_t)i * pg.kcap + kk];
 #pragma unroll
   for (int ax = 0; ax < kMaxDim; ++ax) {
+    inc[ax] = 0.0;
     if ((uint32_t)ax < pg.dim) {
       const double* q = a.axr + w.axr_off + (uint64_t)ax * 3 * K;
       const double c0 = q[kk], c1 = q[K + kk], c2 = q[2 * K + kk];
       const double x0 = red(c0 + mm((double)s0, c1, p, pinv), p, pinv);
-      X[ax] = mm(x0, c2, p, pinv);
-      X[kIncSlot + ax] = mm(c1, c2, p, pinv);
+      C[kResStep - 1 - ax] = mm(x0, c2, p, pinv);
+      inc[ax] = mm(c1, c2, p, pinv);
     }
   }
+  C[kResStep] = (double)s0;
 }
 
-__device__ __forceinline__ void advance_axes(v16d& X, uint32_t dim, double p, double pinv) {
+// Axes and step to the next step.
+__device__ __forceinline__ void advance_axes(v16d& C, const double (&inc)[kMaxDim], uint32_t dim, double p,
+                                             double pinv) {
 #pragma unroll
   for (int ax = 0; ax < kMaxDim; ++ax)
-    if ((uint32_t)ax < dim) X[ax] = red(X[ax] + X[kIncSlot + ax], p, pinv);
+    if ((uint32_t)ax < dim) C[kResStep - 1 - ax] = red(C[kResStep - 1 - ax] + inc[ax], p, pinv);
+  C[kResStep] += 1.0;
 }
 
 // Ordered tree combine of the C sub-chunk products; lane (0, k) ends with the
@@ -284,17 +286,21 @@ __global__ void __launch_bounds__(256) k_walk_chunk(const KArgs a) {
   extern __shared__ double lds[];
   const WalkSetup S = walk_setup(a, lds, true);
   const double p = S.ln.p, pinv = S.ln.pinv;
-  v16d R = 0.0, X = 0.0, Cv = 0.0;
-  typename EBank<M>::type E = 0.0;
-  load_rns_inputs(a, S.w, S.pg, S.K, S.kk, S.ln, S.sb, X, Cv);
+  v16d R = 0.0, C = 0.0;
+  double inc[kMaxDim];
+  load_rns_inputs(a, S.w, S.pg, S.K, S.kk, S.ln, S.sb, C, inc);
   double P[M * M];
 #pragma unroll
   for (int e = 0; e < M * M; ++e) P[e] = (e % (M + 1) == 0) ? 1.0 : 0.0;
+  const uint32_t kinds = S.pg.kinds;
   for (int64_t s = S.sb; s < S.se; ++s) {
-    X[kStepSlot] = (double)s;
-    run_program(S.code, S.pg.ncode, R, E, X, Cv, S.ldc, S.KP, S.lds_s, S.ln);
-    step_matmul<M>(P, E, S.pg.kinds, S.pg.colred, p, pinv);
-    advance_axes(X, S.pg.dim, p, pinv);
+    run_program(S.code, S.pg.ncode, R, C, S.lc, S.KP, S.lm, S.le, S.ln);
+    double E[M * M];
+#pragma unroll
+    for (int e = 0; e < M * M; ++e)
+      E[e] = (((kinds >> (2 * e)) & 3u) == E_GEN) ? S.le[e * S.ln.T + S.ln.tid] : 0.0;
+    step_matmul<M>(P, E, kinds, S.pg.colred, p, pinv);
+    advance_axes(C, inc, S.pg.dim, p, pinv);
   }
   tree_combine<M>(a, S, P);
 }
@@ -334,14 +340,17 @@ __global__ void __launch_bounds__(512) k_fd_seed(const KArgs a, uint32_t kp) {
   ln.T = blockDim.x;
   const ProgLds L = prog_lds(a, pg, K, KP, lds);
   __syncthreads();
-  const uint2* code = reinterpret_cast<const uint2*>(a.code + pg.code_off);
+  const uint32_t* code = a.code + pg.code_off;
   double* tab = a.fdt + w.fdt_off;
   for (uint32_t j = g; j <= (uint32_t)D; j += J) {
-    v16d R = 0.0, X = 0.0, Cv = 0.0;
-    typename EBank<M>::type E = 0.0;
-    load_rns_inputs(a, w, pg, K, kk, ln, (int64_t)j, X, Cv);
-    X[kStepSlot] = (double)j;
-    run_program(code, pg.ncode, R, E, X, Cv, L.ldc, KP, L.lds_s, ln);
+    v16d R = 0.0, C = 0.0;
+    double inc[kMaxDim];
+    load_rns_inputs(a, w, pg, K, kk, ln, (int64_t)j, C, inc);
+    run_program(code, pg.ncode, R, C, L.lc, KP, L.lm, L.le, ln);
+    double E[M * M];
+#pragma unroll
+    for (int e = 0; e < M * M; ++e)
+      E[e] = (((pg.kinds >> (2 * e)) & 3u) == E_GEN) ? L.le[e * ln.T + ln.tid] : 0.0;
     if (active) {
 #pragma unroll
       for (int e = 0; e < M * M; ++e) tab[((uint64_t)e * (D + 1) + j) * K + k] = E[e];

@@ -27,6 +27,8 @@
 
 #include <algorithm>
 #include <array>
+#include <climits>
+#include <cstdint>
 #include <cstdio>
 #include <map>
 #include <queue>
@@ -414,6 +416,182 @@ std::string encode(const CodeList& cl, const std::vector<Val>& vals, const std::
   return std::string();
 }
 
+// Residue code: lowering to the R / C instruction set with register
+// allocation over R (16 temps). Operand x must sit in R, operand y in R or C;
+// axes, the step and the first 15 - dim constant slots live in C, further
+// constants are loaded from LDS (LDC) where used, zero is made by LIN. Every
+// result goes to R; a matrix entry is also stored to its LDS row. When all
+// 16 temps are taken, the value used furthest in the future is spilled to an
+// LDS row (STM) and reloaded (LDM) before its next use.
+std::string encode_res(const CodeList& cl, const std::vector<Val>& vals, const std::vector<int>& cslot, int dim,
+                       std::vector<uint32_t>& words, int& nrows) {
+  const std::vector<Ins>& code = cl.code;
+  const int nc = (int)code.size(), nvals = (int)vals.size();
+  const int n_cbank = kResStep - dim;  // constant slots held in C
+  std::vector<std::vector<int>> uses(nvals);
+  for (int i = 0; i < nc; ++i)
+    for (int u : {code[i].a, code[i].b})
+      if (u >= 0 && vals[u].kind == V_OP) uses[u].push_back(i);
+  std::vector<size_t> next(nvals, 0);
+  auto next_use = [&](int v) { return next[v] < uses[v].size() ? uses[v][next[v]] : INT32_MAX; };
+  constexpr int kTemp = -2;
+  int held[kBank];
+  for (int& h : held) h = -1;
+  bool pinned[kBank];
+  for (bool& p : pinned) p = false;
+  std::vector<int> reg(nvals, -1), row(nvals, -1);
+  int n_rows = 0;
+  words.clear();
+  auto free_reg = [&]() -> int {
+    for (int r = 0; r < kBank; ++r)
+      if (held[r] == -1) return r;
+    int best = -1, far = -1;
+    for (int r = 0; r < kBank; ++r) {
+      if (pinned[r] || held[r] < 0) continue;
+      const int nu = next_use(held[r]);
+      if (nu > far) {
+        far = nu;
+        best = r;
+      }
+    }
+    if (best < 0) return -1;
+    const int v = held[best];
+    if (row[v] < 0) {
+      row[v] = n_rows++;
+      words.push_back(enc_row(RO_STM, (uint32_t)best, (uint32_t)best, (uint32_t)row[v]));
+    }
+    reg[v] = -1;
+    held[best] = -1;
+    return best;
+  };
+  // operand in bank C: its index, else -1
+  auto c_index = [&](int u) -> int {
+    const Val& x = vals[u];
+    if (x.kind == V_X) return kResStep - 1 - x.idx;
+    if (x.kind == V_N) return kResStep;
+    if (x.kind == V_C && cslot[x.idx] < n_cbank) return cslot[x.idx];
+    return -1;
+  };
+  // value u into a register of R (reload, constant load or zero), pinned
+  auto to_reg = [&](int u) -> int {
+    const Val& x = vals[u];
+    if (x.kind == V_OP && reg[u] >= 0) {
+      pinned[reg[u]] = true;
+      return reg[u];
+    }
+    if (x.kind == V_OP && row[u] < 0) return -1;  // not computed yet: the order is not topological
+    const int r = free_reg();
+    if (r < 0) return -1;
+    if (x.kind == V_OP) {
+      words.push_back(enc_row(RO_LDM, (uint32_t)r, (uint32_t)r, (uint32_t)row[u]));
+      reg[u] = r;
+      held[r] = u;
+    } else {
+      const int ci = c_index(u);
+      if (ci >= 0) {
+        words.push_back(enc_res(RO_LIN, (uint32_t)r, 1, (uint32_t)ci, (uint32_t)r, 0, 0, lin_signs(kLinZero, kLinPlus)));
+      } else if (x.kind == V_C) {
+        words.push_back(enc_row(RO_LDC, (uint32_t)r, (uint32_t)r, (uint32_t)(cslot[x.idx] - n_cbank)));
+      } else {
+        words.push_back(enc_res(RO_LIN, (uint32_t)r, 0, (uint32_t)r, (uint32_t)r, 0, 0, lin_signs(kLinZero, kLinZero)));
+      }
+      held[r] = kTemp;
+    }
+    pinned[r] = true;
+    return r;
+  };
+  // operand y: a slot of C (yc = 1) or a register of R
+  auto to_y = [&](int u, int& yc) -> int {
+    const int ci = c_index(u);
+    if (ci >= 0) {
+      yc = 1;
+      return ci;
+    }
+    yc = 0;
+    return to_reg(u);
+  };
+  for (int i = 0; i < nc; ++i) {
+    const Ins& in = code[i];
+    uint32_t op = RO_MUL, sgn = 0;
+    int x = 0, y = 0, yc = 0;
+    const uint32_t kind = in.op;
+    if (kind == D_MUL || kind == D_ADD || kind == D_SUB) {
+      int a = in.a, b = in.b;
+      bool swapped = false;
+      if (vals[a].kind != V_OP && vals[b].kind == V_OP) {
+        std::swap(a, b);
+        swapped = true;
+      }
+      x = to_reg(a);
+      if (x < 0) return "internal: no free register";
+      y = to_y(b, yc);
+      if (y < 0) return "internal: no free register";
+      if (kind != D_MUL) {
+        op = RO_LIN;
+        const uint32_t sx = (kind == D_SUB && swapped) ? kLinMinus : kLinPlus;
+        const uint32_t sy = (kind == D_SUB && !swapped) ? kLinMinus : kLinPlus;
+        sgn = lin_signs(sx, sy);
+      }
+    } else if (kind == D_SQR || kind == D_CUB) {
+      x = to_reg(in.a);
+      if (x < 0) return "internal: no free register";
+      y = x;
+      if (kind == D_CUB) {
+        const int t = free_reg();
+        if (t < 0) return "internal: no free register";
+        words.push_back(enc_res(RO_MUL, (uint32_t)x, 0, (uint32_t)x, (uint32_t)t, 0, 0, 0));  // t = x^2
+        held[t] = kTemp;
+        pinned[t] = true;
+        y = x;
+        x = t;  // then x^2 * x
+      }
+    } else if (kind == D_NEG || kind == D_MOV) {
+      y = to_y(in.a, yc);
+      if (y < 0) return "internal: no free register";
+      op = RO_LIN;
+      sgn = lin_signs(kLinZero, kind == D_NEG ? kLinMinus : kLinPlus);
+    } else if (kind == D_INV || kind == D_RED || kind == D_NZ1) {
+      x = to_reg(in.a);
+      if (x < 0) return "internal: no free register";
+      op = kind == D_INV ? RO_INV : (kind == D_RED ? RO_RED : RO_NZ1);
+    } else if (kind == D_ZSEL) {
+      x = to_reg(in.a);
+      if (x < 0) return "internal: no free register";
+      y = to_y(in.b, yc);
+      if (y < 0) return "internal: no free register";
+      op = RO_ZSEL;
+    } else {
+      return "internal: unknown operation";
+    }
+    // operands read: advance next uses; free temps and values used for the last time
+    for (int u : {in.a, in.b})
+      if (u >= 0 && vals[u].kind == V_OP && next[u] < uses[u].size() && uses[u][next[u]] == i) ++next[u];
+    for (int u : {in.a, in.b})
+      if (u >= 0 && vals[u].kind == V_OP && next_use(u) == INT32_MAX && reg[u] >= 0) {
+        held[reg[u]] = -1;
+        reg[u] = -1;
+      }
+    for (int r = 0; r < kBank; ++r)
+      if (held[r] == kTemp) held[r] = -1;
+    for (bool& p : pinned) p = false;
+    // destination register; an entry is also stored to its LDS row
+    int e = -1;
+    if (in.entry >= 0) e = in.entry;
+    else if (in.def >= 0 && cl.entry_direct[in.def] >= 0) e = cl.entry_direct[in.def];
+    const int d = free_reg();
+    if (d < 0) return "internal: no free register";
+    if (in.entry < 0 && in.def >= 0 && !uses[in.def].empty()) {
+      held[d] = in.def;
+      reg[in.def] = d;
+    }
+    words.push_back(enc_res(op, (uint32_t)x, (uint32_t)yc, (uint32_t)y, (uint32_t)d, e >= 0 ? 1u : 0u,
+                            e >= 0 ? (uint32_t)e : 0u, sgn));
+  }
+  if (n_rows > (int)kMaxRow) return "too many spilled values";
+  nrows = n_rows;
+  return std::string();
+}
+
 }  // namespace
 
 std::string pack_program(const uint32_t* ops, int n_ops, int m, int dim, int n_const,
@@ -653,11 +831,11 @@ std::string pack_program(const uint32_t* ops, int n_ops, int m, int dim, int n_c
       CodeList cl = build_code(cv, order, entry, pp.kinds, m, true);
       std::vector<uint32_t> scratch_words;
       int ns = 0;
-      const std::string err = encode(cl, cv, dummy_cslot, scratch_words, ns);
+      const std::string err = encode_res(cl, cv, dummy_cslot, dim, scratch_words, ns);
       if (!err.empty()) return err;
-      if (best_slots < 0 || ns < best_slots || (ns == best_slots && cl.code.size() < best_len)) {
+      if (best_slots < 0 || ns < best_slots || (ns == best_slots && scratch_words.size() < best_len)) {
         best_slots = ns;
-        best_len = cl.code.size();
+        best_len = scratch_words.size();
         rcl = std::move(cl);
         rvals = std::move(cv);
         pp.n_inv_batched = g.n_batched;
@@ -671,7 +849,7 @@ std::string pack_program(const uint32_t* ops, int n_ops, int m, int dim, int n_c
   assign_const_slots(rcl, rvals, fcl, fvals, n_const_all, cslot, pp.const_slots);
   pp.const_lds = pp.const_slots.size() > (size_t)kBank ? 1 : 0;
   if (pp.const_slots.size() > 8191 + kBank) return "too many constants";
-  std::string err = encode(rcl, rvals, cslot, pp.code, pp.nslots);
+  std::string err = encode_res(rcl, rvals, cslot, dim, pp.code, pp.nslots);
   if (!err.empty()) return err;
   err = encode(fcl, fvals, cslot, pp.fcode, pp.fnslots);
   if (!err.empty()) return err;

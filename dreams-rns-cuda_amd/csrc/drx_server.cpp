@@ -210,10 +210,22 @@ int round_once(drx_server* s, std::vector<int>& finished) {
     }
   }
   for (Stage& g : stages) {
+    // halve the walk workgroup (never below the widest prime stripe) until
+    // its LDS fits; the sub-chunk split changes, the product does not
+    for (;;) {
+      size_t kp_max = 1;
+      g.lds = 0;
+      for (int ti = g.task_begin; ti < g.task_end; ++ti) {
+        const DevWalk& dw = s->h_walks[ht[ti].walk];
+        g.lds = std::max(g.lds, walk_lds_bytes(ctx->progs[dw.prog].dev, g.kind, g.m, (uint32_t)dw.K, g.threads));
+        kp_max = std::max<size_t>(kp_max, pow2_ceil((uint32_t)dw.K));
+      }
+      if (g.lds <= 65536 || (size_t)g.threads / 2 < kp_max) break;
+      g.threads /= 2;
+    }
     for (int ti = g.task_begin; ti < g.task_end; ++ti) {
       const DevWalk& dw = s->h_walks[ht[ti].walk];
       const DevProgram& dp = ctx->progs[dw.prog].dev;
-      g.lds = std::max(g.lds, walk_lds_bytes(dp, g.kind, g.m, (uint32_t)dw.K, g.threads));
       g.mf_lds = std::max(g.mf_lds, mf_lds_bytes(dp));
       if (g.kind != WALK_INTERP)
         g.seed_lds = std::max(g.seed_lds, seed_lds_bytes(dp, g.seed_kp, fd_seed_threads(g.m, g.seed_kp)));

@@ -26,9 +26,9 @@ __host__ __device__ constexpr int fd_dmax(int m) { return m == 2 ? kFdMax2 : kFd
 
 // One packed program (device copy; offsets index the context pools).
 struct DevProgram {
-  uint32_t code_off;    // residue code: first word in the code pool (two words per instruction)
+  uint32_t code_off;    // residue code: first word in the code pool (one word per instruction)
   uint32_t ncode;       // residue code: instructions
-  uint32_t fcode_off;   // FP64 code (the M_f pass): first word in the code pool
+  uint32_t fcode_off;   // FP64 code (the M_f pass): first word (even) in the code pool, two per instruction
   uint32_t nfcode;      // FP64 code: instructions
   uint32_t nconst;      // constants referenced by the packed code
   uint32_t kcap;        // primes covered by the constant residue table
@@ -37,8 +37,8 @@ struct DevProgram {
   uint32_t m, dim;
   uint32_t kinds;       // 2 bits per entry: EntryKind
   uint32_t colred;      // bit j: reduce column j after P*M
-  uint32_t const_lds;   // 1: slots beyond the 16 of the VGPR bank C live in LDS (bank L)
-  uint32_t nslots;      // LDS temp slots (bank S) of the residue code
+  uint32_t const_lds;   // FP64 code: 1 when slots beyond the first 16 are read from bank L
+  uint32_t nslots;      // residue code: LDS spill rows per lane
   uint32_t fnslots;     // LDS temp slots (bank S) of the FP64 code
   uint32_t pad;
   uint32_t fd_ok;       // every general entry is a polynomial in the step

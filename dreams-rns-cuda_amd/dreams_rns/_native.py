@@ -292,8 +292,9 @@ def pinned_zeros(shape, dtype) -> np.ndarray:
 
 def pack_program(program):
     """Packed device code of ``program`` (no device needed): (words uint32,
-    const_slots int list, meta dict). ``words`` holds the residue code and
-    then the FP64 code (``meta["words"]`` / ``meta["fwords"]`` of them);
+    const_slots int list, meta dict). ``words`` holds the residue code (one
+    word per instruction, ``meta["words"]``) and then, from the next even
+    word, the FP64 code (two words per instruction, ``meta["fwords"]``);
     ``meta["derived"]`` lists the folded constants as (op, a, b) triples.
     Constants must fit in int64."""
     lib = load_library()
@@ -314,9 +315,10 @@ def pack_program(program):
     if rc != 0:
         raise ValueError(err.value.decode())
     nw, nfw = int(meta[0]), int(meta[8])
-    if nw + nfw > cap:
+    total = nw + (nw & 1) + nfw
+    if total > cap:
         raise RuntimeError("packed code larger than the buffer")
-    return words[: nw + nfw].copy(), [int(s) for s in slots[: int(meta[5])]], {
+    return words[:total].copy(), [int(s) for s in slots[: int(meta[5])]], {
         "words": nw, "kinds": int(meta[1]), "colred": int(meta[2]), "const_lds": int(meta[3]),
         "nslots": int(meta[4]), "nconst": int(meta[5]), "fd_ok": int(meta[6]), "fd_degree": int(meta[7]),
         "fwords": nfw, "fnslots": int(meta[9]), "inv_batched": int(meta[11]), "src_ops": int(meta[12]),

@@ -64,8 +64,9 @@ int drx_program_info(drx_ctx* ctx, int prog_id, int* n_dev_ops, int* n_const_use
 int drx_program_release_all(drx_ctx* ctx);
 
 /* Pack a program without a device (inspection and host-side testing of the
- * packer). words receives the residue code followed by the FP64 code (two
- * words per instruction each; up to max_words in total); const_slots[i] =
+ * packer). words receives the residue code (one word per instruction) and,
+ * from the next even word, the FP64 code (two words per instruction), up to
+ * max_words in total; const_slots[i] =
  * constant index of device slot i (an index >= n_const names a folded
  * constant); derived receives (op, a, b) per folded constant, its value being
  * op applied to constants a and b (b = -1 for a unary op). meta receives up to

@@ -41,13 +41,79 @@ def _bal(v: int, p: int) -> int:
     return (v + p // 2) % p - p // 2
 
 
+RO_MUL, RO_LIN, RO_INV, RO_RED, RO_NZ1, RO_ZSEL, RO_LDM, RO_STM, RO_LDC = range(9)
+RES_STEP = 15
+
+
+def decode_res(words):
+    """Residue instructions: (op, x, y_from_C, y, dst, entry_store, entry,
+    LIN signs, LDS row)."""
+    out = []
+    for w in words:
+        w = int(w)
+        out.append((w & 15, (w >> 4) & 15, (w >> 8) & 1, (w >> 9) & 15, (w >> 13) & 15, (w >> 17) & 1,
+                    (w >> 18) & 15, (w >> 22) & 15, w >> 18))
+    return out
+
+
+def _sign(code):
+    return {0: 0, 1: 1, 2: -1}[code]
+
+
+def run_res(code, C, lc, p, nrows):
+    """One step of the residue interpreter with lazy residues as on the
+    device: LIN results stay unreduced, products and reductions are
+    balanced; a multiply (or inversion) operand beyond 2^40 would break the
+    device's exact FP64 product, so the packer must have inserted RED before
+    that. C is the bank (step, axes, first constant slots), lc the further
+    constant slots. Returns the entry bank E (16 slots)."""
+    R = [0] * 16
+    E = [None] * 16
+    M = [0] * max(1, nrows)
+    for op, x, yc, y, d, es, e, sgn, aux in code:
+        if op in (RO_LDM, RO_STM, RO_LDC):
+            assert es == 0, "entry store on a row instruction"
+        xv = R[x]
+        if op in (RO_MUL, RO_LIN, RO_ZSEL):
+            yv = C[y] if yc else R[y]
+        if op == RO_MUL:
+            assert abs(xv) <= LAZY_LIMIT and abs(yv) <= LAZY_LIMIT, "operand beyond 2^40"
+            r = _bal(xv * yv, p)
+        elif op == RO_LIN:
+            r = _sign((sgn >> 2) & 3) * yv + _sign(sgn & 3) * xv
+        elif op == RO_INV:
+            assert abs(xv) <= LAZY_LIMIT, "operand beyond 2^40"
+            r = _bal(pow(xv % p, p - 2, p), p)
+        elif op == RO_RED:
+            r = _bal(xv, p)
+        elif op == RO_NZ1:
+            r = xv if xv % p else 1
+        elif op == RO_ZSEL:
+            r = 0 if yv % p == 0 else xv
+        elif op == RO_LDM:
+            r = M[aux]
+        elif op == RO_STM:
+            M[aux] = xv
+            r = xv
+        elif op == RO_LDC:
+            r = lc[aux]
+        else:
+            raise AssertionError(f"unknown residue op {op}")
+        R[d] = r
+        if es:
+            E[e] = r
+    return E
+
+
 def run_program(code, fp: bool, X: List, consts: Sequence, p: int = 0, nslots: int = 0):
-    """One step of the interpreter; returns the entry bank E (16 slots)."""
-    R = [0] * 16 if not fp else [0.0] * 16
-    S = [0] * max(1, nslots) if not fp else [0.0] * max(1, nslots)
-    E = [0] * 16 if not fp else [0.0] * 16
-    acc = 0 if not fp else 0.0
-    zero = 0 if not fp else 0.0
+    """One step of the FP64 interpreter (two-word instructions); returns the
+    entry bank E (16 slots)."""
+    assert fp
+    R = [0.0] * 16
+    S = [0.0] * max(1, nslots)
+    E = [0.0] * 16
+    acc = 0.0
+    zero = 0.0
 
     def fetch(bank, i):
         if bank == SB_R:
@@ -67,22 +133,10 @@ def run_program(code, fp: bool, X: List, consts: Sequence, p: int = 0, nslots: i
     for op, db, d, ab, a, bb, b in code:
         x = fetch(ab, a)
         y = fetch(bb, b)
-        if fp:
-            assert op not in (D_NZ1, D_ZSEL), "residue-only instruction in the FP64 code"
-            r = {D_ADD: lambda: x + y, D_SUB: lambda: x - y, D_MUL: lambda: x * y, D_NEG: lambda: -x,
-                 D_SQR: lambda: x * x, D_CUB: lambda: _cube(x),
-                 D_INV: lambda: (1.0 / x if abs(x) > 1e-300 else 0.0), D_RED: lambda: x, D_MOV: lambda: x}[op]()
-        else:
-            # lazy residues as on the device: sums stay unreduced, products and
-            # reductions are balanced; a multiply operand beyond 2^40 would
-            # break the device's exact FP64 product, so the packer must have
-            # inserted RED before that
-            if op in (D_MUL, D_SQR, D_CUB, D_INV, D_RED):
-                assert abs(x) <= LAZY_LIMIT and (op != D_MUL or abs(y) <= LAZY_LIMIT), "operand beyond 2^40"
-            r = {D_ADD: lambda: x + y, D_SUB: lambda: x - y, D_MUL: lambda: _bal(x * y, p),
-                 D_NEG: lambda: -x, D_SQR: lambda: _bal(x * x, p), D_CUB: lambda: _bal(x * x * x, p),
-                 D_INV: lambda: _bal(pow(x % p, p - 2, p), p), D_RED: lambda: _bal(x, p), D_MOV: lambda: x,
-                 D_NZ1: lambda: x if x % p else 1, D_ZSEL: lambda: 0 if y % p == 0 else x}[op]()
+        assert op not in (D_NZ1, D_ZSEL), "residue-only instruction in the FP64 code"
+        r = {D_ADD: lambda: x + y, D_SUB: lambda: x - y, D_MUL: lambda: x * y, D_NEG: lambda: -x,
+             D_SQR: lambda: x * x, D_CUB: lambda: _cube(x),
+             D_INV: lambda: (1.0 / x if abs(x) > 1e-300 else 0.0), D_RED: lambda: x, D_MOV: lambda: x}[op]()
         if db == DB_R:
             R[d & 15] = r
         elif db == DB_E:
@@ -115,8 +169,9 @@ def all_constants(program, meta, p=None):
 def walk(program, words, meta, const_slots, primes, axes, depth, init_res, init_f):
     """Residues (m, m, K) and M_f list per step of a walk through the model."""
     m, dim = program.m, program.dim
-    code = decode(words[: meta["words"]])
-    fcode = decode(words[meta["words"]:])
+    code = decode_res(words[: meta["words"]])
+    fw = meta["words"] + (meta["words"] & 1)  # the FP64 code starts on an even word
+    fcode = decode(words[fw:])
     kinds = meta["kinds"]
     K = len(primes)
     P = [[[int(init_res[i * m + j][k]) for j in range(m)] for i in range(m)] for k in range(K)]
@@ -124,15 +179,22 @@ def walk(program, words, meta, const_slots, primes, axes, depth, init_res, init_
         p = int(p)
         cp = all_constants(program, meta, p)
         consts = [cp[ci] for ci in const_slots]
+        n_cbank = RES_STEP - dim
+        lc = consts[n_cbank:]
         for s in range(depth):
-            X = [0] * 16
+            C = [0] * 16
+            for i in range(min(n_cbank, len(consts))):
+                C[i] = consts[i]
             for ax in range(dim):
                 sn, sd, tn, td = axes[ax]
                 num, den = sn * td + s * tn * sd, sd * td
-                X[ax] = num % p * pow(den % p, p - 2, p) % p if den != 1 else num % p
-            X[STEP_SLOT] = s % p
-            E = run_program(code, False, X, consts, p, meta["nslots"])
-            assert all(abs(v) <= LAZY_LIMIT for v in E), "entry beyond 2^40"
+                C[RES_STEP - 1 - ax] = num % p * pow(den % p, p - 2, p) % p if den != 1 else num % p
+            C[RES_STEP] = s
+            E = run_res(code, C, lc, p, meta["nslots"])
+            for e in range(m * m):
+                if (kinds >> (2 * e)) & 3 == 2:
+                    assert E[e] is not None, "general entry never stored"
+                    assert abs(E[e]) <= LAZY_LIMIT, "entry beyond 2^40"
             C = [[0] * m for _ in range(m)]
             for i in range(m):
                 for j in range(m):

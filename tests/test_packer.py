@@ -57,6 +57,14 @@ def test_golden_cmf_programs(golden):
         _check(p, axes, 4)
 
 
+def _res_code(words, meta):
+    return devmodel.decode_res(words[: meta["words"]])
+
+
+def _fp_code(words, meta):
+    return devmodel.decode(words[meta["words"] + (meta["words"] & 1):])
+
+
 def _prog(m, dim, ops, consts, dirs=None):
     return CmfProgram(m, dim, [Instruction(Opcode(o[0]), *o[1:]) for o in ops] + [Instruction(Opcode.END)],
                       consts, dirs or [1] * dim)
@@ -86,8 +94,8 @@ def test_long_sums_insert_reduction():
     ops.append((6, 1, 1, 3))
     ops.append((11, 3, 1, 1))
     p = _prog(2, 1, ops, [])
-    words, _, _ = N.pack_program(p)
-    assert any((int(w) & 255) == devmodel.D_RED for w in words[::2])
+    words, _, meta = N.pack_program(p)
+    assert any(c[0] == devmodel.RO_RED for c in _res_code(words, meta))
     _check(p, [(123456789, 1, 987654321, 1)], 2)
 
 
@@ -156,8 +164,7 @@ def test_constant_subexpressions_fold():
     assert {devmodel.D_MUL, devmodel.D_INV, devmodel.D_ADD, devmodel.D_SQR, devmodel.D_NEG} <= ops_folded
     assert meta["fd_ok"] == 1 and meta["fd_degree"] == 1
     words, _, meta = N.pack_program(p)
-    ops_left = {int(w) & 255 for w in words[: meta["words"]: 2]}
-    assert devmodel.D_INV not in ops_left
+    assert all(c[0] != devmodel.RO_INV for c in _res_code(words, meta))
 
 
 def _inv_program(nested):
@@ -183,9 +190,8 @@ def test_independent_inversions_share_one():
     meta = _check(p, [(0, 1, 1, 1)], 7)
     assert meta["inv_batched"] == 4
     words, _, meta = N.pack_program(p)
-    code = devmodel.decode(words[: meta["words"]])
-    assert sum(1 for c in code if c[0] == devmodel.D_INV) == 1
-    assert sum(1 for c in devmodel.decode(words[meta["words"]:]) if c[0] == devmodel.D_INV) == 4
+    assert sum(1 for c in _res_code(words, meta) if c[0] == devmodel.RO_INV) == 1
+    assert sum(1 for c in _fp_code(words, meta) if c[0] == devmodel.D_INV) == 4
 
 
 def test_dependent_inversion_stays_separate():
@@ -193,4 +199,4 @@ def test_dependent_inversion_stays_separate():
     meta = _check(p, [(0, 1, 1, 1)], 7)
     assert meta["inv_batched"] == 3
     words, _, meta = N.pack_program(p)
-    assert sum(1 for c in devmodel.decode(words[: meta["words"]]) if c[0] == devmodel.D_INV) == 2
+    assert sum(1 for c in _res_code(words, meta) if c[0] == devmodel.RO_INV) == 2
