@@ -923,6 +923,7 @@ int drx_program_upload(drx_ctx* ctx, const uint32_t* ops, int n_ops, int m, int 
     if (op == RO_MUL) mmo += 1;
     else if (op == RO_INV) mmo += 60;
     else if (op == RO_NZ1 || op == RO_ZSEL || op == RO_RED) mmo += 1;  // one reduction and a select
+    else if (op == RO_MULNZ) mmo += 2;
     else if (op == RO_LIN) mao += 1;
   }
   for (int j = 0; j < m; ++j) {

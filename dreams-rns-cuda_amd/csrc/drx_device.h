@@ -42,6 +42,7 @@ enum ResOp : uint32_t {
   RO_LDM = 6,   // the lane's spill row
   RO_STM = 7,   // spill row <- x (destination = x's own register)
   RO_LDC = 8,   // LDS constant row (constant slot 15 - dim + row)
+  RO_MULNZ = 9, // x * (y, or 1 when y = 0 mod p)
 };
 __host__ __device__ constexpr uint32_t enc_res(uint32_t op, uint32_t x, uint32_t y_c, uint32_t y, uint32_t d,
                                                uint32_t e_store, uint32_t e, uint32_t sgn) {
@@ -73,6 +74,7 @@ enum DevOp : uint32_t {
   D_MOV = 9,   // a
   D_NZ1 = 10,  // a, or 1 when a = 0 mod p (residue code only)
   D_ZSEL = 11, // a, or 0 when b = 0 mod p (residue code only)
+  D_MULNZ = 12,  // a * (b, or 1 when b = 0 mod p) (residue code only)
 };
 // Destination banks.
 enum DstBank : uint32_t { DB_R = 0, DB_E = 1, DB_S = 2, DB_NONE = 3 };

@@ -57,6 +57,12 @@ __device__ __forceinline__ void exec_res(uint32_t w, v16d& R, const v16d& C, con
         r = (red(y, ln.p, ln.pinv) == 0.0) ? 0.0 : x;
         break;
       }
+      case RO_MULNZ: {
+        const uint32_t yi = (w >> 9) & 15u;
+        const double y = (w & 0x100u) ? C[yi] : R[yi];
+        r = mm(x, (red(y, ln.p, ln.pinv) == 0.0) ? 1.0 : y, ln.p, ln.pinv);
+        break;
+      }
       case RO_LDM: r = lm[row * ln.T + ln.tid]; break;
       case RO_STM:
         lm[row * ln.T + ln.tid] = x;

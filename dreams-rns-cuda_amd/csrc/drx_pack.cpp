@@ -10,17 +10,19 @@
 //      and FP64 values are computed at upload with the same operations);
 //   3. keep only values reachable from a general entry; classify entries as
 //      zero / one / general; find the entries' degrees in the step;
-//   4. FP64 code: the remaining operations in the reference's order;
+//   4. FP64 code: the remaining operations, one reciprocal per INV;
 //   5. residue code: step-dependent INVs whose arguments do not depend on
 //      another INV share one inversion (Montgomery's trick: prefix products,
-//      one INV, back-substitution; a zero argument is replaced by 1 and its
-//      result masked to 0, so INV(0) = 0 as in the reference); RED is
-//      inserted where a sum could exceed 2^40 (the exactness bound of the
-//      FP64 residue multiply), tracking magnitude bounds per value;
-//   6. route operands: axes / step / constants read their banks directly, a
-//      value used by the very next instruction is read from the accumulator,
-//      everything else gets a register by linear scan (16 VGPR slots, then
-//      LDS slots). Both codes share one constant-slot numbering.
+//      one INV, back-substitution; a zero argument counts as 1 in the
+//      products and its result is masked to 0, so INV(0) = 0 as in the
+//      reference); RED is inserted where a sum could exceed 2^40 (the
+//      exactness bound of the FP64 residue multiply), tracking magnitude
+//      bounds per value;
+//   6. residue code: lowering to the R / C register banks (x operand in R,
+//      y in R or C, results in R, entries also stored to LDS rows) with
+//      furthest-next-use register allocation (spills to LDS rows); FP64
+//      code: operand routing over its banks and linear-scan allocation. The
+//      evaluation order is the best of a few candidates for each code.
 // Reordering independent operations changes no result: every value is the
 // same operation on the same operand values, in FP64 as in residues.
 #include "drx_pack.h"
@@ -64,7 +66,9 @@ struct Ins {            // device instruction before register assignment
 constexpr double kBoundLimit = 1099511627776.0;  // 2^40
 constexpr double kMulBound = 2147483648.0;       // 2^31
 
-bool binary_op(uint32_t op) { return op == D_ADD || op == D_SUB || op == D_MUL || op == D_ZSEL; }
+bool binary_op(uint32_t op) {
+  return op == D_ADD || op == D_SUB || op == D_MUL || op == D_ZSEL || op == D_MULNZ;
+}
 
 // A code list over a value graph, with its entry routing.
 struct CodeList {
@@ -177,10 +181,9 @@ void assign_const_slots(const CodeList& rcl, const std::vector<Val>& rvals, cons
 
 // Residue graph: the values, with every needed INV whose argument does not
 // depend on another INV replaced by its share of one inversion (Montgomery's
-// trick): z_j = NZ1(a_j), c_j = z_0 ... z_j, t = INV(c_{k-1}); going down,
-// w_j = t c_{j-1} and t <- t z_j; INV(a_j) = ZSEL(w_j, a_j). With
-// `recompute_z`, the back-substitution recomputes z_j instead of keeping it
-// live across the inversion. prio: the position in the reference's order
+// trick, zero arguments counted as 1): c_0 = NZ1(a_0), c_j = MULNZ(c_{j-1},
+// a_j), t = INV(c_{k-1}); going down, w_j = t c_{j-1} and t <- MULNZ(t, a_j);
+// INV(a_j) = ZSEL(w_j, a_j). prio: the position in the reference's order
 // (new values: just after their last argument).
 struct ResGraph {
   std::vector<Val> vals;
@@ -188,7 +191,7 @@ struct ResGraph {
   int n_batched = 0;
 };
 
-ResGraph residue_graph(const std::vector<Val>& vals, const std::vector<char>& needed, bool recompute_z) {
+ResGraph residue_graph(const std::vector<Val>& vals, const std::vector<char>& needed) {
   ResGraph g;
   g.vals = vals;
   const int nv = (int)vals.size();
@@ -214,21 +217,19 @@ ResGraph residue_graph(const std::vector<Val>& vals, const std::vector<char>& ne
     g.prio.push_back(pr);
     return (int)g.vals.size() - 1;
   };
-  std::vector<int> arg(k), z(k), c(k);
+  std::vector<int> arg(k), c(k);
   double last = 0;
   for (int j = 0; j < k; ++j) {
     arg[j] = vals[inv_vals[j]].a;
     const double pr = g.prio[arg[j]] + 0.5;
     last = std::max(last, pr);
-    z[j] = add(D_NZ1, arg[j], 0, pr);
-    c[j] = j == 0 ? z[0] : add(D_MUL, c[j - 1], z[j], pr);
+    c[j] = j == 0 ? add(D_NZ1, arg[0], 0, pr) : add(D_MULNZ, c[j - 1], arg[j], pr);
   }
   int t = add(D_INV, c[k - 1], 0, last);
   std::vector<int> w(k);
   for (int j = k - 1; j >= 1; --j) {
     w[j] = add(D_MUL, t, c[j - 1], last);
-    const int zj = recompute_z ? add(D_NZ1, arg[j], 0, last) : z[j];
-    t = add(D_MUL, t, zj, last);
+    t = add(D_MULNZ, t, arg[j], last);
   }
   w[0] = t;
   for (int j = 0; j < k; ++j) {
@@ -419,10 +420,12 @@ std::string encode(const CodeList& cl, const std::vector<Val>& vals, const std::
 // Residue code: lowering to the R / C instruction set with register
 // allocation over R (16 temps). Operand x must sit in R, operand y in R or C;
 // axes, the step and the first 15 - dim constant slots live in C, further
-// constants are loaded from LDS (LDC) where used, zero is made by LIN. Every
-// result goes to R; a matrix entry is also stored to its LDS row. When all
-// 16 temps are taken, the value used furthest in the future is spilled to an
-// LDS row (STM) and reloaded (LDM) before its next use.
+// constants are loaded from LDS (LDC), zero is made by LIN. A value brought
+// into R stays there while it has uses (an input or a constant is simply
+// dropped when its register is needed). Every result goes to R; a matrix
+// entry is also stored to its LDS row. When all 16 temps are taken, the value
+// used furthest in the future leaves R; a computed one is spilled to an LDS
+// row (STM) first and reloaded (LDM) before its next use.
 std::string encode_res(const CodeList& cl, const std::vector<Val>& vals, const std::vector<int>& cslot, int dim,
                        std::vector<uint32_t>& words, int& nrows) {
   const std::vector<Ins>& code = cl.code;
@@ -431,7 +434,7 @@ std::string encode_res(const CodeList& cl, const std::vector<Val>& vals, const s
   std::vector<std::vector<int>> uses(nvals);
   for (int i = 0; i < nc; ++i)
     for (int u : {code[i].a, code[i].b})
-      if (u >= 0 && vals[u].kind == V_OP) uses[u].push_back(i);
+      if (u >= 0) uses[u].push_back(i);
   std::vector<size_t> next(nvals, 0);
   auto next_use = [&](int v) { return next[v] < uses[v].size() ? uses[v][next[v]] : INT32_MAX; };
   constexpr int kTemp = -2;
@@ -456,7 +459,7 @@ std::string encode_res(const CodeList& cl, const std::vector<Val>& vals, const s
     }
     if (best < 0) return -1;
     const int v = held[best];
-    if (row[v] < 0) {
+    if (vals[v].kind == V_OP && row[v] < 0) {
       row[v] = n_rows++;
       words.push_back(enc_row(RO_STM, (uint32_t)best, (uint32_t)best, (uint32_t)row[v]));
     }
@@ -472,10 +475,10 @@ std::string encode_res(const CodeList& cl, const std::vector<Val>& vals, const s
     if (x.kind == V_C && cslot[x.idx] < n_cbank) return cslot[x.idx];
     return -1;
   };
-  // value u into a register of R (reload, constant load or zero), pinned
+  // value u into a register of R (reload, input copy, constant load or zero), pinned
   auto to_reg = [&](int u) -> int {
     const Val& x = vals[u];
-    if (x.kind == V_OP && reg[u] >= 0) {
+    if (reg[u] >= 0) {
       pinned[reg[u]] = true;
       return reg[u];
     }
@@ -484,8 +487,6 @@ std::string encode_res(const CodeList& cl, const std::vector<Val>& vals, const s
     if (r < 0) return -1;
     if (x.kind == V_OP) {
       words.push_back(enc_row(RO_LDM, (uint32_t)r, (uint32_t)r, (uint32_t)row[u]));
-      reg[u] = r;
-      held[r] = u;
     } else {
       const int ci = c_index(u);
       if (ci >= 0) {
@@ -495,19 +496,25 @@ std::string encode_res(const CodeList& cl, const std::vector<Val>& vals, const s
       } else {
         words.push_back(enc_res(RO_LIN, (uint32_t)r, 0, (uint32_t)r, (uint32_t)r, 0, 0, lin_signs(kLinZero, kLinZero)));
       }
-      held[r] = kTemp;
     }
+    reg[u] = r;
+    held[r] = u;
     pinned[r] = true;
     return r;
   };
-  // operand y: a slot of C (yc = 1) or a register of R
+  // operand y: a register of R already holding it, a slot of C (yc = 1), or a
+  // register it is brought into
   auto to_y = [&](int u, int& yc) -> int {
+    yc = 0;
+    if (reg[u] >= 0) {
+      pinned[reg[u]] = true;
+      return reg[u];
+    }
     const int ci = c_index(u);
     if (ci >= 0) {
       yc = 1;
       return ci;
     }
-    yc = 0;
     return to_reg(u);
   };
   for (int i = 0; i < nc; ++i) {
@@ -518,7 +525,9 @@ std::string encode_res(const CodeList& cl, const std::vector<Val>& vals, const s
     if (kind == D_MUL || kind == D_ADD || kind == D_SUB) {
       int a = in.a, b = in.b;
       bool swapped = false;
-      if (vals[a].kind != V_OP && vals[b].kind == V_OP) {
+      // x should be the operand already in R (or a computed value)
+      const bool a_r = reg[a] >= 0 || vals[a].kind == V_OP, b_r = reg[b] >= 0 || vals[b].kind == V_OP;
+      if (!a_r && b_r) {
         std::swap(a, b);
         swapped = true;
       }
@@ -554,20 +563,20 @@ std::string encode_res(const CodeList& cl, const std::vector<Val>& vals, const s
       x = to_reg(in.a);
       if (x < 0) return "internal: no free register";
       op = kind == D_INV ? RO_INV : (kind == D_RED ? RO_RED : RO_NZ1);
-    } else if (kind == D_ZSEL) {
+    } else if (kind == D_ZSEL || kind == D_MULNZ) {
       x = to_reg(in.a);
       if (x < 0) return "internal: no free register";
       y = to_y(in.b, yc);
       if (y < 0) return "internal: no free register";
-      op = RO_ZSEL;
+      op = kind == D_ZSEL ? RO_ZSEL : RO_MULNZ;
     } else {
       return "internal: unknown operation";
     }
     // operands read: advance next uses; free temps and values used for the last time
     for (int u : {in.a, in.b})
-      if (u >= 0 && vals[u].kind == V_OP && next[u] < uses[u].size() && uses[u][next[u]] == i) ++next[u];
+      if (u >= 0 && next[u] < uses[u].size() && uses[u][next[u]] == i) ++next[u];
     for (int u : {in.a, in.b})
-      if (u >= 0 && vals[u].kind == V_OP && next_use(u) == INT32_MAX && reg[u] >= 0) {
+      if (u >= 0 && next_use(u) == INT32_MAX && reg[u] >= 0) {
         held[reg[u]] = -1;
         reg[u] = -1;
       }
@@ -817,14 +826,13 @@ std::string pack_program(const uint32_t* ops, int n_ops, int m, int dim, int n_c
   }
 
   // ---- residue code: shared inversion, then the evaluation order (of a few
-  // candidates) that needs the fewest LDS temp slots, then the shortest code
+  // candidates) giving the shortest code, then the fewest LDS spill rows
   CodeList rcl;
   std::vector<Val> rvals;
   int best_slots = -1;
   size_t best_len = 0;
-  for (int recompute = 0; recompute < 2; ++recompute) {
-    const ResGraph g = residue_graph(vals, needed, recompute != 0);
-    if (g.n_batched == 0 && recompute) break;  // nothing batched: one graph only
+  {
+    const ResGraph g = residue_graph(vals, needed);
     for (int policy = 0; policy < kPolicies; ++policy) {
       std::vector<Val> cv = g.vals;
       const std::vector<int> order = schedule(cv, g.prio, roots, policy);
@@ -833,7 +841,7 @@ std::string pack_program(const uint32_t* ops, int n_ops, int m, int dim, int n_c
       int ns = 0;
       const std::string err = encode_res(cl, cv, dummy_cslot, dim, scratch_words, ns);
       if (!err.empty()) return err;
-      if (best_slots < 0 || ns < best_slots || (ns == best_slots && scratch_words.size() < best_len)) {
+      if (best_slots < 0 || scratch_words.size() < best_len || (scratch_words.size() == best_len && ns < best_slots)) {
         best_slots = ns;
         best_len = scratch_words.size();
         rcl = std::move(cl);

@@ -41,7 +41,7 @@ def _bal(v: int, p: int) -> int:
     return (v + p // 2) % p - p // 2
 
 
-RO_MUL, RO_LIN, RO_INV, RO_RED, RO_NZ1, RO_ZSEL, RO_LDM, RO_STM, RO_LDC = range(9)
+RO_MUL, RO_LIN, RO_INV, RO_RED, RO_NZ1, RO_ZSEL, RO_LDM, RO_STM, RO_LDC, RO_MULNZ = range(10)
 RES_STEP = 15
 
 
@@ -74,7 +74,7 @@ def run_res(code, C, lc, p, nrows):
         if op in (RO_LDM, RO_STM, RO_LDC):
             assert es == 0, "entry store on a row instruction"
         xv = R[x]
-        if op in (RO_MUL, RO_LIN, RO_ZSEL):
+        if op in (RO_MUL, RO_LIN, RO_ZSEL, RO_MULNZ):
             yv = C[y] if yc else R[y]
         if op == RO_MUL:
             assert abs(xv) <= LAZY_LIMIT and abs(yv) <= LAZY_LIMIT, "operand beyond 2^40"
@@ -90,6 +90,9 @@ def run_res(code, C, lc, p, nrows):
             r = xv if xv % p else 1
         elif op == RO_ZSEL:
             r = 0 if yv % p == 0 else xv
+        elif op == RO_MULNZ:
+            assert abs(xv) <= LAZY_LIMIT and abs(yv) <= LAZY_LIMIT, "operand beyond 2^40"
+            r = _bal(xv * (yv if yv % p else 1), p)
         elif op == RO_LDM:
             r = M[aux]
         elif op == RO_STM:
