@@ -95,6 +95,48 @@ int walk_kind(const HostProgram& hp, bool interp_only) {
   return pcf_shape(hp) >= 0 ? WALK_FD_PCF : WALK_FD;
 }
 
+size_t device_lds_limit(drx_ctx* ctx) {
+  int v = 0;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) == hipSuccess && v > 0)
+    return (size_t)v;
+  return 65536;
+}
+
+void merge_lds_classes(std::vector<Stage>& stages) {
+  // adjacent stages of one walk kernel run as one when the larger LDS keeps
+  // the smaller one's occupancy: a separate launch would only add a tail
+  std::vector<Stage> out;
+  for (const Stage& g : stages) {
+    if (!out.empty()) {
+      Stage& a = out.back();
+      if (a.m == g.m && a.kind == g.kind && a.kind == WALK_INTERP && a.threads == g.threads &&
+          a.task_end == g.task_begin && a.span_end == g.span_begin && a.range_end == g.range_begin &&
+          a.seed_end == g.seed_begin) {
+        const size_t lmax = std::max(a.lds, g.lds);
+        const int oa = walk_blocks_per_cu(a.m, a.kind, a.threads, std::min(a.lds, g.lds));
+        const int om = walk_blocks_per_cu(a.m, a.kind, a.threads, lmax);
+        if (oa > 0 && om == oa) {
+          a.task_end = g.task_end;
+          a.span_end = g.span_end;
+          a.range_end = g.range_end;
+          a.seed_end = g.seed_end;
+          a.lds = lmax;
+          a.mf_lds = std::max(a.mf_lds, g.mf_lds);
+          a.seed_lds = std::max(a.seed_lds, g.seed_lds);
+          a.seed_kp = std::max(a.seed_kp, g.seed_kp);
+          continue;
+        }
+      }
+    }
+    out.push_back(g);
+  }
+  stages.swap(out);
+}
+
+int lds_class(const HostProgram& hp, int kind) {
+  return kind == WALK_INTERP ? std::min<int>(3, ((int)hp.dev.nslots + 7) / 8) : 0;
+}
+
 int pcf_shape(const HostProgram& hp) {
   // companion PCF [[0, b], [1, a]]: entry (0, 0) zero, (1, 0) one; b and a of
   // any kind (a static 0 or 1 is a table of degree 0)
@@ -118,13 +160,14 @@ size_t walk_lds_bytes(const DevProgram& pg, int kind, int m, uint32_t K, int thr
   if (kind != WALK_INTERP) return 8 * tree;
   const size_t KP = pow2_ceil(K);
   const size_t nl = lds_const_rows(pg);
-  // constant rows, spill rows and entry rows; the tree overlays them
-  return 8 * std::max(nl * KP + ((size_t)pg.nslots + (size_t)m * m) * threads, tree);
+  // constant and axis-increment rows by prime, spill and entry rows by
+  // thread; the tree overlays them
+  return 8 * std::max((nl + pg.dim) * KP + ((size_t)pg.nslots + (size_t)m * m) * threads, tree);
 }
 
 size_t seed_lds_bytes(const DevProgram& pg, int kp, int seed_threads) {
   const size_t nl = lds_const_rows(pg);
-  return 8 * (nl * (size_t)kp + ((size_t)pg.nslots + (size_t)pg.m * pg.m) * seed_threads);
+  return 8 * ((nl + pg.dim) * (size_t)kp + ((size_t)pg.nslots + (size_t)pg.m * pg.m) * seed_threads);
 }
 
 size_t mf_lds_bytes(const DevProgram& pg) {
@@ -423,7 +466,54 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
   };
   const bool interp_only = pl->cfg.interp_only != 0;
   auto kind_of = [&](int i) { return walk_kind(ctx->progs[b.walks[i].prog], interp_only); };
-  auto gkey = [&](int i) { return (int)ctx->progs[b.walks[i].prog].dev.m * 4 + kind_of(i); };
+  // interpreted walks of one order: LDS classes whose LDS size would lower
+  // the occupancy of the smaller classes get stages of their own
+  std::vector<int> group(n, 0);
+  {
+    const bool plain_mode = pl->cfg.mode == DRX_MODE_PLAIN;
+    for (int m = 2; m <= kMaxM; ++m) {
+      size_t cls_lds[4] = {0, 0, 0, 0};
+      bool present[4] = {false, false, false, false};
+      int kp = 1;
+      for (int i = 0; i < n; ++i) {
+        const HostProgram& hp = ctx->progs[b.walks[i].prog];
+        if ((int)hp.dev.m == m && kind_of(i) == WALK_INTERP) kp = std::max<int>(kp, pow2_ceil(b.walks[i].K));
+      }
+      const int thr = plain_mode ? kp : std::max<int>(thr_cfg, kp);
+      for (int i = 0; i < n; ++i) {
+        const HostProgram& hp = ctx->progs[b.walks[i].prog];
+        if ((int)hp.dev.m != m || kind_of(i) != WALK_INTERP) continue;
+        const int c = lds_class(hp, WALK_INTERP);
+        present[c] = true;
+        cls_lds[c] = std::max(cls_lds[c], walk_lds_bytes(hp.dev, WALK_INTERP, m, (uint32_t)b.walks[i].K, thr));
+      }
+      int rep[4] = {0, 1, 2, 3};
+      int cur = -1;
+      size_t cur_lds = 0;
+      for (int c = 0; c < 4; ++c) {
+        if (!present[c]) continue;
+        if (cur >= 0) {
+          const size_t merged = std::max(cur_lds, cls_lds[c]);
+          const int o0 = walk_blocks_per_cu(m, WALK_INTERP, thr, cur_lds);
+          if (o0 > 0 && walk_blocks_per_cu(m, WALK_INTERP, thr, merged) == o0) {
+            rep[c] = cur;
+            cur_lds = merged;
+            continue;
+          }
+        }
+        cur = c;
+        cur_lds = cls_lds[c];
+      }
+      for (int i = 0; i < n; ++i) {
+        const HostProgram& hp = ctx->progs[b.walks[i].prog];
+        if ((int)hp.dev.m == m && kind_of(i) == WALK_INTERP) group[i] = rep[lds_class(hp, WALK_INTERP)];
+      }
+    }
+  }
+  auto gkey = [&](int i) {
+    const HostProgram& hp = ctx->progs[b.walks[i].prog];
+    return ((int)hp.dev.m * 4 + kind_of(i)) * 4 + group[i];
+  };
   auto shape_of = [&](int i) { return kind_of(i) == WALK_FD_PCF ? pcf_shape(ctx->progs[b.walks[i].prog]) : 0; };
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
     const int kx = gkey(x), ky = gkey(y);
@@ -481,10 +571,13 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
     // tasks
     const bool plain = pl->cfg.mode == DRX_MODE_PLAIN;
     const int64_t chunk = plain ? std::max<int64_t>(w.depth, 1) : chunk_cfg;
-    if (pl->stages.empty() || pl->stages.back().m != m || pl->stages.back().kind != kind) {
+    const int lclass = group[i];
+    if (pl->stages.empty() || pl->stages.back().m != m || pl->stages.back().kind != kind ||
+        pl->stages.back().lclass != lclass) {
       Stage g;
       g.m = m;
       g.kind = kind;
+      g.lclass = lclass;
       g.task_begin = g.task_end = (int)ht.size();
       g.span_begin = g.span_end = (int)hs.size();
       g.seed_begin = g.seed_end = (int)hs.size();
@@ -531,12 +624,7 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
     mat += (double)w.depth * K * hp.ma_ops;
   }
   // LDS per stage (max over its programs), checked against the device limit
-  size_t lds_limit = 65536;
-  {
-    int v = 0;
-    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) == hipSuccess && v > 0)
-      lds_limit = std::min<size_t>((size_t)v, 65536);
-  }
+  const size_t lds_limit = device_lds_limit(ctx);
   for (auto& g : pl->stages) {
     // the walk kernel's LDS grows with the workgroup: halve it (never below
     // the widest prime stripe) until the stage fits; the sub-chunk split

@@ -115,6 +115,7 @@ enum { kEvFork = 6, kEvJoin = 7, kEvWin = 12, kMaxWin = 8 };
 // One group of a launch: walks of one matrix order and walk-kernel family.
 struct Stage {
   int m = 0, kind = 0;             // WalkKind
+  int lclass = 0;                  // lds_class of its walks
   int task_begin = 0, task_end = 0;
   int span_begin = 0, span_end = 0;
   int seed_begin = 0, seed_end = 0;    // spans to seed (forward-difference stages)
@@ -147,6 +148,15 @@ size_t seed_lds_bytes(const DevProgram& pg, int kp, int seed_threads);
 size_t mf_lds_bytes(const DevProgram& pg);
 // Walk-kernel family of program `hp` (forward differences unless interp_only).
 int walk_kind(const HostProgram& hp, bool interp_only);
+// Interpreted walks are grouped by their spill rows (0, 1-8, 9-16, more),
+// so that a program that spills much does not set the LDS size (and the
+// occupancy) of every walk of its order.
+int lds_class(const HostProgram& hp, int kind);
+// Joins adjacent stages of different LDS classes whose merged LDS size keeps
+// the occupancy of the smaller.
+void merge_lds_classes(std::vector<Stage>& stages);
+// LDS one workgroup may allocate on the context's device (64 KiB if unknown).
+size_t device_lds_limit(drx_ctx* ctx);
 // Task shape for WALK_FD_PCF: 8 * deg b + deg a.
 int pcf_shape(const HostProgram& hp);
 int fail(drx_ctx* ctx, int code, const char* fmt, ...) __attribute__((format(printf, 3, 4)));

@@ -69,6 +69,9 @@ struct GarnerArgs {
 
 
 hipError_t launch_walk(int m, int kind, const KArgs& a, int ntask, int threads, size_t lds, hipStream_t s);
+// Resident workgroups per CU of the interpreting walk kernel for order m at
+// this workgroup size and LDS; -1 when unknown.
+int walk_blocks_per_cu(int m, int kind, int threads, size_t lds);
 // Newton tables of forward-difference walks: one block per span of `threads`
 // lanes, `kp` (power of two >= every K) per point group; threads % kp == 0.
 hipError_t launch_fd_seed(int m, const KArgs& a, int nspan, int kp, int threads, size_t lds, hipStream_t s);

@@ -84,6 +84,9 @@ __device__ __forceinline__ void run_program(const uint32_t* __restrict__ code, u
 #pragma unroll 1
   for (uint32_t pc = 0; pc < n; ++pc) {
     const uint32_t w = wn;
+    // the current word is complete before the next one is requested, so the
+    // wait for it does not also wait for the next load
+    __asm__ volatile("" ::"s"(w) : "memory");
     wn = code[pc + 1];
     exec_res(w, R, C, lc, KP, lm, le, ln);
   }
@@ -147,11 +150,12 @@ __device__ __forceinline__ uint32_t pow2_at_least(uint32_t k) {
 // Interpreting kernels keep the step, the axes and the first 15 - dim
 // constant slots in the VGPR bank C, further constant slots in LDS
 // ([rows][KP] residues, by prime), spilled temps and the matrix entries in
-// LDS ([rows][T] and [M*M][T], by thread); the tree buffer [C][M*M][KP]
-// reuses that LDS once the steps are done.
+// LDS ([rows][T] and [M*M][T], by thread), the axes' per-step increments in
+// LDS ([dim][KP], by prime); the tree buffer [C][M*M][KP] reuses that LDS
+// once the steps are done.
 
 struct ProgLds {
-  double *lc, *lm, *le;
+  double *lc, *lm, *le, *li;
 };
 
 // Constant slots beyond those of bank C into LDS, as residues for K primes in
@@ -166,6 +170,7 @@ __device__ __forceinline__ ProgLds prog_lds(const KArgs& a, const DevProgram& pg
   L.lc = lds;
   L.lm = L.lc + (uint64_t)nl * KP;
   L.le = L.lm + (uint64_t)pg.nslots * T;
+  L.li = L.le + (uint64_t)pg.m * pg.m * T;  // [dim][KP], by prime
   for (uint32_t i = tid; i < nl * KP; i += T) {
     const uint32_t ci = i / KP, ck = i % KP;
     L.lc[i] = (ck < K) ? a.cres[pg.cres_off + (uint64_t)(nc + ci) * pg.kcap + ck] : 0.0;
@@ -182,7 +187,7 @@ struct WalkSetup {
   bool active;
   int64_t sb, se;
   const uint32_t* code;
-  double *lc, *lm, *le, *tree;
+  double *lc, *lm, *le, *li, *tree;
 };
 
 // Task, walk and lane geometry; with `interp`, the program constants in LDS.
@@ -214,8 +219,9 @@ __device__ __forceinline__ WalkSetup walk_setup(const KArgs& a, double* lds, boo
     S.lc = L.lc;
     S.lm = L.lm;
     S.le = L.le;
+    S.li = L.li;
   } else {
-    S.lc = S.lm = S.le = nullptr;
+    S.lc = S.lm = S.le = S.li = nullptr;
   }
   __syncthreads();
   const int64_t L = S.t.s_end - S.t.s_begin;
@@ -228,10 +234,11 @@ __device__ __forceinline__ WalkSetup walk_setup(const KArgs& a, double* lds, boo
 }
 
 // Bank C: the first constant slots, the axis values at step s0 (C[14 - i])
-// and the step s0 (C[15]); the axes' per-step increments into inc.
+// and the step s0 (C[15]); with li, the axes' per-step increments into the
+// LDS table li[dim][KP] (the caller synchronises before reading it).
 __device__ __forceinline__ void load_rns_inputs(const KArgs& a, const DevWalk& w, const DevProgram& pg,
                                                 uint32_t K, uint32_t kk, const Lane& ln, int64_t s0, v16d& C,
-                                                double (&inc)[kMaxDim]) {
+                                                double* __restrict__ li, uint32_t KP) {
   const double p = ln.p, pinv = ln.pinv;
   const uint32_t nc = (uint32_t)kResStep - pg.dim;
 #pragma unroll
@@ -239,25 +246,55 @@ __device__ __forceinline__ void load_rns_inputs(const KArgs& a, const DevWalk& w
     if ((uint32_t)i < nc && (uint32_t)i < pg.nconst) C[i] = a.cres[pg.cres_off + (uint64_t)i * pg.kcap + kk];
 #pragma unroll
   for (int ax = 0; ax < kMaxDim; ++ax) {
-    inc[ax] = 0.0;
     if ((uint32_t)ax < pg.dim) {
       const double* q = a.axr + w.axr_off + (uint64_t)ax * 3 * K;
       const double c0 = q[kk], c1 = q[K + kk], c2 = q[2 * K + kk];
       const double x0 = red(c0 + mm((double)s0, c1, p, pinv), p, pinv);
       C[kResStep - 1 - ax] = mm(x0, c2, p, pinv);
-      inc[ax] = mm(c1, c2, p, pinv);
+      if (li) li[ax * KP + ln.k] = mm(c1, c2, p, pinv);
     }
   }
   C[kResStep] = (double)s0;
 }
 
-// Axes and step to the next step.
-__device__ __forceinline__ void advance_axes(v16d& C, const double (&inc)[kMaxDim], uint32_t dim, double p,
-                                             double pinv) {
-#pragma unroll
-  for (int ax = 0; ax < kMaxDim; ++ax)
-    if ((uint32_t)ax < dim) C[kResStep - 1 - ax] = red(C[kResStep - 1 - ax] + inc[ax], p, pinv);
+// Axes and step to the next step (a uniform loop over the program's axes).
+__device__ __forceinline__ void advance_axes(v16d& C, const double* __restrict__ li, uint32_t KP, uint32_t dim,
+                                             const Lane& ln) {
+#pragma unroll 1
+  for (uint32_t ax = 0; ax < dim; ++ax) {
+    const uint32_t i = (uint32_t)kResStep - 1u - ax;
+    C[i] = red(C[i] + li[ax * KP + ln.k], ln.p, ln.pinv);
+  }
   C[kResStep] += 1.0;
+}
+
+// P <- P * M with M's entries in the lane's LDS entry rows (each read once),
+// one row of P at a time; `kinds` marks the statically zero / one entries
+// (uniform branches).
+template <int M>
+__device__ __forceinline__ void step_matmul_rows(double (&P)[M * M], const double* __restrict__ le, uint32_t kinds,
+                                                 uint32_t colred, const Lane& ln) {
+  double E[M * M];
+#pragma unroll
+  for (int e = 0; e < M * M; ++e) E[e] = (((kinds >> (2 * e)) & 3u) == E_GEN) ? le[e * ln.T + ln.tid] : 0.0;
+#pragma unroll
+  for (int i = 0; i < M; ++i) {
+    double row[M];
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < M; ++k) {
+        const uint32_t kd = (kinds >> (2 * (k * M + j))) & 3u;
+        if (kd == E_GEN) acc += mm(P[i * M + k], E[k * M + j], ln.p, ln.pinv);
+        else if (kd == E_ONE) acc += P[i * M + k];
+      }
+      if ((colred >> j) & 1u) acc = red(acc, ln.p, ln.pinv);
+      row[j] = acc;
+    }
+#pragma unroll
+    for (int j = 0; j < M; ++j) P[i * M + j] = row[j];
+  }
 }
 
 // Ordered tree combine of the C sub-chunk products; lane (0, k) ends with the
@@ -293,20 +330,15 @@ __global__ void __launch_bounds__(256) k_walk_chunk(const KArgs a) {
   const WalkSetup S = walk_setup(a, lds, true);
   const double p = S.ln.p, pinv = S.ln.pinv;
   v16d R = 0.0, C = 0.0;
-  double inc[kMaxDim];
-  load_rns_inputs(a, S.w, S.pg, S.K, S.kk, S.ln, S.sb, C, inc);
+  load_rns_inputs(a, S.w, S.pg, S.K, S.kk, S.ln, S.sb, C, S.c == 0 ? S.li : nullptr, S.KP);
+  __syncthreads();
   double P[M * M];
 #pragma unroll
   for (int e = 0; e < M * M; ++e) P[e] = (e % (M + 1) == 0) ? 1.0 : 0.0;
-  const uint32_t kinds = S.pg.kinds;
   for (int64_t s = S.sb; s < S.se; ++s) {
     run_program(S.code, S.pg.ncode, R, C, S.lc, S.KP, S.lm, S.le, S.ln);
-    double E[M * M];
-#pragma unroll
-    for (int e = 0; e < M * M; ++e)
-      E[e] = (((kinds >> (2 * e)) & 3u) == E_GEN) ? S.le[e * S.ln.T + S.ln.tid] : 0.0;
-    step_matmul<M>(P, E, kinds, S.pg.colred, p, pinv);
-    advance_axes(C, inc, S.pg.dim, p, pinv);
+    step_matmul_rows<M>(P, S.le, S.pg.kinds, S.pg.colred, S.ln);
+    advance_axes(C, S.li, S.KP, S.pg.dim, S.ln);
   }
   tree_combine<M>(a, S, P);
 }
@@ -350,8 +382,7 @@ __global__ void __launch_bounds__(512) k_fd_seed(const KArgs a, uint32_t kp) {
   double* tab = a.fdt + w.fdt_off;
   for (uint32_t j = g; j <= (uint32_t)D; j += J) {
     v16d R = 0.0, C = 0.0;
-    double inc[kMaxDim];
-    load_rns_inputs(a, w, pg, K, kk, ln, (int64_t)j, C, inc);
+    load_rns_inputs(a, w, pg, K, kk, ln, (int64_t)j, C, nullptr, KP);
     run_program(code, pg.ncode, R, C, L.lc, KP, L.lm, L.le, ln);
     double E[M * M];
 #pragma unroll
@@ -1140,6 +1171,17 @@ hipError_t launch_walk(int m, int kind, const KArgs& a, int ntask, int threads, 
     else return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+int walk_blocks_per_cu(int m, int kind, int threads, size_t lds) {
+  int blocks = 0;
+  hipError_t e = hipErrorInvalidValue;
+  if (kind == WALK_INTERP) {
+    if (m == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_walk_chunk<2>, threads, lds);
+    else if (m == 3) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_walk_chunk<3>, threads, lds);
+    else if (m == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_walk_chunk<4>, threads, lds);
+  }
+  return e == hipSuccess ? blocks : -1;
 }
 
 hipError_t launch_fd_seed(int m, const KArgs& a, int nspan, int kp, int threads, size_t lds, hipStream_t s) {

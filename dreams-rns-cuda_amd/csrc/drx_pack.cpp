@@ -444,6 +444,7 @@ std::string encode_res(const CodeList& cl, const std::vector<Val>& vals, const s
   for (bool& p : pinned) p = false;
   std::vector<int> reg(nvals, -1), row(nvals, -1);
   int n_rows = 0;
+  std::vector<int> free_rows;  // rows of spilled values that are dead
   words.clear();
   auto free_reg = [&]() -> int {
     for (int r = 0; r < kBank; ++r)
@@ -460,7 +461,12 @@ std::string encode_res(const CodeList& cl, const std::vector<Val>& vals, const s
     if (best < 0) return -1;
     const int v = held[best];
     if (vals[v].kind == V_OP && row[v] < 0) {
-      row[v] = n_rows++;
+      if (!free_rows.empty()) {
+        row[v] = free_rows.back();
+        free_rows.pop_back();
+      } else {
+        row[v] = n_rows++;
+      }
       words.push_back(enc_row(RO_STM, (uint32_t)best, (uint32_t)best, (uint32_t)row[v]));
     }
     reg[v] = -1;
@@ -575,11 +581,17 @@ std::string encode_res(const CodeList& cl, const std::vector<Val>& vals, const s
     // operands read: advance next uses; free temps and values used for the last time
     for (int u : {in.a, in.b})
       if (u >= 0 && next[u] < uses[u].size() && uses[u][next[u]] == i) ++next[u];
-    for (int u : {in.a, in.b})
-      if (u >= 0 && next_use(u) == INT32_MAX && reg[u] >= 0) {
+    for (int u : {in.a, in.b}) {
+      if (u < 0 || next_use(u) != INT32_MAX) continue;
+      if (reg[u] >= 0) {
         held[reg[u]] = -1;
         reg[u] = -1;
       }
+      if (row[u] >= 0) {
+        free_rows.push_back(row[u]);
+        row[u] = -1;
+      }
+    }
     for (int r = 0; r < kBank; ++r)
       if (held[r] == kTemp) held[r] = -1;
     for (bool& p : pinned) p = false;

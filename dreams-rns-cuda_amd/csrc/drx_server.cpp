@@ -137,19 +137,20 @@ int round_once(drx_server* s, std::vector<int>& finished) {
     if (s->slots[i].used && s->slots[i].epoch != ctx->epoch)
       return fail(ctx, DRX_ESTATE, "programs were released while requests were resident");
   struct TaskRef {
-    int slot, m, kind, shape;
+    int slot, m, kind, lclass, shape;
   };
   std::vector<TaskRef> refs;
   for (int i = 0; i < s->nslots; ++i) {
     if (!s->slots[i].used) continue;
     const HostProgram& hp = ctx->progs[s->slots[i].w.prog];
     const int kind = walk_kind(hp, false);
-    refs.push_back({i, s->slots[i].m, kind, kind == WALK_FD_PCF ? pcf_shape(hp) : 0});
+    refs.push_back({i, s->slots[i].m, kind, lds_class(hp, kind), kind == WALK_FD_PCF ? pcf_shape(hp) : 0});
   }
   if (refs.empty()) return DRX_OK;
   std::stable_sort(refs.begin(), refs.end(), [](const TaskRef& a, const TaskRef& b) {
     if (a.m != b.m) return a.m < b.m;
     if (a.kind != b.kind) return a.kind < b.kind;
+    if (a.lclass != b.lclass) return a.lclass < b.lclass;
     return a.shape < b.shape;
   });
   std::vector<DevTask> ht;
@@ -170,10 +171,11 @@ int round_once(drx_server* s, std::vector<int>& finished) {
     mf_total = (mf_total + 1) & ~(uint64_t)1;  // 16-byte aligned M_f
     t.mf_off = mf_total;
     mf_total += (uint64_t)(t.s_end - t.s_begin) * r.m * r.m;
-    if (stages.empty() || stages.back().m != r.m || stages.back().kind != r.kind) {
+    if (stages.empty() || stages.back().m != r.m || stages.back().kind != r.kind || stages.back().lclass != r.lclass) {
       Stage g;
       g.m = r.m;
       g.kind = r.kind;
+      g.lclass = r.lclass;
       g.task_begin = g.task_end = g.span_begin = g.span_end = (int)ht.size();
       g.seed_begin = g.seed_end = (int)hseed.size();
       g.range_begin = g.range_end = (int)hr.size();
@@ -209,6 +211,7 @@ int round_once(drx_server* s, std::vector<int>& finished) {
       finished.push_back(r.slot);
     }
   }
+  const size_t lds_limit = device_lds_limit(ctx);
   for (Stage& g : stages) {
     // halve the walk workgroup (never below the widest prime stripe) until
     // its LDS fits; the sub-chunk split changes, the product does not
@@ -220,7 +223,7 @@ int round_once(drx_server* s, std::vector<int>& finished) {
         g.lds = std::max(g.lds, walk_lds_bytes(ctx->progs[dw.prog].dev, g.kind, g.m, (uint32_t)dw.K, g.threads));
         kp_max = std::max<size_t>(kp_max, pow2_ceil((uint32_t)dw.K));
       }
-      if (g.lds <= 65536 || (size_t)g.threads / 2 < kp_max) break;
+      if (g.lds <= lds_limit || (size_t)g.threads / 2 < kp_max) break;
       g.threads /= 2;
     }
     for (int ti = g.task_begin; ti < g.task_end; ++ti) {
@@ -230,9 +233,10 @@ int round_once(drx_server* s, std::vector<int>& finished) {
       if (g.kind != WALK_INTERP)
         g.seed_lds = std::max(g.seed_lds, seed_lds_bytes(dp, g.seed_kp, fd_seed_threads(g.m, g.seed_kp)));
     }
-    if (std::max(g.lds, std::max(g.mf_lds, g.seed_lds)) > 65536)
-      return fail(ctx, DRX_ERANGE, "server round needs more than 64 KiB of LDS");
+    if (std::max(g.lds, std::max(g.mf_lds, g.seed_lds)) > lds_limit)
+      return fail(ctx, DRX_ERANGE, "server round needs more LDS than a workgroup may have");
   }
+  merge_lds_classes(stages);
   DRX_HIPCHK(ctx, s->d_tasks.ensure(ht.size() * sizeof(DevTask)));
   DRX_HIPCHK(ctx, s->d_spans.ensure(hs.size() * sizeof(DevSpan)));
   DRX_HIPCHK(ctx, s->d_fspans.ensure(std::max<size_t>(1, hf.size()) * sizeof(DevSpan)));
