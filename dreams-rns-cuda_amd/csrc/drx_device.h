@@ -54,6 +54,10 @@ __host__ __device__ constexpr uint32_t enc_row(uint32_t op, uint32_t x, uint32_t
 constexpr uint32_t kLinZero = 0, kLinPlus = 1, kLinMinus = 2;
 __host__ __device__ constexpr uint32_t lin_signs(uint32_t sx, uint32_t sy) { return sx | (sy << 2); }
 constexpr int kResStep = 15;     // C slot of the step; axis i at C[14 - i]
+// The residue code is executed in groups of kResGroup instructions read by
+// one scalar load; the packer pads it with no-ops (R[0] <- R[0]).
+constexpr int kResGroup = 8;
+constexpr uint32_t kResNop = enc_res(RO_LIN, 0, 0, 0, 0, 0, 0, lin_signs(kLinPlus, kLinZero));
 constexpr uint32_t kMaxRow = (1u << 14) - 1;
 
 // -------------------------------------------------------- FP64 encoding

@@ -75,20 +75,18 @@ __device__ __forceinline__ void exec_res(uint32_t w, v16d& R, const v16d& C, con
   if (w & 0x20000u) le[((w >> 18) & 15u) * ln.T + ln.tid] = r;
 }
 
-// The packed residue program over one prime per lane; the next instruction
-// word is loaded while the current one executes (the code pool is padded).
+// The packed residue program over one prime per lane, kResGroup instructions
+// per scalar load (n is a multiple of kResGroup).
 __device__ __forceinline__ void run_program(const uint32_t* __restrict__ code, uint32_t n, v16d& R, const v16d& C,
                                             const double* __restrict__ lc, uint32_t KP, double* __restrict__ lm,
                                             double* __restrict__ le, const Lane& ln) {
-  uint32_t wn = code[0];
 #pragma unroll 1
-  for (uint32_t pc = 0; pc < n; ++pc) {
-    const uint32_t w = wn;
-    // the current word is complete before the next one is requested, so the
-    // wait for it does not also wait for the next load
-    __asm__ volatile("" ::"s"(w) : "memory");
-    wn = code[pc + 1];
-    exec_res(w, R, C, lc, KP, lm, le, ln);
+  for (uint32_t pc = 0; pc < n; pc += kResGroup) {
+    uint32_t w[kResGroup];
+#pragma unroll
+    for (int j = 0; j < kResGroup; ++j) w[j] = code[pc + j];
+#pragma unroll
+    for (int j = 0; j < kResGroup; ++j) exec_res(w[j], R, C, lc, KP, lm, le, ln);
   }
 }
 

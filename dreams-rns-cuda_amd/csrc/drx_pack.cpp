@@ -871,6 +871,7 @@ std::string pack_program(const uint32_t* ops, int n_ops, int m, int dim, int n_c
   if (pp.const_slots.size() > 8191 + kBank) return "too many constants";
   std::string err = encode_res(rcl, rvals, cslot, dim, pp.code, pp.nslots);
   if (!err.empty()) return err;
+  while (pp.code.size() % kResGroup) pp.code.push_back(kResNop);
   err = encode(fcl, fvals, cslot, pp.fcode, pp.fnslots);
   if (!err.empty()) return err;
   *out = std::move(pp);
