@@ -8,6 +8,7 @@ than the reference's numpy structure:
                       residues whenever no INV operand vanishes mod p
                       (runner.py:338-413, cmf_walk.py:363-431 define the
                       product; this computes it without RNS).
+* ``exact_vec_walk``  the state-vector walk v <- M v over the rationals.
 * ``modular_walk``    a per-prime, pure-Python-int bytecode interpreter.
                       Covers the INV(0) = 0 rule (runner.py:252-256: Fermat
                       ``pow(0, p-2, p) == 0``) that the rational walk cannot.
@@ -93,6 +94,17 @@ def exact_walk(program, depth: int, init, shift, traj) -> List[List[Fraction]]:
     for step in range(depth):
         P = _matmul(P, eval_exact(program, step, _axis_values(program, step, shift, traj)))
     return P
+
+
+def exact_vec_walk(program, depth: int, v0, shift, traj) -> List[Fraction]:
+    """v(depth) = M(depth-1) ... M(0) v(0) over Q (the state-vector walk,
+    cmf_walk.py:445-531, without RNS)."""
+    v = [Fraction(x) for x in v0]
+    n = len(v)
+    for step in range(depth):
+        M = eval_exact(program, step, _axis_values(program, step, shift, traj))
+        v = [sum((M[i][j] * v[j] for j in range(n)), Fraction(0)) for i in range(n)]
+    return v
 
 
 def reduce_mod(x: Fraction, p: int) -> Optional[int]:

@@ -248,6 +248,63 @@ def run_cmf_walk(program, depth: int, K: int, shift_vals, trajectory_vals=None) 
     }
 
 
+def run_cmf_walk_vec(program, depth: int, K: int, shift_vals, initial_state, acc_idx: int, const_idx: int,
+                     trajectory_vals=None) -> Dict[str, Any]:
+    """State-vector walk v(N) = M(N-1) ... M(0) v(0) with rational shift /
+    trajectory and an exact rational initial state (cmf_walk.py:445-531):
+    residues of v(0) are num * den^(p-2) mod p; per step v <- M v mod p with a
+    reduction after every term (cmf_walk.py:435-441); the shadow is
+    v_f <- M_f @ v_f, rescaled by max|v_f| when above 1e10."""
+    r = program.m
+    pp = generate_rns_primes(K)
+    ops = _ops(program)
+    ctab = const_residues(program, pp)
+    sn, sd = parse_rationals(shift_vals)
+    if trajectory_vals is not None:
+        tn, td = parse_rationals(trajectory_vals)
+    else:
+        tn = list(program.directions)
+        td = [1] * len(tn)
+    inv_cd = []
+    for s_, t_ in zip(sd, td):
+        cd = s_ * t_
+        inv_cd.append(np.ones(len(pp), dtype=np.int64) if cd == 1 else
+                      np.array([pow(cd, int(p) - 2, int(p)) for p in pp], dtype=np.int64))
+    v = np.zeros((r, len(pp)), dtype=np.int64)
+    for i in range(r):
+        f = initial_state[i]
+        if f == 0:
+            continue
+        for ki, p in enumerate(int(q) for q in pp):
+            num = f.numerator % p
+            v[i, ki] = num if f.denominator == 1 else num * pow(f.denominator, p - 2, p) % p
+    v_f = np.array([float(f) for f in initial_state], dtype=np.float64)
+    log_scale = 0.0
+    for step in range(depth):
+        axis_res, axis_f = [], []
+        for a in range(len(tn)):
+            k_num = sn[a] * td[a] + step * tn[a] * sd[a]
+            axis_res.append((np.int64(k_num) % pp * inv_cd[a]) % pp)
+            axis_f.append(float(k_num) / float(sd[a] * td[a]))
+        M = eval_rns(ops, r, step, axis_res, pp, ctab)
+        M_f = eval_float(ops, r, step, axis_f, program.constants)
+        w = np.zeros_like(v)
+        for i in range(r):
+            for j in range(r):
+                w[i] = (w[i] + M[i, j] * v[j]) % pp
+        v = w
+        v_f = M_f @ v_f
+        mx = np.max(np.abs(v_f))
+        if mx > 1e10:
+            v_f /= mx
+            log_scale += math.log(mx)
+    return {
+        "p_residues": v[acc_idx], "q_residues": v[const_idx],
+        "p_float": v_f[acc_idx], "q_float": v_f[const_idx],
+        "log_scale": log_scale, "primes": pp,
+    }
+
+
 def crt_reconstruct(residues, primes) -> Tuple[int, int]:
     """Incremental CRT: x in [0, M), M = prod(primes) (runner.py:57-76)."""
     x = int(residues[0])

@@ -172,3 +172,68 @@ def test_readme_known_answer_delta_float(golden):
     ref = golden["verify_pcf"][0]["out"]
     assert abs(float.fromhex(ref["delta_float"]) - (-0.998855)) < 5e-7
     assert ref["p_bits"] == 992
+
+
+def _zeta_rows():
+    import odd_zeta as oz
+
+    fx = oz.load()
+    progs = {name: oz.compile_spec(spec) for name, spec in fx["specs"].items()}
+    return oz, fx, progs
+
+
+def test_vec_walk_oracle_matches_reference_odd_zeta_estimates():
+    # 126 state-vector walks (zeta(5), zeta(7), zeta(9): 4x4 / 6x6 / 8x8,
+    # 6 / 9 / 12 axes, rational shifts and trajectories) at depth 50, K = 16:
+    # the reference's p_float / q_float rounded to 15 digits
+    oz, fx, progs = _zeta_rows()
+    for row in fx["results"]:
+        spec = fx["specs"][row["cmf"]]
+        v0 = oz.initial_state(spec["n"])
+        r = ow.run_cmf_walk_vec(progs[row["cmf"]], row["depth"], row["K"], oz.parse_vec(row["shift"]), v0,
+                                spec["accumulator_idx"], spec["constant_idx"], oz.parse_vec(row["trajectory"]))
+        est = r["p_float"] / r["q_float"]
+        assert abs(est - row["est"]) <= 1e-12 * abs(row["est"]) + 1e-15, (row, est)
+
+
+def test_vec_walk_oracle_residues_pinned_by_exact_rational_walk():
+    # the exact rational state vector reduced mod p equals the RNS residues
+    oz, fx, progs = _zeta_rows()
+    for row in fx["results"][::21]:
+        spec = fx["specs"][row["cmf"]]
+        v0 = oz.initial_state(spec["n"])
+        shift, traj = oz.parse_vec(row["shift"]), oz.parse_vec(row["trajectory"])
+        prog = progs[row["cmf"]]
+        r = ow.run_cmf_walk_vec(prog, 12, 8, shift, v0, spec["accumulator_idx"], spec["constant_idx"], traj)
+        v = exact.exact_vec_walk(prog, 12, v0, shift, traj)
+        for key, idx in (("p_residues", spec["accumulator_idx"]), ("q_residues", spec["constant_idx"])):
+            want = [exact.reduce_mod(v[idx], int(p)) for p in r["primes"]]
+            assert r[key].tolist() == want, (row, key)
+
+
+def test_vec_walk_oracle_is_the_transposed_matrix_walk():
+    # v(N)^T = v(0)^T M(0)^T ... M(N-1)^T: the matrix walk of the transposed
+    # program from P0 = rows v(0)^T gives the same residues and shadow bits
+    from dreams_rns.compiler import CmfProgram, Instruction, Opcode
+
+    oz, fx, progs = _zeta_rows()
+    spec = fx["specs"]["zeta_5"]
+    prog = progs["zeta_5"]
+    tops = [Instruction(i.op, i.arg0, i.arg2, i.arg1) if i.op == Opcode.STORE else i for i in prog.opcodes]
+    tprog = CmfProgram(prog.m, prog.dim, tops, prog.constants, prog.directions)
+    v0 = oz.initial_state(spec["n"])
+    shift, traj = oz.parse_vec("[2,3,2,3,1,2]"), oz.parse_vec("[1/3,2,1,2,1,1]")
+    r = ow.run_cmf_walk_vec(prog, 30, 8, shift, v0, spec["accumulator_idx"], spec["constant_idx"], traj)
+    pp = ow.generate_rns_primes(8)
+    m = prog.m
+    init_res = np.zeros((m * m, 8), dtype=np.int64)
+    for j in range(m):
+        init_res[j] = [exact.reduce_mod(v0[j], int(p)) for p in pp]   # row 0 = v0
+    init_f = np.zeros(m * m)
+    init_f[:m] = [float(x) for x in v0]
+    axes = [(s.numerator, s.denominator, t.numerator, t.denominator) for s, t in zip(shift, traj)]
+    res, Pf, ls = cref.walk(tprog, 30, pp, axes, init_res, init_f)
+    assert res[0, spec["accumulator_idx"]].tolist() == r["p_residues"].tolist()
+    assert res[0, spec["constant_idx"]].tolist() == r["q_residues"].tolist()
+    assert abs(Pf[0, spec["accumulator_idx"]] / Pf[0, spec["constant_idx"]] - r["p_float"] / r["q_float"]) <= \
+        1e-13 * abs(r["p_float"] / r["q_float"])
