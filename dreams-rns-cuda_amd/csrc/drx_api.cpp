@@ -377,12 +377,17 @@ struct drx_plan {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   std::vector<drx_plan*> plain;  // PLAIN mode: one sub-plan per walk
+  // state-vector walks: residue groups, FP64 groups, walk order
+  bool vec = false;
+  DBuf d_vgroups, d_fgroups, d_vorder;
+  int nvg = 0, nfg = 0, vthreads = 0, fthreads = 0;
+  size_t vlds = 0, flds = 0;
   ~drx_plan() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
     for (DBuf* d : {&d_walks, &d_tasks, &d_spans, &d_ranges, &d_res0, &d_fst0, &d_res, &d_fst, &d_axr, &d_axi,
                     &d_axd, &d_cp, &d_mf, &d_fdt, &d_fout, &d_rout, &d_pout, &d_pfull, &d_limbs, &d_neg, &d_topk,
-                    &d_topk_ws, &d_rowk})
+                    &d_topk_ws, &d_rowk, &d_vgroups, &d_fgroups, &d_vorder})
       d->release();
     for (drx_plan* p : plain) delete p;
   }
@@ -397,7 +402,34 @@ int upload(drx_ctx* ctx, DBuf& d, const std::vector<T>& h) {
   return DRX_OK;
 }
 
+// Vector walks: the residue kernel on the library's stream and the FP64
+// shadow on stream2 side by side; CRT after the residues, top-k after both.
+int launch_vec_sequence(drx_plan* pl, bool timed) {
+  drx_ctx* ctx = pl->ctx;
+  hipStream_t s = ctx->stream, s2 = ctx->stream2;
+  hipEvent_t* ev = ctx->ev;
+  HIPCHK(ctx, hipEventRecord(ev[kEvFork], s));
+  HIPCHK(ctx, hipStreamWaitEvent(s2, ev[kEvFork], 0));
+  HIPCHK(ctx, launch_vec_fp(pl->args, pl->nfg, pl->fthreads, pl->flds, s2));
+  HIPCHK(ctx, hipEventRecord(ev[kEvJoin], s2));
+  if (timed) HIPCHK(ctx, hipEventRecord(ev[0], s));
+  HIPCHK(ctx, launch_vec_walk(pl->args, pl->nvg, pl->vthreads, pl->vlds, s));
+  if (timed) {
+    HIPCHK(ctx, hipEventRecord(ev[1], s));
+    HIPCHK(ctx, hipEventRecord(ev[2], s));
+  }
+  if (pl->b.crt_limbs) HIPCHK(ctx, launch_garner(pl->gargs, s));
+  HIPCHK(ctx, hipStreamWaitEvent(s, ev[kEvJoin], 0));
+  if (timed) HIPCHK(ctx, hipEventRecord(ev[3], s));
+  if (pl->b.top_k > 0)
+    HIPCHK(ctx, launch_topk(pl->d_fout.as<double>() + 3, 6, pl->n, pl->b.top_k, pl->d_topk.as<int32_t>(),
+                            pl->d_topk_ws.p, s));
+  if (timed) HIPCHK(ctx, hipEventRecord(ev[4], s));
+  return DRX_OK;
+}
+
 int launch_sequence(drx_plan* pl, bool timed) {
+  if (pl->vec) return launch_vec_sequence(pl, timed);
   drx_ctx* ctx = pl->ctx;
   hipStream_t s = ctx->stream;
   int rc = launch_stages(ctx, pl->args, pl->stages, pl->args.spans, 1, timed, pl->win, pl->nwin);
@@ -417,6 +449,224 @@ int launch_sequence(drx_plan* pl, bool timed) {
   return DRX_OK;
 }
 
+// Kernel arguments shared by both plan kinds: device pools and outputs.
+void set_common_args(drx_ctx* ctx, drx_plan* pl) {
+  const drx_batch& b = pl->b;
+  KArgs& a = pl->args;
+  a.walks = pl->d_walks.as<DevWalk>();
+  a.progs = ctx->d_progs.as<DevProgram>();
+  a.invs = ctx->d_invs.as<double>();
+  a.code = ctx->d_code.as<uint32_t>();
+  a.cres = ctx->d_cres.as<double>();
+  a.cflt = ctx->d_cflt.as<double>();
+  a.prime_p = ctx->d_prime_p.as<double>();
+  a.prime_pinv = ctx->d_prime_pinv.as<double>();
+  a.prime_e = ctx->d_prime_e.as<uint32_t>();
+  a.axr = pl->d_axr.as<double>();
+  a.axi = pl->d_axi.as<int64_t>();
+  a.axd = pl->d_axd.as<double>();
+  a.res_in = pl->d_res0.as<double>();
+  a.fst_in = pl->d_fst0.as<double>();
+  a.fout = pl->d_fout.as<double>();
+  a.rout = pl->d_rout.as<int64_t>();
+  a.pout = b.full ? pl->d_pout.as<int64_t>() : nullptr;
+  a.fcompact = nullptr;
+  a.ffull = b.full_f ? pl->d_pfull.as<double>() : nullptr;
+  a.out_ld = (uint64_t)pl->ld;
+  GarnerArgs& g = pl->gargs;
+  g.res = pl->d_rout.as<int64_t>();
+  g.res_stride = (uint64_t)pl->ld;
+  g.n = b.crt_limbs ? 2 * pl->n : 0;
+  g.K = pl->kmax;
+  g.row_k = b.crt_limbs ? pl->d_rowk.as<int32_t>() : nullptr;
+  g.prime_p = ctx->d_prime_p.as<double>();
+  g.prime_pinv = ctx->d_prime_pinv.as<double>();
+  g.prime_u = ctx->d_prime_u.as<uint32_t>();
+  g.inv_table = ctx->d_inv.as<uint32_t>();
+  g.inv_ld = (uint64_t)ctx->kinv;
+  g.limbs = pl->d_limbs.as<uint32_t>();
+  g.neg = pl->d_neg.as<uint8_t>();
+}
+
+// Output buffers shared by both plan kinds.
+int alloc_outputs(drx_ctx* ctx, drx_plan* pl) {
+  const drx_batch& b = pl->b;
+  const int n = pl->n;
+  HIPCHK(ctx, pl->d_fout.ensure((size_t)std::max(1, n) * 6 * 8));
+  HIPCHK(ctx, pl->d_rout.ensure((size_t)std::max(1, n) * 2 * pl->ld * 8));
+  if (b.full) HIPCHK(ctx, pl->d_pout.ensure((size_t)std::max(1, n) * 16 * pl->ld * 8));
+  if (b.full_f) HIPCHK(ctx, pl->d_pfull.ensure((size_t)std::max(1, n) * 16 * 8));
+  if (b.top_k > 0) {
+    if (b.top_k > n) return fail(ctx, DRX_EINVAL, "top_k=%d > n=%d", b.top_k, n);
+    HIPCHK(ctx, pl->d_topk.ensure((size_t)b.top_k * 4));
+    HIPCHK(ctx, pl->d_topk_ws.ensure(std::max<size_t>(16, topk_workspace_bytes(n, b.top_k))));
+  }
+  if (b.crt_limbs) {
+    int rc;
+    if ((rc = ensure_inv_table(ctx, pl->kmax))) return rc;
+    std::vector<int32_t> rowk(2 * (size_t)n);
+    for (int i = 0; i < n; ++i) rowk[2 * i] = rowk[2 * i + 1] = b.walks[i].K;
+    if ((rc = upload(ctx, pl->d_rowk, rowk))) return rc;
+    HIPCHK(ctx, pl->d_limbs.ensure((size_t)std::max(1, n) * 2 * pl->ld * 4));
+    HIPCHK(ctx, pl->d_neg.ensure((size_t)std::max(1, n) * 2));
+  }
+  return DRX_OK;
+}
+
+// State-vector walks (cmf_walk.py:445-531). Walks are ordered by (program,
+// depth, K); consecutive walks sharing them form a residue group of
+// T / KP walks (one lane per walk and prime) and an FP64 group of up to
+// 256 walks (one lane per walk), so each workgroup interprets one program for
+// one trip count.
+int plan_build_vec(drx_ctx* ctx, drx_plan* pl) {
+  const drx_batch& b = pl->b;
+  const int n = pl->n;
+  pl->vec = true;
+  const int thr = pl->cfg.threads > 0 ? pl->cfg.threads : 256;
+  if ((thr & (thr - 1)) || thr < 32 || thr > 256)
+    return fail(ctx, DRX_EINVAL, "threads=%d must be a power of two in [32, 256]", thr);
+  std::vector<int> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    const drx_walk &wx = b.walks[x], &wy = b.walks[y];
+    if (wx.prog != wy.prog) return wx.prog < wy.prog;
+    if (wx.depth != wy.depth) return wx.depth > wy.depth;
+    return wx.K < wy.K;
+  });
+  std::vector<DevWalk> hw(n);
+  std::vector<double> res0, fst0, axr, axd;
+  std::vector<int64_t> axi;
+  double rsteps = 0, mmt = 0, mat = 0;
+  int kpmax = 1;
+  for (int i = 0; i < n; ++i) {
+    const drx_walk& w = b.walks[i];
+    const HostProgram& hp = ctx->progs[w.prog];
+    const int m = (int)hp.dev.m, dim = (int)hp.dev.dim, K = w.K;
+    if (w.init_kind != DRX_INIT_GIVEN) return fail(ctx, DRX_EINVAL, "walk %d: a vector walk needs a given state", i);
+    if (w.out_p < 0 || w.out_p >= m || w.out_q < 0 || w.out_q >= m)
+      return fail(ctx, DRX_EINVAL, "walk %d: state entries %d, %d outside [0, %d)", i, w.out_p, w.out_q, m);
+    DevWalk& dw = hw[i];
+    dw.prog = w.prog;
+    dw.K = K;
+    dw.m = m;
+    dw.dim = dim;
+    dw.pe = w.out_p;
+    dw.qe = w.out_q;
+    dw.target = w.target;
+    dw.res_off = res0.size();
+    dw.fst_off = fst0.size();
+    dw.axr_off = axr.size();
+    dw.axi_off = axi.size();
+    dw.axd_off = axd.size();
+    for (int j = 0; j < m; ++j) {
+      for (int k = 0; k < K; ++k) {
+        const uint32_t v = b.init_res_pool[w.init_off + (int64_t)j * K + k];
+        if (v >= ctx->primes[k]) return fail(ctx, DRX_EINVAL, "walk %d: initial residue not canonical", i);
+        res0.push_back((double)v);
+      }
+      fst0.push_back(b.init_f_pool[w.init_f_off + j]);
+    }
+    const size_t a0 = axr.size(), i0 = axi.size(), d0 = axd.size();
+    axr.resize(a0 + (size_t)dim * 3 * K);
+    axi.resize(i0 + (size_t)dim * 2);
+    axd.resize(d0 + (size_t)dim);
+    if (dim > 0) {
+      const int rc0 = axis_tables(ctx, b.axis_pool + w.axis_off, dim, K, w.depth, axr.data() + a0, axi.data() + i0,
+                                  axd.data() + d0);
+      if (rc0) return fail(ctx, rc0, "walk %d: %s", i, std::string(ctx->err).c_str());
+    }
+    kpmax = std::max<int>(kpmax, pow2_ceil(K));
+    rsteps += (double)w.depth * K;
+    mmt += (double)w.depth * K * hp.mm_ops;
+    mat += (double)w.depth * K * hp.ma_ops;
+  }
+  if (kpmax > thr) return fail(ctx, DRX_EINVAL, "threads=%d below the %d lanes of one walk's primes", thr, kpmax);
+  // groups
+  std::vector<DevVGroup> vg, fg;
+  std::vector<int32_t> vorder(order.begin(), order.end());
+  for (int a0 = 0; a0 < n;) {
+    const drx_walk& w = b.walks[order[a0]];
+    int e = a0;
+    while (e < n && b.walks[order[e]].prog == w.prog && b.walks[order[e]].depth == w.depth) ++e;
+    for (int f = a0; f < e; f += 256) {  // FP64 groups: one lane per walk
+      DevVGroup g{};
+      g.first = f;
+      g.n = std::min(256, e - f);
+      g.prog = w.prog;
+      g.K = 0;
+      g.depth = w.depth;
+      fg.push_back(g);
+    }
+    for (int f = a0; f < e;) {  // residue groups: equal K, thr / KP walks
+      const int K = b.walks[order[f]].K;
+      const int per = thr / (int)pow2_ceil(K);
+      int c = f;
+      while (c < e && c - f < per && b.walks[order[c]].K == K) ++c;
+      DevVGroup g{};
+      g.first = f;
+      g.n = c - f;
+      g.prog = w.prog;
+      g.K = K;
+      g.depth = w.depth;
+      vg.push_back(g);
+      f = c;
+    }
+    a0 = e;
+  }
+  // LDS: residue groups (constant rows by prime; spill, state A/B and
+  // increment rows by thread); FP64 groups (constants, temps)
+  const size_t lds_limit = device_lds_limit(ctx);
+  size_t vlds = 8, flds = 8;
+  for (const DevVGroup& g : vg) {
+    const DevProgram& dp = ctx->progs[g.prog].dev;
+    const size_t nl = lds_const_rows(dp);
+    vlds = std::max(vlds, 8 * (nl * pow2_ceil(g.K) + ((size_t)dp.nslots + 2 * dp.m + dp.dim) * (size_t)thr));
+  }
+  for (const DevVGroup& g : fg) {
+    const DevProgram& dp = ctx->progs[g.prog].dev;
+    flds = std::max(flds, 8 * ((size_t)dp.nconst + (size_t)dp.fnslots * 256));
+  }
+  if (vlds > lds_limit || flds > lds_limit)
+    return fail(ctx, DRX_ERANGE, "vector walks need %zu / %zu bytes of LDS (limit %zu)", vlds, flds, lds_limit);
+  pl->nvg = (int)vg.size();
+  pl->nfg = (int)fg.size();
+  pl->vthreads = thr;
+  pl->fthreads = 256;
+  pl->vlds = vlds;
+  pl->flds = flds;
+  pl->residue_steps = rsteps;
+  pl->mm_total = mmt;
+  pl->ma_total = mat;
+  pl->ntask = pl->nvg;
+  int rc;
+  if ((rc = upload(ctx, pl->d_walks, hw)) || (rc = upload(ctx, pl->d_res0, res0)) || (rc = upload(ctx, pl->d_fst0, fst0)) ||
+      (rc = upload(ctx, pl->d_axr, axr)) || (rc = upload(ctx, pl->d_axi, axi)) || (rc = upload(ctx, pl->d_axd, axd)) ||
+      (rc = upload(ctx, pl->d_vgroups, vg)) || (rc = upload(ctx, pl->d_fgroups, fg)) ||
+      (rc = upload(ctx, pl->d_vorder, vorder)))
+    return rc;
+  if ((rc = alloc_outputs(ctx, pl))) return rc;
+  if ((rc = upload_programs(ctx))) return rc;
+  set_common_args(ctx, pl);
+  pl->args.vgroups = pl->d_vgroups.as<DevVGroup>();
+  pl->args.fgroups = pl->d_fgroups.as<DevVGroup>();
+  pl->args.vorder = pl->d_vorder.as<int32_t>();
+  for (int i = 0; i < n; ++i) pl->walk_order.push_back(i);
+  if (pl->cfg.use_graph) {
+    HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const int rc2 = launch_sequence(pl, false);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (rc2) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc2;
+    }
+    if (e != hipSuccess) return fail(ctx, DRX_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    pl->graph = graph;
+    HIPCHK(ctx, hipGraphInstantiate(&pl->gexec, graph, nullptr, nullptr, 0));
+  }
+  return DRX_OK;
+}
+
 // The walk kernel's task array is laid out per walk in step order; stages
 // hold one matrix order and walk-kernel family each; within a stage walks run
 // longest first.
@@ -425,7 +675,7 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
   const int n = b.n;
   const int np = (int)ctx->progs.size();
   const int K_ctx = (int)ctx->primes.size();
-  int kmax = 1;
+  int kmax = 1, nvec = 0;
   for (int i = 0; i < n; ++i) {
     const drx_walk& w = b.walks[i];
     if (w.prog < 0 || w.prog >= np) return fail(ctx, DRX_EINVAL, "walk %d: bad program id %d", i, w.prog);
@@ -444,12 +694,18 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
       return fail(ctx, DRX_EINVAL, "walk %d: DRX_INIT_GIVEN without init pools", i);
     if (hp.dev.dim > 0 && !b.axis_pool) return fail(ctx, DRX_EINVAL, "walk %d: no axis pool", i);
     if (pow2_ceil(w.K) > 256) return fail(ctx, DRX_ERANGE, "walk %d: K=%d above the 256 primes of one workgroup", i, w.K);
+    if (w.mode != DRX_WALK_MATRIX && w.mode != DRX_WALK_VECTOR) return fail(ctx, DRX_EINVAL, "walk %d: bad mode", i);
+    if (w.mode != hp.mode)
+      return fail(ctx, DRX_EINVAL, "walk %d: program %d was uploaded for the other walk mode", i, w.prog);
+    nvec += w.mode == DRX_WALK_VECTOR;
     kmax = std::max(kmax, w.K);
   }
   if (b.res_stride < kmax) return fail(ctx, DRX_EINVAL, "res_stride %d < max K %d", b.res_stride, kmax);
   pl->n = n;
   pl->kmax = kmax;
   pl->ld = b.res_stride;
+  if (nvec > 0 && nvec < n) return fail(ctx, DRX_EINVAL, "a batch holds walks of one mode (matrix or vector)");
+  if (nvec > 0) return plan_build_vec(ctx, pl);
 
   const int chunk_cfg = pl->cfg.chunk_steps > 0 ? pl->cfg.chunk_steps : 4096;
   const int thr_cfg = pl->cfg.threads > 0 ? pl->cfg.threads : 256;
@@ -901,7 +1157,15 @@ int drx_set_primes(drx_ctx* ctx, const int64_t* primes, int K) {
 
 int drx_program_upload(drx_ctx* ctx, const uint32_t* ops, int n_ops, int m, int dim, const int64_t* consts,
                        const uint32_t* const_res, const double* const_f, int n_const, int kcap, int* prog_id) {
+  return drx_program_upload_mode(ctx, ops, n_ops, m, dim, consts, const_res, const_f, n_const, kcap,
+                                 DRX_WALK_MATRIX, prog_id);
+}
+
+int drx_program_upload_mode(drx_ctx* ctx, const uint32_t* ops, int n_ops, int m, int dim, const int64_t* consts,
+                            const uint32_t* const_res, const double* const_f, int n_const, int kcap, int mode,
+                            int* prog_id) {
   if (!ctx) return DRX_EINVAL;
+  if (mode != DRX_WALK_MATRIX && mode != DRX_WALK_VECTOR) return fail(ctx, DRX_EINVAL, "unknown walk mode %d", mode);
   if (!prog_id || (n_ops > 0 && !ops) || n_ops < 0 || n_const < 0)
     return fail(ctx, DRX_EINVAL, "bad program arguments");
   std::lock_guard<std::mutex> lock(ctx->mu);
@@ -928,7 +1192,9 @@ int drx_program_upload(drx_ctx* ctx, const uint32_t* ops, int n_ops, int m, int 
     }
   }
   HostProgram hp;
-  const std::string err = pack_program(ops, n_ops, m, dim, n_const, is_one, is_zero, &hp.pk);
+  hp.mode = mode;
+  const std::string err = pack_program(ops, n_ops, m, dim, n_const, is_one, is_zero, &hp.pk,
+                                       mode == DRX_WALK_VECTOR ? kPackVector : kPackMatrix);
   if (!err.empty()) return fail(ctx, DRX_EINVAL, "%s", err.c_str());
   const PackedProgram& pk = hp.pk;
   const int nc = (int)pk.const_slots.size();
@@ -994,7 +1260,8 @@ int drx_program_upload(drx_ctx* ctx, const uint32_t* ops, int n_ops, int m, int 
   d.fd_ok = (uint32_t)pk.fd_ok;
   d.fd_degree = (uint32_t)pk.fd_degree;
   d.fd_deg = pk.fd_deg;
-  hp.fd = pk.fd_ok && ((m == 2 && pk.fd_degree <= kFdMax2) || (m == 3 && pk.fd_degree <= kFdMax3));
+  hp.fd = mode == DRX_WALK_MATRIX && pk.fd_ok &&
+          ((m == 2 && pk.fd_degree <= kFdMax2) || (m == 3 && pk.fd_degree <= kFdMax3));
   ctx->h_code.insert(ctx->h_code.end(), pk.code.begin(), pk.code.end());
   if (ctx->h_code.size() % 2) ctx->h_code.push_back(0);
   ctx->h_code.insert(ctx->h_code.end(), pk.fcode.begin(), pk.fcode.end());
@@ -1014,7 +1281,8 @@ int drx_program_upload(drx_ctx* ctx, const uint32_t* ops, int n_ops, int m, int 
     else if (op == RO_MULNZ) mmo += 2;
     else if (op == RO_LIN) mao += 1;
   }
-  for (int j = 0; j < m; ++j) {
+  // the step product (a vector walk's product is part of its code)
+  for (int j = 0; j < m && mode == DRX_WALK_MATRIX; ++j) {
     int terms = 0;
     for (int k = 0; k < m; ++k) {
       const uint32_t kd = (pk.kinds >> (2 * (k * m + j))) & 3u;
@@ -1058,7 +1326,7 @@ int drx_program_info(drx_ctx* ctx, int prog_id, int* n_dev_ops, int* n_const_use
   return DRX_OK;
 }
 
-int drx_program_pack(const uint32_t* ops, int n_ops, int m, int dim, const int64_t* consts, int n_const,
+int drx_program_pack(const uint32_t* ops, int n_ops, int m, int dim, const int64_t* consts, int n_const, int mode,
                      uint32_t* words, int max_words, int* const_slots, int max_slots, int32_t* derived,
                      int max_derived, int64_t* meta, int max_meta, char* err, int err_len) {
   if ((n_ops > 0 && !ops) || n_ops < 0 || n_const < 0 || (n_const > 0 && !consts) || !meta || max_meta < 0)
@@ -1069,7 +1337,9 @@ int drx_program_pack(const uint32_t* ops, int n_ops, int m, int dim, const int64
     is_zero[i] = consts[i] == 0;
   }
   PackedProgram pk;
-  const std::string msg = pack_program(ops, n_ops, m, dim, n_const, is_one, is_zero, &pk);
+  if (mode != DRX_WALK_MATRIX && mode != DRX_WALK_VECTOR) return DRX_EINVAL;
+  const std::string msg = pack_program(ops, n_ops, m, dim, n_const, is_one, is_zero, &pk,
+                                       mode == DRX_WALK_VECTOR ? kPackVector : kPackMatrix);
   if (!msg.empty()) {
     if (err && err_len > 0) std::snprintf(err, (size_t)err_len, "%s", msg.c_str());
     return DRX_EINVAL;

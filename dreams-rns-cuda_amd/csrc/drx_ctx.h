@@ -59,6 +59,7 @@ struct DBuf {
 struct HostProgram {
   PackedProgram pk;
   DevProgram dev;
+  int mode = 0;  // DRX_WALK_MATRIX or DRX_WALK_VECTOR packing
   int fd = 0;  // runs on the forward-difference kernel
   // executed work per step, for the roofline: modular multiplies / adds
   int mm_ops = 0, ma_ops = 0;

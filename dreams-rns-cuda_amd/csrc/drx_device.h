@@ -29,7 +29,8 @@ namespace drx {
 // row of that entry.
 //   [3:0] op   [7:4] x   [8] y from C (else R)   [12:9] y   [16:13] dst
 //   [17] store to entry row [21:18]   [25:22] LIN signs (sx [23:22], sy [25:24])
-// LDM / STM / LDC carry an LDS row in [31:18] instead (and no entry store).
+// LDM / STM / LDC / LDV carry an LDS row in [31:18] instead (and no entry
+// store). In a vector walk the "entries" are the new state vector.
 // LIN computes sy * y + sx * x with each sign coded 0 -> 0, 1 -> +1, 2 -> -1
 // (ADD, SUB, reversed SUB, NEG, MOV, constant loads and zero).
 enum ResOp : uint32_t {
@@ -43,6 +44,7 @@ enum ResOp : uint32_t {
   RO_STM = 7,   // spill row <- x (destination = x's own register)
   RO_LDC = 8,   // LDS constant row (constant slot 15 - dim + row)
   RO_MULNZ = 9, // x * (y, or 1 when y = 0 mod p)
+  RO_LDV = 10,  // the lane's state row (vector walks)
 };
 __host__ __device__ constexpr uint32_t enc_res(uint32_t op, uint32_t x, uint32_t y_c, uint32_t y, uint32_t d,
                                                uint32_t e_store, uint32_t e, uint32_t sgn) {
@@ -82,14 +84,13 @@ enum DevOp : uint32_t {
 };
 // Destination banks.
 enum DstBank : uint32_t { DB_R = 0, DB_E = 1, DB_S = 2, DB_NONE = 3 };
-// Source banks: R temps (VGPR), X axes 0..6 / per-step increments 8..14 /
-// step 15 (VGPR),
+// Source banks: R temps (VGPR), X axes 0..14 / step 15 (VGPR),
 // C constants (VGPR), L constants (LDS), S temps (LDS), A accumulator, Z zero.
-enum SrcBank : uint32_t { SB_R = 0, SB_X = 1, SB_C = 2, SB_L = 3, SB_S = 4, SB_A = 5, SB_Z = 6 };
+// V: the lane's state vector (vector walks).
+enum SrcBank : uint32_t { SB_R = 0, SB_X = 1, SB_C = 2, SB_L = 3, SB_S = 4, SB_A = 5, SB_Z = 6, SB_V = 7 };
 
 constexpr int kBank = 16;       // VGPR bank width (doubles)
 constexpr int kStepSlot = 15;   // X[15] holds the step
-constexpr int kIncSlot = 8;     // X[8 + i] holds axis i's per-step increment
 
 __host__ __device__ constexpr uint32_t enc_w0(uint32_t op, uint32_t dbank, uint32_t dst) {
   return op | (dbank << 8) | (dst << 10);

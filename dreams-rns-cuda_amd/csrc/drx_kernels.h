@@ -38,10 +38,13 @@ struct KArgs {
   double* fcompact;   // optional [out_row][6]: copy of fout of the finalized walks
   double* ffull;      // optional [out_row][16]: final P_f
   uint64_t out_ld;
+  const DevVGroup* vgroups;  // vector walks: residue groups
+  const DevVGroup* fgroups;  // vector walks: FP64 groups (one lane per walk)
+  const int32_t* vorder;     // vector walks: walk indices grouped
 };
 
-// Instructions per scalar code load: the interpreters read the program in
-// 64-byte pieces instead of waiting for one 8-byte scalar load per
+// FP64 instructions per scalar code load: the M_f interpreter reads its
+// program in 64-byte pieces instead of waiting for one 8-byte scalar load per
 // instruction; the code pool is padded by kCodeFetch instructions.
 constexpr int kCodeFetch = 8;
 
@@ -69,6 +72,10 @@ struct GarnerArgs {
 
 
 hipError_t launch_walk(int m, int kind, const KArgs& a, int ntask, int threads, size_t lds, hipStream_t s);
+// Vector walks: residues (one block per residue group of `threads` lanes) and
+// the FP64 shadow (one block per FP64 group, one lane per walk).
+hipError_t launch_vec_walk(const KArgs& a, int ngroup, int threads, size_t lds, hipStream_t s);
+hipError_t launch_vec_fp(const KArgs& a, int ngroup, int threads, size_t lds, hipStream_t s);
 // Resident workgroups per CU of the interpreting walk kernel for order m at
 // this workgroup size and LDS; -1 when unknown.
 int walk_blocks_per_cu(int m, int kind, int threads, size_t lds);

@@ -36,7 +36,7 @@ __device__ __forceinline__ double lin_sign(uint32_t code) {
 // and `le` their entry rows ([rows][T], by thread).
 __device__ __forceinline__ void exec_res(uint32_t w, v16d& R, const v16d& C, const double* __restrict__ lc,
                                          uint32_t KP, double* __restrict__ lm, double* __restrict__ le,
-                                         const Lane& ln) {
+                                         const double* __restrict__ lv, const Lane& ln) {
   const uint32_t op = w & 15u;
   const double x = R[(w >> 4) & 15u];
   double r;
@@ -68,6 +68,7 @@ __device__ __forceinline__ void exec_res(uint32_t w, v16d& R, const v16d& C, con
         lm[row * ln.T + ln.tid] = x;
         r = x;
         break;
+      case RO_LDV: r = lv[row * ln.T + ln.tid]; break;
       default: r = lc[row * KP + ln.k]; break;  // RO_LDC
     }
   }
@@ -79,14 +80,15 @@ __device__ __forceinline__ void exec_res(uint32_t w, v16d& R, const v16d& C, con
 // per scalar load (n is a multiple of kResGroup).
 __device__ __forceinline__ void run_program(const uint32_t* __restrict__ code, uint32_t n, v16d& R, const v16d& C,
                                             const double* __restrict__ lc, uint32_t KP, double* __restrict__ lm,
-                                            double* __restrict__ le, const Lane& ln) {
+                                            double* __restrict__ le, const double* __restrict__ lv,
+                                            const Lane& ln) {
 #pragma unroll 1
   for (uint32_t pc = 0; pc < n; pc += kResGroup) {
     uint32_t w[kResGroup];
 #pragma unroll
     for (int j = 0; j < kResGroup; ++j) w[j] = code[pc + j];
 #pragma unroll
-    for (int j = 0; j < kResGroup; ++j) exec_res(w[j], R, C, lc, KP, lm, le, ln);
+    for (int j = 0; j < kResGroup; ++j) exec_res(w[j], R, C, lc, KP, lm, le, lv, ln);
   }
 }
 
@@ -233,10 +235,11 @@ __device__ __forceinline__ WalkSetup walk_setup(const KArgs& a, double* lds, boo
 
 // Bank C: the first constant slots, the axis values at step s0 (C[14 - i])
 // and the step s0 (C[15]); with li, the axes' per-step increments into the
-// LDS table li[dim][KP] (the caller synchronises before reading it).
+// LDS table li[dim][li_ld] at column li_col (the caller synchronises before
+// reading it).
 __device__ __forceinline__ void load_rns_inputs(const KArgs& a, const DevWalk& w, const DevProgram& pg,
                                                 uint32_t K, uint32_t kk, const Lane& ln, int64_t s0, v16d& C,
-                                                double* __restrict__ li, uint32_t KP) {
+                                                double* __restrict__ li, uint32_t li_ld, uint32_t li_col) {
   const double p = ln.p, pinv = ln.pinv;
   const uint32_t nc = (uint32_t)kResStep - pg.dim;
 #pragma unroll
@@ -249,19 +252,19 @@ __device__ __forceinline__ void load_rns_inputs(const KArgs& a, const DevWalk& w
       const double c0 = q[kk], c1 = q[K + kk], c2 = q[2 * K + kk];
       const double x0 = red(c0 + mm((double)s0, c1, p, pinv), p, pinv);
       C[kResStep - 1 - ax] = mm(x0, c2, p, pinv);
-      if (li) li[ax * KP + ln.k] = mm(c1, c2, p, pinv);
+      if (li) li[ax * li_ld + li_col] = mm(c1, c2, p, pinv);
     }
   }
   C[kResStep] = (double)s0;
 }
 
 // Axes and step to the next step (a uniform loop over the program's axes).
-__device__ __forceinline__ void advance_axes(v16d& C, const double* __restrict__ li, uint32_t KP, uint32_t dim,
-                                             const Lane& ln) {
+__device__ __forceinline__ void advance_axes(v16d& C, const double* __restrict__ li, uint32_t li_ld, uint32_t li_col,
+                                             uint32_t dim, const Lane& ln) {
 #pragma unroll 1
   for (uint32_t ax = 0; ax < dim; ++ax) {
     const uint32_t i = (uint32_t)kResStep - 1u - ax;
-    C[i] = red(C[i] + li[ax * KP + ln.k], ln.p, ln.pinv);
+    C[i] = red(C[i] + li[ax * li_ld + li_col], ln.p, ln.pinv);
   }
   C[kResStep] += 1.0;
 }
@@ -328,15 +331,15 @@ __global__ void __launch_bounds__(256) k_walk_chunk(const KArgs a) {
   const WalkSetup S = walk_setup(a, lds, true);
   const double p = S.ln.p, pinv = S.ln.pinv;
   v16d R = 0.0, C = 0.0;
-  load_rns_inputs(a, S.w, S.pg, S.K, S.kk, S.ln, S.sb, C, S.c == 0 ? S.li : nullptr, S.KP);
+  load_rns_inputs(a, S.w, S.pg, S.K, S.kk, S.ln, S.sb, C, S.c == 0 ? S.li : nullptr, S.KP, S.k);
   __syncthreads();
   double P[M * M];
 #pragma unroll
   for (int e = 0; e < M * M; ++e) P[e] = (e % (M + 1) == 0) ? 1.0 : 0.0;
   for (int64_t s = S.sb; s < S.se; ++s) {
-    run_program(S.code, S.pg.ncode, R, C, S.lc, S.KP, S.lm, S.le, S.ln);
+    run_program(S.code, S.pg.ncode, R, C, S.lc, S.KP, S.lm, S.le, nullptr, S.ln);
     step_matmul_rows<M>(P, S.le, S.pg.kinds, S.pg.colred, S.ln);
-    advance_axes(C, S.li, S.KP, S.pg.dim, S.ln);
+    advance_axes(C, S.li, S.KP, S.k, S.pg.dim, S.ln);
   }
   tree_combine<M>(a, S, P);
 }
@@ -380,8 +383,8 @@ __global__ void __launch_bounds__(512) k_fd_seed(const KArgs a, uint32_t kp) {
   double* tab = a.fdt + w.fdt_off;
   for (uint32_t j = g; j <= (uint32_t)D; j += J) {
     v16d R = 0.0, C = 0.0;
-    load_rns_inputs(a, w, pg, K, kk, ln, (int64_t)j, C, nullptr, KP);
-    run_program(code, pg.ncode, R, C, L.lc, KP, L.lm, L.le, ln);
+    load_rns_inputs(a, w, pg, K, kk, ln, (int64_t)j, C, nullptr, KP, k);
+    run_program(code, pg.ncode, R, C, L.lc, KP, L.lm, L.le, nullptr, ln);
     double E[M * M];
 #pragma unroll
     for (int e = 0; e < M * M; ++e)
@@ -587,9 +590,10 @@ __global__ void k_combine(const KArgs a) {
 // Constants (banks C and L) are read from an LDS copy of the FP64 constant
 // table: a uniform constant bank would sit in SGPRs and be copied to VGPRs on
 // every dynamically indexed read.
-__device__ __forceinline__ double fetch_fp(uint32_t bank, uint32_t i, const v16d& R, const v16d& X, double acc,
-                                           const double* __restrict__ ldc, const double* __restrict__ lds_s,
-                                           uint32_t tid, uint32_t T) {
+template <bool VEC>
+__device__ __forceinline__ double fetch_fp(uint32_t bank, uint32_t i, const v16d& R, const v16d& X, const v16d& V,
+                                           double acc, const double* __restrict__ ldc,
+                                           const double* __restrict__ lds_s, uint32_t tid, uint32_t T) {
   switch (bank) {
     case SB_R: return R[i & 15u];
     case SB_X: return X[i & 15u];
@@ -597,41 +601,41 @@ __device__ __forceinline__ double fetch_fp(uint32_t bank, uint32_t i, const v16d
     case SB_L: return ldc[kBank + i];
     case SB_S: return lds_s[(uint64_t)i * T + tid];
     case SB_A: return acc;
+    case SB_V: return VEC ? V[i & 15u] : 0.0;
     default: return 0.0;
   }
 }
 
-template <typename EV>
-__device__ __forceinline__ void exec_fp(uint2 w, double& acc, v16d& R, EV& E, const v16d& X,
+// One FP64 instruction; with VEC, bank V is the lane's state vector.
+template <bool VEC, typename EV>
+__device__ __forceinline__ void exec_fp(uint2 w, double& acc, v16d& R, EV& E, const v16d& X, const v16d& V,
                                         const double* __restrict__ ldc, double* __restrict__ lds_s, uint32_t tid,
                                         uint32_t T) {
-  {
-    const uint32_t op = w.x & 255u;
-    const double x = fetch_fp(w.y & 7u, (w.y >> 3) & 8191u, R, X, acc, ldc, lds_s, tid, T);
-    double r;
-    if (op == D_MUL || op == D_ADD || op == D_SUB) {
-      const double y = fetch_fp((w.y >> 16) & 7u, w.y >> 19, R, X, acc, ldc, lds_s, tid, T);
-      r = (op == D_MUL) ? x * y : ((op == D_ADD) ? x + y : x - y);
-    } else {
-      switch (op) {
-        case D_NEG: r = -x; break;
-        case D_SQR: r = x * x; break;
-        case D_CUB: r = cube_cr(x); break;
-        case D_INV: r = finv(x); break;
-        default: r = x; break;  // D_RED (no-op in FP64), D_MOV
-      }
+  const uint32_t op = w.x & 255u;
+  const double x = fetch_fp<VEC>(w.y & 7u, (w.y >> 3) & 8191u, R, X, V, acc, ldc, lds_s, tid, T);
+  double r;
+  if (op == D_MUL || op == D_ADD || op == D_SUB) {
+    const double y = fetch_fp<VEC>((w.y >> 16) & 7u, w.y >> 19, R, X, V, acc, ldc, lds_s, tid, T);
+    r = (op == D_MUL) ? x * y : ((op == D_ADD) ? x + y : x - y);
+  } else {
+    switch (op) {
+      case D_NEG: r = -x; break;
+      case D_SQR: r = x * x; break;
+      case D_CUB: r = cube_cr(x); break;
+      case D_INV: r = finv(x); break;
+      default: r = x; break;  // D_RED (no-op in FP64), D_MOV
     }
-    const uint32_t db = (w.x >> 8) & 3u, d = w.x >> 10;
-    if (db == DB_R) R[d & 15u] = r;
-    else if (db == DB_E) E[d & (sizeof(EV) / sizeof(double) - 1)] = r;
-    else if (db == DB_S) lds_s[(uint64_t)d * T + tid] = r;
-    acc = r;
   }
+  const uint32_t db = (w.x >> 8) & 3u, d = w.x >> 10;
+  if (db == DB_R) R[d & 15u] = r;
+  else if (db == DB_E) E[d & (sizeof(EV) / sizeof(double) - 1)] = r;
+  else if (db == DB_S) lds_s[(uint64_t)d * T + tid] = r;
+  acc = r;
 }
 
-template <typename EV>
+template <bool VEC, typename EV>
 __device__ __forceinline__ void run_program_fp(const uint2* __restrict__ code, uint32_t n, v16d& R, EV& E,
-                                               const v16d& X, const double* __restrict__ ldc,
+                                               const v16d& X, const v16d& V, const double* __restrict__ ldc,
                                                double* __restrict__ lds_s, uint32_t tid, uint32_t T) {
   double acc = 0.0;
 #pragma unroll 1
@@ -641,7 +645,7 @@ __device__ __forceinline__ void run_program_fp(const uint2* __restrict__ code, u
     for (int j = 0; j < kCodeFetch; ++j) w[j] = code[pc + j];
 #pragma unroll
     for (int j = 0; j < kCodeFetch; ++j)
-      if (pc + j < n) exec_fp(w[j], acc, R, E, X, ldc, lds_s, tid, T);
+      if (pc + j < n) exec_fp<VEC>(w[j], acc, R, E, X, V, ldc, lds_s, tid, T);
   }
 }
 
@@ -666,15 +670,13 @@ __global__ void __launch_bounds__(256) k_mf(const KArgs a) {
   v16d R = 0.0, X = 0.0;
   const int64_t want = r.s_begin + tid;
   const int64_t s = want < r.s_end ? want : r.s_end - 1;
-#pragma unroll
-  for (int ax = 0; ax < kMaxDim; ++ax) {
-    if ((uint32_t)ax < pg.dim) {
-      const int64_t* c = a.axi + w.axi_off + 2 * ax;
-      X[ax] = (double)(c[0] + s * c[1]) / a.axd[w.axd_off + ax];
-    }
+#pragma unroll 1
+  for (uint32_t ax = 0; ax < pg.dim; ++ax) {
+    const int64_t* c = a.axi + w.axi_off + 2 * ax;
+    X[ax] = (double)(c[0] + s * c[1]) / a.axd[w.axd_off + ax];
   }
   X[kStepSlot] = (double)s;
-  run_program_fp(code, pg.nfcode, R, E, X, ldcf, lds_s, tid, T);
+  run_program_fp<false>(code, pg.nfcode, R, E, X, X, ldcf, lds_s, tid, T);
   if (want < r.s_end) {
     double* out = a.mf + t.mf_off + (uint64_t)(s - t.s_begin) * M * M;
 #pragma unroll
@@ -879,6 +881,149 @@ __global__ void __launch_bounds__(256) k_shadow(const KArgs a, int nspan, int fi
       o[2] = ls;
       o[3] = delta;
       o[4] = logq;
+    }
+  }
+}
+
+// ------------------------------------------------------------ vector walks
+// State-vector walks v(N) = M(N-1) ... M(0) v(0) (cmf_walk.py:445-531). The
+// program is packed so that it computes v' = M(s) v itself: the state
+// entries are its inputs (LDV) and its outputs (entry stores), so one step is
+// one interpretation. One lane per (walk, prime) runs the whole depth in
+// order, the state double-buffered in the lane's LDS rows; the walks of a
+// workgroup share program, depth and K, so the instruction stream is uniform.
+__global__ void __launch_bounds__(256) k_walk_vec(const KArgs a) {
+  extern __shared__ double lds[];
+  const DevVGroup vg = a.vgroups[blockIdx.x];
+  const DevProgram pg = a.progs[vg.prog];
+  const uint32_t T = blockDim.x, tid = threadIdx.x, K = (uint32_t)vg.K, KP = pow2_at_least(K);
+  const uint32_t g = tid / KP, k = tid % KP, m = pg.m;
+  const bool active = g < (uint32_t)vg.n && k < K;
+  const uint32_t gg = g < (uint32_t)vg.n ? g : (uint32_t)vg.n - 1u;
+  const uint32_t kk = k < K ? k : 0u;
+  const int32_t wi = a.vorder[vg.first + gg];
+  const DevWalk w = a.walks[wi];
+  Lane ln;
+  ln.p = a.prime_p[kk];
+  ln.pinv = a.prime_pinv[kk];
+  ln.pe = a.prime_e[kk];
+  ln.k = k;
+  ln.tid = tid;
+  ln.T = T;
+  // LDS: constant rows [nl][KP], spill rows [nslots][T], state A and B
+  // [m][T] each, axis increments [dim][T]
+  const ProgLds L = prog_lds(a, pg, K, KP, lds);
+  double* sa = L.lm + (uint64_t)pg.nslots * T;
+  double* sb = sa + (uint64_t)m * T;
+  double* li = sb + (uint64_t)m * T;
+  v16d R = 0.0, C = 0.0;
+  load_rns_inputs(a, w, pg, K, kk, ln, 0, C, li, T, tid);
+#pragma unroll 1
+  for (uint32_t j = 0; j < m; ++j) sa[j * T + tid] = a.res_in[w.res_off + (uint64_t)j * K + kk];
+  __syncthreads();  // constant rows
+  const uint32_t* code = a.code + pg.code_off;
+  double* cur = sa;
+  double* nxt = sb;
+#pragma unroll 1
+  for (int64_t s = 0; s < vg.depth; ++s) {
+    run_program(code, pg.ncode, R, C, L.lc, KP, L.lm, nxt, cur, ln);
+    double* t = cur;
+    cur = nxt;
+    nxt = t;
+    advance_axes(C, li, T, tid, pg.dim, ln);
+  }
+  if (active) {
+    const double p = ln.p, pinv = ln.pinv;
+    int64_t* ro = a.rout + (uint64_t)wi * 2 * a.out_ld;
+    ro[k] = (int64_t)canon(cur[(uint32_t)w.pe * T + tid], p, pinv);
+    ro[a.out_ld + k] = (int64_t)canon(cur[(uint32_t)w.qe * T + tid], p, pinv);
+    if (a.pout) {
+      int64_t* po = a.pout + (uint64_t)wi * 16 * a.out_ld;
+      for (uint32_t j = 0; j < m; ++j) po[(uint64_t)j * a.out_ld + k] = (int64_t)canon(cur[j * T + tid], p, pinv);
+    }
+  }
+}
+
+// FP64 shadow of vector walks, one lane per walk (walks of a workgroup share
+// program and depth): v_f <- M_f v_f by the same packed program in FP64, then
+// if max |v_f| > 1e10 (and no entry is NaN, as numpy's max would return NaN)
+// v_f /= max, log_scale carried as a mantissa product and a binary exponent
+// (cmf_walk.py:513-520). Ends with delta_float (runner.py:86-100).
+__global__ void __launch_bounds__(256) k_vec_fp(const KArgs a) {
+  extern __shared__ double lds[];
+  const DevVGroup vg = a.fgroups[blockIdx.x];
+  const DevProgram pg = a.progs[vg.prog];
+  const uint32_t T = blockDim.x, tid = threadIdx.x, m = pg.m;
+  const bool active = tid < (uint32_t)vg.n;
+  const int32_t wi = a.vorder[vg.first + (active ? tid : (uint32_t)vg.n - 1u)];
+  const DevWalk w = a.walks[wi];
+  double* ldcf = lds;
+  double* lds_s = ldcf + pg.nconst;
+  for (uint32_t i = tid; i < pg.nconst; i += T) ldcf[i] = a.cflt[pg.cflt_off + i];
+  __syncthreads();
+  const uint2* code = reinterpret_cast<const uint2*>(a.code + pg.fcode_off);
+  v16d R = 0.0, X = 0.0, V = 0.0, E = 0.0;
+#pragma unroll
+  for (int j = 0; j < 16; ++j)
+    if ((uint32_t)j < m) V[j] = a.fst_in[w.fst_off + j];
+  double mant = 1.0;
+  int ex = 0;
+#pragma unroll 1
+  for (int64_t s = 0; s < vg.depth; ++s) {
+#pragma unroll 1
+    for (uint32_t ax = 0; ax < pg.dim; ++ax) {
+      const int64_t* c = a.axi + w.axi_off + 2 * ax;
+      X[ax] = (double)(c[0] + s * c[1]) / a.axd[w.axd_off + ax];
+    }
+    X[kStepSlot] = (double)s;
+    run_program_fp<true>(code, pg.nfcode, R, E, X, V, ldcf, lds_s, tid, T);
+    double mx = 0.0;
+    bool nan = false;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const double t = ((uint32_t)i < m) ? __builtin_fabs(E[i]) : 0.0;
+      nan = nan || (t != t);
+      mx = t > mx ? t : mx;
+    }
+    // branch-free: a step without rescale divides by 1 (exact) and
+    // multiplies the mantissa by 2^-1 * 2^1
+    const double dv = (!nan && mx > 1e10) ? mx : 1.0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) V[i] = ((uint32_t)i < m) ? E[i] / dv : 0.0;
+    int e1 = 0, e2 = 0;
+    mant *= __builtin_frexp(dv, &e1);
+    mant = __builtin_frexp(mant, &e2);
+    ex += e1 + e2;
+  }
+  if (active) {
+    double pf = 0.0, qf = 0.0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      pf = (i == w.pe) ? V[i] : pf;
+      qf = (i == w.qe) ? V[i] : qf;
+    }
+    const double ls = log(mant) + (double)ex * 0.69314718055994530942;
+    double delta, logq;
+    if (!isfinite(pf) || !isfinite(qf) || __builtin_fabs(qf) < 1e-300) {
+      delta = -1e10;
+      logq = 0.0;
+    } else {
+      const double err = __builtin_fabs(pf / qf - w.target);
+      logq = ls + log(__builtin_fabs(qf));
+      if (err < 1e-300) delta = 100.0;
+      else if (logq <= 0.0) delta = -1e10;
+      else delta = -(1.0 + log(err) / logq);
+    }
+    double* o = a.fout + (uint64_t)wi * 6;
+    o[0] = pf;
+    o[1] = qf;
+    o[2] = ls;
+    o[3] = delta;
+    o[4] = logq;
+    o[5] = 0.0;
+    if (a.ffull) {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) a.ffull[(uint64_t)wi * 16 + i] = ((uint32_t)i < m) ? V[i] : 0.0;
     }
   }
 }
@@ -1168,6 +1313,20 @@ hipError_t launch_walk(int m, int kind, const KArgs& a, int ntask, int threads, 
     else if (m == 4) hipLaunchKernelGGL(k_walk_chunk<4>, grid, block, lds, s, a);
     else return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_vec_walk(const KArgs& a, int ngroup, int threads, size_t lds, hipStream_t s) {
+  if (ngroup <= 0) return hipSuccess;
+  if (threads <= 0 || threads > 256) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_walk_vec, dim3(ngroup), dim3(threads), lds, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_vec_fp(const KArgs& a, int ngroup, int threads, size_t lds, hipStream_t s) {
+  if (ngroup <= 0) return hipSuccess;
+  if (threads <= 0 || threads > 256) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_vec_fp, dim3(ngroup), dim3(threads), lds, s, a);
   return hipGetLastError();
 }
 

@@ -47,7 +47,7 @@ enum RefOp : uint32_t {
   R_NEG = 7, R_POW2 = 8, R_POW3 = 9, R_INV = 10, R_STORE = 11, R_END = 12
 };
 
-enum ValKind { V_ZERO, V_X, V_N, V_C, V_OP };
+enum ValKind { V_ZERO, V_X, V_N, V_C, V_OP, V_S };  // V_S: state entry idx (vector walks)
 
 struct Val {
   ValKind kind;
@@ -76,11 +76,14 @@ struct CodeList {
   std::vector<int> entry_direct;  // value -> entry its producer writes (-1: none)
 };
 
-// Instructions for the values in `order` (RED inserted when `with_red`),
-// then trailing moves for entries whose value cannot be written into E by
-// its producer: inputs, values shared by several entries or used later.
-CodeList build_code(std::vector<Val>& vals, const std::vector<int>& order, const std::vector<int>& entry,
-                    uint32_t kinds, int m, bool with_red) {
+// Instructions for the values in `order` (RED inserted when `with_red`;
+// with `reduce_out` also before an output that could pass 2^31), then
+// trailing moves for outputs whose value cannot be written by its producer:
+// inputs, values shared by several outputs or used later. Output o (an entry
+// of M, or a state entry of a vector walk) is `out[o]`, emitted when
+// `out_gen[o]`.
+CodeList build_code(std::vector<Val>& vals, const std::vector<int>& order, std::vector<int> out,
+                    const std::vector<char>& out_gen, bool with_red, bool reduce_out) {
   CodeList cl;
   std::vector<double> bound(vals.size(), 0.0);
   auto src_bound = [&](int v) -> double {
@@ -88,6 +91,7 @@ CodeList build_code(std::vector<Val>& vals, const std::vector<int>& order, const
       case V_ZERO: return 0.0;
       case V_X: return kMulBound;
       case V_N: return kMulBound;
+      case V_S: return kMulBound;
       case V_C: return kMulBound / 2;
       default: return bound[v];
     }
@@ -122,18 +126,35 @@ CodeList build_code(std::vector<Val>& vals, const std::vector<int>& order, const
       cl.code.push_back(Ins{x.op, x.a, unary ? -1 : x.b, v});
     }
   }
+  const int nout = (int)out.size();
+  if (with_red && reduce_out) {
+    // a vector walk's state must come back reduced: it is next step's input
+    for (int o = 0; o < nout; ++o) {
+      const int v = out[o];
+      if (!out_gen[o] || vals[v].kind != V_OP || bound[v] <= kMulBound) continue;
+      Val r{V_OP};
+      r.op = D_RED;
+      r.a = v;
+      vals.push_back(r);
+      bound.push_back(kMulBound);
+      const int nv2 = (int)vals.size() - 1;
+      cl.code.push_back(Ins{D_RED, v, -1, nv2});
+      for (int o2 = o; o2 < nout; ++o2)
+        if (out[o2] == v) out[o2] = nv2;
+    }
+  }
   const int nvals = (int)vals.size();
   std::vector<int> n_entry_of(nvals, 0), uses(nvals, 0);
-  for (int e = 0; e < m * m; ++e)
-    if (((kinds >> (2 * e)) & 3u) == E_GEN) ++n_entry_of[entry[e]];
+  for (int e = 0; e < nout; ++e)
+    if (out_gen[e]) ++n_entry_of[out[e]];
   for (const Ins& in : cl.code) {
     if (in.a >= 0) ++uses[in.a];
     if (in.b >= 0) ++uses[in.b];
   }
   cl.entry_direct.assign(nvals, -1);
-  for (int e = 0; e < m * m; ++e) {
-    if (((kinds >> (2 * e)) & 3u) != E_GEN) continue;
-    const int v = entry[e];
+  for (int e = 0; e < nout; ++e) {
+    if (!out_gen[e]) continue;
+    const int v = out[e];
     if (vals[v].kind == V_OP && n_entry_of[v] == 1 && uses[v] == 0 && cl.entry_direct[v] < 0) {
       cl.entry_direct[v] = e;
     } else {
@@ -369,6 +390,9 @@ std::string encode(const CodeList& cl, const std::vector<Val>& vals, const std::
         const int cs = cslot[x.idx];
         sb[q] = cs < kBank ? SB_C : SB_L;
         si[q] = (uint32_t)(cs < kBank ? cs : cs - kBank);
+      } else if (x.kind == V_S) {
+        sb[q] = SB_V;
+        si[q] = (uint32_t)x.idx;
       } else if (def_pos[u] == i - 1) {
         sb[q] = SB_A;
       } else {
@@ -493,6 +517,8 @@ std::string encode_res(const CodeList& cl, const std::vector<Val>& vals, const s
     if (r < 0) return -1;
     if (x.kind == V_OP) {
       words.push_back(enc_row(RO_LDM, (uint32_t)r, (uint32_t)r, (uint32_t)row[u]));
+    } else if (x.kind == V_S) {
+      words.push_back(enc_row(RO_LDV, (uint32_t)r, (uint32_t)r, (uint32_t)x.idx));
     } else {
       const int ci = c_index(u);
       if (ci >= 0) {
@@ -617,10 +643,13 @@ std::string encode_res(const CodeList& cl, const std::vector<Val>& vals, const s
 
 std::string pack_program(const uint32_t* ops, int n_ops, int m, int dim, int n_const,
                          const std::vector<char>& const_is_one, const std::vector<char>& const_is_zero,
-                         PackedProgram* out) {
+                         PackedProgram* out, int mode) {
   char msg[160];
-  if (m < 2 || m > kMaxM) {
-    std::snprintf(msg, sizeof msg, "matrix order m=%d outside [2, %d]", m, kMaxM);
+  const bool vec = mode == kPackVector;
+  const int mmax = vec ? kMaxVecM : kMaxM;
+  if (mode != kPackMatrix && mode != kPackVector) return "unknown packing mode";
+  if (m < 2 || m > mmax) {
+    std::snprintf(msg, sizeof msg, "%s order m=%d outside [2, %d]", vec ? "state vector" : "matrix", m, mmax);
     return msg;
   }
   if (dim < 0 || dim > kMaxDim) {
@@ -731,19 +760,66 @@ std::string pack_program(const uint32_t* ops, int n_ops, int m, int dim, int n_c
   is_one.resize(n_const_all, 0);
   is_zero.resize(n_const_all, 0);
 
-  // ---- entry kinds and liveness
+  // ---- entry kinds
+  auto is_const_val = [&](int v, const std::vector<char>& tab) { return vals[v].kind == V_C && tab[vals[v].idx]; };
+  std::vector<uint32_t> ekind(m * m);
+  for (int e = 0; e < m * m; ++e) {
+    const int v = entry[e];
+    ekind[e] = E_GEN;
+    if (vals[v].kind == V_ZERO || is_const_val(v, is_zero)) ekind[e] = E_ZERO;
+    else if (is_const_val(v, is_one)) ekind[e] = E_ONE;
+    if (!vec) pp.kinds |= ekind[e] << (2 * e);
+  }
+  // ---- outputs: the general entries of M, or for a vector walk the new
+  // state v'_i = sum_j M_ij v_j over the nonzero entries in column order
+  std::vector<int> outv;
+  std::vector<char> out_gen;
+  if (!vec) {
+    outv = entry;
+    for (int e = 0; e < m * m; ++e) out_gen.push_back(ekind[e] == E_GEN);
+  } else {
+    std::vector<int> state(m);
+    for (int j = 0; j < m; ++j) {
+      vals.push_back(Val{V_S, j});
+      state[j] = (int)vals.size() - 1;
+    }
+    for (int i = 0; i < m; ++i) {
+      int acc = 0;  // V_ZERO: a row without nonzero entries
+      for (int j = 0; j < m; ++j) {
+        const uint32_t kd = ekind[i * m + j];
+        if (kd == E_ZERO) continue;
+        int term = state[j];
+        if (kd == E_GEN) {
+          Val t{V_OP};
+          t.op = D_MUL;
+          t.a = entry[i * m + j];
+          t.b = state[j];
+          vals.push_back(t);
+          term = (int)vals.size() - 1;
+        }
+        if (acc == 0) {
+          acc = term;
+        } else {
+          Val t{V_OP};
+          t.op = D_ADD;
+          t.a = acc;
+          t.b = term;
+          vals.push_back(t);
+          acc = (int)vals.size() - 1;
+        }
+      }
+      outv.push_back(acc);
+      out_gen.push_back(1);
+    }
+  }
+  const int nout = (int)outv.size();
+
+  // ---- liveness
   const int nv = (int)vals.size();
   std::vector<char> needed(nv, 0);
   std::vector<int> stack;
-  auto is_const_val = [&](int v, const std::vector<char>& tab) { return vals[v].kind == V_C && tab[vals[v].idx]; };
-  for (int e = 0; e < m * m; ++e) {
-    const int v = entry[e];
-    uint32_t kind = E_GEN;
-    if (vals[v].kind == V_ZERO || is_const_val(v, is_zero)) kind = E_ZERO;
-    else if (is_const_val(v, is_one)) kind = E_ONE;
-    pp.kinds |= kind << (2 * e);
-    if (kind == E_GEN) stack.push_back(v);
-  }
+  for (int o = 0; o < nout; ++o)
+    if (out_gen[o]) stack.push_back(outv[o]);
   while (!stack.empty()) {
     const int v = stack.back();
     stack.pop_back();
@@ -754,8 +830,9 @@ std::string pack_program(const uint32_t* ops, int n_ops, int m, int dim, int n_c
       stack.push_back(vals[v].b);
     }
   }
-  // polynomial degree of every needed value in the step (-1: not a polynomial)
-  {
+  // polynomial degree of every needed value in the step (-1: not a
+  // polynomial); forward differences serve matrix walks only
+  if (!vec) {
     std::vector<int> deg(nv, 0);
     for (int v = 0; v < nv; ++v) {
       const Val& x = vals[v];
@@ -764,6 +841,7 @@ std::string pack_program(const uint32_t* ops, int n_ops, int m, int dim, int n_c
         case V_C: deg[v] = 0; break;
         case V_X: deg[v] = 1; break;
         case V_N: deg[v] = 1; break;
+        case V_S: deg[v] = -1; break;
         default: {
           const int da = deg[x.a], db = deg[x.b];
           if (x.op == D_INV || da < 0 || (db < 0 && binary_op(x.op))) {
@@ -786,7 +864,7 @@ std::string pack_program(const uint32_t* ops, int n_ops, int m, int dim, int n_c
     int ok = 1, dmax = 0;
     uint64_t packed = 0;
     for (int e = 0; e < m * m; ++e) {
-      if (((pp.kinds >> (2 * e)) & 3u) != E_GEN) continue;
+      if (ekind[e] != E_GEN) continue;
       const int d = deg[entry[e]];
       if (d < 0) {
         ok = 0;
@@ -798,20 +876,19 @@ std::string pack_program(const uint32_t* ops, int n_ops, int m, int dim, int n_c
     pp.fd_ok = ok;
     pp.fd_degree = ok ? dmax : 0;
     pp.fd_deg = ok ? packed : 0;
-  }
-  for (int j = 0; j < m; ++j) {
-    int nz = 0, ones = 0;
-    for (int k = 0; k < m; ++k) {
-      const uint32_t kd = (pp.kinds >> (2 * (k * m + j))) & 3u;
-      nz += kd != E_ZERO;
-      ones += kd == E_ONE;
+    for (int j = 0; j < m; ++j) {
+      int nz = 0, ones = 0;
+      for (int k = 0; k < m; ++k) {
+        nz += ekind[k * m + j] != E_ZERO;
+        ones += ekind[k * m + j] == E_ONE;
+      }
+      if (ones > 0 && nz >= 2) pp.colred |= 1u << j;
     }
-    if (ones > 0 && nz >= 2) pp.colred |= 1u << j;
   }
 
   std::vector<int> roots;
-  for (int e = 0; e < m * m; ++e)
-    if (((pp.kinds >> (2 * e)) & 3u) == E_GEN) roots.push_back(entry[e]);
+  for (int o = 0; o < nout; ++o)
+    if (out_gen[o]) roots.push_back(outv[o]);
   std::vector<int> dummy_cslot(n_const_all, 0);
 
   // ---- FP64 code: the same operations as the reference (one reciprocal per
@@ -824,7 +901,7 @@ std::string pack_program(const uint32_t* ops, int n_ops, int m, int dim, int n_c
     int best = -1;
     for (int policy = 0; policy < kPolicies; ++policy) {
       std::vector<Val> cv = vals;
-      CodeList cl = build_code(cv, schedule(cv, prio, roots, policy), entry, pp.kinds, m, false);
+      CodeList cl = build_code(cv, schedule(cv, prio, roots, policy), outv, out_gen, false, false);
       std::vector<uint32_t> scratch_words;
       int ns = 0;
       const std::string err = encode(cl, cv, dummy_cslot, scratch_words, ns);
@@ -848,7 +925,7 @@ std::string pack_program(const uint32_t* ops, int n_ops, int m, int dim, int n_c
     for (int policy = 0; policy < kPolicies; ++policy) {
       std::vector<Val> cv = g.vals;
       const std::vector<int> order = schedule(cv, g.prio, roots, policy);
-      CodeList cl = build_code(cv, order, entry, pp.kinds, m, true);
+      CodeList cl = build_code(cv, order, outv, out_gen, true, vec);
       std::vector<uint32_t> scratch_words;
       int ns = 0;
       const std::string err = encode_res(cl, cv, dummy_cslot, dim, scratch_words, ns);

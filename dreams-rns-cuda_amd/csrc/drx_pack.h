@@ -44,10 +44,14 @@ struct PackedProgram {
   int n_mul = 0, n_sqr = 0, n_cub = 0, n_inv = 0, n_add = 0, n_sub = 0, n_neg = 0;
 };
 
+constexpr int kPackMatrix = 0;  // outputs: the general entries of M(s)
+constexpr int kPackVector = 1;  // outputs: the new state v' = M(s) v of a vector walk
+constexpr int kMaxVecM = 16;    // state entries of a vector walk
+
 // `const_is_one[i]` / `const_is_zero[i]`: constant i equals 1 / 0 exactly.
 // Returns an empty string on success, else an error message.
 std::string pack_program(const uint32_t* ops, int n_ops, int m, int dim, int n_const,
                          const std::vector<char>& const_is_one, const std::vector<char>& const_is_zero,
-                         PackedProgram* out);
+                         PackedProgram* out, int mode = kPackMatrix);
 
 }  // namespace drx

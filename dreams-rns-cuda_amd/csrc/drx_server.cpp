@@ -450,6 +450,8 @@ int drx_server_submit(drx_server* s, const drx_walk* w, const int64_t* axis_vals
     std::lock_guard<std::mutex> lk(ctx->mu);
     if (w->prog < 0 || w->prog >= (int)ctx->progs.size()) return fail(ctx, DRX_EINVAL, "bad program id %d", w->prog);
     const DevProgram& dp = ctx->progs[w->prog].dev;
+    if (w->mode != DRX_WALK_MATRIX || ctx->progs[w->prog].mode != DRX_WALK_MATRIX)
+      return fail(ctx, DRX_EINVAL, "the server runs matrix walks");
     if (w->K < 1 || w->K > s->kslot || w->K > (int)dp.kcap || pow2_ceil(w->K) > 256)
       return fail(ctx, DRX_EINVAL, "bad K=%d", w->K);
     if (w->depth < 0 || w->depth > ((int64_t)1 << 30)) return fail(ctx, DRX_EINVAL, "bad depth");

@@ -8,7 +8,7 @@ namespace drx {
 
 constexpr int kMaxM = 4;          // matrix order handled by the kernels
 constexpr int kMaxK = 1024;       // primes in one context
-constexpr int kMaxDim = 7;        // axes per program (X bank: axes, increments, step)
+constexpr int kMaxDim = 15;       // axes per program (FP64 bank X: axes 0..14, step 15)
 constexpr int kMaxConst = 1023;   // constants per program (10-bit field)
 constexpr int kMaxCode = 1 << 16; // device words per program
 constexpr int kFdMax2 = 6;        // forward-difference degree limit, 2x2 programs
@@ -52,8 +52,9 @@ struct DevWalk {
   int32_t K;
   int32_t m;
   int32_t dim;
-  uint64_t res_off;     // P residues [m*m][K] (double, balanced)
-  uint64_t fst_off;     // FP64 shadow state, fst_size(m) doubles
+  int32_t pe, qe;       // vector walks: state entries of p and q
+  uint64_t res_off;     // P residues [m*m][K] (double, balanced); vector walks: v(0) residues [m][K]
+  uint64_t fst_off;     // FP64 shadow state, fst_size(m) doubles; vector walks: v_f(0) [m]
   uint64_t fdt_off;     // forward-difference walks: Newton tables at step 0, [m*m][fd_dmax(m)+1][K]
   uint64_t axr_off;     // axis residues [dim][3][K]: c0, c1, c2 (double, balanced)
   uint64_t axi_off;     // axis integers [dim][2]: c0, c1 (int64) for the FP64 axis value
@@ -77,6 +78,16 @@ struct DevSpan {
   int32_t task_begin;   // tasks of this walk are contiguous and in step order
   int32_t task_end;
   int32_t out_row;      // row of this walk in the finalize outputs
+};
+
+// Walks of one program, depth and prime count run together by a vector-walk
+// kernel: walks vorder[first .. first + n).
+struct DevVGroup {
+  int32_t first;
+  int32_t n;
+  int32_t prog;
+  int32_t K;
+  int64_t depth;
 };
 
 // A block of consecutive steps of one task for the FP64 M_f(s) evaluation.

@@ -23,7 +23,7 @@ from .runner import (
     run_pcf_walk,
     verify_pcf,
 )
-from .cmf_walk import run_cmf_walk
+from .cmf_walk import run_cmf_walk, run_cmf_walk_vec, run_cmf_walk_vec_batch
 from .gpu_walk import gpu_available, run_cmf_walk_batch_gpu, run_cmf_walk_gpu, run_pcf_walk_batch_gpu, run_pcf_walk_gpu
 
 __version__ = "0.2.0"
@@ -45,6 +45,8 @@ __all__ = [
     "run_pcf_walk",
     "verify_pcf",
     "run_cmf_walk",
+    "run_cmf_walk_vec",
+    "run_cmf_walk_vec_batch",
     "gpu_available",
     "run_pcf_walk_gpu",
     "run_pcf_walk_batch_gpu",

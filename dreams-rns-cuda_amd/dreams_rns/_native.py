@@ -16,9 +16,10 @@ import numpy as np
 
 from ._build import LIB_PATH
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 PACK_META = 13  # values drx_program_pack reports
 DRX_INIT_PCF, DRX_INIT_IDENTITY, DRX_INIT_GIVEN = 0, 1, 2
+DRX_WALK_MATRIX, DRX_WALK_VECTOR = 0, 1
 MODE_BATCHED, MODE_PLAIN, MODE_PERSISTENT = 0, 1, 2
 
 _ERRORS = {-1: ValueError, -2: RuntimeError, -3: MemoryError, -4: OverflowError, -5: RuntimeError, -6: BlockingIOError}
@@ -31,15 +32,16 @@ class NativeUnavailable(RuntimeError):
 class Walk(C.Structure):
     _fields_ = [
         ("prog", C.c_int32), ("K", C.c_int32), ("depth", C.c_int64),
-        ("init_kind", C.c_int32), ("reserved", C.c_int32), ("a0", C.c_int64),
+        ("init_kind", C.c_int32), ("mode", C.c_int32), ("a0", C.c_int64),
         ("init_off", C.c_int64), ("init_f_off", C.c_int64), ("axis_off", C.c_int64),
-        ("target", C.c_double),
+        ("target", C.c_double), ("out_p", C.c_int32), ("out_q", C.c_int32),
     ]
 
 
 WALK_DTYPE = np.dtype([
-    ("prog", "<i4"), ("K", "<i4"), ("depth", "<i8"), ("init_kind", "<i4"), ("reserved", "<i4"),
+    ("prog", "<i4"), ("K", "<i4"), ("depth", "<i8"), ("init_kind", "<i4"), ("mode", "<i4"),
     ("a0", "<i8"), ("init_off", "<i8"), ("init_f_off", "<i8"), ("axis_off", "<i8"), ("target", "<f8"),
+    ("out_p", "<i4"), ("out_q", "<i4"),
 ], align=True)
 assert WALK_DTYPE.itemsize == C.sizeof(Walk)
 
@@ -86,12 +88,15 @@ SIGNATURES = {
     "drx_set_primes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "drx_program_upload": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
                                      C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int)]),
+    "drx_program_upload_mode": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                          C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                          C.POINTER(C.c_int)]),
     "drx_program_info": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                    C.POINTER(C.c_uint32)]),
     "drx_program_release_all": (C.c_int, [C.c_void_p]),
-    "drx_program_pack": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
-                                   C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
-                                   C.c_char_p, C.c_int]),
+    "drx_program_pack": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int,
+                                   C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                                   C.c_int, C.c_char_p, C.c_int]),
     "drx_walk_batch": (C.c_int, [C.c_void_p, C.POINTER(Batch), C.POINTER(LaunchCfg)]),
     "drx_plan_create": (C.c_int, [C.c_void_p, C.POINTER(Batch), C.POINTER(LaunchCfg), C.POINTER(C.c_void_p)]),
     "drx_plan_run": (C.c_int, [C.c_void_p]),
@@ -195,9 +200,11 @@ class Context:
             self.primes = arr
 
     # -- programs
-    def program_id(self, program) -> int:
+    def program_id(self, program, mode: int = DRX_WALK_MATRIX) -> int:
+        """Upload ``program`` once per packing mode (DRX_WALK_MATRIX: the
+        code computes M(s); DRX_WALK_VECTOR: it computes v' = M(s) v)."""
         key = (program.m, program.dim, tuple((int(i.op), i.arg0, i.arg1, i.arg2) for i in program.opcodes),
-               tuple(int(c) for c in program.constants))
+               tuple(int(c) for c in program.constants), int(mode))
         with self._lock:
             pid = self._programs.get(key)
             if pid is not None:
@@ -209,14 +216,15 @@ class Context:
             out = C.c_int(-1)
             if all(-(1 << 63) <= c < (1 << 63) for c in consts):
                 carr = np.array(consts, dtype=np.int64) if nc else np.zeros(1, np.int64)
-                rc = self.lib.drx_program_upload(self.h, _ptr(ops), len(ops), program.m, program.dim,
-                                                 _ptr(carr), None, None, nc, 0, C.byref(out))
+                rc = self.lib.drx_program_upload_mode(self.h, _ptr(ops), len(ops), program.m, program.dim,
+                                                      _ptr(carr), None, None, nc, 0, int(mode), C.byref(out))
             else:
                 pr = [int(p) for p in self.primes]
                 res = np.array([[c % p for p in pr] for c in consts], dtype=np.uint32)
                 flt = np.array([float(c) for c in consts], dtype=np.float64)
-                rc = self.lib.drx_program_upload(self.h, _ptr(ops), len(ops), program.m, program.dim,
-                                                 None, _ptr(res), _ptr(flt), nc, len(pr), C.byref(out))
+                rc = self.lib.drx_program_upload_mode(self.h, _ptr(ops), len(ops), program.m, program.dim,
+                                                      None, _ptr(res), _ptr(flt), nc, len(pr), int(mode),
+                                                      C.byref(out))
             self._check(rc)
             self._programs[key] = out.value
             return out.value
@@ -290,7 +298,7 @@ def pinned_zeros(shape, dtype) -> np.ndarray:
     return arr
 
 
-def pack_program(program):
+def pack_program(program, mode: int = DRX_WALK_MATRIX):
     """Packed device code of ``program`` (no device needed): (words uint32,
     const_slots int list, meta dict). ``words`` holds the residue code (one
     word per instruction, ``meta["words"]``) and then, from the next even
@@ -310,7 +318,7 @@ def pack_program(program):
     meta = np.zeros(PACK_META, dtype=np.int64)
     err = C.create_string_buffer(256)
     rc = lib.drx_program_pack(ops.ctypes.data, n, program.m, program.dim, consts.ctypes.data,
-                              len(program.constants), words.ctypes.data, cap, slots.ctypes.data, len(slots),
+                              len(program.constants), int(mode), words.ctypes.data, cap, slots.ctypes.data, len(slots),
                               derived.ctypes.data, len(derived), meta.ctypes.data, PACK_META, err, 256)
     if rc != 0:
         raise ValueError(err.value.decode())

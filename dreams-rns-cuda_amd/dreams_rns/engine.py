@@ -23,12 +23,15 @@ INT64_MIN, INT64_MAX = -(1 << 63), (1 << 63) - 1
 
 @dataclass
 class WalkSpec:
-    """One walk: P0 * M(0) * ... * M(depth-1) mod the first K primes.
+    """One walk: P0 * M(0) * ... * M(depth-1) mod the first K primes, or with
+    ``vector`` the state-vector walk v(depth) = M(depth-1) ... M(0) v(0)
+    (cmf_walk.py:445-531) with p = v[out_p], q = v[out_q].
 
     ``axes`` holds (sn, sd, tn, td) per axis: axis i at step s equals
     (sn*td + s*tn*sd) / (sd*td) (python/dreams_rns/cmf_walk.py:206-224).
     ``init`` is "pcf" (A = [[1, a0], [0, 1]], runner.py:376-382), "identity"
-    (cmf_walk.py:396-402) or "given" (init_res / init_f).
+    (cmf_walk.py:396-402) or "given" (init_res [m*m][K] / init_f [m*m]; for a
+    vector walk [m][K] / [m]).
     """
 
     program: Any
@@ -40,6 +43,9 @@ class WalkSpec:
     init_res: Optional[np.ndarray] = None
     init_f: Optional[np.ndarray] = None
     target: float = 0.0
+    vector: bool = False
+    out_p: int = 0
+    out_q: int = 0
 
 
 def pcf_spec(program, a0: int, depth: int, K: int = 32, shift_val: int = 1, target: float = 0.0) -> WalkSpec:
@@ -71,8 +77,7 @@ def parse_rational_list(vals) -> Tuple[List[int], List[int]]:
     return nums, dens
 
 
-def cmf_spec(program, depth: int, K: int, shift_vals, trajectory_vals=None, target: float = 0.0) -> WalkSpec:
-    """run_cmf_walk's walk from the identity (cmf_walk.py:363-431)."""
+def _axes(program, shift_vals, trajectory_vals):
     sn, sd = parse_rational_list(shift_vals)
     if trajectory_vals is not None:
         tn, td = parse_rational_list(trajectory_vals)
@@ -81,9 +86,42 @@ def cmf_spec(program, depth: int, K: int, shift_vals, trajectory_vals=None, targ
     dim = program.dim
     if min(len(sn), len(tn)) < dim:
         raise IndexError(f"program has {dim} axes but {len(sn)} shifts / {len(tn)} trajectory values")
-    axes = [(sn[i], sd[i], tn[i], td[i]) for i in range(dim)]
-    return WalkSpec(program=program, depth=int(depth), K=int(K), axes=axes, init="identity",
-                    target=float(target))
+    return [(sn[i], sd[i], tn[i], td[i]) for i in range(dim)]
+
+
+def cmf_spec(program, depth: int, K: int, shift_vals, trajectory_vals=None, target: float = 0.0) -> WalkSpec:
+    """run_cmf_walk's walk from the identity (cmf_walk.py:363-431)."""
+    return WalkSpec(program=program, depth=int(depth), K=int(K), axes=_axes(program, shift_vals, trajectory_vals),
+                    init="identity", target=float(target))
+
+
+def vec_spec(program, depth: int, K: int, shift_vals, initial_state, acc_idx: int, const_idx: int,
+             trajectory_vals=None, primes=None, target: float = 0.0) -> WalkSpec:
+    """run_cmf_walk_vec's walk (cmf_walk.py:445-531): v(0) from exact
+    rationals (residues num * den^(p-2) mod p, cmf_walk.py:490-502; FP64
+    float(f)); p = v[acc_idx], q = v[const_idx]. ``primes`` (the first K
+    primes) fixes the residues of v(0)."""
+    from fractions import Fraction
+    from .primes import generate_rns_primes
+
+    m = program.m
+    if len(initial_state) != m:
+        raise ValueError(f"initial state has {len(initial_state)} entries, the program's order is {m}")
+    if not (0 <= acc_idx < m and 0 <= const_idx < m):
+        raise IndexError("accumulator / constant index outside the state vector")
+    pr = [int(p) for p in (primes if primes is not None else generate_rns_primes(K))[:K]]
+    res = np.zeros((m, K), dtype=np.uint32)
+    f = np.zeros(m, dtype=np.float64)
+    for i, x in enumerate(initial_state):
+        x = Fraction(x)
+        f[i] = float(x)
+        if x != 0:
+            for k, p in enumerate(pr):
+                num = x.numerator % p
+                res[i, k] = num if x.denominator == 1 else num * pow(x.denominator, p - 2, p) % p
+    return WalkSpec(program=program, depth=int(depth), K=int(K), axes=_axes(program, shift_vals, trajectory_vals),
+                    init="given", init_res=res, init_f=f, target=float(target), vector=True,
+                    out_p=int(acc_idx), out_q=int(const_idx))
 
 
 @dataclass
@@ -145,11 +183,14 @@ def _pack_walks(specs: Sequence[WalkSpec], ctx: N.Context):
     res_off = 0
     for i, s in enumerate(specs):
         w = walks[i]
-        w["prog"] = ctx.program_id(s.program)
+        w["prog"] = ctx.program_id(s.program, N.DRX_WALK_VECTOR if s.vector else N.DRX_WALK_MATRIX)
         w["K"] = s.K
         w["depth"] = s.depth
         w["target"] = s.target
         w["axis_off"] = len(axis)
+        w["mode"] = N.DRX_WALK_VECTOR if s.vector else N.DRX_WALK_MATRIX
+        w["out_p"] = s.out_p
+        w["out_q"] = s.out_q
         for q in s.axes:
             axis.extend(int(v) for v in q)
         m = s.program.m
@@ -167,8 +208,9 @@ def _pack_walks(specs: Sequence[WalkSpec], ctx: N.Context):
                 r[3] = 1
                 f = np.array([1.0, float(int(s.a0)), 0.0, 1.0])
             elif s.init == "given":
-                r = np.ascontiguousarray(s.init_res, dtype=np.uint32).reshape(m * m, s.K)
-                f = np.ascontiguousarray(s.init_f, dtype=np.float64).reshape(m * m)
+                rows = m if s.vector else m * m
+                r = np.ascontiguousarray(s.init_res, dtype=np.uint32).reshape(rows, s.K)
+                f = np.ascontiguousarray(s.init_f, dtype=np.float64).reshape(rows)
             else:
                 raise ValueError(f"unknown init kind {s.init!r}")
             w["init_kind"] = N.DRX_INIT_GIVEN

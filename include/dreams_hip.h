@@ -18,7 +18,7 @@
 extern "C" {
 #endif
 
-#define DRX_ABI_VERSION 2
+#define DRX_ABI_VERSION 3
 
 enum {
   DRX_OK = 0,
@@ -57,6 +57,14 @@ int drx_set_primes(drx_ctx* ctx, const int64_t* primes, int K);
 int drx_program_upload(drx_ctx* ctx, const uint32_t* ops, int n_ops, int m, int dim,
                        const int64_t* consts, const uint32_t* const_res, const double* const_f,
                        int n_const, int kcap, int* prog_id);
+/* As drx_program_upload, packed for walks of kind `mode`: DRX_WALK_MATRIX
+ * (the code computes M(s); m <= 4) or DRX_WALK_VECTOR (the code computes the
+ * next state v' = M(s) v of a state-vector walk, cmf_walk.py:435-520;
+ * m <= 16). The matrix walk's program and the vector walk's are different
+ * uploads of the same bytecode. */
+int drx_program_upload_mode(drx_ctx* ctx, const uint32_t* ops, int n_ops, int m, int dim,
+                            const int64_t* consts, const uint32_t* const_res, const double* const_f,
+                            int n_const, int kcap, int mode, int* prog_id);
 int drx_program_info(drx_ctx* ctx, int prog_id, int* n_dev_ops, int* n_const_used,
                      uint32_t* entry_kinds);
 /* Drop every program. Plans built before, and server requests submitted
@@ -81,8 +89,8 @@ int drx_program_release_all(drx_ctx* ctx);
  * bytecode is interpreted as written (python/dreams_rns/runner.py:217-316). */
 #define DRX_PACK_META 13
 int drx_program_pack(const uint32_t* ops, int n_ops, int m, int dim, const int64_t* consts, int n_const,
-                     uint32_t* words, int max_words, int* const_slots, int max_slots, int32_t* derived,
-                     int max_derived, int64_t* meta, int max_meta, char* err, int err_len);
+                     int mode, uint32_t* words, int max_words, int* const_slots, int max_slots,
+                     int32_t* derived, int max_derived, int64_t* meta, int max_meta, char* err, int err_len);
 
 /* ---- walks ------------------------------------------------------------- */
 enum { DRX_INIT_PCF = 0, DRX_INIT_IDENTITY = 1, DRX_INIT_GIVEN = 2 };
@@ -94,17 +102,25 @@ enum { DRX_MODE_BATCHED = 0, DRX_MODE_PLAIN = 1, DRX_MODE_PERSISTENT = 2 };
  * value (sn*td + s*tn*sd) / (sd*td) (python/dreams_rns/cmf_walk.py:206-224).
  * A PCF walk (runner.py:338-413) is sn = shift_val, tn = direction,
  * sd = td = 1, init_kind DRX_INIT_PCF with a0. */
+/* A state-vector walk (mode DRX_WALK_VECTOR, run_cmf_walk_vec,
+ * cmf_walk.py:445-531) is v(depth) = M(depth-1) ... M(0) v(0) with p =
+ * v[out_p], q = v[out_q]; its program is uploaded with mode DRX_WALK_VECTOR,
+ * its initial state given (DRX_INIT_GIVEN: [m][K] residues and [m] FP64
+ * entries). A batch holds walks of one mode. */
+enum { DRX_WALK_MATRIX = 0, DRX_WALK_VECTOR = 1 };
 typedef struct drx_walk {
   int32_t prog;
   int32_t K;
   int64_t depth;
   int32_t init_kind;
-  int32_t reserved;
+  int32_t mode;        /* DRX_WALK_MATRIX or DRX_WALK_VECTOR */
   int64_t a0;          /* DRX_INIT_PCF: A = [[1, a0], [0, 1]] */
-  int64_t init_off;    /* DRX_INIT_GIVEN: first of m*m*K residues ([m*m][K]) in init_res_pool */
-  int64_t init_f_off;  /* DRX_INIT_GIVEN: first of m*m FP64 entries in init_f_pool */
+  int64_t init_off;    /* DRX_INIT_GIVEN: first of m*m*K residues ([m*m][K]; vector walks [m][K]) in init_res_pool */
+  int64_t init_f_off;  /* DRX_INIT_GIVEN: first of m*m FP64 entries (vector walks m) in init_f_pool */
   int64_t axis_off;    /* index of sn of axis 0 in the axis pool */
   double target;       /* L for delta_float */
+  int32_t out_p;       /* vector walks: state entry of p */
+  int32_t out_q;       /* vector walks: state entry of q */
 } drx_walk;
 
 typedef struct drx_walk_out {

@@ -16,7 +16,7 @@ from typing import List, Sequence
 
 D_ADD, D_SUB, D_MUL, D_NEG, D_SQR, D_CUB, D_INV, D_RED, D_MOV, D_NZ1, D_ZSEL = range(1, 12)
 DB_R, DB_E, DB_S, DB_NONE = range(4)
-SB_R, SB_X, SB_C, SB_L, SB_S, SB_A, SB_Z = range(7)
+SB_R, SB_X, SB_C, SB_L, SB_S, SB_A, SB_Z, SB_V = range(8)
 STEP_SLOT, INC_SLOT = 15, 8
 
 
@@ -41,7 +41,7 @@ def _bal(v: int, p: int) -> int:
     return (v + p // 2) % p - p // 2
 
 
-RO_MUL, RO_LIN, RO_INV, RO_RED, RO_NZ1, RO_ZSEL, RO_LDM, RO_STM, RO_LDC, RO_MULNZ = range(10)
+RO_MUL, RO_LIN, RO_INV, RO_RED, RO_NZ1, RO_ZSEL, RO_LDM, RO_STM, RO_LDC, RO_MULNZ, RO_LDV = range(11)
 RES_STEP = 15
 
 
@@ -60,7 +60,7 @@ def _sign(code):
     return {0: 0, 1: 1, 2: -1}[code]
 
 
-def run_res(code, C, lc, p, nrows):
+def run_res(code, C, lc, p, nrows, state=None):
     """One step of the residue interpreter with lazy residues as on the
     device: LIN results stay unreduced, products and reductions are
     balanced; a multiply (or inversion) operand beyond 2^40 would break the
@@ -71,7 +71,7 @@ def run_res(code, C, lc, p, nrows):
     E = [None] * 16
     M = [0] * max(1, nrows)
     for op, x, yc, y, d, es, e, sgn, aux in code:
-        if op in (RO_LDM, RO_STM, RO_LDC):
+        if op in (RO_LDM, RO_STM, RO_LDC, RO_LDV):
             assert es == 0, "entry store on a row instruction"
         xv = R[x]
         if op in (RO_MUL, RO_LIN, RO_ZSEL, RO_MULNZ):
@@ -100,6 +100,8 @@ def run_res(code, C, lc, p, nrows):
             r = xv
         elif op == RO_LDC:
             r = lc[aux]
+        elif op == RO_LDV:
+            r = state[aux]
         else:
             raise AssertionError(f"unknown residue op {op}")
         R[d] = r
@@ -108,7 +110,7 @@ def run_res(code, C, lc, p, nrows):
     return E
 
 
-def run_program(code, fp: bool, X: List, consts: Sequence, p: int = 0, nslots: int = 0):
+def run_program(code, fp: bool, X: List, consts: Sequence, p: int = 0, nslots: int = 0, V=None):
     """One step of the FP64 interpreter (two-word instructions); returns the
     entry bank E (16 slots)."""
     assert fp
@@ -131,6 +133,8 @@ def run_program(code, fp: bool, X: List, consts: Sequence, p: int = 0, nslots: i
             return S[i]
         if bank == SB_A:
             return acc
+        if bank == SB_V:
+            return V[i & 15]
         return zero
 
     for op, db, d, ab, a, bb, b in code:
@@ -227,3 +231,55 @@ def walk(program, words, meta, const_slots, primes, axes, depth, init_res, init_
             mf.append(E[e] if kd == 2 else (1.0 if kd == 1 else 0.0))
         mfs.append(mf)
     return res, mfs
+
+
+def walk_vec(program, words, meta, const_slots, primes, axes, depth, v0_res, v0_f, acc_idx, const_idx):
+    """A state-vector walk through the model of the vector-packed code: the
+    residue code reads the state with LDV and stores the new state as its
+    entries; the FP64 code reads bank V; the shadow rescales by max |v_f|
+    above 1e10 (NaN: no rescale)."""
+    import math
+
+    m, dim = program.m, program.dim
+    code = decode_res(words[: meta["words"]])
+    fcode = decode(words[meta["words"] + (meta["words"] & 1):])
+    out_res = []
+    for k, p in enumerate(primes):
+        p = int(p)
+        cp = all_constants(program, meta, p)
+        consts = [cp[ci] for ci in const_slots]
+        n_cbank = RES_STEP - dim
+        lc = consts[n_cbank:]
+        v = [int(v0_res[j][k]) for j in range(m)]
+        for s in range(depth):
+            C = [0] * 16
+            for i in range(min(n_cbank, len(consts))):
+                C[i] = consts[i]
+            for ax in range(dim):
+                sn, sd, tn, td = axes[ax]
+                num, den = sn * td + s * tn * sd, sd * td
+                C[RES_STEP - 1 - ax] = num % p * pow(den % p, p - 2, p) % p if den != 1 else num % p
+            C[RES_STEP] = s
+            E = run_res(code, C, lc, p, meta["nslots"], v)
+            assert all(E[i] is not None and abs(E[i]) < (1 << 31) for i in range(m)), "state not stored reduced"
+            v = E[:m]
+        out_res.append([x % p for x in v])
+    cf = all_constants(program, meta)
+    fconsts = [cf[ci] for ci in const_slots]
+    vf = [float(x) for x in v0_f] + [0.0] * (16 - m)
+    log_scale = 0.0
+    for s in range(depth):
+        X = [0.0] * 16
+        for ax in range(dim):
+            sn, sd, tn, td = axes[ax]
+            X[ax] = float(sn * td + s * tn * sd) / float(sd * td)
+        X[STEP_SLOT] = float(s)
+        E = run_program(fcode, True, X, fconsts, 0, meta["fnslots"], vf)
+        new = [E[i] for i in range(m)]
+        mx = max(abs(x) for x in new)
+        if not any(x != x for x in new) and mx > 1e10:
+            new = [x / mx for x in new]
+            log_scale += math.log(mx)
+        vf = new + [0.0] * (16 - m)
+    res = [[out_res[k][j] for k in range(len(primes))] for j in range(m)]
+    return res, vf[:m], log_scale

@@ -141,7 +141,9 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_header():
     import ctypes as C
 
-    assert C.sizeof(N.Walk) == 64 and N.WALK_DTYPE.itemsize == 64
+    assert C.sizeof(N.Walk) == 72 and N.WALK_DTYPE.itemsize == 72
+    assert N.Walk.mode.offset == 20 and N.Walk.out_p.offset == 64 and N.Walk.out_q.offset == 68
+    assert N.WALK_DTYPE.fields["mode"][1] == 20 and N.WALK_DTYPE.fields["out_q"][1] == 68
     assert C.sizeof(N.WalkOut) == 48 and N.OUT_DTYPE.itemsize == 48
     assert C.sizeof(N.ServerResult) == 8 + 48 + 8 + 24
 

@@ -200,3 +200,40 @@ def test_dependent_inversion_stays_separate():
     assert meta["inv_batched"] == 3
     words, _, meta = N.pack_program(p)
     assert sum(1 for c in _res_code(words, meta) if c[0] == devmodel.RO_INV) == 2
+
+
+def test_vector_packing_runs_the_state_walk():
+    # vector-packed odd-zeta programs (4x4 / 6x6 / 8x8, 6 / 9 / 12 axes):
+    # the code computes v' = M(s) v; through the device model it equals the
+    # oracle's state-vector walk in residues and, within rounding, in FP64
+    from fractions import Fraction
+    import odd_zeta as oz
+
+    fx = oz.load()
+    for name, spec in fx["specs"].items():
+        prog = oz.compile_spec(spec)
+        words, slots, meta = N.pack_program(prog, N.DRX_WALK_VECTOR)
+        v0 = oz.initial_state(spec["n"])
+        shift = oz.parse_vec("[2,3,2,3,1,2,3,1,2,3,1,2]")[: spec["dim"]]
+        traj = [Fraction(1, 3)] + [Fraction(d) for d in spec["directions"][1:]]
+        axes = [(s.numerator, s.denominator, t.numerator, t.denominator) for s, t in zip(shift, traj)]
+        v0_res = [[(x.numerator % p) * pow(x.denominator, p - 2, p) % p for p in PRIMES] for x in v0]
+        res, vf, ls = devmodel.walk_vec(prog, words, meta, slots, PRIMES, axes, 7, v0_res, [float(x) for x in v0],
+                                        spec["accumulator_idx"], spec["constant_idx"])
+        ref = ow.run_cmf_walk_vec(prog, 7, len(PRIMES), shift, v0, spec["accumulator_idx"], spec["constant_idx"],
+                                  traj)
+        assert res[spec["accumulator_idx"]] == ref["p_residues"].tolist(), name
+        assert res[spec["constant_idx"]] == ref["q_residues"].tolist(), name
+        for got, want in ((vf[spec["accumulator_idx"]], ref["p_float"]), (vf[spec["constant_idx"]], ref["q_float"])):
+            assert abs(got - want) <= 1e-13 * abs(want), (name, got, want)
+        assert abs(ls - ref["log_scale"]) <= 1e-12 * max(1.0, abs(ls)), name
+
+
+def test_vector_packing_limits():
+    p = compile_cmf_from_dict({(0, 0): "x", (1, 1): "1"}, 2, 1, ["x"])
+    N.pack_program(p, N.DRX_WALK_VECTOR)
+    big = compile_cmf_from_dict({(i, i): "x + %d" % i for i in range(17)}, 17, 1, ["x"])
+    with pytest.raises(ValueError):
+        N.pack_program(big, N.DRX_WALK_VECTOR)        # 17 state entries
+    with pytest.raises(ValueError):
+        N.pack_program(p, 5)                           # unknown mode
