@@ -39,16 +39,17 @@ def log(*a):
 
 
 # ------------------------------------------------------------- work model
-def program_work(prog) -> float:
+def program_work(prog, vector: bool = False) -> float:
     """Integer-VALU ops per residue-step by the fixed convention
     W = 6*MM + 2*MA with MM = #MUL + #POW2 + 2*#POW3 + 56*#INV + m^3 and
-    MA = #ADD + #SUB + #NEG + m^2 (m - 1), counted on the walk's bytecode."""
+    MA = #ADD + #SUB + #NEG + m^2 (m - 1), counted on the walk's bytecode;
+    a state-vector walk's product is a matrix-vector product (m^2, m (m - 1))."""
     cnt = {}
     for ins in prog.opcodes:
         cnt[int(ins.op)] = cnt.get(int(ins.op), 0) + 1
     m = prog.m
-    mm = cnt.get(6, 0) + cnt.get(8, 0) + 2 * cnt.get(9, 0) + 56 * cnt.get(10, 0) + m ** 3
-    ma = cnt.get(4, 0) + cnt.get(5, 0) + cnt.get(7, 0) + m * m * (m - 1)
+    mm = cnt.get(6, 0) + cnt.get(8, 0) + 2 * cnt.get(9, 0) + 56 * cnt.get(10, 0) + (m * m if vector else m ** 3)
+    ma = cnt.get(4, 0) + cnt.get(5, 0) + cnt.get(7, 0) + (m * (m - 1) if vector else m * m * (m - 1))
     return 6.0 * mm + 2.0 * ma
 
 
@@ -163,7 +164,31 @@ def other_workload(args):
 
     cf = lambda mat, m, dim, axes, dirs: D.compile_cmf_from_dict(mat, m, dim, axes, dirs)
     t0 = time.perf_counter()
-    if args.workload == "cmf3":
+    cpu = None
+    if args.workload == "zeta5":
+        from dreams_rns.engine import vec_spec
+
+        cfg = synth.zeta5_sweep(cf)
+        specs = [vec_spec(cfg["program"], cfg["depth"], cfg["K"], sh, cfg["v0"], cfg["acc"], cfg["const"], tr)
+                 for sh in cfg["shifts"] for tr in cfg["trajs"]]
+        top_k = 64
+        desc = (f"zeta5: {len(specs)} state-vector walks of the zeta(5) CMF (4x4, 6 axes), "
+                f"{len(cfg['shifts'])} shifts x {len(cfg['trajs'])} trajectories, depth {cfg['depth']}, K={cfg['K']}")
+        if not args.no_cpu:
+            from oracle import walk as ow
+
+            t = time.perf_counter()
+            nw = 0
+            for sh, tr in zip(cfg["shifts"], cfg["trajs"]):
+                ow.run_cmf_walk_vec(cfg["program"], cfg["depth"], cfg["K"], sh, cfg["v0"], cfg["acc"], cfg["const"], tr)
+                nw += 1
+                if time.perf_counter() - t > 10.0:
+                    break
+            el = time.perf_counter() - t
+            cpu = {"value": nw * cfg["depth"] * cfg["K"] / el, "unit": "residue-steps/s", "cores": 1, "kind": "port",
+                   "sample": f"{nw} walks of the sweep through the numpy restatement, one core"}
+            log(f"cpu: {cpu['value']:.3e} residue-steps/s ({nw} walks)")
+    elif args.workload == "cmf3":
         n = args.walks if args.walks != 4096 else 1024
         cfg = synth.cmf3(cf, n_walks=n)
         specs = [cmf_spec(w["program"], cfg["depth"], cfg["K"], w["shift"], target=0.25) for w in cfg["walks"]]
@@ -181,10 +206,13 @@ def other_workload(args):
                 specs.append(cmf_spec(w["program"], w["depth"], w["K"], w["shift"], target=w["L"]))
         top_k = cfg["top_k"]
         desc = f"mix4: {n} uneven walks (2x2/3x3/4x4, depth 500..50000, K 16..128)"
-    work = sum(program_work(s.program) * s.depth * s.K for s in specs)
+    work = sum(program_work(s.program, s.vector) * s.depth * s.K for s in specs)
     rsteps = float(sum(s.depth * s.K for s in specs))
     log(f"generated {len(specs)} walks in {time.perf_counter() - t0:.1f}s")
     tuned = None
+    if args.workload == "zeta5":
+        args.threads = args.threads or 256
+        args.chunk = args.chunk or 4096
     if not (args.threads or args.chunk):
         from dreams_rns.engine import autotune
 
@@ -219,8 +247,9 @@ def other_workload(args):
         "roofline": {"bound": "valu", "achieved": achieved / 1e12 if achieved else None,
                      "peak": PEAK_LANE_OPS / 1e12, "unit": "TOP/s",
                      "frac": achieved / PEAK_LANE_OPS if achieved else None, "traffic": None,
-                     "kernel": walk_kernel_name([s.program for s in specs]), "kernel_ms": walk_avg,
-                     "work_per_launch": work},
+                     "kernel": "k_walk_vec" if args.workload == "zeta5" else walk_kernel_name([s.program for s in specs]),
+                     "kernel_ms": walk_avg, "work_per_launch": work},
+        "cpu_baseline": cpu,
     }))
 
 
@@ -241,7 +270,7 @@ def main():
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--graph", action="store_true", help="replay the batch as a hipGraph")
-    ap.add_argument("--workload", choices=("pcf2", "cmf3", "mix4"), default="pcf2",
+    ap.add_argument("--workload", choices=("pcf2", "cmf3", "mix4", "zeta5"), default="pcf2",
                     help="pcf2 (the headline line); cmf3 / mix4 measure the other batch shapes "
                          "(no CPU or served legs)")
     ap.add_argument("--profile", action="store_true",

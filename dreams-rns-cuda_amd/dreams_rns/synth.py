@@ -221,3 +221,54 @@ def serve5_request(rng, compile_fn) -> Dict:
     spec = cmf_walk_spec(rng, m, compile_fn, 10, 400, False)
     spec.update({"kind": "cmf", "depth": depth, "K": K, "L": float(rng.uniform(-5, 5))})
     return spec
+
+
+# ------------------------------------------- state-vector walks (odd zeta)
+# The zeta(5) state-vector CMF of the reference's sweep drivers
+# (python/odd_zeta_specs.jsonl, first line): a 4x4 matrix over 6 axes, the
+# state v(0) = (1/2, 0, 1, 1) and p = v[2], q = v[3].
+ZETA5 = {
+    "matrix": {(0, 0): "-x0/(2*x1)", (1, 0): "-x2/(2*x3*x4**2)", (1, 1): "-x2/(2*x3)",
+               (2, 0): "-x0/(x1*x5**5) + 5*x2/(4*x3*x4**2*x5**3)", (2, 1): "5*x2/(4*x3*x5**3)",
+               (2, 2): "1", (3, 3): "1"},
+    "m": 4, "axes": ["x0", "x1", "x2", "x3", "x4", "x5"], "directions": [1, 2, 1, 2, 1, 1],
+    "default_shifts": [2, 3, 2, 3, 1, 2], "acc": 2, "const": 3,
+}
+
+
+def zeta5_sweep(compile_fn, n_shift: int = 64, n_traj: int = 256, depth: int = 1000, K: int = 16) -> Dict:
+    """A sweep of the zeta(5) walk over shifts x trajectories, as the
+    reference's odd-zeta drivers run it (odd_zeta_sweep.py:481-494; its
+    documented sweep is 50 shifts x 1000 trajectories at depth 1000, K = 16):
+    shifts are the default with one axis moved by an offset in {+-1/4 .. +-3},
+    trajectories the default directions with one or two axes scaled by
+    rational multipliers. Deterministic."""
+    from fractions import Fraction
+
+    z = ZETA5
+    dim = len(z["axes"])
+    offs = [Fraction(o) for o in ("1", "-1", "1/2", "-1/2", "1/3", "-1/3", "2", "-2", "1/4", "-1/4", "3", "-3")]
+    mults = [Fraction(x) for x in ("1/3", "1/2", "2/3", "3/2", "2", "3", "-1", "-1/2", "-1/3", "4/3", "5/2")]
+    shifts = [[Fraction(s) for s in z["default_shifts"]]]
+    for o, a in itertools.product(offs, range(dim)):
+        if len(shifts) >= n_shift:
+            break
+        v = list(shifts[0])
+        v[a] += o
+        shifts.append(v)
+    trajs = [[Fraction(d) for d in z["directions"]]]
+    for mu, a in itertools.product(mults, range(dim)):
+        v = list(trajs[0])
+        v[a] *= mu
+        trajs.append(v)
+    for (m1, m2), (a, b) in itertools.product(itertools.product(mults, mults), itertools.combinations(range(dim), 2)):
+        if len(trajs) >= n_traj:
+            break
+        v = list(trajs[0])
+        v[a] *= m1
+        v[b] *= m2
+        trajs.append(v)
+    program = compile_fn(z["matrix"], z["m"], dim, z["axes"], z["directions"])
+    v0 = [Fraction(1, 2), Fraction(0), Fraction(1), Fraction(1)]
+    return {"program": program, "shifts": shifts[:n_shift], "trajs": trajs[:n_traj], "depth": depth, "K": K,
+            "v0": v0, "acc": z["acc"], "const": z["const"]}
