@@ -158,7 +158,7 @@ static size_t lds_const_rows(const DevProgram& pg) {
 size_t walk_lds_bytes(const DevProgram& pg, int kind, int m, uint32_t K, int threads) {
   const size_t tree = (size_t)threads * m * m;  // [C][m*m][KP] with C * KP = threads
   if (kind != WALK_INTERP) return 8 * tree;
-  const size_t KP = pow2_ceil(K);
+  const size_t KP = pow2_ceil(std::min<uint32_t>(K, kSliceK));  // one prime slice
   const size_t nl = lds_const_rows(pg);
   // constant and axis-increment rows by prime, spill and entry rows by
   // thread; the tree overlays them
@@ -693,7 +693,8 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
     if (w.init_kind == DRX_INIT_GIVEN && (!b.init_res_pool || !b.init_f_pool))
       return fail(ctx, DRX_EINVAL, "walk %d: DRX_INIT_GIVEN without init pools", i);
     if (hp.dev.dim > 0 && !b.axis_pool) return fail(ctx, DRX_EINVAL, "walk %d: no axis pool", i);
-    if (pow2_ceil(w.K) > 256) return fail(ctx, DRX_ERANGE, "walk %d: K=%d above the 256 primes of one workgroup", i, w.K);
+    if (w.mode == DRX_WALK_VECTOR && pow2_ceil(w.K) > 256)
+      return fail(ctx, DRX_ERANGE, "walk %d: K=%d above the 256 primes of a vector walk", i, w.K);
     if (w.mode != DRX_WALK_MATRIX && w.mode != DRX_WALK_VECTOR) return fail(ctx, DRX_EINVAL, "walk %d: bad mode", i);
     if (w.mode != hp.mode)
       return fail(ctx, DRX_EINVAL, "walk %d: program %d was uploaded for the other walk mode", i, w.prog);
@@ -721,7 +722,9 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
     return (double)w.depth * (double)w.K * (1.0 + ctx->progs[w.prog].pk.code.size());
   };
   const bool interp_only = pl->cfg.interp_only != 0;
-  auto kind_of = [&](int i) { return walk_kind(ctx->progs[b.walks[i].prog], interp_only); };
+  // walks of more than kSliceK primes run as prime slices of the interpreter
+  auto kind_of = [&](int i) { return walk_kind(ctx->progs[b.walks[i].prog], interp_only || b.walks[i].K > kSliceK); };
+  auto slice_kp = [](int K) { return (int)pow2_ceil((uint32_t)std::min(K, kSliceK)); };
   // interpreted walks of one order: LDS classes whose LDS size would lower
   // the occupancy of the smaller classes get stages of their own
   std::vector<int> group(n, 0);
@@ -733,7 +736,7 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
       int kp = 1;
       for (int i = 0; i < n; ++i) {
         const HostProgram& hp = ctx->progs[b.walks[i].prog];
-        if ((int)hp.dev.m == m && kind_of(i) == WALK_INTERP) kp = std::max<int>(kp, pow2_ceil(b.walks[i].K));
+        if ((int)hp.dev.m == m && kind_of(i) == WALK_INTERP) kp = std::max<int>(kp, slice_kp(b.walks[i].K));
       }
       const int thr = plain_mode ? kp : std::max<int>(thr_cfg, kp);
       for (int i = 0; i < n; ++i) {
@@ -838,11 +841,11 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
       g.span_begin = g.span_end = (int)hs.size();
       g.seed_begin = g.seed_end = (int)hs.size();
       g.range_begin = g.range_end = (int)hr.size();
-      g.threads = plain ? (int)pow2_ceil(K) : std::max<int>(thr_cfg, pow2_ceil(K));
+      g.threads = plain ? slice_kp(K) : std::max<int>(thr_cfg, slice_kp(K));
       pl->stages.push_back(g);
     }
     Stage& g = pl->stages.back();
-    g.threads = std::max<int>(g.threads, pow2_ceil(K));
+    g.threads = std::max<int>(g.threads, slice_kp(K));
     g.seed_kp = std::max<int>(g.seed_kp, pow2_ceil(K));
     DevSpan sp{};
     sp.walk = i;
@@ -860,14 +863,18 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
       t.mf_off = mf_total;
       cp_total += (uint64_t)m * m * K;
       mf_total += (uint64_t)(t.s_end - t.s_begin) * m * m;
-      for (int64_t r = t.s_begin; r < t.s_end; r += kMfRange) {
+      for (int64_t r = t.s_begin; r < t.s_end; r += kMfRange) {  // M_f once per range: on its first slice
         DevRange rg{};
         rg.task = (int)ht.size();
         rg.s_begin = r;
         rg.s_end = std::min<int64_t>(t.s_end, r + kMfRange);
         hr.push_back(rg);
       }
-      ht.push_back(t);
+      for (int k0 = 0; k0 < K; k0 += kSliceK) {
+        t.k0 = k0;
+        t.kn = std::min(kSliceK, K - k0);
+        ht.push_back(t);
+      }
     }
     sp.task_end = (int)ht.size();
     hs.push_back(sp);
@@ -891,7 +898,7 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
       for (int si = g.span_begin; si < g.span_end; ++si) {
         const DevWalk& dw = hw[hs[si].walk];
         g.lds = std::max(g.lds, walk_lds_bytes(ctx->progs[dw.prog].dev, g.kind, g.m, (uint32_t)dw.K, g.threads));
-        kp_max = std::max<size_t>(kp_max, pow2_ceil((uint32_t)dw.K));
+        kp_max = std::max<size_t>(kp_max, slice_kp(dw.K));
       }
       if (g.lds <= lds_limit || (size_t)g.threads / 2 < kp_max) break;
       g.threads /= 2;
@@ -1583,7 +1590,7 @@ int drx_autotune(drx_ctx* ctx, const drx_batch* b, const drx_launch_cfg* base, i
   }
   int kmax = 1;
   for (const drx_walk& w : smp->walks) kmax = std::max(kmax, w.K);
-  const int kp = (int)pow2_ceil((uint32_t)kmax);
+  const int kp = (int)pow2_ceil((uint32_t)std::min(kmax, kSliceK));
   double best_t = 1e300;
   drx_launch_cfg best_cfg = cfg0;
   for (int thr : thr_c) {

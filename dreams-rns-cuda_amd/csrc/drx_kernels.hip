@@ -162,7 +162,7 @@ struct ProgLds {
 // a KP-wide stripe (row i = constant slot 15 - dim + i); the caller
 // synchronises.
 __device__ __forceinline__ ProgLds prog_lds(const KArgs& a, const DevProgram& pg, uint32_t K, uint32_t KP,
-                                            double* lds) {
+                                            uint32_t k0, double* lds) {
   const uint32_t T = blockDim.x, tid = threadIdx.x;
   const uint32_t nc = (uint32_t)kResStep - pg.dim;
   const uint32_t nl = pg.nconst > nc ? pg.nconst - nc : 0u;
@@ -173,7 +173,7 @@ __device__ __forceinline__ ProgLds prog_lds(const KArgs& a, const DevProgram& pg
   L.li = L.le + (uint64_t)pg.m * pg.m * T;  // [dim][KP], by prime
   for (uint32_t i = tid; i < nl * KP; i += T) {
     const uint32_t ci = i / KP, ck = i % KP;
-    L.lc[i] = (ck < K) ? a.cres[pg.cres_off + (uint64_t)(nc + ci) * pg.kcap + ck] : 0.0;
+    L.lc[i] = (ck < K) ? a.cres[pg.cres_off + (uint64_t)(nc + ci) * pg.kcap + k0 + ck] : 0.0;
   }
   return L;
 }
@@ -183,7 +183,7 @@ struct WalkSetup {
   DevWalk w;
   DevProgram pg;
   Lane ln;
-  uint32_t K, KP, C, c, k, kk;
+  uint32_t Kw, k0, K, KP, C, c, k, kk, kg;  // walk's K; slice k0 + [0, K); kg = k0 + kk
   bool active;
   int64_t sb, se;
   const uint32_t* code;
@@ -197,16 +197,19 @@ __device__ __forceinline__ WalkSetup walk_setup(const KArgs& a, double* lds, boo
   S.w = a.walks[S.t.walk];
   S.pg = a.progs[S.w.prog];
   const uint32_t T = blockDim.x, tid = threadIdx.x;
-  S.K = (uint32_t)S.w.K;
+  S.Kw = (uint32_t)S.w.K;
+  S.k0 = (uint32_t)S.t.k0;
+  S.K = (uint32_t)S.t.kn;
   S.KP = pow2_at_least(S.K);
   S.C = T / S.KP;
   S.k = tid % S.KP;
   S.c = tid / S.KP;
   S.active = S.k < S.K;
   S.kk = S.active ? S.k : 0u;
-  S.ln.p = a.prime_p[S.kk];
-  S.ln.pinv = a.prime_pinv[S.kk];
-  S.ln.pe = a.prime_e[S.kk];
+  S.kg = S.k0 + S.kk;
+  S.ln.p = a.prime_p[S.kg];
+  S.ln.pinv = a.prime_pinv[S.kg];
+  S.ln.pe = a.prime_e[S.kg];
   S.ln.k = S.k;
   S.ln.tid = tid;
   S.ln.T = T;
@@ -215,7 +218,7 @@ __device__ __forceinline__ WalkSetup walk_setup(const KArgs& a, double* lds, boo
   // the steps are done (tree_combine synchronises before its first write)
   S.tree = lds;
   if (interp) {
-    const ProgLds L = prog_lds(a, S.pg, S.K, S.KP, lds);
+    const ProgLds L = prog_lds(a, S.pg, S.K, S.KP, S.k0, lds);
     S.lc = L.lc;
     S.lm = L.lm;
     S.le = L.le;
@@ -320,7 +323,7 @@ __device__ __forceinline__ void tree_combine(const KArgs& a, const WalkSetup& S,
   }
   if (c == 0 && S.active) {
 #pragma unroll
-    for (int e = 0; e < M * M; ++e) a.cp[S.t.cp_off + (uint64_t)e * S.K + k] = P[e];
+    for (int e = 0; e < M * M; ++e) a.cp[S.t.cp_off + (uint64_t)e * S.Kw + S.k0 + k] = P[e];
   }
 }
 
@@ -331,7 +334,7 @@ __global__ void __launch_bounds__(256) k_walk_chunk(const KArgs a) {
   const WalkSetup S = walk_setup(a, lds, true);
   const double p = S.ln.p, pinv = S.ln.pinv;
   v16d R = 0.0, C = 0.0;
-  load_rns_inputs(a, S.w, S.pg, S.K, S.kk, S.ln, S.sb, C, S.c == 0 ? S.li : nullptr, S.KP, S.k);
+  load_rns_inputs(a, S.w, S.pg, S.Kw, S.kg, S.ln, S.sb, C, S.c == 0 ? S.li : nullptr, S.KP, S.k);
   __syncthreads();
   double P[M * M];
 #pragma unroll
@@ -377,7 +380,7 @@ __global__ void __launch_bounds__(512) k_fd_seed(const KArgs a, uint32_t kp) {
   ln.k = k;
   ln.tid = threadIdx.x;
   ln.T = blockDim.x;
-  const ProgLds L = prog_lds(a, pg, K, KP, lds);
+  const ProgLds L = prog_lds(a, pg, K, KP, 0u, lds);
   __syncthreads();
   const uint32_t* code = a.code + pg.code_off;
   double* tab = a.fdt + w.fdt_off;
@@ -461,16 +464,16 @@ __device__ __forceinline__ void pcf_fd_body(const KArgs& a, const WalkSetup& S) 
   constexpr int U = fd_unroll(DB > DA ? DB : DA);
   constexpr int D = kFdMax2;
   const double p = S.ln.p, pinv = S.ln.pinv;
-  const double* tb = a.fdt + S.w.fdt_off + (uint64_t)1 * (D + 1) * S.K + S.kk;  // entry (0, 1) = b
-  const double* ta = a.fdt + S.w.fdt_off + (uint64_t)3 * (D + 1) * S.K + S.kk;  // entry (1, 1) = a
+  const double* tb = a.fdt + S.w.fdt_off + (uint64_t)1 * (D + 1) * S.Kw + S.kg;  // entry (0, 1) = b
+  const double* ta = a.fdt + S.w.fdt_off + (uint64_t)3 * (D + 1) * S.Kw + S.kg;  // entry (1, 1) = a
   double Bt[DB + 1], At[DA + 1];
   const uint32_t kb = (S.pg.kinds >> 2) & 3u, ka = (S.pg.kinds >> 6) & 3u;  // uniform
 #pragma unroll
-  for (int j = 0; j <= DB; ++j) Bt[j] = (kb == E_GEN) ? tb[(uint64_t)j * S.K] : (j == 0 && kb == E_ONE ? 1.0 : 0.0);
+  for (int j = 0; j <= DB; ++j) Bt[j] = (kb == E_GEN) ? tb[(uint64_t)j * S.Kw] : (j == 0 && kb == E_ONE ? 1.0 : 0.0);
 #pragma unroll
-  for (int j = 0; j <= DA; ++j) At[j] = (ka == E_GEN) ? ta[(uint64_t)j * S.K] : (j == 0 && ka == E_ONE ? 1.0 : 0.0);
-  fd_jump<DB>(Bt, S.sb, a, S.kk, p, pinv);
-  fd_jump<DA>(At, S.sb, a, S.kk, p, pinv);
+  for (int j = 0; j <= DA; ++j) At[j] = (ka == E_GEN) ? ta[(uint64_t)j * S.Kw] : (j == 0 && ka == E_ONE ? 1.0 : 0.0);
+  fd_jump<DB>(Bt, S.sb, a, S.kg, p, pinv);
+  fd_jump<DA>(At, S.sb, a, S.kg, p, pinv);
   double x0 = 1.0, y0 = 0.0, x1 = 0.0, y1 = 1.0;
   int64_t n = S.se - S.sb;
   for (; n >= U; n -= U) {
@@ -535,10 +538,10 @@ __global__ void __launch_bounds__(256) k_walk_fd(const KArgs a) {
   double N[M * M][D + 1];
 #pragma unroll
   for (int e = 0; e < M * M; ++e) {
-    const double* t = a.fdt + S.w.fdt_off + (uint64_t)e * (D + 1) * S.K + S.kk;
+    const double* t = a.fdt + S.w.fdt_off + (uint64_t)e * (D + 1) * S.Kw + S.kg;
 #pragma unroll
-    for (int j = 0; j <= D; ++j) N[e][j] = t[(uint64_t)j * S.K];
-    fd_jump<D>(N[e], S.sb, a, S.kk, p, pinv);
+    for (int j = 0; j <= D; ++j) N[e][j] = t[(uint64_t)j * S.Kw];
+    fd_jump<D>(N[e], S.sb, a, S.kg, p, pinv);
   }
   double P[M * M];
 #pragma unroll
@@ -561,7 +564,9 @@ __global__ void __launch_bounds__(256) k_walk_fd(const KArgs a) {
 }
 
 // --------------------------------------------------------- ordered combine
-// state <- state * cp[t0] * cp[t1] * ... over the walk's tasks (step order).
+// state <- state * cp[t0] * cp[t1] * ... over the walk's step ranges in
+// order (the prime slices of a range share its chunk product: one task per
+// range, the one with k0 == 0, stands for it).
 template <int M>
 __global__ void k_combine(const KArgs a) {
   const DevSpan sp = a.spans[blockIdx.x];
@@ -573,6 +578,7 @@ __global__ void k_combine(const KArgs a) {
 #pragma unroll
     for (int e = 0; e < M * M; ++e) S[e] = a.res_in[w.res_off + (uint64_t)e * K + k];
     for (int ti = sp.task_begin; ti < sp.task_end; ++ti) {
+      if (a.tasks[ti].k0 != 0) continue;  // uniform
       const uint64_t off = a.tasks[ti].cp_off;
       double Q[M * M];
 #pragma unroll
@@ -912,7 +918,7 @@ __global__ void __launch_bounds__(256) k_walk_vec(const KArgs a) {
   ln.T = T;
   // LDS: constant rows [nl][KP], spill rows [nslots][T], state A and B
   // [m][T] each, axis increments [dim][T]
-  const ProgLds L = prog_lds(a, pg, K, KP, lds);
+  const ProgLds L = prog_lds(a, pg, K, KP, 0u, lds);
   double* sa = L.lm + (uint64_t)pg.nslots * T;
   double* sb = sa + (uint64_t)m * T;
   double* li = sb + (uint64_t)m * T;

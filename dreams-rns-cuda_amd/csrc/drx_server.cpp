@@ -168,6 +168,8 @@ int round_once(drx_server* s, std::vector<int>& finished) {
     t.s_begin = sl.step;
     t.s_end = std::min<int64_t>(sl.w.depth, sl.step + s->cfg.round_steps);
     t.cp_off = (uint64_t)ht.size() * slot_res(s);
+    t.k0 = 0;
+    t.kn = dw.K;
     mf_total = (mf_total + 1) & ~(uint64_t)1;  // 16-byte aligned M_f
     t.mf_off = mf_total;
     mf_total += (uint64_t)(t.s_end - t.s_begin) * r.m * r.m;

@@ -70,7 +70,14 @@ struct DevTask {
   int64_t s_end;
   uint64_t mf_off;      // M_f of step s at mf[mf_off + (s - s_begin) * m * m]
   uint64_t cp_off;      // chunk product [m*m][K] in the chunk-product pool
+  int32_t k0;           // primes [k0, k0 + kn) of the walk: walks of more than
+  int32_t kn;           // kSliceK primes run as several tasks per step range
 };
+
+// Widest prime stripe of one walk workgroup; an interpreted walk with more
+// primes splits each step range into slices of at most this many primes
+// (the primes are independent; the slices share the range's chunk product).
+constexpr int kSliceK = 256;
 
 // Per-walk span of tasks for the ordered combine / shadow passes.
 struct DevSpan {

@@ -23,7 +23,7 @@ from .runner import (
     run_pcf_walk,
     verify_pcf,
 )
-from .cmf_walk import run_cmf_walk, run_cmf_walk_vec, run_cmf_walk_vec_batch
+from .cmf_walk import compile_cmf_spec, run_cmf_walk, run_cmf_walk_vec, run_cmf_walk_vec_batch, shift_to_axis_offsets
 from .gpu_walk import gpu_available, run_cmf_walk_batch_gpu, run_cmf_walk_gpu, run_pcf_walk_batch_gpu, run_pcf_walk_gpu
 
 __version__ = "0.2.0"
@@ -47,6 +47,8 @@ __all__ = [
     "run_cmf_walk",
     "run_cmf_walk_vec",
     "run_cmf_walk_vec_batch",
+    "compile_cmf_spec",
+    "shift_to_axis_offsets",
     "gpu_available",
     "run_pcf_walk_gpu",
     "run_pcf_walk_batch_gpu",

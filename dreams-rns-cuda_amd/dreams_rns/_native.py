@@ -18,6 +18,7 @@ from ._build import LIB_PATH
 
 ABI_VERSION = 3
 PACK_META = 13  # values drx_program_pack reports
+MAX_K = 1024  # primes one context holds (walks of more than 256 primes run as prime slices)
 DRX_INIT_PCF, DRX_INIT_IDENTITY, DRX_INIT_GIVEN = 0, 1, 2
 DRX_WALK_MATRIX, DRX_WALK_VECTOR = 0, 1
 MODE_BATCHED, MODE_PLAIN, MODE_PERSISTENT = 0, 1, 2
@@ -187,7 +188,9 @@ class Context:
         with self._lock:
             if K <= len(self.primes):
                 return
-            want = max(K, 2 * len(self.primes), 64)
+            if K > MAX_K:
+                raise ValueError(f"K={K} above the {MAX_K} primes of one context")
+            want = min(max(K, 2 * len(self.primes), 64), MAX_K)
             pr = generate_rns_primes(want)
             if len(pr) < K:
                 raise ValueError(f"only {len(pr)} primes in (2^30, 2^31) are available")

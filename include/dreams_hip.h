@@ -106,7 +106,11 @@ enum { DRX_MODE_BATCHED = 0, DRX_MODE_PLAIN = 1, DRX_MODE_PERSISTENT = 2 };
  * cmf_walk.py:445-531) is v(depth) = M(depth-1) ... M(0) v(0) with p =
  * v[out_p], q = v[out_q]; its program is uploaded with mode DRX_WALK_VECTOR,
  * its initial state given (DRX_INIT_GIVEN: [m][K] residues and [m] FP64
- * entries). A batch holds walks of one mode. */
+ * entries). A batch holds walks of one mode.
+ * K: a matrix walk takes up to 1024 primes (above 256 it runs as prime
+ * slices of 256 on the interpreting kernel: the primes are independent, the
+ * results are those of one walk), a vector walk up to 256, a served walk up
+ * to 256. */
 enum { DRX_WALK_MATRIX = 0, DRX_WALK_VECTOR = 1 };
 typedef struct drx_walk {
   int32_t prog;

@@ -1,0 +1,132 @@
+"""GPU parity of the pFq CMF sweep (a100_sweep.py:336-528 over the
+reference's sweep_data subset) and of walks wider than one workgroup's 256
+primes: residues, full matrices and CRT integers bit-exact against the C
+oracle, the FP64 shadow within 1e-12 relative."""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+import dreams_rns as D
+from dreams_rns.cmf_walk import compile_cmf_spec
+from dreams_rns.engine import BatchPlan, cmf_spec
+from dreams_rns.sweep import CmfSweep, sweep_offsets
+from oracle import cref
+from oracle import walk as ow
+
+pytestmark = pytest.mark.gpu
+REL = 1e-12
+FIX = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sweep_3f2.json")
+
+
+def _close(a, b, rel=REL):
+    a, b = float(a), float(b)
+    return a == b or abs(a - b) <= rel * max(abs(a), abs(b))
+
+
+@pytest.fixture(scope="module")
+def sweep_fix():
+    with open(FIX) as f:
+        return json.load(f)
+
+
+def _axes(offsets, traj):
+    return [(s, 1, t, 1) for s, t in zip(offsets, traj)]
+
+
+def test_sweep_subset_against_c_oracle(gpu_ctx, sweep_fix):
+    specs = [sweep_fix["specs"][0], sweep_fix["specs"][37]]
+    trajs = [sweep_fix["trajectories"][i] for i in (0, 4, 9, 21)]
+    shifts = sweep_fix["shifts"]
+    depth, K = 120, 32
+    primes = D.generate_rns_primes(K)
+    sw = CmfSweep(specs, trajs, shifts, depth=depth, K=K, max_walks=100, crt=True)
+    n_rows = n_rec = 0
+    for batch in sw:
+        print("sweep batch", len(batch), flush=True)
+        for i in range(len(batch)):
+            si, ti = int(batch.spec_idx[i]), int(batch.traj_idx[i])
+            prog = compile_cmf_spec(specs[si], trajs[ti])
+            rr, Pf, ls = cref.walk(prog, depth, primes, _axes(batch.offsets[i], trajs[ti]))
+            d = batch.walk_dict(i)
+            m = prog.m
+            assert d["p_residues"].tolist() == rr[0, m - 1].tolist(), (si, ti, batch.offsets[i])
+            assert d["q_residues"].tolist() == rr[m - 1, m - 1].tolist(), (si, ti, batch.offsets[i])
+            assert _close(d["p_float"], Pf[0, m - 1]) and _close(d["q_float"], Pf[m - 1, m - 1])
+            assert _close(d["log_scale"], ls)
+            xp, M = ow.crt_reconstruct(rr[0, m - 1].tolist(), primes.tolist())
+            xq, _ = ow.crt_reconstruct(rr[m - 1, m - 1].tolist(), primes.tolist())
+            assert (d["p_crt"], d["q_crt"]) == (ow.centered(xp, M), ow.centered(xq, M))
+        recs = list(batch.records())
+        for r in recs:
+            assert r["shift_vals"] == sweep_offsets([shifts[r["shift_idx"]]], 3, 5)[0]
+        n_rows += len(batch)
+        n_rec += len(recs)
+    assert sw.skipped == []
+    assert n_rec == len(specs) * len(trajs) * len(shifts)
+    assert n_rows < n_rec  # shifts of equal offsets walked once
+
+
+@pytest.mark.parametrize("K", [257, 300, 1000])
+def test_prime_slices_match_c_oracle(gpu_ctx, sweep_fix, K):
+    # walks of more than 256 primes run as prime slices; several step ranges
+    # per walk (chunk_steps 32) make every slice combine in order
+    spec = sweep_fix["specs"][5]
+    cases = [([1, 2, 1, 3, 1], sweep_fix["trajectories"][7]), ([4, 1, 2, 1, 1], sweep_fix["trajectories"][20]),
+             ([2, 2, 2, 2, 2], [1, 1, 1, 1, 1])]
+    depth = 100
+    primes = D.generate_rns_primes(K)
+    specs, refs = [], []
+    for off, traj in cases:
+        prog = compile_cmf_spec(spec, traj)
+        specs.append(cmf_spec(prog, depth, K, off))
+        refs.append(cref.walk(prog, depth, primes, _axes(off, traj)))
+    print("slices K", K, flush=True)
+    with BatchPlan(specs, full=True, crt=True, chunk_steps=32) as plan:
+        res = plan.run()
+    for i, (rr, Pf, ls) in enumerate(refs):
+        assert np.array_equal(res.full_matrix(i, 4), rr), (K, i)
+        d = res.walk_dict(i)
+        assert d["primes"].tolist() == primes.tolist()
+        assert _close(d["p_float"], Pf[0, 3]) and _close(d["q_float"], Pf[3, 3]) and _close(d["log_scale"], ls)
+        xp, M = ow.crt_reconstruct(rr[0, 3].tolist(), primes.tolist())
+        xq, _ = ow.crt_reconstruct(rr[3, 3].tolist(), primes.tolist())
+        assert res.crt_values(i) == (ow.centered(xp, M), ow.centered(xq, M))
+
+
+def test_mixed_widths_in_one_batch(gpu_ctx, sweep_fix):
+    # walks of 64 and of 600 primes, 4x4 and 2x2, share one batch
+    spec = sweep_fix["specs"][11]
+    prog4 = compile_cmf_spec(spec, sweep_fix["trajectories"][3])
+    prog2 = compile_cmf_spec({"p": 1, "q": 1, "a_params": ["1/2"], "b_params": ["3/2"]}, [1, 2])
+    plan_in = [(prog4, 64, [1, 2, 3, 1, 2]), (prog4, 600, [2, 1, 1, 4, 1]), (prog2, 600, [3, 1]),
+               (prog2, 64, [1, 1]), (prog4, 600, [1, 1, 1, 1, 1])]
+    depth = 90
+    kmax = max(k for _, k, _ in plan_in)
+    primes = D.generate_rns_primes(kmax)
+    specs = [cmf_spec(p, depth, k, off) for p, k, off in plan_in]
+    with BatchPlan(specs, full=True) as plan:
+        res = plan.run()
+    for i, (p, k, off) in enumerate(plan_in):
+        rr, Pf, ls = cref.walk(p, depth, primes[:k], _axes(off, p.directions))
+        assert np.array_equal(res.full_matrix(i, p.m), rr), i
+        d = res.walk_dict(i)
+        m = p.m
+        assert _close(d["p_float"], Pf[0, m - 1]) and _close(d["q_float"], Pf[m - 1, m - 1]), i
+
+
+def test_batch_api_with_sweep_offsets(gpu_ctx, sweep_fix):
+    spec = sweep_fix["specs"][20]
+    traj = sweep_fix["trajectories"][12]
+    prog = compile_cmf_spec(spec, traj)
+    offs = sweep_offsets(sweep_fix["shifts"][:9], 3, 5)
+    got = D.run_cmf_walk_batch_gpu(prog, 80, 48, shift_val_list=offs, batch_size=4)
+    assert len(got) == len(offs)
+    for sv, g in zip(offs, got):
+        one = D.run_cmf_walk(prog, 80, 48, sv)
+        assert g["shift_vals"] == sv
+        assert g["p_residues"].tolist() == one["p_residues"].tolist()
+        assert g["q_residues"].tolist() == one["q_residues"].tolist()
+        assert g["p_float"].hex() == one["p_float"].hex() and g["q_float"].hex() == one["q_float"].hex()
