@@ -43,10 +43,32 @@ def program_work(prog, vector: bool = False) -> float:
     """Integer-VALU ops per residue-step by the fixed convention
     W = 6*MM + 2*MA with MM = #MUL + #POW2 + 2*#POW3 + 56*#INV + m^3 and
     MA = #ADD + #SUB + #NEG + m^2 (m - 1), counted on the walk's bytecode;
-    a state-vector walk's product is a matrix-vector product (m^2, m (m - 1))."""
+    a state-vector walk's product is a matrix-vector product (m^2, m (m - 1)).
+    Instructions whose operands are all constants (a rational constant's
+    LOAD_C p; LOAD_C q; INV; MUL) compute the same value at every step and
+    are not counted."""
     cnt = {}
+    const_regs = set()
     for ins in prog.opcodes:
-        cnt[int(ins.op)] = cnt.get(int(ins.op), 0) + 1
+        op, a0, a1, a2 = int(ins.op), ins.arg0, ins.arg1, ins.arg2
+        if op == 2:  # LOAD_C
+            const_regs.add(a1)
+            continue
+        if op in (4, 5, 6):  # ADD SUB MUL
+            if a0 in const_regs and a1 in const_regs:
+                const_regs.add(a2)
+                continue
+            const_regs.discard(a2)
+        elif op in (7, 8, 9, 10):  # NEG POW2 POW3 INV
+            if a0 in const_regs:
+                const_regs.add(a1)
+                continue
+            const_regs.discard(a1)
+        elif op == 1:  # LOAD_X
+            const_regs.discard(a1)
+        elif op == 3:  # LOAD_N
+            const_regs.discard(a0)
+        cnt[op] = cnt.get(op, 0) + 1
     m = prog.m
     mm = cnt.get(6, 0) + cnt.get(8, 0) + 2 * cnt.get(9, 0) + 56 * cnt.get(10, 0) + (m * m if vector else m ** 3)
     ma = cnt.get(4, 0) + cnt.get(5, 0) + cnt.get(7, 0) + (m * (m - 1) if vector else m * m * (m - 1))
@@ -67,6 +89,10 @@ def walk_kernel_name(progs) -> str:
             k = meta["kinds"]
             pcf = (k & 3) == 0 and ((k >> 4) & 3) == 1  # entry (0,0) zero, (1,0) one
             names["k_walk_fd_pcf" if pcf else "k_walk_fd<2>"] += 1
+        elif meta["fd_ok"] and p.m in (3, 4) and meta["fd_degree"] <= 3 and all(
+                ((meta["kinds"] >> (2 * (i * p.m + j))) & 3) == (1 if i == j + 1 else 0)
+                for i in range(p.m) for j in range(p.m - 1)):
+            names[f"k_walk_fd_comp<{p.m}>"] += 1  # companion: sub-diagonal ones, polynomial last column
         elif meta["fd_ok"] and p.m == 3 and meta["fd_degree"] <= 3:
             names["k_walk_fd<3>"] += 1
         else:
@@ -156,8 +182,11 @@ def served_run(specs_pool, rate, duration, round_steps, seed=1005):
 # ------------------------------------------------------- other workloads
 def other_workload(args):
     """cmf3: 1024 3x3 CMFs with INV, depth 10000, K = 64; mix4: 16384 uneven
-    walks (2x2 / 3x3 / 4x4, depth 500..50000, K 16..128), top-128. One JSON
-    line in the same form (no CPU or served legs)."""
+    walks (2x2 / 3x3 / 4x4, depth 500..50000, K 16..128), top-128; zeta5:
+    16384 state-vector walks; sweep3f2: the pFq CMF sweep of one 3F2 spec
+    over 64 trajectories x 512 shifts at depth 2000, K = 1000 (shifts of
+    equal integer offsets walked once), with GPU CRT of every walk. One JSON
+    line in the same form (no served leg; a CPU leg for zeta5 / sweep3f2)."""
     import dreams_rns as D
     from dreams_rns import synth
     from dreams_rns.engine import BatchPlan, cmf_spec, pcf_spec
@@ -188,6 +217,36 @@ def other_workload(args):
             cpu = {"value": nw * cfg["depth"] * cfg["K"] / el, "unit": "residue-steps/s", "cores": 1, "kind": "port",
                    "sample": f"{nw} walks of the sweep through the numpy restatement, one core"}
             log(f"cpu: {cpu['value']:.3e} residue-steps/s ({nw} walks)")
+    elif args.workload == "sweep3f2":
+        from dreams_rns.sweep import CmfSweep
+
+        with open(os.path.join(ROOT, "tests", "golden", "sweep_3f2.json")) as f:
+            fix = json.load(f)
+        spec = fix["specs"][0]
+        depth, K = 2000, 1000  # the sweep driver's defaults for CMF mode
+        sw = CmfSweep([spec], fix["bench_trajectories"], fix["bench_shifts"], depth=depth, K=K)
+        rows = list(sw._rows())
+        specs = [cmf_spec(prog, depth, K, so) for (_, _, prog, so, _) in rows]
+        sweep_walks = sw.total_walks
+        top_k = 0
+        desc = (f"sweep3f2: {spec['name']} x {len(fix['bench_trajectories'])} trajectories x "
+                f"{len(fix['bench_shifts'])} shifts ({sweep_walks} sweep walks, {len(specs)} distinct starts), "
+                f"4x4 per-axis companion, depth {depth}, K={K}")
+        if not args.no_cpu:
+            from oracle import walk as ow
+
+            t = time.perf_counter()
+            nw = 0
+            for (_, _, prog, so, _) in rows:
+                ow.run_cmf_walk(prog, 50, K, so)
+                nw += 1
+                if time.perf_counter() - t > 10.0:
+                    break
+            el = time.perf_counter() - t
+            cpu = {"value": nw * 50 * K / el, "unit": "residue-steps/s", "cores": 1, "kind": "port",
+                   "sample": f"{nw} walks of the sweep at depth 50 (the per-step cost does not depend on "
+                             f"the depth) through the numpy restatement, one core"}
+            log(f"cpu: {cpu['value']:.3e} residue-steps/s ({nw} walks)")
     elif args.workload == "cmf3":
         n = args.walks if args.walks != 4096 else 1024
         cfg = synth.cmf3(cf, n_walks=n)
@@ -210,7 +269,7 @@ def other_workload(args):
     rsteps = float(sum(s.depth * s.K for s in specs))
     log(f"generated {len(specs)} walks in {time.perf_counter() - t0:.1f}s")
     tuned = None
-    if args.workload == "zeta5":
+    if args.workload in ("zeta5", "sweep3f2"):
         args.threads = args.threads or 256
         args.chunk = args.chunk or 4096
     if not (args.threads or args.chunk):
@@ -222,6 +281,7 @@ def other_workload(args):
         log(f"autotuned in {time.perf_counter() - t:.1f}s: threads {args.threads}, chunk {args.chunk}")
     plan = BatchPlan(specs, crt=True, top_k=top_k, threads=args.threads, chunk_steps=args.chunk,
                      use_graph=args.graph)
+    log(f"plan built: {time.perf_counter() - t0:.1f}s since start")
     ctx = plan.ctx
     for _ in range(args.warmup):
         plan.run()
@@ -233,10 +293,16 @@ def other_workload(args):
         tm = ctx.last_timing()
         walk_ms.append(tm[0])
         log(f"step {i}: {step_ms[-1]:.2f} ms (walk kernels {tm[0]:.3f} ms, device {tm[4]:.3f})")
+    st = plan.stats()
     plan.close()
     ms = float(np.mean(step_ms))
     walk_avg = float(np.mean(walk_ms)) if not args.graph else float("nan")
     achieved = work / (walk_avg * 1e-3) if walk_avg == walk_avg else None
+    w_exec = 6.0 * float(st["modmul"]) + 2.0 * float(st["modadd"])
+    executed = ({"achieved": w_exec / (walk_avg * 1e-3) / 1e12, "frac": w_exec / (walk_avg * 1e-3) / PEAK_LANE_OPS,
+                 "work_per_launch": w_exec,
+                 "model": "6*modmul + 2*modadd of the modular operations the chosen kernels execute (library count)"}
+                if walk_avg == walk_avg else None)
     print(json.dumps({
         "metric": METRIC, "value": rsteps / (ms * 1e-3), "unit": "residue-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
@@ -248,8 +314,13 @@ def other_workload(args):
                      "peak": PEAK_LANE_OPS / 1e12, "unit": "TOP/s",
                      "frac": achieved / PEAK_LANE_OPS if achieved else None, "traffic": None,
                      "kernel": "k_walk_vec" if args.workload == "zeta5" else walk_kernel_name([s.program for s in specs]),
-                     "kernel_ms": walk_avg, "work_per_launch": work},
+                     "kernel_ms": walk_avg, "work_per_launch": work,
+                     "work_model": "W = 6*MM + 2*MA lane-ops per residue-step on each walk's bytecode, constant-only "
+                                   "instructions excluded (per-step program evaluation; forward differences skip it, "
+                                   "so frac can pass 1)",
+                     "executed": executed},
         "cpu_baseline": cpu,
+        **({"sweep_walks_per_s": sweep_walks / (ms * 1e-3)} if args.workload == "sweep3f2" else {}),
     }))
 
 
@@ -270,9 +341,9 @@ def main():
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--graph", action="store_true", help="replay the batch as a hipGraph")
-    ap.add_argument("--workload", choices=("pcf2", "cmf3", "mix4", "zeta5"), default="pcf2",
-                    help="pcf2 (the headline line); cmf3 / mix4 measure the other batch shapes "
-                         "(no CPU or served legs)")
+    ap.add_argument("--workload", choices=("pcf2", "cmf3", "mix4", "zeta5", "sweep3f2"), default="pcf2",
+                    help="pcf2 (the headline line); cmf3 / mix4 / zeta5 / sweep3f2 measure the other batch shapes "
+                         "(no served leg; a CPU leg for zeta5 / sweep3f2)")
     ap.add_argument("--profile", action="store_true",
                     help="for counter collection: run the batch --steps times and a 512 MiB device copy "
                          "(the calibration kernel for FETCH_SIZE / WRITE_SIZE), print nothing else")
@@ -308,6 +379,7 @@ def main():
         log(f"autotuned in {time.perf_counter() - t:.1f}s: threads {args.threads}, chunk {args.chunk}")
     plan = BatchPlan(specs, crt=True, top_k=top_k, threads=args.threads, chunk_steps=args.chunk,
                      use_graph=args.graph)
+    log(f"plan built: {time.perf_counter() - t0:.1f}s since start")
     ctx = plan.ctx
     if args.profile:
         for _ in range(args.steps):

@@ -92,7 +92,14 @@ int upload_programs(drx_ctx* ctx) {
 // p_j^{-1} mod p_i for the first k primes (lazily extended).
 int walk_kind(const HostProgram& hp, bool interp_only) {
   if (interp_only || !hp.fd) return WALK_INTERP;
-  return pcf_shape(hp) >= 0 ? WALK_FD_PCF : WALK_FD;
+  if (pcf_shape(hp) >= 0) return WALK_FD_PCF;
+  return comp_shape(hp.dev) >= 0 ? WALK_FD_COMP : WALK_FD;
+}
+
+int walk_shape(const HostProgram& hp, int kind) {
+  if (kind == WALK_FD_PCF) return pcf_shape(hp);
+  if (kind == WALK_FD_COMP) return comp_shape(hp.dev);
+  return 0;
 }
 
 size_t device_lds_limit(drx_ctx* ctx) {
@@ -149,6 +156,25 @@ int pcf_shape(const HostProgram& hp) {
   return 8 * db + da;
 }
 
+int comp_shape(const DevProgram& d) {
+  // companion CMF: ones on the sub-diagonal, zeros elsewhere left of the last
+  // column; the last column polynomial (a static 0 or 1 is degree 0)
+  const int m = (int)d.m;
+  if (!d.fd_ok || m < 3 || m > 4) return -1;
+  int deg = 0;
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < m; ++j) {
+      const int e = i * m + j;
+      const uint32_t kd = (d.kinds >> (2 * e)) & 3u;
+      if (j < m - 1) {
+        if (kd != (i == j + 1 ? (uint32_t)E_ONE : (uint32_t)E_ZERO)) return -1;
+      } else if (kd == E_GEN) {
+        deg = std::max(deg, (int)((d.fd_deg >> (4 * e)) & 15u));
+      }
+    }
+  return deg <= kFdMax3 ? deg : -1;
+}
+
 // Constant slots of the residue code beyond those held in the VGPR bank C.
 static size_t lds_const_rows(const DevProgram& pg) {
   const uint32_t nc = (uint32_t)kResStep - pg.dim;
@@ -198,7 +224,7 @@ void order_ranges(std::vector<Stage>& stages, std::vector<DevRange>& ranges, int
   }
 }
 
-int launch_stages(drx_ctx* ctx, const KArgs& a, const std::vector<Stage>& stages, const DevSpan* seed_spans,
+int launch_stages(drx_ctx* ctx, const KArgs& a, const std::vector<Stage>& stages, const DevTask* seed_tasks,
                   int finalize, bool timed, int64_t win, int nwin) {
   hipStream_t s1 = ctx->stream, s2 = ctx->stream2, s3 = ctx->stream3;
   hipEvent_t* ev = ctx->ev;
@@ -211,7 +237,7 @@ int launch_stages(drx_ctx* ctx, const KArgs& a, const std::vector<Stage>& stages
   for (const Stage& g : stages) {
     if (g.kind == WALK_INTERP || g.seed_end <= g.seed_begin) continue;
     KArgs fa = a;
-    fa.spans = seed_spans + g.seed_begin;
+    fa.tasks = seed_tasks + g.seed_begin;
     HIPCHK(ctx, launch_fd_seed(g.m, fa, g.seed_end - g.seed_begin, g.seed_kp, fd_seed_threads(g.m, g.seed_kp),
                                g.seed_lds, s1));
   }
@@ -368,7 +394,8 @@ struct drx_plan {
   int64_t win = 1;  // FP64 chain window (steps) and count
   int nwin = 1;
   std::vector<int> walk_order;  // span index -> walk
-  DBuf d_walks, d_tasks, d_spans, d_ranges, d_res0, d_fst0, d_res, d_fst, d_axr, d_axi, d_axd, d_cp, d_mf, d_fdt;
+  DBuf d_walks, d_tasks, d_seeds, d_spans, d_ranges, d_res0, d_fst0, d_res, d_fst, d_axr, d_axi, d_axd, d_cp, d_mf,
+      d_fdt;
   DBuf d_fout, d_rout, d_pout, d_pfull, d_limbs, d_neg, d_topk, d_topk_ws, d_rowk;
   KArgs args{};
   GarnerArgs gargs{};
@@ -385,7 +412,7 @@ struct drx_plan {
   ~drx_plan() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
-    for (DBuf* d : {&d_walks, &d_tasks, &d_spans, &d_ranges, &d_res0, &d_fst0, &d_res, &d_fst, &d_axr, &d_axi,
+    for (DBuf* d : {&d_walks, &d_tasks, &d_seeds, &d_spans, &d_ranges, &d_res0, &d_fst0, &d_res, &d_fst, &d_axr, &d_axi,
                     &d_axd, &d_cp, &d_mf, &d_fdt, &d_fout, &d_rout, &d_pout, &d_pfull, &d_limbs, &d_neg, &d_topk,
                     &d_topk_ws, &d_rowk, &d_vgroups, &d_fgroups, &d_vorder})
       d->release();
@@ -432,7 +459,7 @@ int launch_sequence(drx_plan* pl, bool timed) {
   if (pl->vec) return launch_vec_sequence(pl, timed);
   drx_ctx* ctx = pl->ctx;
   hipStream_t s = ctx->stream;
-  int rc = launch_stages(ctx, pl->args, pl->stages, pl->args.spans, 1, timed, pl->win, pl->nwin);
+  int rc = launch_stages(ctx, pl->args, pl->stages, pl->d_seeds.as<DevTask>(), 1, timed, pl->win, pl->nwin);
   if (rc) return rc;
   // residue outputs and CRT need only the combined residues: they run while
   // the shadow chain is still going; the final P_f copy needs the shadow
@@ -722,8 +749,8 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
     return (double)w.depth * (double)w.K * (1.0 + ctx->progs[w.prog].pk.code.size());
   };
   const bool interp_only = pl->cfg.interp_only != 0;
-  // walks of more than kSliceK primes run as prime slices of the interpreter
-  auto kind_of = [&](int i) { return walk_kind(ctx->progs[b.walks[i].prog], interp_only || b.walks[i].K > kSliceK); };
+  // walks of more than kSliceK primes run as prime slices
+  auto kind_of = [&](int i) { return walk_kind(ctx->progs[b.walks[i].prog], interp_only); };
   auto slice_kp = [](int K) { return (int)pow2_ceil((uint32_t)std::min(K, kSliceK)); };
   // interpreted walks of one order: LDS classes whose LDS size would lower
   // the occupancy of the smaller classes get stages of their own
@@ -773,7 +800,7 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
     const HostProgram& hp = ctx->progs[b.walks[i].prog];
     return ((int)hp.dev.m * 4 + kind_of(i)) * 4 + group[i];
   };
-  auto shape_of = [&](int i) { return kind_of(i) == WALK_FD_PCF ? pcf_shape(ctx->progs[b.walks[i].prog]) : 0; };
+  auto shape_of = [&](int i) { return walk_shape(ctx->progs[b.walks[i].prog], kind_of(i)); };
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
     const int kx = gkey(x), ky = gkey(y);
     if (kx != ky) return kx < ky;
@@ -783,7 +810,7 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
   });
 
   std::vector<DevWalk> hw(n);
-  std::vector<DevTask> ht;
+  std::vector<DevTask> ht, hseed;
   std::vector<DevSpan> hs;
   std::vector<DevRange> hr;
   std::vector<double> res0, fst0, axr, axd;
@@ -839,14 +866,14 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
       g.lclass = lclass;
       g.task_begin = g.task_end = (int)ht.size();
       g.span_begin = g.span_end = (int)hs.size();
-      g.seed_begin = g.seed_end = (int)hs.size();
+      g.seed_begin = g.seed_end = (int)hseed.size();
       g.range_begin = g.range_end = (int)hr.size();
       g.threads = plain ? slice_kp(K) : std::max<int>(thr_cfg, slice_kp(K));
       pl->stages.push_back(g);
     }
     Stage& g = pl->stages.back();
     g.threads = std::max<int>(g.threads, slice_kp(K));
-    g.seed_kp = std::max<int>(g.seed_kp, pow2_ceil(K));
+    g.seed_kp = std::max<int>(g.seed_kp, slice_kp(K));
     DevSpan sp{};
     sp.walk = i;
     sp.out_row = i;
@@ -874,12 +901,14 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
         t.k0 = k0;
         t.kn = std::min(kSliceK, K - k0);
         ht.push_back(t);
+        if (kind != WALK_INTERP && s0 == 0) hseed.push_back(t);  // Newton tables: one seed per slice
       }
     }
     sp.task_end = (int)ht.size();
     hs.push_back(sp);
     g.task_end = (int)ht.size();
-    g.span_end = g.seed_end = (int)hs.size();
+    g.span_end = (int)hs.size();
+    g.seed_end = (int)hseed.size();
     g.range_end = (int)hr.size();
     pl->walk_order.push_back(i);
     rsteps += (double)w.depth * K;
@@ -926,7 +955,8 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
   pl->ma_total = mat;
 
   int rc;
-  if ((rc = upload(ctx, pl->d_walks, hw)) || (rc = upload(ctx, pl->d_tasks, ht)) || (rc = upload(ctx, pl->d_spans, hs)) ||
+  if ((rc = upload(ctx, pl->d_walks, hw)) || (rc = upload(ctx, pl->d_tasks, ht)) || (rc = upload(ctx, pl->d_seeds, hseed)) ||
+      (rc = upload(ctx, pl->d_spans, hs)) ||
       (rc = upload(ctx, pl->d_ranges, hr)) ||
       (rc = upload(ctx, pl->d_res0, res0)) || (rc = upload(ctx, pl->d_fst0, fst0)) || (rc = upload(ctx, pl->d_axr, axr)) ||
       (rc = upload(ctx, pl->d_axi, axi)) || (rc = upload(ctx, pl->d_axd, axd)))
@@ -1268,7 +1298,8 @@ int drx_program_upload_mode(drx_ctx* ctx, const uint32_t* ops, int n_ops, int m,
   d.fd_degree = (uint32_t)pk.fd_degree;
   d.fd_deg = pk.fd_deg;
   hp.fd = mode == DRX_WALK_MATRIX && pk.fd_ok &&
-          ((m == 2 && pk.fd_degree <= kFdMax2) || (m == 3 && pk.fd_degree <= kFdMax3));
+          ((m == 2 && pk.fd_degree <= kFdMax2) || (m == 3 && pk.fd_degree <= kFdMax3) ||
+           (m == 4 && comp_shape(d) >= 0));
   ctx->h_code.insert(ctx->h_code.end(), pk.code.begin(), pk.code.end());
   if (ctx->h_code.size() % 2) ctx->h_code.push_back(0);
   ctx->h_code.insert(ctx->h_code.end(), pk.fcode.begin(), pk.fcode.end());

@@ -119,7 +119,7 @@ struct Stage {
   int lclass = 0;                  // lds_class of its walks
   int task_begin = 0, task_end = 0;
   int span_begin = 0, span_end = 0;
-  int seed_begin = 0, seed_end = 0;    // spans to seed (forward-difference stages)
+  int seed_begin = 0, seed_end = 0;    // seed tasks (forward-difference stages)
   int range_begin = 0, range_end = 0;  // M_f blocks, ordered by window
   std::vector<int> win_range;          // [nwin + 1]: ranges of window c are win_range[c] .. win_range[c + 1]
   int threads = 0, seed_kp = 0;    // walk workgroup; seed primes per point group (pow2 >= every K)
@@ -128,13 +128,13 @@ struct Stage {
 
 // Launch the walk pipeline of `stages` with the argument block `a`: on
 // stream2 the FP64 M_f pass and the shadow (with delta_float when
-// `finalize`), on stream the Newton-table seeds (spans from `seed_spans`),
+// `finalize`), on stream the Newton-table seeds (tasks from `seed_tasks`),
 // the walk kernels and the ordered combine. join_stages makes stream wait for
 // stream2 (work on stream before it overlaps the shadow chain). With
 // `timed`, events 0..3 mark walk start, walk end, combine end, join.
 // The FP64 chain runs in `nwin` windows of `win` steps: the M_f pass of
 // window c on stream2, the shadow of window c on stream3 behind it.
-int launch_stages(drx_ctx* ctx, const KArgs& a, const std::vector<Stage>& stages, const DevSpan* seed_spans,
+int launch_stages(drx_ctx* ctx, const KArgs& a, const std::vector<Stage>& stages, const DevTask* seed_tasks,
                   int finalize, bool timed, int64_t win, int nwin);
 int join_stages(drx_ctx* ctx, bool timed);
 // Window length (a multiple of kMfRange) and count for walks of up to
@@ -149,6 +149,11 @@ size_t seed_lds_bytes(const DevProgram& pg, int kp, int seed_threads);
 size_t mf_lds_bytes(const DevProgram& pg);
 // Walk-kernel family of program `hp` (forward differences unless interp_only).
 int walk_kind(const HostProgram& hp, bool interp_only);
+// Task shape of a walk of this kind (the forward-difference kernels' case).
+int walk_shape(const HostProgram& hp, int kind);
+// Last-column degree of a companion-shaped polynomial program of order 3 or
+// 4 (the companion forward-difference kernel), else -1.
+int comp_shape(const DevProgram& d);
 // Interpreted walks are grouped by their spill rows (0, 1-8, 9-16, more),
 // so that a program that spills much does not set the LDS size (and the
 // occupancy) of every walk of its order.

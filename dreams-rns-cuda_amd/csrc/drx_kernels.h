@@ -49,7 +49,7 @@ struct KArgs {
 constexpr int kCodeFetch = 8;
 
 // Walk kernel families.
-enum WalkKind { WALK_INTERP = 0, WALK_FD = 1, WALK_FD_PCF = 2 };
+enum WalkKind { WALK_INTERP = 0, WALK_FD = 1, WALK_FD_PCF = 2, WALK_FD_COMP = 3 };
 // M_f pass: one lane per step, kMfBlock lanes per block; a range (one block)
 // covers kMfRange consecutive steps of one task.
 constexpr int kMfBlock = 256;
@@ -79,9 +79,10 @@ hipError_t launch_vec_fp(const KArgs& a, int ngroup, int threads, size_t lds, hi
 // Resident workgroups per CU of the interpreting walk kernel for order m at
 // this workgroup size and LDS; -1 when unknown.
 int walk_blocks_per_cu(int m, int kind, int threads, size_t lds);
-// Newton tables of forward-difference walks: one block per span of `threads`
-// lanes, `kp` (power of two >= every K) per point group; threads % kp == 0.
-hipError_t launch_fd_seed(int m, const KArgs& a, int nspan, int kp, int threads, size_t lds, hipStream_t s);
+// Newton tables of forward-difference walks: one block per seed task (a
+// prime slice of a walk, a.tasks) of `threads` lanes, `kp` (power of two >=
+// every slice's prime count) per point group; threads % kp == 0.
+hipError_t launch_fd_seed(int m, const KArgs& a, int nseed, int kp, int threads, size_t lds, hipStream_t s);
 // Seed block for primes padded to kp: point groups of kp lanes, at most 512 lanes.
 inline int fd_seed_threads(int m, int kp) {
   const int pts = fd_dmax(m) + 1;

@@ -357,36 +357,36 @@ __global__ void __launch_bounds__(256) k_walk_chunk(const KArgs a) {
 // with Delta^j f(sb) = sum_t C(sb, t) N_{j+t} and runs its steps with no
 // interpreter at all.
 
-// One workgroup per walk (span): lane (g, k) = point group g, prime k, with
-// KP = `kp` lanes per group. Phase 1 interprets the program at steps
-// g, g + J, ... (J = blockDim / KP) and stores the raw entry values in the
-// walk's table; phase 2 turns each prime's D + 1 values into Newton forward
-// differences in place.
+// One workgroup per prime slice of a walk (a seed task: walk, k0, kn):
+// lane (g, k) = point group g, prime k0 + k, with KP = `kp` lanes per group.
+// Phase 1 interprets the program at steps g, g + J, ... (J = blockDim / KP)
+// and stores the raw entry values in the walk's table; phase 2 turns each
+// prime's D + 1 values into Newton forward differences in place.
 template <int M>
 __global__ void __launch_bounds__(512) k_fd_seed(const KArgs a, uint32_t kp) {
   constexpr int D = fd_dmax(M);
   extern __shared__ double lds[];
-  const DevSpan sp = a.spans[blockIdx.x];
-  const DevWalk w = a.walks[sp.walk];
+  const DevTask st = a.tasks[blockIdx.x];
+  const DevWalk w = a.walks[st.walk];
   const DevProgram pg = a.progs[w.prog];
-  const uint32_t K = (uint32_t)w.K, KP = kp, J = blockDim.x / kp;
+  const uint32_t Kw = (uint32_t)w.K, k0 = (uint32_t)st.k0, K = (uint32_t)st.kn, KP = kp, J = blockDim.x / kp;
   const uint32_t k = threadIdx.x % KP, g = threadIdx.x / KP;
   const bool active = k < K;
-  const uint32_t kk = active ? k : 0u;
+  const uint32_t kk = active ? k : 0u, kg = k0 + kk;
   Lane ln;
-  ln.p = a.prime_p[kk];
-  ln.pinv = a.prime_pinv[kk];
-  ln.pe = a.prime_e[kk];
+  ln.p = a.prime_p[kg];
+  ln.pinv = a.prime_pinv[kg];
+  ln.pe = a.prime_e[kg];
   ln.k = k;
   ln.tid = threadIdx.x;
   ln.T = blockDim.x;
-  const ProgLds L = prog_lds(a, pg, K, KP, 0u, lds);
+  const ProgLds L = prog_lds(a, pg, K, KP, k0, lds);
   __syncthreads();
   const uint32_t* code = a.code + pg.code_off;
-  double* tab = a.fdt + w.fdt_off;
+  double* tab = a.fdt + w.fdt_off + k0;  // column k of the slice is prime k0 + k
   for (uint32_t j = g; j <= (uint32_t)D; j += J) {
     v16d R = 0.0, C = 0.0;
-    load_rns_inputs(a, w, pg, K, kk, ln, (int64_t)j, C, nullptr, KP, k);
+    load_rns_inputs(a, w, pg, Kw, kg, ln, (int64_t)j, C, nullptr, KP, k);
     run_program(code, pg.ncode, R, C, L.lc, KP, L.lm, L.le, nullptr, ln);
     double E[M * M];
 #pragma unroll
@@ -394,7 +394,7 @@ __global__ void __launch_bounds__(512) k_fd_seed(const KArgs a, uint32_t kp) {
       E[e] = (((pg.kinds >> (2 * e)) & 3u) == E_GEN) ? L.le[e * ln.T + ln.tid] : 0.0;
     if (active) {
 #pragma unroll
-      for (int e = 0; e < M * M; ++e) tab[((uint64_t)e * (D + 1) + j) * K + k] = E[e];
+      for (int e = 0; e < M * M; ++e) tab[((uint64_t)e * (D + 1) + j) * Kw + k] = E[e];
     }
   }
   __syncthreads();
@@ -405,13 +405,13 @@ __global__ void __launch_bounds__(512) k_fd_seed(const KArgs a, uint32_t kp) {
     for (int e = 0; e < M * M; ++e) {
       double N[D + 1];
 #pragma unroll
-      for (int j = 0; j <= D; ++j) N[j] = tab[((uint64_t)e * (D + 1) + j) * K + k];
+      for (int j = 0; j <= D; ++j) N[j] = tab[((uint64_t)e * (D + 1) + j) * Kw + k];
 #pragma unroll
       for (int i = 1; i <= D; ++i)
 #pragma unroll
         for (int j = D; j >= i; --j) N[j] = N[j] - N[j - 1];
 #pragma unroll
-      for (int j = 0; j <= D; ++j) tab[((uint64_t)e * (D + 1) + j) * K + k] = red(N[j], ln.p, ln.pinv);
+      for (int j = 0; j <= D; ++j) tab[((uint64_t)e * (D + 1) + j) * Kw + k] = red(N[j], ln.p, ln.pinv);
     }
   }
 }
@@ -561,6 +561,77 @@ __global__ void __launch_bounds__(256) k_walk_fd(const KArgs a) {
     }
   }
   tree_combine<M>(a, S, P);
+}
+
+// Companion-shaped programs (ones on the sub-diagonal, zeros elsewhere but
+// the last column c(s), of degree <= D in the step: the pFq companions of
+// cmf_walk.py:32-99): one step maps each row x of P to
+// (x_1, ..., x_{M-1}, sum_k x_k c_k(s)), M^2 modular products, and advances
+// the last column's Newton tables. A static 0 or 1 in the last column is a
+// table of degree 0.
+template <int M, int D>
+__device__ __forceinline__ void comp_step(double (&P)[M][M], double (&T)[M][D + 1], double p, double pinv) {
+#pragma unroll
+  for (int i = 0; i < M; ++i) {
+    double acc = mm(P[i][0], T[0][0], p, pinv);
+#pragma unroll
+    for (int k = 1; k < M; ++k) acc += mm(P[i][k], T[k][0], p, pinv);
+#pragma unroll
+    for (int k = 0; k + 1 < M; ++k) P[i][k] = P[i][k + 1];
+    P[i][M - 1] = acc;  // below M p / 2 in magnitude
+  }
+#pragma unroll
+  for (int i = 0; i < M; ++i) fd_advance<D>(T[i]);
+}
+
+template <int M, int D>
+__device__ __forceinline__ void comp_fd_body(const KArgs& a, const WalkSetup& S) {
+  constexpr int U = fd_unroll(D);
+  constexpr int DT = fd_dmax(M);  // table rows stored per entry
+  const double p = S.ln.p, pinv = S.ln.pinv;
+  double T[M][D + 1];
+#pragma unroll
+  for (int i = 0; i < M; ++i) {
+    const int e = i * M + M - 1;
+    const uint32_t kd = (S.pg.kinds >> (2 * e)) & 3u;  // uniform
+    const double* t = a.fdt + S.w.fdt_off + (uint64_t)e * (DT + 1) * S.Kw + S.kg;
+#pragma unroll
+    for (int j = 0; j <= D; ++j) T[i][j] = (kd == E_GEN) ? t[(uint64_t)j * S.Kw] : (j == 0 && kd == E_ONE ? 1.0 : 0.0);
+    fd_jump<D>(T[i], S.sb, a, S.kg, p, pinv);
+  }
+  double P[M][M];
+#pragma unroll
+  for (int i = 0; i < M; ++i)
+#pragma unroll
+    for (int j = 0; j < M; ++j) P[i][j] = (i == j) ? 1.0 : 0.0;
+  int64_t n = S.se - S.sb;
+  for (; n >= U; n -= U) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) comp_step<M, D>(P, T, p, pinv);
+#pragma unroll
+    for (int i = 0; i < M; ++i) fd_reduce<D>(T[i], p, pinv);
+  }
+  for (; n > 0; --n) comp_step<M, D>(P, T, p, pinv);
+  double Q[M * M];
+#pragma unroll
+  for (int i = 0; i < M; ++i)
+#pragma unroll
+    for (int j = 0; j < M; ++j) Q[i * M + j] = P[i][j];
+  tree_combine<M>(a, S, Q);
+}
+
+// The task's shape is the last column's degree (uniform over the workgroup).
+template <int M>
+__global__ void __launch_bounds__(256) k_walk_fd_comp(const KArgs a) {
+  extern __shared__ double lds[];
+  const WalkSetup S = walk_setup(a, lds, false);
+  switch (S.t.shape) {
+    case 0: comp_fd_body<M, 0>(a, S); break;
+    case 1: comp_fd_body<M, 1>(a, S); break;
+    case 2: comp_fd_body<M, 2>(a, S); break;
+    case 3: comp_fd_body<M, 3>(a, S); break;
+    default: break;
+  }
 }
 
 // --------------------------------------------------------- ordered combine
@@ -1313,6 +1384,10 @@ hipError_t launch_walk(int m, int kind, const KArgs& a, int ntask, int threads, 
     if (m == 2) hipLaunchKernelGGL(k_walk_fd<2>, grid, block, lds, s, a);
     else if (m == 3) hipLaunchKernelGGL(k_walk_fd<3>, grid, block, lds, s, a);
     else return hipErrorInvalidValue;
+  } else if (kind == WALK_FD_COMP) {
+    if (m == 3) hipLaunchKernelGGL(k_walk_fd_comp<3>, grid, block, lds, s, a);
+    else if (m == 4) hipLaunchKernelGGL(k_walk_fd_comp<4>, grid, block, lds, s, a);
+    else return hipErrorInvalidValue;
   } else {
     if (m == 2) hipLaunchKernelGGL(k_walk_chunk<2>, grid, block, lds, s, a);
     else if (m == 3) hipLaunchKernelGGL(k_walk_chunk<3>, grid, block, lds, s, a);
@@ -1347,11 +1422,12 @@ int walk_blocks_per_cu(int m, int kind, int threads, size_t lds) {
   return e == hipSuccess ? blocks : -1;
 }
 
-hipError_t launch_fd_seed(int m, const KArgs& a, int nspan, int kp, int threads, size_t lds, hipStream_t s) {
-  if (nspan <= 0) return hipSuccess;
-  if (kp <= 0 || threads % kp != 0 || threads > 512) return hipErrorInvalidValue;
-  if (m == 2) hipLaunchKernelGGL(k_fd_seed<2>, dim3(nspan), dim3(threads), lds, s, a, (uint32_t)kp);
-  else if (m == 3) hipLaunchKernelGGL(k_fd_seed<3>, dim3(nspan), dim3(threads), lds, s, a, (uint32_t)kp);
+hipError_t launch_fd_seed(int m, const KArgs& a, int nseed, int kp, int threads, size_t lds, hipStream_t s) {
+  if (nseed <= 0) return hipSuccess;
+  if (kp <= 0 || kp > kSliceK || threads % kp != 0 || threads > 512) return hipErrorInvalidValue;
+  if (m == 2) hipLaunchKernelGGL(k_fd_seed<2>, dim3(nseed), dim3(threads), lds, s, a, (uint32_t)kp);
+  else if (m == 3) hipLaunchKernelGGL(k_fd_seed<3>, dim3(nseed), dim3(threads), lds, s, a, (uint32_t)kp);
+  else if (m == 4) hipLaunchKernelGGL(k_fd_seed<4>, dim3(nseed), dim3(threads), lds, s, a, (uint32_t)kp);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }

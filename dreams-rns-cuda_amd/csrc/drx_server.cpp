@@ -50,7 +50,7 @@ struct SrvDone {
 struct drx_server {
   drx_ctx* ctx = nullptr;
   drx_server_cfg cfg{};
-  int kslot = 0;        // residues per entry per slot (= primes set)
+  int kslot = 0;        // residues per entry per slot (the primes set, at most 256)
   int nslots = 0;
   std::thread th;
   std::mutex mu;
@@ -76,9 +76,10 @@ namespace {
 
 size_t slot_res(const drx_server* s) { return (size_t)16 * s->kslot; }
 size_t slot_axr(const drx_server* s) { return (size_t)kMaxDim * 3 * s->kslot; }
-// Newton tables: the larger of 2x2 (4 entries x 7) and 3x3 (9 entries x 4) per prime
+// Newton tables: the largest of 2x2 (4 entries x 7), 3x3 (9 entries x 4) and
+// 4x4 companions (16 entries x 4) per prime
 size_t slot_fdt(const drx_server* s) {
-  const size_t a = 4 * (kFdMax2 + 1), b = 9 * (kFdMax3 + 1);
+  const size_t a = 4 * (kFdMax2 + 1), b = 16 * (kFdMax3 + 1);
   return (a > b ? a : b) * s->kslot;
 }
 
@@ -144,7 +145,7 @@ int round_once(drx_server* s, std::vector<int>& finished) {
     if (!s->slots[i].used) continue;
     const HostProgram& hp = ctx->progs[s->slots[i].w.prog];
     const int kind = walk_kind(hp, false);
-    refs.push_back({i, s->slots[i].m, kind, lds_class(hp, kind), kind == WALK_FD_PCF ? pcf_shape(hp) : 0});
+    refs.push_back({i, s->slots[i].m, kind, lds_class(hp, kind), walk_shape(hp, kind)});
   }
   if (refs.empty()) return DRX_OK;
   std::stable_sort(refs.begin(), refs.end(), [](const TaskRef& a, const TaskRef& b) {
@@ -153,8 +154,8 @@ int round_once(drx_server* s, std::vector<int>& finished) {
     if (a.lclass != b.lclass) return a.lclass < b.lclass;
     return a.shape < b.shape;
   });
-  std::vector<DevTask> ht;
-  std::vector<DevSpan> hs, hf, hseed;
+  std::vector<DevTask> ht, hseed;
+  std::vector<DevSpan> hs, hf;
   std::vector<DevRange> hr;
   std::vector<Stage> stages;
   uint64_t mf_total = 0;
@@ -204,7 +205,7 @@ int round_once(drx_server* s, std::vector<int>& finished) {
     g.range_end = (int)hr.size();
     hs.push_back(sp);
     if (r.kind != WALK_INTERP && sl.step == 0) {  // Newton tables once, on the walk's first round
-      hseed.push_back(sp);
+      hseed.push_back(t);
       g.seed_end = (int)hseed.size();
     }
     if (t.s_end >= sl.w.depth) {
@@ -242,7 +243,7 @@ int round_once(drx_server* s, std::vector<int>& finished) {
   DRX_HIPCHK(ctx, s->d_tasks.ensure(ht.size() * sizeof(DevTask)));
   DRX_HIPCHK(ctx, s->d_spans.ensure(hs.size() * sizeof(DevSpan)));
   DRX_HIPCHK(ctx, s->d_fspans.ensure(std::max<size_t>(1, hf.size()) * sizeof(DevSpan)));
-  DRX_HIPCHK(ctx, s->d_sspans.ensure(std::max<size_t>(1, hseed.size()) * sizeof(DevSpan)));
+  DRX_HIPCHK(ctx, s->d_sspans.ensure(std::max<size_t>(1, hseed.size()) * sizeof(DevTask)));
   DRX_HIPCHK(ctx, s->d_ranges.ensure(std::max<size_t>(1, hr.size()) * sizeof(DevRange)));
   DRX_HIPCHK(ctx, s->d_cp.ensure(ht.size() * slot_res(s) * 8));
   DRX_HIPCHK(ctx, s->d_mf.ensure(std::max<uint64_t>(16, mf_total) * 8));
@@ -253,7 +254,7 @@ int round_once(drx_server* s, std::vector<int>& finished) {
   if (!hf.empty())
     DRX_HIPCHK(ctx, hipMemcpyAsync(s->d_fspans.p, hf.data(), hf.size() * sizeof(DevSpan), hipMemcpyHostToDevice, st));
   if (!hseed.empty())
-    DRX_HIPCHK(ctx, hipMemcpyAsync(s->d_sspans.p, hseed.data(), hseed.size() * sizeof(DevSpan), hipMemcpyHostToDevice, st));
+    DRX_HIPCHK(ctx, hipMemcpyAsync(s->d_sspans.p, hseed.data(), hseed.size() * sizeof(DevTask), hipMemcpyHostToDevice, st));
   if (!hr.empty())
     DRX_HIPCHK(ctx, hipMemcpyAsync(s->d_ranges.p, hr.data(), hr.size() * sizeof(DevRange), hipMemcpyHostToDevice, st));
   int rc = upload_programs(ctx);
@@ -288,7 +289,7 @@ int round_once(drx_server* s, std::vector<int>& finished) {
   a.ffull = nullptr;
   a.out_ld = (uint64_t)s->kslot;
   for (Stage& g : stages) g.win_range = {g.range_begin, g.range_end};  // one window per round
-  rc = launch_stages(ctx, a, stages, s->d_sspans.as<DevSpan>(), 1, false, INT64_MAX, 1);
+  rc = launch_stages(ctx, a, stages, s->d_sspans.as<DevTask>(), 1, false, INT64_MAX, 1);
   if (!rc) rc = join_stages(ctx, false);
   if (rc) return rc;
   if (!hf.empty()) {
@@ -415,7 +416,7 @@ int drx_server_create(drx_ctx* ctx, const drx_server_cfg* cfg, drx_server** out)
   if (!s) return fail(ctx, DRX_ENOMEM, "out of host memory");
   s->ctx = ctx;
   s->cfg = *cfg;
-  s->kslot = (int)ctx->primes.size();
+  s->kslot = std::min<int>((int)ctx->primes.size(), kSliceK);  // a served walk takes at most 256 primes
   const int64_t budget = cfg->mem_budget > 0 ? cfg->mem_budget : ((int64_t)1 << 30);
   // resident state + per-round scratch of one slot
   const int64_t per_slot =

@@ -12,7 +12,7 @@ constexpr int kMaxDim = 15;       // axes per program (FP64 bank X: axes 0..14, 
 constexpr int kMaxConst = 1023;   // constants per program (10-bit field)
 constexpr int kMaxCode = 1 << 16; // device words per program
 constexpr int kFdMax2 = 6;        // forward-difference degree limit, 2x2 programs
-constexpr int kFdMax3 = 3;        // forward-difference degree limit, 3x3 programs
+constexpr int kFdMax3 = 3;        // forward-difference degree limit, 3x3 and 4x4 (companion) programs
 constexpr int kFdInv = 6;         // small inverses 1/t mod p kept per prime (t <= kFdInv)
 
 // FP64 shadow state per walk: P_f [m*m], then log_scale as a mantissa
@@ -65,7 +65,8 @@ struct DevWalk {
 // A contiguous step range of one walk processed by one workgroup.
 struct DevTask {
   int32_t walk;
-  int32_t shape;        // companion-PCF forward-difference walks: 8 * deg b + deg a, else 0
+  int32_t shape;        // forward-difference walks: companion PCF 8 * deg b + deg a, companion
+                        // CMF the last column's degree; else 0
   int64_t s_begin;
   int64_t s_end;
   uint64_t mf_off;      // M_f of step s at mf[mf_off + (s - s_begin) * m * m]
@@ -74,9 +75,9 @@ struct DevTask {
   int32_t kn;           // kSliceK primes run as several tasks per step range
 };
 
-// Widest prime stripe of one walk workgroup; an interpreted walk with more
-// primes splits each step range into slices of at most this many primes
-// (the primes are independent; the slices share the range's chunk product).
+// Widest prime stripe of one walk workgroup; a walk with more primes splits
+// each step range into slices of at most this many primes (the primes are
+// independent; the slices share the range's chunk product).
 constexpr int kSliceK = 256;
 
 // Per-walk span of tasks for the ordered combine / shadow passes.

@@ -5,7 +5,8 @@ dim5_trajectories.json, sweep_data/shifts/dim5_shifts.json).
 The files are data (pFq spec records, direction vectors, shift records); the
 subset keeps every 16th spec (with its n-form companion "matrix" strings,
 which pin the per-axis companion builder), 24 trajectories spread over the
-list, and every 8th shift. Run here, where the reference tree is mounted:
+list and every 8th shift; and, for the sweep benchmark, the first 64
+trajectories with all 512 shifts. Run here, where the reference tree is mounted:
 
     python tests/golden/make_sweep_subset.py
 """
@@ -37,6 +38,10 @@ def main() -> None:
         "trajectories": [tl[i] for i in tsel],
         "shift_index": list(range(0, len(shifts["shifts"]), 8)),
         "shifts": shifts["shifts"][::8],
+        # the sweep a driver runs with --max-traj 64: the first 64
+        # trajectories and every shift
+        "bench_trajectories": tl[:64],
+        "bench_shifts": shifts["shifts"],
     }
     with open(OUT, "w") as f:
         json.dump(out, f, separators=(",", ":"))

@@ -130,3 +130,32 @@ def test_batch_api_with_sweep_offsets(gpu_ctx, sweep_fix):
         assert g["p_residues"].tolist() == one["p_residues"].tolist()
         assert g["q_residues"].tolist() == one["q_residues"].tolist()
         assert g["p_float"].hex() == one["p_float"].hex() and g["q_float"].hex() == one["q_float"].hex()
+
+
+def test_companion_forward_differences_equal_interpreter(gpu_ctx):
+    # pFq companions (3x3 and 4x4, last columns of degree 0..3 with static
+    # ones) run on the companion forward-difference kernel; forcing the
+    # interpreter must give the same residues, shadow and CRT, bit for bit,
+    # across step ranges (chunk_steps 64) and prime slices (K 300)
+    pfq = [(2, 1, ["1/2", "-1/3"], ["3/2"]), (0, 2, [], ["1/2", "1/3"]), (2, 2, ["1", "2"], ["-1/2", "5/4"]),
+           (3, 2, ["-1/4", "-1/6", "1/6"], ["-1/3", "1/4"]), (3, 1, ["1/2", "1/2", "1"], ["2"]),
+           (1, 2, ["1/3"], ["1", "1/2"]), (3, 3, ["1", "1", "1"], ["2", "2", "2"])]
+    rng = np.random.default_rng(12)
+    specs = []
+    for (p, q, a, b), depth, K in zip(pfq * 2, [1, 2, 7, 100, 333, 1000, 5000] * 2, [16, 32, 300, 64, 300, 31, 128] * 2):
+        spec = {"p": p, "q": q, "a_params": a, "b_params": b}
+        traj = [int(v) for v in rng.integers(-2, 3, p + q)]
+        prog = compile_cmf_spec(spec, traj)
+        off = [int(v) for v in rng.integers(1, 5, p + q)]
+        specs.append(cmf_spec(prog, depth, K, off))
+    print("companion fd vs interpreter", len(specs), flush=True)
+    a = BatchPlan(specs, full=True, crt=True, chunk_steps=64).run()
+    b = BatchPlan(specs, full=True, crt=True, chunk_steps=64, interp_only=True).run()
+    assert np.array_equal(a.full, b.full)
+    assert np.array_equal(a.full_f, b.full_f)
+    assert np.array_equal(a.limbs, b.limbs) and np.array_equal(a.neg, b.neg)
+    primes = D.generate_rns_primes(300)
+    for i in (0, 3, 4, 6, 9, 12):
+        s = specs[i]
+        rr, Pf, ls = cref.walk(s.program, s.depth, primes[: s.K], list(s.axes))
+        assert np.array_equal(a.full_matrix(i, s.program.m), rr), i
