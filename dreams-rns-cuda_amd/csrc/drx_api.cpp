@@ -878,7 +878,11 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
     sp.walk = i;
     sp.out_row = i;
     sp.task_begin = (int)ht.size();
-    const int shape = shape_of(i);
+    // a companion walk from the identity in one step range, with only p and q
+    // asked for: the chunk product is needed in rows 0 and m - 1 only
+    const bool rows_only = kind == WALK_FD_COMP && w.init_kind == DRX_INIT_IDENTITY && !b.full &&
+                           w.depth <= (int64_t)chunk;
+    const int shape = shape_of(i) | (rows_only ? kShapeRows : 0);
     mf_total = (mf_total + 1) & ~(uint64_t)1;  // 16-byte aligned M_f per walk (its tasks stay contiguous)
     for (int64_t s0 = 0; s0 < w.depth; s0 += chunk) {
       DevTask t{};

@@ -394,15 +394,18 @@ __global__ void __launch_bounds__(512) k_fd_seed(const KArgs a, uint32_t kp) {
       E[e] = (((pg.kinds >> (2 * e)) & 3u) == E_GEN) ? L.le[e * ln.T + ln.tid] : 0.0;
     if (active) {
 #pragma unroll
-      for (int e = 0; e < M * M; ++e) tab[((uint64_t)e * (D + 1) + j) * Kw + k] = E[e];
+      for (int e = 0; e < M * M; ++e)
+        if (((pg.kinds >> (2 * e)) & 3u) == E_GEN) tab[((uint64_t)e * (D + 1) + j) * Kw + k] = E[e];
     }
   }
   __syncthreads();
   // all D + 1 points: the differences above an entry's degree come out as
-  // exact zeros, which the full-depth kernel relies on
+  // exact zeros, which the full-depth kernel relies on. Only general entries
+  // have tables; the walk kernels read no others.
   if (g == 0 && active) {
 #pragma unroll
     for (int e = 0; e < M * M; ++e) {
+      if (((pg.kinds >> (2 * e)) & 3u) != E_GEN) continue;  // uniform
       double N[D + 1];
 #pragma unroll
       for (int j = 0; j <= D; ++j) N[j] = tab[((uint64_t)e * (D + 1) + j) * Kw + k];
@@ -538,9 +541,10 @@ __global__ void __launch_bounds__(256) k_walk_fd(const KArgs a) {
   double N[M * M][D + 1];
 #pragma unroll
   for (int e = 0; e < M * M; ++e) {
+    const bool gen = ((kinds >> (2 * e)) & 3u) == E_GEN;  // uniform; other entries have no table
     const double* t = a.fdt + S.w.fdt_off + (uint64_t)e * (D + 1) * S.Kw + S.kg;
 #pragma unroll
-    for (int j = 0; j <= D; ++j) N[e][j] = t[(uint64_t)j * S.Kw];
+    for (int j = 0; j <= D; ++j) N[e][j] = gen ? t[(uint64_t)j * S.Kw] : 0.0;
     fd_jump<D>(N[e], S.sb, a, S.kg, p, pinv);
   }
   double P[M * M];
@@ -569,10 +573,12 @@ __global__ void __launch_bounds__(256) k_walk_fd(const KArgs a) {
 // (x_1, ..., x_{M-1}, sum_k x_k c_k(s)), M^2 modular products, and advances
 // the last column's Newton tables. A static 0 or 1 in the last column is a
 // table of degree 0.
-template <int M, int D>
+// With ROWS only rows 0 and M - 1 of P are carried (the rows of p and q).
+template <int M, int D, bool ROWS>
 __device__ __forceinline__ void comp_step(double (&P)[M][M], double (&T)[M][D + 1], double p, double pinv) {
 #pragma unroll
   for (int i = 0; i < M; ++i) {
+    if (ROWS && i != 0 && i != M - 1) continue;
     double acc = mm(P[i][0], T[0][0], p, pinv);
 #pragma unroll
     for (int k = 1; k < M; ++k) acc += mm(P[i][k], T[k][0], p, pinv);
@@ -584,7 +590,7 @@ __device__ __forceinline__ void comp_step(double (&P)[M][M], double (&T)[M][D + 
   for (int i = 0; i < M; ++i) fd_advance<D>(T[i]);
 }
 
-template <int M, int D>
+template <int M, int D, bool ROWS>
 __device__ __forceinline__ void comp_fd_body(const KArgs& a, const WalkSetup& S) {
   constexpr int U = fd_unroll(D);
   constexpr int DT = fd_dmax(M);  // table rows stored per entry
@@ -607,29 +613,38 @@ __device__ __forceinline__ void comp_fd_body(const KArgs& a, const WalkSetup& S)
   int64_t n = S.se - S.sb;
   for (; n >= U; n -= U) {
 #pragma unroll
-    for (int u = 0; u < U; ++u) comp_step<M, D>(P, T, p, pinv);
+    for (int u = 0; u < U; ++u) comp_step<M, D, ROWS>(P, T, p, pinv);
 #pragma unroll
     for (int i = 0; i < M; ++i) fd_reduce<D>(T[i], p, pinv);
   }
-  for (; n > 0; --n) comp_step<M, D>(P, T, p, pinv);
+  for (; n > 0; --n) comp_step<M, D, ROWS>(P, T, p, pinv);
   double Q[M * M];
 #pragma unroll
   for (int i = 0; i < M; ++i)
 #pragma unroll
-    for (int j = 0; j < M; ++j) Q[i * M + j] = P[i][j];
+    for (int j = 0; j < M; ++j) Q[i * M + j] = (ROWS && i != 0 && i != M - 1) ? 0.0 : P[i][j];
   tree_combine<M>(a, S, Q);
 }
 
-// The task's shape is the last column's degree (uniform over the workgroup).
+// The task's shape is the last column's degree (uniform over the workgroup);
+// bit kShapeRows marks a task whose chunk product is needed in rows 0 and
+// M - 1 only (the walk's one task from the identity, no full state asked),
+// which the kernel uses when the task is one sub-chunk (C == 1): half the
+// products for a 4x4 companion.
 template <int M>
 __global__ void __launch_bounds__(256) k_walk_fd_comp(const KArgs a) {
   extern __shared__ double lds[];
   const WalkSetup S = walk_setup(a, lds, false);
-  switch (S.t.shape) {
-    case 0: comp_fd_body<M, 0>(a, S); break;
-    case 1: comp_fd_body<M, 1>(a, S); break;
-    case 2: comp_fd_body<M, 2>(a, S); break;
-    case 3: comp_fd_body<M, 3>(a, S); break;
+  const bool rows = (S.t.shape & kShapeRows) != 0 && S.C == 1;  // uniform
+  switch ((S.t.shape & (kShapeRows - 1)) * 2 + (rows ? 1 : 0)) {
+    case 0: comp_fd_body<M, 0, false>(a, S); break;
+    case 1: comp_fd_body<M, 0, true>(a, S); break;
+    case 2: comp_fd_body<M, 1, false>(a, S); break;
+    case 3: comp_fd_body<M, 1, true>(a, S); break;
+    case 4: comp_fd_body<M, 2, false>(a, S); break;
+    case 5: comp_fd_body<M, 2, true>(a, S); break;
+    case 6: comp_fd_body<M, 3, false>(a, S); break;
+    case 7: comp_fd_body<M, 3, true>(a, S); break;
     default: break;
   }
 }
@@ -1159,22 +1174,33 @@ __global__ void __launch_bounds__(256) k_garner_wave(const GarnerArgs g) {
     pu[jj] = in ? g.prime_u[i] : 3u;
     t[jj] = in ? (double)r[i] : 0.0;
   }
-  // mixed-radix digits: after step j, t_j is final
+  // mixed-radix digits: after step j, t_j is final. The inverses of step
+  // j + 1 are loaded while step j computes (each register's next load is
+  // issued right after its use), so the table's latency hides behind the
+  // arithmetic.
 #pragma unroll
   for (int jr = 0; jr < J; ++jr) {
     const int jbase = jr * 64;
     const int lim = (K - jbase < 64) ? K - jbase : 64;
+    if (lim <= 0) break;  // uniform
+    uint32_t wn[J];
+#pragma unroll
+    for (int jj = jr; jj < J; ++jj) {
+      const int i = jj * 64 + lane;
+      wn[jj] = (i > jbase && i < K) ? g.inv_table[(uint64_t)jbase * g.inv_ld + i] : 0u;
+    }
     for (int jl = 0; jl < lim; ++jl) {
       const int j = jbase + jl;
       const double vj = (double)__builtin_amdgcn_readlane((int)t[jr], jl);
-      const uint32_t* inv = g.inv_table + (uint64_t)j * g.inv_ld;
+      const bool more = jl + 1 < lim;
+      const uint32_t* inv_next = g.inv_table + (uint64_t)(j + 1) * g.inv_ld;
 #pragma unroll
       for (int jj = jr; jj < J; ++jj) {
         const int i = jj * 64 + lane;
         const bool upd = i > j && i < K;
-        const double w = upd ? (double)inv[i] : 0.0;
-        const double x = canon(mm(t[jj] - vj, w, pd[jj], pv[jj]), pd[jj], pv[jj]);
+        const double x = canon(mm(t[jj] - vj, (double)wn[jj], pd[jj], pv[jj]), pd[jj], pv[jj]);
         t[jj] = upd ? x : t[jj];
+        wn[jj] = (more && i > j + 1 && i < K) ? inv_next[i] : 0u;
       }
     }
   }

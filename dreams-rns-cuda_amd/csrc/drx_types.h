@@ -66,7 +66,7 @@ struct DevWalk {
 struct DevTask {
   int32_t walk;
   int32_t shape;        // forward-difference walks: companion PCF 8 * deg b + deg a, companion
-                        // CMF the last column's degree; else 0
+                        // CMF the last column's degree (| kShapeRows); else 0
   int64_t s_begin;
   int64_t s_end;
   uint64_t mf_off;      // M_f of step s at mf[mf_off + (s - s_begin) * m * m]
@@ -74,6 +74,9 @@ struct DevTask {
   int32_t k0;           // primes [k0, k0 + kn) of the walk: walks of more than
   int32_t kn;           // kSliceK primes run as several tasks per step range
 };
+
+// Companion CMF task whose chunk product is needed in rows 0 and m - 1 only.
+constexpr int32_t kShapeRows = 256;
 
 // Widest prime stripe of one walk workgroup; a walk with more primes splits
 // each step range into slices of at most this many primes (the primes are

@@ -159,3 +159,33 @@ def test_companion_forward_differences_equal_interpreter(gpu_ctx):
         s = specs[i]
         rr, Pf, ls = cref.walk(s.program, s.depth, primes[: s.K], list(s.axes))
         assert np.array_equal(a.full_matrix(i, s.program.m), rr), i
+
+
+def test_companion_rows_only_path(gpu_ctx):
+    # without the full state, a companion walk from the identity in one step
+    # range carries rows 0 and m - 1 only when its workgroup holds one
+    # sub-chunk (256 primes per slice); p, q, the shadow and CRT must equal
+    # the interpreter's, and walks of two ranges (depth > 4096) the full path
+    pfq = [(3, 2, ["-1/4", "-1/6", "1/6"], ["-1/3", "1/4"]), (2, 1, ["1/2", "-1/3"], ["3/2"]),
+           (0, 2, [], ["1/2", "1/3"]), (3, 3, ["1", "1", "1"], ["2", "2", "2"])]
+    rng = np.random.default_rng(21)
+    specs = []
+    for (p, q, a, b), depth, K in zip(pfq * 3, [1, 9, 500, 2000, 4096, 5000, 777, 3, 64, 1500, 4097, 250],
+                                      [256, 300, 1000, 512, 256, 257, 64, 1000, 256, 700, 300, 32]):
+        spec = {"p": p, "q": q, "a_params": a, "b_params": b}
+        prog = compile_cmf_spec(spec, [int(v) for v in rng.integers(-2, 3, p + q)])
+        specs.append(cmf_spec(prog, depth, K, [int(v) for v in rng.integers(1, 5, p + q)]))
+    print("companion rows-only", len(specs), flush=True)
+    a = BatchPlan(specs, crt=True).run()
+    b = BatchPlan(specs, crt=True, interp_only=True).run()
+    assert np.array_equal(a.residues, b.residues)
+    assert np.array_equal(a.limbs, b.limbs) and np.array_equal(a.neg, b.neg)
+    for key in ("p_float", "q_float", "log_scale"):
+        assert [x.hex() for x in a.out[key].tolist()] == [x.hex() for x in b.out[key].tolist()], key
+    primes = D.generate_rns_primes(1000)
+    for i in (2, 3, 5, 7):
+        s = specs[i]
+        rr, Pf, ls = cref.walk(s.program, s.depth, primes[: s.K], list(s.axes))
+        d = a.walk_dict(i)
+        m = s.program.m
+        assert d["p_residues"].tolist() == rr[0, m - 1].tolist() and d["q_residues"].tolist() == rr[m - 1, m - 1].tolist(), i
