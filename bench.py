@@ -293,6 +293,36 @@ def other_workload(args):
         tm = ctx.last_timing()
         walk_ms.append(tm[0])
         log(f"step {i}: {step_ms[-1]:.2f} ms (walk kernels {tm[0]:.3f} ms, device {tm[4]:.3f})")
+    post = None
+    if args.workload == "sweep3f2":
+        # the step after the walk: the sweep's hit selection over this batch
+        from dreams_rns.constants import load_constants
+        from dreams_rns.postwalk import sweep_hits
+        from dreams_rns.sweep import SweepBatch
+
+        bank = load_constants(dps=200)
+        sb = SweepBatch(spec_idx=np.array([r[0] for r in rows], dtype=np.int32),
+                        traj_idx=np.array([r[1] for r in rows], dtype=np.int32), offsets=[r[3] for r in rows],
+                        shift_rows=[r[4] for r in rows], result=plan.run(), crt=True)
+        t = time.perf_counter()
+        hits = sweep_hits(sb, [spec], fix["bench_trajectories"], bank, proximity=1e-3, dps=200)
+        post = {"ms": (time.perf_counter() - t) * 1e3, "hits": len(hits), "walks": sweep_walks,
+                "what": "a100_sweep.py:426-505 hit selection (CRT check, proximity, delta against the 40-constant "
+                        "bank) over the batch's results, on the host"}
+        log(f"hit selection: {post['ms']:.1f} ms, {len(hits)} hits")
+        if not args.no_cpu:
+            from oracle import postwalk as opw
+
+            t = time.perf_counter()
+            ns = 0
+            while ns < len(rows) and time.perf_counter() - t < 5.0:
+                opw.walk_record(sb.result.walk_dict(ns), bank, 1e-3, 200)
+                ns += 1
+            per = (time.perf_counter() - t) / ns
+            post["cpu_loop_ms_per_walk"] = per * 1e3
+            post["cpu_loop_sample"] = (f"{ns} walks of the batch through the walk-by-walk restatement of the "
+                                       "reference loop (Python CRT, mpmath), one core")
+            log(f"reference-style loop: {per * 1e3:.2f} ms per walk")
     st = plan.stats()
     plan.close()
     ms = float(np.mean(step_ms))
@@ -320,7 +350,8 @@ def other_workload(args):
                                    "so frac can pass 1)",
                      "executed": executed},
         "cpu_baseline": cpu,
-        **({"sweep_walks_per_s": sweep_walks / (ms * 1e-3)} if args.workload == "sweep3f2" else {}),
+        **({"sweep_walks_per_s": sweep_walks / (ms * 1e-3), "hit_selection": post} if args.workload == "sweep3f2"
+           else {}),
     }))
 
 

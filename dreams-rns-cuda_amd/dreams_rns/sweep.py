@@ -20,7 +20,10 @@ with its own FP64 shadow.
 
 from __future__ import annotations
 
+import csv
 import json
+import os
+import time
 from dataclasses import dataclass, field
 from typing import Any, Dict, Iterator, List, Optional, Sequence, Tuple
 
@@ -29,7 +32,8 @@ import numpy as np
 from .cmf_walk import compile_cmf_spec, shift_to_axis_offsets
 from .engine import BatchPlan, BatchResult, cmf_spec
 
-__all__ = ["load_cmf_specs", "load_trajectories", "load_shifts", "sweep_offsets", "SweepBatch", "CmfSweep"]
+__all__ = ["load_cmf_specs", "load_trajectories", "load_shifts", "sweep_offsets", "SweepBatch", "CmfSweep",
+           "run_cmf_sweep", "save_hits"]
 
 
 def load_cmf_specs(path: str) -> List[Dict]:
@@ -173,3 +177,52 @@ class CmfSweep:
                 rows = []
         if rows:
             yield self._run(rows)
+
+
+def run_cmf_sweep(specs: Sequence[Dict], trajectories: Sequence[Sequence[int]], shifts: Sequence[Dict],
+                  constants: Sequence[Dict[str, Any]], depth: int = 2000, K: int = 1000, proximity: float = 1e-3,
+                  dps: int = 200, max_walks: int = 8192, **plan_kw) -> Tuple[List[Dict[str, Any]], Dict[str, Any]]:
+    """The CMF sweep with its hit selection (a100_sweep.py:336-528): every
+    spec x trajectory x shift walked on the GPU (GPU CRT), then
+    ``postwalk.sweep_hits``. Returns (hits, stats) with stats total_runs
+    (specs x trajectories x shifts, skipped pairs included, as the reference
+    counts them), total_hits, skipped, elapsed_s, device_s (batches) and
+    post_s (hit selection)."""
+    from .postwalk import sweep_hits
+
+    t0 = time.perf_counter()
+    sw = CmfSweep(specs, trajectories, shifts, depth=depth, K=K, max_walks=max_walks, crt=True, plan_kw=plan_kw)
+    hits: List[Dict[str, Any]] = []
+    dev = post = 0.0
+    it = iter(sw)
+    while True:
+        t = time.perf_counter()
+        try:
+            batch = next(it)
+        except StopIteration:
+            break
+        dev += time.perf_counter() - t
+        t = time.perf_counter()
+        hits += sweep_hits(batch, specs, trajectories, constants, proximity, dps)
+        post += time.perf_counter() - t
+    return hits, {"total_runs": sw.total_walks, "total_hits": len(hits), "skipped": list(sw.skipped),
+                  "elapsed_s": time.perf_counter() - t0, "device_s": dev, "post_s": post}
+
+
+def save_hits(hits: Sequence[Dict[str, Any]], output_dir: str, prefix: str = "hits") -> str:
+    """Hits to <prefix>.jsonl (every field) and, when there are any,
+    <prefix>.csv (columns of the first hit, lists written as their str)
+    (a100_sweep.py:533-556). Returns the JSONL path."""
+    os.makedirs(output_dir, exist_ok=True)
+    path = os.path.join(output_dir, f"{prefix}.jsonl")
+    with open(path, "w") as f:
+        for h in hits:
+            f.write(json.dumps(h, default=str) + "\n")
+    if hits:
+        keys = list(hits[0].keys())
+        with open(os.path.join(output_dir, f"{prefix}.csv"), "w", newline="") as f:
+            w = csv.DictWriter(f, fieldnames=keys)
+            w.writeheader()
+            for h in hits:
+                w.writerow({k: str(v) if isinstance(v, list) else v for k, v in h.items()})
+    return path

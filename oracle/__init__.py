@@ -20,6 +20,8 @@ Modules
 ``cref``   ctypes binding to ``oracle/cwalk.c`` (gcc), a C restatement of
            the residue walk and of the FMA-chain shadow, fast enough to
            check thousands of walks.
+``postwalk`` the CMF sweep's per-walk post-processing (CRT check, constant
+           proximity, delta against the bank) restated walk by walk.
 
 Parity status of each function is listed in the module docstrings; nothing
 here is "parity unpinned" except where a docstring says so.

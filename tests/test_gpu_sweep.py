@@ -189,3 +189,45 @@ def test_companion_rows_only_path(gpu_ctx):
         d = a.walk_dict(i)
         m = s.program.m
         assert d["p_residues"].tolist() == rr[0, m - 1].tolist() and d["q_residues"].tolist() == rr[m - 1, m - 1].tolist(), i
+
+
+def test_sweep_hits_on_device_batches(gpu_ctx, sweep_fix):
+    # the vectorised hit selection over real device batches equals the
+    # reference loop run walk by walk (oracle/postwalk.py), for overflowed
+    # CRTs (3F2, depth 300, K 32) and exact ones (an integer 2F1, depth 25)
+    from dreams_rns.constants import load_constants
+    from dreams_rns.postwalk import sweep_hits
+    from oracle import postwalk as opw
+
+    bank = load_constants(dps=60)
+    cases = [
+        ([sweep_fix["specs"][0], sweep_fix["specs"][44]], [sweep_fix["trajectories"][i] for i in (0, 3, 17)],
+         sweep_fix["shifts"][:24], 300, 32),
+        ([{"name": "2F1(1,1;2)", "spec_hash": "int", "rank": 3, "p": 2, "q": 1, "dim": 3, "a_params": ["1", "1"],
+           "b_params": ["2"]}], [[1, 1, 1], [1, 0, 1], [2, 1, 1]], sweep_fix["shifts"][:16], 25, 32),
+    ]
+    n_hits = 0
+    n_exact = 0
+    for specs, trajs, shifts, depth, K in cases:
+        for batch in CmfSweep(specs, trajs, shifts, depth=depth, K=K, crt=True):
+            print("hits batch", len(batch), flush=True)
+            got = sweep_hits(batch, specs, trajs, bank, proximity=1e-3, dps=60)
+            ref = []
+            for i in range(len(batch)):
+                d = batch.result.walk_dict(i)
+                h = opw.walk_record(d, bank, 1e-3, 60)
+                p_big, q_big = batch.result.crt_values(i)
+                if q_big != 0 and max(abs(p_big), abs(q_big)).bit_length() < 1000:
+                    n_exact += 1
+                if h is None:
+                    continue
+                si, ti = int(batch.spec_idx[i]), int(batch.traj_idx[i])
+                for s in batch.shift_rows[i]:
+                    ref.append((si, ti, s, {"cmf_name": specs[si]["name"], "traj_idx": ti, "shift_idx": s, **h}))
+            ref.sort(key=lambda t: t[:3])
+            ref = [t[3] for t in ref]
+            assert len(got) == len(ref)
+            for g, r in zip(got, ref):
+                assert {k: g[k] for k in r} == r
+            n_hits += len(got)
+    assert n_exact > 0
