@@ -747,6 +747,10 @@ __device__ __forceinline__ void run_program_fp(const uint2* __restrict__ code, u
 // M_f[0..M)[j] in one piece.
 template <int M>
 __global__ void __launch_bounds__(256) k_mf(const KArgs a) {
+  // the shadow chain waits for each window of M_f while the walk kernel
+  // fills the device: M_f waves (scalar-decode bound) issue before the walk's
+  // VALU-bound waves on a shared SIMD
+  __builtin_amdgcn_s_setprio(2);
   extern __shared__ double lds[];
   const DevRange r = a.ranges[blockIdx.x];
   const DevTask t = a.tasks[r.task];
@@ -892,6 +896,7 @@ __device__ __forceinline__ void shadow_run_tile(ShadowLane& L, const double (&t)
 // from the state of the previous window (the initial state when w0 == 0).
 template <int M>
 __global__ void __launch_bounds__(256) k_shadow(const KArgs a, int nspan, int finalize, int64_t w0, int64_t w1) {
+  __builtin_amdgcn_s_setprio(3);  // the chain's few waves first
   constexpr int G = (M == 2) ? 4 : 16;
   constexpr int T = kShadowTile;
   const int lg = threadIdx.x % G;
