@@ -231,3 +231,35 @@ def test_sweep_hits_on_device_batches(gpu_ctx, sweep_fix):
                 assert {k: g[k] for k in r} == r
             n_hits += len(got)
     assert n_exact > 0
+
+
+def test_served_companion_walks_equal_batched(gpu_ctx, sweep_fix):
+    # 3x3 / 4x4 companions run on the companion forward-difference kernel in
+    # the server too (Newton tables seeded on a walk's first round, carried
+    # across rounds); each result equals its batched run bit for bit
+    from dreams_rns.server import WalkServer
+
+    rng = np.random.default_rng(8)
+    specs = []
+    for i in range(10):
+        spec = sweep_fix["specs"][3 * i] if i % 2 == 0 else {"p": 2, "q": 1, "a_params": ["1/2", "1/3"],
+                                                                "b_params": ["5/4"]}
+        dim = 5 if i % 2 == 0 else 3
+        prog = compile_cmf_spec(spec, [int(v) for v in rng.integers(-2, 3, dim)])
+        specs.append(cmf_spec(prog, int(rng.integers(30, 700)), int(rng.choice([16, 32, 64])),
+                              [int(v) for v in rng.integers(1, 5, dim)]))
+    with BatchPlan(specs) as plan:
+        ref = plan.run()
+        ref = [ref.walk_dict(i) for i in range(len(specs))]
+    print("served companions", len(specs), flush=True)
+    with WalkServer(round_steps=53, max_slots=6) as srv:
+        tickets = {srv.submit(s): i for i, s in enumerate(specs)}
+        results = srv.drain(timeout=120)
+    assert len(results) == len(specs)
+    for r in results:
+        assert r["status"] == 0
+        i = tickets[r["ticket"]]
+        assert r["p_residues"].tolist() == ref[i]["p_residues"].tolist(), i
+        assert r["q_residues"].tolist() == ref[i]["q_residues"].tolist(), i
+        for key in ("p_float", "q_float", "log_scale"):
+            assert float(r[key]).hex() == float(ref[i][key]).hex(), (i, key)
