@@ -9,6 +9,7 @@
 #include <new>
 #include <numeric>
 
+#include "drx_crt.h"
 #include "drx_ctx.h"
 #include "drx_device.h"
 
@@ -324,6 +325,84 @@ int ensure_inv_table(drx_ctx* ctx, int k) {
   return DRX_OK;
 }
 
+// Matrix-core CRT table of the first K primes: M = prod p_i as 32-bit limbs,
+// w_i = (M / p_i)^-1 mod p_i, the balanced base-256 digits of every M / p_i
+// (column i of BT, rows t; the top digit is the final carry), M and
+// (M - 1) / 2 padded to L + 2 limbs. Built once per K; a prime change other
+// than an extension drops the tables (their buffers are retired).
+int ensure_crt_table(drx_ctx* ctx, int K, const CrtTable** out) {
+  for (const auto& t : ctx->crt_tables)
+    if (t->K == K) {
+      *out = t.get();
+      return DRX_OK;
+    }
+  if (K < 1 || K > (int)ctx->primes.size()) return fail(ctx, DRX_EINVAL, "K=%d exceeds the primes set", K);
+  std::vector<uint32_t> M{1u};
+  for (int i = 0; i < K; ++i) {
+    uint64_t carry = 0;
+    for (auto& limb : M) {
+      const uint64_t x = (uint64_t)limb * ctx->primes[i] + carry;
+      limb = (uint32_t)x;
+      carry = x >> 32;
+    }
+    if (carry) M.push_back((uint32_t)carry);
+  }
+  std::unique_ptr<CrtTable> t(new CrtTable());
+  t->K = K;
+  t->L = (int)M.size();
+  t->Kp = (K + 15) / 16 * 16;
+  t->T = 4 * t->L + 1;
+  t->Tp = (t->T + 127) / 128 * 128;
+  const int L = t->L, Kp = t->Kp;
+  std::vector<int8_t> bt((size_t)t->Tp * Kp, 0);
+  std::vector<double> w(K);
+  std::vector<uint32_t> q(L);
+  for (int i = 0; i < K; ++i) {
+    const uint64_t p = ctx->primes[i];
+    uint64_t rem = 0;
+    for (int l = L - 1; l >= 0; --l) {  // q = M / p (exact)
+      const uint64_t cur = (rem << 32) | M[l];
+      q[l] = (uint32_t)(cur / p);
+      rem = cur % p;
+    }
+    if (rem != 0) return fail(ctx, DRX_EINVAL, "internal: M not divisible by prime %d", i);
+    uint64_t r = 0;
+    for (int l = L - 1; l >= 0; --l) r = ((r << 32) | q[l]) % p;  // (M / p) mod p
+    w[i] = (double)inv_mod((uint32_t)r, (uint32_t)p);
+    int carry = 0;
+    for (int d = 0; d < 4 * L; ++d) {
+      int v = (int)((q[d / 4] >> (8 * (d % 4))) & 255u) + carry;
+      carry = v >= 128 ? 1 : 0;
+      bt[(size_t)d * Kp + i] = (int8_t)(v - 256 * carry);
+    }
+    bt[(size_t)(4 * L) * Kp + i] = (int8_t)carry;
+  }
+  std::vector<uint32_t> m(L + 2, 0u), h(L + 2, 0u);
+  for (int l = 0; l < L; ++l) m[l] = M[l];
+  for (int l = 0; l < L; ++l) h[l] = (M[l] >> 1) | (l + 1 < L ? (M[l + 1] << 31) : 0u);  // (M - 1) / 2, M odd
+  HIPCHK(ctx, t->d_bt.ensure(bt.size()));
+  HIPCHK(ctx, hipMemcpy(t->d_bt.p, bt.data(), bt.size(), hipMemcpyHostToDevice));
+  HIPCHK(ctx, t->d_w.ensure(w.size() * 8));
+  HIPCHK(ctx, hipMemcpy(t->d_w.p, w.data(), w.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(ctx, t->d_m.ensure(m.size() * 4));
+  HIPCHK(ctx, hipMemcpy(t->d_m.p, m.data(), m.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(ctx, t->d_h.ensure(h.size() * 4));
+  HIPCHK(ctx, hipMemcpy(t->d_h.p, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+  *out = t.get();
+  ctx->crt_tables.push_back(std::move(t));
+  return DRX_OK;
+}
+
+void retire_crt_tables(drx_ctx* ctx) {
+  for (auto& t : ctx->crt_tables)
+    for (DBuf* d : {&t->d_bt, &t->d_w, &t->d_m, &t->d_h})
+      if (d->p) {
+        ctx->retired.push_back(d->p);
+        d->p = nullptr;
+        d->cap = 0;
+      }
+  ctx->crt_tables.clear();
+}
 
 int axis_tables(drx_ctx* ctx, const int64_t* q, int dim, int K, int64_t depth, double* axr, int64_t* axi,
                 double* axd) {
@@ -399,6 +478,13 @@ struct drx_plan {
   DBuf d_fout, d_rout, d_pout, d_pfull, d_limbs, d_neg, d_topk, d_topk_ws, d_rowk;
   KArgs args{};
   GarnerArgs gargs{};
+  // CRT of rows of at least kCrtMatrixMinK primes: one matrix-core group per K
+  struct CrtGroup {
+    DBuf d_rows, d_a, d_c, d_k;
+    CrtArgs args{};
+  };
+  std::vector<std::unique_ptr<CrtGroup>> crt_groups;
+  DBuf d_grows;  // rows of the Garner kernel when some rows take the matrix path
   int ntask = 0;
   double residue_steps = 0, mm_total = 0, ma_total = 0;
   hipGraph_t graph = nullptr;
@@ -414,8 +500,10 @@ struct drx_plan {
     if (graph) (void)hipGraphDestroy(graph);
     for (DBuf* d : {&d_walks, &d_tasks, &d_seeds, &d_spans, &d_ranges, &d_res0, &d_fst0, &d_res, &d_fst, &d_axr, &d_axi,
                     &d_axd, &d_cp, &d_mf, &d_fdt, &d_fout, &d_rout, &d_pout, &d_pfull, &d_limbs, &d_neg, &d_topk,
-                    &d_topk_ws, &d_rowk, &d_vgroups, &d_fgroups, &d_vorder})
+                    &d_topk_ws, &d_rowk, &d_vgroups, &d_fgroups, &d_vorder, &d_grows})
       d->release();
+    for (auto& g : crt_groups)
+      for (DBuf* d : {&g->d_rows, &g->d_a, &g->d_c, &g->d_k}) d->release();
     for (drx_plan* p : plain) delete p;
   }
 };
@@ -431,10 +519,101 @@ int upload(drx_ctx* ctx, DBuf& d, const std::vector<T>& h) {
 
 // Vector walks: the residue kernel on the library's stream and the FP64
 // shadow on stream2 side by side; CRT after the residues, top-k after both.
+// CRT of the plan's p and q rows: the Garner kernel for rows of fewer than
+// kCrtMatrixMinK primes, one matrix-core group per wider K.
+int launch_crt(drx_plan* pl, hipStream_t s) {
+  drx_ctx* ctx = pl->ctx;
+  if (pl->gargs.n > 0) HIPCHK(ctx, launch_garner(pl->gargs, s));
+  for (const auto& g : pl->crt_groups) HIPCHK(ctx, launch_crt_matrix(g->args, s));
+  return DRX_OK;
+}
+
+// Split the CRT rows (2 per walk) by prime count and set up both paths; runs
+// after the outputs are allocated and GarnerArgs' common fields are set.
+int setup_crt(drx_ctx* ctx, drx_plan* pl) {
+  const drx_batch& b = pl->b;
+  pl->crt_groups.clear();
+  GarnerArgs& g = pl->gargs;
+  if (!b.crt_limbs) {
+    g.n = 0;
+    return DRX_OK;
+  }
+  std::vector<int32_t> grows;
+  std::vector<std::pair<int, std::vector<int32_t>>> wide;
+  int gk = 1;
+  for (int i = 0; i < pl->n; ++i) {
+    const int K = b.walks[i].K;
+    for (int v = 0; v < 2; ++v) {
+      if (K < kCrtMatrixMinK) {
+        grows.push_back(2 * i + v);
+        gk = std::max(gk, K);
+        continue;
+      }
+      auto it = std::find_if(wide.begin(), wide.end(), [&](const std::pair<int, std::vector<int32_t>>& e) {
+        return e.first == K;
+      });
+      if (it == wide.end()) {
+        wide.push_back({K, {}});
+        it = wide.end() - 1;
+      }
+      it->second.push_back(2 * i + v);
+    }
+  }
+  int rc;
+  g.n = (int)grows.size();
+  g.K = gk;
+  g.rows = nullptr;
+  if (!wide.empty() && !grows.empty()) {
+    if ((rc = upload(ctx, pl->d_grows, grows))) return rc;
+    g.rows = pl->d_grows.as<int32_t>();
+  }
+  if (!grows.empty()) {
+    if ((rc = ensure_inv_table(ctx, gk))) return rc;
+    g.inv_table = ctx->d_inv.as<uint32_t>();
+    g.inv_ld = (uint64_t)ctx->kinv;
+  }
+  for (auto& e : wide) {
+    const CrtTable* t = nullptr;
+    if ((rc = ensure_crt_table(ctx, e.first, &t))) return rc;
+    std::unique_ptr<drx_plan::CrtGroup> grp(new drx_plan::CrtGroup());
+    const int ng = (int)e.second.size(), npad = (ng + 127) / 128 * 128;
+    if ((size_t)t->L + 2 > (size_t)pl->ld) return fail(ctx, DRX_EINVAL, "internal: CRT rows narrower than M");
+    if ((rc = upload(ctx, grp->d_rows, e.second))) return rc;
+    HIPCHK(ctx, grp->d_a.ensure((size_t)4 * npad * t->Kp));
+    HIPCHK(ctx, grp->d_c.ensure((size_t)4 * t->Tp * npad * 4));
+    HIPCHK(ctx, grp->d_k.ensure((size_t)npad * 4));
+    CrtArgs& c = grp->args;
+    c.res = pl->d_rout.as<int64_t>();
+    c.res_stride = (uint64_t)pl->ld;
+    c.rows = grp->d_rows.as<int32_t>();
+    c.n = ng;
+    c.npad = npad;
+    c.K = t->K;
+    c.Kp = t->Kp;
+    c.T = t->T;
+    c.Tp = t->Tp;
+    c.L = t->L;
+    c.prime_p = ctx->d_prime_p.as<double>();
+    c.prime_pinv = ctx->d_prime_pinv.as<double>();
+    c.w = t->d_w.as<double>();
+    c.bt = t->d_bt.as<int8_t>();
+    c.m = t->d_m.as<uint32_t>();
+    c.h = t->d_h.as<uint32_t>();
+    c.A = grp->d_a.as<int8_t>();
+    c.C = grp->d_c.as<int32_t>();
+    c.kq = grp->d_k.as<int32_t>();
+    c.limbs = pl->d_limbs.as<uint32_t>();
+    c.neg = pl->d_neg.as<uint8_t>();
+    pl->crt_groups.push_back(std::move(grp));
+  }
+  return DRX_OK;
+}
+
 int launch_vec_sequence(drx_plan* pl, bool timed) {
   drx_ctx* ctx = pl->ctx;
   hipStream_t s = ctx->stream, s2 = ctx->stream2;
   hipEvent_t* ev = ctx->ev;
+  int rc;
   HIPCHK(ctx, hipEventRecord(ev[kEvFork], s));
   HIPCHK(ctx, hipStreamWaitEvent(s2, ev[kEvFork], 0));
   HIPCHK(ctx, launch_vec_fp(pl->args, pl->nfg, pl->fthreads, pl->flds, s2));
@@ -445,7 +624,7 @@ int launch_vec_sequence(drx_plan* pl, bool timed) {
     HIPCHK(ctx, hipEventRecord(ev[1], s));
     HIPCHK(ctx, hipEventRecord(ev[2], s));
   }
-  if (pl->b.crt_limbs) HIPCHK(ctx, launch_garner(pl->gargs, s));
+  if (pl->b.crt_limbs && (rc = launch_crt(pl, s))) return rc;
   HIPCHK(ctx, hipStreamWaitEvent(s, ev[kEvJoin], 0));
   if (timed) HIPCHK(ctx, hipEventRecord(ev[3], s));
   if (pl->b.top_k > 0)
@@ -465,10 +644,10 @@ int launch_sequence(drx_plan* pl, bool timed) {
   // the shadow chain is still going; the final P_f copy needs the shadow
   const bool ffull = pl->args.ffull != nullptr;
   if (!ffull) HIPCHK(ctx, launch_finalize(pl->args, (int)pl->walk_order.size(), pl->b.full != nullptr, s));
-  if (!ffull && pl->b.crt_limbs) HIPCHK(ctx, launch_garner(pl->gargs, s));
+  if (!ffull && pl->b.crt_limbs && (rc = launch_crt(pl, s))) return rc;
   if ((rc = join_stages(ctx, timed))) return rc;
   if (ffull) HIPCHK(ctx, launch_finalize(pl->args, (int)pl->walk_order.size(), pl->b.full != nullptr, s));
-  if (ffull && pl->b.crt_limbs) HIPCHK(ctx, launch_garner(pl->gargs, s));
+  if (ffull && pl->b.crt_limbs && (rc = launch_crt(pl, s))) return rc;
   if (pl->b.top_k > 0)
     HIPCHK(ctx, launch_topk(pl->d_fout.as<double>() + 3, 6, pl->n, pl->b.top_k, pl->d_topk.as<int32_t>(),
                             pl->d_topk_ws.p, s));
@@ -530,7 +709,6 @@ int alloc_outputs(drx_ctx* ctx, drx_plan* pl) {
   }
   if (b.crt_limbs) {
     int rc;
-    if ((rc = ensure_inv_table(ctx, pl->kmax))) return rc;
     std::vector<int32_t> rowk(2 * (size_t)n);
     for (int i = 0; i < n; ++i) rowk[2 * i] = rowk[2 * i + 1] = b.walks[i].K;
     if ((rc = upload(ctx, pl->d_rowk, rowk))) return rc;
@@ -674,6 +852,7 @@ int plan_build_vec(drx_ctx* ctx, drx_plan* pl) {
   if ((rc = alloc_outputs(ctx, pl))) return rc;
   if ((rc = upload_programs(ctx))) return rc;
   set_common_args(ctx, pl);
+  if ((rc = setup_crt(ctx, pl))) return rc;
   pl->args.vgroups = pl->d_vgroups.as<DevVGroup>();
   pl->args.fgroups = pl->d_fgroups.as<DevVGroup>();
   pl->args.vorder = pl->d_vorder.as<int32_t>();
@@ -980,7 +1159,6 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
     HIPCHK(ctx, pl->d_topk_ws.ensure(std::max<size_t>(16, topk_workspace_bytes(n, b.top_k))));
   }
   if (b.crt_limbs) {
-    if ((rc = ensure_inv_table(ctx, kmax))) return rc;
     std::vector<int32_t> rowk(2 * (size_t)n);
     for (int i = 0; i < n; ++i) rowk[2 * i] = rowk[2 * i + 1] = b.walks[i].K;
     if ((rc = upload(ctx, pl->d_rowk, rowk))) return rc;
@@ -1031,6 +1209,7 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
   g.inv_ld = (uint64_t)ctx->kinv;
   g.limbs = pl->d_limbs.as<uint32_t>();
   g.neg = pl->d_neg.as<uint8_t>();
+  if ((rc = setup_crt(ctx, pl))) return rc;
 
   if (pl->cfg.use_graph) {
     HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
@@ -1131,8 +1310,10 @@ void drx_ctx_destroy(drx_ctx* ctx) {
   if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
   if (ctx->stream3) (void)hipStreamSynchronize(ctx->stream3);
   for (DBuf* d : {&ctx->d_prime_p, &ctx->d_prime_pinv, &ctx->d_prime_e, &ctx->d_prime_u, &ctx->d_invs, &ctx->d_inv, &ctx->d_code,
-                  &ctx->d_cres, &ctx->d_cflt, &ctx->d_progs, &ctx->d_tmp0, &ctx->d_tmp1, &ctx->d_tmp2, &ctx->d_tmp3})
+                  &ctx->d_cres, &ctx->d_cflt, &ctx->d_progs, &ctx->d_tmp0, &ctx->d_tmp1, &ctx->d_tmp2, &ctx->d_tmp3,
+                  &ctx->d_crt_rows, &ctx->d_crt_a, &ctx->d_crt_c, &ctx->d_crt_k})
     d->release();
+  retire_crt_tables(ctx);
   for (void* p : ctx->retired) (void)hipFree(p);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
@@ -1175,7 +1356,10 @@ int drx_set_primes(drx_ctx* ctx, const int64_t* primes, int K) {
   HIPCHK(ctx, hipSetDevice(ctx->device));
   // an extension keeps every earlier plan valid; any other change retires them
   const bool extends = pr.size() >= ctx->primes.size() && std::equal(ctx->primes.begin(), ctx->primes.end(), pr.begin());
-  if (!extends) ++ctx->epoch;
+  if (!extends) {
+    ++ctx->epoch;
+    retire_crt_tables(ctx);
+  }
   HIPCHK(ctx, ctx->d_prime_p.ensure_keep(K * 8, ctx->retired));
   HIPCHK(ctx, ctx->d_prime_pinv.ensure_keep(K * 8, ctx->retired));
   HIPCHK(ctx, ctx->d_prime_e.ensure_keep(K * 4, ctx->retired));
@@ -1698,13 +1882,52 @@ int drx_crt(drx_ctx* ctx, const int64_t* residues, int n, int K, int stride, uin
       const int64_t r = residues[(size_t)i * stride + k];
       if (r < 0 || r >= (int64_t)ctx->primes[k]) return fail(ctx, DRX_EINVAL, "residue row %d prime %d not canonical", i, k);
     }
-  int rc = ensure_inv_table(ctx, K);
-  if (rc) return rc;
   const size_t rows = (size_t)n * stride;
   HIPCHK(ctx, ctx->d_tmp0.ensure(rows * 8));
   HIPCHK(ctx, ctx->d_tmp2.ensure(rows * 4));
   HIPCHK(ctx, ctx->d_tmp3.ensure((size_t)n));
   HIPCHK(ctx, hipMemcpyAsync(ctx->d_tmp0.p, residues, rows * 8, hipMemcpyHostToDevice, ctx->stream));
+  int rc;
+  if (K >= kCrtMatrixMinK) {  // matrix-core CRT
+    const CrtTable* t = nullptr;
+    if ((rc = ensure_crt_table(ctx, K, &t))) return rc;
+    const int npad = (n + 127) / 128 * 128;
+    std::vector<int32_t> ids(n);
+    std::iota(ids.begin(), ids.end(), 0);
+    HIPCHK(ctx, ctx->d_crt_rows.ensure((size_t)n * 4));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_crt_rows.p, ids.data(), (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, ctx->d_crt_a.ensure((size_t)4 * npad * t->Kp));
+    HIPCHK(ctx, ctx->d_crt_c.ensure((size_t)4 * t->Tp * npad * 4));
+    HIPCHK(ctx, ctx->d_crt_k.ensure((size_t)npad * 4));
+    CrtArgs c{};
+    c.res = ctx->d_tmp0.as<int64_t>();
+    c.res_stride = (uint64_t)stride;
+    c.rows = ctx->d_crt_rows.as<int32_t>();
+    c.n = n;
+    c.npad = npad;
+    c.K = K;
+    c.Kp = t->Kp;
+    c.T = t->T;
+    c.Tp = t->Tp;
+    c.L = t->L;
+    c.prime_p = ctx->d_prime_p.as<double>();
+    c.prime_pinv = ctx->d_prime_pinv.as<double>();
+    c.w = t->d_w.as<double>();
+    c.bt = t->d_bt.as<int8_t>();
+    c.m = t->d_m.as<uint32_t>();
+    c.h = t->d_h.as<uint32_t>();
+    c.A = ctx->d_crt_a.as<int8_t>();
+    c.C = ctx->d_crt_c.as<int32_t>();
+    c.kq = ctx->d_crt_k.as<int32_t>();
+    c.limbs = ctx->d_tmp2.as<uint32_t>();
+    c.neg = ctx->d_tmp3.as<uint8_t>();
+    HIPCHK(ctx, launch_crt_matrix(c, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(limbs, ctx->d_tmp2.p, rows * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(neg, ctx->d_tmp3.p, (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return DRX_OK;
+  }
+  if ((rc = ensure_inv_table(ctx, K))) return rc;
   GarnerArgs g{};
   g.res = ctx->d_tmp0.as<int64_t>();
   g.res_stride = (uint64_t)stride;

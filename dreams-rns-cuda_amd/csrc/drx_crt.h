@@ -1,0 +1,37 @@
+// Matrix-core CRT for wide prime sets (internal to libdreams_hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace drx {
+
+// Rows of at least this many primes take the matrix-core CRT; fewer take
+// the one-wavefront Garner kernel.
+constexpr int kCrtMatrixMinK = 257;
+
+struct CrtArgs {
+  const int64_t* res;      // canonical residue rows, stride res_stride
+  uint64_t res_stride;
+  const int32_t* rows;     // [n] row index of compact row j (in res, limbs and neg)
+  int n, npad;             // rows, rows padded to 128
+  int K, Kp;               // primes, padded to 16 (the reduction)
+  int T, Tp;               // base-256 digit columns of the M_i, padded to 128
+  int L;                   // 32-bit limbs of M
+  const double* prime_p;
+  const double* prime_pinv;
+  const double* w;         // [K] (M / p_i)^-1 mod p_i
+  const int8_t* bt;        // [Tp][Kp] balanced base-256 digits of M / p_i (column i)
+  const uint32_t* m;       // [L + 2] limbs of M (zeros past L)
+  const uint32_t* h;       // [L + 2] limbs of (M - 1) / 2
+  int8_t* A;               // [4][npad][Kp] digit planes of the centred c_i
+  int32_t* C;              // [4][Tp][npad] plane products
+  int32_t* kq;             // [npad] rint(sum c_i / p_i)
+  uint32_t* limbs;         // out: magnitude limbs, row stride res_stride (zeros past L + 2)
+  uint8_t* neg;            // out: 1 if the centred value is negative
+};
+
+// Digits, GEMM and fold on stream s (no allocation).
+hipError_t launch_crt_matrix(const CrtArgs& g, hipStream_t s);
+
+}  // namespace drx

@@ -1,0 +1,182 @@
+// Matrix-core CRT for wide prime sets (internal to libdreams_hip).
+//
+// For residues r_i of x modulo the first K primes (M = prod p_i,
+// M_i = M / p_i, w_i = M_i^-1 mod p_i) the centred x is
+//   S - k M,  S = sum_i c_i M_i,  c_i = r_i w_i mod p_i centred into
+//   (-p_i/2, p_i/2),  k = rint(sum_i c_i / p_i),
+// up to one multiple of M that an exact comparison with (M - 1) / 2 settles.
+// S is an integer matrix product: the c_i of a row in four balanced base-256
+// digits (planes b) times the M_i in balanced base-256 digits (columns t),
+// int8 x int8 -> int32 on the MFMA units (|sum| < K 2^14 < 2^26), and
+// S = sum_{b,t} C[b][t] 256^(b + t), folded into 32-bit limbs by one lane per
+// row. Work per row: 4 K T multiply-adds with T ~ 4 * 31 K / 32, against
+// Garner's K^2 / 2 dependent modular steps; the matrix units make it the
+// faster one for wide K.
+#include "drx_crt.h"
+
+#include "drx_device.h"
+
+namespace drx {
+
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+// One block per (padded) row: the row's digit planes A[b][j][0..Kp) and
+// k_j = rint(sum c_i / p_i). Rows past n get zero planes (the GEMM reads the
+// whole padded tile).
+__global__ void __launch_bounds__(256) k_crt_digits(const CrtArgs g) {
+  __shared__ double red_buf[256];
+  const int j = blockIdx.x, tid = threadIdx.x;
+  const size_t plane = (size_t)g.npad * g.Kp;
+  int8_t* A = g.A + (size_t)j * g.Kp;
+  if (j >= g.n) {  // uniform over the block
+    for (int i = tid; i < g.Kp; i += blockDim.x)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) A[b * plane + i] = 0;
+    return;
+  }
+  const int64_t* r = g.res + (size_t)g.rows[j] * g.res_stride;
+  double f = 0.0;
+  for (int i = tid; i < g.Kp; i += blockDim.x) {
+    int c = 0;
+    if (i < g.K) {
+      const double p = g.prime_p[i], pinv = g.prime_pinv[i];
+      double x = canon(mm((double)r[i], g.w[i], p, pinv), p, pinv);
+      if (x > 0.5 * (p - 1.0)) x -= p;  // centred: |x| < 2^30
+      c = (int)x;
+      f = __builtin_fma(x, pinv, f);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int d = (int)(int8_t)(uint8_t)(c & 255);  // balanced digit in [-128, 127]
+      A[b * plane + i] = (int8_t)d;
+      c = (c - d) >> 8;  // exact: c - d is a multiple of 256
+    }
+  }
+  red_buf[tid] = f;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (tid < s) red_buf[tid] += red_buf[tid + s];
+    __syncthreads();
+  }
+  if (tid == 0) g.kq[j] = (int32_t)__builtin_rint(red_buf[0]);
+}
+
+// C[b][t][j] = sum_i A[b][j][i] BT[t][i]: a block of 4 wavefronts computes
+// a 128 x 128 tile (each wave 64 rows x 64 columns, 2 x 2 MFMA 32x32x16 i8
+// tiles); rows, columns and the reduction are padded, so there are no edges.
+// MFMA operands: lane l supplies row (or column) l % 32, reduction elements
+// 8 (l / 32) .. 8 (l / 32) + 7; accumulator element e of lane l is row
+// 8 (e / 4) + 4 (l / 32) + e % 4, column l % 32.
+__global__ void __launch_bounds__(256) k_crt_gemm(const CrtArgs g) {
+  const int b = blockIdx.z, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int row0 = blockIdx.y * 128 + (wave >> 1) * 64;
+  const int col0 = blockIdx.x * 128 + (wave & 1) * 64;
+  const int8_t* A = g.A + (size_t)b * g.npad * g.Kp;
+  const int kl = (lane >> 5) * 8, rl = lane & 31;
+  const int8_t* a0p = A + (size_t)(row0 + rl) * g.Kp + kl;
+  const int8_t* a1p = A + (size_t)(row0 + 32 + rl) * g.Kp + kl;
+  const int8_t* b0p = g.bt + (size_t)(col0 + rl) * g.Kp + kl;
+  const int8_t* b1p = g.bt + (size_t)(col0 + 32 + rl) * g.Kp + kl;
+  v16i c00 = {0}, c01 = {0}, c10 = {0}, c11 = {0};
+  for (int k0 = 0; k0 < g.Kp; k0 += 16) {
+    const long a0 = *reinterpret_cast<const long*>(a0p + k0);
+    const long a1 = *reinterpret_cast<const long*>(a1p + k0);
+    const long b0 = *reinterpret_cast<const long*>(b0p + k0);
+    const long b1 = *reinterpret_cast<const long*>(b1p + k0);
+    c00 = __builtin_amdgcn_mfma_i32_32x32x16_i8(a0, b0, c00, 0, 0, 0);
+    c01 = __builtin_amdgcn_mfma_i32_32x32x16_i8(a0, b1, c01, 0, 0, 0);
+    c10 = __builtin_amdgcn_mfma_i32_32x32x16_i8(a1, b0, c10, 0, 0, 0);
+    c11 = __builtin_amdgcn_mfma_i32_32x32x16_i8(a1, b1, c11, 0, 0, 0);
+  }
+  int32_t* C = g.C + (size_t)b * g.Tp * g.npad;
+  const int rq = (lane >> 5) * 4;
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    const int r = (e >> 2) * 8 + rq + (e & 3);
+    C[(size_t)(col0 + rl) * g.npad + row0 + r] = c00[e];
+    C[(size_t)(col0 + 32 + rl) * g.npad + row0 + r] = c01[e];
+    C[(size_t)(col0 + rl) * g.npad + row0 + 32 + r] = c10[e];
+    C[(size_t)(col0 + 32 + rl) * g.npad + row0 + 32 + r] = c11[e];
+  }
+}
+
+// One lane per row: S - k M into NL = L + 2 two's-complement limbs, then at
+// most one +-M so that |x| <= (M - 1) / 2, then the magnitude limbs (zeros
+// up to the row stride) and the sign.
+__global__ void __launch_bounds__(256) k_crt_fold(const CrtArgs g) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= g.n) return;
+  const int NL = g.L + 2;
+  const int64_t kq = g.kq[j];
+  uint32_t* out = g.limbs + (size_t)g.rows[j] * g.res_stride;
+  const size_t pl = (size_t)g.Tp * g.npad;
+  int64_t carry = 0;
+  for (int l = 0; l < NL; ++l) {
+    int64_t v = carry;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int u = 4 * l + q;
+      int64_t acc = 0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int t = u - b;
+        if (t >= 0 && t < g.T) acc += g.C[b * pl + (size_t)t * g.npad + j];
+      }
+      v += acc * ((int64_t)1 << (8 * q));  // |v| < 2^53
+    }
+    v -= kq * (int64_t)g.m[l];
+    const uint32_t limb = (uint32_t)v;
+    carry = (v - (int64_t)limb) >> 32;
+    out[l] = limb;
+  }
+  // x = sum out[l] 2^(32 l) + sw 2^(32 NL) with the sign word sw in {-1, 0};
+  // |x| < 3M/2 (the rounded k is off by at most one)
+  int64_t sw = carry;
+  int fix = 0;  // -1: x -= M, +1: x += M
+  if (sw == 0) {
+    int cmp = 0;  // x against H = (M - 1) / 2, from the top limb
+    for (int l = NL - 1; l >= 0 && cmp == 0; --l) {
+      const uint32_t a = out[l], h = g.h[l];
+      cmp = a > h ? 1 : (a < h ? -1 : 0);
+    }
+    if (cmp > 0) fix = -1;
+  } else {
+    int64_t c = 0;  // x + H: its sign word is sw + c
+    for (int l = 0; l < NL; ++l) c = ((int64_t)out[l] + (int64_t)g.h[l] + c) >> 32;
+    if (sw + c < 0) fix = 1;
+  }
+  if (fix != 0) {
+    int64_t c = 0;
+    for (int l = 0; l < NL; ++l) {
+      const int64_t s = (int64_t)out[l] + (int64_t)fix * (int64_t)g.m[l] + c;
+      const uint32_t limb = (uint32_t)s;
+      c = (s - (int64_t)limb) >> 32;
+      out[l] = limb;
+    }
+    sw += c;
+  }
+  const bool neg = sw < 0;
+  if (neg) {  // magnitude: two's-complement negation
+    uint64_t c = 1;
+    for (int l = 0; l < NL; ++l) {
+      const uint64_t s = (uint64_t)(uint32_t)~out[l] + c;
+      out[l] = (uint32_t)s;
+      c = s >> 32;
+    }
+  }
+  for (uint64_t l = NL; l < g.res_stride; ++l) out[l] = 0u;
+  g.neg[g.rows[j]] = (uint8_t)(neg ? 1 : 0);
+}
+
+hipError_t launch_crt_matrix(const CrtArgs& g, hipStream_t s) {
+  if (g.n <= 0) return hipSuccess;
+  if (g.npad % 128 || g.Tp % 128 || g.Kp % 16 || g.npad < g.n || g.Kp < g.K || g.Tp < g.T ||
+      (size_t)g.L + 2 > g.res_stride)
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_crt_digits, dim3(g.npad), dim3(256), 0, s, g);
+  hipLaunchKernelGGL(k_crt_gemm, dim3(g.Tp / 128, g.npad / 128, 4), dim3(256), 0, s, g);
+  hipLaunchKernelGGL(k_crt_fold, dim3((g.n + 255) / 256), dim3(256), 0, s, g);
+  return hipGetLastError();
+}
+
+}  // namespace drx

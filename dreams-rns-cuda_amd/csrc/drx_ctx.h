@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -70,6 +71,12 @@ struct Timing {
   int walk_launches = 0;
 };
 
+// Matrix-core CRT tables of the first K primes (drx_crt.h), built once per K.
+struct CrtTable {
+  int K = 0, Kp = 0, T = 0, Tp = 0, L = 0;
+  DBuf d_bt, d_w, d_m, d_h;
+};
+
 }  // namespace drx
 
 struct drx_ctx {
@@ -93,6 +100,9 @@ struct drx_ctx {
   bool progs_dirty = true;
   // scratch for standalone CRT / top-k
   drx::DBuf d_tmp0, d_tmp1, d_tmp2, d_tmp3;
+  // matrix-core CRT: tables per K, scratch of the standalone CRT
+  std::vector<std::unique_ptr<drx::CrtTable>> crt_tables;
+  drx::DBuf d_crt_rows, d_crt_a, d_crt_c, d_crt_k;
   drx::Timing timing;
   hipEvent_t ev[24] = {};
   // Bumped when the prime table changes other than by extension or the
@@ -170,6 +180,8 @@ bool is_prime_u32(uint32_t n);
 double now_s();
 int upload_programs(drx_ctx* ctx);
 int ensure_inv_table(drx_ctx* ctx, int k);
+// The matrix-core CRT table of the first K primes (built on first use).
+int ensure_crt_table(drx_ctx* ctx, int K, const CrtTable** out);
 
 inline uint32_t mod_i64(int64_t c, uint32_t p) {
   int64_t r = c % (int64_t)p;

@@ -56,8 +56,9 @@ constexpr int kMfBlock = 256;
 constexpr int kMfRange = kMfBlock;
 
 struct GarnerArgs {
-  const int64_t* res;     // [n] rows of canonical residues, row stride res_stride
+  const int64_t* res;     // rows of canonical residues, row stride res_stride
   uint64_t res_stride;
+  const int32_t* rows;    // optional [n] row indices (else rows 0 .. n)
   int n;
   int K;                  // primes per row unless row_k is given
   const int32_t* row_k;   // optional per-row prime count

@@ -1164,8 +1164,9 @@ __global__ void k_finalize(const KArgs a, int full) {
 template <int J>
 __global__ void __launch_bounds__(256) k_garner_wave(const GarnerArgs g) {
   const int lane = threadIdx.x & 63;
-  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (row >= g.n) return;  // the whole wavefront leaves together
+  const int idx = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (idx >= g.n) return;  // the whole wavefront leaves together
+  const int row = g.rows ? g.rows[idx] : idx;
   const int K = g.row_k ? g.row_k[row] : g.K;
   const int64_t* r = g.res + (uint64_t)row * g.res_stride;
   double t[J], pd[J], pv[J];
