@@ -350,7 +350,7 @@ int ensure_crt_table(drx_ctx* ctx, int K, const CrtTable** out) {
   std::unique_ptr<CrtTable> t(new CrtTable());
   t->K = K;
   t->L = (int)M.size();
-  t->Kp = (K + 15) / 16 * 16;
+  t->Kp = (K + 31) / 32 * 32;
   t->T = 4 * t->L + 1;
   t->Tp = (t->T + 127) / 128 * 128;
   const int L = t->L, Kp = t->Kp;
@@ -581,7 +581,7 @@ int setup_crt(drx_ctx* ctx, drx_plan* pl) {
     if ((rc = upload(ctx, grp->d_rows, e.second))) return rc;
     HIPCHK(ctx, grp->d_a.ensure((size_t)4 * npad * t->Kp));
     HIPCHK(ctx, grp->d_c.ensure((size_t)4 * t->Tp * npad * 4));
-    HIPCHK(ctx, grp->d_k.ensure((size_t)npad * 4));
+    HIPCHK(ctx, grp->d_k.ensure((size_t)npad * 12));
     CrtArgs& c = grp->args;
     c.res = pl->d_rout.as<int64_t>();
     c.res_stride = (uint64_t)pl->ld;
@@ -602,6 +602,7 @@ int setup_crt(drx_ctx* ctx, drx_plan* pl) {
     c.A = grp->d_a.as<int8_t>();
     c.C = grp->d_c.as<int32_t>();
     c.kq = grp->d_k.as<int32_t>();
+    c.fd = reinterpret_cast<double*>(grp->d_k.as<char>() + (size_t)npad * 4);
     c.limbs = pl->d_limbs.as<uint32_t>();
     c.neg = pl->d_neg.as<uint8_t>();
     pl->crt_groups.push_back(std::move(grp));
@@ -1898,7 +1899,7 @@ int drx_crt(drx_ctx* ctx, const int64_t* residues, int n, int K, int stride, uin
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_crt_rows.p, ids.data(), (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, ctx->d_crt_a.ensure((size_t)4 * npad * t->Kp));
     HIPCHK(ctx, ctx->d_crt_c.ensure((size_t)4 * t->Tp * npad * 4));
-    HIPCHK(ctx, ctx->d_crt_k.ensure((size_t)npad * 4));
+    HIPCHK(ctx, ctx->d_crt_k.ensure((size_t)npad * 12));
     CrtArgs c{};
     c.res = ctx->d_tmp0.as<int64_t>();
     c.res_stride = (uint64_t)stride;
@@ -1919,6 +1920,7 @@ int drx_crt(drx_ctx* ctx, const int64_t* residues, int n, int K, int stride, uin
     c.A = ctx->d_crt_a.as<int8_t>();
     c.C = ctx->d_crt_c.as<int32_t>();
     c.kq = ctx->d_crt_k.as<int32_t>();
+    c.fd = reinterpret_cast<double*>(ctx->d_crt_k.as<char>() + (size_t)npad * 4);
     c.limbs = ctx->d_tmp2.as<uint32_t>();
     c.neg = ctx->d_tmp3.as<uint8_t>();
     HIPCHK(ctx, launch_crt_matrix(c, ctx->stream));

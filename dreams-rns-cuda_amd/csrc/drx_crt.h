@@ -15,7 +15,7 @@ struct CrtArgs {
   uint64_t res_stride;
   const int32_t* rows;     // [n] row index of compact row j (in res, limbs and neg)
   int n, npad;             // rows, rows padded to 128
-  int K, Kp;               // primes, padded to 16 (the reduction)
+  int K, Kp;               // primes, padded to 32 (the reduction)
   int T, Tp;               // base-256 digit columns of the M_i, padded to 128
   int L;                   // 32-bit limbs of M
   const double* prime_p;
@@ -27,6 +27,7 @@ struct CrtArgs {
   int8_t* A;               // [4][npad][Kp] digit planes of the centred c_i
   int32_t* C;              // [4][Tp][npad] plane products
   int32_t* kq;             // [npad] rint(sum c_i / p_i)
+  double* fd;              // [npad] sum c_i / p_i - kq (the FP64 estimate of x / M)
   uint32_t* limbs;         // out: magnitude limbs, row stride res_stride (zeros past L + 2)
   uint8_t* neg;            // out: 1 if the centred value is negative
 };

@@ -19,6 +19,7 @@
 namespace drx {
 
 typedef int v16i __attribute__((ext_vector_type(16)));
+typedef int v4i __attribute__((ext_vector_type(4)));
 
 // One block per (padded) row: the row's digit planes A[b][j][0..Kp) and
 // k_j = rint(sum c_i / p_i). Rows past n get zero planes (the GEMM reads the
@@ -58,35 +59,40 @@ __global__ void __launch_bounds__(256) k_crt_digits(const CrtArgs g) {
     if (tid < s) red_buf[tid] += red_buf[tid + s];
     __syncthreads();
   }
-  if (tid == 0) g.kq[j] = (int32_t)__builtin_rint(red_buf[0]);
+  if (tid == 0) {
+    const double k = __builtin_rint(red_buf[0]);
+    g.kq[j] = (int32_t)k;
+    g.fd[j] = red_buf[0] - k;  // x / M up to the summation error (< K^2 2^-53)
+  }
 }
 
 // C[b][t][j] = sum_i A[b][j][i] BT[t][i]: a block of 4 wavefronts computes
-// a 128 x 128 tile (each wave 64 rows x 64 columns, 2 x 2 MFMA 32x32x16 i8
+// a 128 x 128 tile (each wave 64 rows x 64 columns, 2 x 2 MFMA 32x32x32 i8
 // tiles); rows, columns and the reduction are padded, so there are no edges.
-// MFMA operands: lane l supplies row (or column) l % 32, reduction elements
-// 8 (l / 32) .. 8 (l / 32) + 7; accumulator element e of lane l is row
+// MFMA operands: lane l supplies row (or column) l % 32 and 16 reduction
+// elements, the same 16 for the A and the B operand (here 16 (l / 32) ..
+// 16 (l / 32) + 15 of the 32); accumulator element e of lane l is row
 // 8 (e / 4) + 4 (l / 32) + e % 4, column l % 32.
 __global__ void __launch_bounds__(256) k_crt_gemm(const CrtArgs g) {
   const int b = blockIdx.z, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int row0 = blockIdx.y * 128 + (wave >> 1) * 64;
   const int col0 = blockIdx.x * 128 + (wave & 1) * 64;
   const int8_t* A = g.A + (size_t)b * g.npad * g.Kp;
-  const int kl = (lane >> 5) * 8, rl = lane & 31;
+  const int kl = (lane >> 5) * 16, rl = lane & 31;
   const int8_t* a0p = A + (size_t)(row0 + rl) * g.Kp + kl;
   const int8_t* a1p = A + (size_t)(row0 + 32 + rl) * g.Kp + kl;
   const int8_t* b0p = g.bt + (size_t)(col0 + rl) * g.Kp + kl;
   const int8_t* b1p = g.bt + (size_t)(col0 + 32 + rl) * g.Kp + kl;
   v16i c00 = {0}, c01 = {0}, c10 = {0}, c11 = {0};
-  for (int k0 = 0; k0 < g.Kp; k0 += 16) {
-    const long a0 = *reinterpret_cast<const long*>(a0p + k0);
-    const long a1 = *reinterpret_cast<const long*>(a1p + k0);
-    const long b0 = *reinterpret_cast<const long*>(b0p + k0);
-    const long b1 = *reinterpret_cast<const long*>(b1p + k0);
-    c00 = __builtin_amdgcn_mfma_i32_32x32x16_i8(a0, b0, c00, 0, 0, 0);
-    c01 = __builtin_amdgcn_mfma_i32_32x32x16_i8(a0, b1, c01, 0, 0, 0);
-    c10 = __builtin_amdgcn_mfma_i32_32x32x16_i8(a1, b0, c10, 0, 0, 0);
-    c11 = __builtin_amdgcn_mfma_i32_32x32x16_i8(a1, b1, c11, 0, 0, 0);
+  for (int k0 = 0; k0 < g.Kp; k0 += 32) {
+    const v4i a0 = *reinterpret_cast<const v4i*>(a0p + k0);
+    const v4i a1 = *reinterpret_cast<const v4i*>(a1p + k0);
+    const v4i b0 = *reinterpret_cast<const v4i*>(b0p + k0);
+    const v4i b1 = *reinterpret_cast<const v4i*>(b1p + k0);
+    c00 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b0, c00, 0, 0, 0);
+    c01 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b1, c01, 0, 0, 0);
+    c10 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b0, c10, 0, 0, 0);
+    c11 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b1, c11, 0, 0, 0);
   }
   int32_t* C = g.C + (size_t)b * g.Tp * g.npad;
   const int rq = (lane >> 5) * 4;
@@ -100,68 +106,93 @@ __global__ void __launch_bounds__(256) k_crt_gemm(const CrtArgs g) {
   }
 }
 
-// One lane per row: S - k M into NL = L + 2 two's-complement limbs, then at
-// most one +-M so that |x| <= (M - 1) / 2, then the magnitude limbs (zeros
-// up to the row stride) and the sign.
+// s (S - k M) into NL = L + 2 limbs of one row, four limbs per round so
+// that the 64 loads of a round are in flight together; returns the final
+// carry (the sign word: 0 or -1).
+__device__ __forceinline__ int64_t fold_limbs(const CrtArgs& g, int j, int64_t kq, int64_t s, uint32_t* out,
+                                              int NL) {
+  const size_t pl = (size_t)g.Tp * g.npad;
+  int64_t carry = 0;
+  for (int l0 = 0; l0 < NL; l0 += 4) {
+    int32_t cv[4][4][4];
+#pragma unroll
+    for (int lq = 0; lq < 4; ++lq)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const int t = 4 * (l0 + lq) + q - b;
+          cv[lq][q][b] = (t >= 0 && t < g.T) ? g.C[b * pl + (size_t)t * g.npad + j] : 0;
+        }
+#pragma unroll
+    for (int lq = 0; lq < 4; ++lq) {
+      const int l = l0 + lq;
+      if (l >= NL) break;
+      int64_t v = 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        v += (int64_t)(cv[lq][q][0] + cv[lq][q][1] + cv[lq][q][2] + cv[lq][q][3]) * ((int64_t)1 << (8 * q));
+      v = s * (v - kq * (int64_t)g.m[l]) + carry;  // |v| < 2^53
+      const uint32_t limb = (uint32_t)v;
+      carry = (v - (int64_t)limb) >> 32;
+      out[l] = limb;
+    }
+  }
+  return carry;
+}
+
+// One lane per row. Away from the edges of (-1/2, 1/2), the FP64 estimate of
+// x / M fixes the sign of x = S - k M and |x| < (M - 1) / 2, so one pass of
+// -(S - k M) or S - k M gives the magnitude (checked: its final carry is
+// 0). Otherwise (or if the check fails) the exact path: S - k M in two's
+// complement, at most one +-M against (M - 1) / 2, then the magnitude.
 __global__ void __launch_bounds__(256) k_crt_fold(const CrtArgs g) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= g.n) return;
   const int NL = g.L + 2;
   const int64_t kq = g.kq[j];
+  const double d = g.fd[j];
   uint32_t* out = g.limbs + (size_t)g.rows[j] * g.res_stride;
-  const size_t pl = (size_t)g.Tp * g.npad;
-  int64_t carry = 0;
-  for (int l = 0; l < NL; ++l) {
-    int64_t v = carry;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int u = 4 * l + q;
-      int64_t acc = 0;
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const int t = u - b;
-        if (t >= 0 && t < g.T) acc += g.C[b * pl + (size_t)t * g.npad + j];
-      }
-      v += acc * ((int64_t)1 << (8 * q));  // |v| < 2^53
-    }
-    v -= kq * (int64_t)g.m[l];
-    const uint32_t limb = (uint32_t)v;
-    carry = (v - (int64_t)limb) >> 32;
-    out[l] = limb;
-  }
-  // x = sum out[l] 2^(32 l) + sw 2^(32 NL) with the sign word sw in {-1, 0};
-  // |x| < 3M/2 (the rounded k is off by at most one)
-  int64_t sw = carry;
-  int fix = 0;  // -1: x -= M, +1: x += M
-  if (sw == 0) {
-    int cmp = 0;  // x against H = (M - 1) / 2, from the top limb
-    for (int l = NL - 1; l >= 0 && cmp == 0; --l) {
-      const uint32_t a = out[l], h = g.h[l];
-      cmp = a > h ? 1 : (a < h ? -1 : 0);
-    }
-    if (cmp > 0) fix = -1;
+  const bool fast = __builtin_fabs(d) > 1e-6 && __builtin_fabs(d) < 0.5 - 1e-6;
+  const int64_t s = (fast && d < 0.0) ? -1 : 1;
+  int64_t sw = fold_limbs(g, j, kq, s, out, NL);
+  bool neg;
+  if (fast && sw == 0) {
+    neg = s < 0;
   } else {
-    int64_t c = 0;  // x + H: its sign word is sw + c
-    for (int l = 0; l < NL; ++l) c = ((int64_t)out[l] + (int64_t)g.h[l] + c) >> 32;
-    if (sw + c < 0) fix = 1;
-  }
-  if (fix != 0) {
-    int64_t c = 0;
-    for (int l = 0; l < NL; ++l) {
-      const int64_t s = (int64_t)out[l] + (int64_t)fix * (int64_t)g.m[l] + c;
-      const uint32_t limb = (uint32_t)s;
-      c = (s - (int64_t)limb) >> 32;
-      out[l] = limb;
+    if (s < 0) sw = fold_limbs(g, j, kq, 1, out, NL);
+    // x = sum out[l] 2^(32 l) + sw 2^(32 NL), sw in {-1, 0}; |x| < 3M/2
+    int fix = 0;  // -1: x -= M, +1: x += M
+    if (sw == 0) {
+      int cmp = 0;  // x against H = (M - 1) / 2, from the top limb
+      for (int l = NL - 1; l >= 0 && cmp == 0; --l) {
+        const uint32_t a = out[l], h = g.h[l];
+        cmp = a > h ? 1 : (a < h ? -1 : 0);
+      }
+      if (cmp > 0) fix = -1;
+    } else {
+      int64_t c = 0;  // x + H: its sign word is sw + c
+      for (int l = 0; l < NL; ++l) c = ((int64_t)out[l] + (int64_t)g.h[l] + c) >> 32;
+      if (sw + c < 0) fix = 1;
     }
-    sw += c;
-  }
-  const bool neg = sw < 0;
-  if (neg) {  // magnitude: two's-complement negation
-    uint64_t c = 1;
-    for (int l = 0; l < NL; ++l) {
-      const uint64_t s = (uint64_t)(uint32_t)~out[l] + c;
-      out[l] = (uint32_t)s;
-      c = s >> 32;
+    if (fix != 0) {
+      int64_t c = 0;
+      for (int l = 0; l < NL; ++l) {
+        const int64_t v = (int64_t)out[l] + (int64_t)fix * (int64_t)g.m[l] + c;
+        const uint32_t limb = (uint32_t)v;
+        c = (v - (int64_t)limb) >> 32;
+        out[l] = limb;
+      }
+      sw += c;
+    }
+    neg = sw < 0;
+    if (neg) {  // magnitude: two's-complement negation
+      uint64_t c = 1;
+      for (int l = 0; l < NL; ++l) {
+        const uint64_t v = (uint64_t)(uint32_t)~out[l] + c;
+        out[l] = (uint32_t)v;
+        c = v >> 32;
+      }
     }
   }
   for (uint64_t l = NL; l < g.res_stride; ++l) out[l] = 0u;
@@ -170,7 +201,7 @@ __global__ void __launch_bounds__(256) k_crt_fold(const CrtArgs g) {
 
 hipError_t launch_crt_matrix(const CrtArgs& g, hipStream_t s) {
   if (g.n <= 0) return hipSuccess;
-  if (g.npad % 128 || g.Tp % 128 || g.Kp % 16 || g.npad < g.n || g.Kp < g.K || g.Tp < g.T ||
+  if (g.npad % 128 || g.Tp % 128 || g.Kp % 32 || g.npad < g.n || g.Kp < g.K || g.Tp < g.T ||
       (size_t)g.L + 2 > g.res_stride)
     return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_crt_digits, dim3(g.npad), dim3(256), 0, s, g);
