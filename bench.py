@@ -39,6 +39,26 @@ def log(*a):
 
 
 # ------------------------------------------------------------- work model
+def timed_with_copy(plan, steps: int) -> list:
+    """Wall time (ms) of whole runs that also copy every output to pinned
+    host memory: the PCIe-inclusive rate, reported beside the HBM-resident
+    one."""
+    out = []
+    for _ in range(max(1, min(steps, 5))):
+        t = time.perf_counter()
+        plan.run()
+        out.append((time.perf_counter() - t) * 1e3)
+    log(f"with the copy back: {np.mean(out):.3f} ms per step")
+    return out
+
+
+def with_copy(rsteps: float, copy_ms: list) -> dict:
+    ms = float(np.mean(copy_ms))
+    return {"ms_per_step": ms, "value": rsteps / (ms * 1e-3), "unit": "residue-steps/s",
+            "what": "the same step plus the copy of every output (residues, CRT limbs, FP64 results, top-k) "
+                    "to pinned host memory"}
+
+
 def program_work(prog, vector: bool = False) -> float:
     """Integer-VALU ops per residue-step by the fixed convention
     W = 6*MM + 2*MA with MM = #MUL + #POW2 + 2*#POW3 + 56*#INV + m^3 and
@@ -288,11 +308,12 @@ def other_workload(args):
     step_ms, walk_ms = [], []
     for i in range(args.steps):
         t = time.perf_counter()
-        plan.run()
+        plan.run(fetch=False)  # the outputs stay in HBM
         step_ms.append((time.perf_counter() - t) * 1e3)
         tm = ctx.last_timing()
         walk_ms.append(tm[0])
         log(f"step {i}: {step_ms[-1]:.2f} ms (walk kernels {tm[0]:.3f} ms, device {tm[4]:.3f})")
+    copy_ms = timed_with_copy(plan, args.steps)
     post = None
     if args.workload == "sweep3f2":
         # the step after the walk: the sweep's hit selection over this batch
@@ -340,6 +361,7 @@ def other_workload(args):
         "config": {"workload": desc, "walks": len(specs), "top_k": top_k, "graph": bool(args.graph),
                    "threads": args.threads, "chunk_steps": args.chunk, "autotuned": tuned is not None},
         "latency_batch_ms": {"p50": float(np.percentile(step_ms, 50)), "p99": float(np.percentile(step_ms, 99))},
+        "with_copy_back": with_copy(rsteps, copy_ms),
         "roofline": {"bound": "valu", "achieved": achieved / 1e12 if achieved else None,
                      "peak": PEAK_LANE_OPS / 1e12, "unit": "TOP/s",
                      "frac": achieved / PEAK_LANE_OPS if achieved else None, "traffic": None,
@@ -424,12 +446,13 @@ def main():
     walk_ms, step_ms = [], []
     for i in range(args.steps):
         t = time.perf_counter()
-        plan.run()
+        plan.run(fetch=False)  # the outputs stay in HBM
         step_ms.append((time.perf_counter() - t) * 1e3)
         tm = ctx.last_timing()
         walk_ms.append(tm[0])
         log(f"step {i}: {step_ms[-1]:.2f} ms (walk kernel {tm[0]:.3f} ms, combine {tm[1]:.3f}, "
             f"shadow wait {tm[2]:.3f}, tail {tm[3]:.3f}, device {tm[4]:.3f})")
+    copy_ms = timed_with_copy(plan, args.steps)
     res = plan.result
     ms = float(np.mean(step_ms))
     value = rsteps / (ms * 1e-3)
@@ -500,6 +523,7 @@ def main():
                    "threads": args.threads or 256, "chunk_steps": args.chunk or 4096, "graph": bool(args.graph),
                    "autotuned": tuned is not None},
         "latency_batch_ms": {"p50": float(np.percentile(step_ms, 50)), "p99": float(np.percentile(step_ms, 99))},
+        "with_copy_back": with_copy(rsteps, copy_ms),
         "plain": {"value": plain, "sample_walks": n_plain, "x": value / plain},
         "served": served,
         "roofline": {"bound": "valu", "achieved": achieved / 1e12 if achieved else None,

@@ -1096,8 +1096,8 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
     g.range_end = (int)hr.size();
     pl->walk_order.push_back(i);
     rsteps += (double)w.depth * K;
-    mmt += (double)w.depth * K * hp.mm_ops;
-    mat += (double)w.depth * K * hp.ma_ops;
+    mmt += (double)w.depth * K * (rows_only ? hp.mm_ops_rows : hp.mm_ops);
+    mat += (double)w.depth * K * (rows_only ? hp.ma_ops_rows : hp.ma_ops);
   }
   // LDS per stage (max over its programs), checked against the device limit
   const size_t lds_limit = device_lds_limit(ctx);
@@ -1247,7 +1247,7 @@ int plan_fetch(drx_plan* pl) {
   return DRX_OK;
 }
 
-int plan_run_one(drx_plan* pl, bool timed) {
+int plan_run_one(drx_plan* pl, bool timed, bool fetch = true) {
   drx_ctx* ctx = pl->ctx;
   if (pl->d_pout.p && pl->b.full)
     HIPCHK(ctx, hipMemsetAsync(pl->d_pout.p, 0, (size_t)pl->n * 16 * pl->ld * 8, ctx->stream));
@@ -1259,7 +1259,7 @@ int plan_run_one(drx_plan* pl, bool timed) {
     int rc = launch_sequence(pl, timed);
     if (rc) return rc;
   }
-  return plan_fetch(pl);
+  return fetch ? plan_fetch(pl) : DRX_OK;
 }
 
 }  // namespace
@@ -1523,20 +1523,29 @@ int drx_program_upload_mode(drx_ctx* ctx, const uint32_t* ops, int n_ops, int m,
     // interpreted program runs only when the tables are seeded
     mmo = 0;
     mao = 0;
+    int tab = 0, mm1 = 0, ma1 = 0;  // table additions; product work per carried row
     for (int e = 0; e < m * m; ++e)
-      if (((pk.kinds >> (2 * e)) & 3u) == E_GEN) mao += (int)((pk.fd_deg >> (4 * e)) & 15u);
+      if (((pk.kinds >> (2 * e)) & 3u) == E_GEN) tab += (int)((pk.fd_deg >> (4 * e)) & 15u);
     for (int j = 0; j < m; ++j) {
       int terms = 0;
       for (int k = 0; k < m; ++k) {
         const uint32_t kd = (pk.kinds >> (2 * (k * m + j))) & 3u;
-        if (kd == E_GEN) mmo += m;
+        if (kd == E_GEN) mm1 += 1;
         if (kd != E_ZERO) ++terms;
       }
-      if (terms > 1) mao += m * (terms - 1);
+      if (terms > 1) ma1 += terms - 1;
     }
+    mmo = m * mm1;
+    mao = tab + m * ma1;
+    hp.mm_ops_rows = 2 * mm1;
+    hp.ma_ops_rows = tab + 2 * ma1;
   }
   hp.mm_ops = mmo;
   hp.ma_ops = mao;
+  if (!hp.fd) {
+    hp.mm_ops_rows = mmo;
+    hp.ma_ops_rows = mao;
+  }
   ctx->progs.push_back(std::move(hp));
   ctx->progs_dirty = true;
   *prog_id = (int)ctx->progs.size() - 1;
@@ -1676,7 +1685,8 @@ int drx_plan_create(drx_ctx* ctx, const drx_batch* b, const drx_launch_cfg* cfg,
   return DRX_OK;
 }
 
-int drx_plan_run(drx_plan* pl) {
+namespace {
+int plan_run(drx_plan* pl, bool fetch) {
   if (!pl) return DRX_EINVAL;
   drx_ctx* ctx = pl->ctx;
   std::lock_guard<std::mutex> lock(ctx->mu);
@@ -1716,7 +1726,7 @@ int drx_plan_run(drx_plan* pl) {
     ctx->timing = tm;
     return DRX_OK;
   }
-  int rc = plan_run_one(pl, true);
+  int rc = plan_run_one(pl, true, fetch);
   if (rc) return rc;
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   float a = 0, b2 = 0, c = 0, d = 0, tot = 0;
@@ -1734,6 +1744,23 @@ int drx_plan_run(drx_plan* pl) {
   tm.total_ms = tot;
   tm.walk_launches = (int)pl->stages.size();
   ctx->timing = tm;
+  return DRX_OK;
+}
+}  // namespace
+
+int drx_plan_run(drx_plan* pl) { return plan_run(pl, true); }
+
+int drx_plan_run_resident(drx_plan* pl) { return plan_run(pl, false); }
+
+int drx_plan_fetch(drx_plan* pl) {
+  if (!pl) return DRX_EINVAL;
+  drx_ctx* ctx = pl->ctx;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  if (!pl->plain.empty() || pl->cfg.mode == DRX_MODE_PLAIN) return DRX_OK;  // plain runs copy back per walk
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const int rc = plan_fetch(pl);
+  if (rc) return rc;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return DRX_OK;
 }
 

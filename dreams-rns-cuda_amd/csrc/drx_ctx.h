@@ -64,6 +64,8 @@ struct HostProgram {
   int fd = 0;  // runs on the forward-difference kernel
   // executed work per step, for the roofline: modular multiplies / adds
   int mm_ops = 0, ma_ops = 0;
+  // the same when only rows 0 and m - 1 of the product are carried
+  int mm_ops_rows = 0, ma_ops_rows = 0;
 };
 
 struct Timing {

@@ -84,10 +84,14 @@ int walk_blocks_per_cu(int m, int kind, int threads, size_t lds);
 // prime slice of a walk, a.tasks) of `threads` lanes, `kp` (power of two >=
 // every slice's prime count) per point group; threads % kp == 0.
 hipError_t launch_fd_seed(int m, const KArgs& a, int nseed, int kp, int threads, size_t lds, hipStream_t s);
-// Seed block for primes padded to kp: point groups of kp lanes, at most 512 lanes.
+// Seed block for primes padded to kp: point groups of kp lanes, at most 256
+// lanes (the kernel needs over 128 VGPRs, so a CU holds 8 of its waves: two
+// blocks of 256 lanes overlap each other's latency where one block of 512
+// would leave the CU idle between blocks; 3.3 ms against 5.5 ms for the
+// 4x4 seeds of 8256 walks at K = 1000).
 inline int fd_seed_threads(int m, int kp) {
   const int pts = fd_dmax(m) + 1;
-  int j = 512 / kp;
+  int j = 256 / kp;
   if (j < 1) j = 1;
   if (j > pts) j = pts;
   return j * kp;

@@ -248,14 +248,29 @@ __device__ __forceinline__ void load_rns_inputs(const KArgs& a, const DevWalk& w
 #pragma unroll
   for (int i = 0; i < kResStep; ++i)
     if ((uint32_t)i < nc && (uint32_t)i < pg.nconst) C[i] = a.cres[pg.cres_off + (uint64_t)i * pg.kcap + kk];
+  // axes in groups of 4: the 12 loads of a group are issued together (axes
+  // past dim re-read the last axis, so every address is valid), then used
+  constexpr int G = 4;
 #pragma unroll
-  for (int ax = 0; ax < kMaxDim; ++ax) {
-    if ((uint32_t)ax < pg.dim) {
+  for (int g0 = 0; g0 < kMaxDim; g0 += G) {
+    if ((uint32_t)g0 >= pg.dim) break;  // uniform
+    double c[G][3];
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      const uint32_t ax = min((uint32_t)(g0 + i), pg.dim - 1u);
       const double* q = a.axr + w.axr_off + (uint64_t)ax * 3 * K;
-      const double c0 = q[kk], c1 = q[K + kk], c2 = q[2 * K + kk];
-      const double x0 = red(c0 + mm((double)s0, c1, p, pinv), p, pinv);
-      C[kResStep - 1 - ax] = mm(x0, c2, p, pinv);
-      if (li) li[ax * li_ld + li_col] = mm(c1, c2, p, pinv);
+      c[i][0] = q[kk];
+      c[i][1] = q[K + kk];
+      c[i][2] = q[2 * K + kk];
+    }
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      const int ax = g0 + i;
+      if (ax < kMaxDim && (uint32_t)ax < pg.dim) {
+        const double x0 = red(c[i][0] + mm((double)s0, c[i][1], p, pinv), p, pinv);
+        C[kResStep - 1 - ax] = mm(x0, c[i][2], p, pinv);
+        if (li) li[ax * li_ld + li_col] = mm(c[i][1], c[i][2], p, pinv);
+      }
     }
   }
   C[kResStep] = (double)s0;
@@ -787,9 +802,9 @@ __global__ void __launch_bounds__(256) k_mf(const KArgs a) {
 // P_f <- P_f @ M_f(s) for every step in order (runner.py:388-405), each entry
 // a sequential FMA chain acc = a0*b0; acc = fma(ak, bk, acc); then
 // mx = max|P_f| (NaN propagates) and, if mx > 1e10, P_f /= mx and log_scale
-// grows by ln mx. G lanes per walk, lane (i, j) holding entry P_f[i][j]: rows
-// come from the other lanes of the group (DPP quad permutes for 2x2,
-// ds_bpermute otherwise) and the maximum is a butterfly over the group.
+// grows by ln mx. For 3x3 and 4x4, 16 lanes per walk, lane (i, j) holding
+// entry P_f[i][j]: rows come from the other lanes of the group (ds_bpermute)
+// and the maximum is a butterfly over the group; 2x2 walks take k_shadow2.
 // log_scale is carried as a mantissa product and a binary exponent, both
 // exact in step order, and ln is taken once when it is read.
 template <int CTRL>
@@ -799,24 +814,16 @@ __device__ __forceinline__ double dpp_f64(double x) {
   const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
   return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
-// quad_perm selectors (lane l of a quad reads lane sel[l])
-constexpr int kQuadRow0 = 0xA0;  // [0, 0, 2, 2]: P[i][0] of the lane's row
-constexpr int kQuadRow1 = 0xF5;  // [1, 1, 3, 3]: P[i][1]
-constexpr int kQuadX1 = 0xB1;    // [1, 0, 3, 2]: partner at xor 1
-constexpr int kQuadX2 = 0x4E;    // [2, 3, 0, 1]: partner at xor 2
+// quad_perm selector (lane l of a quad reads lane sel[l]): the partner at xor 1
+constexpr int kQuadX1 = 0xB1;  // [1, 0, 3, 2]
 
 template <int M>
 __device__ __forceinline__ double group_row(double v, int gbase, int i, int q) {
-  if (M == 2) return q == 0 ? dpp_f64<kQuadRow0>(v) : dpp_f64<kQuadRow1>(v);
   return __shfl(v, gbase + i * M + q, 64);
 }
 
 template <int M>
 __device__ __forceinline__ double group_max(double v) {
-  if (M == 2) {
-    v = fmax(v, dpp_f64<kQuadX1>(v));
-    return fmax(v, dpp_f64<kQuadX2>(v));
-  }
 #pragma unroll
   for (int d = 1; d < 16; d <<= 1) v = fmax(v, __shfl_xor(v, d, 64));
   return v;
@@ -824,10 +831,6 @@ __device__ __forceinline__ double group_max(double v) {
 
 template <int M>
 __device__ __forceinline__ int group_or(int v) {
-  if (M == 2) {
-    v |= __builtin_amdgcn_mov_dpp(v, kQuadX1, 0xF, 0xF, false);
-    return v | __builtin_amdgcn_mov_dpp(v, kQuadX2, 0xF, 0xF, false);
-  }
 #pragma unroll
   for (int d = 1; d < 16; d <<= 1) v |= __shfl_xor(v, d, 64);
   return v;
@@ -869,14 +872,8 @@ __device__ __forceinline__ void shadow_tile(double (&t)[kShadowTile][M], const d
   for (int u = 0; u < kShadowTile; ++u) {
     const int64_t sl = s0 + u < last ? s0 + u : last;
     const double* c = mf + (uint64_t)sl * M * M + (uint64_t)ej * M;
-    if (M == 2) {
-      const double2 v = *reinterpret_cast<const double2*>(c);
-      t[u][0] = v.x;
-      t[u][M - 1] = v.y;
-    } else {
 #pragma unroll
-      for (int q = 0; q < M; ++q) t[u][q] = c[q];
-    }
+    for (int q = 0; q < M; ++q) t[u][q] = c[q];
   }
 }
 
@@ -896,8 +893,9 @@ __device__ __forceinline__ void shadow_run_tile(ShadowLane& L, const double (&t)
 // from the state of the previous window (the initial state when w0 == 0).
 template <int M>
 __global__ void __launch_bounds__(256) k_shadow(const KArgs a, int nspan, int finalize, int64_t w0, int64_t w1) {
+  static_assert(M == 3 || M == 4, "2x2 walks take k_shadow2");
   __builtin_amdgcn_s_setprio(3);  // the chain's few waves first
-  constexpr int G = (M == 2) ? 4 : 16;
+  constexpr int G = 16;
   constexpr int T = kShadowTile;
   const int lg = threadIdx.x % G;
   const int gid = (int)((blockIdx.x * blockDim.x + threadIdx.x) / G);
@@ -961,6 +959,135 @@ __global__ void __launch_bounds__(256) k_shadow(const KArgs a, int nspan, int fi
     const double pf = __shfl(Pm, gbase + M - 1, 64), qf = __shfl(Pm, gbase + M * M - 1, 64);
     if (live && lg == 0) {
       const double ls = log(mant) + ex * 0.69314718055994530942;
+      double delta, logq;
+      if (!isfinite(pf) || !isfinite(qf) || __builtin_fabs(qf) < 1e-300) {
+        delta = -1e10;
+        logq = 0.0;
+      } else {
+        const double err = __builtin_fabs(pf / qf - w.target);
+        logq = ls + log(__builtin_fabs(qf));
+        if (err < 1e-300) delta = 100.0;
+        else if (logq <= 0.0) delta = -1e10;
+        else delta = -(1.0 + log(err) / logq);
+      }
+      double* o = a.fout + (uint64_t)sp.walk * 6;
+      o[0] = pf;
+      o[1] = qf;
+      o[2] = ls;
+      o[3] = delta;
+      o[4] = logq;
+    }
+  }
+}
+
+// 2x2 walks (every PCF) with one lane per row of P_f: lane (walk, i) holds
+// P_f[i][0], P_f[i][1] and reads the whole M_f(s), so the product needs no
+// exchange between lanes; only the maximum and the NaN test cross the lane
+// pair (one DPP each). The operations and their order per entry are those of
+// k_shadow (acc = P[i][0] * M[0][j]; acc = fma(P[i][1], M[1][j], acc); the
+// same maximum, test, division and log_scale mantissa), so the results are
+// bit for bit the same; the chain per step is about half as long.
+constexpr int kShadow2Tile = 16;  // M_f steps per register tile (4 doubles each)
+
+__device__ __forceinline__ void shadow2_tile(double4 (&t)[kShadow2Tile], const double* __restrict__ mf, int64_t s0,
+                                             int64_t last) {
+#pragma unroll
+  for (int u = 0; u < kShadow2Tile; ++u) {
+    const int64_t sl = s0 + u < last ? s0 + u : last;
+    const double2* c = reinterpret_cast<const double2*>(mf + (uint64_t)sl * 4);  // 16-byte aligned
+    const double2 lo = c[0], hi = c[1];  // column-major: M00, M10 | M01, M11
+    t[u] = make_double4(lo.x, lo.y, hi.x, hi.y);
+  }
+}
+
+template <bool MASK>
+__device__ __forceinline__ void shadow2_run_tile(double& x0, double& x1, double& mant, int& ex,
+                                                 const double4 (&t)[kShadow2Tile], int64_t s0, int64_t nsteps) {
+#pragma unroll
+  for (int u = 0; u < kShadow2Tile; ++u) {
+    const double4 m = t[u];
+    const double c0 = __builtin_fma(x1, m.y, x0 * m.x);
+    const double c1 = __builtin_fma(x1, m.w, x0 * m.z);
+    const double r = fmax(__builtin_fabs(c0), __builtin_fabs(c1));
+    const double mx = fmax(r, dpp_f64<kQuadX1>(r));
+    int nan = (c0 != c0 || c1 != c1) ? 1 : 0;
+    nan |= __builtin_amdgcn_mov_dpp(nan, kQuadX1, 0xF, 0xF, false);
+    const bool resc = nan == 0 && mx > 1e10;
+    const bool in = !MASK || s0 + u < nsteps;
+    const double y0 = resc ? c0 / mx : c0, y1 = resc ? c1 / mx : c1;
+    int e1 = 0;
+    const double m1 = __builtin_frexp(mx, &e1);
+    const bool take = resc && in;
+    x0 = in ? y0 : x0;
+    x1 = in ? y1 : x1;
+    mant *= take ? m1 : 1.0;
+    ex += take ? e1 : 0;
+  }
+}
+
+__global__ void __launch_bounds__(64) k_shadow2(const KArgs a, int nspan, int finalize, int64_t w0, int64_t w1) {
+  __builtin_amdgcn_s_setprio(3);  // the chain's few waves first
+  constexpr int T = kShadow2Tile;
+  const int row = threadIdx.x & 1;
+  const int gid = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 1);
+  const bool live = gid < nspan;
+  const DevSpan sp = a.spans[live ? gid : nspan - 1];
+  const DevWalk w = a.walks[sp.walk];
+  const double* st_in = (w0 == 0 ? a.fst_in : a.fst) + w.fst_off;
+  double x0 = st_in[2 * row], x1 = st_in[2 * row + 1];
+  double mant;
+  int ex;
+  {
+    int e0 = 0;
+    mant = __builtin_frexp(st_in[4], &e0);
+    ex = e0 + (int)st_in[5];
+  }
+  int64_t nsteps = 0;
+  const double* mf = a.mf;
+  if (sp.task_end > sp.task_begin) {
+    const DevTask t0 = a.tasks[sp.task_begin], t1 = a.tasks[sp.task_end - 1];
+    const int64_t total = t1.s_end - t0.s_begin;  // the walk's tasks are contiguous in the M_f pool
+    const int64_t end = total < w1 ? total : w1;
+    if (end > w0) {
+      mf = a.mf + t0.mf_off + (uint64_t)w0 * 4;
+      nsteps = end - w0;
+    }
+  }
+  const int64_t last = nsteps > 0 ? nsteps - 1 : 0;
+  double4 ta[T], tb[T];
+  shadow2_tile(ta, mf, 0, last);
+  int64_t s = 0;
+  for (; s + 2 * T <= nsteps; s += 2 * T) {
+    shadow2_tile(tb, mf, s + T, last);
+    shadow2_run_tile<false>(x0, x1, mant, ex, ta, s, nsteps);
+    shadow2_tile(ta, mf, s + 2 * T, last);
+    shadow2_run_tile<false>(x0, x1, mant, ex, tb, s + T, nsteps);
+    int e2 = 0;
+    mant = __builtin_frexp(mant, &e2);
+    ex += e2;
+  }
+  if (s < nsteps) {
+    shadow2_tile(tb, mf, s + T, last);
+    shadow2_run_tile<true>(x0, x1, mant, ex, ta, s, nsteps);
+    shadow2_run_tile<true>(x0, x1, mant, ex, tb, s + T, nsteps);
+  }
+  int e3 = 0;
+  mant = __builtin_frexp(mant, &e3);
+  ex += e3;
+  double* st = a.fst + w.fst_off;
+  if (live) {
+    st[2 * row] = x0;
+    st[2 * row + 1] = x1;
+    if (row == 0) {
+      st[4] = mant;
+      st[5] = (double)ex;
+    }
+  }
+  if (finalize) {
+    // delta_float (runner.py:86-100) on the row-0 lane: p = P_f[0][1], q = P_f[1][1]
+    const double pf = x1, qf = dpp_f64<kQuadX1>(x1);
+    if (live && row == 0) {
+      const double ls = log(mant) + (double)ex * 0.69314718055994530942;
       double delta, logq;
       if (!isfinite(pf) || !isfinite(qf) || __builtin_fabs(qf) < 1e-300) {
         delta = -1e10;
@@ -1484,10 +1611,14 @@ hipError_t launch_combine(int m, const KArgs& a, int nspan, hipStream_t s) {
 
 hipError_t launch_shadow(int m, const KArgs& a, int nspan, int finalize, int64_t w0, int64_t w1, hipStream_t s) {
   if (nspan <= 0) return hipSuccess;
-  const int g = (m == 2) ? 4 : 16;
+  if (m == 2) {  // one lane per row
+    const int nb2 = (int)(((int64_t)nspan * 2 + 63) / 64);
+    hipLaunchKernelGGL(k_shadow2, dim3(nb2), dim3(64), 0, s, a, nspan, finalize, w0, w1);
+    return hipGetLastError();
+  }
+  const int g = 16;
   const int tb = 256, nb = (int)(((int64_t)nspan * g + tb - 1) / tb);
-  if (m == 2) hipLaunchKernelGGL(k_shadow<2>, dim3(nb), dim3(tb), 0, s, a, nspan, finalize, w0, w1);
-  else if (m == 3) hipLaunchKernelGGL(k_shadow<3>, dim3(nb), dim3(tb), 0, s, a, nspan, finalize, w0, w1);
+  if (m == 3) hipLaunchKernelGGL(k_shadow<3>, dim3(nb), dim3(tb), 0, s, a, nspan, finalize, w0, w1);
   else if (m == 4) hipLaunchKernelGGL(k_shadow<4>, dim3(nb), dim3(tb), 0, s, a, nspan, finalize, w0, w1);
   else return hipErrorInvalidValue;
   return hipGetLastError();

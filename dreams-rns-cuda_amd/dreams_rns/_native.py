@@ -101,6 +101,8 @@ SIGNATURES = {
     "drx_walk_batch": (C.c_int, [C.c_void_p, C.POINTER(Batch), C.POINTER(LaunchCfg)]),
     "drx_plan_create": (C.c_int, [C.c_void_p, C.POINTER(Batch), C.POINTER(LaunchCfg), C.POINTER(C.c_void_p)]),
     "drx_plan_run": (C.c_int, [C.c_void_p]),
+    "drx_plan_run_resident": (C.c_int, [C.c_void_p]),
+    "drx_plan_fetch": (C.c_int, [C.c_void_p]),
     "drx_plan_stats": (C.c_int, [C.c_void_p, C.c_void_p]),
     "drx_plan_destroy": (None, [C.c_void_p]),
     "drx_crt": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),

@@ -330,8 +330,18 @@ class BatchPlan:
         self.n = n
         self.mode = mode
 
-    def run(self) -> BatchResult:
-        self.ctx._check(self.ctx.lib.drx_plan_run(self.h))
+    def run(self, fetch: bool = True) -> BatchResult:
+        """Run the batch. With ``fetch=False`` the outputs stay in device
+        memory (``self.result`` keeps the previous contents until ``fetch``)."""
+        if fetch:
+            self.ctx._check(self.ctx.lib.drx_plan_run(self.h))
+        else:
+            self.ctx._check(self.ctx.lib.drx_plan_run_resident(self.h))
+        return self.result
+
+    def fetch(self) -> BatchResult:
+        """Copy the last run's outputs into ``self.result``."""
+        self.ctx._check(self.ctx.lib.drx_plan_fetch(self.h))
         return self.result
 
     def stats(self) -> Dict[str, float]:

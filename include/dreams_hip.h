@@ -176,6 +176,16 @@ int drx_walk_batch(drx_ctx* ctx, const drx_batch* b, const drx_launch_cfg* cfg);
 typedef struct drx_plan drx_plan;
 int drx_plan_create(drx_ctx* ctx, const drx_batch* b, const drx_launch_cfg* cfg, drx_plan** out);
 int drx_plan_run(drx_plan* plan);
+/* drx_plan_run in two steps (no counterpart in the reference, whose walk
+ * functions return host arrays: these let a caller keep the results in HBM
+ * and time the device work apart from the copy over PCIe).
+ * drx_plan_run_resident runs the whole pipeline and synchronises, leaving
+ * every output in the plan's device buffers; drx_plan_fetch copies the last
+ * run's outputs into the drx_batch buffers and synchronises. A plan in
+ * DRX_MODE_PLAIN copies each walk back as it finishes (that is its point),
+ * so for it both behave as drx_plan_run does. */
+int drx_plan_run_resident(drx_plan* plan);
+int drx_plan_fetch(drx_plan* plan);
 /* Work of the plan: residue-steps (sum depth*K) and the modular operation
  * counts summed over residue-steps: [0] residue-steps, [1] modular multiplies,
  * [2] modular additions, [3] walk-kernel tasks. */

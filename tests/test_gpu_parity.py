@@ -144,6 +144,29 @@ def test_plain_mode_matches_batched(gpu_ctx):
         assert [_hex(v) for v in a.out["log_scale"]] == [_hex(v) for v in r.out["log_scale"]]
 
 
+def test_resident_run_then_fetch_equals_run(gpu_ctx):
+    # drx_plan_run_resident leaves the outputs in HBM: the host buffers keep
+    # their old contents until drx_plan_fetch, which then gives what
+    # drx_plan_run gives (residues, CRT limbs and signs, FP64 results, top-k)
+    cfg = synth.pcf2(n_walks=40)
+    specs = [pcf_spec(D.compile_pcf(w["a"], w["b"]), D.pcf_initial_values(w["a"]), 500, 32)
+             for w in cfg["walks"]]
+    ref = BatchPlan(specs, crt=True, top_k=8).run()
+    with BatchPlan(specs, crt=True, top_k=8) as plan:
+        r = plan.result
+        r.residues[...] = -1
+        plan.run(fetch=False)
+        assert (r.residues == -1).all()
+        plan.fetch()
+        assert np.array_equal(r.residues, ref.residues)
+        assert np.array_equal(r.limbs, ref.limbs) and np.array_equal(r.neg, ref.neg)
+        assert np.array_equal(r.topk, ref.topk)
+        assert [_hex(v) for v in r.out["delta_float"]] == [_hex(v) for v in ref.out["delta_float"]]
+        r.residues[...] = -1
+        plan.run()  # the one-call form still copies back
+        assert np.array_equal(r.residues, ref.residues)
+
+
 def test_inv_of_zero_is_zero(gpu_ctx):
     # b(n) = -1/(n+1) at shift -3: n + 1 = 0 at step 2 (INV(0) = 0 in residues, 0.0 in floats)
     p = D.compile_pcf("3", "-1/(n+1)")
