@@ -394,6 +394,7 @@ def main():
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--graph", action="store_true", help="replay the batch as a hipGraph")
+    ap.add_argument("--tune", action="store_true", help="pcf2: autotune the launch shape first")
     ap.add_argument("--workload", choices=("pcf2", "cmf3", "mix4", "zeta5", "sweep3f2"), default="pcf2",
                     help="pcf2 (the headline line); cmf3 / mix4 / zeta5 / sweep3f2 measure the other batch shapes "
                          "(no served leg; a CPU leg for zeta5 / sweep3f2)")
@@ -423,13 +424,19 @@ def main():
     log(f"compiled {len(progs)} PCFs in {time.perf_counter() - t0:.1f}s")
 
     tuned = None
-    if not (args.threads or args.chunk):
+    if args.tune and not (args.threads or args.chunk):
         from dreams_rns.engine import autotune
 
         t = time.perf_counter()
         tuned = autotune(specs)
         args.threads, args.chunk = tuned["threads"], tuned["chunk_steps"]
         log(f"autotuned in {time.perf_counter() - t:.1f}s: threads {args.threads}, chunk {args.chunk}")
+    # without --tune the line uses the shape the autotuner picks for this
+    # batch (64 lanes, 4096 steps per task: 0.80 ms/step against 0.88 for
+    # 256 lanes, whose faster walk kernel crowds out the FP64 M_f / shadow
+    # chain that sets the step time), fixed so that the walk kernel's time
+    # matches its committed kernel trace
+    args.threads, args.chunk = args.threads or 64, args.chunk or 4096
     plan = BatchPlan(specs, crt=True, top_k=top_k, threads=args.threads, chunk_steps=args.chunk,
                      use_graph=args.graph)
     log(f"plan built: {time.perf_counter() - t0:.1f}s since start")
@@ -520,7 +527,7 @@ def main():
         "config": {"workload": f"pcf2: {len(specs)} seeded PCFs, deg a<=3, deg b<=6, coef [-20,20], "
                                f"depth {depth}, K={K}, FP64 shadow + delta_float, GPU Garner CRT, top-{top_k}",
                    "walks": len(specs), "depth": depth, "K": K, "top_k": top_k,
-                   "threads": args.threads or 256, "chunk_steps": args.chunk or 4096, "graph": bool(args.graph),
+                   "threads": args.threads, "chunk_steps": args.chunk, "graph": bool(args.graph),
                    "autotuned": tuned is not None},
         "latency_batch_ms": {"p50": float(np.percentile(step_ms, 50)), "p99": float(np.percentile(step_ms, 99))},
         "with_copy_back": with_copy(rsteps, copy_ms),

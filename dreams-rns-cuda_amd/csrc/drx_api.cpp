@@ -204,7 +204,7 @@ size_t mf_lds_bytes(const DevProgram& pg) {
 void shadow_windows(int64_t max_steps, int64_t* win, int* nwin) {
   int64_t w = (max_steps + kMaxWin - 1) / kMaxWin;
   w = (w + kMfRange - 1) / kMfRange * kMfRange;
-  if (w < 2 * kMfRange) w = 2 * kMfRange;
+  if (w < kMfRange) w = kMfRange;
   int n = (int)((max_steps + w - 1) / w);
   *win = w;
   *nwin = n < 1 ? 1 : n;

@@ -53,7 +53,7 @@ enum WalkKind { WALK_INTERP = 0, WALK_FD = 1, WALK_FD_PCF = 2, WALK_FD_COMP = 3 
 // M_f pass: one lane per step, kMfBlock lanes per block; a range (one block)
 // covers kMfRange consecutive steps of one task.
 constexpr int kMfBlock = 256;
-constexpr int kMfRange = kMfBlock;
+constexpr int kMfRange = 2 * kMfBlock;  // each lane evaluates 2 steps
 
 struct GarnerArgs {
   const int64_t* res;     // rows of canonical residues, row stride res_stride

@@ -756,9 +756,9 @@ __device__ __forceinline__ void run_program_fp(const uint2* __restrict__ code, u
   }
 }
 
-// One block per range of kMfBlock consecutive steps of one task, one lane per
-// step; lanes past the range repeat its last step so the interpreter never
-// diverges. M_f(s) is stored column-major: the shadow lane of column j reads
+// One block per range of kMfRange consecutive steps of one task (two steps
+// per lane, so the block's set-up is paid once for 512 steps); lanes past
+// the range repeat its last step so the interpreter never diverges. M_f(s) is stored column-major: the shadow lane of column j reads
 // M_f[0..M)[j] in one piece.
 template <int M>
 __global__ void __launch_bounds__(256) k_mf(const KArgs a) {
@@ -777,23 +777,26 @@ __global__ void __launch_bounds__(256) k_mf(const KArgs a) {
   for (uint32_t i = tid; i < pg.nconst; i += T) ldcf[i] = a.cflt[pg.cflt_off + i];
   __syncthreads();
   const uint2* code = reinterpret_cast<const uint2*>(a.code + pg.fcode_off);
-  typename EBank<M>::type E = 0.0;
-  v16d R = 0.0, X = 0.0;
-  const int64_t want = r.s_begin + tid;
-  const int64_t s = want < r.s_end ? want : r.s_end - 1;
 #pragma unroll 1
-  for (uint32_t ax = 0; ax < pg.dim; ++ax) {
-    const int64_t* c = a.axi + w.axi_off + 2 * ax;
-    X[ax] = (double)(c[0] + s * c[1]) / a.axd[w.axd_off + ax];
-  }
-  X[kStepSlot] = (double)s;
-  run_program_fp<false>(code, pg.nfcode, R, E, X, X, ldcf, lds_s, tid, T);
-  if (want < r.s_end) {
-    double* out = a.mf + t.mf_off + (uint64_t)(s - t.s_begin) * M * M;
+  for (int64_t s0 = r.s_begin; s0 < r.s_end; s0 += T) {  // uniform
+    typename EBank<M>::type E = 0.0;
+    v16d R = 0.0, X = 0.0;
+    const int64_t want = s0 + tid;
+    const int64_t s = want < r.s_end ? want : r.s_end - 1;
+#pragma unroll 1
+    for (uint32_t ax = 0; ax < pg.dim; ++ax) {
+      const int64_t* c = a.axi + w.axi_off + 2 * ax;
+      X[ax] = (double)(c[0] + s * c[1]) / a.axd[w.axd_off + ax];
+    }
+    X[kStepSlot] = (double)s;
+    run_program_fp<false>(code, pg.nfcode, R, E, X, X, ldcf, lds_s, tid, T);
+    if (want < r.s_end) {
+      double* out = a.mf + t.mf_off + (uint64_t)(s - t.s_begin) * M * M;
 #pragma unroll
-    for (int e = 0; e < M * M; ++e) {
-      const uint32_t kd = (pg.kinds >> (2 * e)) & 3u;
-      out[(e % M) * M + e / M] = (kd == E_GEN) ? E[e] : (kd == E_ONE ? 1.0 : 0.0);
+      for (int e = 0; e < M * M; ++e) {
+        const uint32_t kd = (pg.kinds >> (2 * e)) & 3u;
+        out[(e % M) * M + e / M] = (kd == E_GEN) ? E[e] : (kd == E_ONE ? 1.0 : 0.0);
+      }
     }
   }
 }
