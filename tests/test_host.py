@@ -86,6 +86,35 @@ def test_serialize_round_trip(golden):
             (int(i.op), i.arg0, i.arg1, i.arg2) for i in p.opcodes]
 
 
+def test_serialize_byte_layout():
+    # the layout compiler.py:53-62 writes, spelled out field by field in
+    # explicit little-endian: header (m, dim, n_ops, n_const) as u32, then
+    # 16 bytes (op, a0, a1, a2) per instruction, int64 constants, int32
+    # directions
+    import struct
+
+    from dreams_rns.compiler import Instruction, Opcode
+
+    p = CmfProgram(m=2, dim=2, opcodes=[Instruction(Opcode.LOAD_X, 0, 0, 0), Instruction(Opcode.LOAD_C, 1, 1, 0),
+                                        Instruction(Opcode.MUL, 2, 0, 1), Instruction(Opcode.STORE, 2, 3, 0),
+                                        Instruction(Opcode.END, 0, 0, 0)],
+                   constants=[-7, 2 ** 62], directions=[3, -1])
+    want = bytes.fromhex("02000000" "02000000" "05000000" "02000000")
+    # opcode numbers of compiler.py:14-28: LOAD_X 1, LOAD_C 2, MUL 6, STORE 11, END 12
+    for op, a0, a1, a2 in [(1, 0, 0, 0), (2, 1, 1, 0), (6, 2, 0, 1), (11, 2, 3, 0), (12, 0, 0, 0)]:
+        want += b"".join(x.to_bytes(4, "little") for x in (op, a0, a1, a2))
+    want += (-7).to_bytes(8, "little", signed=True) + (2 ** 62).to_bytes(8, "little", signed=True)
+    want += (3).to_bytes(4, "little", signed=True) + (-1).to_bytes(4, "little", signed=True)
+    got = p.serialize()
+    assert got == want
+    assert len(got) == 16 + 16 * 5 + 8 * 2 + 4 * 2
+    q = CmfProgram.deserialize(got)
+    assert (q.constants, q.directions) == ([-7, 2 ** 62], [3, -1])
+    # a constant past int64 cannot be written ('q' format), as in the reference
+    with pytest.raises(struct.error):
+        CmfProgram(m=2, dim=0, opcodes=[], constants=[2 ** 63]).serialize()
+
+
 # ----------------------------------------------- FP64 residue arithmetic
 def _mm(a: float, b: float, p: float, pinv: float) -> float:
     """The device's balanced modular multiply, step by step in IEEE doubles."""
