@@ -122,7 +122,7 @@ def run_cmf_walk_vec_batch(program: CmfProgram, depth: int, K: int, shift_list, 
     drivers' loop over shifts x trajectories, odd_zeta_sweep.py:481-494):
     walk i uses shift_list[i] and trajectory_list[i] (default: the program's
     directions). With ``crt``, each dict also holds the centred CRT integers
-    ``p_crt`` / ``q_crt`` from the GPU Garner pass."""
+    ``p_crt`` / ``q_crt`` from the GPU CRT pass."""
     n = len(shift_list)
     trajs = trajectory_list if trajectory_list is not None else [None] * n
     if len(trajs) != n:

@@ -3,8 +3,9 @@
 A batch is a list of :class:`WalkSpec`. :class:`BatchPlan` packs it once
 (programs, initial matrices, axis tables -> device), then ``run()`` executes
 the whole batch on the GPU -- interpreter + chunked P*M products + ordered LDS
-tree combine, FP64 shadow with delta_float, Garner CRT and top-k -- and copies
-the results back. A plan can be run repeatedly; every run restarts from the
+tree combine, FP64 shadow with delta_float, CRT (Garner up to 256 primes,
+matrix cores above) and top-k -- and copies the results back (``run(fetch=False)``
+leaves them in device memory until ``fetch()``). A plan can be run repeatedly; every run restarts from the
 initial matrices.
 """
 
@@ -150,7 +151,7 @@ class BatchResult:
         }
 
     def crt_values(self, i: int) -> Tuple[int, int]:
-        """Centred CRT integers (p, q) of walk i from the GPU Garner limbs."""
+        """Centred CRT integers (p, q) of walk i from the GPU CRT limbs."""
         if self.limbs is None:
             raise RuntimeError("plan was built without crt=True")
         K = int(self.Ks[i])

@@ -69,8 +69,9 @@ def _ctx_for_primes(primes) -> N.Context:
 def crt_reconstruct(residues, primes) -> Tuple[int, int]:
     """x in [0, M) with x = r_i mod p_i, and M = prod(p_i) (runner.py:57-76).
 
-    Garner's mixed-radix CRT runs on the GPU (drx_crt); the host only joins
-    the returned limbs into a Python int.
+    The CRT runs on the GPU (drx_crt: Garner's mixed radix up to 256 primes,
+    the matrix-core CRT above); the host only joins the returned limbs into
+    a Python int.
     """
     pr = [int(p) for p in primes]
     K = len(pr)

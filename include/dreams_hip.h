@@ -208,10 +208,11 @@ void drx_plan_destroy(drx_plan* plan);
 int drx_autotune(drx_ctx* ctx, const drx_batch* b, const drx_launch_cfg* base, int max_walks, double budget_s,
                  drx_launch_cfg* best, double* best_ms);
 
-/* Garner CRT of n rows of K canonical residues (row stride `stride`) over the
+/* CRT of n rows of K canonical residues (row stride `stride`) over the
  * first K context primes: magnitude limbs (little-endian uint32, K per row)
- * and sign of centered(x, M). Replaces crt_reconstruct + centered
- * (runner.py:57-81). */
+ * and sign of centered(x, M). Rows of up to 256 primes take a Garner
+ * kernel, wider rows (up to 1024) the matrix-core CRT. Replaces
+ * crt_reconstruct + centered (runner.py:57-81). */
 int drx_crt(drx_ctx* ctx, const int64_t* residues, int n, int K, int stride, uint32_t* limbs,
             uint8_t* neg);
 
