@@ -126,7 +126,8 @@ def measured_counters(kernel: str, workload: str):
     summaries: HBM bytes per launch (FETCH_SIZE / WRITE_SIZE scaled by the
     copy calibration) and VALU instructions per launch, or None when the
     summary is for another kernel."""
-    path = os.path.join(ROOT, "profiles", "walk_kernel_counters.json")
+    name = "walk_kernel_counters.json" if workload == "pcf2" else f"walk_kernel_counters_{workload}.json"
+    path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as f:
             t = json.load(f)
@@ -349,10 +350,17 @@ def other_workload(args):
     ms = float(np.mean(step_ms))
     walk_avg = float(np.mean(walk_ms)) if not args.graph else float("nan")
     achieved = work / (walk_avg * 1e-3) if walk_avg == walk_avg else None
-    w_exec = 6.0 * float(st["modmul"]) + 2.0 * float(st["modadd"])
+    kname = "k_walk_vec" if args.workload == "zeta5" else walk_kernel_name([s.program for s in specs])
+    cnt = measured_counters(kname, args.workload)
+    traffic = float(cnt["traffic_bytes"]) if cnt else None
+    if cnt:  # VALU instructions the counters measured for this kernel on this batch
+        w_exec = float(cnt["valu_insts"]) * 64
+        model = f"SQ_INSTS_VALU x 64 per launch ({cnt['source']})"
+    else:
+        w_exec = 6.0 * float(st["modmul"]) + 2.0 * float(st["modadd"])
+        model = "6*modmul + 2*modadd of the modular operations the chosen kernels execute (library count)"
     executed = ({"achieved": w_exec / (walk_avg * 1e-3) / 1e12, "frac": w_exec / (walk_avg * 1e-3) / PEAK_LANE_OPS,
-                 "work_per_launch": w_exec,
-                 "model": "6*modmul + 2*modadd of the modular operations the chosen kernels execute (library count)"}
+                 "work_per_launch": w_exec, "model": model}
                 if walk_avg == walk_avg else None)
     print(json.dumps({
         "metric": METRIC, "value": rsteps / (ms * 1e-3), "unit": "residue-steps/s", "n_gpus": 1,
@@ -364,8 +372,8 @@ def other_workload(args):
         "with_copy_back": with_copy(rsteps, copy_ms),
         "roofline": {"bound": "valu", "achieved": achieved / 1e12 if achieved else None,
                      "peak": PEAK_LANE_OPS / 1e12, "unit": "TOP/s",
-                     "frac": achieved / PEAK_LANE_OPS if achieved else None, "traffic": None,
-                     "kernel": "k_walk_vec" if args.workload == "zeta5" else walk_kernel_name([s.program for s in specs]),
+                     "frac": achieved / PEAK_LANE_OPS if achieved else None, "traffic": traffic,
+                     "kernel": kname,
                      "kernel_ms": walk_avg, "work_per_launch": work,
                      "work_model": "W = 6*MM + 2*MA lane-ops per residue-step on each walk's bytecode, constant-only "
                                    "instructions excluded (per-step program evaluation; forward differences skip it, "
