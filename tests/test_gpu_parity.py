@@ -167,6 +167,28 @@ def test_resident_run_then_fetch_equals_run(gpu_ctx):
         assert np.array_equal(r.residues, ref.residues)
 
 
+def test_short_and_ragged_depths_in_one_batch(gpu_ctx):
+    # depth 0 (the initial matrix itself), 1, 2 and depths that end inside
+    # a table-reduction period, beside a full-length walk, through the
+    # forward-difference PCF kernel and the interpreter; each equals its
+    # single-walk oracle run (residues bit-exact, FP64 bits equal)
+    pcfs = [("3*n+1", "-n**2"), ("2", "n**2"), ("n**3 - 2*n", "5*n**6 + 1"), ("7", "-3*n**2+n")]
+    depths = [0, 1, 2, 3, 5, 9, 13, 1000]
+    specs, refs = [], []
+    for i, d in enumerate(depths):
+        a, b = pcfs[i % len(pcfs)]
+        prog = D.compile_pcf(a, b)
+        specs.append(pcf_spec(prog, D.pcf_initial_values(a), d, 16))
+        refs.append(ow.run_pcf_walk(prog, D.pcf_initial_values(a), d, 16, 1))
+    for interp in (False, True):
+        res = BatchPlan(specs, interp_only=interp).run()
+        for i, ref in enumerate(refs):
+            d = res.walk_dict(i)
+            assert d["p_residues"].tolist() == ref["p_residues"].tolist(), (interp, depths[i])
+            assert d["q_residues"].tolist() == ref["q_residues"].tolist(), (interp, depths[i])
+            assert _hex(d["p_float"]) == _hex(ref["p_float"]) and _hex(d["q_float"]) == _hex(ref["q_float"])
+
+
 def test_inv_of_zero_is_zero(gpu_ctx):
     # b(n) = -1/(n+1) at shift -3: n + 1 = 0 at step 2 (INV(0) = 0 in residues, 0.0 in floats)
     p = D.compile_pcf("3", "-1/(n+1)")
