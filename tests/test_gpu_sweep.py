@@ -69,7 +69,7 @@ def test_sweep_subset_against_c_oracle(gpu_ctx, sweep_fix):
     assert n_rows < n_rec  # shifts of equal offsets walked once
 
 
-@pytest.mark.parametrize("K", [257, 300, 1000])
+@pytest.mark.parametrize("K", [257, 300, 1000, 1024])
 def test_prime_slices_match_c_oracle(gpu_ctx, sweep_fix, K):
     # walks of more than 256 primes run as prime slices; several step ranges
     # per walk (chunk_steps 32) make every slice combine in order
@@ -94,6 +94,14 @@ def test_prime_slices_match_c_oracle(gpu_ctx, sweep_fix, K):
         xp, M = ow.crt_reconstruct(rr[0, 3].tolist(), primes.tolist())
         xq, _ = ow.crt_reconstruct(rr[3, 3].tolist(), primes.tolist())
         assert res.crt_values(i) == (ow.centered(xp, M), ow.centered(xq, M))
+
+
+def test_more_primes_than_one_context_holds_is_refused(gpu_ctx, sweep_fix):
+    # 1024 primes is the limit of a matrix walk: one more is refused before
+    # anything runs, with the reference's exception type for bad arguments
+    prog = compile_cmf_spec(sweep_fix["specs"][5], [1, 1, 1, 1, 1])
+    with pytest.raises(ValueError, match="1025"):
+        BatchPlan([cmf_spec(prog, 10, 1025, [1, 1, 1, 1, 1])])
 
 
 def test_mixed_widths_in_one_batch(gpu_ctx, sweep_fix):
