@@ -305,12 +305,13 @@ static uint32_t inv_mod(uint32_t a, uint32_t m) {
   return (uint32_t)(t0 < 0 ? t0 + m : t0);
 }
 
-// Garner's table inv[j][i] = p_j^{-1} mod p_i, built once for every prime
-// of the context so the device copy never moves while plans hold it.
+// Garner's table inv[j][i] = p_j^{-1} mod p_i for the wavefront kernel's
+// rows (at most 256 primes), built once for the first 256 primes of the
+// context so the device copy never moves while plans hold it.
 int ensure_inv_table(drx_ctx* ctx, int k) {
   if (k <= ctx->kinv) return DRX_OK;
-  const int kall = (int)ctx->primes.size();
-  if (k > kall) return fail(ctx, DRX_EINVAL, "K=%d exceeds the %d primes set", k, kall);
+  const int kall = std::min((int)ctx->primes.size(), kCrtMatrixMinK - 1);
+  if (k > kall) return fail(ctx, DRX_EINVAL, "K=%d exceeds the %d primes of the Garner table", k, kall);
   std::vector<uint32_t> t((size_t)kall * kall, 0);
   for (int j = 0; j < kall; ++j)
     for (int i = 0; i < kall; ++i)
@@ -322,6 +323,26 @@ int ensure_inv_table(drx_ctx* ctx, int k) {
   HIPCHK(ctx, hipMemcpy(ctx->d_inv.p, t.data(), t.size() * 4, hipMemcpyHostToDevice));
   ctx->inv_table.swap(t);
   ctx->kinv = kall;
+  return DRX_OK;
+}
+
+// w_i = (p_0 ... p_{i-1})^{-1} mod p_i for the wide Garner kernel, extended
+// to the first k primes (O(i) per new entry); an outgrown device copy is
+// retired, not freed, since plans hold its address.
+int ensure_garner_w(drx_ctx* ctx, int k) {
+  if (k > (int)ctx->primes.size()) return fail(ctx, DRX_EINVAL, "K=%d exceeds the %zu primes set", k, ctx->primes.size());
+  const int have = (int)ctx->garner_w.size();
+  if (k <= have && ctx->d_garner_w.p) return DRX_OK;
+  // grow in steps so that a run of slightly wider rows does not rebuild it each time
+  const int want = std::min<int>((int)ctx->primes.size(), std::max(k, std::max(2 * have, 256)));
+  for (int i = have; i < want; ++i) {
+    const uint64_t p = ctx->primes[i];
+    uint64_t prod = 1 % p;
+    for (int j = 0; j < i; ++j) prod = prod * (ctx->primes[j] % p) % p;
+    ctx->garner_w.push_back((double)inv_mod((uint32_t)prod, (uint32_t)p));
+  }
+  HIPCHK(ctx, ctx->d_garner_w.ensure_keep(ctx->garner_w.size() * 8, ctx->retired));
+  HIPCHK(ctx, hipMemcpy(ctx->d_garner_w.p, ctx->garner_w.data(), ctx->garner_w.size() * 8, hipMemcpyHostToDevice));
   return DRX_OK;
 }
 
@@ -477,14 +498,16 @@ struct drx_plan {
       d_fdt;
   DBuf d_fout, d_rout, d_pout, d_pfull, d_limbs, d_neg, d_topk, d_topk_ws, d_rowk;
   KArgs args{};
-  GarnerArgs gargs{};
-  // CRT of rows of at least kCrtMatrixMinK primes: one matrix-core group per K
+  GarnerArgs gargs{};   // rows of at most 256 primes: the wavefront Garner kernel
+  GarnerArgs wargs{};   // wider rows outside the matrix path: the wide Garner kernel
+  // matrix-core CRT: one group per prime count; the groups run one after the
+  // other on one stream and share one scratch (digit planes, plane products, k)
   struct CrtGroup {
-    DBuf d_rows, d_a, d_c, d_k;
+    DBuf d_rows;
     CrtArgs args{};
   };
   std::vector<std::unique_ptr<CrtGroup>> crt_groups;
-  DBuf d_grows;  // rows of the Garner kernel when some rows take the matrix path
+  DBuf d_grows, d_wrows, d_crt_a, d_crt_c, d_crt_k;
   int ntask = 0;
   double residue_steps = 0, mm_total = 0, ma_total = 0;
   hipGraph_t graph = nullptr;
@@ -500,10 +523,10 @@ struct drx_plan {
     if (graph) (void)hipGraphDestroy(graph);
     for (DBuf* d : {&d_walks, &d_tasks, &d_seeds, &d_spans, &d_ranges, &d_res0, &d_fst0, &d_res, &d_fst, &d_axr, &d_axi,
                     &d_axd, &d_cp, &d_mf, &d_fdt, &d_fout, &d_rout, &d_pout, &d_pfull, &d_limbs, &d_neg, &d_topk,
-                    &d_topk_ws, &d_rowk, &d_vgroups, &d_fgroups, &d_vorder, &d_grows})
+                    &d_topk_ws, &d_rowk, &d_vgroups, &d_fgroups, &d_vorder, &d_grows, &d_wrows, &d_crt_a,
+                    &d_crt_c, &d_crt_k})
       d->release();
-    for (auto& g : crt_groups)
-      for (DBuf* d : {&g->d_rows, &g->d_a, &g->d_c, &g->d_k}) d->release();
+    for (auto& g : crt_groups) g->d_rows.release();
     for (drx_plan* p : plain) delete p;
   }
 };
@@ -517,53 +540,102 @@ int upload(drx_ctx* ctx, DBuf& d, const std::vector<T>& h) {
   return DRX_OK;
 }
 
-// Vector walks: the residue kernel on the library's stream and the FP64
-// shadow on stream2 side by side; CRT after the residues, top-k after both.
-// CRT of the plan's p and q rows: the Garner kernel for rows of fewer than
-// kCrtMatrixMinK primes, one matrix-core group per wider K.
+// CRT of the plan's p and q rows: the wavefront Garner kernel for rows of at
+// most 256 primes, the matrix-core CRT for prime counts with enough rows
+// (kCrtMatrixMinRows, up to kCrtMatrixMaxK primes), the wide Garner kernel
+// for the other wide rows.
 int launch_crt(drx_plan* pl, hipStream_t s) {
   drx_ctx* ctx = pl->ctx;
   if (pl->gargs.n > 0) HIPCHK(ctx, launch_garner(pl->gargs, s));
+  if (pl->wargs.n > 0) HIPCHK(ctx, launch_garner_wide(pl->wargs, s));
   for (const auto& g : pl->crt_groups) HIPCHK(ctx, launch_crt_matrix(g->args, s));
   return DRX_OK;
 }
 
-// Split the CRT rows (2 per walk) by prime count and set up both paths; runs
-// after the outputs are allocated and GarnerArgs' common fields are set.
+// The CRT path of rows of K primes when `nrows` rows share that K.
+enum CrtPath { CRT_WAVE = 0, CRT_WIDE = 1, CRT_MATRIX = 2 };
+static int crt_path(int K, size_t nrows) {
+  if (K < kCrtMatrixMinK) return CRT_WAVE;
+  return (K <= kCrtMatrixMaxK && nrows >= (size_t)kCrtMatrixMinRows) ? CRT_MATRIX : CRT_WIDE;
+}
+
+// Matrix-core CRT arguments of one group (rows `rows`, scratch shared).
+static void crt_matrix_args(drx_ctx* ctx, const CrtTable* t, int n, int npad, int32_t* rows, const int64_t* res,
+                            uint64_t stride, int8_t* A, int32_t* C, int32_t* kq, double* fd, uint32_t* limbs,
+                            uint8_t* neg, CrtArgs& c) {
+  c.res = res;
+  c.res_stride = stride;
+  c.rows = rows;
+  c.n = n;
+  c.npad = npad;
+  c.K = t->K;
+  c.Kp = t->Kp;
+  c.T = t->T;
+  c.Tp = t->Tp;
+  c.L = t->L;
+  c.prime_p = ctx->d_prime_p.as<double>();
+  c.prime_pinv = ctx->d_prime_pinv.as<double>();
+  c.w = t->d_w.as<double>();
+  c.bt = t->d_bt.as<int8_t>();
+  c.m = t->d_m.as<uint32_t>();
+  c.h = t->d_h.as<uint32_t>();
+  c.A = A;
+  c.C = C;
+  c.kq = kq;
+  c.fd = fd;
+  c.limbs = limbs;
+  c.neg = neg;
+}
+
+// Split the CRT rows (2 per walk) by path; runs after the outputs are
+// allocated and GarnerArgs' common fields are set.
 int setup_crt(drx_ctx* ctx, drx_plan* pl) {
   const drx_batch& b = pl->b;
   pl->crt_groups.clear();
   GarnerArgs& g = pl->gargs;
+  GarnerArgs& wg = pl->wargs;
+  wg = g;
+  wg.n = 0;
   if (!b.crt_limbs) {
     g.n = 0;
     return DRX_OK;
   }
-  std::vector<int32_t> grows;
-  std::vector<std::pair<int, std::vector<int32_t>>> wide;
-  int gk = 1;
+  std::vector<std::pair<int, std::vector<int32_t>>> byk;  // rows by prime count
   for (int i = 0; i < pl->n; ++i) {
     const int K = b.walks[i].K;
-    for (int v = 0; v < 2; ++v) {
-      if (K < kCrtMatrixMinK) {
-        grows.push_back(2 * i + v);
-        gk = std::max(gk, K);
-        continue;
-      }
-      auto it = std::find_if(wide.begin(), wide.end(), [&](const std::pair<int, std::vector<int32_t>>& e) {
-        return e.first == K;
-      });
-      if (it == wide.end()) {
-        wide.push_back({K, {}});
-        it = wide.end() - 1;
-      }
-      it->second.push_back(2 * i + v);
+    auto it = std::find_if(byk.begin(), byk.end(), [&](const std::pair<int, std::vector<int32_t>>& e) {
+      return e.first == K;
+    });
+    if (it == byk.end()) {
+      byk.push_back({K, {}});
+      it = byk.end() - 1;
+    }
+    it->second.push_back(2 * i);
+    it->second.push_back(2 * i + 1);
+  }
+  std::vector<int32_t> grows, wrows;
+  std::vector<std::pair<int, std::vector<int32_t>>> wide;
+  int gk = 1, wk = 1;
+  for (auto& e : byk) {
+    switch (crt_path(e.first, e.second.size())) {
+      case CRT_WAVE:
+        grows.insert(grows.end(), e.second.begin(), e.second.end());
+        gk = std::max(gk, e.first);
+        break;
+      case CRT_WIDE:
+        wrows.insert(wrows.end(), e.second.begin(), e.second.end());
+        wk = std::max(wk, e.first);
+        break;
+      default:
+        wide.push_back(std::move(e));
     }
   }
   int rc;
+  const bool all_wave = grows.size() == (size_t)2 * pl->n;
   g.n = (int)grows.size();
   g.K = gk;
   g.rows = nullptr;
-  if (!wide.empty() && !grows.empty()) {
+  if (!grows.empty() && !all_wave) {
     if ((rc = upload(ctx, pl->d_grows, grows))) return rc;
     g.rows = pl->d_grows.as<int32_t>();
   }
@@ -572,44 +644,47 @@ int setup_crt(drx_ctx* ctx, drx_plan* pl) {
     g.inv_table = ctx->d_inv.as<uint32_t>();
     g.inv_ld = (uint64_t)ctx->kinv;
   }
+  if (!wrows.empty()) {
+    if ((rc = ensure_garner_w(ctx, wk))) return rc;
+    if ((rc = upload(ctx, pl->d_wrows, wrows))) return rc;
+    wg.rows = pl->d_wrows.as<int32_t>();
+    wg.n = (int)wrows.size();
+    wg.K = wk;
+    wg.w = ctx->d_garner_w.as<double>();
+  }
+  size_t a_bytes = 0, c_bytes = 0, k_bytes = 0;
+  std::vector<const CrtTable*> tabs;
   for (auto& e : wide) {
     const CrtTable* t = nullptr;
     if ((rc = ensure_crt_table(ctx, e.first, &t))) return rc;
+    if ((size_t)t->L + 2 > (size_t)pl->ld) return fail(ctx, DRX_EINVAL, "internal: CRT rows narrower than M");
+    const size_t npad = (e.second.size() + 127) / 128 * 128;
+    a_bytes = std::max(a_bytes, (size_t)4 * npad * t->Kp);
+    c_bytes = std::max(c_bytes, (size_t)4 * t->Tp * npad * 4);
+    k_bytes = std::max(k_bytes, npad * 12);
+    tabs.push_back(t);
+  }
+  if (!wide.empty()) {
+    HIPCHK(ctx, pl->d_crt_a.ensure(a_bytes));
+    HIPCHK(ctx, pl->d_crt_c.ensure(c_bytes));
+    HIPCHK(ctx, pl->d_crt_k.ensure(k_bytes));
+  }
+  for (size_t gi = 0; gi < wide.size(); ++gi) {
+    const auto& e = wide[gi];
     std::unique_ptr<drx_plan::CrtGroup> grp(new drx_plan::CrtGroup());
     const int ng = (int)e.second.size(), npad = (ng + 127) / 128 * 128;
-    if ((size_t)t->L + 2 > (size_t)pl->ld) return fail(ctx, DRX_EINVAL, "internal: CRT rows narrower than M");
     if ((rc = upload(ctx, grp->d_rows, e.second))) return rc;
-    HIPCHK(ctx, grp->d_a.ensure((size_t)4 * npad * t->Kp));
-    HIPCHK(ctx, grp->d_c.ensure((size_t)4 * t->Tp * npad * 4));
-    HIPCHK(ctx, grp->d_k.ensure((size_t)npad * 12));
-    CrtArgs& c = grp->args;
-    c.res = pl->d_rout.as<int64_t>();
-    c.res_stride = (uint64_t)pl->ld;
-    c.rows = grp->d_rows.as<int32_t>();
-    c.n = ng;
-    c.npad = npad;
-    c.K = t->K;
-    c.Kp = t->Kp;
-    c.T = t->T;
-    c.Tp = t->Tp;
-    c.L = t->L;
-    c.prime_p = ctx->d_prime_p.as<double>();
-    c.prime_pinv = ctx->d_prime_pinv.as<double>();
-    c.w = t->d_w.as<double>();
-    c.bt = t->d_bt.as<int8_t>();
-    c.m = t->d_m.as<uint32_t>();
-    c.h = t->d_h.as<uint32_t>();
-    c.A = grp->d_a.as<int8_t>();
-    c.C = grp->d_c.as<int32_t>();
-    c.kq = grp->d_k.as<int32_t>();
-    c.fd = reinterpret_cast<double*>(grp->d_k.as<char>() + (size_t)npad * 4);
-    c.limbs = pl->d_limbs.as<uint32_t>();
-    c.neg = pl->d_neg.as<uint8_t>();
+    crt_matrix_args(ctx, tabs[gi], ng, npad, grp->d_rows.as<int32_t>(), pl->d_rout.as<int64_t>(), (uint64_t)pl->ld,
+                    pl->d_crt_a.as<int8_t>(), pl->d_crt_c.as<int32_t>(), pl->d_crt_k.as<int32_t>(),
+                    reinterpret_cast<double*>(pl->d_crt_k.as<char>() + (size_t)npad * 4), pl->d_limbs.as<uint32_t>(),
+                    pl->d_neg.as<uint8_t>(), grp->args);
     pl->crt_groups.push_back(std::move(grp));
   }
   return DRX_OK;
 }
 
+// Vector walks: the residue kernel on the library's stream and the FP64
+// shadow on stream2 side by side; CRT after the residues, top-k after both.
 int launch_vec_sequence(drx_plan* pl, bool timed) {
   drx_ctx* ctx = pl->ctx;
   hipStream_t s = ctx->stream, s2 = ctx->stream2;
@@ -888,6 +963,9 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
     if (w.prog < 0 || w.prog >= np) return fail(ctx, DRX_EINVAL, "walk %d: bad program id %d", i, w.prog);
     if (w.K < 1 || w.K > K_ctx)
       return fail(ctx, DRX_EINVAL, "walk %d: K=%d outside [1, %d] (primes set)", i, w.K, K_ctx);
+    if (w.K > ctx->kwalk)
+      return fail(ctx, DRX_ERANGE, "walk %d: K=%d but only the first %d primes lie in (2^30, 2^31), as walks need",
+                  i, w.K, ctx->kwalk);
     const HostProgram& hp = ctx->progs[w.prog];
     if (w.K > (int)hp.dev.kcap)
       return fail(ctx, DRX_EINVAL, "walk %d: K=%d exceeds the program's constant table (%u)", i, w.K, hp.dev.kcap);
@@ -1330,13 +1408,21 @@ int drx_set_primes(drx_ctx* ctx, const int64_t* primes, int K) {
   if (!ctx) return DRX_EINVAL;
   if (!primes || K < 1 || K > kMaxK) return fail(ctx, DRX_EINVAL, "K=%d outside [1, %d]", K, kMaxK);
   std::vector<uint32_t> pr(K);
-  for (int i = 0; i < K; ++i) {
-    const int64_t p = primes[i];
-    if (p <= ((int64_t)1 << 30) || p >= ((int64_t)1 << 31) || !is_prime_u32((uint32_t)p))
-      return fail(ctx, DRX_EINVAL, "prime %d (%lld) is not a prime in (2^30, 2^31)", i, (long long)p);
-    for (int j = 0; j < i; ++j)
-      if (pr[j] == (uint32_t)p) return fail(ctx, DRX_EINVAL, "prime %lld repeated", (long long)p);
-    pr[i] = (uint32_t)p;
+  int kwalk = 0;
+  {
+    std::vector<uint32_t> sorted;
+    sorted.reserve(K);
+    for (int i = 0; i < K; ++i) {
+      const int64_t p = primes[i];
+      if (p < 3 || p >= ((int64_t)1 << 31) || !is_prime_u32((uint32_t)p))
+        return fail(ctx, DRX_EINVAL, "prime %d (%lld) is not an odd prime below 2^31", i, (long long)p);
+      pr[i] = (uint32_t)p;
+      if (kwalk == i && p > ((int64_t)1 << 30)) ++kwalk;
+      sorted.push_back((uint32_t)p);
+    }
+    std::sort(sorted.begin(), sorted.end());
+    if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+      return fail(ctx, DRX_EINVAL, "a prime is repeated");
   }
   std::lock_guard<std::mutex> lock(ctx->mu);
   if (!ctx->primes.empty() && (size_t)K >= ctx->primes.size() &&
@@ -1360,6 +1446,7 @@ int drx_set_primes(drx_ctx* ctx, const int64_t* primes, int K) {
   if (!extends) {
     ++ctx->epoch;
     retire_crt_tables(ctx);
+    ctx->garner_w.clear();  // its device copy is rewritten (retired if outgrown) on next use
   }
   HIPCHK(ctx, ctx->d_prime_p.ensure_keep(K * 8, ctx->retired));
   HIPCHK(ctx, ctx->d_prime_pinv.ensure_keep(K * 8, ctx->retired));
@@ -1378,6 +1465,7 @@ int drx_set_primes(drx_ctx* ctx, const int64_t* primes, int K) {
   }
   ctx->primes.swap(pr);
   ctx->kinv = 0;
+  ctx->kwalk = kwalk;
   return DRX_OK;
 }
 
@@ -1400,7 +1488,9 @@ int drx_program_upload_mode(drx_ctx* ctx, const uint32_t* ops, int n_ops, int m,
     return fail(ctx, DRX_EINVAL, "constants missing (give consts or const_res + const_f)");
   const int K = (int)ctx->primes.size();
   const bool given = const_res != nullptr;
-  const int kc = given ? std::min(kcap, K) : K;
+  // constant residues for the first kcap primes (all primes set when kcap <= 0 and
+  // the constants are given as int64): walks of this program may use up to kc primes
+  const int kc = (given || kcap > 0) ? std::min(kcap, K) : K;
   if (given && kcap < 1) return fail(ctx, DRX_EINVAL, "kcap must be >= 1");
   std::vector<char> is_one(n_const, 0), is_zero(n_const, 0);
   for (int i = 0; i < n_const; ++i) {
@@ -1916,7 +2006,8 @@ int drx_crt(drx_ctx* ctx, const int64_t* residues, int n, int K, int stride, uin
   HIPCHK(ctx, ctx->d_tmp3.ensure((size_t)n));
   HIPCHK(ctx, hipMemcpyAsync(ctx->d_tmp0.p, residues, rows * 8, hipMemcpyHostToDevice, ctx->stream));
   int rc;
-  if (K >= kCrtMatrixMinK) {  // matrix-core CRT
+  const int path = crt_path(K, (size_t)n);
+  if (path == CRT_MATRIX) {
     const CrtTable* t = nullptr;
     if ((rc = ensure_crt_table(ctx, K, &t))) return rc;
     const int npad = (n + 127) / 128 * 128;
@@ -1928,49 +2019,34 @@ int drx_crt(drx_ctx* ctx, const int64_t* residues, int n, int K, int stride, uin
     HIPCHK(ctx, ctx->d_crt_c.ensure((size_t)4 * t->Tp * npad * 4));
     HIPCHK(ctx, ctx->d_crt_k.ensure((size_t)npad * 12));
     CrtArgs c{};
-    c.res = ctx->d_tmp0.as<int64_t>();
-    c.res_stride = (uint64_t)stride;
-    c.rows = ctx->d_crt_rows.as<int32_t>();
-    c.n = n;
-    c.npad = npad;
-    c.K = K;
-    c.Kp = t->Kp;
-    c.T = t->T;
-    c.Tp = t->Tp;
-    c.L = t->L;
-    c.prime_p = ctx->d_prime_p.as<double>();
-    c.prime_pinv = ctx->d_prime_pinv.as<double>();
-    c.w = t->d_w.as<double>();
-    c.bt = t->d_bt.as<int8_t>();
-    c.m = t->d_m.as<uint32_t>();
-    c.h = t->d_h.as<uint32_t>();
-    c.A = ctx->d_crt_a.as<int8_t>();
-    c.C = ctx->d_crt_c.as<int32_t>();
-    c.kq = ctx->d_crt_k.as<int32_t>();
-    c.fd = reinterpret_cast<double*>(ctx->d_crt_k.as<char>() + (size_t)npad * 4);
-    c.limbs = ctx->d_tmp2.as<uint32_t>();
-    c.neg = ctx->d_tmp3.as<uint8_t>();
+    crt_matrix_args(ctx, t, n, npad, ctx->d_crt_rows.as<int32_t>(), ctx->d_tmp0.as<int64_t>(), (uint64_t)stride,
+                    ctx->d_crt_a.as<int8_t>(), ctx->d_crt_c.as<int32_t>(), ctx->d_crt_k.as<int32_t>(),
+                    reinterpret_cast<double*>(ctx->d_crt_k.as<char>() + (size_t)npad * 4), ctx->d_tmp2.as<uint32_t>(),
+                    ctx->d_tmp3.as<uint8_t>(), c);
     HIPCHK(ctx, launch_crt_matrix(c, ctx->stream));
-    HIPCHK(ctx, hipMemcpyAsync(limbs, ctx->d_tmp2.p, rows * 4, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipMemcpyAsync(neg, ctx->d_tmp3.p, (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    return DRX_OK;
+  } else {
+    GarnerArgs g{};
+    g.res = ctx->d_tmp0.as<int64_t>();
+    g.res_stride = (uint64_t)stride;
+    g.n = n;
+    g.K = K;
+    g.row_k = nullptr;
+    g.prime_p = ctx->d_prime_p.as<double>();
+    g.prime_pinv = ctx->d_prime_pinv.as<double>();
+    g.prime_u = ctx->d_prime_u.as<uint32_t>();
+    g.limbs = ctx->d_tmp2.as<uint32_t>();
+    g.neg = ctx->d_tmp3.as<uint8_t>();
+    if (path == CRT_WAVE) {
+      if ((rc = ensure_inv_table(ctx, K))) return rc;
+      g.inv_table = ctx->d_inv.as<uint32_t>();
+      g.inv_ld = (uint64_t)ctx->kinv;
+      HIPCHK(ctx, launch_garner(g, ctx->stream));
+    } else {
+      if ((rc = ensure_garner_w(ctx, K))) return rc;
+      g.w = ctx->d_garner_w.as<double>();
+      HIPCHK(ctx, launch_garner_wide(g, ctx->stream));
+    }
   }
-  if ((rc = ensure_inv_table(ctx, K))) return rc;
-  GarnerArgs g{};
-  g.res = ctx->d_tmp0.as<int64_t>();
-  g.res_stride = (uint64_t)stride;
-  g.n = n;
-  g.K = K;
-  g.row_k = nullptr;
-  g.prime_p = ctx->d_prime_p.as<double>();
-  g.prime_pinv = ctx->d_prime_pinv.as<double>();
-  g.prime_u = ctx->d_prime_u.as<uint32_t>();
-  g.inv_table = ctx->d_inv.as<uint32_t>();
-  g.inv_ld = (uint64_t)ctx->kinv;
-  g.limbs = ctx->d_tmp2.as<uint32_t>();
-  g.neg = ctx->d_tmp3.as<uint8_t>();
-  HIPCHK(ctx, launch_garner(g, ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(limbs, ctx->d_tmp2.p, rows * 4, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(neg, ctx->d_tmp3.p, (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));

@@ -6,9 +6,14 @@
 
 namespace drx {
 
-// Rows of at least this many primes take the matrix-core CRT; fewer take
-// the one-wavefront Garner kernel.
+// Rows of at most 256 primes take the one-wavefront Garner kernel. Wider
+// rows take the matrix-core CRT when their prime count is at most
+// kCrtMatrixMaxK and at least kCrtMatrixMinRows rows share it (the GEMM pads
+// rows to 128 and needs a table per prime count); every other wide row takes
+// the wide Garner kernel (one workgroup per row, no per-K table).
 constexpr int kCrtMatrixMinK = 257;
+constexpr int kCrtMatrixMaxK = 1024;
+constexpr int kCrtMatrixMinRows = 64;
 
 struct CrtArgs {
   const int64_t* res;      // canonical residue rows, stride res_stride

@@ -91,9 +91,14 @@ struct drx_ctx {
   std::vector<uint32_t> primes;
   drx::DBuf d_prime_p, d_prime_pinv, d_prime_e, d_prime_u;
   drx::DBuf d_invs;  // [K][kFdInv + 1] balanced 1/t mod p_k (forward-difference table jumps)
-  std::vector<uint32_t> inv_table;  // [kinv][kinv]: inv_table[j*kinv+i] = p_j^{-1} mod p_i
+  std::vector<uint32_t> inv_table;  // [kinv][kinv]: inv_table[j*kinv+i] = p_j^{-1} mod p_i (kinv <= 256)
   int kinv = 0;
   drx::DBuf d_inv;
+  // wide Garner: w_i = (p_0 ... p_{i-1})^{-1} mod p_i for the first garner_w.size() primes
+  std::vector<double> garner_w;
+  drx::DBuf d_garner_w;
+  // leading primes in (2^30, 2^31): walks take at most this many (the CRT takes any odd primes)
+  int kwalk = 0;
   // programs
   std::vector<drx::HostProgram> progs;
   std::vector<uint32_t> h_code;
@@ -182,6 +187,8 @@ bool is_prime_u32(uint32_t n);
 double now_s();
 int upload_programs(drx_ctx* ctx);
 int ensure_inv_table(drx_ctx* ctx, int k);
+// The wide Garner table for the first k primes (extended on demand).
+int ensure_garner_w(drx_ctx* ctx, int k);
 // The matrix-core CRT table of the first K primes (built on first use).
 int ensure_crt_table(drx_ctx* ctx, int K, const CrtTable** out);
 

@@ -69,7 +69,23 @@ struct GarnerArgs {
   uint64_t inv_ld;
   uint32_t* limbs;        // [n][res_stride] little-endian magnitude of the centred value (zero past K)
   uint8_t* neg;           // [n] 1 if the centred value is negative
+  const double* w;        // wide Garner: w_i = (p_0 ... p_{i-1})^{-1} mod p_i (canonical)
 };
+
+// Wide Garner (rows of any width up to kMaxK): one workgroup per row.
+// Workgroup size and LDS bytes for rows of up to kmax primes.
+inline int garner_wide_threads(int kmax) { return kmax <= 1024 ? 256 : 1024; }
+// Digits (and limbs) per thread: the kernel's template width, 1, 2, 4 or 8.
+inline int garner_wide_regs(int kmax) {
+  const int nt = garner_wide_threads(kmax);
+  int r = 1;
+  while (r * nt < kmax) r *= 2;
+  return r;
+}
+inline size_t garner_wide_lds(int kmax) {
+  const int nt = garner_wide_threads(kmax);
+  return (size_t)8 * nt * garner_wide_regs(kmax) + (size_t)8 * nt + 64;
+}
 
 
 hipError_t launch_walk(int m, int kind, const KArgs& a, int ntask, int threads, size_t lds, hipStream_t s);
@@ -104,6 +120,8 @@ hipError_t launch_combine(int m, const KArgs& a, int nspan, hipStream_t s);
 hipError_t launch_shadow(int m, const KArgs& a, int nspan, int finalize, int64_t w0, int64_t w1, hipStream_t s);
 hipError_t launch_finalize(const KArgs& a, int nspan, int full, hipStream_t s);
 hipError_t launch_garner(const GarnerArgs& g, hipStream_t s);
+// Rows of up to g.K primes (g.K <= kMaxK), one workgroup per row; needs g.w.
+hipError_t launch_garner_wide(const GarnerArgs& g, hipStream_t s);
 // Top-k walk indices by value (larger first, NaN last, ties to the lower
 // index). `ws` holds topk_workspace_bytes(n, k) bytes (unused when 0).
 struct TopCand {

@@ -1440,6 +1440,155 @@ __global__ void __launch_bounds__(256) k_garner_wave(const GarnerArgs g) {
   if (lane == 0) g.neg[row] = (uint8_t)neg;
 }
 
+// ------------------------------------------------------- wide Garner CRT
+// Rows of any width up to kMaxK: one workgroup of NT threads per row, digit
+// i in thread i % NT, register i / NT. The mixed-radix digits need no
+// pairwise inverse table: digit i carries u_i = r_i - (x mod p_i) and
+// Q_i = p_0 ... p_{j-1} mod p_i; once digit j is final
+// (v_j = u_j w_j, w_j = (p_0 ... p_{j-1})^{-1} mod p_j) every later digit
+// takes u_i -= v_j Q_i and Q_i *= p_j. One barrier per digit, the next
+// digit's owner publishing it before the barrier. Sign and negation as in
+// the wavefront kernel, by block reductions. Horner
+// x = v_0 + p_0 (v_1 + p_1 (...)) runs on limbs held redundantly: with
+// every limb below 2^33, lo(L_t p) + hi(L_{t-1} p) is again below 2^33, so a
+// step needs only the neighbouring thread's top high word and no carry
+// propagation; one thread normalises the limbs at the end.
+template <int R>
+__global__ void __launch_bounds__(1024) k_garner_wide(const GarnerArgs g) {
+  extern __shared__ unsigned char gw_lds[];
+  const int NT = (int)blockDim.x, tid = (int)threadIdx.x;
+  uint64_t* lb = reinterpret_cast<uint64_t*>(gw_lds);   // [NT * R] limbs, written last
+  uint32_t* dig = reinterpret_cast<uint32_t*>(gw_lds);  // [K] digits, overlaid by lb
+  uint32_t* xch = reinterpret_cast<uint32_t*>(gw_lds + (size_t)8 * NT * R);  // [2][NT]
+  double* vb = reinterpret_cast<double*>(gw_lds + (size_t)8 * NT * R + (size_t)8 * NT);  // [2]
+  int* rd = reinterpret_cast<int*>(vb + 2);  // [3]: top digit, sign, lowest nonzero digit
+  const int row = g.rows ? g.rows[blockIdx.x] : (int)blockIdx.x;
+  const int K = g.row_k ? g.row_k[row] : g.K;
+  const int64_t* r = g.res + (uint64_t)row * g.res_stride;
+  double u[R], q[R], pd[R], pv[R];
+  uint32_t v[R];
+#pragma unroll
+  for (int c = 0; c < R; ++c) {
+    const int i = tid + c * NT;
+    const bool in = i < K;
+    pd[c] = in ? g.prime_p[i] : 3.0;
+    pv[c] = in ? g.prime_pinv[i] : 1.0 / 3.0;
+    u[c] = in ? (double)r[i] : 0.0;
+    q[c] = 1.0;
+    v[c] = 0u;
+  }
+  if (tid == 0) {  // digit 0 is r_0 (w_0 = 1)
+    v[0] = (uint32_t)u[0];
+    vb[0] = u[0];
+    rd[0] = -1;
+    rd[1] = 0;
+    rd[2] = K;
+  }
+  __syncthreads();
+  for (int j = 0; j + 1 < K; ++j) {
+    const double vj = vb[j & 1];
+    const double pj = g.prime_p[j];
+    const int nxt = j + 1;
+#pragma unroll
+    for (int c = 0; c < R; ++c) {
+      const int i = tid + c * NT;
+      if (i > j && i < K) {
+        u[c] = red(u[c] - mm(vj, q[c], pd[c], pv[c]), pd[c], pv[c]);
+        q[c] = mm(q[c], pj, pd[c], pv[c]);
+      }
+    }
+    if (nxt % NT == tid) {
+      const int cn = nxt / NT;
+      const double wn = g.w[nxt];
+#pragma unroll
+      for (int c = 0; c < R; ++c)
+        if (c == cn) {
+          const double vn = canon(mm(u[c], wn, pd[c], pv[c]), pd[c], pv[c]);
+          v[c] = (uint32_t)vn;
+          vb[nxt & 1] = vn;
+        }
+    }
+    __syncthreads();
+  }
+  // sign: the most significant digit that differs from (p_i - 1) / 2
+  int top = -1;
+#pragma unroll
+  for (int c = 0; c < R; ++c) {
+    const int i = tid + c * NT;
+    if (i < K && v[c] != (((uint32_t)pd[c] - 1u) >> 1)) top = i;
+  }
+  if (top >= 0) atomicMax(&rd[0], top);
+  __syncthreads();
+  const int tp = rd[0];
+  if (tp >= 0 && tp % NT == tid) {
+#pragma unroll
+    for (int c = 0; c < R; ++c)
+      if (c == tp / NT) rd[1] = v[c] > (((uint32_t)pd[c] - 1u) >> 1) ? 1 : 0;
+  }
+  __syncthreads();
+  const int neg = rd[1];
+  if (neg) {  // digits of M - x = (M - 1 - x) + 1: the +1 ripples through the low zero digits
+    int low = K;
+#pragma unroll
+    for (int c = R - 1; c >= 0; --c) {
+      const int i = tid + c * NT;
+      if (i < K && v[c] != 0u) low = i;
+    }
+    if (low < K) atomicMin(&rd[2], low);
+    __syncthreads();
+    const int z = rd[2];
+#pragma unroll
+    for (int c = 0; c < R; ++c) {
+      const int i = tid + c * NT;
+      const uint32_t pu = (uint32_t)pd[c];
+      if (i < K) v[c] = i < z ? 0u : (i == z ? pu - v[c] : pu - 1u - v[c]);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < R; ++c) {
+    const int i = tid + c * NT;
+    if (i < K) dig[i] = v[c];
+  }
+  __syncthreads();
+  // Horner from the top digit; this thread holds limbs tid R .. tid R + R - 1
+  uint64_t L[R];
+#pragma unroll
+  for (int b = 0; b < R; ++b) L[b] = 0ull;
+  if (tid == 0) L[0] = dig[K - 1];
+  for (int i = K - 2; i >= 0; --i) {
+    const uint64_t p = (uint64_t)g.prime_u[i];
+    const uint32_t vi = dig[i];
+    uint64_t lo[R], hi[R];
+#pragma unroll
+    for (int b = 0; b < R; ++b) {
+      const uint64_t z = L[b] * p;  // L < 2^33, p < 2^31: z < 2^64
+      lo[b] = z & 0xffffffffull;
+      hi[b] = z >> 32;
+    }
+    xch[(i & 1) * NT + tid] = (uint32_t)hi[R - 1];
+    __syncthreads();
+    const uint64_t hin = tid ? (uint64_t)xch[(i & 1) * NT + tid - 1] : (uint64_t)vi;
+    L[0] = lo[0] + hin;
+#pragma unroll
+    for (int b = 1; b < R; ++b) L[b] = lo[b] + hi[b - 1];
+  }
+  __syncthreads();  // every digit read before the limbs overlay them
+#pragma unroll
+  for (int b = 0; b < R; ++b) lb[tid * R + b] = L[b];
+  __syncthreads();
+  uint32_t* limb = g.limbs + (uint64_t)row * g.res_stride;
+  if (tid == 0) {
+    uint64_t carry = 0;
+    for (int t = 0; t < K; ++t) {
+      const uint64_t s = lb[t] + carry;
+      limb[t] = (uint32_t)s;
+      carry = s >> 32;
+    }
+    g.neg[row] = (uint8_t)neg;
+  }
+  for (uint64_t t = (uint64_t)K + tid; t < g.res_stride; t += NT) limb[t] = 0u;
+}
+
 // ------------------------------------------------------------------ top-k
 // Order: larger delta first, NaN last, ties to the lower index. A double
 // maps to a 64-bit key whose unsigned order is the numeric order (-0 and +0
@@ -1641,8 +1790,22 @@ hipError_t launch_garner(const GarnerArgs& g, hipStream_t s) {
   if (g.K <= 64) hipLaunchKernelGGL(k_garner_wave<1>, grid, block, 0, s, g);
   else if (g.K <= 128) hipLaunchKernelGGL(k_garner_wave<2>, grid, block, 0, s, g);
   else if (g.K <= 256) hipLaunchKernelGGL(k_garner_wave<4>, grid, block, 0, s, g);
-  else if (g.K <= kMaxK) hipLaunchKernelGGL(k_garner_wave<kMaxK / 64>, grid, block, 0, s, g);
-  else return hipErrorInvalidValue;
+  else return hipErrorInvalidValue;  // wider rows: launch_garner_wide or the matrix-core CRT
+  return hipGetLastError();
+}
+
+hipError_t launch_garner_wide(const GarnerArgs& g, hipStream_t s) {
+  if (g.n <= 0) return hipSuccess;
+  if (g.K < 1 || g.K > kMaxK || !g.w) return hipErrorInvalidValue;
+  const dim3 grid(g.n), block(garner_wide_threads(g.K));
+  const size_t lds = garner_wide_lds(g.K);
+  switch (garner_wide_regs(g.K)) {
+    case 1: hipLaunchKernelGGL(k_garner_wide<1>, grid, block, lds, s, g); break;
+    case 2: hipLaunchKernelGGL(k_garner_wide<2>, grid, block, lds, s, g); break;
+    case 4: hipLaunchKernelGGL(k_garner_wide<4>, grid, block, lds, s, g); break;
+    case 8: hipLaunchKernelGGL(k_garner_wide<8>, grid, block, lds, s, g); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

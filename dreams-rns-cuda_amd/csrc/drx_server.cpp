@@ -457,6 +457,7 @@ int drx_server_submit(drx_server* s, const drx_walk* w, const int64_t* axis_vals
       return fail(ctx, DRX_EINVAL, "the server runs matrix walks");
     if (w->K < 1 || w->K > s->kslot || w->K > (int)dp.kcap || pow2_ceil(w->K) > 256)
       return fail(ctx, DRX_EINVAL, "bad K=%d", w->K);
+    if (w->K > ctx->kwalk) return fail(ctx, DRX_ERANGE, "K=%d: walks need primes in (2^30, 2^31)", w->K);
     if (w->depth < 0 || w->depth > ((int64_t)1 << 30)) return fail(ctx, DRX_EINVAL, "bad depth");
     if (w->init_kind == DRX_INIT_GIVEN) return fail(ctx, DRX_EINVAL, "server walks take PCF or identity initial matrices");
     if (w->init_kind == DRX_INIT_PCF && dp.m != 2) return fail(ctx, DRX_EINVAL, "PCF initial matrix needs m=2");

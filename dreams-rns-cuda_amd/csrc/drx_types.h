@@ -7,7 +7,7 @@
 namespace drx {
 
 constexpr int kMaxM = 4;          // matrix order handled by the kernels
-constexpr int kMaxK = 1024;       // primes in one context
+constexpr int kMaxK = 8192;       // primes in one context
 constexpr int kMaxDim = 15;       // axes per program (FP64 bank X: axes 0..14, step 15)
 constexpr int kMaxConst = 1023;   // constants per program (10-bit field)
 constexpr int kMaxCode = 1 << 16; // device words per program

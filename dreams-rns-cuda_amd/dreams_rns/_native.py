@@ -18,7 +18,7 @@ from ._build import LIB_PATH
 
 ABI_VERSION = 3
 PACK_META = 13  # values drx_program_pack reports
-MAX_K = 1024  # primes one context holds (walks of more than 256 primes run as prime slices)
+MAX_K = 8192  # primes one context holds (walks of more than 256 primes run as prime slices)
 DRX_INIT_PCF, DRX_INIT_IDENTITY, DRX_INIT_GIVEN = 0, 1, 2
 DRX_WALK_MATRIX, DRX_WALK_VECTOR = 0, 1
 MODE_BATCHED, MODE_PLAIN, MODE_PERSISTENT = 0, 1, 2
@@ -164,7 +164,7 @@ class Context:
         self.h = h
         self.device = device
         self.primes = np.zeros(0, dtype=np.int64)
-        self._programs: Dict[tuple, int] = {}
+        self._programs: Dict[tuple, Tuple[int, int]] = {}  # key -> (program id, primes its constants cover)
         self._lock = threading.RLock()
         self.ensure_primes(n_primes)
 
@@ -196,24 +196,30 @@ class Context:
             pr = generate_rns_primes(want)
             if len(pr) < K:
                 raise ValueError(f"only {len(pr)} primes in (2^30, 2^31) are available")
-            if len(self.primes) and self._programs:
-                # constant tables cover the old table only: re-upload lazily
-                self._check(self.lib.drx_program_release_all(self.h))
-                self._programs.clear()
+            # an extension keeps the loaded programs (each covers its own prefix of primes)
             arr = np.ascontiguousarray(pr, dtype=np.int64)
             self._check(self.lib.drx_set_primes(self.h, _ptr(arr), len(arr)))
             self.primes = arr
 
     # -- programs
-    def program_id(self, program, mode: int = DRX_WALK_MATRIX) -> int:
+    def program_id(self, program, mode: int = DRX_WALK_MATRIX, K: int = 0) -> int:
         """Upload ``program`` once per packing mode (DRX_WALK_MATRIX: the
-        code computes M(s); DRX_WALK_VECTOR: it computes v' = M(s) v)."""
+        code computes M(s); DRX_WALK_VECTOR: it computes v' = M(s) v), with
+        constant residues for at least the first ``K`` primes (a power of two
+        of at least 256 primes, so that walks of a few more primes reuse it);
+        a wider walk later uploads it again."""
         key = (program.m, program.dim, tuple((int(i.op), i.arg0, i.arg1, i.arg2) for i in program.opcodes),
                tuple(int(c) for c in program.constants), int(mode))
         with self._lock:
-            pid = self._programs.get(key)
-            if pid is not None:
-                return pid
+            hit = self._programs.get(key)
+            if hit is not None and hit[1] >= K:
+                return hit[0]
+            kcap = 256
+            while kcap < K:
+                kcap *= 2
+            kcap = min(kcap, len(self.primes))
+            if kcap < K:
+                raise ValueError(f"K={K} above the {len(self.primes)} primes set")
             ops = np.array(key[2], dtype=np.uint32).reshape(-1, 4) if key[2] else np.zeros((0, 4), np.uint32)
             ops = np.ascontiguousarray(ops)
             consts = [int(c) for c in program.constants]
@@ -222,16 +228,16 @@ class Context:
             if all(-(1 << 63) <= c < (1 << 63) for c in consts):
                 carr = np.array(consts, dtype=np.int64) if nc else np.zeros(1, np.int64)
                 rc = self.lib.drx_program_upload_mode(self.h, _ptr(ops), len(ops), program.m, program.dim,
-                                                      _ptr(carr), None, None, nc, 0, int(mode), C.byref(out))
+                                                      _ptr(carr), None, None, nc, kcap, int(mode), C.byref(out))
             else:
-                pr = [int(p) for p in self.primes]
+                pr = [int(p) for p in self.primes[:kcap]]
                 res = np.array([[c % p for p in pr] for c in consts], dtype=np.uint32)
                 flt = np.array([float(c) for c in consts], dtype=np.float64)
                 rc = self.lib.drx_program_upload_mode(self.h, _ptr(ops), len(ops), program.m, program.dim,
-                                                      None, _ptr(res), _ptr(flt), nc, len(pr), int(mode),
+                                                      None, _ptr(res), _ptr(flt), nc, kcap, int(mode),
                                                       C.byref(out))
             self._check(rc)
-            self._programs[key] = out.value
+            self._programs[key] = (out.value, kcap)
             return out.value
 
     def program_info(self, pid: int) -> Tuple[int, int, int]:

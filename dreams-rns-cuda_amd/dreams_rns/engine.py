@@ -184,7 +184,7 @@ def _pack_walks(specs: Sequence[WalkSpec], ctx: N.Context):
     res_off = 0
     for i, s in enumerate(specs):
         w = walks[i]
-        w["prog"] = ctx.program_id(s.program, N.DRX_WALK_VECTOR if s.vector else N.DRX_WALK_MATRIX)
+        w["prog"] = ctx.program_id(s.program, N.DRX_WALK_VECTOR if s.vector else N.DRX_WALK_MATRIX, int(s.K))
         w["K"] = s.K
         w["depth"] = s.depth
         w["target"] = s.target

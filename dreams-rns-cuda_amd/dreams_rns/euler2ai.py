@@ -133,8 +133,9 @@ def verify_pcf_records(records: Sequence[Dict[str, Any]], depth: int = 2000, K: 
     elapsed_s}. K = 0 takes ``estimate_K`` per PCF. A PCF that does not
     compile (None: imaginary) gives SKIP rows; one whose compilation raises
     gives no row on its first task and SKIP rows after, as in the reference;
-    one whose walk fails (K above the engine's 1024 primes included) gives
-    an ERROR row, then SKIP rows."""
+    one whose walk or measurement fails (K above the engine's 8192 primes
+    included) gives an ERROR row, then SKIP rows; the other PCFs of its
+    device batch are unaffected."""
     t0 = time.perf_counter()
     tasks = expand_tasks(records, max_tasks)
     pcfs: Dict[Tuple[str, str], Dict[str, Any]] = {}
@@ -157,17 +158,31 @@ def verify_pcf_records(records: Sequence[Dict[str, Any]], depth: int = 2000, K: 
             entry["state"], entry["err"] = "compile_error", str(e)
         pcfs[key] = entry
     todo = [k for k, e in pcfs.items() if e["state"] == "ok"]
+
+    def walk_and_measure(keys):
+        # one device batch; a PCF whose measurement raises fails alone
+        specs = [pcf_spec(pcfs[k]["prog"], pcfs[k]["a0"], depth, pcfs[k]["K"], 1) for k in keys]
+        with BatchPlan(specs, crt=True) as plan:
+            res = plan.run()
+            for i, k in enumerate(keys):
+                try:
+                    pcfs[k]["result"] = _measure(pcfs[k], res, i, depth, dps, delta_tol)
+                except Exception as e:
+                    pcfs[k]["state"], pcfs[k]["err"] = "walk_error", str(e)
+
     for b0 in range(0, len(todo), max_walks):
         chunk = todo[b0:b0 + max_walks]
         try:
-            specs = [pcf_spec(pcfs[k]["prog"], pcfs[k]["a0"], depth, pcfs[k]["K"], 1) for k in chunk]
-            with BatchPlan(specs, crt=True) as plan:
-                res = plan.run()
-                for i, k in enumerate(chunk):
-                    pcfs[k]["result"] = _measure(pcfs[k], res, i, depth, dps, delta_tol)
-        except Exception as e:  # the batch's walks fail together
+            walk_and_measure(chunk)
+        except Exception:
+            # the batch was refused or failed as a whole: walk its PCFs one at
+            # a time so that only the failing ones get the error row, as the
+            # reference's per-PCF loop does (euler2ai_full_verify.py:380-394)
             for k in chunk:
-                pcfs[k]["state"], pcfs[k]["err"] = "walk_error", str(e)
+                try:
+                    walk_and_measure([k])
+                except Exception as e:
+                    pcfs[k]["state"], pcfs[k]["err"] = "walk_error", str(e)
     rows: List[Dict[str, Any]] = []
     seen = set()
     n_total = n_match = n_skip = n_error = 0

@@ -43,18 +43,26 @@ _crt_ctx = {}
 
 def _ctx_for_primes(primes) -> N.Context:
     """The default context when ``primes`` is a prefix of its prime table,
-    otherwise a cached context holding exactly these primes."""
+    otherwise a cached context holding exactly these primes (any distinct
+    odd primes below 2^31)."""
+    from .primes import is_prime
+
     pr = np.asarray([int(p) for p in primes], dtype=np.int64)
     ctx = N.context()
-    ctx.ensure_primes(len(pr))
-    if np.array_equal(ctx.primes[: len(pr)], pr):
-        return ctx
+    if len(pr) <= N.MAX_K and all((1 << 30) < int(p) < (1 << 31) for p in pr):
+        ctx.ensure_primes(len(pr))
+        if np.array_equal(ctx.primes[: len(pr)], pr):
+            return ctx
     key = tuple(int(p) for p in pr)
     c = _crt_ctx.get(key)
     if c is None:
+        if len(key) > N.MAX_K:
+            raise ValueError(f"{len(key)} primes: the engine's CRT takes at most {N.MAX_K}")
         for p in key:
-            if not (1 << 30) < p < (1 << 31):
-                raise ValueError(f"prime {p} outside (2^30, 2^31): not an RNS prime of this engine")
+            if not (2 < p < (1 << 31) and is_prime(p)):
+                raise ValueError(f"modulus {p} is not an odd prime below 2^31: not an RNS prime of this engine")
+        if len(set(key)) != len(key):
+            raise ValueError("repeated prime: the moduli are not coprime")
         c = N.Context.__new__(N.Context)
         N.Context.__init__(c, ctx.device, n_primes=0)
         arr = np.ascontiguousarray(pr)
@@ -67,19 +75,26 @@ def _ctx_for_primes(primes) -> N.Context:
 
 
 def crt_reconstruct(residues, primes) -> Tuple[int, int]:
-    """x in [0, M) with x = r_i mod p_i, and M = prod(p_i) (runner.py:57-76).
+    """x with x = r_i mod p_i, and M = prod(p_i), over the first
+    len(residues) primes (runner.py:57-76): an IndexError when there are
+    fewer primes than residues or no residue at all, extra primes unused.
+    Like the reference, x starts from the first residue as given, so x is
+    in [0, M) for canonical residues and r_0 + ((x - r_0) mod M) in general.
 
-    The CRT runs on the GPU (drx_crt: Garner's mixed radix up to 256 primes,
-    the matrix-core CRT above); the host only joins the returned limbs into
-    a Python int.
+    The CRT runs on the GPU (drx_crt: Garner's mixed radix, or the matrix-core
+    CRT for many wide rows); the host only joins the returned limbs into a
+    Python int. The moduli must be distinct odd primes below 2^31
+    (ValueError otherwise; the reference's Fermat inverse assumes primes).
     """
-    pr = [int(p) for p in primes]
-    K = len(pr)
+    res_in = [int(r) for r in residues]
+    K = len(res_in)
     if K == 0:
         raise IndexError("empty residue list")
-    res = np.array([int(r) % p for r, p in zip(residues, pr)], dtype=np.int64).reshape(1, K)
-    if res.shape[1] != K:
-        raise IndexError("residues and primes differ in length")
+    pr = [int(p) for p in primes]
+    if len(pr) < K:
+        raise IndexError(f"{K} residues but only {len(pr)} primes")
+    pr = pr[:K]
+    res = np.array([r % p for r, p in zip(res_in, pr)], dtype=np.int64).reshape(1, K)
     ctx = _ctx_for_primes(pr)
     limbs = np.zeros((1, K), dtype=np.uint32)
     neg = np.zeros(1, dtype=np.uint8)
@@ -87,7 +102,8 @@ def crt_reconstruct(residues, primes) -> Tuple[int, int]:
     M = math.prod(pr)
     mag = int.from_bytes(limbs.tobytes(), "little")
     x = M - mag if neg[0] else mag
-    return x, M
+    r0 = res_in[0]
+    return r0 + (x - r0) % M, M
 
 
 def centered(x: int, M: int) -> int:

@@ -37,7 +37,7 @@ class WalkServer:
         if spec.init not in ("pcf", "identity"):
             raise ValueError("the server takes PCF or identity initial matrices")
         w = N.Walk()
-        w.prog = self.ctx.program_id(spec.program)
+        w.prog = self.ctx.program_id(spec.program, N.DRX_WALK_MATRIX, int(spec.K))
         w.K = spec.K
         w.depth = spec.depth
         w.init_kind = N.DRX_INIT_PCF if spec.init == "pcf" else N.DRX_INIT_IDENTITY

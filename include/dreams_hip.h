@@ -41,9 +41,12 @@ const char* drx_last_error(const drx_ctx* ctx);
 
 /* Prime table shared by every walk of the context; a walk with K primes uses
  * the first K. Replaces generate_rns_primes (python/dreams_rns/runner.py:33-52),
- * whose output the caller passes here (odd primes in (2^30, 2^31)). Extending
- * the table keeps existing plans valid; any other change (allowed only while
- * no programs are loaded) makes them fail with DRX_ESTATE. */
+ * whose output the caller passes here (distinct odd primes below 2^31, at most
+ * 8192). Walks need their primes in (2^30, 2^31): a walk may use only the
+ * leading primes of the table that lie there (DRX_ERANGE otherwise); the CRT
+ * takes any of them, as crt_reconstruct (runner.py:57-76) takes any primes.
+ * Extending the table keeps existing plans valid; any other change (allowed
+ * only while no programs are loaded) makes them fail with DRX_ESTATE. */
 int drx_set_primes(drx_ctx* ctx, const int64_t* primes, int K);
 
 /* ---- programs ----------------------------------------------------------
@@ -52,7 +55,9 @@ int drx_set_primes(drx_ctx* ctx, const int64_t* primes, int K);
  * exactly Instruction.pack (compiler.py:39-41). Constants are given either as
  * int64 values (`consts`, reduced here) or, for constants that do not fit,
  * pre-reduced: `const_res` [n_const][kcap] canonical residues for the first
- * kcap primes plus `const_f` [n_const] FP64 values (Python float(c)).
+ * kcap primes plus `const_f` [n_const] FP64 values (Python float(c)). Walks
+ * of the program may use the first kcap primes (int64 constants with
+ * kcap <= 0: every prime set at upload).
  * Replaces _precompute_const_residues (runner.py:182-195). */
 int drx_program_upload(drx_ctx* ctx, const uint32_t* ops, int n_ops, int m, int dim,
                        const int64_t* consts, const uint32_t* const_res, const double* const_f,
@@ -107,10 +112,9 @@ enum { DRX_MODE_BATCHED = 0, DRX_MODE_PLAIN = 1, DRX_MODE_PERSISTENT = 2 };
  * v[out_p], q = v[out_q]; its program is uploaded with mode DRX_WALK_VECTOR,
  * its initial state given (DRX_INIT_GIVEN: [m][K] residues and [m] FP64
  * entries). A batch holds walks of one mode.
- * K: a matrix walk takes up to 1024 primes (above 256 it runs as prime
- * slices of 256 on the interpreting kernel: the primes are independent, the
- * results are those of one walk), a vector walk up to 256, a served walk up
- * to 256. */
+ * K: a matrix walk takes up to 8192 primes (above 256 it runs as prime
+ * slices of 256: the primes are independent, the results are those of one
+ * walk), a vector walk up to 256, a served walk up to 256. */
 enum { DRX_WALK_MATRIX = 0, DRX_WALK_VECTOR = 1 };
 typedef struct drx_walk {
   int32_t prog;
@@ -210,9 +214,11 @@ int drx_autotune(drx_ctx* ctx, const drx_batch* b, const drx_launch_cfg* base, i
 
 /* CRT of n rows of K canonical residues (row stride `stride`) over the
  * first K context primes: magnitude limbs (little-endian uint32, K per row)
- * and sign of centered(x, M). Rows of up to 256 primes take a Garner
- * kernel, wider rows (up to 1024) the matrix-core CRT. Replaces
- * crt_reconstruct + centered (runner.py:57-81). */
+ * and sign of centered(x, M). Rows of up to 256 primes take the wavefront
+ * Garner kernel; wider rows (up to 8192 primes) the matrix-core CRT when at
+ * least 64 rows of at most 1024 primes share the call, else the wide Garner
+ * kernel (one workgroup per row). Replaces crt_reconstruct + centered
+ * (runner.py:57-81). */
 int drx_crt(drx_ctx* ctx, const int64_t* residues, int n, int K, int stride, uint32_t* limbs,
             uint8_t* neg);
 

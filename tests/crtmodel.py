@@ -84,3 +84,68 @@ def crt_rows(res: np.ndarray, primes) -> List[int]:
         assert -H <= x <= H
         out.append(x)
     return out
+
+
+def wide_garner_w(primes) -> List[int]:
+    """w_i = (p_0 ... p_{i-1})^-1 mod p_i, the table of the wide Garner kernel
+    (w_0 = 1)."""
+    pr = [int(p) for p in primes]
+    out = []
+    for i, p in enumerate(pr):
+        prod = 1
+        for q in pr[:i]:
+            prod = prod * q % p
+        out.append(pow(prod, p - 2, p))
+    return out
+
+
+def wide_garner_row(res_row, primes, NT: int, R: int) -> Tuple[int, int]:
+    """(magnitude, negative) of the centred CRT of one row, computed the way
+    k_garner_wide does: digit i in thread i % NT, register i / NT; the
+    u/Q recurrence for the mixed-radix digits; sign and negation from the
+    digits; Horner on redundant limbs (thread t holds limbs t R .. t R + R - 1,
+    each kept below 2^33, one neighbour word per step); final carry pass."""
+    pr = [int(p) for p in primes]
+    K = len(pr)
+    assert K <= NT * R
+    w = wide_garner_w(pr)
+    u = [int(res_row[i]) for i in range(K)]
+    Q = [1] * K
+    v = [0] * K
+    v[0] = u[0] % pr[0]
+    for j in range(K - 1):
+        for i in range(j + 1, K):  # every later digit, in whichever thread holds it
+            u[i] = (u[i] - v[j] * Q[i]) % pr[i]
+            Q[i] = Q[i] * pr[j] % pr[i]
+        v[j + 1] = u[j + 1] * w[j + 1] % pr[j + 1]
+    h = [(p - 1) // 2 for p in pr]
+    diff = [i for i in range(K) if v[i] != h[i]]
+    neg = bool(diff) and v[diff[-1]] > h[diff[-1]]
+    if neg:
+        z = min(i for i in range(K) if v[i] != 0)
+        v = [0 if i < z else (pr[i] - v[i] if i == z else pr[i] - 1 - v[i]) for i in range(K)]
+    NL = NT * R
+    L = [0] * NL
+    L[0] = v[K - 1]
+    for i in range(K - 2, -1, -1):
+        z = [L[t] * pr[i] for t in range(NL)]
+        assert max(z) < 1 << 64
+        lo = [x & 0xFFFFFFFF for x in z]
+        hi = [x >> 32 for x in z]
+        assert hi[NL - 1] == 0  # the dropped top word is always zero
+        new = []
+        for t in range(NL):
+            tid, b = divmod(t, R)
+            if b == 0:  # the neighbouring thread's top word (thread 0: the digit)
+                new.append(lo[t] + (hi[t - 1] if tid else v[i]))
+            else:
+                new.append(lo[t] + hi[t - 1])
+        L = new
+        assert max(L) < 1 << 33
+    carry, mag = 0, 0
+    for t in range(K):
+        s = L[t] + carry
+        mag |= (s & 0xFFFFFFFF) << (32 * t)
+        carry = s >> 32
+    assert carry == 0 and all(x == 0 for x in L[K:])
+    return mag, neg

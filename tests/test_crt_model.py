@@ -38,3 +38,26 @@ def test_model_equals_exact_crt():
         # random residue rows (any x in [0, M))
         res = np.array([[rng.randrange(int(p)) for p in primes] for _ in range(30)], dtype=np.int64)
         assert cm.crt_rows(res, primes) == [_centered_crt(r, primes) for r in res]
+
+
+def test_wide_garner_model_equals_exact_crt():
+    # the wide Garner kernel's steps (tests/crtmodel.py wide_garner_row) on
+    # layouts with one and several digits per thread, against exact CRT;
+    # small odd primes included (the CRT accepts any odd primes below 2^31)
+    rng = random.Random(11)
+    cases = [(ow.generate_rns_primes(K), nt, r) for K, nt, r in ((1, 4, 1), (5, 4, 2), (13, 4, 4), (31, 8, 4), (40, 16, 4))]
+    cases.append(([3, 5, 7, 11, 13, 101, 65537], 2, 4))
+    for primes, nt, r in cases:
+        pr = [int(p) for p in primes]
+        M = 1
+        for p in pr:
+            M *= p
+        H = (M - 1) // 2
+        vals = [0, 1, -1, H, -H, H - 1, -(H - 1), M // 3, -(M // 7)] + [rng.randrange(-H, H + 1) for _ in range(20)]
+        for v in vals:
+            mag, neg = cm.wide_garner_row([v % p for p in pr], pr, nt, r)
+            assert (-mag if neg else mag) == v, (pr[:3], nt, r, v)
+        for _ in range(10):
+            row = [rng.randrange(p) for p in pr]
+            mag, neg = cm.wide_garner_row(row, pr, nt, r)
+            assert (-mag if neg else mag) == _centered_crt(row, pr)

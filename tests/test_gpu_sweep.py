@@ -97,11 +97,19 @@ def test_prime_slices_match_c_oracle(gpu_ctx, sweep_fix, K):
 
 
 def test_more_primes_than_one_context_holds_is_refused(gpu_ctx, sweep_fix):
-    # 1024 primes is the limit of a matrix walk: one more is refused before
-    # anything runs, with the reference's exception type for bad arguments
+    # 8192 primes is the limit of a matrix walk: 1025 runs (five prime slices)
+    # and equals the C oracle, one more than 8192 is refused before anything
+    # runs, with the reference's exception type for bad arguments
     prog = compile_cmf_spec(sweep_fix["specs"][5], [1, 1, 1, 1, 1])
-    with pytest.raises(ValueError, match="1025"):
-        BatchPlan([cmf_spec(prog, 10, 1025, [1, 1, 1, 1, 1])])
+    primes = D.generate_rns_primes(1025)
+    with BatchPlan([cmf_spec(prog, 10, 1025, [1, 1, 1, 1, 1])], full=True, crt=True) as plan:
+        res = plan.run()
+    rr, _, _ = cref.walk(prog, 10, primes, _axes([1, 1, 1, 1, 1], [1, 1, 1, 1, 1]))
+    assert np.array_equal(res.full_matrix(0, 4), rr)
+    xq, M = ow.crt_reconstruct(rr[3, 3].tolist(), primes.tolist())
+    assert res.crt_values(0)[1] == ow.centered(xq, M)
+    with pytest.raises(ValueError, match="8193"):
+        BatchPlan([cmf_spec(prog, 10, 8193, [1, 1, 1, 1, 1])])
 
 
 def test_mixed_widths_in_one_batch(gpu_ctx, sweep_fix):
