@@ -1378,6 +1378,10 @@ int drx_ctx_create(int device, drx_ctx** out) {
       return DRX_EHIP;
     }
   }
+  if (noalloc_init(ctx) != DRX_OK) {
+    drx_ctx_destroy(ctx);
+    return DRX_EHIP;
+  }
   *out = ctx;
   return DRX_OK;
 }
@@ -1466,7 +1470,11 @@ int drx_set_primes(drx_ctx* ctx, const int64_t* primes, int K) {
   ctx->primes.swap(pr);
   ctx->kinv = 0;
   ctx->kwalk = kwalk;
-  return DRX_OK;
+  // both Garner tables now, so that no CRT call (drx_crt_noalloc above all)
+  // has to build or allocate them
+  int rc = ensure_inv_table(ctx, std::min(K, kCrtMatrixMinK - 1));
+  if (!rc) rc = ensure_garner_w(ctx, K);
+  return rc;
 }
 
 int drx_program_upload(drx_ctx* ctx, const uint32_t* ops, int n_ops, int m, int dim, const int64_t* consts,

@@ -99,6 +99,8 @@ struct drx_ctx {
   drx::DBuf d_garner_w;
   // leading primes in (2^30, 2^31): walks take at most this many (the CRT takes any odd primes)
   int kwalk = 0;
+  // device address of the allocation-free CRT's staging array on this device (drx_noalloc.hip)
+  void* scratch = nullptr;
   // programs
   std::vector<drx::HostProgram> progs;
   std::vector<uint32_t> h_code;
@@ -189,6 +191,9 @@ int upload_programs(drx_ctx* ctx);
 int ensure_inv_table(drx_ctx* ctx, int k);
 // The wide Garner table for the first k primes (extended on demand).
 int ensure_garner_w(drx_ctx* ctx, int k);
+// Looks up the allocation-free CRT's device scratch for ctx->device (the
+// device must be current); called once by drx_ctx_create.
+int noalloc_init(drx_ctx* ctx);
 // The matrix-core CRT table of the first K primes (built on first use).
 int ensure_crt_table(drx_ctx* ctx, int K, const CrtTable** out);
 

@@ -17,7 +17,7 @@ CSRC = os.path.join(os.path.dirname(PKG_DIR), "csrc")
 LIB_PATH = os.path.join(PKG_DIR, "libdreams_hip.so")
 BUILD_DIR = os.path.join(os.path.dirname(PKG_DIR), "_build")
 
-SOURCES = ["drx_kernels.hip", "drx_crt.hip", "drx_api.cpp", "drx_pack.cpp", "drx_server.cpp"]
+SOURCES = ["drx_kernels.hip", "drx_crt.hip", "drx_noalloc.hip", "drx_api.cpp", "drx_pack.cpp", "drx_server.cpp"]
 ARCH = os.environ.get("DRX_OFFLOAD_ARCH", "gfx950")
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",

@@ -106,6 +106,7 @@ SIGNATURES = {
     "drx_plan_stats": (C.c_int, [C.c_void_p, C.c_void_p]),
     "drx_plan_destroy": (None, [C.c_void_p]),
     "drx_crt": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    "drx_crt_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "drx_topk": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     "drx_copy_calibrate": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p]),
     "drx_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_void_p]),

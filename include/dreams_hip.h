@@ -222,6 +222,18 @@ int drx_autotune(drx_ctx* ctx, const drx_batch* b, const drx_launch_cfg* base, i
 int drx_crt(drx_ctx* ctx, const int64_t* residues, int n, int K, int stride, uint32_t* limbs,
             uint8_t* neg);
 
+/* drx_crt without any device allocation. No counterpart in the reference:
+ * it is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). Same inputs, outputs and
+ * results as drx_crt; every row takes a Garner kernel. Its device memory is
+ * a fixed 8 MiB staging array per device, held until the process ends:
+ * n * stride may be at most about 699000 residues (DRX_ERANGE above).
+ * Calls that overlap on one device share that array and race; calls on one
+ * context are serialised. Page-locked host buffers (drx_host_alloc) are
+ * copied directly, pageable ones through the HIP runtime's staging. */
+int drx_crt_noalloc(drx_ctx* ctx, const int64_t* residues, int n, int K, int stride, uint32_t* limbs,
+                    uint8_t* neg);
+
 /* Indices of the k largest values (ties: lower index first; NaN last).
  * No reference counterpart in Python; the reference CUDA sketch is
  * k_topk_select (src/cuda/persistent_kernel.cu:796-841). */
