@@ -41,6 +41,9 @@ struct KArgs {
   const DevVGroup* vgroups;  // vector walks: residue groups
   const DevVGroup* fgroups;  // vector walks: FP64 groups (one lane per walk)
   const int32_t* vorder;     // vector walks: walk indices grouped
+  double* axr_w;             // server admission: writable views of axr / axi / axd
+  int64_t* axi_w;
+  double* axd_w;
 };
 
 // FP64 instructions per scalar code load: the M_f interpreter reads its
@@ -119,6 +122,10 @@ hipError_t launch_combine(int m, const KArgs& a, int nspan, hipStream_t s);
 // fst_in, later windows continue from fst); delta_float when `finalize`.
 hipError_t launch_shadow(int m, const KArgs& a, int nspan, int finalize, int64_t w0, int64_t w1, hipStream_t s);
 hipError_t launch_finalize(const KArgs& a, int nspan, int full, hipStream_t s);
+// Server admission: initial residues (canonical, as the host tables), FP64
+// state and axis tables of every admitted walk, written into its slot
+// (a.walks[rec.slot] gives the offsets). One block per record.
+hipError_t launch_admit(const KArgs& a, const DevAdmit* recs, int n, hipStream_t s);
 hipError_t launch_garner(const GarnerArgs& g, hipStream_t s);
 // Rows of up to g.K primes (g.K <= kMaxK), one workgroup per row; needs g.w.
 hipError_t launch_garner_wide(const GarnerArgs& g, hipStream_t s);

@@ -1,6 +1,7 @@
 // Walk kernels for gfx950: bytecode interpreter over residue channels, chunked
 // P*M products with an ordered LDS tree combine, FP64 shadow, finalisation,
 // Garner CRT and top-k.
+#include "../../include/dreams_hip.h"
 #include "drx_kernels.h"
 
 #include "drx_device.h"
@@ -1282,6 +1283,57 @@ __global__ void k_finalize(const KArgs a, int full) {
   }
 }
 
+// --------------------------------------------------------------- admission
+// The host tables of a newly served walk (init_tables / axis_tables in the
+// library's host code) computed on the device from its record: identity or
+// PCF initial matrix (canonical residues), FP64 state, and per axis the
+// balanced residues of c0, c1 and den^-1 (0 when p divides den, Fermat), the
+// int64 c0, c1 and the FP64 den.
+__device__ __forceinline__ uint32_t mod_i64_dev(int64_t c, uint32_t p) {
+  const int64_t r = c % (int64_t)p;
+  return (uint32_t)(r < 0 ? r + (int64_t)p : r);
+}
+
+__device__ __forceinline__ double balanced_dev(uint32_t r, uint32_t p) {
+  return r > p / 2 ? (double)r - (double)p : (double)r;
+}
+
+__global__ void __launch_bounds__(256) k_admit(const KArgs a, const DevAdmit* __restrict__ recs) {
+  const DevAdmit& r = recs[blockIdx.x];
+  const DevWalk w = a.walks[r.slot];
+  const uint32_t K = (uint32_t)w.K, m = (uint32_t)w.m, dim = (uint32_t)w.dim;
+  double* res = a.res + w.res_off;
+  double* axr = a.axr_w + w.axr_off;
+  for (uint32_t k = threadIdx.x; k < K; k += blockDim.x) {
+    const uint32_t p = (uint32_t)a.prime_p[k];
+    const double pd = a.prime_p[k], pinv = a.prime_pinv[k];
+    for (uint32_t e = 0; e < m * m; ++e) res[(uint64_t)e * K + k] = (e % (m + 1) == 0) ? 1.0 : 0.0;
+    if (r.init_kind == DRX_INIT_PCF) res[K + k] = (double)mod_i64_dev(r.a0, p);
+    for (uint32_t ax = 0; ax < dim; ++ax) {
+      const uint32_t dr = mod_i64_dev(r.den[ax], p);
+      double inv = 1.0;
+      if (dr != 1u) inv = canon(inv_fermat((double)dr, pd, pinv, a.prime_e[k]), pd, pinv);
+      double* base = axr + (uint64_t)ax * 3 * K;
+      base[k] = balanced_dev(mod_i64_dev(r.c0[ax], p), p);
+      base[K + k] = balanced_dev(mod_i64_dev(r.c1[ax], p), p);
+      base[2 * K + k] = balanced_dev((uint32_t)inv, p);
+    }
+  }
+  if (threadIdx.x == 0) {
+    double* f = a.fst + w.fst_off;
+    for (uint32_t e = 0; e < m * m; ++e) f[e] = (e % (m + 1) == 0) ? 1.0 : 0.0;
+    if (r.init_kind == DRX_INIT_PCF) f[1] = (double)r.a0;
+    f[m * m] = 1.0;  // log_scale = ln(1.0) + 0 * ln 2
+    f[m * m + 1] = 0.0;
+  }
+  if (threadIdx.x < dim) {
+    const uint32_t ax = threadIdx.x;
+    a.axi_w[w.axi_off + 2 * ax] = r.c0[ax];
+    a.axi_w[w.axi_off + 2 * ax + 1] = r.c1[ax];
+    a.axd_w[w.axd_off + ax] = (double)r.den[ax];
+  }
+}
+
 // -------------------------------------------------------------- Garner CRT
 // One thread per (walk, value): mixed-radix digits v_i of x in [0, M), the
 // sign of the centred value (x > (M-1)/2 <=> digits compare above
@@ -1775,6 +1827,13 @@ hipError_t launch_shadow(int m, const KArgs& a, int nspan, int finalize, int64_t
   if (m == 3) hipLaunchKernelGGL(k_shadow<3>, dim3(nb), dim3(tb), 0, s, a, nspan, finalize, w0, w1);
   else if (m == 4) hipLaunchKernelGGL(k_shadow<4>, dim3(nb), dim3(tb), 0, s, a, nspan, finalize, w0, w1);
   else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_admit(const KArgs& a, const DevAdmit* recs, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (!a.axr_w || !a.axi_w || !a.axd_w) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_admit, dim3(n), dim3(256), 0, s, a, recs);
   return hipGetLastError();
 }
 

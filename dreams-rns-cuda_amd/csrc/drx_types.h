@@ -101,6 +101,16 @@ struct DevVGroup {
   int64_t depth;
 };
 
+// A walk admitted into a server slot: its initial matrix and axis tables
+// are built on the device from this record (axis i at step s is
+// (c0 + s c1) / den, the combined rational of cmf_walk.py:206-224).
+struct DevAdmit {
+  int32_t slot;
+  int32_t init_kind;   // DRX_INIT_PCF or DRX_INIT_IDENTITY
+  int64_t a0;          // DRX_INIT_PCF: A = [[1, a0], [0, 1]]
+  int64_t c0[kMaxDim], c1[kMaxDim], den[kMaxDim];
+};
+
 // A block of consecutive steps of one task for the FP64 M_f(s) evaluation.
 struct DevRange {
   int32_t task;

@@ -16,7 +16,7 @@ import numpy as np
 
 from ._build import LIB_PATH
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 PACK_META = 13  # values drx_program_pack reports
 MAX_K = 8192  # primes one context holds (walks of more than 256 primes run as prime slices)
 DRX_INIT_PCF, DRX_INIT_IDENTITY, DRX_INIT_GIVEN = 0, 1, 2
@@ -71,12 +71,13 @@ class LaunchCfg(C.Structure):
 
 class ServerCfg(C.Structure):
     _fields_ = [("max_slots", C.c_int), ("round_steps", C.c_int), ("threads", C.c_int),
-                ("mem_budget", C.c_int64)]
+                ("max_walk_tasks", C.c_int), ("mem_budget", C.c_int64)]
 
 
 class ServerResult(C.Structure):
     _fields_ = [("ticket", C.c_uint64), ("out", WalkOut), ("K", C.c_int32), ("status", C.c_int32),
-                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double)]
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
+                ("neg", C.c_uint8 * 2), ("pad", C.c_uint8 * 6)]
 
 
 # exported symbol -> (restype, argtypes)
@@ -115,7 +116,7 @@ SIGNATURES = {
     "drx_last_timing": (C.c_int, [C.c_void_p, C.c_void_p]),
     "drx_server_create": (C.c_int, [C.c_void_p, C.POINTER(ServerCfg), C.POINTER(C.c_void_p)]),
     "drx_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Walk), C.c_void_p, C.POINTER(C.c_uint64)]),
-    "drx_server_poll": (C.c_int, [C.c_void_p, C.POINTER(ServerResult), C.c_void_p, C.c_int, C.c_int,
+    "drx_server_poll": (C.c_int, [C.c_void_p, C.POINTER(ServerResult), C.c_void_p, C.c_void_p, C.c_int, C.c_int,
                                   C.POINTER(C.c_int)]),
     "drx_server_pending": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "drx_server_capacity": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),

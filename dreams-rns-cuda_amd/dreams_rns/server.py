@@ -2,9 +2,13 @@
 
 Requests (:class:`~dreams_rns.engine.WalkSpec`) are queued; a native thread
 admits them into device-resident slots as slots and memory free up and
-advances every resident walk by ``round_steps`` steps per round, so requests
-that arrive while others run join at the next round and share its launches.
-Each finished request returns the reference's result dict plus timing.
+advances every resident walk by ``round_steps`` steps per round (several such
+step ranges side by side while few walks are resident), so requests that
+arrive while others run join at the next round and share its launches. Each
+finished request returns the reference's result dict, the centred CRT
+integers of p and q (computed on the GPU in the request's last round, as
+the reference's drivers follow each walk with crt_reconstruct + centered,
+python/euler2ai_full_verify.py:331-342) and its timing.
 """
 
 from __future__ import annotations
@@ -21,23 +25,33 @@ from .engine import WalkSpec
 
 class WalkServer:
     def __init__(self, ctx: Optional[N.Context] = None, max_slots: int = 0, round_steps: int = 1000,
-                 threads: int = 0, mem_budget: int = 0, max_K: int = 128):
+                 threads: int = 0, mem_budget: int = 0, max_K: int = 128, max_walk_tasks: int = 0):
         self.ctx = ctx or N.context()
         self.ctx.ensure_primes(max_K)
-        cfg = N.ServerCfg(int(max_slots), int(round_steps), int(threads), int(mem_budget))
+        cfg = N.ServerCfg(int(max_slots), int(round_steps), int(threads), int(max_walk_tasks), int(mem_budget))
         h = C.c_void_p()
         self.ctx._check(self.ctx.lib.drx_server_create(self.ctx.h, C.byref(cfg), C.byref(h)))
         self.h = h
-        self.stride = len(self.ctx.primes)
+        self.stride = min(len(self.ctx.primes), 256)
         self.submitted = 0
         self.completed = 0
         self._pending: Dict[int, WalkSpec] = {}
+        self._pids: Dict[int, tuple] = {}  # id(program) -> (program, program id, K covered)
+        self._cap = 0
+
+    def _program_id(self, program, K: int) -> int:
+        hit = self._pids.get(id(program))
+        if hit is not None and hit[0] is program and hit[2] >= K:
+            return hit[1]
+        pid = self.ctx.program_id(program, N.DRX_WALK_MATRIX, int(K))
+        self._pids[id(program)] = (program, pid, int(K))
+        return pid
 
     def submit(self, spec: WalkSpec) -> int:
         if spec.init not in ("pcf", "identity"):
             raise ValueError("the server takes PCF or identity initial matrices")
         w = N.Walk()
-        w.prog = self.ctx.program_id(spec.program, N.DRX_WALK_MATRIX, int(spec.K))
+        w.prog = self._program_id(spec.program, spec.K)
         w.K = spec.K
         w.depth = spec.depth
         w.init_kind = N.DRX_INIT_PCF if spec.init == "pcf" else N.DRX_INIT_IDENTITY
@@ -50,12 +64,24 @@ class WalkServer:
         self._pending[t.value] = spec
         return t.value
 
+    def _buffers(self, n: int):
+        if n > self._cap:
+            self._cap = max(n, 2 * self._cap, 64)
+            self._res = (N.ServerResult * self._cap)()
+            self._resid = np.zeros((self._cap, 2, self.stride), dtype=np.int64)
+            self._limbs = np.zeros((self._cap, 2, self.stride), dtype=np.uint32)
+        return self._res, self._resid, self._limbs
+
     def poll(self, max_results: int = 4096) -> List[Dict[str, Any]]:
-        res = (N.ServerResult * max_results)()
-        resid = np.zeros((max_results, 2, self.stride), dtype=np.int64)
+        """Finished requests: the reference's walk dict plus ``p_big`` /
+        ``q_big`` (centred CRT integers), ``delta_float``, ``status`` and
+        timestamps (``latency_s`` = submission to CRT integers in host
+        memory)."""
+        want = max(1, min(int(max_results), self.submitted - self.completed))
+        res, resid, limbs = self._buffers(want)
         n = C.c_int(0)
-        self.ctx._check(self.ctx.lib.drx_server_poll(self.h, res, resid.ctypes.data, self.stride, max_results,
-                                                     C.byref(n)))
+        self.ctx._check(self.ctx.lib.drx_server_poll(self.h, res, resid.ctypes.data, limbs.ctypes.data, self.stride,
+                                                     want, C.byref(n)))
         out = []
         for i in range(n.value):
             r = res[i]
@@ -70,6 +96,10 @@ class WalkServer:
                 "t_submit": r.t_submit, "t_admit": r.t_admit, "t_done": r.t_done,
                 "latency_s": r.t_done - r.t_submit, "spec": spec,
             }
+            if r.status == 0:
+                for v, key in ((0, "p_big"), (1, "q_big")):
+                    mag = int.from_bytes(limbs[i, v, :K].tobytes(), "little")
+                    d[key] = -mag if r.neg[v] else mag
             out.append(d)
         self.completed += n.value
         return out

@@ -18,7 +18,7 @@
 extern "C" {
 #endif
 
-#define DRX_ABI_VERSION 3
+#define DRX_ABI_VERSION 4
 
 enum {
   DRX_OK = 0,
@@ -261,12 +261,18 @@ int drx_last_timing(const drx_ctx* ctx, double* t6);
 
 /* ---- continuous-batching server ---------------------------------------
  * Requests are admitted into device-resident slots as memory allows; every
- * round advances each resident walk by up to `round_steps` steps, so walks
- * submitted while others run join at the next round boundary. */
+ * round advances each resident walk by `round_steps` steps or, while few
+ * walks are resident, by up to `max_walk_tasks` such step ranges run side by
+ * side (their ordered product is the same), so walks submitted while others
+ * run join at the next round boundary. Finished walks get the GPU CRT of p
+ * and q in the same round. No counterpart in the reference: its drivers call
+ * run_pcf_walk + crt_reconstruct per request (python/euler2ai_full_verify.py:
+ * 331-342, python/a100_sweep.py:416-436). */
 typedef struct drx_server_cfg {
   int max_slots;        /* resident walks */
-  int round_steps;      /* steps per walk per round */
+  int round_steps;      /* steps of one step range */
   int threads;          /* workgroup size, 0 = tuned */
+  int max_walk_tasks;   /* step ranges per walk per round at light load (0 = 16) */
   int64_t mem_budget;   /* bytes of device state for slots (0 = 1 GiB) */
 } drx_server_cfg;
 
@@ -277,14 +283,18 @@ typedef struct drx_server_result {
   drx_walk_out out;
   int32_t K;
   int32_t status;          /* 0 ok, < 0 error */
-  double t_submit, t_admit, t_done;  /* seconds, steady clock */
+  double t_submit, t_admit, t_done;  /* seconds, steady clock; t_done: CRT integers in host memory */
+  uint8_t neg[2];          /* sign of the centred CRT of p and q */
+  uint8_t pad[6];
 } drx_server_result;
 
 int drx_server_create(drx_ctx* ctx, const drx_server_cfg* cfg, drx_server** out);
 /* Copies the walk (and its axis values) into the server queue. */
 int drx_server_submit(drx_server* s, const drx_walk* w, const int64_t* axis_vals, uint64_t* ticket);
-/* Up to max finished results; residues[i] gets 2*K int64 (p then q) at stride. */
-int drx_server_poll(drx_server* s, drx_server_result* res, int64_t* residues, int stride, int max,
+/* Up to max finished results; residues[i] gets 2*K int64 (p then q) and,
+ * when given, limbs[i] the CRT magnitudes of p and q (K little-endian uint32
+ * each, as drx_crt), both [max][2][stride]; the signs are in res[i].neg. */
+int drx_server_poll(drx_server* s, drx_server_result* res, int64_t* residues, uint32_t* limbs, int stride, int max,
                     int* n_done);
 int drx_server_pending(drx_server* s, int* n_queued, int* n_resident);
 /* Resident slots the memory budget (and max_slots) allow. */

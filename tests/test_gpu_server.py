@@ -34,9 +34,9 @@ def _mixed_specs(n_pcf=24, n_cmf=8, seed=5):
 
 
 def _alone(spec):
-    with BatchPlan([spec]) as plan:
+    with BatchPlan([spec], crt=True) as plan:
         r = plan.run()
-    return r.walk_dict(0), float(r.out[0]["delta_float"])
+        return r.walk_dict(0), float(r.out[0]["delta_float"]), r.crt_values(0)
 
 
 def test_served_results_equal_alone(gpu_ctx):
@@ -56,9 +56,10 @@ def test_served_results_equal_alone(gpu_ctx):
     for r in results:
         assert r["status"] == 0
         i = tickets[r["ticket"]]
-        ref, dref = alone[i]
+        ref, dref, crt = alone[i]
         assert r["p_residues"].tolist() == ref["p_residues"].tolist(), i
         assert r["q_residues"].tolist() == ref["q_residues"].tolist(), i
+        assert (r["p_big"], r["q_big"]) == crt, i
         for key in ("p_float", "q_float", "log_scale"):
             assert float(r[key]).hex() == float(ref[key]).hex(), (i, key)
         assert float(r["delta_float"]).hex() == float(dref).hex(), i

@@ -174,7 +174,31 @@ def test_struct_layouts_match_header():
     assert N.Walk.mode.offset == 20 and N.Walk.out_p.offset == 64 and N.Walk.out_q.offset == 68
     assert N.WALK_DTYPE.fields["mode"][1] == 20 and N.WALK_DTYPE.fields["out_q"][1] == 68
     assert C.sizeof(N.WalkOut) == 48 and N.OUT_DTYPE.itemsize == 48
-    assert C.sizeof(N.ServerResult) == 8 + 48 + 8 + 24
+    assert C.sizeof(N.ServerResult) == 8 + 48 + 8 + 24 + 8
+
+
+def test_struct_layouts_match_the_c_compiler(tmp_path):
+    # sizes and offsets of every ctypes mirror against gcc on the header itself
+    import ctypes as C
+    import subprocess
+
+    mirrors = {"drx_walk": N.Walk, "drx_walk_out": N.WalkOut, "drx_batch": N.Batch, "drx_launch_cfg": N.LaunchCfg,
+               "drx_server_cfg": N.ServerCfg, "drx_server_result": N.ServerResult}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "dreams_hip.h"', "int main(void) {"]
+    for cname, py in mirrors.items():
+        lines.append(f'  printf("{cname} %zu\\n", sizeof({cname}));')
+        for f in py._fields_:
+            lines.append(f'  printf("{cname}.{f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
+    lines.append("  return 0;\n}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", f"-I{ROOT}/include", str(src), "-o", str(exe)], check=True)
+    got = dict(ln.split() for ln in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines())
+    for cname, py in mirrors.items():
+        assert int(got[cname]) == C.sizeof(py), cname
+        for f in py._fields_:
+            assert int(got[f"{cname}.{f[0]}"]) == getattr(py, f[0]).offset, (cname, f[0])
 
 
 # -------------------------------------------------------------- generators
