@@ -182,26 +182,33 @@ int round_launch(drx_server* s, const std::vector<SrvReq>& adm, const std::vecto
     if (s->slots[i].used && s->slots[i].epoch != ctx->epoch)
       return fail(ctx, DRX_ESTATE, "programs were released while requests were resident");
   struct TaskRef {
-    int slot, m, kind, lclass, shape;
+    int slot, m, kind, lclass, shape, prog, K;
+    double cost;  // work of one step range: primes x residue-code length
   };
   std::vector<TaskRef> refs;
   for (int i = 0; i < s->nslots; ++i) {
     if (!s->slots[i].used) continue;
     const HostProgram& hp = ctx->progs[s->slots[i].w.prog];
     const int kind = walk_kind(hp, false);
-    refs.push_back({i, s->slots[i].m, kind, lds_class(hp, kind), walk_shape(hp, kind)});
+    refs.push_back({i, s->slots[i].m, kind, lds_class(hp, kind), walk_shape(hp, kind), s->slots[i].w.prog,
+                    s->slots[i].w.K, (double)s->slots[i].w.K * (double)(1 + hp.pk.code.size())});
   }
   if (refs.empty()) return DRX_OK;
+  // within a stage the costliest step ranges first, so that the launch ends
+  // on short ones; walks of one program and width stay adjacent (consecutive
+  // workgroups then read the same code and constants through the scalar cache)
   std::stable_sort(refs.begin(), refs.end(), [](const TaskRef& a, const TaskRef& b) {
     if (a.m != b.m) return a.m < b.m;
     if (a.kind != b.kind) return a.kind < b.kind;
     if (a.lclass != b.lclass) return a.lclass < b.lclass;
-    return a.shape < b.shape;
+    if (a.shape != b.shape) return a.shape < b.shape;
+    if (a.cost != b.cost) return a.cost > b.cost;
+    return a.prog < b.prog;
   });
   // step ranges per walk this round: enough walk-kernel tasks to fill the
   // device while few walks are resident, one per walk under load
   const int nres = (int)refs.size();
-  const int cap = s->cfg.max_walk_tasks > 0 ? s->cfg.max_walk_tasks : 16;
+  const int cap = s->cfg.max_walk_tasks > 0 ? s->cfg.max_walk_tasks : 32;
   const int tpw = std::max(1, std::min(cap, s->target_tasks / nres));
   const int64_t rs = s->cfg.round_steps;
   s->ht.clear();
@@ -545,8 +552,9 @@ int drx_server_create(drx_ctx* ctx, const drx_server_cfg* cfg, drx_server** out)
   }
   s->nslots = (int)cap;
   // the extra step ranges of lightly loaded rounds use what the slots leave
-  // of the budget, up to four walk-kernel tasks per CU
-  s->target_tasks = (int)std::min<int64_t>(4 * (int64_t)ncu, cap + (budget - cap * per_slot) / per_task);
+  // of the budget, up to 16 walk-kernel tasks per CU: a launch of only a
+  // few times the resident workgroups loses much of its time to its tail
+  s->target_tasks = (int)std::min<int64_t>(16 * (int64_t)ncu, cap + (budget - cap * per_slot) / per_task);
   s->slots.resize(s->nslots);
   s->h_walks.resize(s->nslots);
   const size_t ns = (size_t)s->nslots;

@@ -272,7 +272,7 @@ typedef struct drx_server_cfg {
   int max_slots;        /* resident walks */
   int round_steps;      /* steps of one step range */
   int threads;          /* workgroup size, 0 = tuned */
-  int max_walk_tasks;   /* step ranges per walk per round at light load (0 = 16) */
+  int max_walk_tasks;   /* step ranges per walk per round at light load (0 = 32) */
   int64_t mem_budget;   /* bytes of device state for slots (0 = 1 GiB) */
 } drx_server_cfg;
 
