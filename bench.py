@@ -5,19 +5,21 @@ PCFs (deg a <= 3, deg b <= 6, coefficients in [-20, 20]), depth 2000, K = 32
 primes, FP64 shadow with delta_float against a seeded target per walk, GPU
 Garner CRT of p and q for every walk and top-64 by delta_float. Inputs are
 packed and resident in HBM before the timed region; each step re-runs the
-batch from the initial matrices and copies all results to the host.
+batch from the initial matrices.
 
 Printed (one JSON line): residue-steps/s of the batched path (value), the
-plain one-walk-per-launch GPU path and the CPU reference restatement on
-bounded samples, p50/p99 latency of the batch and of a short served stream,
-and the roofline of the dominant kernel (the walk kernel).
+same with the copy of every output to the host, the plain
+one-walk-per-launch GPU path and the CPU reference restatement on bounded
+samples, the served stream (Poisson arrivals of serve5 requests swept over
+offered loads: p50/p99 latency to CRT integers on the host, served rate,
+saturation), the cmf3 and mix4 batches (one warm-up and two timed steps
+each), and the roofline of the dominant kernel.
 """
 
 from __future__ import annotations
 
 import argparse
 import json
-import math
 import multiprocessing as mp
 import os
 import sys
@@ -31,7 +33,12 @@ for _p in (os.path.join(ROOT, "dreams-rns-cuda_amd"), ROOT):
 import numpy as np  # noqa: E402
 
 METRIC = "residue-steps/sec batched+served vs plain (bit-exact CRT); p50/p99 latency"
-PEAK_LANE_OPS = 256 * 64 * 2.4e9  # CU x lanes x clock (rocminfo: 256 CUs, 2400 MHz)
+# FP64 VALU lane-ops per second: 256 CUs x 4 SIMDs x 16 FP64 lanes x 2.4 GHz
+# (rocminfo: 256 CUs, 2400 MHz), i.e. the 78.6 TFLOP/s FP64 vector peak
+# counted in operations (an FMA is one)
+PEAK_LANE_OPS = 256 * 64 * 2.4e9
+# the GPU box gives one GPU a share of 16 host cores
+BOX_CPU_SHARE = 16
 
 
 def log(*a):
@@ -39,34 +46,13 @@ def log(*a):
 
 
 # ------------------------------------------------------------- work model
-def timed_with_copy(plan, steps: int) -> list:
-    """Wall time (ms) of whole runs that also copy every output to pinned
-    host memory: the PCIe-inclusive rate, reported beside the HBM-resident
-    one."""
-    out = []
-    for _ in range(max(1, min(steps, 5))):
-        t = time.perf_counter()
-        plan.run()
-        out.append((time.perf_counter() - t) * 1e3)
-    log(f"with the copy back: {np.mean(out):.3f} ms per step")
-    return out
-
-
-def with_copy(rsteps: float, copy_ms: list) -> dict:
-    ms = float(np.mean(copy_ms))
-    return {"ms_per_step": ms, "value": rsteps / (ms * 1e-3), "unit": "residue-steps/s",
-            "what": "the same step plus the copy of every output (residues, CRT limbs, FP64 results, top-k) "
-                    "to pinned host memory"}
-
-
 def program_work(prog, vector: bool = False) -> float:
-    """Integer-VALU ops per residue-step by the fixed convention
-    W = 6*MM + 2*MA with MM = #MUL + #POW2 + 2*#POW3 + 56*#INV + m^3 and
-    MA = #ADD + #SUB + #NEG + m^2 (m - 1), counted on the walk's bytecode;
-    a state-vector walk's product is a matrix-vector product (m^2, m (m - 1)).
-    Instructions whose operands are all constants (a rational constant's
-    LOAD_C p; LOAD_C q; INV; MUL) compute the same value at every step and
-    are not counted."""
+    """Bytecode-equivalent ops per residue-step: W = 6*MM + 2*MA with
+    MM = #MUL + #POW2 + 2*#POW3 + 56*#INV + m^3 and MA = #ADD + #SUB + #NEG +
+    m^2 (m - 1), counted on the walk's bytecode as the reference evaluates it
+    every step; a state-vector walk's product is a matrix-vector product
+    (m^2, m (m - 1)). Instructions whose operands are all constants give the
+    same value every step and are not counted."""
     cnt = {}
     const_regs = set()
     for ins in prog.opcodes:
@@ -124,12 +110,10 @@ def walk_kernel_name(progs) -> str:
 def measured_counters(kernel: str, workload: str):
     """Counter summary of `kernel` on `workload` committed beside the rocprofv3
     summaries: HBM bytes per launch (FETCH_SIZE / WRITE_SIZE scaled by the
-    copy calibration) and VALU instructions per launch, or None when the
-    summary is for another kernel."""
+    copy calibration) and VALU instructions per launch, or None."""
     name = "walk_kernel_counters.json" if workload == "pcf2" else f"walk_kernel_counters_{workload}.json"
-    path = os.path.join(ROOT, "profiles", name)
     try:
-        with open(path) as f:
+        with open(os.path.join(ROOT, "profiles", name)) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
@@ -138,7 +122,59 @@ def measured_counters(kernel: str, workload: str):
     return t
 
 
+def roofline(stats, walk_ms, equiv_work, kname, workload):
+    """The walk kernel against the FP64 VALU peak. `achieved` counts the
+    arithmetic the chosen kernels execute, from the library's own operation
+    count of their code (drx_plan_stats: 6 FP64 ops per modular multiply --
+    product, error term, quotient, rounding, two corrections -- and one per
+    lazy addition), over the kernels' device time measured with HIP events on
+    their stream. The committed counter summary of the same kernel and
+    workload (when present) gives HBM traffic and SQ_INSTS_VALU x 64 per
+    launch as a cross-check. `equivalent` is the bytecode convention W of
+    the reference's per-step evaluation, which forward differences skip."""
+    if not walk_ms == walk_ms or walk_ms <= 0:
+        return None
+    executed = 6.0 * float(stats["modmul"]) + float(stats["modadd"])
+    ach = executed / (walk_ms * 1e-3)
+    cnt = measured_counters(kname, workload)
+    out = {"bound": "valu", "achieved": ach / 1e12, "peak": PEAK_LANE_OPS / 1e12, "unit": "TOP/s",
+           "frac": ach / PEAK_LANE_OPS, "traffic": float(cnt["traffic_bytes"]) if cnt else None,
+           "kernel": kname, "kernel_ms": walk_ms, "work_per_launch": executed,
+           "work_model": "executed FP64 lane-ops of the chosen kernels: 6 per modular multiply, 1 per lazy "
+                         "addition (library count, drx_plan_stats)",
+           "equivalent": {"achieved": equiv_work / (walk_ms * 1e-3) / 1e12,
+                          "frac": equiv_work / (walk_ms * 1e-3) / PEAK_LANE_OPS, "work_per_launch": equiv_work,
+                          "model": "W = 6*MM + 2*MA per residue-step of per-step bytecode evaluation (the "
+                                   "reference's algorithm); forward differences skip that evaluation, so this "
+                                   "rate can pass the peak"}}
+    if cnt:
+        v = float(cnt["valu_insts"]) * 64
+        out["counters"] = {"valu_lane_ops_per_launch": v, "frac": v / (walk_ms * 1e-3) / PEAK_LANE_OPS,
+                           "source": cnt["source"], "commit": cnt.get("commit")}
+    return out
+
+
 # ------------------------------------------------------------ CPU baseline
+def cpu_share():
+    """Cores this process may run on, at most the box's share for one GPU."""
+    try:
+        visible = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        visible = os.cpu_count() or 1
+    return min(BOX_CPU_SHARE, visible), visible
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def _cpu_walk(args):
     import dreams_rns as D
     from oracle import walk as ow
@@ -170,75 +206,73 @@ def cpu_baseline(walks, depth, K, n_one, n_all, cores):
 
 
 # ------------------------------------------------------------------ served
-def served_run(specs_pool, rate, duration, round_steps, seed=1005):
-    """Poisson arrivals of walks drawn from ``specs_pool`` at ``rate`` req/s
-    for ``duration`` s into the continuous-batching server."""
+def served_sweep(rates, seconds, mem_budget, seed_base=1005):
+    """The served stream: Poisson arrivals of serve5 requests (m in
+    {2, 3, 4}, depth log-U[1000, 100000], K in {32, 64, 128}) at each offered
+    rate for `seconds`, then the queue drained. Per rate: p50/p99 latency
+    (submission to CRT integers on the host), served and offered
+    residue-steps/s."""
+    from dreams_rns import serve5, synth
     from dreams_rns.server import WalkServer
 
-    rng = np.random.default_rng(seed)
-    n = int(rng.poisson(rate * duration))
-    times = np.sort(rng.uniform(0.0, duration, size=n))
-    picks = rng.integers(0, len(specs_pool), size=n)
-    lat, rsteps = [], 0
-    with WalkServer(round_steps=round_steps) as srv:
-        t0 = time.perf_counter()
-        tickets = {}
-        for t, i in zip(times, picks):
-            now = time.perf_counter() - t0
-            if t > now:
-                time.sleep(t - now)
-            tickets[srv.submit(specs_pool[i])] = specs_pool[i]
-            for r in srv.poll():
-                lat.append(r["latency_s"])
-        for r in srv.drain():
-            lat.append(r["latency_s"])
-        wall = time.perf_counter() - t0
-        rsteps = sum(s.depth * s.K for s in tickets.values())
-    lat = np.array(lat)
-    return {"rate_req_s": rate, "requests": int(n), "residue_steps_per_s": rsteps / wall,
-            "p50_ms": float(np.percentile(lat, 50) * 1e3) if n else None,
-            "p99_ms": float(np.percentile(lat, 99) * 1e3) if n else None, "wall_s": wall}
+    pool = serve5.program_pool(16)
+    per_rate = []
+    with WalkServer(round_steps=1000, mem_budget=mem_budget) as srv:
+        slots = srv.capacity()
+        warm = [serve5.request_spec(dict(r, depth=1000, K=128, L=0.0)) for m in pool for r in pool[m]]
+        serve5.run_stream(srv, warm, np.linspace(0.0, 0.05, len(warm)), timeout=120)  # uploads every program
+        for rate in rates:
+            arr = synth.serve5_arrivals(rate, seconds)
+            rng = np.random.default_rng(seed_base + int(rate))
+            specs = [serve5.request_spec(serve5.draw_request(rng, pool)) for _ in arr]
+            st = serve5.run_stream(srv, specs, arr, timeout=300)["stats"]
+            st["rate_req_s"] = float(rate)
+            per_rate.append(st)
+            log(f"served {rate:g} req/s: p50 {st['p50_ms']:.1f} ms, p99 {st['p99_ms']:.1f} ms, served "
+                f"{st['served_residue_steps_per_s']:.3e} of {st['offered_residue_steps_per_s']:.3e} residue-steps/s")
+    keep = [s for s in per_rate if s["served_residue_steps_per_s"] >= 0.9 * s["offered_residue_steps_per_s"]]
+    top = max(per_rate, key=lambda s: s["rate_req_s"])
+    cap = top["served_residue_steps_per_s"]
+    return {
+        "workload": "serve5: Poisson arrivals, m in {2,3,4} (pcf2-shape PCFs, 3x3/4x4 CMFs of 10-400 ops), depth "
+                    "log-U[1000,100000], K in {32,64,128}; programs from a seeded pool of 48 generator draws",
+        "seconds_per_rate": seconds, "slots": slots, "rates": per_rate,
+        "saturation_req_s": max((s["rate_req_s"] for s in keep), default=None),
+        "capacity_residue_steps_per_s": cap,
+        "capacity_req_s": cap / top["mean_residue_steps_per_request"],
+        "saturation_rule": "highest offered rate whose served rate is within 10% of the offered one; capacity = "
+                           "served rate at the highest offered rate (queue never empty)",
+    }
 
 
-# ------------------------------------------------------- other workloads
-def other_workload(args):
-    """cmf3: 1024 3x3 CMFs with INV, depth 10000, K = 64; mix4: 16384 uneven
-    walks (2x2 / 3x3 / 4x4, depth 500..50000, K 16..128), top-128; zeta5:
-    16384 state-vector walks; sweep3f2: the pFq CMF sweep of one 3F2 spec
-    over 64 trajectories x 512 shifts at depth 2000, K = 1000 (shifts of
-    equal integer offsets walked once), with GPU CRT of every walk. One JSON
-    line in the same form (no served leg; a CPU leg for zeta5 / sweep3f2)."""
+# ------------------------------------------------------- batch workloads
+def build_workload(name, n_walks=None):
+    """(specs, top_k, description, extra) of a batch workload."""
     import dreams_rns as D
     from dreams_rns import synth
-    from dreams_rns.engine import BatchPlan, cmf_spec, pcf_spec
+    from dreams_rns.engine import cmf_spec, pcf_spec
 
     cf = lambda mat, m, dim, axes, dirs: D.compile_cmf_from_dict(mat, m, dim, axes, dirs)
-    t0 = time.perf_counter()
-    cpu = None
-    if args.workload == "zeta5":
+    extra = {}
+    if name == "pcf2":
+        cfg = synth.pcf2(n_walks=n_walks or 4096)
+        specs = [pcf_spec(D.compile_pcf(w["a"], w["b"]), D.pcf_initial_values(w["a"]), cfg["depth"], cfg["K"], 1,
+                          target=w["L"]) for w in cfg["walks"]]
+        extra["cfg"] = cfg
+        desc = (f"pcf2: {len(specs)} seeded PCFs, deg a<=3, deg b<=6, coef [-20,20], depth {cfg['depth']}, "
+                f"K={cfg['K']}, FP64 shadow + delta_float, GPU Garner CRT, top-{cfg['top_k']}")
+        return specs, cfg["top_k"], desc, extra
+    if name == "zeta5":
         from dreams_rns.engine import vec_spec
 
         cfg = synth.zeta5_sweep(cf)
         specs = [vec_spec(cfg["program"], cfg["depth"], cfg["K"], sh, cfg["v0"], cfg["acc"], cfg["const"], tr)
                  for sh in cfg["shifts"] for tr in cfg["trajs"]]
-        top_k = 64
+        extra["cfg"] = cfg
         desc = (f"zeta5: {len(specs)} state-vector walks of the zeta(5) CMF (4x4, 6 axes), "
                 f"{len(cfg['shifts'])} shifts x {len(cfg['trajs'])} trajectories, depth {cfg['depth']}, K={cfg['K']}")
-        if not args.no_cpu:
-            from oracle import walk as ow
-
-            t = time.perf_counter()
-            nw = 0
-            for sh, tr in zip(cfg["shifts"], cfg["trajs"]):
-                ow.run_cmf_walk_vec(cfg["program"], cfg["depth"], cfg["K"], sh, cfg["v0"], cfg["acc"], cfg["const"], tr)
-                nw += 1
-                if time.perf_counter() - t > 10.0:
-                    break
-            el = time.perf_counter() - t
-            cpu = {"value": nw * cfg["depth"] * cfg["K"] / el, "unit": "residue-steps/s", "cores": 1, "kind": "port",
-                   "sample": f"{nw} walks of the sweep through the numpy restatement, one core"}
-            log(f"cpu: {cpu['value']:.3e} residue-steps/s ({nw} walks)")
-    elif args.workload == "sweep3f2":
+        return specs, 64, desc, extra
+    if name == "sweep3f2":
         from dreams_rns.sweep import CmfSweep
 
         with open(os.path.join(ROOT, "tests", "golden", "sweep_3f2.json")) as f:
@@ -248,72 +282,141 @@ def other_workload(args):
         sw = CmfSweep([spec], fix["bench_trajectories"], fix["bench_shifts"], depth=depth, K=K)
         rows = list(sw._rows())
         specs = [cmf_spec(prog, depth, K, so) for (_, _, prog, so, _) in rows]
-        sweep_walks = sw.total_walks
-        top_k = 0
+        extra.update(fix=fix, spec=spec, rows=rows, sweep_walks=sw.total_walks)
         desc = (f"sweep3f2: {spec['name']} x {len(fix['bench_trajectories'])} trajectories x "
-                f"{len(fix['bench_shifts'])} shifts ({sweep_walks} sweep walks, {len(specs)} distinct starts), "
+                f"{len(fix['bench_shifts'])} shifts ({sw.total_walks} sweep walks, {len(specs)} distinct starts), "
                 f"4x4 per-axis companion, depth {depth}, K={K}")
-        if not args.no_cpu:
-            from oracle import walk as ow
-
-            t = time.perf_counter()
-            nw = 0
-            for (_, _, prog, so, _) in rows:
-                ow.run_cmf_walk(prog, 50, K, so)
-                nw += 1
-                if time.perf_counter() - t > 10.0:
-                    break
-            el = time.perf_counter() - t
-            cpu = {"value": nw * 50 * K / el, "unit": "residue-steps/s", "cores": 1, "kind": "port",
-                   "sample": f"{nw} walks of the sweep at depth 50 (the per-step cost does not depend on "
-                             f"the depth) through the numpy restatement, one core"}
-            log(f"cpu: {cpu['value']:.3e} residue-steps/s ({nw} walks)")
-    elif args.workload == "cmf3":
-        n = args.walks if args.walks != 4096 else 1024
-        cfg = synth.cmf3(cf, n_walks=n)
+        return specs, 0, desc, extra
+    if name == "cmf3":
+        cfg = synth.cmf3(cf, n_walks=n_walks or 1024)
         specs = [cmf_spec(w["program"], cfg["depth"], cfg["K"], w["shift"], target=0.25) for w in cfg["walks"]]
-        top_k = 64
-        desc = f"cmf3: {n} 3x3 CMFs (100-250 ops with INV), depth {cfg['depth']}, K={cfg['K']}"
-    else:
-        n = args.walks if args.walks != 4096 else 16384
-        cfg = synth.mix4(cf, n_walks=n)
-        specs = []
-        for w in cfg["walks"]:
-            if w["kind"] == "pcf":
-                p = D.compile_pcf(w["a"], w["b"])
-                specs.append(pcf_spec(p, D.pcf_initial_values(w["a"]), w["depth"], w["K"], 1, target=w["L"]))
-            else:
-                specs.append(cmf_spec(w["program"], w["depth"], w["K"], w["shift"], target=w["L"]))
-        top_k = cfg["top_k"]
-        desc = f"mix4: {n} uneven walks (2x2/3x3/4x4, depth 500..50000, K 16..128)"
-    work = sum(program_work(s.program, s.vector) * s.depth * s.K for s in specs)
+        desc = f"cmf3: {len(specs)} 3x3 CMFs (100-250 ops with INV), depth {cfg['depth']}, K={cfg['K']}"
+        return specs, 64, desc, extra
+    cfg = synth.mix4(cf, n_walks=n_walks or 16384)
+    specs = []
+    for w in cfg["walks"]:
+        if w["kind"] == "pcf":
+            p = D.compile_pcf(w["a"], w["b"])
+            specs.append(pcf_spec(p, D.pcf_initial_values(w["a"]), w["depth"], w["K"], 1, target=w["L"]))
+        else:
+            specs.append(cmf_spec(w["program"], w["depth"], w["K"], w["shift"], target=w["L"]))
+    desc = f"mix4: {len(specs)} uneven walks (2x2/3x3/4x4, depth 500..50000, K 16..128), top-{cfg['top_k']}"
+    return specs, cfg["top_k"], desc, extra
+
+
+def timed_steps(plan, steps, warmup, label):
+    """Whole-batch runs with the outputs left in HBM: step wall times and
+    walk-kernel device times (ms)."""
+    for _ in range(warmup):
+        plan.run()
+    step_ms, walk_ms = [], []
+    for i in range(steps):
+        t = time.perf_counter()
+        plan.run(fetch=False)
+        step_ms.append((time.perf_counter() - t) * 1e3)
+        tm = plan.ctx.last_timing()
+        walk_ms.append(tm[0])
+        log(f"{label} step {i}: {step_ms[-1]:.2f} ms (walk kernels {tm[0]:.3f} ms, combine {tm[1]:.3f}, "
+            f"shadow wait {tm[2]:.3f}, tail {tm[3]:.3f}, device {tm[4]:.3f})")
+    return step_ms, walk_ms
+
+
+def timed_with_copy(plan, steps: int) -> list:
+    """Wall time (ms) of whole runs that also copy every output to pinned
+    host memory: the PCIe-inclusive rate, reported beside the HBM-resident
+    one."""
+    out = []
+    for _ in range(max(1, min(steps, 5))):
+        t = time.perf_counter()
+        plan.run()
+        out.append((time.perf_counter() - t) * 1e3)
+    log(f"with the copy back: {np.mean(out):.3f} ms per step")
+    return out
+
+
+def with_copy(rsteps: float, copy_ms: list) -> dict:
+    ms = float(np.mean(copy_ms))
+    return {"ms_per_step": ms, "value": rsteps / (ms * 1e-3), "unit": "residue-steps/s",
+            "what": "the same step plus the copy of every output (residues, CRT limbs, FP64 results, top-k) "
+                    "to pinned host memory"}
+
+
+def batch_rate(name, steps=2, warmup=1, threads=256, chunk=512):
+    """One batch workload timed in the default line: residue-steps/s with
+    the outputs in HBM (plan built before the timed region)."""
+    from dreams_rns.engine import BatchPlan
+
+    t0 = time.perf_counter()
+    specs, top_k, desc, _ = build_workload(name)
+    gen_s = time.perf_counter() - t0
     rsteps = float(sum(s.depth * s.K for s in specs))
+    equiv = sum(program_work(s.program, s.vector) * s.depth * s.K for s in specs)
+    with BatchPlan(specs, crt=True, top_k=top_k, threads=threads, chunk_steps=chunk) as plan:
+        step_ms, walk_ms = timed_steps(plan, steps, warmup, name)
+        st = plan.stats()
+    ms = float(np.mean(step_ms))
+    kname = walk_kernel_name([s.program for s in specs])
+    return {"workload": desc, "value": rsteps / (ms * 1e-3), "unit": "residue-steps/s", "ms_per_step": ms,
+            "steps": steps, "warmup": warmup, "threads": threads, "chunk_steps": chunk, "generate_s": gen_s,
+            "roofline": roofline(st, float(np.mean(walk_ms)), equiv, kname, name)}
+
+
+def workload_line(args):
+    """--workload cmf3 | mix4 | zeta5 | sweep3f2: one JSON line in the same
+    form (a CPU leg for zeta5 / sweep3f2)."""
+    from dreams_rns.engine import BatchPlan, autotune
+
+    t0 = time.perf_counter()
+    specs, top_k, desc, extra = build_workload(args.workload, args.walks)
     log(f"generated {len(specs)} walks in {time.perf_counter() - t0:.1f}s")
+    cpu = None
+    if not args.no_cpu and args.workload == "zeta5":
+        from oracle import walk as ow
+
+        cfg = extra["cfg"]
+        t = time.perf_counter()
+        nw = 0
+        for sh, tr in zip(cfg["shifts"], cfg["trajs"]):
+            ow.run_cmf_walk_vec(cfg["program"], cfg["depth"], cfg["K"], sh, cfg["v0"], cfg["acc"], cfg["const"], tr)
+            nw += 1
+            if time.perf_counter() - t > 10.0:
+                break
+        el = time.perf_counter() - t
+        cpu = {"value": nw * cfg["depth"] * cfg["K"] / el, "unit": "residue-steps/s", "cores": 1, "kind": "port",
+               "cpu_model": cpu_model(), "sample": f"{nw} walks of the sweep through the numpy restatement, one core"}
+    elif not args.no_cpu and args.workload == "sweep3f2":
+        from oracle import walk as ow
+
+        K = specs[0].K
+        t = time.perf_counter()
+        nw = 0
+        for (_, _, prog, so, _) in extra["rows"]:
+            ow.run_cmf_walk(prog, 50, K, so)
+            nw += 1
+            if time.perf_counter() - t > 10.0:
+                break
+        el = time.perf_counter() - t
+        cpu = {"value": nw * 50 * K / el, "unit": "residue-steps/s", "cores": 1, "kind": "port",
+               "cpu_model": cpu_model(),
+               "sample": f"{nw} walks of the sweep at depth 50 (the per-step cost does not depend on the depth) "
+                         f"through the numpy restatement, one core"}
+    if cpu:
+        log(f"cpu: {cpu['value']:.3e} residue-steps/s")
     tuned = None
     if args.workload in ("zeta5", "sweep3f2"):
         args.threads = args.threads or 256
         args.chunk = args.chunk or 4096
     if not (args.threads or args.chunk):
-        from dreams_rns.engine import autotune
-
         t = time.perf_counter()
         tuned = autotune(specs, budget_s=60.0)
         args.threads, args.chunk = tuned["threads"], tuned["chunk_steps"]
         log(f"autotuned in {time.perf_counter() - t:.1f}s: threads {args.threads}, chunk {args.chunk}")
+    rsteps = float(sum(s.depth * s.K for s in specs))
+    equiv = sum(program_work(s.program, s.vector) * s.depth * s.K for s in specs)
     plan = BatchPlan(specs, crt=True, top_k=top_k, threads=args.threads, chunk_steps=args.chunk,
-                     use_graph=args.graph)
+                     use_graph=args.graph, mode=args.mode)
     log(f"plan built: {time.perf_counter() - t0:.1f}s since start")
-    ctx = plan.ctx
-    for _ in range(args.warmup):
-        plan.run()
-    step_ms, walk_ms = [], []
-    for i in range(args.steps):
-        t = time.perf_counter()
-        plan.run(fetch=False)  # the outputs stay in HBM
-        step_ms.append((time.perf_counter() - t) * 1e3)
-        tm = ctx.last_timing()
-        walk_ms.append(tm[0])
-        log(f"step {i}: {step_ms[-1]:.2f} ms (walk kernels {tm[0]:.3f} ms, device {tm[4]:.3f})")
+    step_ms, walk_ms = timed_steps(plan, args.steps, args.warmup, args.workload)
     copy_ms = timed_with_copy(plan, args.steps)
     post = None
     if args.workload == "sweep3f2":
@@ -322,13 +425,14 @@ def other_workload(args):
         from dreams_rns.postwalk import sweep_hits
         from dreams_rns.sweep import SweepBatch
 
+        rows, fix, spec = extra["rows"], extra["fix"], extra["spec"]
         bank = load_constants(dps=200)
         sb = SweepBatch(spec_idx=np.array([r[0] for r in rows], dtype=np.int32),
                         traj_idx=np.array([r[1] for r in rows], dtype=np.int32), offsets=[r[3] for r in rows],
                         shift_rows=[r[4] for r in rows], result=plan.run(), crt=True)
         t = time.perf_counter()
         hits = sweep_hits(sb, [spec], fix["bench_trajectories"], bank, proximity=1e-3, dps=200)
-        post = {"ms": (time.perf_counter() - t) * 1e3, "hits": len(hits), "walks": sweep_walks,
+        post = {"ms": (time.perf_counter() - t) * 1e3, "hits": len(hits), "walks": extra["sweep_walks"],
                 "what": "a100_sweep.py:426-505 hit selection (CRT check, proximity, delta against the 40-constant "
                         "bank) over the batch's results, on the host"}
         log(f"hit selection: {post['ms']:.1f} ms, {len(hits)} hits")
@@ -344,44 +448,23 @@ def other_workload(args):
             post["cpu_loop_ms_per_walk"] = per * 1e3
             post["cpu_loop_sample"] = (f"{ns} walks of the batch through the walk-by-walk restatement of the "
                                        "reference loop (Python CRT, mpmath), one core")
-            log(f"reference-style loop: {per * 1e3:.2f} ms per walk")
     st = plan.stats()
     plan.close()
     ms = float(np.mean(step_ms))
     walk_avg = float(np.mean(walk_ms)) if not args.graph else float("nan")
-    achieved = work / (walk_avg * 1e-3) if walk_avg == walk_avg else None
     kname = "k_walk_vec" if args.workload == "zeta5" else walk_kernel_name([s.program for s in specs])
-    cnt = measured_counters(kname, args.workload)
-    traffic = float(cnt["traffic_bytes"]) if cnt else None
-    if cnt:  # VALU instructions the counters measured for this kernel on this batch
-        w_exec = float(cnt["valu_insts"]) * 64
-        model = f"SQ_INSTS_VALU x 64 per launch ({cnt['source']})"
-    else:
-        w_exec = 6.0 * float(st["modmul"]) + 2.0 * float(st["modadd"])
-        model = "6*modmul + 2*modadd of the modular operations the chosen kernels execute (library count)"
-    executed = ({"achieved": w_exec / (walk_avg * 1e-3) / 1e12, "frac": w_exec / (walk_avg * 1e-3) / PEAK_LANE_OPS,
-                 "work_per_launch": w_exec, "model": model}
-                if walk_avg == walk_avg else None)
     print(json.dumps({
         "metric": METRIC, "value": rsteps / (ms * 1e-3), "unit": "residue-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": desc, "walks": len(specs), "top_k": top_k, "graph": bool(args.graph),
+        "config": {"workload": desc, "walks": len(specs), "top_k": top_k, "graph": bool(args.graph), "mode": args.mode,
                    "threads": args.threads, "chunk_steps": args.chunk, "autotuned": tuned is not None},
         "latency_batch_ms": {"p50": float(np.percentile(step_ms, 50)), "p99": float(np.percentile(step_ms, 99))},
         "with_copy_back": with_copy(rsteps, copy_ms),
-        "roofline": {"bound": "valu", "achieved": achieved / 1e12 if achieved else None,
-                     "peak": PEAK_LANE_OPS / 1e12, "unit": "TOP/s",
-                     "frac": achieved / PEAK_LANE_OPS if achieved else None, "traffic": traffic,
-                     "kernel": kname,
-                     "kernel_ms": walk_avg, "work_per_launch": work,
-                     "work_model": "W = 6*MM + 2*MA lane-ops per residue-step on each walk's bytecode, constant-only "
-                                   "instructions excluded (per-step program evaluation; forward differences skip it, "
-                                   "so frac can pass 1)",
-                     "executed": executed},
+        "roofline": roofline(st, walk_avg, equiv, kname, args.workload),
         "cpu_baseline": cpu,
-        **({"sweep_walks_per_s": sweep_walks / (ms * 1e-3), "hit_selection": post} if args.workload == "sweep3f2"
-           else {}),
+        **({"sweep_walks_per_s": extra["sweep_walks"] / (ms * 1e-3), "hit_selection": post}
+           if args.workload == "sweep3f2" else {}),
     }))
 
 
@@ -391,21 +474,23 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--walks", type=int, default=4096)
+    ap.add_argument("--walks", type=int, default=0)
     ap.add_argument("--plain-sample", type=int, default=32)
     ap.add_argument("--cpu-one", type=int, default=32)
     ap.add_argument("--cpu-all", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-rate", type=float, default=1000.0)
-    ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--no-extra", action="store_true", help="skip the cmf3 / mix4 batches of the default line")
+    ap.add_argument("--serve-rates", default="100,1000,3000,10000")
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-budget-gib", type=float, default=4.0)
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--mode", choices=("batched", "persistent"), default="batched")
     ap.add_argument("--graph", action="store_true", help="replay the batch as a hipGraph")
     ap.add_argument("--tune", action="store_true", help="pcf2: autotune the launch shape first")
     ap.add_argument("--workload", choices=("pcf2", "cmf3", "mix4", "zeta5", "sweep3f2"), default="pcf2",
-                    help="pcf2 (the headline line); cmf3 / mix4 / zeta5 / sweep3f2 measure the other batch shapes "
-                         "(no served leg; a CPU leg for zeta5 / sweep3f2)")
+                    help="pcf2 (the headline line); cmf3 / mix4 / zeta5 / sweep3f2 measure the other batch shapes")
     ap.add_argument("--profile", action="store_true",
                     help="for counter collection: run the batch --steps times and a 512 MiB device copy "
                          "(the calibration kernel for FETCH_SIZE / WRITE_SIZE), print nothing else")
@@ -416,18 +501,25 @@ def main():
         sys.exit(2)
 
     import dreams_rns as D
-    from dreams_rns import synth
-    from dreams_rns.engine import BatchPlan, pcf_spec
+    from dreams_rns.engine import BatchPlan
 
     t0 = time.perf_counter()
-    if args.workload != "pcf2":
-        return other_workload(args)
-    cfg = synth.pcf2(n_walks=args.walks)
-    depth, K, top_k = cfg["depth"], cfg["K"], cfg["top_k"]
-    progs = [D.compile_pcf(w["a"], w["b"]) for w in cfg["walks"]]
-    specs = [pcf_spec(p, D.pcf_initial_values(w["a"]), depth, K, 1, target=w["L"])
-             for p, w in zip(progs, cfg["walks"])]
-    work = sum(program_work(p) * depth * K for p in progs)
+    if args.workload != "pcf2" and not args.profile:
+        return workload_line(args)
+    if args.profile and args.workload != "pcf2":
+        specs, top_k, _, _ = build_workload(args.workload, args.walks)
+        with BatchPlan(specs, crt=True, top_k=top_k, threads=args.threads or 256, chunk_steps=args.chunk or 512,
+                       mode=args.mode) as plan:
+            for _ in range(args.steps):
+                plan.run()
+            ms = plan.ctx.copy_calibrate(512 << 20, reps=1)
+        log("profile run done: %s x%d, copy 512 MiB (%.3f ms)" % (args.workload, args.steps, ms))
+        return
+    specs, top_k, desc, extra = build_workload("pcf2", args.walks)
+    cfg = extra["cfg"]
+    depth, K = cfg["depth"], cfg["K"]
+    progs = [s.program for s in specs]
+    equiv = sum(program_work(p) * depth * K for p in progs)
     rsteps = float(len(specs) * depth * K)
     log(f"compiled {len(progs)} PCFs in {time.perf_counter() - t0:.1f}s")
 
@@ -440,13 +532,12 @@ def main():
         args.threads, args.chunk = tuned["threads"], tuned["chunk_steps"]
         log(f"autotuned in {time.perf_counter() - t:.1f}s: threads {args.threads}, chunk {args.chunk}")
     # without --tune the line uses the shape the autotuner picks for this
-    # batch (64 lanes, 4096 steps per task: 0.80 ms/step against 0.88 for
-    # 256 lanes, whose faster walk kernel crowds out the FP64 M_f / shadow
-    # chain that sets the step time), fixed so that the walk kernel's time
-    # matches its committed kernel trace
+    # batch (64 lanes, 4096 steps per task: the faster 256-lane walk kernel
+    # crowds out the FP64 M_f / shadow chain that sets the step time), fixed
+    # so that the walk kernel's time matches its committed kernel trace
     args.threads, args.chunk = args.threads or 64, args.chunk or 4096
     plan = BatchPlan(specs, crt=True, top_k=top_k, threads=args.threads, chunk_steps=args.chunk,
-                     use_graph=args.graph)
+                     use_graph=args.graph, mode=args.mode)
     log(f"plan built: {time.perf_counter() - t0:.1f}s since start")
     ctx = plan.ctx
     if args.profile:
@@ -456,22 +547,13 @@ def main():
         ms = ctx.copy_calibrate(512 << 20, reps=1)
         log("profile run done: batch x%d, copy 512 MiB (%.3f ms)" % (args.steps, ms))
         return
-    for _ in range(args.warmup):
-        plan.run()
-    walk_ms, step_ms = [], []
-    for i in range(args.steps):
-        t = time.perf_counter()
-        plan.run(fetch=False)  # the outputs stay in HBM
-        step_ms.append((time.perf_counter() - t) * 1e3)
-        tm = ctx.last_timing()
-        walk_ms.append(tm[0])
-        log(f"step {i}: {step_ms[-1]:.2f} ms (walk kernel {tm[0]:.3f} ms, combine {tm[1]:.3f}, "
-            f"shadow wait {tm[2]:.3f}, tail {tm[3]:.3f}, device {tm[4]:.3f})")
+    step_ms, walk_ms = timed_steps(plan, args.steps, args.warmup, "pcf2")
     copy_ms = timed_with_copy(plan, args.steps)
     res = plan.result
     ms = float(np.mean(step_ms))
     value = rsteps / (ms * 1e-3)
     walk_avg = float(np.mean(walk_ms)) if not args.graph else float("nan")
+    st = plan.stats()
     # bit-exactness spot check of the timed output against the C restatement
     from oracle import cref
 
@@ -497,28 +579,26 @@ def main():
 
     served = None
     if not args.no_serve:
-        served = served_run(specs[:512], args.serve_rate, args.serve_seconds, round_steps=1000)
-        log(f"served: {served}")
+        rates = [float(r) for r in args.serve_rates.split(",") if r]
+        served = served_sweep(rates, args.serve_seconds, int(args.serve_budget_gib * (1 << 30)))
+
+    extra_lines = {}
+    if not args.no_extra:
+        for name in ("cmf3", "mix4"):
+            extra_lines[name] = batch_rate(name)
+            log(f"{name}: {extra_lines[name]['value']:.3e} residue-steps/s, {extra_lines[name]['ms_per_step']:.1f} ms")
 
     cpu = None
     if not args.no_cpu:
-        cores = min(16, os.cpu_count() or 1)
+        cores, visible = cpu_share()
         one, allc = cpu_baseline(cfg["walks"], depth, K, args.cpu_one, args.cpu_all, cores)
-        cpu = {"value": allc, "unit": "residue-steps/s", "cores": cores, "kind": "port",
-               "one_core": one,
-               "sample": f"{args.cpu_all} pcf2 walks (all cores) and {args.cpu_one} (one core) at depth "
-                         f"{depth}, K={K}: numpy restatement of run_pcf_walk + crt_reconstruct + centered"}
-        log(f"cpu: one core {one:.3e}, {cores} cores {allc:.3e} residue-steps/s")
+        cpu = {"value": allc, "unit": "residue-steps/s", "cores": cores, "kind": "port", "one_core": one,
+               "cpu_model": cpu_model(), "cpus_visible": visible,
+               "sample": f"{args.cpu_all} pcf2 walks ({cores} processes, one thread each) and {args.cpu_one} (one "
+                         f"core) at depth {depth}, K={K}: numpy restatement of run_pcf_walk + crt_reconstruct + "
+                         f"centered"}
+        log(f"cpu ({cpu['cpu_model']}): one core {one:.3e}, {cores} cores {allc:.3e} residue-steps/s")
 
-    achieved = work / (walk_avg * 1e-3) if walk_avg == walk_avg else None
-    kname = walk_kernel_name(progs)
-    cnt = measured_counters(kname, "pcf2")
-    traffic = float(cnt["traffic_bytes"]) if cnt else None
-    executed = None
-    if cnt and walk_avg == walk_avg:
-        ex = float(cnt["valu_insts"]) * 64 / (walk_avg * 1e-3)
-        executed = {"achieved": ex / 1e12, "frac": ex / PEAK_LANE_OPS,
-                    "valu_lane_ops_per_launch": float(cnt["valu_insts"]) * 64}
     line = {
         "metric": METRIC,
         "value": value,
@@ -532,23 +612,15 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"pcf2: {len(specs)} seeded PCFs, deg a<=3, deg b<=6, coef [-20,20], "
-                               f"depth {depth}, K={K}, FP64 shadow + delta_float, GPU Garner CRT, top-{top_k}",
-                   "walks": len(specs), "depth": depth, "K": K, "top_k": top_k,
+        "config": {"workload": desc, "walks": len(specs), "depth": depth, "K": K, "top_k": top_k,
                    "threads": args.threads, "chunk_steps": args.chunk, "graph": bool(args.graph),
-                   "autotuned": tuned is not None},
+                   "mode": args.mode, "autotuned": tuned is not None},
         "latency_batch_ms": {"p50": float(np.percentile(step_ms, 50)), "p99": float(np.percentile(step_ms, 99))},
         "with_copy_back": with_copy(rsteps, copy_ms),
         "plain": {"value": plain, "sample_walks": n_plain, "x": value / plain},
         "served": served,
-        "roofline": {"bound": "valu", "achieved": achieved / 1e12 if achieved else None,
-                     "peak": PEAK_LANE_OPS / 1e12, "unit": "TOP/s",
-                     "frac": achieved / PEAK_LANE_OPS if achieved else None, "traffic": traffic,
-                     "kernel": kname, "kernel_ms": walk_avg,
-                     "work_per_launch": work,
-                     "work_model": "W = 6*MM + 2*MA lane-ops per residue-step on each walk's bytecode "
-                                   "(per-step program evaluation; forward differences skip it, so frac can pass 1)",
-                     "executed": executed},
+        **extra_lines,
+        "roofline": roofline(st, walk_avg, equiv, walk_kernel_name(progs), "pcf2"),
         "cpu_baseline": cpu,
     }
     if cpu:
