@@ -104,10 +104,11 @@ int walk_shape(const HostProgram& hp, int kind) {
 }
 
 size_t device_lds_limit(drx_ctx* ctx) {
+  // less 64 bytes: the persistent walk kernels' task word is static LDS beside the dynamic buffer
   int v = 0;
-  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) == hipSuccess && v > 0)
-    return (size_t)v;
-  return 65536;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) == hipSuccess && v > 64)
+    return (size_t)v - 64;
+  return 65536 - 64;
 }
 
 void merge_lds_classes(std::vector<Stage>& stages) {
@@ -264,11 +265,19 @@ int launch_stages(drx_ctx* ctx, const KArgs& a, const std::vector<Stage>& stages
   };
   int rc = mf_window(0);
   if (rc) return rc;
+  bool persist = false;
+  for (const Stage& g : stages) persist = persist || g.persist > 0;
+  if (persist) {
+    if (!a.queue) return fail(ctx, DRX_EINVAL, "internal: persistent stages without task counters");
+    HIPCHK(ctx, launch_fill_u32(a.queue, (int)stages.size(), 0u, s1));
+  }
   if (timed) HIPCHK(ctx, hipEventRecord(ev[0], s1));
-  for (const Stage& g : stages) {
+  for (size_t gi = 0; gi < stages.size(); ++gi) {
+    const Stage& g = stages[gi];
     KArgs wa = a;
     wa.tasks += g.task_begin;
-    HIPCHK(ctx, launch_walk(g.m, g.kind, wa, g.task_end - g.task_begin, g.threads, g.lds, s1));
+    HIPCHK(ctx, launch_walk(g.m, g.kind, wa, g.task_end - g.task_begin, g.threads, g.lds, s1, g.persist,
+                            g.persist > 0 ? a.queue + gi : nullptr));
   }
   if (timed) HIPCHK(ctx, hipEventRecord(ev[1], s1));
   for (int c = 0; c < nwin; ++c) {
@@ -497,6 +506,7 @@ struct drx_plan {
   DBuf d_walks, d_tasks, d_seeds, d_spans, d_ranges, d_res0, d_fst0, d_res, d_fst, d_axr, d_axi, d_axd, d_cp, d_mf,
       d_fdt;
   DBuf d_fout, d_rout, d_pout, d_pfull, d_limbs, d_neg, d_topk, d_topk_ws, d_rowk;
+  DBuf d_queue;  // persistent mode: one task counter per stage
   KArgs args{};
   GarnerArgs gargs{};   // rows of at most 256 primes: the wavefront Garner kernel
   GarnerArgs wargs{};   // wider rows outside the matrix path: the wide Garner kernel
@@ -524,7 +534,7 @@ struct drx_plan {
     for (DBuf* d : {&d_walks, &d_tasks, &d_seeds, &d_spans, &d_ranges, &d_res0, &d_fst0, &d_res, &d_fst, &d_axr, &d_axi,
                     &d_axd, &d_cp, &d_mf, &d_fdt, &d_fout, &d_rout, &d_pout, &d_pfull, &d_limbs, &d_neg, &d_topk,
                     &d_topk_ws, &d_rowk, &d_vgroups, &d_fgroups, &d_vorder, &d_grows, &d_wrows, &d_crt_a,
-                    &d_crt_c, &d_crt_k})
+                    &d_crt_c, &d_crt_k, &d_queue})
       d->release();
     for (auto& g : crt_groups) g->d_rows.release();
     for (drx_plan* p : plain) delete p;
@@ -1205,6 +1215,24 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
     if (need > lds_limit)
       return fail(ctx, DRX_ERANGE, "program needs %zu bytes of LDS (limit %zu); use fewer threads", need, lds_limit);
   }
+  if (pl->cfg.mode == DRX_MODE_PERSISTENT) {
+    // each stage's walk kernel as a persistent grid of the workgroups the
+    // device holds at once, pulling the stage's tasks (longest first) from a
+    // queue; the whole sequence is replayed as one captured graph
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || ncu < 1)
+      ncu = 1;
+    for (Stage& g : pl->stages) {
+      // four times the resident workgroups, each retiring after its quota of
+      // tasks: a grid of only the resident ones would hold every slot until
+      // its queue is empty and starve the FP64 M_f pass on the other stream
+      // (mix4: 2.82 s per batch against 1.90 s)
+      const int occ = walk_blocks_per_cu(g.m, g.kind, g.threads, g.lds, true);
+      g.persist = 4 * std::max(1, occ) * ncu;
+    }
+    HIPCHK(ctx, pl->d_queue.ensure(std::max<size_t>(1, pl->stages.size()) * 4));
+    pl->cfg.use_graph = 1;
+  }
   {
     int64_t maxd = 0;
     for (int i = 0; i < n; ++i) maxd = std::max<int64_t>(maxd, b.walks[i].depth);
@@ -1248,6 +1276,7 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
 
   KArgs& a = pl->args;
   a.tasks = pl->d_tasks.as<DevTask>();
+  a.queue = pl->d_queue.p ? pl->d_queue.as<uint32_t>() : nullptr;
   a.walks = pl->d_walks.as<DevWalk>();
   a.progs = ctx->d_progs.as<DevProgram>();
   a.spans = pl->d_spans.as<DevSpan>();

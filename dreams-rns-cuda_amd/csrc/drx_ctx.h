@@ -142,6 +142,7 @@ struct Stage {
   int range_begin = 0, range_end = 0;  // M_f blocks, ordered by window
   std::vector<int> win_range;          // [nwin + 1]: ranges of window c are win_range[c] .. win_range[c + 1]
   int threads = 0, seed_kp = 0;    // walk workgroup; seed primes per point group (pow2 >= every K)
+  int persist = 0;                 // > 0: persistent walk kernel of this many workgroups (task queue)
   size_t lds = 0, seed_lds = 0, mf_lds = 0;
 };
 

@@ -41,6 +41,7 @@ struct KArgs {
   const DevVGroup* vgroups;  // vector walks: residue groups
   const DevVGroup* fgroups;  // vector walks: FP64 groups (one lane per walk)
   const int32_t* vorder;     // vector walks: walk indices grouped
+  uint32_t* queue;           // persistent walk kernels: one task counter per stage
   double* axr_w;             // server admission: writable views of axr / axi / axd
   int64_t* axi_w;
   double* axd_w;
@@ -91,14 +92,20 @@ inline size_t garner_wide_lds(int kmax) {
 }
 
 
-hipError_t launch_walk(int m, int kind, const KArgs& a, int ntask, int threads, size_t lds, hipStream_t s);
+// Walk kernel of one stage over a.tasks[0 .. ntask): one workgroup per task,
+// or with persist_blocks > 0 a persistent grid of that many workgroups
+// (at most ntask) taking tasks from the counter `queue` (cleared before).
+hipError_t launch_walk(int m, int kind, const KArgs& a, int ntask, int threads, size_t lds, hipStream_t s,
+                       int persist_blocks = 0, uint32_t* queue = nullptr);
+hipError_t launch_fill_u32(uint32_t* p, int n, uint32_t v, hipStream_t s);
 // Vector walks: residues (one block per residue group of `threads` lanes) and
 // the FP64 shadow (one block per FP64 group, one lane per walk).
 hipError_t launch_vec_walk(const KArgs& a, int ngroup, int threads, size_t lds, hipStream_t s);
 hipError_t launch_vec_fp(const KArgs& a, int ngroup, int threads, size_t lds, hipStream_t s);
-// Resident workgroups per CU of the interpreting walk kernel for order m at
-// this workgroup size and LDS; -1 when unknown.
-int walk_blocks_per_cu(int m, int kind, int threads, size_t lds);
+// Resident workgroups per CU of the walk kernel of order m and family kind
+// (its grid or its persistent variant) at this workgroup size and LDS; -1
+// when unknown.
+int walk_blocks_per_cu(int m, int kind, int threads, size_t lds, bool persistent = false);
 // Newton tables of forward-difference walks: one block per seed task (a
 // prime slice of a walk, a.tasks) of `threads` lanes, `kp` (power of two >=
 // every slice's prime count) per point group; threads % kp == 0.

@@ -1,6 +1,8 @@
 // Walk kernels for gfx950: bytecode interpreter over residue channels, chunked
 // P*M products with an ordered LDS tree combine, FP64 shadow, finalisation,
 // Garner CRT and top-k.
+#include <algorithm>
+
 #include "../../include/dreams_hip.h"
 #include "drx_kernels.h"
 
@@ -192,9 +194,9 @@ struct WalkSetup {
 };
 
 // Task, walk and lane geometry; with `interp`, the program constants in LDS.
-__device__ __forceinline__ WalkSetup walk_setup(const KArgs& a, double* lds, bool interp) {
+__device__ __forceinline__ WalkSetup walk_setup(const KArgs& a, double* lds, bool interp, int task) {
   WalkSetup S;
-  S.t = a.tasks[blockIdx.x];
+  S.t = a.tasks[task];
   S.w = a.walks[S.t.walk];
   S.pg = a.progs[S.w.prog];
   const uint32_t T = blockDim.x, tid = threadIdx.x;
@@ -343,11 +345,37 @@ __device__ __forceinline__ void tree_combine(const KArgs& a, const WalkSetup& S,
   }
 }
 
+// Task loop of a walk kernel. A grid launch has one workgroup per task (its
+// block index). A persistent launch has a smaller grid: each workgroup takes
+// the stage's next task from `queue` -- a counter in device memory cleared
+// before the launch, advanced by a vector-memory atomic -- in task order
+// (longest first), until none is left or it has run `quota` tasks (then it
+// retires, so that work of the other streams, the FP64 M_f pass above all,
+// gets workgroup slots; grid * quota >= ntask, so every task is taken). The
+// queue balances uneven tasks and no workgroup ever waits for another. The
+// task index is uniform over the workgroup; the barrier after it is read
+// orders the next task's LDS setup after every lane's end of the previous task.
+template <bool PERSIST, typename F>
+__device__ __forceinline__ void for_tasks(int ntask, uint32_t* queue, int quota, F body) {
+  if (!PERSIST) {
+    body((int)blockIdx.x);
+    return;
+  }
+  __shared__ int next;
+  for (int done = 0; done < quota; ++done) {
+    if (threadIdx.x == 0) next = (int)atomicAdd(queue, 1u);
+    __syncthreads();
+    const int t = next;
+    __syncthreads();  // every lane has read `next` before it is written again
+    if (t >= ntask) break;
+    body(t);
+  }
+}
+
 // Interpreting walk: the program runs once per step and residue channel.
 template <int M>
-__global__ void __launch_bounds__(256) k_walk_chunk(const KArgs a) {
-  extern __shared__ double lds[];
-  const WalkSetup S = walk_setup(a, lds, true);
+__device__ __forceinline__ void walk_chunk_task(const KArgs& a, double* lds, int task) {
+  const WalkSetup S = walk_setup(a, lds, true, task);
   const double p = S.ln.p, pinv = S.ln.pinv;
   v16d R = 0.0, C = 0.0;
   load_rns_inputs(a, S.w, S.pg, S.Kw, S.kg, S.ln, S.sb, C, S.c == 0 ? S.li : nullptr, S.KP, S.k);
@@ -361,6 +389,12 @@ __global__ void __launch_bounds__(256) k_walk_chunk(const KArgs a) {
     advance_axes(C, S.li, S.KP, S.k, S.pg.dim, S.ln);
   }
   tree_combine<M>(a, S, P);
+}
+
+template <int M, bool PERSIST>
+__global__ void __launch_bounds__(256) k_walk_chunk(const KArgs a, int ntask, uint32_t* queue, int quota) {
+  extern __shared__ double lds[];
+  for_tasks<PERSIST>(ntask, queue, quota, [&](int task) { walk_chunk_task<M>(a, lds, task); });
 }
 
 // ------------------------------------------------ forward-difference walks
@@ -532,13 +566,18 @@ __device__ __forceinline__ void pcf_fd_body(const KArgs& a, const WalkSetup& S) 
 
 // Companion-PCF walks; the task's shape (8 deg b + deg a) picks a fully
 // unrolled body (the shape is uniform over the workgroup).
-__global__ void __launch_bounds__(256) k_walk_fd_pcf(const KArgs a) {
-  extern __shared__ double lds[];
-  const WalkSetup S = walk_setup(a, lds, false);
+__device__ __forceinline__ void walk_fd_pcf_task(const KArgs& a, double* lds, int task) {
+  const WalkSetup S = walk_setup(a, lds, false, task);
   switch (S.t.shape) {
     DRX_PCF_ROW(0) DRX_PCF_ROW(1) DRX_PCF_ROW(2) DRX_PCF_ROW(3) DRX_PCF_ROW(4) DRX_PCF_ROW(5) DRX_PCF_ROW(6)
     default: break;
   }
+}
+
+template <bool PERSIST>
+__global__ void __launch_bounds__(256) k_walk_fd_pcf(const KArgs a, int ntask, uint32_t* queue, int quota) {
+  extern __shared__ double lds[];
+  for_tasks<PERSIST>(ntask, queue, quota, [&](int task) { walk_fd_pcf_task(a, lds, task); });
 }
 #undef DRX_PCF_ROW
 #undef DRX_PCF_CASE
@@ -547,11 +586,10 @@ __global__ void __launch_bounds__(256) k_walk_fd_pcf(const KArgs a) {
 // every general entry's table is advanced at the full depth D (the Newton
 // differences above an entry's degree are zero).
 template <int M>
-__global__ void __launch_bounds__(256) k_walk_fd(const KArgs a) {
+__device__ __forceinline__ void walk_fd_task(const KArgs& a, double* lds, int task) {
   constexpr int D = fd_dmax(M);
   constexpr int U = fd_unroll(D);
-  extern __shared__ double lds[];
-  const WalkSetup S = walk_setup(a, lds, false);
+  const WalkSetup S = walk_setup(a, lds, false, task);
   const double p = S.ln.p, pinv = S.ln.pinv;
   const uint32_t kinds = S.pg.kinds;
   double N[M * M][D + 1];
@@ -581,6 +619,12 @@ __global__ void __launch_bounds__(256) k_walk_fd(const KArgs a) {
     }
   }
   tree_combine<M>(a, S, P);
+}
+
+template <int M, bool PERSIST>
+__global__ void __launch_bounds__(256) k_walk_fd(const KArgs a, int ntask, uint32_t* queue, int quota) {
+  extern __shared__ double lds[];
+  for_tasks<PERSIST>(ntask, queue, quota, [&](int task) { walk_fd_task<M>(a, lds, task); });
 }
 
 // Companion-shaped programs (ones on the sub-diagonal, zeros elsewhere but
@@ -648,9 +692,8 @@ __device__ __forceinline__ void comp_fd_body(const KArgs& a, const WalkSetup& S)
 // which the kernel uses when the task is one sub-chunk (C == 1): half the
 // products for a 4x4 companion.
 template <int M>
-__global__ void __launch_bounds__(256) k_walk_fd_comp(const KArgs a) {
-  extern __shared__ double lds[];
-  const WalkSetup S = walk_setup(a, lds, false);
+__device__ __forceinline__ void walk_fd_comp_task(const KArgs& a, double* lds, int task) {
+  const WalkSetup S = walk_setup(a, lds, false, task);
   const bool rows = (S.t.shape & kShapeRows) != 0 && S.C == 1;  // uniform
   switch ((S.t.shape & (kShapeRows - 1)) * 2 + (rows ? 1 : 0)) {
     case 0: comp_fd_body<M, 0, false>(a, S); break;
@@ -663,6 +706,12 @@ __global__ void __launch_bounds__(256) k_walk_fd_comp(const KArgs a) {
     case 7: comp_fd_body<M, 3, true>(a, S); break;
     default: break;
   }
+}
+
+template <int M, bool PERSIST>
+__global__ void __launch_bounds__(256) k_walk_fd_comp(const KArgs a, int ntask, uint32_t* queue, int quota) {
+  extern __shared__ double lds[];
+  for_tasks<PERSIST>(ntask, queue, quota, [&](int task) { walk_fd_comp_task<M>(a, lds, task); });
 }
 
 // --------------------------------------------------------- ordered combine
@@ -1739,27 +1788,40 @@ __global__ void k_topk_rank(const double* __restrict__ d, int stride, int n, int
 }
 
 // ---------------------------------------------------------------- launchers
-hipError_t launch_walk(int m, int kind, const KArgs& a, int ntask, int threads, size_t lds, hipStream_t s) {
-  if (ntask <= 0) return hipSuccess;
-  const dim3 grid(ntask), block(threads);
+template <bool P>
+static hipError_t launch_walk_t(int m, int kind, const KArgs& a, int ntask, dim3 grid, int threads, size_t lds,
+                                hipStream_t s, uint32_t* queue, int quota) {
+  const dim3 block(threads);
   if (kind == WALK_FD_PCF) {
     if (m != 2) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_walk_fd_pcf, grid, block, lds, s, a);
+    hipLaunchKernelGGL((k_walk_fd_pcf<P>), grid, block, lds, s, a, ntask, queue, quota);
   } else if (kind == WALK_FD) {
-    if (m == 2) hipLaunchKernelGGL(k_walk_fd<2>, grid, block, lds, s, a);
-    else if (m == 3) hipLaunchKernelGGL(k_walk_fd<3>, grid, block, lds, s, a);
+    if (m == 2) hipLaunchKernelGGL((k_walk_fd<2, P>), grid, block, lds, s, a, ntask, queue, quota);
+    else if (m == 3) hipLaunchKernelGGL((k_walk_fd<3, P>), grid, block, lds, s, a, ntask, queue, quota);
     else return hipErrorInvalidValue;
   } else if (kind == WALK_FD_COMP) {
-    if (m == 3) hipLaunchKernelGGL(k_walk_fd_comp<3>, grid, block, lds, s, a);
-    else if (m == 4) hipLaunchKernelGGL(k_walk_fd_comp<4>, grid, block, lds, s, a);
+    if (m == 3) hipLaunchKernelGGL((k_walk_fd_comp<3, P>), grid, block, lds, s, a, ntask, queue, quota);
+    else if (m == 4) hipLaunchKernelGGL((k_walk_fd_comp<4, P>), grid, block, lds, s, a, ntask, queue, quota);
     else return hipErrorInvalidValue;
   } else {
-    if (m == 2) hipLaunchKernelGGL(k_walk_chunk<2>, grid, block, lds, s, a);
-    else if (m == 3) hipLaunchKernelGGL(k_walk_chunk<3>, grid, block, lds, s, a);
-    else if (m == 4) hipLaunchKernelGGL(k_walk_chunk<4>, grid, block, lds, s, a);
+    if (m == 2) hipLaunchKernelGGL((k_walk_chunk<2, P>), grid, block, lds, s, a, ntask, queue, quota);
+    else if (m == 3) hipLaunchKernelGGL((k_walk_chunk<3, P>), grid, block, lds, s, a, ntask, queue, quota);
+    else if (m == 4) hipLaunchKernelGGL((k_walk_chunk<4, P>), grid, block, lds, s, a, ntask, queue, quota);
     else return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+hipError_t launch_walk(int m, int kind, const KArgs& a, int ntask, int threads, size_t lds, hipStream_t s,
+                       int persist_blocks, uint32_t* queue) {
+  if (ntask <= 0) return hipSuccess;
+  if (persist_blocks > 0) {
+    if (!queue) return hipErrorInvalidValue;
+    const int grid = std::min(persist_blocks, ntask);
+    const int quota = (ntask + grid - 1) / grid;
+    return launch_walk_t<true>(m, kind, a, ntask, dim3(grid), threads, lds, s, queue, quota);
+  }
+  return launch_walk_t<false>(m, kind, a, ntask, dim3(ntask), threads, lds, s, nullptr, 1);
 }
 
 hipError_t launch_vec_walk(const KArgs& a, int ngroup, int threads, size_t lds, hipStream_t s) {
@@ -1776,15 +1838,42 @@ hipError_t launch_vec_fp(const KArgs& a, int ngroup, int threads, size_t lds, hi
   return hipGetLastError();
 }
 
-int walk_blocks_per_cu(int m, int kind, int threads, size_t lds) {
+template <bool P>
+static int walk_blocks_per_cu_t(int m, int kind, int threads, size_t lds) {
   int blocks = 0;
   hipError_t e = hipErrorInvalidValue;
   if (kind == WALK_INTERP) {
-    if (m == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_walk_chunk<2>, threads, lds);
-    else if (m == 3) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_walk_chunk<3>, threads, lds);
-    else if (m == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_walk_chunk<4>, threads, lds);
+    if (m == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_walk_chunk<2, P>, threads, lds);
+    else if (m == 3) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_walk_chunk<3, P>, threads, lds);
+    else if (m == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_walk_chunk<4, P>, threads, lds);
+  } else if (kind == WALK_FD_PCF && m == 2) {
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_walk_fd_pcf<P>, threads, lds);
+  } else if (kind == WALK_FD) {
+    if (m == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_walk_fd<2, P>, threads, lds);
+    else if (m == 3) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_walk_fd<3, P>, threads, lds);
+  } else if (kind == WALK_FD_COMP) {
+    if (m == 3) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_walk_fd_comp<3, P>, threads, lds);
+    else if (m == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_walk_fd_comp<4, P>, threads, lds);
   }
   return e == hipSuccess ? blocks : -1;
+}
+
+int walk_blocks_per_cu(int m, int kind, int threads, size_t lds, bool persistent) {
+  return persistent ? walk_blocks_per_cu_t<true>(m, kind, threads, lds)
+                    : walk_blocks_per_cu_t<false>(m, kind, threads, lds);
+}
+
+// Counters of the persistent walk kernels' task queues, cleared by a kernel
+// (not a memset) so that the clear can sit in a captured graph.
+__global__ void k_fill_u32(uint32_t* p, int n, uint32_t v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+hipError_t launch_fill_u32(uint32_t* p, int n, uint32_t v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
 }
 
 hipError_t launch_fd_seed(int m, const KArgs& a, int nseed, int kp, int threads, size_t lds, hipStream_t s) {

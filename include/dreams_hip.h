@@ -158,6 +158,12 @@ typedef struct drx_batch {
   int32_t* topk_idx;              /* [top_k] walk indices by delta_float desc, ties low index */
 } drx_batch;
 
+/* DRX_MODE_PERSISTENT: each stage's walk kernel (one per matrix order and
+ * kernel family) runs as a persistent grid of the workgroups the device
+ * holds at once, which take the stage's tasks (step ranges of walks, longest
+ * first) from a device queue; the whole launch sequence, FP64 shadow streams
+ * included, is captured as one hipGraph and replayed per run. Results are
+ * those of DRX_MODE_BATCHED. State-vector walks keep their grid kernels. */
 typedef struct drx_launch_cfg {
   int mode;         /* DRX_MODE_* */
   int threads;      /* workgroup size, power of two in [32, 256]; 0 = tuned */

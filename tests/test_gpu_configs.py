@@ -95,13 +95,55 @@ def test_mix4_shape_batched_equals_plain_and_oracle(gpu_ctx):
             _oracle_check(res, i, specs[i], primes)
     with BatchPlan(specs, mode="plain", crt=True, top_k=16) as plan:
         b = _copy(plan.run())
-    assert np.array_equal(a["residues"], b["residues"])
-    assert np.array_equal(a["limbs"], b["limbs"]) and np.array_equal(a["neg"], b["neg"])
-    for key in ("p_float", "q_float", "log_scale", "delta_float"):
-        assert [float(v).hex() for v in a["out"][key]] == [float(v).hex() for v in b["out"][key]], key
-    assert a["topk"].tolist() == b["topk"].tolist()
+    # persistent mode: task-queue walk kernels, the whole sequence one graph,
+    # replayed three times with host work between the replays
+    with BatchPlan(specs, mode="persistent", crt=True, top_k=16) as plan:
+        c = [_copy(plan.run()) for _ in range(3)]
+    for other in [b] + c:
+        assert np.array_equal(a["residues"], other["residues"])
+        assert np.array_equal(a["limbs"], other["limbs"]) and np.array_equal(a["neg"], other["neg"])
+        for key in ("p_float", "q_float", "log_scale", "delta_float"):
+            assert [float(v).hex() for v in a["out"][key]] == [float(v).hex() for v in other["out"][key]], key
+        assert a["topk"].tolist() == other["topk"].tolist()
     dl = a["out"]["delta_float"].tolist()
     assert a["topk"].tolist() == sorted(range(len(dl)), key=lambda j: (-dl[j], j))[:16]
+
+
+def _family(spec):
+    # the walk kernel family the library picks for this program
+    from dreams_rns._native import pack_program
+
+    meta, m = pack_program(spec.program)[2], spec.program.m
+    if meta["fd_ok"] and m == 2 and meta["fd_degree"] <= 6:
+        k = meta["kinds"]
+        return "fd_pcf" if (k & 3) == 0 and ((k >> 4) & 3) == 1 else "fd"
+    if meta["fd_ok"] and m in (3, 4) and meta["fd_degree"] <= 3 and all(
+            ((meta["kinds"] >> (2 * (i * m + j))) & 3) == (1 if i == j + 1 else 0)
+            for i in range(m) for j in range(m - 1)):
+        return "fd_comp"
+    if meta["fd_ok"] and m == 3 and meta["fd_degree"] <= 3:
+        return "fd"
+    return "interp"
+
+
+def test_mix4_costliest_walks_against_c_oracle(gpu_ctx):
+    # for every (matrix order, kernel family) of a 400-walk mix4 batch, the
+    # deepest walk at the highest K (depth up to 50000, K 128, interpreted
+    # 4x4 programs with INV) against the C restatement, in the batch
+    specs = _mix_specs(400)
+    groups = {}
+    for i, s in enumerate(specs):
+        groups.setdefault((s.program.m, _family(s)), []).append(i)
+    picks = {key: max(idx, key=lambda j: (specs[j].depth * specs[j].K, j)) for key, idx in groups.items()}
+    assert {(2, "fd_pcf"), (3, "interp"), (4, "interp")} <= set(picks)
+    primes = D.generate_rns_primes(128)
+    with BatchPlan(specs, crt=True, top_k=16, threads=256, chunk_steps=512) as plan:
+        res = plan.run()
+        for key, i in sorted(picks.items()):
+            s = specs[i]
+            print("mix4 costliest", key, "walk", i, "depth", s.depth, "K", s.K, len(s.program.opcodes), "ops",
+                  flush=True)
+            _oracle_check(res, i, s, primes)
 
 
 def test_pcf2_full_batch_size_independent_properties(gpu_ctx):
