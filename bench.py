@@ -491,6 +491,8 @@ def main():
     ap.add_argument("--tune", action="store_true", help="pcf2: autotune the launch shape first")
     ap.add_argument("--workload", choices=("pcf2", "cmf3", "mix4", "zeta5", "sweep3f2"), default="pcf2",
                     help="pcf2 (the headline line); cmf3 / mix4 / zeta5 / sweep3f2 measure the other batch shapes")
+    ap.add_argument("--rates", action="store_true",
+                    help="print the VALU issue rates (drx_valu_rate) behind the FP64 residue arithmetic and exit")
     ap.add_argument("--profile", action="store_true",
                     help="for counter collection: run the batch --steps times and a 512 MiB device copy "
                          "(the calibration kernel for FETCH_SIZE / WRITE_SIZE), print nothing else")
@@ -504,6 +506,17 @@ def main():
     from dreams_rns.engine import BatchPlan
 
     t0 = time.perf_counter()
+    if args.rates:
+        from dreams_rns import _native as N
+
+        ctx = N.context()
+        names = ["FP64 FMA", "32-bit multiply-high", "32x32+64-bit multiply-add", "residue modmul mm()"]
+        for op, name in enumerate(names):
+            r, pk = ctx.valu_rate(op)
+            ops = 6 if op == 3 else 1
+            print(json.dumps({"op": name, "per_s": r, "lane_ops_per_s": r * ops, "peak_lane_ops_per_s": pk,
+                              "frac_of_lane_peak": r * ops / pk, "lane_ops_each": ops}))
+        return
     if args.workload != "pcf2" and not args.profile:
         return workload_line(args)
     if args.profile and args.workload != "pcf2":

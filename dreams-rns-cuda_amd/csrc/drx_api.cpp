@@ -2140,6 +2140,29 @@ int drx_copy_calibrate(drx_ctx* ctx, uint64_t bytes, int reps, double* ms) {
   return DRX_OK;
 }
 
+int drx_valu_rate(drx_ctx* ctx, int op, double* ops_per_s, double* peak_per_s) {
+  if (!ctx || !ops_per_s || !peak_per_s || op < 0 || op > 3) return DRX_EINVAL;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  int ncu = 0, khz = 0;
+  HIPCHK(ctx, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  HIPCHK(ctx, hipDeviceGetAttribute(&khz, hipDeviceAttributeClockRate, ctx->device));
+  HIPCHK(ctx, ctx->d_tmp1.ensure(64));
+  const int blocks = 8 * ncu, iters = op == 3 ? 4096 : 16384;
+  hipError_t e = launch_rate(op, blocks, 64, ctx->d_tmp1.as<double>(), ctx->stream);  // warm
+  if (e == hipSuccess) e = hipEventRecord(ctx->ev[10], ctx->stream);
+  if (e == hipSuccess) e = launch_rate(op, blocks, iters, ctx->d_tmp1.as<double>(), ctx->stream);
+  if (e == hipSuccess) e = hipEventRecord(ctx->ev[11], ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev[10], ctx->ev[11]);
+  HIPCHK(ctx, e);
+  const double ops = (double)blocks * 256 * iters * kRateChains;
+  *ops_per_s = ops / (ms * 1e-3);
+  *peak_per_s = (double)ncu * 64 * khz * 1e3;
+  return DRX_OK;
+}
+
 int drx_last_timing(const drx_ctx* ctx, double* t6) {
   if (!ctx || !t6) return DRX_EINVAL;
   t6[0] = ctx->timing.walk_ms;

@@ -145,5 +145,8 @@ struct TopCand {
 size_t topk_workspace_bytes(int n, int k);
 hipError_t launch_topk(const double* d, int stride, int n, int k, int32_t* out, void* ws, hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, size_t n16, hipStream_t s);
+// Issue-rate probe (kRateChains chains per lane, `iters` rounds) of op 0..3.
+constexpr int kRateChains = 8;
+hipError_t launch_rate(int op, int blocks, int iters, double* out, hipStream_t s);
 
 }  // namespace drx

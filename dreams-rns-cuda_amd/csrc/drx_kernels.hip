@@ -1957,6 +1957,52 @@ hipError_t launch_garner_wide(const GarnerArgs& g, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Issue-rate probe: every lane runs `iters` rounds of kRateChains
+// independent chains of one operation (no memory traffic but the final
+// store, which keeps the chains alive). op 0: FP64 FMA; 1: 32-bit
+// multiply-high; 2: 32x32+64-bit multiply-add; 3: the residue modmul mm()
+// (an FP64 two-product and quotient, 6 FP64 operations).
+
+template <int OP>
+__global__ void __launch_bounds__(256) k_rate(int iters, double a, double b, double* out) {
+  double x[kRateChains];
+  uint32_t u[kRateChains];
+  uint64_t w[kRateChains];
+#pragma unroll
+  for (int c = 0; c < kRateChains; ++c) {
+    x[c] = (double)(threadIdx.x + c);
+    u[c] = 0x9e3779b9u * (threadIdx.x + 1u + c);
+    w[c] = (uint64_t)u[c] << 7;
+  }
+  const uint32_t ua = (uint32_t)a | 1u;
+  const double p = 2147483647.0, pinv = 1.0 / 2147483647.0;
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int c = 0; c < kRateChains; ++c) {
+      if (OP == 0) x[c] = __builtin_fma(x[c], a, b);
+      else if (OP == 1) u[c] = __umulhi(u[c], ua + (uint32_t)c);
+      else if (OP == 2) w[c] = (uint64_t)u[c] * (uint64_t)(ua + (uint32_t)c) + w[c];
+      else x[c] = mm(x[c], b, p, pinv);
+    }
+  }
+  double acc = 0.0;
+#pragma unroll
+  for (int c = 0; c < kRateChains; ++c) acc += x[c] + (double)u[c] + (double)w[c];
+  if (acc == 1.2345) out[0] = acc;  // practically never: the chains stay live
+}
+
+hipError_t launch_rate(int op, int blocks, int iters, double* out, hipStream_t s) {
+  const dim3 grid(blocks), block(256);
+  switch (op) {
+    case 0: hipLaunchKernelGGL(k_rate<0>, grid, block, 0, s, iters, 1.0000001, 1e-9, out); break;
+    case 1: hipLaunchKernelGGL(k_rate<1>, grid, block, 0, s, iters, 2654435769.0, 0.0, out); break;
+    case 2: hipLaunchKernelGGL(k_rate<2>, grid, block, 0, s, iters, 2654435769.0, 0.0, out); break;
+    case 3: hipLaunchKernelGGL(k_rate<3>, grid, block, 0, s, iters, 0.0, 1234567.0, out); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 // Plain 16-byte-vector copy: the calibration kernel for HBM byte counters
 // (it moves exactly 2 x n16 x 16 bytes) and the achievable-bandwidth probe.
 __global__ void __launch_bounds__(256) k_copy_calib(const uint4* __restrict__ src, uint4* __restrict__ dst,

@@ -114,6 +114,7 @@ SIGNATURES = {
     "drx_host_alloc": (C.c_int, [C.c_uint64, C.c_void_p]),
     "drx_host_free": (None, [C.c_void_p]),
     "drx_last_timing": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "drx_valu_rate": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "drx_server_create": (C.c_int, [C.c_void_p, C.POINTER(ServerCfg), C.POINTER(C.c_void_p)]),
     "drx_server_submit": (C.c_int, [C.c_void_p, C.POINTER(Walk), C.c_void_p, C.POINTER(C.c_uint64)]),
     "drx_server_poll": (C.c_int, [C.c_void_p, C.POINTER(ServerResult), C.c_void_p, C.c_void_p, C.c_int, C.c_int,
@@ -265,6 +266,13 @@ class Context:
         idx = np.zeros(max(1, int(k)), dtype=np.int32)
         self._check(self.lib.drx_topk(self.h, _ptr(v), len(v), int(k), _ptr(idx)))
         return idx[: int(k)]
+
+    def valu_rate(self, op: int):
+        """(measured ops/s, peak lane-ops/s) of VALU op 0 FP64 FMA, 1 32-bit
+        multiply-high, 2 32x32+64 multiply-add, 3 the residue modmul."""
+        r, pk = C.c_double(0.0), C.c_double(0.0)
+        self._check(self.lib.drx_valu_rate(self.h, int(op), C.byref(r), C.byref(pk)))
+        return r.value, pk.value
 
     def copy_calibrate(self, nbytes: int, reps: int = 10) -> float:
         """Mean device ms of a plain copy of ``nbytes`` (reads and writes

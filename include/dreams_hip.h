@@ -258,6 +258,15 @@ void drx_host_free(void* p);
  * copy bandwidth the walk kernels are compared with. Allocates and frees. */
 int drx_copy_calibrate(drx_ctx* ctx, uint64_t bytes, int reps, double* ms);
 
+/* Issue rate of one VALU operation over the whole device: every lane runs
+ * independent chains of op 0 (FP64 FMA), 1 (32-bit multiply-high), 2
+ * (32x32+64-bit multiply-add) or 3 (the engine's residue modmul: FP64
+ * two-product and quotient, counted as one operation). *ops_per_s gets the
+ * measured rate, *peak_per_s CUs x 64 lanes x the device clock. No
+ * counterpart in the reference: the measurement behind the FP64 residue
+ * arithmetic of the walk kernels. */
+int drx_valu_rate(drx_ctx* ctx, int op, double* ops_per_s, double* peak_per_s);
+
 /* Device time of the last drx_walk_batch, from HIP events on the library's
  * stream: [0] walk kernels, [1] combine, [2] wait for the FP64 shadow chain
  * (which runs on a second stream beside the walk), [3] finalize + CRT +
