@@ -63,3 +63,19 @@ def parse_vec(s) -> List[Fraction]:
     if isinstance(s, (int, float)):
         return [Fraction(s)]
     return [Fraction(t.strip()) for t in s.strip("[]").split(",")]
+
+
+def exact_walk_axes(spec: Dict, shift: int, trajectory: int):
+    """Shift and trajectory vectors of the multi-axis spec that reproduce the
+    one-variable walk k = shift + step * trajectory of the reference's exact
+    run (odd_zeta_exact.py:113-206 over the k-form matrix of
+    odd_zeta_cmf_generator.py:29-176): each axis of the spec stands for a
+    linear function a k + b of k (its direction a and its value b + a at
+    k = 1 in the spec's default shift), so its shift is a shift + b and its
+    trajectory a trajectory."""
+    sh, tr = [], []
+    for d, s0 in zip(spec["directions"], spec["default_shifts"]):
+        b = s0 - d
+        sh.append(d * shift + b)
+        tr.append(d * trajectory)
+    return sh, tr

@@ -113,3 +113,30 @@ def test_vec_walks_refuse_a_mixed_batch(gpu_ctx):
     from dreams_rns.engine import pcf_spec
     with pytest.raises(ValueError):
         BatchPlan([v, pcf_spec(pcf, 0, 5, 8)])
+
+
+def test_odd_zeta_exact_ratios_at_depth_5000(gpu_ctx):
+    # the 25 exact big-integer runs of the reference (zeta(5), shifts 1..10,
+    # trajectories 1..7, depth 5000, odd_zeta_exact.py:113-206): each is the
+    # multi-axis walk along k = shift + step * trajectory; est = p_float /
+    # q_float within 1e-12 of the exact ratio, and one walk's residues and
+    # CRT equal to the oracle's
+    fx = oz.load()
+    spec = fx["specs"]["zeta_5"]
+    prog = oz.compile_spec(spec)
+    v0 = oz.initial_state(spec["n"])
+    rows = fx["exact"]
+    axes = [oz.exact_walk_axes(spec, e["shift"], e["trajectory"]) for e in rows]
+    out = D.run_cmf_walk_vec_batch(prog, 5000, 16, [a[0] for a in axes], v0, spec["accumulator_idx"],
+                                   spec["constant_idx"], [a[1] for a in axes], crt=True)
+    print("odd zeta exact", len(rows), "walks at depth 5000", flush=True)
+    for e, d in zip(rows, out):
+        assert e["depth"] == 5000
+        est = d["p_float"] / d["q_float"]
+        assert abs(est - float(e["ratio"])) <= 1e-12 * float(e["ratio"]), (e, est)
+    ref = ow.run_cmf_walk_vec(prog, 5000, 16, axes[7][0], v0, spec["accumulator_idx"], spec["constant_idx"], axes[7][1])
+    assert out[7]["p_residues"].tolist() == ref["p_residues"].tolist()
+    assert out[7]["q_residues"].tolist() == ref["q_residues"].tolist()
+    pl = [int(p) for p in ref["primes"]]
+    xp, M = ow.crt_reconstruct(ref["p_residues"].tolist(), pl)
+    assert out[7]["p_crt"] == ow.centered(xp, M)

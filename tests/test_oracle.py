@@ -167,6 +167,43 @@ def test_delta_float_closed_form():
     assert d == -(1.0 + math.log(0.5) / (10.0 + math.log(2.0)))
 
 
+def test_delta_exact_special_cases_and_value():
+    # runner.py:103-125: q = 0 and |q| <= 1 give -inf, p/q == L gives +inf;
+    # a value against a second route (the error from a 300-digit pi, logs in
+    # FP64), so a dropped 1 + or a sign or a swapped log fails
+    import mpmath as mp
+
+    assert ow.compute_dreams_delta_exact(5, 0, 0.5) == float("-inf")
+    assert ow.compute_dreams_delta_exact(1, 2, 0.5) == float("inf")
+    assert ow.compute_dreams_delta_exact(3, -1, 0.5) == float("-inf")
+    mp.mp.dps = 300
+    pi300 = +mp.pi
+    err = float(abs(mp.mpf(22) / 7 - pi300))
+    want = -(1.0 + math.log(err) / math.log(7.0))
+    got = ow.compute_dreams_delta_exact(22, 7, mp.nstr(pi300, 250), dps=200)
+    assert abs(got - want) <= 1e-13 * abs(want) and abs(got - 2.42929) < 1e-5
+    got = ow.compute_dreams_delta_exact(-355, -113, mp.nstr(pi300, 250), dps=200)
+    err = float(abs(mp.mpf(355) / 113 - pi300))
+    assert abs(got - (-(1.0 + math.log(err) / math.log(113.0)))) <= 1e-13 * abs(got)
+
+
+def test_oracle_verify_pcf_equals_reference_record(golden):
+    # the reference's own verify_pcf output for the README PCF (depth 2000,
+    # K = 32) pins the oracle's verify_pcf: every field
+    import dreams_rns as D
+
+    rec = golden["verify_pcf"][0]
+    got = ow.verify_pcf(*rec["args"], compile_pcf=D.compile_pcf, pcf_initial_values=D.pcf_initial_values)
+    out = rec["out"]
+    for key in ("a", "b", "limit", "depth", "K", "p_bits"):
+        assert got[key] == out[key], key
+    for key in ("target", "est_float", "delta_exact"):
+        assert float(got[key]).hex() == out[key], key
+    for key in ("delta_float", "log_q"):
+        want = float.fromhex(out[key])
+        assert abs(got[key] - want) <= 1e-12 * abs(want), key
+
+
 def test_readme_known_answer_delta_float(golden):
     # README.md:19-32: a = 2, b = n^2, L = 2/(4 - pi), depth 2000, K 32
     ref = golden["verify_pcf"][0]["out"]
@@ -237,3 +274,22 @@ def test_vec_walk_oracle_is_the_transposed_matrix_walk():
     assert res[0, spec["constant_idx"]].tolist() == r["q_residues"].tolist()
     assert abs(Pf[0, spec["accumulator_idx"]] / Pf[0, spec["constant_idx"]] - r["p_float"] / r["q_float"]) <= \
         1e-13 * abs(r["p_float"] / r["q_float"])
+
+
+def test_vec_walk_oracle_matches_reference_exact_ratios_at_depth_5000():
+    # the reference's exact big-integer runs (odd_zeta_exact_results.jsonl:
+    # zeta(5), depth 5000, k = shift + step * trajectory) pin the oracle's
+    # FP64 state-vector walk on the multi-axis spec along the same line
+    import odd_zeta as oz
+
+    fx = oz.load()
+    spec = fx["specs"]["zeta_5"]
+    prog = oz.compile_spec(spec)
+    v0 = oz.initial_state(spec["n"])
+    picks = [e for e in fx["exact"] if (e["shift"], e["trajectory"]) in ((1, 1), (2, 3), (10, 7))]
+    assert len(picks) == 3
+    for e in picks:
+        sh, tr = oz.exact_walk_axes(spec, e["shift"], e["trajectory"])
+        r = ow.run_cmf_walk_vec(prog, e["depth"], 16, sh, v0, spec["accumulator_idx"], spec["constant_idx"], tr)
+        est = float(r["p_float"] / r["q_float"])
+        assert abs(est - float(e["ratio"])) <= 1e-13 * float(e["ratio"]), e
