@@ -20,47 +20,52 @@
 namespace drx {
 
 // ------------------------------------------------------- residue encoding
-// The residue code is one 32-bit word per instruction over two VGPR banks
-// per lane: R (16 temps) and C (C[15] = step, C[14 - i] = axis i,
-// C[0 .. 15 - dim) = the first constant slots). Operand x always comes from
-// R and operand y from R or C, and every result goes to R, so an instruction
-// costs two indexed register reads, one indexed write and one or two uniform
-// branches. A result that is a matrix entry is also stored to the lane's LDS
-// row of that entry.
-//   [3:0] op   [7:4] x   [8] y from C (else R)   [12:9] y   [16:13] dst
-//   [17] store to entry row [21:18]   [25:22] LIN signs (sx [23:22], sy [25:24])
-// LDM / STM / LDC / LDV carry an LDS row in [31:18] instead (and no entry
-// store). In a vector walk the "entries" are the new state vector.
-// LIN computes sy * y + sx * x with each sign coded 0 -> 0, 1 -> +1, 2 -> -1
-// (ADD, SUB, reversed SUB, NEG, MOV, constant loads and zero).
+// The residue code is one 32-bit word per instruction over one bank B of 32
+// doubles per lane in VGPRs: B[0, 16) are temps, B[16 + j] is slot j of the
+// inputs C (C[15] = step, C[14 - i] = axis i, C[0 .. 15 - dim) = the first
+// constant slots). Both operands come from anywhere in B and every result
+// goes to a temp, so an arithmetic instruction costs two indexed register
+// reads and one indexed write (one index window each) and one or two
+// uniform branches.
+//   [3:0] op   [8:4] x   [13:9] y   [18:14] dst   [31:19] aux
+// aux is an LDS row for LDM / STM / LDC / LDV / STE, or the LIN signs: sx in
+// [20:19] and sy in [22:21], each a 2-bit two's complement number in
+// {-1, 0, +1}; LIN computes sy * y + sx * x (SUB, NEG, zero). A matrix entry
+// (in a vector walk: an entry of the new state vector) leaves through STE,
+// which stores x to the lane's LDS row of that entry. Every instruction
+// writes dst; STM, STE and NOP, which compute nothing, carry dst = x and
+// write x back.
 enum ResOp : uint32_t {
-  RO_MUL = 0,   // x * y
-  RO_LIN = 1,   // sy * y + sx * x
-  RO_INV = 2,   // x^(p-2)
-  RO_RED = 3,   // x mod p (balanced)
-  RO_NZ1 = 4,   // x, or 1 when x = 0 mod p
-  RO_ZSEL = 5,  // x, or 0 when y = 0 mod p
-  RO_LDM = 6,   // the lane's spill row
-  RO_STM = 7,   // spill row <- x (destination = x's own register)
-  RO_LDC = 8,   // LDS constant row (constant slot 15 - dim + row)
-  RO_MULNZ = 9, // x * (y, or 1 when y = 0 mod p)
-  RO_LDV = 10,  // the lane's state row (vector walks)
+  RO_MUL = 0,    // x * y
+  RO_ADD = 1,    // x + y
+  RO_LIN = 2,    // sy * y + sx * x
+  RO_MULNZ = 3,  // x * (y, or 1 when y = 0 mod p)
+  RO_ZSEL = 4,   // x, or 0 when y = 0 mod p
+  RO_INV = 5,    // x^(p-2)
+  RO_RED = 6,    // x mod p (balanced)
+  RO_NZ1 = 7,    // x, or 1 when x = 0 mod p
+  RO_LDM = 8,    // the lane's spill row
+  RO_LDC = 9,    // LDS constant row (constant slot 15 - dim + row)
+  RO_LDV = 10,   // the lane's state row (vector walks)
+  RO_STM = 11,   // spill row <- x (dst = x)
+  RO_STE = 12,   // entry row <- x (dst = x)
+  RO_NOP = 15,   // nothing: B[x] <- B[x] (pads the code to whole groups)
 };
-__host__ __device__ constexpr uint32_t enc_res(uint32_t op, uint32_t x, uint32_t y_c, uint32_t y, uint32_t d,
-                                               uint32_t e_store, uint32_t e, uint32_t sgn) {
-  return op | (x << 4) | (y_c << 8) | (y << 9) | (d << 13) | (e_store << 17) | (e << 18) | (sgn << 22);
+constexpr int kResTemps = 16;    // temps B[0, 16)
+constexpr int kResCBase = 16;    // input slot j at B[16 + j]
+constexpr int kResStep = 15;     // input slot of the step; axis i at slot 14 - i
+__host__ __device__ constexpr uint32_t enc_res(uint32_t op, uint32_t x, uint32_t y, uint32_t d, uint32_t aux) {
+  return op | (x << 4) | (y << 9) | (d << 14) | (aux << 19);
 }
-__host__ __device__ constexpr uint32_t enc_row(uint32_t op, uint32_t x, uint32_t d, uint32_t row) {
-  return op | (x << 4) | (d << 13) | (row << 18);
+// LIN signs (each in {-1, 0, +1}) as the aux field.
+__host__ __device__ constexpr uint32_t lin_signs(int sx, int sy) {
+  return ((uint32_t)sx & 3u) | (((uint32_t)sy & 3u) << 2);
 }
-constexpr uint32_t kLinZero = 0, kLinPlus = 1, kLinMinus = 2;
-__host__ __device__ constexpr uint32_t lin_signs(uint32_t sx, uint32_t sy) { return sx | (sy << 2); }
-constexpr int kResStep = 15;     // C slot of the step; axis i at C[14 - i]
 // The residue code is executed in groups of kResGroup instructions read by
-// one scalar load; the packer pads it with no-ops (R[0] <- R[0]).
+// one scalar load; the packer pads it with no-ops.
 constexpr int kResGroup = 8;
-constexpr uint32_t kResNop = enc_res(RO_LIN, 0, 0, 0, 0, 0, 0, lin_signs(kLinPlus, kLinZero));
-constexpr uint32_t kMaxRow = (1u << 14) - 1;
+constexpr uint32_t kResNop = RO_NOP;
+constexpr uint32_t kMaxRow = (1u << 13) - 1;
 
 // -------------------------------------------------------- FP64 encoding
 // The FP64 code (the M_f pass) is two 32-bit words per instruction:

@@ -13,13 +13,29 @@ namespace drx {
 // ------------------------------------------------------------- interpreter
 // One instruction stream per workgroup (the whole block walks one program),
 // so fetch, decode and bank dispatch are wave-uniform scalar work. Banks are
-// 16-wide vectors of doubles indexed with uniform indices (VGPR index mode).
+// vectors of doubles indexed with uniform indices (VGPR index mode).
 typedef double v16d __attribute__((ext_vector_type(16)));
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef uint32_t v32u __attribute__((ext_vector_type(32)));
 
 // Entry bank: m*m doubles rounded up to a power of two.
 template <int M> struct EBank { typedef v16d type; };
 template <> struct EBank<2> { typedef v4d type; };
+
+// The residue code's register bank: 32 doubles held as their low and high
+// words in two 32-wide vectors, so one uniform index selects both halves in
+// a single index window (a 32-wide vector of doubles would be demoted to
+// scratch memory).
+struct RBank {
+  v32u lo, hi;
+  __device__ __forceinline__ double get(uint32_t i) const {
+    return __hiloint2double((int)hi[i], (int)lo[i]);
+  }
+  __device__ __forceinline__ void set(uint32_t i, double v) {
+    lo[i] = (uint32_t)__double2loint(v);
+    hi[i] = (uint32_t)__double2hiint(v);
+  }
+};
 
 struct Lane {
   double p, pinv;
@@ -28,60 +44,58 @@ struct Lane {
   uint32_t tid, T;   // for LDS temp slots
 };
 
-// Per-lane sign of a LIN operand: code 0 -> 0, 1 -> +1, 2 -> -1 (uniform).
-__device__ __forceinline__ double lin_sign(uint32_t code) {
-  return code == kLinZero ? 0.0 : (code == kLinPlus ? 1.0 : -1.0);
+// A 2-bit two's complement LIN sign at bit `lo` of the word, as a double.
+__device__ __forceinline__ double lin_sign(uint32_t w, int lo) {
+  return (double)((int32_t)(w << (30 - lo)) >> 30);
 }
 
-// One residue instruction (see the encoding in drx_device.h). Bank C holds
-// the step, the axes and the first constant slots; `lc` is the LDS table of
-// further constant slots ([rows][KP], by prime), `lm` the lanes' spill rows
-// and `le` their entry rows ([rows][T], by thread).
-__device__ __forceinline__ void exec_res(uint32_t w, v16d& R, const v16d& C, const double* __restrict__ lc,
-                                         uint32_t KP, double* __restrict__ lm, double* __restrict__ le,
+// One residue instruction (see the encoding in drx_device.h). Bank B holds
+// the temps and the inputs (step, axes, first constant slots); `lc` is the
+// LDS table of further constant slots ([rows][KP], by prime), `lm` the lanes'
+// spill rows, `le` their entry rows and `lv` their state rows ([rows][T], by
+// thread). MUL and ADD, most of any program, take the first two branches;
+// STM, STE and NOP carry dst = x, so every instruction ends in one write.
+__device__ __forceinline__ void exec_res(uint32_t w, RBank& B, const double* __restrict__ lc, uint32_t KP,
+                                         double* __restrict__ lm, double* __restrict__ le,
                                          const double* __restrict__ lv, const Lane& ln) {
   const uint32_t op = w & 15u;
-  const double x = R[(w >> 4) & 15u];
+  const double x = B.get((w >> 4) & 31u);
+  const double y = B.get((w >> 9) & 31u);
   double r;
-  if (op <= RO_LIN) {
-    const uint32_t yi = (w >> 9) & 15u;
-    const double y = (w & 0x100u) ? C[yi] : R[yi];
-    if (op == RO_MUL) r = mm(x, y, ln.p, ln.pinv);
-    else r = __builtin_fma(y, lin_sign((w >> 24) & 3u), x * lin_sign((w >> 22) & 3u));
+  if (op == RO_MUL) {
+    r = mm(x, y, ln.p, ln.pinv);
+  } else if (op == RO_ADD) {
+    r = x + y;
   } else {
-    const uint32_t row = w >> 18;
+    const uint32_t row = w >> 19;
     switch (op) {
+      case RO_LIN: r = __builtin_fma(y, lin_sign(w, 21), x * lin_sign(w, 19)); break;
+      case RO_MULNZ: r = mm(x, (red(y, ln.p, ln.pinv) == 0.0) ? 1.0 : y, ln.p, ln.pinv); break;
+      case RO_ZSEL: r = (red(y, ln.p, ln.pinv) == 0.0) ? 0.0 : x; break;
       case RO_INV: r = inv_fermat(x, ln.p, ln.pinv, ln.pe); break;
       case RO_RED: r = red(x, ln.p, ln.pinv); break;
       case RO_NZ1: r = (red(x, ln.p, ln.pinv) == 0.0) ? 1.0 : x; break;
-      case RO_ZSEL: {
-        const uint32_t yi = (w >> 9) & 15u;
-        const double y = (w & 0x100u) ? C[yi] : R[yi];
-        r = (red(y, ln.p, ln.pinv) == 0.0) ? 0.0 : x;
-        break;
-      }
-      case RO_MULNZ: {
-        const uint32_t yi = (w >> 9) & 15u;
-        const double y = (w & 0x100u) ? C[yi] : R[yi];
-        r = mm(x, (red(y, ln.p, ln.pinv) == 0.0) ? 1.0 : y, ln.p, ln.pinv);
-        break;
-      }
       case RO_LDM: r = lm[row * ln.T + ln.tid]; break;
+      case RO_LDC: r = lc[row * KP + ln.k]; break;
+      case RO_LDV: r = lv[row * ln.T + ln.tid]; break;
       case RO_STM:
         lm[row * ln.T + ln.tid] = x;
         r = x;
         break;
-      case RO_LDV: r = lv[row * ln.T + ln.tid]; break;
-      default: r = lc[row * KP + ln.k]; break;  // RO_LDC
+      case RO_STE:
+        le[row * ln.T + ln.tid] = x;
+        r = x;
+        break;
+      default: r = x; break;  // RO_NOP
     }
   }
-  R[(w >> 13) & 15u] = r;
-  if (w & 0x20000u) le[((w >> 18) & 15u) * ln.T + ln.tid] = r;
+  // one write per instruction (the ops that write nothing carry dst = x)
+  B.set((w >> 14) & 31u, r);
 }
 
 // The packed residue program over one prime per lane, kResGroup instructions
 // per scalar load (n is a multiple of kResGroup).
-__device__ __forceinline__ void run_program(const uint32_t* __restrict__ code, uint32_t n, v16d& R, const v16d& C,
+__device__ __forceinline__ void run_program(const uint32_t* __restrict__ code, uint32_t n, RBank& B,
                                             const double* __restrict__ lc, uint32_t KP, double* __restrict__ lm,
                                             double* __restrict__ le, const double* __restrict__ lv,
                                             const Lane& ln) {
@@ -91,7 +105,7 @@ __device__ __forceinline__ void run_program(const uint32_t* __restrict__ code, u
 #pragma unroll
     for (int j = 0; j < kResGroup; ++j) w[j] = code[pc + j];
 #pragma unroll
-    for (int j = 0; j < kResGroup; ++j) exec_res(w[j], R, C, lc, KP, lm, le, lv, ln);
+    for (int j = 0; j < kResGroup; ++j) exec_res(w[j], B, lc, KP, lm, le, lv, ln);
   }
 }
 
@@ -239,28 +253,72 @@ __device__ __forceinline__ WalkSetup walk_setup(const KArgs& a, double* lds, boo
   return S;
 }
 
-// Bank C: the first constant slots, the axis values at step s0 (C[14 - i])
-// and the step s0 (C[15]); with li, the axes' per-step increments into the
-// LDS table li[dim][li_ld] at column li_col (the caller synchronises before
-// reading it).
+// The bank's input slots (slot j at B[kResCBase + j]): the first constant
+// slots, the axis values at step s0 (slot 14 - i) and the step s0 (slot 15);
+// with li, the axes' per-step increments into the LDS table li[dim][li_ld] at
+// column li_col (the caller synchronises before reading it). Loads go in
+// groups of 4, all in flight together (indices past the end re-read the
+// last valid one). Two ways to fill the bank, which the register allocator
+// takes differently: GATHER collects the slots in a 16-wide vector and moves
+// them into the bank in one unconditional pass (the walk kernel: 150 VGPRs
+// for 3x3 against 182 for the other way); otherwise the group loops are not
+// unrolled and write the bank through indexed writes in place (the seed
+// kernel: 139 against 228).
+template <bool GATHER>
 __device__ __forceinline__ void load_rns_inputs(const KArgs& a, const DevWalk& w, const DevProgram& pg,
-                                                uint32_t K, uint32_t kk, const Lane& ln, int64_t s0, v16d& C,
+                                                uint32_t K, uint32_t kk, const Lane& ln, int64_t s0, RBank& B,
                                                 double* __restrict__ li, uint32_t li_ld, uint32_t li_col) {
   const double p = ln.p, pinv = ln.pinv;
-  const uint32_t nc = (uint32_t)kResStep - pg.dim;
-#pragma unroll
-  for (int i = 0; i < kResStep; ++i)
-    if ((uint32_t)i < nc && (uint32_t)i < pg.nconst) C[i] = a.cres[pg.cres_off + (uint64_t)i * pg.kcap + kk];
-  // axes in groups of 4: the 12 loads of a group are issued together (axes
-  // past dim re-read the last axis, so every address is valid), then used
   constexpr int G = 4;
+  if (GATHER) {
+    const uint32_t nc = (uint32_t)kResStep - pg.dim;
+    v16d C = 0.0;
 #pragma unroll
-  for (int g0 = 0; g0 < kMaxDim; g0 += G) {
-    if ((uint32_t)g0 >= pg.dim) break;  // uniform
+    for (int i = 0; i < kResStep; ++i)
+      if ((uint32_t)i < nc && (uint32_t)i < pg.nconst) C[i] = a.cres[pg.cres_off + (uint64_t)i * pg.kcap + kk];
+#pragma unroll
+    for (int g0 = 0; g0 < kMaxDim; g0 += G) {
+      if ((uint32_t)g0 >= pg.dim) break;  // uniform
+      double c[G][3];
+#pragma unroll
+      for (int i = 0; i < G; ++i) {
+        const uint32_t ax = min((uint32_t)(g0 + i), pg.dim - 1u);
+        const double* q = a.axr + w.axr_off + (uint64_t)ax * 3 * K;
+        c[i][0] = q[kk];
+        c[i][1] = q[K + kk];
+        c[i][2] = q[2 * K + kk];
+      }
+#pragma unroll
+      for (int i = 0; i < G; ++i) {
+        const int ax = g0 + i;
+        if (ax < kMaxDim && (uint32_t)ax < pg.dim) {
+          const double x0 = red(c[i][0] + mm((double)s0, c[i][1], p, pinv), p, pinv);
+          C[kResStep - 1 - ax] = mm(x0, c[i][2], p, pinv);
+          if (li) li[ax * li_ld + li_col] = mm(c[i][1], c[i][2], p, pinv);
+        }
+      }
+    }
+    C[kResStep] = (double)s0;
+#pragma unroll
+    for (int j = 0; j <= kResStep; ++j) B.set(kResCBase + j, C[j]);
+    return;
+  }
+  const uint32_t nc = min((uint32_t)kResStep - pg.dim, pg.nconst);
+#pragma unroll 1
+  for (uint32_t i0 = 0; i0 < nc; i0 += G) {
+    double c[G];
+#pragma unroll
+    for (int i = 0; i < G; ++i) c[i] = a.cres[pg.cres_off + (uint64_t)min(i0 + i, nc - 1u) * pg.kcap + kk];
+#pragma unroll
+    for (int i = 0; i < G; ++i)
+      if (i0 + i < nc) B.set(kResCBase + i0 + i, c[i]);
+  }
+#pragma unroll 1
+  for (uint32_t g0 = 0; g0 < pg.dim; g0 += G) {
     double c[G][3];
 #pragma unroll
     for (int i = 0; i < G; ++i) {
-      const uint32_t ax = min((uint32_t)(g0 + i), pg.dim - 1u);
+      const uint32_t ax = min(g0 + i, pg.dim - 1u);
       const double* q = a.axr + w.axr_off + (uint64_t)ax * 3 * K;
       c[i][0] = q[kk];
       c[i][1] = q[K + kk];
@@ -268,26 +326,26 @@ __device__ __forceinline__ void load_rns_inputs(const KArgs& a, const DevWalk& w
     }
 #pragma unroll
     for (int i = 0; i < G; ++i) {
-      const int ax = g0 + i;
-      if (ax < kMaxDim && (uint32_t)ax < pg.dim) {
+      const uint32_t ax = g0 + i;
+      if (ax < pg.dim) {
         const double x0 = red(c[i][0] + mm((double)s0, c[i][1], p, pinv), p, pinv);
-        C[kResStep - 1 - ax] = mm(x0, c[i][2], p, pinv);
+        B.set(kResCBase + kResStep - 1 - ax, mm(x0, c[i][2], p, pinv));
         if (li) li[ax * li_ld + li_col] = mm(c[i][1], c[i][2], p, pinv);
       }
     }
   }
-  C[kResStep] = (double)s0;
+  B.set(kResCBase + kResStep, (double)s0);
 }
 
 // Axes and step to the next step (a uniform loop over the program's axes).
-__device__ __forceinline__ void advance_axes(v16d& C, const double* __restrict__ li, uint32_t li_ld, uint32_t li_col,
+__device__ __forceinline__ void advance_axes(RBank& B, const double* __restrict__ li, uint32_t li_ld, uint32_t li_col,
                                              uint32_t dim, const Lane& ln) {
 #pragma unroll 1
   for (uint32_t ax = 0; ax < dim; ++ax) {
-    const uint32_t i = (uint32_t)kResStep - 1u - ax;
-    C[i] = red(C[i] + li[ax * li_ld + li_col], ln.p, ln.pinv);
+    const uint32_t i = (uint32_t)(kResCBase + kResStep - 1) - ax;
+    B.set(i, red(B.get(i) + li[ax * li_ld + li_col], ln.p, ln.pinv));
   }
-  C[kResStep] += 1.0;
+  B.set(kResCBase + kResStep, B.get(kResCBase + kResStep) + 1.0);
 }
 
 // P <- P * M with M's entries in the lane's LDS entry rows (each read once),
@@ -377,16 +435,18 @@ template <int M>
 __device__ __forceinline__ void walk_chunk_task(const KArgs& a, double* lds, int task) {
   const WalkSetup S = walk_setup(a, lds, true, task);
   const double p = S.ln.p, pinv = S.ln.pinv;
-  v16d R = 0.0, C = 0.0;
-  load_rns_inputs(a, S.w, S.pg, S.Kw, S.kg, S.ln, S.sb, C, S.c == 0 ? S.li : nullptr, S.KP, S.k);
+  RBank B;
+  B.lo = 0u;
+  B.hi = 0u;
+  load_rns_inputs<true>(a, S.w, S.pg, S.Kw, S.kg, S.ln, S.sb, B, S.c == 0 ? S.li : nullptr, S.KP, S.k);
   __syncthreads();
   double P[M * M];
 #pragma unroll
   for (int e = 0; e < M * M; ++e) P[e] = (e % (M + 1) == 0) ? 1.0 : 0.0;
   for (int64_t s = S.sb; s < S.se; ++s) {
-    run_program(S.code, S.pg.ncode, R, C, S.lc, S.KP, S.lm, S.le, nullptr, S.ln);
+    run_program(S.code, S.pg.ncode, B, S.lc, S.KP, S.lm, S.le, nullptr, S.ln);
     step_matmul_rows<M>(P, S.le, S.pg.kinds, S.pg.colred, S.ln);
-    advance_axes(C, S.li, S.KP, S.k, S.pg.dim, S.ln);
+    advance_axes(B, S.li, S.KP, S.k, S.pg.dim, S.ln);
   }
   tree_combine<M>(a, S, P);
 }
@@ -435,9 +495,11 @@ __global__ void __launch_bounds__(512) k_fd_seed(const KArgs a, uint32_t kp) {
   const uint32_t* code = a.code + pg.code_off;
   double* tab = a.fdt + w.fdt_off + k0;  // column k of the slice is prime k0 + k
   for (uint32_t j = g; j <= (uint32_t)D; j += J) {
-    v16d R = 0.0, C = 0.0;
-    load_rns_inputs(a, w, pg, Kw, kg, ln, (int64_t)j, C, nullptr, KP, k);
-    run_program(code, pg.ncode, R, C, L.lc, KP, L.lm, L.le, nullptr, ln);
+    RBank B;
+    B.lo = 0u;
+    B.hi = 0u;
+    load_rns_inputs<false>(a, w, pg, Kw, kg, ln, (int64_t)j, B, nullptr, KP, k);
+    run_program(code, pg.ncode, B, L.lc, KP, L.lm, L.le, nullptr, ln);
     double E[M * M];
 #pragma unroll
     for (int e = 0; e < M * M; ++e)
@@ -1195,8 +1257,10 @@ __global__ void __launch_bounds__(256) k_walk_vec(const KArgs a) {
   double* sa = L.lm + (uint64_t)pg.nslots * T;
   double* sb = sa + (uint64_t)m * T;
   double* li = sb + (uint64_t)m * T;
-  v16d R = 0.0, C = 0.0;
-  load_rns_inputs(a, w, pg, K, kk, ln, 0, C, li, T, tid);
+  RBank B;
+  B.lo = 0u;
+  B.hi = 0u;
+  load_rns_inputs<false>(a, w, pg, K, kk, ln, 0, B, li, T, tid);
 #pragma unroll 1
   for (uint32_t j = 0; j < m; ++j) sa[j * T + tid] = a.res_in[w.res_off + (uint64_t)j * K + kk];
   __syncthreads();  // constant rows
@@ -1205,11 +1269,11 @@ __global__ void __launch_bounds__(256) k_walk_vec(const KArgs a) {
   double* nxt = sb;
 #pragma unroll 1
   for (int64_t s = 0; s < vg.depth; ++s) {
-    run_program(code, pg.ncode, R, C, L.lc, KP, L.lm, nxt, cur, ln);
+    run_program(code, pg.ncode, B, L.lc, KP, L.lm, nxt, cur, ln);
     double* t = cur;
     cur = nxt;
     nxt = t;
-    advance_axes(C, li, T, tid, pg.dim, ln);
+    advance_axes(B, li, T, tid, pg.dim, ln);
   }
   if (active) {
     const double p = ln.p, pinv = ln.pinv;

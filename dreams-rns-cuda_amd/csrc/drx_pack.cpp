@@ -441,20 +441,21 @@ std::string encode(const CodeList& cl, const std::vector<Val>& vals, const std::
   return std::string();
 }
 
-// Residue code: lowering to the R / C instruction set with register
-// allocation over R (16 temps). Operand x must sit in R, operand y in R or C;
-// axes, the step and the first 15 - dim constant slots live in C, further
-// constants are loaded from LDS (LDC), zero is made by LIN. A value brought
-// into R stays there while it has uses (an input or a constant is simply
-// dropped when its register is needed). Every result goes to R; a matrix
-// entry is also stored to its LDS row. When all 16 temps are taken, the value
-// used furthest in the future leaves R; a computed one is spilled to an LDS
-// row (STM) first and reloaded (LDM) before its next use.
+// Residue code: lowering to the bank instruction set with register
+// allocation over the 16 temps of bank B. Axes, the step and the first
+// 15 - dim constant slots sit in B's input slots and are read in place;
+// further constants are loaded from LDS (LDC), a vector walk's state entries
+// too (LDV), and zero is made by LIN. A value brought into a temp stays there
+// while it has uses (an LDS input is simply dropped when its temp is
+// needed). Every result goes to a temp; a matrix entry (or new state entry)
+// then leaves through STE. When all temps are taken, the value used furthest
+// in the future leaves; a computed one is spilled to an LDS row (STM) first
+// and reloaded (LDM) before its next use.
 std::string encode_res(const CodeList& cl, const std::vector<Val>& vals, const std::vector<int>& cslot, int dim,
                        std::vector<uint32_t>& words, int& nrows) {
   const std::vector<Ins>& code = cl.code;
   const int nc = (int)code.size(), nvals = (int)vals.size();
-  const int n_cbank = kResStep - dim;  // constant slots held in C
+  const int n_cbank = kResStep - dim;  // constant slots held in the bank
   std::vector<std::vector<int>> uses(nvals);
   for (int i = 0; i < nc; ++i)
     for (int u : {code[i].a, code[i].b})
@@ -462,19 +463,20 @@ std::string encode_res(const CodeList& cl, const std::vector<Val>& vals, const s
   std::vector<size_t> next(nvals, 0);
   auto next_use = [&](int v) { return next[v] < uses[v].size() ? uses[v][next[v]] : INT32_MAX; };
   constexpr int kTemp = -2;
-  int held[kBank];
+  int held[kResTemps];
   for (int& h : held) h = -1;
-  bool pinned[kBank];
+  bool pinned[kResTemps];
   for (bool& p : pinned) p = false;
   std::vector<int> reg(nvals, -1), row(nvals, -1);
   int n_rows = 0;
   std::vector<int> free_rows;  // rows of spilled values that are dead
   words.clear();
+  const uint32_t any = kResCBase + kResStep;  // an operand that is always finite (the step)
   auto free_reg = [&]() -> int {
-    for (int r = 0; r < kBank; ++r)
+    for (int r = 0; r < kResTemps; ++r)
       if (held[r] == -1) return r;
     int best = -1, far = -1;
-    for (int r = 0; r < kBank; ++r) {
+    for (int r = 0; r < kResTemps; ++r) {
       if (pinned[r] || held[r] < 0) continue;
       const int nu = next_use(held[r]);
       if (nu > far) {
@@ -491,114 +493,95 @@ std::string encode_res(const CodeList& cl, const std::vector<Val>& vals, const s
       } else {
         row[v] = n_rows++;
       }
-      words.push_back(enc_row(RO_STM, (uint32_t)best, (uint32_t)best, (uint32_t)row[v]));
+      words.push_back(enc_res(RO_STM, (uint32_t)best, any, (uint32_t)best, (uint32_t)row[v]));
     }
     reg[v] = -1;
     held[best] = -1;
     return best;
   };
-  // operand in bank C: its index, else -1
-  auto c_index = [&](int u) -> int {
+  // input held in the bank's input slots: its bank index, else -1
+  auto in_bank = [&](int u) -> int {
     const Val& x = vals[u];
-    if (x.kind == V_X) return kResStep - 1 - x.idx;
-    if (x.kind == V_N) return kResStep;
-    if (x.kind == V_C && cslot[x.idx] < n_cbank) return cslot[x.idx];
+    if (x.kind == V_X) return kResCBase + kResStep - 1 - x.idx;
+    if (x.kind == V_N) return kResCBase + kResStep;
+    if (x.kind == V_C && cslot[x.idx] < n_cbank) return kResCBase + cslot[x.idx];
     return -1;
   };
-  // value u into a register of R (reload, input copy, constant load or zero), pinned
-  auto to_reg = [&](int u) -> int {
-    const Val& x = vals[u];
+  // operand u: the bank index holding it, after bringing it into a temp
+  // (reload, LDS input or zero) when it is not in the bank; the temp is pinned
+  auto operand = [&](int u) -> int {
     if (reg[u] >= 0) {
       pinned[reg[u]] = true;
       return reg[u];
     }
+    const int b = in_bank(u);
+    if (b >= 0) return b;
+    const Val& x = vals[u];
     if (x.kind == V_OP && row[u] < 0) return -1;  // not computed yet: the order is not topological
     const int r = free_reg();
     if (r < 0) return -1;
-    if (x.kind == V_OP) {
-      words.push_back(enc_row(RO_LDM, (uint32_t)r, (uint32_t)r, (uint32_t)row[u]));
-    } else if (x.kind == V_S) {
-      words.push_back(enc_row(RO_LDV, (uint32_t)r, (uint32_t)r, (uint32_t)x.idx));
-    } else {
-      const int ci = c_index(u);
-      if (ci >= 0) {
-        words.push_back(enc_res(RO_LIN, (uint32_t)r, 1, (uint32_t)ci, (uint32_t)r, 0, 0, lin_signs(kLinZero, kLinPlus)));
-      } else if (x.kind == V_C) {
-        words.push_back(enc_row(RO_LDC, (uint32_t)r, (uint32_t)r, (uint32_t)(cslot[x.idx] - n_cbank)));
-      } else {
-        words.push_back(enc_res(RO_LIN, (uint32_t)r, 0, (uint32_t)r, (uint32_t)r, 0, 0, lin_signs(kLinZero, kLinZero)));
-      }
-    }
+    if (x.kind == V_OP) words.push_back(enc_res(RO_LDM, any, any, (uint32_t)r, (uint32_t)row[u]));
+    else if (x.kind == V_S) words.push_back(enc_res(RO_LDV, any, any, (uint32_t)r, (uint32_t)x.idx));
+    else if (x.kind == V_C) words.push_back(enc_res(RO_LDC, any, any, (uint32_t)r, (uint32_t)(cslot[x.idx] - n_cbank)));
+    else words.push_back(enc_res(RO_LIN, any, any, (uint32_t)r, lin_signs(0, 0)));  // zero
     reg[u] = r;
     held[r] = u;
     pinned[r] = true;
     return r;
   };
-  // operand y: a register of R already holding it, a slot of C (yc = 1), or a
-  // register it is brought into
-  auto to_y = [&](int u, int& yc) -> int {
-    yc = 0;
-    if (reg[u] >= 0) {
-      pinned[reg[u]] = true;
-      return reg[u];
-    }
-    const int ci = c_index(u);
-    if (ci >= 0) {
-      yc = 1;
-      return ci;
-    }
-    return to_reg(u);
-  };
   for (int i = 0; i < nc; ++i) {
     const Ins& in = code[i];
-    uint32_t op = RO_MUL, sgn = 0;
-    int x = 0, y = 0, yc = 0;
+    uint32_t op = RO_MUL, aux = 0;
+    int x = 0, y = 0;
     const uint32_t kind = in.op;
-    if (kind == D_MUL || kind == D_ADD || kind == D_SUB) {
-      int a = in.a, b = in.b;
-      bool swapped = false;
-      // x should be the operand already in R (or a computed value)
-      const bool a_r = reg[a] >= 0 || vals[a].kind == V_OP, b_r = reg[b] >= 0 || vals[b].kind == V_OP;
-      if (!a_r && b_r) {
-        std::swap(a, b);
-        swapped = true;
-      }
-      x = to_reg(a);
+    int e = -1;  // entry row the result leaves through
+    if (in.entry >= 0) e = in.entry;
+    else if (in.def >= 0 && cl.entry_direct[in.def] >= 0) e = cl.entry_direct[in.def];
+    if (kind == D_MOV) {
+      // an output that is an input or a value also used elsewhere: stored from where it is
+      x = operand(in.a);
       if (x < 0) return "internal: no free register";
-      y = to_y(b, yc);
+      if (e < 0) return "internal: a move that is no output";
+      op = RO_STE;
+      aux = (uint32_t)e;
+      y = (int)any;
+    } else if (kind == D_MUL || kind == D_ADD || kind == D_SUB) {
+      x = operand(in.a);
+      if (x < 0) return "internal: no free register";
+      y = operand(in.b);
       if (y < 0) return "internal: no free register";
-      if (kind != D_MUL) {
+      if (kind == D_ADD) op = RO_ADD;
+      else if (kind == D_SUB) {
         op = RO_LIN;
-        const uint32_t sx = (kind == D_SUB && swapped) ? kLinMinus : kLinPlus;
-        const uint32_t sy = (kind == D_SUB && !swapped) ? kLinMinus : kLinPlus;
-        sgn = lin_signs(sx, sy);
+        aux = lin_signs(1, -1);
       }
     } else if (kind == D_SQR || kind == D_CUB) {
-      x = to_reg(in.a);
+      x = operand(in.a);
       if (x < 0) return "internal: no free register";
       y = x;
       if (kind == D_CUB) {
         const int t = free_reg();
         if (t < 0) return "internal: no free register";
-        words.push_back(enc_res(RO_MUL, (uint32_t)x, 0, (uint32_t)x, (uint32_t)t, 0, 0, 0));  // t = x^2
+        words.push_back(enc_res(RO_MUL, (uint32_t)x, (uint32_t)x, (uint32_t)t, 0));  // t = x^2
         held[t] = kTemp;
         pinned[t] = true;
-        y = x;
         x = t;  // then x^2 * x
       }
-    } else if (kind == D_NEG || kind == D_MOV) {
-      y = to_y(in.a, yc);
-      if (y < 0) return "internal: no free register";
-      op = RO_LIN;
-      sgn = lin_signs(kLinZero, kind == D_NEG ? kLinMinus : kLinPlus);
-    } else if (kind == D_INV || kind == D_RED || kind == D_NZ1) {
-      x = to_reg(in.a);
+    } else if (kind == D_NEG) {
+      x = operand(in.a);
       if (x < 0) return "internal: no free register";
+      y = x;
+      op = RO_LIN;
+      aux = lin_signs(-1, 0);
+    } else if (kind == D_INV || kind == D_RED || kind == D_NZ1) {
+      x = operand(in.a);
+      if (x < 0) return "internal: no free register";
+      y = (int)any;
       op = kind == D_INV ? RO_INV : (kind == D_RED ? RO_RED : RO_NZ1);
     } else if (kind == D_ZSEL || kind == D_MULNZ) {
-      x = to_reg(in.a);
+      x = operand(in.a);
       if (x < 0) return "internal: no free register";
-      y = to_y(in.b, yc);
+      y = operand(in.b);
       if (y < 0) return "internal: no free register";
       op = kind == D_ZSEL ? RO_ZSEL : RO_MULNZ;
     } else {
@@ -618,21 +601,22 @@ std::string encode_res(const CodeList& cl, const std::vector<Val>& vals, const s
         row[u] = -1;
       }
     }
-    for (int r = 0; r < kBank; ++r)
+    for (int r = 0; r < kResTemps; ++r)
       if (held[r] == kTemp) held[r] = -1;
     for (bool& p : pinned) p = false;
-    // destination register; an entry is also stored to its LDS row
-    int e = -1;
-    if (in.entry >= 0) e = in.entry;
-    else if (in.def >= 0 && cl.entry_direct[in.def] >= 0) e = cl.entry_direct[in.def];
+    if (op == RO_STE) {
+      words.push_back(enc_res(RO_STE, (uint32_t)x, (uint32_t)y, (uint32_t)x, aux));
+      continue;
+    }
+    // destination temp; an entry also leaves through STE
     const int d = free_reg();
     if (d < 0) return "internal: no free register";
-    if (in.entry < 0 && in.def >= 0 && !uses[in.def].empty()) {
+    if (in.def >= 0 && !uses[in.def].empty()) {
       held[d] = in.def;
       reg[in.def] = d;
     }
-    words.push_back(enc_res(op, (uint32_t)x, (uint32_t)yc, (uint32_t)y, (uint32_t)d, e >= 0 ? 1u : 0u,
-                            e >= 0 ? (uint32_t)e : 0u, sgn));
+    words.push_back(enc_res(op, (uint32_t)x, (uint32_t)y, (uint32_t)d, aux));
+    if (e >= 0) words.push_back(enc_res(RO_STE, (uint32_t)d, any, (uint32_t)d, (uint32_t)e));
   }
   if (n_rows > (int)kMaxRow) return "too many spilled values";
   nrows = n_rows;

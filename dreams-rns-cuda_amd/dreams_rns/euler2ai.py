@@ -30,37 +30,45 @@ FIELDNAMES = ["task_idx", "pcf_idx", "source_idx", "a", "b", "limit", "trajector
               "p_bits", "depth", "K"]
 
 
-def estimate_K(a_str: str, b_str: str, depth: int, safety: float = 1.5) -> int:
-    """Primes for an exact CRT of a depth-``depth`` PCF walk
-    (euler2ai_full_verify.py:40-92): bits = max_deg * log2(N!) (Stirling) +
-    N * log2(max_coeff + 1), times ``safety``, plus 100; K = max(bits // 31
-    + 1, 64). A polynomial that does not parse gives the fallback
-    max(4 * log2(N! ) * safety / 31 + 1, 1000)."""
+def _poly_size(exprs, n):
+    """(degree, coefficient bound) of the PCF's polynomials in ``n``: the
+    largest degree (at least 1) and the largest |integer coefficient|, with
+    100 standing in for an expression whose coefficients cannot be read as
+    integers (the reference's rule). A degree that cannot be taken raises."""
     import sympy as sp
 
+    deg = max([1] + [int(sp.degree(e, n)) for e in exprs if e.has(n)])
+    bound = 1
+    for e in exprs:
+        try:
+            for c in sp.Poly(e, n).all_coeffs():
+                bound = max(bound, abs(int(c)))
+        except Exception:
+            bound = max(bound, 100)
+    return deg, bound
+
+
+def estimate_K(a_str: str, b_str: str, depth: int, safety: float = 1.5) -> int:
+    """Primes for an exact CRT of a depth-``depth`` PCF walk: the same number
+    euler2ai_full_verify.py:40-92 computes.
+
+    |q_N| grows like prod_n |M(n)|, i.e. about N!^d * c^N for polynomials of
+    degree d with coefficients up to c, so the walk needs about
+    d * log2(N!) + N * log2(c + 1) bits (log2(N!) by Stirling's series to the
+    1/2 log(2 pi N) term; a coefficient bound of 1 adds nothing). That times
+    ``safety``, plus 100 bits, in 31-bit primes, at least 64. When the
+    polynomials cannot be read, a degree-4 bound of at least 1000 primes."""
+    import sympy as sp
+
+    N = depth
     try:
-        a_expr, b_expr = sp.sympify(a_str), sp.sympify(b_str)
-        n = sp.Symbol("n")
-        deg_a = sp.degree(a_expr, n) if a_expr.has(n) else 0
-        deg_b = sp.degree(b_expr, n) if b_expr.has(n) else 0
-        max_deg = max(int(deg_a), int(deg_b), 1)
-        max_coeff = 1
-        for expr in (a_expr, b_expr):
-            try:
-                for c in sp.Poly(expr, n).all_coeffs():
-                    max_coeff = max(max_coeff, abs(int(c)))
-            except Exception:
-                max_coeff = max(max_coeff, 100)
-        if depth > 1:
-            log2_fact = depth * math.log2(depth / math.e) + 0.5 * math.log2(2 * math.pi * depth)
-        else:
-            log2_fact = 1
-        coeff_bits = math.log2(max_coeff + 1) if max_coeff > 1 else 0
-        bits = int((max_deg * log2_fact + depth * coeff_bits) * safety) + 100
+        d, c = _poly_size((sp.sympify(a_str), sp.sympify(b_str)), sp.Symbol("n"))
+        log2_fact = N * math.log2(N / math.e) + 0.5 * math.log2(2 * math.pi * N) if N > 1 else 1
+        coeff_bits = math.log2(c + 1) if c > 1 else 0
+        bits = int((d * log2_fact + N * coeff_bits) * safety) + 100
         return max(bits // 31 + 1, 64)
     except Exception:
-        log2_fact = depth * math.log2(max(depth, 2) / math.e)
-        return max(int(4 * log2_fact * safety / 31) + 1, 1000)
+        return max(int(4 * N * math.log2(max(N, 2) / math.e) * safety / 31) + 1, 1000)
 
 
 def load_cmf_pcfs(path: str) -> List[Dict[str, Any]]:

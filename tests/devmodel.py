@@ -41,46 +41,50 @@ def _bal(v: int, p: int) -> int:
     return (v + p // 2) % p - p // 2
 
 
-RO_MUL, RO_LIN, RO_INV, RO_RED, RO_NZ1, RO_ZSEL, RO_LDM, RO_STM, RO_LDC, RO_MULNZ, RO_LDV = range(11)
+(RO_MUL, RO_ADD, RO_LIN, RO_MULNZ, RO_ZSEL, RO_INV, RO_RED, RO_NZ1, RO_LDM, RO_LDC, RO_LDV, RO_STM, RO_STE) = range(13)
+RO_NOP = 15
 RES_STEP = 15
+RES_TEMPS, RES_CBASE = 16, 16
 
 
 def decode_res(words):
-    """Residue instructions: (op, x, y_from_C, y, dst, entry_store, entry,
-    LIN signs, LDS row)."""
-    out = []
-    for w in words:
-        w = int(w)
-        out.append((w & 15, (w >> 4) & 15, (w >> 8) & 1, (w >> 9) & 15, (w >> 13) & 15, (w >> 17) & 1,
-                    (w >> 18) & 15, (w >> 22) & 15, w >> 18))
-    return out
+    """Residue instructions: (op, x, y, dst, aux) with x, y bank indices
+    (temps 0..15, input slot j at 16 + j), dst a bank index and aux the LDS
+    row or the LIN signs."""
+    return [(w & 15, (w >> 4) & 31, (w >> 9) & 31, (w >> 14) & 31, w >> 19) for w in (int(v) for v in words)]
 
 
-def _sign(code):
-    return {0: 0, 1: 1, 2: -1}[code]
+def _sext2(v):
+    v &= 3
+    return v - 4 if v & 2 else v
 
 
 def run_res(code, C, lc, p, nrows, state=None):
     """One step of the residue interpreter with lazy residues as on the
-    device: LIN results stay unreduced, products and reductions are
+    device: ADD / LIN results stay unreduced, products and reductions are
     balanced; a multiply (or inversion) operand beyond 2^40 would break the
     device's exact FP64 product, so the packer must have inserted RED before
-    that. C is the bank (step, axes, first constant slots), lc the further
-    constant slots. Returns the entry bank E (16 slots)."""
-    R = [0] * 16
+    that. C holds the input slots (step, axes, first constant slots), lc the
+    further constant slots. Every instruction writes its dst; STM, STE and
+    NOP must carry dst = x and never touch an input slot. Returns the entry
+    bank E (16 slots)."""
+    B = [0] * RES_TEMPS + [int(c) for c in C]
     E = [None] * 16
     M = [0] * max(1, nrows)
-    for op, x, yc, y, d, es, e, sgn, aux in code:
-        if op in (RO_LDM, RO_STM, RO_LDC, RO_LDV):
-            assert es == 0, "entry store on a row instruction"
-        xv = R[x]
-        if op in (RO_MUL, RO_LIN, RO_ZSEL, RO_MULNZ):
-            yv = C[y] if yc else R[y]
+    for op, x, y, d, aux in code:
+        xv, yv = B[x], B[y]
         if op == RO_MUL:
             assert abs(xv) <= LAZY_LIMIT and abs(yv) <= LAZY_LIMIT, "operand beyond 2^40"
             r = _bal(xv * yv, p)
+        elif op == RO_ADD:
+            r = xv + yv
         elif op == RO_LIN:
-            r = _sign((sgn >> 2) & 3) * yv + _sign(sgn & 3) * xv
+            r = _sext2(aux >> 2) * yv + _sext2(aux) * xv
+        elif op == RO_MULNZ:
+            assert abs(xv) <= LAZY_LIMIT and abs(yv) <= LAZY_LIMIT, "operand beyond 2^40"
+            r = _bal(xv * (yv if yv % p else 1), p)
+        elif op == RO_ZSEL:
+            r = 0 if yv % p == 0 else xv
         elif op == RO_INV:
             assert abs(xv) <= LAZY_LIMIT, "operand beyond 2^40"
             r = _bal(pow(xv % p, p - 2, p), p)
@@ -88,25 +92,24 @@ def run_res(code, C, lc, p, nrows, state=None):
             r = _bal(xv, p)
         elif op == RO_NZ1:
             r = xv if xv % p else 1
-        elif op == RO_ZSEL:
-            r = 0 if yv % p == 0 else xv
-        elif op == RO_MULNZ:
-            assert abs(xv) <= LAZY_LIMIT and abs(yv) <= LAZY_LIMIT, "operand beyond 2^40"
-            r = _bal(xv * (yv if yv % p else 1), p)
         elif op == RO_LDM:
             r = M[aux]
-        elif op == RO_STM:
-            M[aux] = xv
-            r = xv
         elif op == RO_LDC:
             r = lc[aux]
         elif op == RO_LDV:
             r = state[aux]
+        elif op in (RO_STM, RO_STE, RO_NOP):
+            assert d == x, "a store or no-op must carry dst = x"
+            if op == RO_STM:
+                M[aux] = xv
+            elif op == RO_STE:
+                E[aux] = xv
+            r = xv
         else:
             raise AssertionError(f"unknown residue op {op}")
-        R[d] = r
-        if es:
-            E[e] = r
+        if op not in (RO_STM, RO_STE, RO_NOP):
+            assert d < RES_TEMPS, "a result written to an input slot"
+        B[d] = r
     return E
 
 

@@ -17,6 +17,14 @@ def test_estimate_k_closed_form():
     assert estimate_K("n", "n**3-7*n", 50) == 64      # the floor
     assert estimate_K("1", "1", 1) == 64
     assert estimate_K("(", "n", 100) == 1000          # does not parse: the fallback
+    # depth 20000 does not parse: 4 * log2 of N! by N log2(N/e), 1.5 safety
+    assert estimate_K("(", "n", 20000) == int(4 * 20000 * math.log2(20000 / math.e) * 1.5 / 31) + 1
+    # coefficients that are not integers count as 100, the degree still counts
+    lf = 100 * math.log2(100 / math.e) + 0.5 * math.log2(2 * math.pi * 100)
+    assert estimate_K("n/2 + x", "n**4", 100) == max((int((4 * lf + 100 * math.log2(101)) * 1.5) + 100) // 31 + 1, 64)
+    # a constant numerator and a 7-term b: the largest coefficient (12) and degree (6)
+    lf = 3000 * math.log2(3000 / math.e) + 0.5 * math.log2(2 * math.pi * 3000)
+    assert estimate_K("5", "n**6 - 12*n + 3", 3000) == (int((6 * lf + 3000 * math.log2(13)) * 1.5) + 100) // 31 + 1
 
 
 def test_task_expansion_and_loader(tmp_path):
