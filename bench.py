@@ -488,6 +488,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--mode", choices=("batched", "persistent"), default="batched")
     ap.add_argument("--graph", action="store_true", help="replay the batch as a hipGraph")
+    ap.add_argument("--chain-cus", type=int, default=0,
+                    help="CUs reserved for the FP64 M_f pass and shadow (CU-masked streams); 0 = shared")
     ap.add_argument("--tune", action="store_true", help="pcf2: autotune the launch shape first")
     ap.add_argument("--workload", choices=("pcf2", "cmf3", "mix4", "zeta5", "sweep3f2"), default="pcf2",
                     help="pcf2 (the headline line); cmf3 / mix4 / zeta5 / sweep3f2 measure the other batch shapes")
@@ -550,7 +552,7 @@ def main():
     # so that the walk kernel's time matches its committed kernel trace
     args.threads, args.chunk = args.threads or 64, args.chunk or 4096
     plan = BatchPlan(specs, crt=True, top_k=top_k, threads=args.threads, chunk_steps=args.chunk,
-                     use_graph=args.graph, mode=args.mode)
+                     use_graph=args.graph, mode=args.mode, chain_cus=args.chain_cus)
     log(f"plan built: {time.perf_counter() - t0:.1f}s since start")
     ctx = plan.ctx
     if args.profile:
@@ -627,7 +629,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": desc, "walks": len(specs), "depth": depth, "K": K, "top_k": top_k,
                    "threads": args.threads, "chunk_steps": args.chunk, "graph": bool(args.graph),
-                   "mode": args.mode, "autotuned": tuned is not None},
+                   "mode": args.mode, "chain_cus": args.chain_cus, "autotuned": tuned is not None},
         "latency_batch_ms": {"p50": float(np.percentile(step_ms, 50)), "p99": float(np.percentile(step_ms, 99))},
         "with_copy_back": with_copy(rsteps, copy_ms),
         "plain": {"value": plain, "sample_walks": n_plain, "x": value / plain},

@@ -70,7 +70,7 @@ int upload_programs(drx_ctx* ctx) {
   const size_t np = ctx->progs.size();
   std::vector<DevProgram> dp(np);
   for (size_t i = 0; i < np; ++i) dp[i] = ctx->progs[i].dev;
-  // padded past the end: the interpreters read code in kCodeFetch-instruction pieces
+  // padded past the end by a margin
   const size_t code_words = ctx->h_code.size() + 2 * kCodeFetch;
   HIPCHK(ctx, ctx->d_code.ensure_keep(code_words * 4, ctx->retired));
   HIPCHK(ctx, ctx->d_cres.ensure_keep(std::max<size_t>(1, ctx->h_cres.size()) * 8, ctx->retired));
@@ -1432,6 +1432,8 @@ void drx_ctx_destroy(drx_ctx* ctx) {
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
   if (ctx->stream3) (void)hipStreamDestroy(ctx->stream3);
+  for (hipStream_t& m : ctx->mstream)
+    if (m) (void)hipStreamDestroy(m);
   delete ctx;
 }
 
@@ -1594,10 +1596,14 @@ int drx_program_upload_mode(drx_ctx* ctx, const uint32_t* ops, int n_ops, int m,
     }
   }
   DevProgram& d = hp.dev;
+  // the residue code and then the FP64 code in the pool, each from a 64-byte
+  // boundary, kResWords words per instruction (the fields unpacked, so the
+  // interpreters use them with no decoding)
+  while (ctx->h_code.size() % 16) ctx->h_code.push_back(0);
   d.code_off = (uint32_t)ctx->h_code.size();
   d.ncode = (uint32_t)pk.code.size();
-  d.fcode_off = (uint32_t)((ctx->h_code.size() + pk.code.size() + 1) & ~(size_t)1);  // 8-byte aligned pairs
-  d.nfcode = (uint32_t)(pk.fcode.size() / 2);
+  d.fcode_off = (uint32_t)(d.code_off + ((kResWords * pk.code.size() + 15) & ~(size_t)15));
+  d.nfcode = (uint32_t)pk.fcode.size();
   d.nconst = (uint32_t)nc;
   d.kcap = (uint32_t)kc;
   d.cres_off = ctx->h_cres.size();
@@ -1616,10 +1622,15 @@ int drx_program_upload_mode(drx_ctx* ctx, const uint32_t* ops, int n_ops, int m,
   hp.fd = mode == DRX_WALK_MATRIX && pk.fd_ok &&
           ((m == 2 && pk.fd_degree <= kFdMax2) || (m == 3 && pk.fd_degree <= kFdMax3) ||
            (m == 4 && comp_shape(d) >= 0));
-  ctx->h_code.insert(ctx->h_code.end(), pk.code.begin(), pk.code.end());
-  if (ctx->h_code.size() % 2) ctx->h_code.push_back(0);
-  ctx->h_code.insert(ctx->h_code.end(), pk.fcode.begin(), pk.fcode.end());
-  if (ctx->h_code.size() % 2) ctx->h_code.push_back(0);  // keep 8-byte alignment
+  for (const uint32_t w : pk.code) {
+    const ResWords u = unpack_res(w);
+    ctx->h_code.insert(ctx->h_code.end(), u.w, u.w + kResWords);
+  }
+  while (ctx->h_code.size() % 16) ctx->h_code.push_back(0);
+  for (const uint32_t w : pk.fcode) {
+    const ResWords u = unpack_res(w);
+    ctx->h_code.insert(ctx->h_code.end(), u.w, u.w + kResWords);
+  }
   for (int s = 0; s < nc; ++s) {
     const int ci = pk.const_slots[s];
     for (int k = 0; k < kc; ++k) ctx->h_cres.push_back(balanced(cr[(size_t)ci * kc + k], ctx->primes[k]));
@@ -1754,6 +1765,10 @@ int drx_plan_create(drx_ctx* ctx, const drx_batch* b, const drx_launch_cfg* cfg,
   pl->epoch = ctx->epoch;
   pl->b = *b;
   if (cfg) pl->cfg = *cfg;
+  if (pl->cfg.chain_cus < 0 || (pl->cfg.chain_cus > 0 && (pl->cfg.use_graph || pl->cfg.mode != DRX_MODE_BATCHED))) {
+    delete pl;
+    return fail(ctx, DRX_EINVAL, "chain_cus takes the batched mode without graph replay");
+  }
   if (pl->cfg.mode == DRX_MODE_PLAIN) {
     // one sub-plan per walk: each run launches and synchronises one walk
     pl->n = b->n;
@@ -1813,6 +1828,53 @@ int drx_plan_create(drx_ctx* ctx, const drx_batch* b, const drx_launch_cfg* cfg,
 }
 
 namespace {
+// CU-masked copies of the context's three streams for a split of `n` CUs:
+// the FP64 streams (M_f pass, shadow) on n CUs chosen so that every group of
+// 32 consecutive CU ids and every residue of the id mod 8 holds the same
+// number of them (the chip's XCDs get equal shares whichever of the two
+// numberings the ids follow), the residue stream on the rest.
+int ensure_masked_streams(drx_ctx* ctx, int n) {
+  if (ctx->masked_cus == n && ctx->mstream[0]) return DRX_OK;
+  int ncu = 0;
+  HIPCHK(ctx, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  if (ncu < 64 || ncu % 32 != 0 || n < 1 || n >= ncu || ncu % n != 0 || (ncu / n) > 8 || 8 % (ncu / n) != 0)
+    return fail(ctx, DRX_EINVAL, "chain_cus=%d: takes a divisor of the %d CUs that leaves 1/2 .. 1/8 of them", n, ncu);
+  const int step = ncu / n;
+  std::vector<uint32_t> mask_fp((ncu + 31) / 32, 0u), mask_res((ncu + 31) / 32, 0u);
+  for (int i = 0; i < ncu; ++i) {
+    const bool fp = ((i % 8) + (i / 32)) % step == 0;
+    (fp ? mask_fp : mask_res)[i / 32] |= 1u << (i % 32);
+  }
+  for (hipStream_t& m : ctx->mstream)
+    if (m) {
+      HIPCHK(ctx, hipStreamSynchronize(m));
+      HIPCHK(ctx, hipStreamDestroy(m));
+      m = nullptr;
+    }
+  ctx->masked_cus = 0;
+  HIPCHK(ctx, hipExtStreamCreateWithCUMask(&ctx->mstream[0], (uint32_t)mask_res.size(), mask_res.data()));
+  HIPCHK(ctx, hipExtStreamCreateWithCUMask(&ctx->mstream[1], (uint32_t)mask_fp.size(), mask_fp.data()));
+  HIPCHK(ctx, hipExtStreamCreateWithCUMask(&ctx->mstream[2], (uint32_t)mask_fp.size(), mask_fp.data()));
+  ctx->masked_cus = n;
+  return DRX_OK;
+}
+
+// The context's streams for the duration of one run (restored on scope exit).
+struct StreamSwap {
+  drx_ctx* ctx;
+  hipStream_t saved[3];
+  explicit StreamSwap(drx_ctx* c) : ctx(c), saved{c->stream, c->stream2, c->stream3} {
+    ctx->stream = ctx->mstream[0];
+    ctx->stream2 = ctx->mstream[1];
+    ctx->stream3 = ctx->mstream[2];
+  }
+  ~StreamSwap() {
+    ctx->stream = saved[0];
+    ctx->stream2 = saved[1];
+    ctx->stream3 = saved[2];
+  }
+};
+
 int plan_run(drx_plan* pl, bool fetch) {
   if (!pl) return DRX_EINVAL;
   drx_ctx* ctx = pl->ctx;
@@ -1820,6 +1882,12 @@ int plan_run(drx_plan* pl, bool fetch) {
   if (pl->epoch != ctx->epoch)
     return fail(ctx, DRX_ESTATE, "the context's primes or programs changed since this plan was built; build it again");
   HIPCHK(ctx, hipSetDevice(ctx->device));
+  std::unique_ptr<StreamSwap> swap;
+  if (pl->cfg.chain_cus > 0 && !pl->gexec && pl->plain.empty() && pl->cfg.mode != DRX_MODE_PLAIN) {
+    const int rc = ensure_masked_streams(ctx, pl->cfg.chain_cus);
+    if (rc) return rc;
+    swap.reset(new StreamSwap(ctx));
+  }
   Timing tm;
   if (!pl->plain.empty() || pl->cfg.mode == DRX_MODE_PLAIN) {
     const double t0 = now_s();

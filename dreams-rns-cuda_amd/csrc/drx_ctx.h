@@ -86,6 +86,10 @@ struct drx_ctx {
   hipStream_t stream = nullptr;   // residue walk, combine, tails
   hipStream_t stream2 = nullptr;  // FP64 M_f pass, forked from `stream`
   hipStream_t stream3 = nullptr;  // FP64 shadow, window by window behind stream2, joined into `stream`
+  // the same three streams with CU masks: the FP64 ones on `masked_cus`
+  // CUs spread over the chip, the first on the others (made on first use)
+  hipStream_t mstream[3] = {nullptr, nullptr, nullptr};
+  int masked_cus = 0;
   std::string err;
   // primes
   std::vector<uint32_t> primes;

@@ -51,6 +51,18 @@ enum ResOp : uint32_t {
   RO_STE = 12,   // entry row <- x (dst = x)
   RO_NOP = 15,   // nothing: B[x] <- B[x] (pads the code to whole groups)
 };
+// The FP64 code of the M_f pass (and of vector walks' shadow) uses the same
+// encoding, bank layout and register allocation with FP64 values in bank B
+// (the constant slots hold the constants' FP64 values): MUL, ADD and LIN
+// (SUB) are the FP64 operations, the LDS and no-op codes are shared, and
+// these replace the residue-only ones:
+enum FpOp : uint32_t {
+  FO_NEG = 3,   // -x
+  FO_CUB = 4,   // x^3, correctly rounded
+  FO_INV = 5,   // 1/x, or 0 when |x| <= 1e-300
+  FO_MOV = 6,   // x
+  FO_ZERO = 7,  // +0
+};
 constexpr int kResTemps = 16;    // temps B[0, 16)
 constexpr int kResCBase = 16;    // input slot j at B[16 + j]
 constexpr int kResStep = 15;     // input slot of the step; axis i at slot 14 - i
@@ -61,18 +73,26 @@ __host__ __device__ constexpr uint32_t enc_res(uint32_t op, uint32_t x, uint32_t
 __host__ __device__ constexpr uint32_t lin_signs(int sx, int sy) {
   return ((uint32_t)sx & 3u) | (((uint32_t)sy & 3u) << 2);
 }
+// The packer emits one word per instruction; in the device's code pool
+// each is stored unpacked as kResWords words {x, y, dst, op | aux << 4}, so
+// the interpreter takes the indices straight from the scalar registers the
+// code is loaded into, with no field extraction.
+constexpr int kResWords = 4;
+struct ResWords {
+  uint32_t w[kResWords];
+};
+__host__ __device__ constexpr ResWords unpack_res(uint32_t w) {
+  return ResWords{{(w >> 4) & 31u, (w >> 9) & 31u, (w >> 14) & 31u, (w & 15u) | ((w >> 19) << 4)}};
+}
 // The residue code is executed in groups of kResGroup instructions read by
-// one scalar load; the packer pads it with no-ops.
+// two scalar loads; the packer pads it with no-ops.
 constexpr int kResGroup = 8;
 constexpr uint32_t kResNop = RO_NOP;
 constexpr uint32_t kMaxRow = (1u << 13) - 1;
 
-// -------------------------------------------------------- FP64 encoding
-// The FP64 code (the M_f pass) is two 32-bit words per instruction:
-//   w0: [7:0] op    [9:8] dst bank   [25:10] dst index
-//   w1: [2:0] a bank [15:3] a index  [18:16] b bank  [31:19] b index
-// Every instruction also leaves its result in the accumulator A, so a value
-// consumed only by the next instruction never touches a register bank.
+// ------------------------------------------------ value operations
+// Operations of the packer's value graph (and of its FP64 code list); the
+// device codes above are generated from them.
 enum DevOp : uint32_t {
   D_ADD = 1,   // a + b
   D_SUB = 2,   // a - b
@@ -87,22 +107,6 @@ enum DevOp : uint32_t {
   D_ZSEL = 11, // a, or 0 when b = 0 mod p (residue code only)
   D_MULNZ = 12,  // a * (b, or 1 when b = 0 mod p) (residue code only)
 };
-// Destination banks.
-enum DstBank : uint32_t { DB_R = 0, DB_E = 1, DB_S = 2, DB_NONE = 3 };
-// Source banks: R temps (VGPR), X axes 0..14 / step 15 (VGPR),
-// C constants (VGPR), L constants (LDS), S temps (LDS), A accumulator, Z zero.
-// V: the lane's state vector (vector walks).
-enum SrcBank : uint32_t { SB_R = 0, SB_X = 1, SB_C = 2, SB_L = 3, SB_S = 4, SB_A = 5, SB_Z = 6, SB_V = 7 };
-
-constexpr int kBank = 16;       // VGPR bank width (doubles)
-constexpr int kStepSlot = 15;   // X[15] holds the step
-
-__host__ __device__ constexpr uint32_t enc_w0(uint32_t op, uint32_t dbank, uint32_t dst) {
-  return op | (dbank << 8) | (dst << 10);
-}
-__host__ __device__ constexpr uint32_t enc_w1(uint32_t abank, uint32_t a, uint32_t bbank, uint32_t b) {
-  return abank | (a << 3) | (bbank << 16) | (b << 19);
-}
 
 // Matrix entry kinds (2 bits per entry, entry e = row*m + col).
 enum EntryKind : uint32_t { E_ZERO = 0, E_ONE = 1, E_GEN = 2 };

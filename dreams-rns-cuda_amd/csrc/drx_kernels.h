@@ -47,9 +47,9 @@ struct KArgs {
   double* axd_w;
 };
 
-// FP64 instructions per scalar code load: the M_f interpreter reads its
-// program in 64-byte pieces instead of waiting for one 8-byte scalar load per
-// instruction; the code pool is padded by kCodeFetch instructions.
+// Words of padding past the end of the code pool (both codes are whole
+// groups of kResGroup instructions, so no read passes the end; the padding
+// is a margin).
 constexpr int kCodeFetch = 8;
 
 // Walk kernel families.

@@ -44,32 +44,40 @@ struct Lane {
   uint32_t tid, T;   // for LDS temp slots
 };
 
-// A 2-bit two's complement LIN sign at bit `lo` of the word, as a double.
-__device__ __forceinline__ double lin_sign(uint32_t w, int lo) {
-  return (double)((int32_t)(w << (30 - lo)) >> 30);
+// A 2-bit two's complement LIN sign at bit `lo` of the aux field, as a double.
+__device__ __forceinline__ double lin_sign(uint32_t aux, int lo) {
+  return (double)((int32_t)(aux << (30 - lo)) >> 30);
 }
 
-// One residue instruction (see the encoding in drx_device.h). Bank B holds
-// the temps and the inputs (step, axes, first constant slots); `lc` is the
-// LDS table of further constant slots ([rows][KP], by prime), `lm` the lanes'
-// spill rows, `le` their entry rows and `lv` their state rows ([rows][T], by
-// thread). MUL and ADD, most of any program, take the first two branches;
-// STM, STE and NOP carry dst = x, so every instruction ends in one write.
-__device__ __forceinline__ void exec_res(uint32_t w, RBank& B, const double* __restrict__ lc, uint32_t KP,
-                                         double* __restrict__ lm, double* __restrict__ le,
-                                         const double* __restrict__ lv, const Lane& ln) {
-  const uint32_t op = w & 15u;
-  const double x = B.get((w >> 4) & 31u);
-  const double y = B.get((w >> 9) & 31u);
+// Code words are read through the constant address space: the code pool is
+// never written while a kernel runs, and loads from it are then scalar loads
+// even in kernels whose loops also store to global memory (through a plain
+// pointer the compiler could not rule out that those stores reach the code).
+typedef const __attribute__((address_space(4))) uint32_t* CodePtr;
+__device__ __forceinline__ CodePtr code_ptr(const uint32_t* p) { return (CodePtr)p; }
+
+// One residue instruction (see the encoding in drx_device.h), given as its
+// unpacked words: x, y and dst index bank B, which holds the temps and the
+// inputs (step, axes, first constant slots); `lc` is the LDS table of
+// further constant slots ([rows][KP], by prime), `lm` the lanes' spill rows,
+// `le` their entry rows and `lv` their state rows ([rows][T], by thread).
+// MUL and ADD, most of any program, take the first two branches; STM, STE
+// and NOP carry dst = x, so every instruction ends in one write.
+__device__ __forceinline__ void exec_res(uint32_t wx, uint32_t wy, uint32_t wd, uint32_t wo, RBank& B,
+                                         const double* __restrict__ lc, uint32_t KP, double* __restrict__ lm,
+                                         double* __restrict__ le, const double* __restrict__ lv, const Lane& ln) {
+  const uint32_t op = wo & 15u;
+  const double x = B.get(wx);
+  const double y = B.get(wy);
   double r;
   if (op == RO_MUL) {
     r = mm(x, y, ln.p, ln.pinv);
   } else if (op == RO_ADD) {
     r = x + y;
   } else {
-    const uint32_t row = w >> 19;
+    const uint32_t row = wo >> 4;
     switch (op) {
-      case RO_LIN: r = __builtin_fma(y, lin_sign(w, 21), x * lin_sign(w, 19)); break;
+      case RO_LIN: r = __builtin_fma(y, lin_sign(row, 2), x * lin_sign(row, 0)); break;
       case RO_MULNZ: r = mm(x, (red(y, ln.p, ln.pinv) == 0.0) ? 1.0 : y, ln.p, ln.pinv); break;
       case RO_ZSEL: r = (red(y, ln.p, ln.pinv) == 0.0) ? 0.0 : x; break;
       case RO_INV: r = inv_fermat(x, ln.p, ln.pinv, ln.pe); break;
@@ -89,23 +97,25 @@ __device__ __forceinline__ void exec_res(uint32_t w, RBank& B, const double* __r
       default: r = x; break;  // RO_NOP
     }
   }
-  // one write per instruction (the ops that write nothing carry dst = x)
-  B.set((w >> 14) & 31u, r);
+  B.set(wd, r);
 }
 
-// The packed residue program over one prime per lane, kResGroup instructions
-// per scalar load (n is a multiple of kResGroup).
-__device__ __forceinline__ void run_program(const uint32_t* __restrict__ code, uint32_t n, RBank& B,
+// The residue program over one prime per lane, kResGroup instructions (their
+// kResWords words each) per group of scalar loads; n, the instruction count,
+// is a multiple of kResGroup.
+__device__ __forceinline__ void run_program(const uint32_t* __restrict__ code_g, uint32_t n, RBank& B,
                                             const double* __restrict__ lc, uint32_t KP, double* __restrict__ lm,
                                             double* __restrict__ le, const double* __restrict__ lv,
                                             const Lane& ln) {
+  const CodePtr code = code_ptr(code_g);
 #pragma unroll 1
   for (uint32_t pc = 0; pc < n; pc += kResGroup) {
-    uint32_t w[kResGroup];
+    uint32_t w[kResGroup * kResWords];
 #pragma unroll
-    for (int j = 0; j < kResGroup; ++j) w[j] = code[pc + j];
+    for (int j = 0; j < kResGroup * kResWords; ++j) w[j] = code[pc * kResWords + j];
 #pragma unroll
-    for (int j = 0; j < kResGroup; ++j) exec_res(w[j], B, lc, KP, lm, le, lv, ln);
+    for (int j = 0; j < kResGroup; ++j)
+      exec_res(w[4 * j], w[4 * j + 1], w[4 * j + 2], w[4 * j + 3], B, lc, KP, lm, le, lv, ln);
   }
 }
 
@@ -260,10 +270,10 @@ __device__ __forceinline__ WalkSetup walk_setup(const KArgs& a, double* lds, boo
 // groups of 4, all in flight together (indices past the end re-read the
 // last valid one). Two ways to fill the bank, which the register allocator
 // takes differently: GATHER collects the slots in a 16-wide vector and moves
-// them into the bank in one unconditional pass (the walk kernel: 150 VGPRs
-// for 3x3 against 182 for the other way); otherwise the group loops are not
-// unrolled and write the bank through indexed writes in place (the seed
-// kernel: 139 against 228).
+// them into the bank in one unconditional pass (the walk kernels: 150 VGPRs
+// for 3x3 against 182 the other way, the vector walk 98 against 120);
+// otherwise the group loops are not unrolled and write the bank through
+// indexed writes in place (the seed kernel: 139 against 228).
 template <bool GATHER>
 __device__ __forceinline__ void load_rns_inputs(const KArgs& a, const DevWalk& w, const DevProgram& pg,
                                                 uint32_t K, uint32_t kk, const Lane& ln, int64_t s0, RBank& B,
@@ -804,76 +814,90 @@ __global__ void k_combine(const KArgs a) {
 }
 
 // ------------------------------------------------------- FP64 M_f(s) pass
-// The program in FP64 (runner.py:263-316 / cmf_walk.py:261-313), one step
+// The program in FP64 (runner.py:263-316 / cmf_walk.py:261-313): the same
+// bank instruction set as the residue code with FP64 operations, one step
 // per lane; all lanes of a block share the walk, so decode is wave-uniform.
-// Constants (banks C and L) are read from an LDS copy of the FP64 constant
-// table: a uniform constant bank would sit in SGPRs and be copied to VGPRs on
-// every dynamically indexed read.
-template <bool VEC>
-__device__ __forceinline__ double fetch_fp(uint32_t bank, uint32_t i, const v16d& R, const v16d& X, const v16d& V,
-                                           double acc, const double* __restrict__ ldc,
-                                           const double* __restrict__ lds_s, uint32_t tid, uint32_t T) {
-  switch (bank) {
-    case SB_R: return R[i & 15u];
-    case SB_X: return X[i & 15u];
-    case SB_C: return ldc[i & 15u];
-    case SB_L: return ldc[kBank + i];
-    case SB_S: return lds_s[(uint64_t)i * T + tid];
-    case SB_A: return acc;
-    case SB_V: return VEC ? V[i & 15u] : 0.0;
-    default: return 0.0;
-  }
-}
-
-// One FP64 instruction; with VEC, bank V is the lane's state vector.
-template <bool VEC, typename EV>
-__device__ __forceinline__ void exec_fp(uint2 w, double& acc, v16d& R, EV& E, const v16d& X, const v16d& V,
-                                        const double* __restrict__ ldc, double* __restrict__ lds_s, uint32_t tid,
-                                        uint32_t T) {
-  const uint32_t op = w.x & 255u;
-  const double x = fetch_fp<VEC>(w.y & 7u, (w.y >> 3) & 8191u, R, X, V, acc, ldc, lds_s, tid, T);
+// Bank B holds the temps, the first constant slots (the FP64 constants, the
+// same in every lane), the axes and the step; further constants are read
+// from an LDS copy of the program's FP64 constants (LDC), spills from the
+// lanes' LDS rows. Entries leave through STE into the register vector E; a
+// vector walk's state is read through LDV from the register vector V.
+template <typename EV>
+__device__ __forceinline__ void exec_f(uint32_t wx, uint32_t wy, uint32_t wd, uint32_t wo, RBank& B, EV& E,
+                                       const v16d& V, const double* __restrict__ ldc, uint32_t lc0,
+                                       double* __restrict__ lm, uint32_t tid, uint32_t T) {
+  const uint32_t op = wo & 15u;
+  const double x = B.get(wx);
+  const double y = B.get(wy);
   double r;
-  if (op == D_MUL || op == D_ADD || op == D_SUB) {
-    const double y = fetch_fp<VEC>((w.y >> 16) & 7u, w.y >> 19, R, X, V, acc, ldc, lds_s, tid, T);
-    r = (op == D_MUL) ? x * y : ((op == D_ADD) ? x + y : x - y);
+  if (op == RO_MUL) {
+    r = x * y;
+  } else if (op == RO_ADD) {
+    r = x + y;
   } else {
+    const uint32_t row = wo >> 4;
     switch (op) {
-      case D_NEG: r = -x; break;
-      case D_SQR: r = x * x; break;
-      case D_CUB: r = cube_cr(x); break;
-      case D_INV: r = finv(x); break;
-      default: r = x; break;  // D_RED (no-op in FP64), D_MOV
+      case RO_LIN: r = __builtin_fma(y, lin_sign(row, 2), x * lin_sign(row, 0)); break;  // x - y: one rounding
+      case FO_NEG: r = -x; break;
+      case FO_CUB: r = cube_cr(x); break;
+      case FO_INV: r = finv(x); break;
+      case FO_ZERO: r = 0.0; break;
+      case RO_LDM: r = lm[row * T + tid]; break;
+      case RO_LDC: r = ldc[lc0 + row]; break;
+      case RO_LDV: r = V[row & 15u]; break;
+      case RO_STM:
+        lm[row * T + tid] = x;
+        r = x;
+        break;
+      case RO_STE:
+        E[row & (sizeof(EV) / sizeof(double) - 1)] = x;
+        r = x;
+        break;
+      default: r = x; break;  // FO_MOV, RO_NOP
     }
   }
-  const uint32_t db = (w.x >> 8) & 3u, d = w.x >> 10;
-  if (db == DB_R) R[d & 15u] = r;
-  else if (db == DB_E) E[d & (sizeof(EV) / sizeof(double) - 1)] = r;
-  else if (db == DB_S) lds_s[(uint64_t)d * T + tid] = r;
-  acc = r;
+  B.set(wd, r);
 }
 
-template <bool VEC, typename EV>
-__device__ __forceinline__ void run_program_fp(const uint2* __restrict__ code, uint32_t n, v16d& R, EV& E,
-                                               const v16d& X, const v16d& V, const double* __restrict__ ldc,
-                                               double* __restrict__ lds_s, uint32_t tid, uint32_t T) {
-  double acc = 0.0;
+template <typename EV>
+__device__ __forceinline__ void run_program_f(const uint32_t* __restrict__ code_g, uint32_t n, RBank& B, EV& E,
+                                              const v16d& V, const double* __restrict__ ldc, uint32_t lc0,
+                                              double* __restrict__ lm, uint32_t tid, uint32_t T) {
+  const CodePtr code = code_ptr(code_g);
 #pragma unroll 1
-  for (uint32_t pc = 0; pc < n; pc += kCodeFetch) {
-    uint2 w[kCodeFetch];
+  for (uint32_t pc = 0; pc < n; pc += kResGroup) {
+    uint32_t w[kResGroup * kResWords];
 #pragma unroll
-    for (int j = 0; j < kCodeFetch; ++j) w[j] = code[pc + j];
+    for (int j = 0; j < kResGroup * kResWords; ++j) w[j] = code[pc * kResWords + j];
 #pragma unroll
-    for (int j = 0; j < kCodeFetch; ++j)
-      if (pc + j < n) exec_fp<VEC>(w[j], acc, R, E, X, V, ldc, lds_s, tid, T);
+    for (int j = 0; j < kResGroup; ++j)
+      exec_f(w[4 * j], w[4 * j + 1], w[4 * j + 2], w[4 * j + 3], B, E, V, ldc, lc0, lm, tid, T);
   }
+}
+
+// The FP64 bank's constant slots, from the LDS copy of the constants.
+__device__ __forceinline__ void load_f_consts(RBank& B, const DevProgram& pg, const double* __restrict__ ldc) {
+  const uint32_t nc = min((uint32_t)kResStep - pg.dim, pg.nconst);
+#pragma unroll 1
+  for (uint32_t i = 0; i < nc; ++i) B.set(kResCBase + i, ldc[i]);
+}
+
+// The FP64 axes at step s, x_i = (c0 + s c1) / den as the reference forms
+// them, and the step.
+__device__ __forceinline__ void set_f_axes(RBank& B, const KArgs& a, const DevWalk& w, uint32_t dim, int64_t s) {
+#pragma unroll 1
+  for (uint32_t ax = 0; ax < dim; ++ax) {
+    const int64_t* c = a.axi + w.axi_off + 2 * ax;
+    B.set(kResCBase + kResStep - 1 - ax, (double)(c[0] + s * c[1]) / a.axd[w.axd_off + ax]);
+  }
+  B.set(kResCBase + kResStep, (double)s);
 }
 
 // One block per range of kMfRange consecutive steps of one task (two steps
 // per lane, so the block's set-up is paid once for 512 steps); lanes past
 // the range repeat its last step so the interpreter never diverges. M_f(s)
 // is stored column-major: the shadow lane of column j reads M_f[0..M)[j] in
-// one piece. (Both steps of a lane through one decode, with two register
-// banks, measured slower: the doubled banks halve the resident waves.)
+// one piece.
 template <int M>
 __global__ void __launch_bounds__(256) k_mf(const KArgs a) {
   // the shadow chain waits for each window of M_f while the walk kernel
@@ -890,20 +914,19 @@ __global__ void __launch_bounds__(256) k_mf(const KArgs a) {
   double* lds_s = ldcf + pg.nconst;
   for (uint32_t i = tid; i < pg.nconst; i += T) ldcf[i] = a.cflt[pg.cflt_off + i];
   __syncthreads();
-  const uint2* code = reinterpret_cast<const uint2*>(a.code + pg.fcode_off);
+  const uint32_t* code = a.code + pg.fcode_off;
+  RBank B;
+  B.lo = 0u;
+  B.hi = 0u;
+  load_f_consts(B, pg, ldcf);
+  const v16d V = 0.0;
 #pragma unroll 1
   for (int64_t s0 = r.s_begin; s0 < r.s_end; s0 += T) {  // uniform
     typename EBank<M>::type E = 0.0;
-    v16d R = 0.0, X = 0.0;
     const int64_t want = s0 + tid;
     const int64_t s = want < r.s_end ? want : r.s_end - 1;
-#pragma unroll 1
-    for (uint32_t ax = 0; ax < pg.dim; ++ax) {
-      const int64_t* c = a.axi + w.axi_off + 2 * ax;
-      X[ax] = (double)(c[0] + s * c[1]) / a.axd[w.axd_off + ax];
-    }
-    X[kStepSlot] = (double)s;
-    run_program_fp<false>(code, pg.nfcode, R, E, X, X, ldcf, lds_s, tid, T);
+    set_f_axes(B, a, w, pg.dim, s);
+    run_program_f(code, pg.nfcode, B, E, V, ldcf, (uint32_t)kResStep - pg.dim, lds_s, tid, T);
     if (want < r.s_end) {
       double* out = a.mf + t.mf_off + (uint64_t)(s - t.s_begin) * M * M;
 #pragma unroll
@@ -1260,7 +1283,7 @@ __global__ void __launch_bounds__(256) k_walk_vec(const KArgs a) {
   RBank B;
   B.lo = 0u;
   B.hi = 0u;
-  load_rns_inputs<false>(a, w, pg, K, kk, ln, 0, B, li, T, tid);
+  load_rns_inputs<true>(a, w, pg, K, kk, ln, 0, B, li, T, tid);
 #pragma unroll 1
   for (uint32_t j = 0; j < m; ++j) sa[j * T + tid] = a.res_in[w.res_off + (uint64_t)j * K + kk];
   __syncthreads();  // constant rows
@@ -1304,8 +1327,12 @@ __global__ void __launch_bounds__(256) k_vec_fp(const KArgs a) {
   double* lds_s = ldcf + pg.nconst;
   for (uint32_t i = tid; i < pg.nconst; i += T) ldcf[i] = a.cflt[pg.cflt_off + i];
   __syncthreads();
-  const uint2* code = reinterpret_cast<const uint2*>(a.code + pg.fcode_off);
-  v16d R = 0.0, X = 0.0, V = 0.0, E = 0.0;
+  const uint32_t* code = a.code + pg.fcode_off;
+  RBank B;
+  B.lo = 0u;
+  B.hi = 0u;
+  load_f_consts(B, pg, ldcf);
+  v16d V = 0.0, E = 0.0;
 #pragma unroll
   for (int j = 0; j < 16; ++j)
     if ((uint32_t)j < m) V[j] = a.fst_in[w.fst_off + j];
@@ -1313,13 +1340,8 @@ __global__ void __launch_bounds__(256) k_vec_fp(const KArgs a) {
   int ex = 0;
 #pragma unroll 1
   for (int64_t s = 0; s < vg.depth; ++s) {
-#pragma unroll 1
-    for (uint32_t ax = 0; ax < pg.dim; ++ax) {
-      const int64_t* c = a.axi + w.axi_off + 2 * ax;
-      X[ax] = (double)(c[0] + s * c[1]) / a.axd[w.axd_off + ax];
-    }
-    X[kStepSlot] = (double)s;
-    run_program_fp<true>(code, pg.nfcode, R, E, X, V, ldcf, lds_s, tid, T);
+    set_f_axes(B, a, w, pg.dim, s);
+    run_program_f(code, pg.nfcode, B, E, V, ldcf, (uint32_t)kResStep - pg.dim, lds_s, tid, T);
     double mx = 0.0;
     bool nan = false;
 #pragma unroll

@@ -343,106 +343,9 @@ std::vector<int> schedule(const std::vector<Val>& vals, const std::vector<double
   return order;
 }
 
-// Operand routing and register allocation of one code list.
-std::string encode(const CodeList& cl, const std::vector<Val>& vals, const std::vector<int>& cslot,
-                   std::vector<uint32_t>& words, int& nslots) {
-  const std::vector<Ins>& code = cl.code;
-  const int nc = (int)code.size(), nvals = (int)vals.size();
-  std::vector<int> def_pos(nvals, -1), last_use(nvals, -1);
-  std::vector<char> only_next(nvals, 1);
-  for (int i = 0; i < nc; ++i)
-    if (code[i].def >= 0) def_pos[code[i].def] = i;
-  for (int i = 0; i < nc; ++i) {
-    for (int u : {code[i].a, code[i].b}) {
-      if (u < 0 || vals[u].kind != V_OP) continue;
-      last_use[u] = std::max(last_use[u], i);
-      if (i != def_pos[u] + 1) only_next[u] = 0;
-    }
-  }
-  // linear scan: slot 0..15 = VGPR bank R, 16.. = LDS bank S
-  std::vector<int> slot(nvals, -1);
-  std::vector<int> free_r, free_s;
-  for (int s = kBank - 1; s >= 0; --s) free_r.push_back(s);
-  int n_s = 0;
-  std::vector<std::vector<int>> dying(nc);
-  for (int v = 0; v < nvals; ++v)
-    if (last_use[v] >= 0 && def_pos[v] >= 0 && !(only_next[v] && cl.entry_direct[v] < 0))
-      dying[last_use[v]].push_back(v);
-  words.clear();
-  words.reserve(2 * nc);
-  for (int i = 0; i < nc; ++i) {
-    const Ins& in = code[i];
-    uint32_t sb[2] = {SB_Z, SB_Z}, si[2] = {0, 0};
-    const int opnd[2] = {in.a, in.b};
-    for (int q = 0; q < 2; ++q) {
-      const int u = opnd[q];
-      if (u < 0) continue;
-      const Val& x = vals[u];
-      if (x.kind == V_ZERO) {
-        sb[q] = SB_Z;
-      } else if (x.kind == V_X) {
-        sb[q] = SB_X;
-        si[q] = (uint32_t)x.idx;
-      } else if (x.kind == V_N) {
-        sb[q] = SB_X;
-        si[q] = kStepSlot;
-      } else if (x.kind == V_C) {
-        const int cs = cslot[x.idx];
-        sb[q] = cs < kBank ? SB_C : SB_L;
-        si[q] = (uint32_t)(cs < kBank ? cs : cs - kBank);
-      } else if (x.kind == V_S) {
-        sb[q] = SB_V;
-        si[q] = (uint32_t)x.idx;
-      } else if (def_pos[u] == i - 1) {
-        sb[q] = SB_A;
-      } else {
-        const int s = slot[u];
-        if (s < 0) return "internal: operand without a register";
-        sb[q] = s < kBank ? SB_R : SB_S;
-        si[q] = s < kBank ? (uint32_t)s : (uint32_t)(s - kBank);
-      }
-    }
-    // free registers of values dying here (before assigning the destination)
-    for (int v : dying[i]) {
-      const int s = slot[v];
-      if (s < 0) continue;
-      if (s < kBank) free_r.push_back(s);
-      else free_s.push_back(s);
-    }
-    uint32_t db = DB_NONE, di = 0;
-    if (in.entry >= 0) {
-      db = DB_E;
-      di = (uint32_t)in.entry;
-    } else if (in.def >= 0) {
-      const int v = in.def;
-      if (cl.entry_direct[v] >= 0) {
-        db = DB_E;
-        di = (uint32_t)cl.entry_direct[v];
-      } else if (last_use[v] >= 0 && !only_next[v]) {
-        int s;
-        if (!free_r.empty()) {
-          s = free_r.back();
-          free_r.pop_back();
-        } else if (!free_s.empty()) {
-          s = free_s.back();
-          free_s.pop_back();
-        } else {
-          s = kBank + n_s++;
-        }
-        slot[v] = s;
-        db = s < kBank ? DB_R : DB_S;
-        di = s < kBank ? (uint32_t)s : (uint32_t)(s - kBank);
-      }
-    }
-    words.push_back(enc_w0(in.op, db, di));
-    words.push_back(enc_w1(sb[0], si[0], sb[1], si[1]));
-  }
-  nslots = n_s;
-  return std::string();
-}
-
-// Residue code: lowering to the bank instruction set with register
-// allocation over the 16 temps of bank B. Axes, the step and the first
+// Residue code (and, with `fp`, the FP64 code of the M_f pass): lowering to
+// the bank instruction set with register allocation over the 16 temps of
+// bank B. Axes, the step and the first
 // 15 - dim constant slots sit in B's input slots and are read in place;
 // further constants are loaded from LDS (LDC), a vector walk's state entries
 // too (LDV), and zero is made by LIN. A value brought into a temp stays there
@@ -452,7 +355,7 @@ std::string encode(const CodeList& cl, const std::vector<Val>& vals, const std::
 // in the future leaves; a computed one is spilled to an LDS row (STM) first
 // and reloaded (LDM) before its next use.
 std::string encode_res(const CodeList& cl, const std::vector<Val>& vals, const std::vector<int>& cslot, int dim,
-                       std::vector<uint32_t>& words, int& nrows) {
+                       std::vector<uint32_t>& words, int& nrows, bool fp) {
   const std::vector<Ins>& code = cl.code;
   const int nc = (int)code.size(), nvals = (int)vals.size();
   const int n_cbank = kResStep - dim;  // constant slots held in the bank
@@ -523,6 +426,7 @@ std::string encode_res(const CodeList& cl, const std::vector<Val>& vals, const s
     if (x.kind == V_OP) words.push_back(enc_res(RO_LDM, any, any, (uint32_t)r, (uint32_t)row[u]));
     else if (x.kind == V_S) words.push_back(enc_res(RO_LDV, any, any, (uint32_t)r, (uint32_t)x.idx));
     else if (x.kind == V_C) words.push_back(enc_res(RO_LDC, any, any, (uint32_t)r, (uint32_t)(cslot[x.idx] - n_cbank)));
+    else if (fp) words.push_back(enc_res(FO_ZERO, any, any, (uint32_t)r, 0));
     else words.push_back(enc_res(RO_LIN, any, any, (uint32_t)r, lin_signs(0, 0)));  // zero
     reg[u] = r;
     held[r] = u;
@@ -555,6 +459,13 @@ std::string encode_res(const CodeList& cl, const std::vector<Val>& vals, const s
         op = RO_LIN;
         aux = lin_signs(1, -1);
       }
+    } else if (fp && (kind == D_CUB || kind == D_INV || kind == D_NEG || kind == D_RED)) {
+      // single FP64 operations: the correctly rounded cube, 1/x with 0 for
+      // |x| <= 1e-300, the sign flip, and RED, which is no operation in FP64
+      x = operand(in.a);
+      if (x < 0) return "internal: no free register";
+      y = x;
+      op = kind == D_CUB ? FO_CUB : (kind == D_INV ? FO_INV : (kind == D_NEG ? FO_NEG : FO_MOV));
     } else if (kind == D_SQR || kind == D_CUB) {
       x = operand(in.a);
       if (x < 0) return "internal: no free register";
@@ -578,6 +489,8 @@ std::string encode_res(const CodeList& cl, const std::vector<Val>& vals, const s
       if (x < 0) return "internal: no free register";
       y = (int)any;
       op = kind == D_INV ? RO_INV : (kind == D_RED ? RO_RED : RO_NZ1);
+    } else if (fp) {
+      return "internal: residue-only operation in the FP64 code";
     } else if (kind == D_ZSEL || kind == D_MULNZ) {
       x = operand(in.a);
       if (x < 0) return "internal: no free register";
@@ -876,22 +789,25 @@ std::string pack_program(const uint32_t* ops, int n_ops, int m, int dim, int n_c
   std::vector<int> dummy_cslot(n_const_all, 0);
 
   // ---- FP64 code: the same operations as the reference (one reciprocal per
-  // INV), in the candidate order needing the fewest LDS temp slots
+  // INV), in the candidate order giving the shortest code, then the fewest
+  // LDS spill rows
   CodeList fcl;
   std::vector<Val> fvals;
   {
     std::vector<double> prio(nv);
     for (int v = 0; v < nv; ++v) prio[v] = v;
     int best = -1;
+    size_t best_flen = 0;
     for (int policy = 0; policy < kPolicies; ++policy) {
       std::vector<Val> cv = vals;
       CodeList cl = build_code(cv, schedule(cv, prio, roots, policy), outv, out_gen, false, false);
       std::vector<uint32_t> scratch_words;
       int ns = 0;
-      const std::string err = encode(cl, cv, dummy_cslot, scratch_words, ns);
+      const std::string err = encode_res(cl, cv, dummy_cslot, dim, scratch_words, ns, true);
       if (!err.empty()) return err;
-      if (best < 0 || ns < best) {
+      if (best < 0 || scratch_words.size() < best_flen || (scratch_words.size() == best_flen && ns < best)) {
         best = ns;
+        best_flen = scratch_words.size();
         fcl = std::move(cl);
         fvals = std::move(cv);
       }
@@ -912,7 +828,7 @@ std::string pack_program(const uint32_t* ops, int n_ops, int m, int dim, int n_c
       CodeList cl = build_code(cv, order, outv, out_gen, true, vec);
       std::vector<uint32_t> scratch_words;
       int ns = 0;
-      const std::string err = encode_res(cl, cv, dummy_cslot, dim, scratch_words, ns);
+      const std::string err = encode_res(cl, cv, dummy_cslot, dim, scratch_words, ns, false);
       if (!err.empty()) return err;
       if (best_slots < 0 || scratch_words.size() < best_len || (scratch_words.size() == best_len && ns < best_slots)) {
         best_slots = ns;
@@ -928,13 +844,14 @@ std::string pack_program(const uint32_t* ops, int n_ops, int m, int dim, int n_c
   // in VGPRs, the rest in LDS), then routing and allocation of both codes
   std::vector<int> cslot;
   assign_const_slots(rcl, rvals, fcl, fvals, n_const_all, cslot, pp.const_slots);
-  pp.const_lds = pp.const_slots.size() > (size_t)kBank ? 1 : 0;
-  if (pp.const_slots.size() > 8191 + kBank) return "too many constants";
-  std::string err = encode_res(rcl, rvals, cslot, dim, pp.code, pp.nslots);
+  pp.const_lds = pp.const_slots.size() > (size_t)(kResStep - dim) ? 1 : 0;
+  if (pp.const_slots.size() > kMaxRow + (size_t)(kResStep - dim)) return "too many constants";
+  std::string err = encode_res(rcl, rvals, cslot, dim, pp.code, pp.nslots, false);
   if (!err.empty()) return err;
   while (pp.code.size() % kResGroup) pp.code.push_back(kResNop);
-  err = encode(fcl, fvals, cslot, pp.fcode, pp.fnslots);
+  err = encode_res(fcl, fvals, cslot, dim, pp.fcode, pp.fnslots, true);
   if (!err.empty()) return err;
+  while (pp.fcode.size() % kResGroup) pp.fcode.push_back(kResNop);
   *out = std::move(pp);
   return std::string();
 }

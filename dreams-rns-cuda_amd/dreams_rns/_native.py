@@ -16,7 +16,7 @@ import numpy as np
 
 from ._build import LIB_PATH
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 PACK_META = 13  # values drx_program_pack reports
 MAX_K = 8192  # primes one context holds (walks of more than 256 primes run as prime slices)
 DRX_INIT_PCF, DRX_INIT_IDENTITY, DRX_INIT_GIVEN = 0, 1, 2
@@ -66,7 +66,7 @@ class Batch(C.Structure):
 
 class LaunchCfg(C.Structure):
     _fields_ = [("mode", C.c_int), ("threads", C.c_int), ("chunk_steps", C.c_int), ("use_graph", C.c_int),
-                ("interp_only", C.c_int)]
+                ("interp_only", C.c_int), ("chain_cus", C.c_int)]
 
 
 class ServerCfg(C.Structure):

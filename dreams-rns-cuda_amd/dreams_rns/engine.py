@@ -279,7 +279,8 @@ class BatchPlan:
 
     def __init__(self, specs: Sequence[WalkSpec], ctx: Optional[N.Context] = None, mode: str = "batched",
                  threads: int = 0, chunk_steps: int = 0, use_graph: bool = False, crt: bool = False,
-                 full: bool = False, top_k: int = 0, interp_only: bool = False, tune: bool = False):
+                 full: bool = False, top_k: int = 0, interp_only: bool = False, tune: bool = False,
+                 chain_cus: int = 0):
         self.ctx = ctx or N.context()
         n = len(specs)
         if n == 0:
@@ -323,7 +324,7 @@ class BatchPlan:
         b.top_k = int(top_k)
         b.topk_idx = r.topk.ctypes.data if top_k else None
         cfg = N.LaunchCfg(self.MODES[mode], int(threads), int(chunk_steps), 1 if use_graph else 0,
-                          1 if interp_only else 0)
+                          1 if interp_only else 0, int(chain_cus))
         self._b, self._cfg = b, cfg
         h = C.c_void_p()
         ctx._check(ctx.lib.drx_plan_create(ctx.h, C.byref(b), C.byref(cfg), C.byref(h)))

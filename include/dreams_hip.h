@@ -18,7 +18,7 @@
 extern "C" {
 #endif
 
-#define DRX_ABI_VERSION 4
+#define DRX_ABI_VERSION 5
 
 enum {
   DRX_OK = 0,
@@ -170,6 +170,8 @@ typedef struct drx_launch_cfg {
   int chunk_steps;  /* max steps of one walk per workgroup task; 0 = tuned */
   int use_graph;    /* capture the launch sequence in a hipGraph and replay it */
   int interp_only;  /* 1: interpret every step even for polynomial programs (no forward differences) */
+  int chain_cus;    /* > 0: that many CUs (spread over the chip) run only the FP64 M_f pass and shadow,
+                       the walk the others; 0: every stream may use every CU. Not with use_graph. */
 } drx_launch_cfg;
 
 /* Run a batch of walks to completion. Replaces run_pcf_walk (runner.py:338),

@@ -1,7 +1,7 @@
 """CPU model of the packed device code (test helper).
 
-Executes the words produced by the packer (drx_program_pack) with the bank /
-accumulator semantics of the device interpreter, using exact integers mod p
+Executes the words produced by the packer (drx_program_pack) with the bank
+semantics of the device interpreters, using exact integers mod p
 for the residue code and Python floats for the FP64 code, then the per-step
 product P <- P * M exactly like the walk kernel's entry-kind matmul. Folded
 constants are computed from the program's constants the way the upload does.
@@ -15,17 +15,8 @@ from __future__ import annotations
 from typing import List, Sequence
 
 D_ADD, D_SUB, D_MUL, D_NEG, D_SQR, D_CUB, D_INV, D_RED, D_MOV, D_NZ1, D_ZSEL = range(1, 12)
-DB_R, DB_E, DB_S, DB_NONE = range(4)
-SB_R, SB_X, SB_C, SB_L, SB_S, SB_A, SB_Z, SB_V = range(8)
-STEP_SLOT, INC_SLOT = 15, 8
-
-
-def decode(words):
-    out = []
-    for i in range(0, len(words), 2):
-        w0, w1 = int(words[i]), int(words[i + 1])
-        out.append((w0 & 255, (w0 >> 8) & 3, w0 >> 10, w1 & 7, (w1 >> 3) & 8191, (w1 >> 16) & 7, w1 >> 19))
-    return out
+FO_NEG, FO_CUB, FO_INV, FO_MOV, FO_ZERO = range(3, 8)
+STEP_SLOT = 15  # the model's axis list X: axis i at X[i], the step at X[15]
 
 
 def _cube(x: float) -> float:
@@ -113,47 +104,60 @@ def run_res(code, C, lc, p, nrows, state=None):
     return E
 
 
-def run_program(code, fp: bool, X: List, consts: Sequence, p: int = 0, nslots: int = 0, V=None):
-    """One step of the FP64 interpreter (two-word instructions); returns the
-    entry bank E (16 slots)."""
-    assert fp
-    R = [0.0] * 16
-    S = [0.0] * max(1, nslots)
+def run_fp(code, X, consts, dim, nslots: int = 0, V=None):
+    """One step of the FP64 interpreter (same encoding as the residue code,
+    FP64 operations): bank B holds 16 temps and the input slots (the first
+    15 - dim constants, axis i at slot 14 - i, the step at slot 15); LDC reads
+    constant slot 15 - dim + row. X: axis values then X[15] = step. Returns
+    the entry bank E (16 slots)."""
+    nc = RES_STEP - dim
+    C = [0.0] * 16
+    for i in range(min(nc, len(consts))):
+        C[i] = consts[i]
+    for ax in range(dim):
+        C[RES_STEP - 1 - ax] = X[ax]
+    C[RES_STEP] = X[STEP_SLOT]
+    B = [0.0] * RES_TEMPS + C
     E = [0.0] * 16
-    acc = 0.0
-    zero = 0.0
-
-    def fetch(bank, i):
-        if bank == SB_R:
-            return R[i & 15]
-        if bank == SB_X:
-            return X[i & 15]
-        if bank == SB_C:
-            return consts[i & 15]
-        if bank == SB_L:
-            return consts[16 + i]
-        if bank == SB_S:
-            return S[i]
-        if bank == SB_A:
-            return acc
-        if bank == SB_V:
-            return V[i & 15]
-        return zero
-
-    for op, db, d, ab, a, bb, b in code:
-        x = fetch(ab, a)
-        y = fetch(bb, b)
-        assert op not in (D_NZ1, D_ZSEL), "residue-only instruction in the FP64 code"
-        r = {D_ADD: lambda: x + y, D_SUB: lambda: x - y, D_MUL: lambda: x * y, D_NEG: lambda: -x,
-             D_SQR: lambda: x * x, D_CUB: lambda: _cube(x),
-             D_INV: lambda: (1.0 / x if abs(x) > 1e-300 else 0.0), D_RED: lambda: x, D_MOV: lambda: x}[op]()
-        if db == DB_R:
-            R[d & 15] = r
-        elif db == DB_E:
-            E[d & 15] = r
-        elif db == DB_S:
-            S[d] = r
-        acc = r
+    M = [0.0] * max(1, nslots)
+    for op, x, y, d, aux in code:
+        xv, yv = B[x], B[y]
+        if op == RO_MUL:
+            r = xv * yv
+        elif op == RO_ADD:
+            r = xv + yv
+        elif op == RO_LIN:
+            sx, sy = _sext2(aux), _sext2(aux >> 2)
+            assert (sx, sy) == (1, -1), "FP64 LIN is x - y"
+            r = xv - yv
+        elif op == FO_NEG:
+            r = -xv
+        elif op == FO_CUB:
+            r = _cube(xv)
+        elif op == FO_INV:
+            r = 1.0 / xv if abs(xv) > 1e-300 else 0.0
+        elif op == FO_MOV:
+            r = xv
+        elif op == FO_ZERO:
+            r = 0.0
+        elif op == RO_LDM:
+            r = M[aux]
+        elif op == RO_LDC:
+            r = consts[nc + aux]
+        elif op == RO_LDV:
+            r = V[aux]
+        elif op in (RO_STM, RO_STE, RO_NOP):
+            assert d == x, "a store or no-op must carry dst = x"
+            if op == RO_STM:
+                M[aux] = xv
+            elif op == RO_STE:
+                E[aux] = xv
+            r = xv
+        else:
+            raise AssertionError(f"unknown FP64 op {op}")
+        if op not in (RO_STM, RO_STE, RO_NOP):
+            assert d < RES_TEMPS, "a result written to an input slot"
+        B[d] = r
     return E
 
 
@@ -181,7 +185,7 @@ def walk(program, words, meta, const_slots, primes, axes, depth, init_res, init_
     m, dim = program.m, program.dim
     code = decode_res(words[: meta["words"]])
     fw = meta["words"] + (meta["words"] & 1)  # the FP64 code starts on an even word
-    fcode = decode(words[fw:])
+    fcode = decode_res(words[fw:fw + meta["fwords"]])
     kinds = meta["kinds"]
     K = len(primes)
     P = [[[int(init_res[i * m + j][k]) for j in range(m)] for i in range(m)] for k in range(K)]
@@ -227,7 +231,7 @@ def walk(program, words, meta, const_slots, primes, axes, depth, init_res, init_
             sn, sd, tn, td = axes[ax]
             X[ax] = float(sn * td + s * tn * sd) / float(sd * td)
         X[STEP_SLOT] = float(s)
-        E = run_program(fcode, True, X, fconsts, 0, meta["fnslots"])
+        E = run_fp(fcode, X, fconsts, dim, meta["fnslots"])
         mf = []
         for e in range(m * m):
             kd = (kinds >> (2 * e)) & 3
@@ -245,7 +249,8 @@ def walk_vec(program, words, meta, const_slots, primes, axes, depth, v0_res, v0_
 
     m, dim = program.m, program.dim
     code = decode_res(words[: meta["words"]])
-    fcode = decode(words[meta["words"] + (meta["words"] & 1):])
+    fw = meta["words"] + (meta["words"] & 1)
+    fcode = decode_res(words[fw:fw + meta["fwords"]])
     out_res = []
     for k, p in enumerate(primes):
         p = int(p)
@@ -277,7 +282,7 @@ def walk_vec(program, words, meta, const_slots, primes, axes, depth, v0_res, v0_
             sn, sd, tn, td = axes[ax]
             X[ax] = float(sn * td + s * tn * sd) / float(sd * td)
         X[STEP_SLOT] = float(s)
-        E = run_program(fcode, True, X, fconsts, 0, meta["fnslots"], vf)
+        E = run_fp(fcode, X, fconsts, dim, meta["fnslots"], vf)
         new = [E[i] for i in range(m)]
         mx = max(abs(x) for x in new)
         if not any(x != x for x in new) and mx > 1e10:

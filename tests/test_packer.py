@@ -62,7 +62,8 @@ def _res_code(words, meta):
 
 
 def _fp_code(words, meta):
-    return devmodel.decode(words[meta["words"] + (meta["words"] & 1):])
+    fw = meta["words"] + (meta["words"] & 1)
+    return devmodel.decode_res(words[fw:fw + meta["fwords"]])
 
 
 def _prog(m, dim, ops, consts, dirs=None):
@@ -191,7 +192,7 @@ def test_independent_inversions_share_one():
     assert meta["inv_batched"] == 4
     words, _, meta = N.pack_program(p)
     assert sum(1 for c in _res_code(words, meta) if c[0] == devmodel.RO_INV) == 1
-    assert sum(1 for c in _fp_code(words, meta) if c[0] == devmodel.D_INV) == 4
+    assert sum(1 for c in _fp_code(words, meta) if c[0] == devmodel.FO_INV) == 4
 
 
 def test_dependent_inversion_stays_separate():
