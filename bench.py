@@ -410,11 +410,13 @@ def workload_line(args):
         t = time.perf_counter()
         tuned = autotune(specs, budget_s=60.0)
         args.threads, args.chunk = tuned["threads"], tuned["chunk_steps"]
-        log(f"autotuned in {time.perf_counter() - t:.1f}s: threads {args.threads}, chunk {args.chunk}")
+        args.walks_per_wg, args.slice = tuned["walks_per_wg"], tuned["slice"]
+        log(f"autotuned in {time.perf_counter() - t:.1f}s: threads {args.threads}, chunk {args.chunk}, "
+            f"walks per workgroup {args.walks_per_wg}, slice {args.slice}")
     rsteps = float(sum(s.depth * s.K for s in specs))
     equiv = sum(program_work(s.program, s.vector) * s.depth * s.K for s in specs)
     plan = BatchPlan(specs, crt=True, top_k=top_k, threads=args.threads, chunk_steps=args.chunk,
-                     use_graph=args.graph, mode=args.mode)
+                     use_graph=args.graph, mode=args.mode, walks_per_wg=args.walks_per_wg, slice=args.slice)
     log(f"plan built: {time.perf_counter() - t0:.1f}s since start")
     step_ms, walk_ms = timed_steps(plan, args.steps, args.warmup, args.workload)
     copy_ms = timed_with_copy(plan, args.steps)
@@ -458,7 +460,8 @@ def workload_line(args):
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": desc, "walks": len(specs), "top_k": top_k, "graph": bool(args.graph), "mode": args.mode,
-                   "threads": args.threads, "chunk_steps": args.chunk, "autotuned": tuned is not None},
+                   "threads": args.threads, "chunk_steps": args.chunk, "walks_per_wg": args.walks_per_wg or 1,
+                   "slice": args.slice or 256, "autotuned": tuned is not None},
         "latency_batch_ms": {"p50": float(np.percentile(step_ms, 50)), "p99": float(np.percentile(step_ms, 99))},
         "with_copy_back": with_copy(rsteps, copy_ms),
         "roofline": roofline(st, walk_avg, equiv, kname, args.workload),
@@ -488,6 +491,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--mode", choices=("batched", "persistent"), default="batched")
     ap.add_argument("--graph", action="store_true", help="replay the batch as a hipGraph")
+    ap.add_argument("--walks-per-wg", type=int, default=0, help="walks per workgroup (0: one, or tuned)")
+    ap.add_argument("--slice", type=int, default=0, help="primes per task of wide walks (0: 256, or tuned)")
     ap.add_argument("--chain-cus", type=int, default=0,
                     help="CUs reserved for the FP64 M_f pass and shadow (CU-masked streams); 0 = shared")
     ap.add_argument("--tune", action="store_true", help="pcf2: autotune the launch shape first")
@@ -545,14 +550,17 @@ def main():
         t = time.perf_counter()
         tuned = autotune(specs)
         args.threads, args.chunk = tuned["threads"], tuned["chunk_steps"]
-        log(f"autotuned in {time.perf_counter() - t:.1f}s: threads {args.threads}, chunk {args.chunk}")
+        args.walks_per_wg, args.slice = tuned["walks_per_wg"], tuned["slice"]
+        log(f"autotuned in {time.perf_counter() - t:.1f}s: threads {args.threads}, chunk {args.chunk}, "
+            f"walks per workgroup {args.walks_per_wg}")
     # without --tune the line uses the shape the autotuner picks for this
     # batch (64 lanes, 4096 steps per task: the faster 256-lane walk kernel
     # crowds out the FP64 M_f / shadow chain that sets the step time), fixed
     # so that the walk kernel's time matches its committed kernel trace
     args.threads, args.chunk = args.threads or 64, args.chunk or 4096
     plan = BatchPlan(specs, crt=True, top_k=top_k, threads=args.threads, chunk_steps=args.chunk,
-                     use_graph=args.graph, mode=args.mode, chain_cus=args.chain_cus)
+                     use_graph=args.graph, mode=args.mode, chain_cus=args.chain_cus,
+                     walks_per_wg=args.walks_per_wg, slice=args.slice)
     log(f"plan built: {time.perf_counter() - t0:.1f}s since start")
     ctx = plan.ctx
     if args.profile:
@@ -629,7 +637,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": desc, "walks": len(specs), "depth": depth, "K": K, "top_k": top_k,
                    "threads": args.threads, "chunk_steps": args.chunk, "graph": bool(args.graph),
-                   "mode": args.mode, "chain_cus": args.chain_cus, "autotuned": tuned is not None},
+                   "mode": args.mode, "chain_cus": args.chain_cus, "walks_per_wg": args.walks_per_wg or 1,
+                   "slice": args.slice or 256, "autotuned": tuned is not None},
         "latency_batch_ms": {"p50": float(np.percentile(step_ms, 50)), "p99": float(np.percentile(step_ms, 99))},
         "with_copy_back": with_copy(rsteps, copy_ms),
         "plain": {"value": plain, "sample_walks": n_plain, "x": value / plain},

@@ -1,10 +1,12 @@
 // C ABI of libdreams_hip: context, primes, programs, batch plans, CRT, top-k.
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <new>
 #include <numeric>
@@ -183,14 +185,15 @@ static size_t lds_const_rows(const DevProgram& pg) {
   return pg.nconst > nc ? pg.nconst - nc : 0;
 }
 
-size_t walk_lds_bytes(const DevProgram& pg, int kind, int m, uint32_t K, int threads) {
-  const size_t tree = (size_t)threads * m * m;  // [C][m*m][KP] with C * KP = threads
+size_t walk_lds_bytes(const DevProgram& pg, int kind, int m, uint32_t K, int threads, int wpg, int slice) {
+  const size_t tree = (size_t)threads * m * m;  // [W][C][m*m][KP] with W * C * KP = threads
   if (kind != WALK_INTERP) return 8 * tree;
-  const size_t KP = pow2_ceil(std::min<uint32_t>(K, kSliceK));  // one prime slice
+  const size_t KP = pow2_ceil(std::min<uint32_t>(K, (uint32_t)slice));  // one prime slice
   const size_t nl = lds_const_rows(pg);
-  // constant and axis-increment rows by prime, spill and entry rows by
-  // thread; the tree overlays them
-  return 8 * std::max((nl + pg.dim) * KP + ((size_t)pg.nslots + (size_t)m * m) * threads, tree);
+  // constant rows by prime, axis-increment rows by lane group and prime,
+  // spill and entry rows by thread; the tree overlays them
+  return 8 * std::max((nl + (size_t)std::max(1, wpg) * pg.dim) * KP +
+                          ((size_t)pg.nslots + (size_t)m * m) * threads, tree);
 }
 
 size_t seed_lds_bytes(const DevProgram& pg, int kp, int seed_threads) {
@@ -275,8 +278,17 @@ int launch_stages(drx_ctx* ctx, const KArgs& a, const std::vector<Stage>& stages
   for (size_t gi = 0; gi < stages.size(); ++gi) {
     const Stage& g = stages[gi];
     KArgs wa = a;
-    wa.tasks += g.task_begin;
-    HIPCHK(ctx, launch_walk(g.m, g.kind, wa, g.task_end - g.task_begin, g.threads, g.lds, s1, g.persist,
+    int units = g.task_end - g.task_begin;
+    if (g.wpg > 1) {  // task indices of the group table are plan-wide
+      wa.gtask = a.gtask + (size_t)g.group_begin * g.wpg;
+      wa.wpg = g.wpg;
+      units = g.group_end - g.group_begin;
+    } else {
+      wa.tasks += g.task_begin;
+      wa.gtask = nullptr;
+      wa.wpg = 1;
+    }
+    HIPCHK(ctx, launch_walk(g.m, g.kind, wa, units, g.threads, g.lds, s1, g.persist,
                             g.persist > 0 ? a.queue + gi : nullptr));
   }
   if (timed) HIPCHK(ctx, hipEventRecord(ev[1], s1));
@@ -507,6 +519,7 @@ struct drx_plan {
       d_fdt;
   DBuf d_fout, d_rout, d_pout, d_pfull, d_limbs, d_neg, d_topk, d_topk_ws, d_rowk;
   DBuf d_queue;  // persistent mode: one task counter per stage
+  DBuf d_gtask;  // walks per workgroup: the stages' task groups
   KArgs args{};
   GarnerArgs gargs{};   // rows of at most 256 primes: the wavefront Garner kernel
   GarnerArgs wargs{};   // wider rows outside the matrix path: the wide Garner kernel
@@ -534,7 +547,7 @@ struct drx_plan {
     for (DBuf* d : {&d_walks, &d_tasks, &d_seeds, &d_spans, &d_ranges, &d_res0, &d_fst0, &d_res, &d_fst, &d_axr, &d_axi,
                     &d_axd, &d_cp, &d_mf, &d_fdt, &d_fout, &d_rout, &d_pout, &d_pfull, &d_limbs, &d_neg, &d_topk,
                     &d_topk_ws, &d_rowk, &d_vgroups, &d_fgroups, &d_vorder, &d_grows, &d_wrows, &d_crt_a,
-                    &d_crt_c, &d_crt_k, &d_queue})
+                    &d_crt_c, &d_crt_k, &d_queue, &d_gtask})
       d->release();
     for (auto& g : crt_groups) g->d_rows.release();
     for (drx_plan* p : plain) delete p;
@@ -1007,6 +1020,12 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
   const int thr_cfg = pl->cfg.threads > 0 ? pl->cfg.threads : 256;
   if ((thr_cfg & (thr_cfg - 1)) || thr_cfg < 32 || thr_cfg > 256)
     return fail(ctx, DRX_EINVAL, "threads=%d must be a power of two in [32, 256]", thr_cfg);
+  const int slice = pl->cfg.slice > 0 ? pl->cfg.slice : kSliceK;
+  if ((slice & (slice - 1)) || slice < 16 || slice > kSliceK)
+    return fail(ctx, DRX_EINVAL, "slice=%d must be a power of two in [16, %d]", slice, kSliceK);
+  const int wpg_cfg = std::max(1, pl->cfg.walks_per_wg);
+  if (wpg_cfg > 256 || (wpg_cfg & (wpg_cfg - 1)))
+    return fail(ctx, DRX_EINVAL, "walks_per_wg=%d must be a power of two up to 256", wpg_cfg);
 
   // order: by matrix order and walk-kernel family, then cost descending
   // (longest first), then the companion-PCF shape (equal shapes adjacent)
@@ -1017,9 +1036,9 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
     return (double)w.depth * (double)w.K * (1.0 + ctx->progs[w.prog].pk.code.size());
   };
   const bool interp_only = pl->cfg.interp_only != 0;
-  // walks of more than kSliceK primes run as prime slices
+  // walks of more than `slice` primes run as prime slices
   auto kind_of = [&](int i) { return walk_kind(ctx->progs[b.walks[i].prog], interp_only); };
-  auto slice_kp = [](int K) { return (int)pow2_ceil((uint32_t)std::min(K, kSliceK)); };
+  auto slice_kp = [&](int K) { return (int)pow2_ceil((uint32_t)std::min(K, slice)); };
   // interpreted walks of one order: LDS classes whose LDS size would lower
   // the occupancy of the smaller classes get stages of their own
   std::vector<int> group(n, 0);
@@ -1039,7 +1058,8 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
         if ((int)hp.dev.m != m || kind_of(i) != WALK_INTERP) continue;
         const int c = lds_class(hp, WALK_INTERP);
         present[c] = true;
-        cls_lds[c] = std::max(cls_lds[c], walk_lds_bytes(hp.dev, WALK_INTERP, m, (uint32_t)b.walks[i].K, thr));
+        cls_lds[c] = std::max(cls_lds[c], walk_lds_bytes(hp.dev, WALK_INTERP, m, (uint32_t)b.walks[i].K, thr,
+                                                         plain_mode ? 1 : std::min(wpg_cfg, thr / kp), slice));
       }
       int rep[4] = {0, 1, 2, 3};
       int cur = -1;
@@ -1169,9 +1189,9 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
         rg.s_end = std::min<int64_t>(t.s_end, r + kMfRange);
         hr.push_back(rg);
       }
-      for (int k0 = 0; k0 < K; k0 += kSliceK) {
+      for (int k0 = 0; k0 < K; k0 += slice) {
         t.k0 = k0;
-        t.kn = std::min(kSliceK, K - k0);
+        t.kn = std::min(slice, K - k0);
         ht.push_back(t);
         if (kind != WALK_INTERP && s0 == 0) hseed.push_back(t);  // Newton tables: one seed per slice
       }
@@ -1192,14 +1212,17 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
   for (auto& g : pl->stages) {
     // the walk kernel's LDS grows with the workgroup: halve it (never below
     // the widest prime stripe) until the stage fits; the sub-chunk split
-    // changes, the product does not
+    // changes, the product does not. Walks per workgroup: as asked, but
+    // every lane group at least one prime stripe wide
     for (;;) {
       size_t kp_max = 1;
+      for (int si = g.span_begin; si < g.span_end; ++si) kp_max = std::max<size_t>(kp_max, slice_kp(hw[hs[si].walk].K));
+      g.wpg = pl->cfg.mode == DRX_MODE_PLAIN ? 1 : std::max(1, std::min(wpg_cfg, g.threads / (int)kp_max));
       g.lds = 0;
       for (int si = g.span_begin; si < g.span_end; ++si) {
         const DevWalk& dw = hw[hs[si].walk];
-        g.lds = std::max(g.lds, walk_lds_bytes(ctx->progs[dw.prog].dev, g.kind, g.m, (uint32_t)dw.K, g.threads));
-        kp_max = std::max<size_t>(kp_max, slice_kp(dw.K));
+        g.lds = std::max(g.lds, walk_lds_bytes(ctx->progs[dw.prog].dev, g.kind, g.m, (uint32_t)dw.K, g.threads,
+                                               g.wpg, slice));
       }
       if (g.lds <= lds_limit || (size_t)g.threads / 2 < kp_max) break;
       g.threads /= 2;
@@ -1214,6 +1237,38 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
     const size_t need = std::max(g.lds, std::max(g.mf_lds, g.seed_lds));
     if (need > lds_limit)
       return fail(ctx, DRX_ERANGE, "program needs %zu bytes of LDS (limit %zu); use fewer threads", need, lds_limit);
+  }
+  // groups of walks per workgroup: the tasks of a stage with wpg > 1 in plan
+  // order (longest first), each appended to the open group of its key -- the
+  // interpreter's lane groups must share program and prime slice (one
+  // instruction stream and one LDS constant table per workgroup), the
+  // forward-difference ones entry kinds, reduced columns, shape and stripe
+  // width -- so a group holds tasks of about equal length; groups keep the
+  // order in which they were opened
+  std::vector<int32_t> hgroup;
+  for (Stage& g : pl->stages) {
+    if (g.wpg <= 1) continue;
+    g.group_begin = (int)(hgroup.size() / g.wpg);
+    std::map<std::array<int64_t, 4>, size_t> open;  // key -> first word of its open group
+    for (int ti = g.task_begin; ti < g.task_end; ++ti) {
+      const DevTask& t = ht[ti];
+      const DevProgram& dp = ctx->progs[hw[t.walk].prog].dev;
+      const std::array<int64_t, 4> key =
+          g.kind == WALK_INTERP
+              ? std::array<int64_t, 4>{(int64_t)hw[t.walk].prog, (int64_t)t.k0, (int64_t)t.kn, 0}
+              : std::array<int64_t, 4>{(int64_t)dp.kinds, (int64_t)dp.colred, (int64_t)t.shape,
+                                       (int64_t)pow2_ceil((uint32_t)t.kn)};
+      auto it = open.find(key);
+      if (it == open.end()) {
+        it = open.emplace(key, hgroup.size()).first;
+        hgroup.resize(hgroup.size() + g.wpg, -1);
+      }
+      size_t slot = it->second;
+      while (hgroup[slot] >= 0) ++slot;
+      hgroup[slot] = ti;
+      if (slot + 1 == it->second + g.wpg) open.erase(it);  // full
+    }
+    g.group_end = (int)(hgroup.size() / g.wpg);
   }
   if (pl->cfg.mode == DRX_MODE_PERSISTENT) {
     // each stage's walk kernel as a persistent grid of the workgroups the
@@ -1249,7 +1304,8 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
       (rc = upload(ctx, pl->d_spans, hs)) ||
       (rc = upload(ctx, pl->d_ranges, hr)) ||
       (rc = upload(ctx, pl->d_res0, res0)) || (rc = upload(ctx, pl->d_fst0, fst0)) || (rc = upload(ctx, pl->d_axr, axr)) ||
-      (rc = upload(ctx, pl->d_axi, axi)) || (rc = upload(ctx, pl->d_axd, axd)))
+      (rc = upload(ctx, pl->d_axi, axi)) || (rc = upload(ctx, pl->d_axd, axd)) ||
+      (!hgroup.empty() && (rc = upload(ctx, pl->d_gtask, hgroup))))
     return rc;
   HIPCHK(ctx, pl->d_res.ensure(std::max<size_t>(1, res0.size()) * 8));
   HIPCHK(ctx, pl->d_fst.ensure(std::max<size_t>(1, fst0.size()) * 8));
@@ -1277,6 +1333,8 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
   KArgs& a = pl->args;
   a.tasks = pl->d_tasks.as<DevTask>();
   a.queue = pl->d_queue.p ? pl->d_queue.as<uint32_t>() : nullptr;
+  a.gtask = hgroup.empty() ? nullptr : pl->d_gtask.as<int32_t>();
+  a.wpg = 1;
   a.walks = pl->d_walks.as<DevWalk>();
   a.progs = ctx->d_progs.as<DevProgram>();
   a.spans = pl->d_spans.as<DevSpan>();
@@ -2007,6 +2065,15 @@ int time_plan(drx_ctx* ctx, const drx_batch* b, const drx_launch_cfg* c, int rep
 
 }  // namespace
 
+namespace {
+// The narrowest walk of a sample (its prime stripe bounds walks per workgroup).
+int kmin_walk(const std::vector<drx_walk>& walks) {
+  int k = INT32_MAX;
+  for (const drx_walk& w : walks) k = std::min(k, w.K);
+  return walks.empty() ? 1 : k;
+}
+}  // namespace
+
 int drx_autotune(drx_ctx* ctx, const drx_batch* b, const drx_launch_cfg* base, int max_walks, double budget_s,
                  drx_launch_cfg* best, double* best_ms) {
   if (!ctx || !b || !best || !best_ms || b->n < 1 || !b->walks) return DRX_EINVAL;
@@ -2064,6 +2131,33 @@ int drx_autotune(drx_ctx* ctx, const drx_batch* b, const drx_launch_cfg* base, i
       best_t = td;
       best_cfg = d;
     }
+  }
+  // then, at that shape, walks per workgroup (lane groups of at least one
+  // prime stripe: fewer sub-chunks per walk, so shorter trees and no table
+  // jumps for one-chunk walks) and the prime stripe of wide walks
+  auto try_cfg = [&](const drx_launch_cfg& c) -> int {
+    double t = 0;
+    const int rc = time_plan(ctx, &smp->batch, &c, reps, &t);
+    if (rc == DRX_ERANGE) return DRX_OK;
+    if (rc) return rc;
+    if (t < best_t * 0.97) {  // a clear gain only
+      best_t = t;
+      best_cfg = c;
+    }
+    return DRX_OK;
+  };
+  const int kp_stripe = (int)pow2_ceil((uint32_t)std::min(kmin_walk(smp->walks), kSliceK));
+  for (int wpg = 2; wpg * kp_stripe <= best_cfg.threads && now_s() - t_start <= budget_s; wpg *= 2) {
+    drx_launch_cfg c = best_cfg;
+    c.walks_per_wg = wpg;
+    const int rc = try_cfg(c);
+    if (rc) return rc;
+  }
+  for (int sl = 64; kmax > sl && sl < kSliceK && now_s() - t_start <= budget_s; sl *= 2) {
+    drx_launch_cfg c = best_cfg;
+    c.slice = sl;
+    const int rc = try_cfg(c);
+    if (rc) return rc;
   }
   best_cfg.use_graph = base ? base->use_graph : 0;
   best_cfg.mode = base ? base->mode : DRX_MODE_BATCHED;

@@ -147,6 +147,8 @@ struct Stage {
   std::vector<int> win_range;          // [nwin + 1]: ranges of window c are win_range[c] .. win_range[c + 1]
   int threads = 0, seed_kp = 0;    // walk workgroup; seed primes per point group (pow2 >= every K)
   int persist = 0;                 // > 0: persistent walk kernel of this many workgroups (task queue)
+  int wpg = 1;                     // walks (tasks) per workgroup; > 1: groups group_begin .. group_end
+  int group_begin = 0, group_end = 0;  // of the plan's group table, wpg task indices each
   size_t lds = 0, seed_lds = 0, mf_lds = 0;
 };
 
@@ -168,7 +170,8 @@ void shadow_windows(int64_t max_steps, int64_t* win, int* nwin);
 // window of its first step) and fills Stage::win_range.
 void order_ranges(std::vector<Stage>& stages, std::vector<DevRange>& ranges, int64_t win, int nwin);
 // LDS bytes of the kernels of one walk (program `pg`, K primes) in a stage.
-size_t walk_lds_bytes(const DevProgram& pg, int kind, int m, uint32_t K, int threads);
+size_t walk_lds_bytes(const DevProgram& pg, int kind, int m, uint32_t K, int threads, int wpg = 1,
+                      int slice = kSliceK);
 size_t seed_lds_bytes(const DevProgram& pg, int kp, int seed_threads);
 size_t mf_lds_bytes(const DevProgram& pg);
 // Walk-kernel family of program `hp` (forward differences unless interp_only).

@@ -42,6 +42,13 @@ struct KArgs {
   const DevVGroup* fgroups;  // vector walks: FP64 groups (one lane per walk)
   const int32_t* vorder;     // vector walks: walk indices grouped
   uint32_t* queue;           // persistent walk kernels: one task counter per stage
+  // walks per workgroup: with wpg > 1 a walk kernel's unit of work is a
+  // group of wpg tasks, gtask[unit * wpg + g] (-1: an empty slot), lane
+  // group g of the workgroup on task g; the plan groups tasks that share the
+  // program (and prime slice) for the interpreter, the entry kinds and shape
+  // for the forward-difference kernels, and the prime stripe width
+  const int32_t* gtask;
+  int32_t wpg;
   double* axr_w;             // server admission: writable views of axr / axi / axd
   int64_t* axi_w;
   double* axd_w;

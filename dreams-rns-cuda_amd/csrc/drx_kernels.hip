@@ -187,9 +187,11 @@ struct ProgLds {
 
 // Constant slots beyond those of bank C into LDS, as residues for K primes in
 // a KP-wide stripe (row i = constant slot 15 - dim + i); the caller
-// synchronises.
+// synchronises. The axis increments follow, [W][dim][KP] by lane group and
+// prime (the walks of a workgroup's lane groups share the program, not the
+// axes).
 __device__ __forceinline__ ProgLds prog_lds(const KArgs& a, const DevProgram& pg, uint32_t K, uint32_t KP,
-                                            uint32_t k0, double* lds) {
+                                            uint32_t k0, double* lds, uint32_t g = 0) {
   const uint32_t T = blockDim.x, tid = threadIdx.x;
   const uint32_t nc = (uint32_t)kResStep - pg.dim;
   const uint32_t nl = pg.nconst > nc ? pg.nconst - nc : 0u;
@@ -197,7 +199,7 @@ __device__ __forceinline__ ProgLds prog_lds(const KArgs& a, const DevProgram& pg
   L.lc = lds;
   L.lm = L.lc + (uint64_t)nl * KP;
   L.le = L.lm + (uint64_t)pg.nslots * T;
-  L.li = L.le + (uint64_t)pg.m * pg.m * T;  // [dim][KP], by prime
+  L.li = L.le + (uint64_t)pg.m * pg.m * T + (uint64_t)g * pg.dim * KP;
   for (uint32_t i = tid; i < nl * KP; i += T) {
     const uint32_t ci = i / KP, ck = i % KP;
     L.lc[i] = (ck < K) ? a.cres[pg.cres_off + (uint64_t)(nc + ci) * pg.kcap + k0 + ck] : 0.0;
@@ -211,27 +213,56 @@ struct WalkSetup {
   DevProgram pg;
   Lane ln;
   uint32_t Kw, k0, K, KP, C, c, k, kk, kg;  // walk's K; slice k0 + [0, K); kg = k0 + kk
+  uint32_t g, Tt;                           // lane group and its lanes (Tt = blockDim / wpg)
   bool active;
   int64_t sb, se;
   const uint32_t* code;
   double *lc, *lm, *le, *li, *tree;
 };
 
-// Task, walk and lane geometry; with `interp`, the program constants in LDS.
-__device__ __forceinline__ WalkSetup walk_setup(const KArgs& a, double* lds, bool interp, int task) {
+// Task, walk and lane geometry of unit `unit` (a task, or with wpg > 1 a
+// group of tasks, one per lane group); with `interp`, the program constants
+// in LDS. What the plan keeps equal over a group -- the program of an
+// interpreted walk, the entry kinds and shape of a forward-difference one,
+// the stripe width -- is read from the first lane, so it stays wave-uniform
+// scalar data.
+__device__ __forceinline__ WalkSetup walk_setup(const KArgs& a, double* lds, bool interp, int unit) {
   WalkSetup S;
+  const uint32_t T = blockDim.x, tid = threadIdx.x;
+  const uint32_t W = a.wpg > 1 ? (uint32_t)a.wpg : 1u;
+  S.Tt = T / W;
+  S.g = tid / S.Tt;
+  const uint32_t lt = tid - S.g * S.Tt;
+  int task = unit;
+  bool slot = true;
+  if (W > 1) {
+    task = a.gtask[(uint64_t)unit * W + S.g];
+    if (task < 0) {  // an empty slot runs the group's first task, inactive
+      task = a.gtask[(uint64_t)unit * W];
+      slot = false;
+    }
+  }
   S.t = a.tasks[task];
   S.w = a.walks[S.t.walk];
-  S.pg = a.progs[S.w.prog];
-  const uint32_t T = blockDim.x, tid = threadIdx.x;
+  int prog = S.w.prog;
+  if (W > 1) {
+    prog = __builtin_amdgcn_readfirstlane(prog);
+    S.t.shape = __builtin_amdgcn_readfirstlane(S.t.shape);
+    if (interp) {  // the LDS constant table is the workgroup's: one slice of one program
+      S.t.k0 = __builtin_amdgcn_readfirstlane(S.t.k0);
+      S.t.kn = __builtin_amdgcn_readfirstlane(S.t.kn);
+    }
+  }
+  S.pg = a.progs[prog];
   S.Kw = (uint32_t)S.w.K;
   S.k0 = (uint32_t)S.t.k0;
   S.K = (uint32_t)S.t.kn;
   S.KP = pow2_at_least(S.K);
-  S.C = T / S.KP;
-  S.k = tid % S.KP;
-  S.c = tid / S.KP;
-  S.active = S.k < S.K;
+  if (W > 1) S.KP = __builtin_amdgcn_readfirstlane(S.KP);
+  S.C = S.Tt / S.KP;
+  S.k = lt % S.KP;
+  S.c = lt / S.KP;
+  S.active = slot && S.k < S.K;
   S.kk = S.active ? S.k : 0u;
   S.kg = S.k0 + S.kk;
   S.ln.p = a.prime_p[S.kg];
@@ -241,11 +272,12 @@ __device__ __forceinline__ WalkSetup walk_setup(const KArgs& a, double* lds, boo
   S.ln.tid = tid;
   S.ln.T = T;
   S.code = a.code + S.pg.code_off;
-  // the tree buffer overlays the constants and temps, which are dead once
-  // the steps are done (tree_combine synchronises before its first write)
+  // the tree buffer ([W][C][m*m][KP]) overlays the constants and temps,
+  // which are dead once the steps are done (tree_combine synchronises before
+  // its first write)
   S.tree = lds;
   if (interp) {
-    const ProgLds L = prog_lds(a, S.pg, S.K, S.KP, S.k0, lds);
+    const ProgLds L = prog_lds(a, S.pg, S.K, S.KP, S.k0, lds, S.g);
     S.lc = L.lc;
     S.lm = L.lm;
     S.le = L.le;
@@ -392,7 +424,7 @@ __device__ __forceinline__ void step_matmul_rows(double (&P)[M * M], const doubl
 template <int M>
 __device__ __forceinline__ void tree_combine(const KArgs& a, const WalkSetup& S, double (&P)[M * M]) {
   const uint32_t c = S.c, k = S.k, KP = S.KP, C = S.C;
-  double* tree = S.tree;
+  double* tree = S.tree + (uint64_t)S.g * S.Tt * M * M;
   __syncthreads();
 #pragma unroll
   for (int e = 0; e < M * M; ++e) tree[((uint64_t)c * M * M + e) * KP + k] = P[e];

@@ -66,7 +66,7 @@ class Batch(C.Structure):
 
 class LaunchCfg(C.Structure):
     _fields_ = [("mode", C.c_int), ("threads", C.c_int), ("chunk_steps", C.c_int), ("use_graph", C.c_int),
-                ("interp_only", C.c_int), ("chain_cus", C.c_int)]
+                ("interp_only", C.c_int), ("chain_cus", C.c_int), ("walks_per_wg", C.c_int), ("slice", C.c_int)]
 
 
 class ServerCfg(C.Structure):

@@ -248,9 +248,10 @@ _TUNED: Dict[tuple, Dict[str, float]] = {}
 
 def autotune(specs: Sequence[WalkSpec], ctx: Optional[N.Context] = None, max_walks: int = 16384,
              budget_s: float = 20.0, interp_only: bool = False) -> Dict[str, float]:
-    """Workgroup size and chunk length for this batch (drx_autotune: whole
+    """Launch shape for this batch -- workgroup size, chunk length, walks per
+    workgroup and the prime stripe of wide walks (drx_autotune: whole
     pipeline runs on a sample of at most ``max_walks`` walks, shrunk to fit
-    ``budget_s``), cached per batch signature."""
+    ``budget_s``) -- cached per batch signature."""
     ctx = ctx or N.context()
     key = (batch_signature(specs), bool(interp_only))
     if key in _TUNED:
@@ -267,7 +268,9 @@ def autotune(specs: Sequence[WalkSpec], ctx: Optional[N.Context] = None, max_wal
     ms = C.c_double(0.0)
     ctx._check(ctx.lib.drx_autotune(ctx.h, C.byref(b), C.byref(base), int(max_walks), float(budget_s),
                                     C.byref(best), C.byref(ms)))
-    out = {"threads": int(best.threads), "chunk_steps": int(best.chunk_steps), "sample_walk_ms": ms.value}
+    out = {"threads": int(best.threads), "chunk_steps": int(best.chunk_steps),
+           "walks_per_wg": max(1, int(best.walks_per_wg)), "slice": int(best.slice) or 256,
+           "sample_walk_ms": ms.value}
     _TUNED[key] = out
     return out
 
@@ -280,7 +283,7 @@ class BatchPlan:
     def __init__(self, specs: Sequence[WalkSpec], ctx: Optional[N.Context] = None, mode: str = "batched",
                  threads: int = 0, chunk_steps: int = 0, use_graph: bool = False, crt: bool = False,
                  full: bool = False, top_k: int = 0, interp_only: bool = False, tune: bool = False,
-                 chain_cus: int = 0):
+                 chain_cus: int = 0, walks_per_wg: int = 0, slice: int = 0):
         self.ctx = ctx or N.context()
         n = len(specs)
         if n == 0:
@@ -293,7 +296,9 @@ class BatchPlan:
         if tune and mode != "plain" and not (threads or chunk_steps):
             t = autotune(specs, ctx=ctx, interp_only=interp_only)
             threads, chunk_steps = t["threads"], t["chunk_steps"]
+            walks_per_wg, slice = t["walks_per_wg"], t["slice"]
         self.threads, self.chunk_steps = int(threads), int(chunk_steps)
+        self.walks_per_wg, self.slice = int(walks_per_wg), int(slice)
         # outputs live in page-locked memory: the copies back run at link speed
         z = N.pinned_zeros
         self.result = BatchResult(
@@ -324,7 +329,7 @@ class BatchPlan:
         b.top_k = int(top_k)
         b.topk_idx = r.topk.ctypes.data if top_k else None
         cfg = N.LaunchCfg(self.MODES[mode], int(threads), int(chunk_steps), 1 if use_graph else 0,
-                          1 if interp_only else 0, int(chain_cus))
+                          1 if interp_only else 0, int(chain_cus), int(walks_per_wg), int(slice))
         self._b, self._cfg = b, cfg
         h = C.c_void_p()
         ctx._check(ctx.lib.drx_plan_create(ctx.h, C.byref(b), C.byref(cfg), C.byref(h)))

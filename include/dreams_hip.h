@@ -172,6 +172,11 @@ typedef struct drx_launch_cfg {
   int interp_only;  /* 1: interpret every step even for polynomial programs (no forward differences) */
   int chain_cus;    /* > 0: that many CUs (spread over the chip) run only the FP64 M_f pass and shadow,
                        the walk the others; 0: every stream may use every CU. Not with use_graph. */
+  int walks_per_wg; /* > 1: a walk workgroup runs that many walks' step ranges side by side, each on
+                       threads / walks_per_wg lanes (fewer sub-chunks per walk); 0 or 1: one; capped
+                       where a walk's prime stripe would not fit; 0 = tuned */
+  int slice;        /* primes per task (the prime stripe a workgroup's lanes cover): a power of two in
+                       [16, 256]; wider walks run as slices of that many primes; 0 = 256 */
 } drx_launch_cfg;
 
 /* Run a batch of walks to completion. Replaces run_pcf_walk (runner.py:338),

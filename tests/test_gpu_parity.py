@@ -383,13 +383,15 @@ def test_autotune_choice_is_valid_and_results_unchanged(gpu_ctx):
              for w in cfg["walks"]]
     t = autotune(specs, max_walks=128, budget_s=10.0)
     assert t["threads"] in (64, 128, 256) and t["chunk_steps"] in (512, 1024, 2048, 4096, 8192, 16384)
+    assert t["walks_per_wg"] in (1, 2, 4, 8) and t["walks_per_wg"] * 32 <= t["threads"]
+    assert t["slice"] == 256  # 32-prime walks have nothing to slice
     assert t["sample_walk_ms"] > 0
     assert autotune(specs) is t  # cached by the batch signature
     with BatchPlan(specs, crt=True, top_k=8) as p0:
         r0 = p0.run()
         ref = (r0.residues.copy(), r0.limbs.copy(), r0.out["log_scale"].copy(), r0.topk.copy())
     with BatchPlan(specs, crt=True, top_k=8, tune=True) as p1:
-        assert (p1.threads, p1.chunk_steps) == (t["threads"], t["chunk_steps"])
+        assert (p1.threads, p1.chunk_steps, p1.walks_per_wg) == (t["threads"], t["chunk_steps"], t["walks_per_wg"])
         r1 = p1.run()
         assert np.array_equal(r1.residues, ref[0]) and np.array_equal(r1.limbs, ref[1])
         assert [v.hex() for v in r1.out["log_scale"].tolist()] == [v.hex() for v in ref[2].tolist()]
