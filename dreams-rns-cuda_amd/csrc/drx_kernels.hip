@@ -220,16 +220,18 @@ struct WalkSetup {
   double *lc, *lm, *le, *li, *tree;
 };
 
-// Task, walk and lane geometry of unit `unit` (a task, or with wpg > 1 a
-// group of tasks, one per lane group); with `interp`, the program constants
-// in LDS. What the plan keeps equal over a group -- the program of an
+// Task, walk and lane geometry of unit `unit` (a task, or with G and
+// wpg > 1 a group of tasks, one per lane group); with `interp`, the program
+// constants in LDS. G selects the kernels built for lane groups: the others
+// keep every task field wave-uniform scalar data. What the plan keeps equal over a group -- the program of an
 // interpreted walk, the entry kinds and shape of a forward-difference one,
 // the stripe width -- is read from the first lane, so it stays wave-uniform
 // scalar data.
+template <bool G>
 __device__ __forceinline__ WalkSetup walk_setup(const KArgs& a, double* lds, bool interp, int unit) {
   WalkSetup S;
   const uint32_t T = blockDim.x, tid = threadIdx.x;
-  const uint32_t W = a.wpg > 1 ? (uint32_t)a.wpg : 1u;
+  const uint32_t W = G && a.wpg > 1 ? (uint32_t)a.wpg : 1u;
   S.Tt = T / W;
   S.g = tid / S.Tt;
   const uint32_t lt = tid - S.g * S.Tt;
@@ -245,7 +247,7 @@ __device__ __forceinline__ WalkSetup walk_setup(const KArgs& a, double* lds, boo
   S.t = a.tasks[task];
   S.w = a.walks[S.t.walk];
   int prog = S.w.prog;
-  if (W > 1) {
+  if (G) {  // lane groups on different tasks: the shared fields from the first lane
     prog = __builtin_amdgcn_readfirstlane(prog);
     S.t.shape = __builtin_amdgcn_readfirstlane(S.t.shape);
     if (interp) {  // the LDS constant table is the workgroup's: one slice of one program
@@ -258,7 +260,7 @@ __device__ __forceinline__ WalkSetup walk_setup(const KArgs& a, double* lds, boo
   S.k0 = (uint32_t)S.t.k0;
   S.K = (uint32_t)S.t.kn;
   S.KP = pow2_at_least(S.K);
-  if (W > 1) S.KP = __builtin_amdgcn_readfirstlane(S.KP);
+  if (G) S.KP = __builtin_amdgcn_readfirstlane(S.KP);
   S.C = S.Tt / S.KP;
   S.k = lt % S.KP;
   S.c = lt / S.KP;
@@ -301,11 +303,10 @@ __device__ __forceinline__ WalkSetup walk_setup(const KArgs& a, double* lds, boo
 // column li_col (the caller synchronises before reading it). Loads go in
 // groups of 4, all in flight together (indices past the end re-read the
 // last valid one). Two ways to fill the bank, which the register allocator
-// takes differently: GATHER collects the slots in a 16-wide vector and moves
-// them into the bank in one unconditional pass (the walk kernels: 150 VGPRs
-// for 3x3 against 182 the other way, the vector walk 98 against 120);
-// otherwise the group loops are not unrolled and write the bank through
-// indexed writes in place (the seed kernel: 139 against 228).
+// takes differently: GATHER writes every slot once in straight-line code
+// (the walk kernels); otherwise the group loops are not unrolled and write
+// the bank through indexed writes in place (the seed kernel: 139 VGPRs
+// against 228 the other way).
 template <bool GATHER>
 __device__ __forceinline__ void load_rns_inputs(const KArgs& a, const DevWalk& w, const DevProgram& pg,
                                                 uint32_t K, uint32_t kk, const Lane& ln, int64_t s0, RBank& B,
@@ -313,36 +314,29 @@ __device__ __forceinline__ void load_rns_inputs(const KArgs& a, const DevWalk& w
   const double p = ln.p, pinv = ln.pinv;
   constexpr int G = 4;
   if (GATHER) {
-    const uint32_t nc = (uint32_t)kResStep - pg.dim;
-    v16d C = 0.0;
+    // every slot in straight-line code: each load from an address that is
+    // valid whether or not the slot is used (an unused slot reads the prime
+    // table and keeps zero), each value selected and written unconditionally
+    const uint32_t nc = min((uint32_t)kResStep - pg.dim, pg.nconst), dim = pg.dim;
+    const double* cbase = a.cres + pg.cres_off + kk;
+    const double* abase = a.axr + w.axr_off + kk;
+    const double* safe = a.prime_p + kk;
 #pragma unroll
-    for (int i = 0; i < kResStep; ++i)
-      if ((uint32_t)i < nc && (uint32_t)i < pg.nconst) C[i] = a.cres[pg.cres_off + (uint64_t)i * pg.kcap + kk];
-#pragma unroll
-    for (int g0 = 0; g0 < kMaxDim; g0 += G) {
-      if ((uint32_t)g0 >= pg.dim) break;  // uniform
-      double c[G][3];
-#pragma unroll
-      for (int i = 0; i < G; ++i) {
-        const uint32_t ax = min((uint32_t)(g0 + i), pg.dim - 1u);
-        const double* q = a.axr + w.axr_off + (uint64_t)ax * 3 * K;
-        c[i][0] = q[kk];
-        c[i][1] = q[K + kk];
-        c[i][2] = q[2 * K + kk];
-      }
-#pragma unroll
-      for (int i = 0; i < G; ++i) {
-        const int ax = g0 + i;
-        if (ax < kMaxDim && (uint32_t)ax < pg.dim) {
-          const double x0 = red(c[i][0] + mm((double)s0, c[i][1], p, pinv), p, pinv);
-          C[kResStep - 1 - ax] = mm(x0, c[i][2], p, pinv);
-          if (li) li[ax * li_ld + li_col] = mm(c[i][1], c[i][2], p, pinv);
-        }
+    for (int j = 0; j < kResStep; ++j) {
+      const int ax = kResStep - 1 - j;  // the axis of slot j, when ax < dim
+      if (ax < kMaxDim && (uint32_t)ax < dim) {  // uniform
+        const double* q = abase + (uint64_t)ax * 3 * K;
+        const double c0 = q[0], c1 = q[K], c2 = q[2 * K];
+        const double x0 = red(c0 + mm((double)s0, c1, p, pinv), p, pinv);
+        B.set(kResCBase + j, mm(x0, c2, p, pinv));
+        if (li) li[ax * li_ld + li_col] = mm(c1, c2, p, pinv);
+      } else {
+        const bool use = (uint32_t)j < nc;  // uniform
+        const double v = *(use ? cbase + (uint64_t)j * pg.kcap : safe);
+        B.set(kResCBase + j, use ? v : 0.0);
       }
     }
-    C[kResStep] = (double)s0;
-#pragma unroll
-    for (int j = 0; j <= kResStep; ++j) B.set(kResCBase + j, C[j]);
+    B.set(kResCBase + kResStep, (double)s0);
     return;
   }
   const uint32_t nc = min((uint32_t)kResStep - pg.dim, pg.nconst);
@@ -390,17 +384,20 @@ __device__ __forceinline__ void advance_axes(RBank& B, const double* __restrict_
   B.set(kResCBase + kResStep, B.get(kResCBase + kResStep) + 1.0);
 }
 
-// P <- P * M with M's entries in the lane's LDS entry rows (each read once),
-// one row of P at a time; `kinds` marks the statically zero / one entries
-// (uniform branches).
+// P <- P * M with M's entries in the lane's LDS entry rows, one row of P at
+// a time; `kinds` marks the statically zero / one entries (uniform
+// branches). Each row reads the entries from LDS again (the compiler barrier
+// keeps it from holding all M*M entries in registers across the rows: for
+// 4x4 that is 32 VGPRs, and the kernel's waves per SIMD depend on them).
 template <int M>
 __device__ __forceinline__ void step_matmul_rows(double (&P)[M * M], const double* __restrict__ le, uint32_t kinds,
                                                  uint32_t colred, const Lane& ln) {
-  double E[M * M];
-#pragma unroll
-  for (int e = 0; e < M * M; ++e) E[e] = (((kinds >> (2 * e)) & 3u) == E_GEN) ? le[e * ln.T + ln.tid] : 0.0;
 #pragma unroll
   for (int i = 0; i < M; ++i) {
+    if (i > 0) __asm__ volatile("" ::: "memory");
+    double E[M * M];
+#pragma unroll
+    for (int e = 0; e < M * M; ++e) E[e] = (((kinds >> (2 * e)) & 3u) == E_GEN) ? le[e * ln.T + ln.tid] : 0.0;
     double row[M];
 #pragma unroll
     for (int j = 0; j < M; ++j) {
@@ -473,13 +470,11 @@ __device__ __forceinline__ void for_tasks(int ntask, uint32_t* queue, int quota,
 }
 
 // Interpreting walk: the program runs once per step and residue channel.
-template <int M>
+template <int M, bool G>
 __device__ __forceinline__ void walk_chunk_task(const KArgs& a, double* lds, int task) {
-  const WalkSetup S = walk_setup(a, lds, true, task);
+  const WalkSetup S = walk_setup<G>(a, lds, true, task);
   const double p = S.ln.p, pinv = S.ln.pinv;
   RBank B;
-  B.lo = 0u;
-  B.hi = 0u;
   load_rns_inputs<true>(a, S.w, S.pg, S.Kw, S.kg, S.ln, S.sb, B, S.c == 0 ? S.li : nullptr, S.KP, S.k);
   __syncthreads();
   double P[M * M];
@@ -493,10 +488,17 @@ __device__ __forceinline__ void walk_chunk_task(const KArgs& a, double* lds, int
   tree_combine<M>(a, S, P);
 }
 
-template <int M, bool PERSIST>
-__global__ void __launch_bounds__(256) k_walk_chunk(const KArgs a, int ntask, uint32_t* queue, int quota) {
+// Waves per SIMD asked of the compiler: the grid kernels for 3x3 fit 4
+// (128 VGPRs, one 8-byte spill in the set-up, none in the step loop), 2x2
+// more and 4x4 3 by themselves. The persistent kernels keep their own
+// count (at 4 waves their step loop would spill).
+constexpr int chunk_waves(int m, bool persist) { return persist ? 1 : (m == 4 ? 3 : 4); }
+
+template <int M, bool PERSIST, bool G>
+__global__ void __launch_bounds__(256, chunk_waves(M, PERSIST)) k_walk_chunk(const KArgs a, int ntask,
+                                                                              uint32_t* queue, int quota) {
   extern __shared__ double lds[];
-  for_tasks<PERSIST>(ntask, queue, quota, [&](int task) { walk_chunk_task<M>(a, lds, task); });
+  for_tasks<PERSIST>(ntask, queue, quota, [&](int task) { walk_chunk_task<M, G>(a, lds, task); });
 }
 
 // ------------------------------------------------ forward-difference walks
@@ -670,18 +672,19 @@ __device__ __forceinline__ void pcf_fd_body(const KArgs& a, const WalkSetup& S) 
 
 // Companion-PCF walks; the task's shape (8 deg b + deg a) picks a fully
 // unrolled body (the shape is uniform over the workgroup).
+template <bool G>
 __device__ __forceinline__ void walk_fd_pcf_task(const KArgs& a, double* lds, int task) {
-  const WalkSetup S = walk_setup(a, lds, false, task);
+  const WalkSetup S = walk_setup<G>(a, lds, false, task);
   switch (S.t.shape) {
     DRX_PCF_ROW(0) DRX_PCF_ROW(1) DRX_PCF_ROW(2) DRX_PCF_ROW(3) DRX_PCF_ROW(4) DRX_PCF_ROW(5) DRX_PCF_ROW(6)
     default: break;
   }
 }
 
-template <bool PERSIST>
+template <bool PERSIST, bool G>
 __global__ void __launch_bounds__(256) k_walk_fd_pcf(const KArgs a, int ntask, uint32_t* queue, int quota) {
   extern __shared__ double lds[];
-  for_tasks<PERSIST>(ntask, queue, quota, [&](int task) { walk_fd_pcf_task(a, lds, task); });
+  for_tasks<PERSIST>(ntask, queue, quota, [&](int task) { walk_fd_pcf_task<G>(a, lds, task); });
 }
 #undef DRX_PCF_ROW
 #undef DRX_PCF_CASE
@@ -689,11 +692,11 @@ __global__ void __launch_bounds__(256) k_walk_fd_pcf(const KArgs a, int ntask, u
 // Any other polynomial program: entry kinds are runtime (uniform) values and
 // every general entry's table is advanced at the full depth D (the Newton
 // differences above an entry's degree are zero).
-template <int M>
+template <int M, bool G>
 __device__ __forceinline__ void walk_fd_task(const KArgs& a, double* lds, int task) {
   constexpr int D = fd_dmax(M);
   constexpr int U = fd_unroll(D);
-  const WalkSetup S = walk_setup(a, lds, false, task);
+  const WalkSetup S = walk_setup<G>(a, lds, false, task);
   const double p = S.ln.p, pinv = S.ln.pinv;
   const uint32_t kinds = S.pg.kinds;
   double N[M * M][D + 1];
@@ -725,10 +728,10 @@ __device__ __forceinline__ void walk_fd_task(const KArgs& a, double* lds, int ta
   tree_combine<M>(a, S, P);
 }
 
-template <int M, bool PERSIST>
+template <int M, bool PERSIST, bool G>
 __global__ void __launch_bounds__(256) k_walk_fd(const KArgs a, int ntask, uint32_t* queue, int quota) {
   extern __shared__ double lds[];
-  for_tasks<PERSIST>(ntask, queue, quota, [&](int task) { walk_fd_task<M>(a, lds, task); });
+  for_tasks<PERSIST>(ntask, queue, quota, [&](int task) { walk_fd_task<M, G>(a, lds, task); });
 }
 
 // Companion-shaped programs (ones on the sub-diagonal, zeros elsewhere but
@@ -795,9 +798,9 @@ __device__ __forceinline__ void comp_fd_body(const KArgs& a, const WalkSetup& S)
 // M - 1 only (the walk's one task from the identity, no full state asked),
 // which the kernel uses when the task is one sub-chunk (C == 1): half the
 // products for a 4x4 companion.
-template <int M>
+template <int M, bool G>
 __device__ __forceinline__ void walk_fd_comp_task(const KArgs& a, double* lds, int task) {
-  const WalkSetup S = walk_setup(a, lds, false, task);
+  const WalkSetup S = walk_setup<G>(a, lds, false, task);
   const bool rows = (S.t.shape & kShapeRows) != 0 && S.C == 1;  // uniform
   switch ((S.t.shape & (kShapeRows - 1)) * 2 + (rows ? 1 : 0)) {
     case 0: comp_fd_body<M, 0, false>(a, S); break;
@@ -812,10 +815,10 @@ __device__ __forceinline__ void walk_fd_comp_task(const KArgs& a, double* lds, i
   }
 }
 
-template <int M, bool PERSIST>
+template <int M, bool PERSIST, bool G>
 __global__ void __launch_bounds__(256) k_walk_fd_comp(const KArgs a, int ntask, uint32_t* queue, int quota) {
   extern __shared__ double lds[];
-  for_tasks<PERSIST>(ntask, queue, quota, [&](int task) { walk_fd_comp_task<M>(a, lds, task); });
+  for_tasks<PERSIST>(ntask, queue, quota, [&](int task) { walk_fd_comp_task<M, G>(a, lds, task); });
 }
 
 // --------------------------------------------------------- ordered combine
@@ -1313,8 +1316,6 @@ __global__ void __launch_bounds__(256) k_walk_vec(const KArgs a) {
   double* sb = sa + (uint64_t)m * T;
   double* li = sb + (uint64_t)m * T;
   RBank B;
-  B.lo = 0u;
-  B.hi = 0u;
   load_rns_inputs<true>(a, w, pg, K, kk, ln, 0, B, li, T, tid);
 #pragma unroll 1
   for (uint32_t j = 0; j < m; ++j) sa[j * T + tid] = a.res_in[w.res_off + (uint64_t)j * K + kk];
@@ -1906,25 +1907,25 @@ __global__ void k_topk_rank(const double* __restrict__ d, int stride, int n, int
 }
 
 // ---------------------------------------------------------------- launchers
-template <bool P>
+template <bool P, bool G>
 static hipError_t launch_walk_t(int m, int kind, const KArgs& a, int ntask, dim3 grid, int threads, size_t lds,
                                 hipStream_t s, uint32_t* queue, int quota) {
   const dim3 block(threads);
   if (kind == WALK_FD_PCF) {
     if (m != 2) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_walk_fd_pcf<P>), grid, block, lds, s, a, ntask, queue, quota);
+    hipLaunchKernelGGL((k_walk_fd_pcf<P, G>), grid, block, lds, s, a, ntask, queue, quota);
   } else if (kind == WALK_FD) {
-    if (m == 2) hipLaunchKernelGGL((k_walk_fd<2, P>), grid, block, lds, s, a, ntask, queue, quota);
-    else if (m == 3) hipLaunchKernelGGL((k_walk_fd<3, P>), grid, block, lds, s, a, ntask, queue, quota);
+    if (m == 2) hipLaunchKernelGGL((k_walk_fd<2, P, G>), grid, block, lds, s, a, ntask, queue, quota);
+    else if (m == 3) hipLaunchKernelGGL((k_walk_fd<3, P, G>), grid, block, lds, s, a, ntask, queue, quota);
     else return hipErrorInvalidValue;
   } else if (kind == WALK_FD_COMP) {
-    if (m == 3) hipLaunchKernelGGL((k_walk_fd_comp<3, P>), grid, block, lds, s, a, ntask, queue, quota);
-    else if (m == 4) hipLaunchKernelGGL((k_walk_fd_comp<4, P>), grid, block, lds, s, a, ntask, queue, quota);
+    if (m == 3) hipLaunchKernelGGL((k_walk_fd_comp<3, P, G>), grid, block, lds, s, a, ntask, queue, quota);
+    else if (m == 4) hipLaunchKernelGGL((k_walk_fd_comp<4, P, G>), grid, block, lds, s, a, ntask, queue, quota);
     else return hipErrorInvalidValue;
   } else {
-    if (m == 2) hipLaunchKernelGGL((k_walk_chunk<2, P>), grid, block, lds, s, a, ntask, queue, quota);
-    else if (m == 3) hipLaunchKernelGGL((k_walk_chunk<3, P>), grid, block, lds, s, a, ntask, queue, quota);
-    else if (m == 4) hipLaunchKernelGGL((k_walk_chunk<4, P>), grid, block, lds, s, a, ntask, queue, quota);
+    if (m == 2) hipLaunchKernelGGL((k_walk_chunk<2, P, G>), grid, block, lds, s, a, ntask, queue, quota);
+    else if (m == 3) hipLaunchKernelGGL((k_walk_chunk<3, P, G>), grid, block, lds, s, a, ntask, queue, quota);
+    else if (m == 4) hipLaunchKernelGGL((k_walk_chunk<4, P, G>), grid, block, lds, s, a, ntask, queue, quota);
     else return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -1937,9 +1938,11 @@ hipError_t launch_walk(int m, int kind, const KArgs& a, int ntask, int threads, 
     if (!queue) return hipErrorInvalidValue;
     const int grid = std::min(persist_blocks, ntask);
     const int quota = (ntask + grid - 1) / grid;
-    return launch_walk_t<true>(m, kind, a, ntask, dim3(grid), threads, lds, s, queue, quota);
+    return a.wpg > 1 ? launch_walk_t<true, true>(m, kind, a, ntask, dim3(grid), threads, lds, s, queue, quota)
+                     : launch_walk_t<true, false>(m, kind, a, ntask, dim3(grid), threads, lds, s, queue, quota);
   }
-  return launch_walk_t<false>(m, kind, a, ntask, dim3(ntask), threads, lds, s, nullptr, 1);
+  return a.wpg > 1 ? launch_walk_t<false, true>(m, kind, a, ntask, dim3(ntask), threads, lds, s, nullptr, 1)
+                   : launch_walk_t<false, false>(m, kind, a, ntask, dim3(ntask), threads, lds, s, nullptr, 1);
 }
 
 hipError_t launch_vec_walk(const KArgs& a, int ngroup, int threads, size_t lds, hipStream_t s) {
@@ -1961,17 +1964,17 @@ static int walk_blocks_per_cu_t(int m, int kind, int threads, size_t lds) {
   int blocks = 0;
   hipError_t e = hipErrorInvalidValue;
   if (kind == WALK_INTERP) {
-    if (m == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_walk_chunk<2, P>, threads, lds);
-    else if (m == 3) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_walk_chunk<3, P>, threads, lds);
-    else if (m == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_walk_chunk<4, P>, threads, lds);
+    if (m == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_walk_chunk<2, P, false>, threads, lds);
+    else if (m == 3) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_walk_chunk<3, P, false>, threads, lds);
+    else if (m == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_walk_chunk<4, P, false>, threads, lds);
   } else if (kind == WALK_FD_PCF && m == 2) {
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_walk_fd_pcf<P>, threads, lds);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_walk_fd_pcf<P, false>, threads, lds);
   } else if (kind == WALK_FD) {
-    if (m == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_walk_fd<2, P>, threads, lds);
-    else if (m == 3) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_walk_fd<3, P>, threads, lds);
+    if (m == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_walk_fd<2, P, false>, threads, lds);
+    else if (m == 3) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_walk_fd<3, P, false>, threads, lds);
   } else if (kind == WALK_FD_COMP) {
-    if (m == 3) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_walk_fd_comp<3, P>, threads, lds);
-    else if (m == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_walk_fd_comp<4, P>, threads, lds);
+    if (m == 3) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_walk_fd_comp<3, P, false>, threads, lds);
+    else if (m == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_walk_fd_comp<4, P, false>, threads, lds);
   }
   return e == hipSuccess ? blocks : -1;
 }
