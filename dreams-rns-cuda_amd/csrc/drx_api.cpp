@@ -230,7 +230,7 @@ void order_ranges(std::vector<Stage>& stages, std::vector<DevRange>& ranges, int
 }
 
 int launch_stages(drx_ctx* ctx, const KArgs& a, const std::vector<Stage>& stages, const DevTask* seed_tasks,
-                  int finalize, bool timed, int64_t win, int nwin) {
+                  int finalize, bool timed, int64_t win, int nwin, bool mf_first) {
   hipStream_t s1 = ctx->stream, s2 = ctx->stream2, s3 = ctx->stream3;
   hipEvent_t* ev = ctx->ev;
   if (nwin < 1 || nwin > kMaxWin) return fail(ctx, DRX_EINVAL, "bad shadow window count %d", nwin);
@@ -246,14 +246,15 @@ int launch_stages(drx_ctx* ctx, const KArgs& a, const std::vector<Stage>& stages
     HIPCHK(ctx, launch_fd_seed(g.m, fa, g.seed_end - g.seed_begin, g.seed_kp, fd_seed_threads(g.m, g.seed_kp),
                                g.seed_lds, s1));
   }
+  hipStream_t smf = mf_first ? s1 : s2;
   auto mf_window = [&](int c) -> int {
     for (const Stage& g : stages) {
       const int r0 = g.win_range[c], r1 = g.win_range[c + 1];
       KArgs ma = a;
       ma.ranges += r0;
-      HIPCHK(ctx, launch_mf(g.m, ma, r1 - r0, g.mf_lds, s2));
+      HIPCHK(ctx, launch_mf(g.m, ma, r1 - r0, g.mf_lds, smf));
     }
-    HIPCHK(ctx, hipEventRecord(ev[kEvWin + c], s2));
+    HIPCHK(ctx, hipEventRecord(ev[kEvWin + c], smf));
     return DRX_OK;
   };
   auto shadow_window = [&](int c) -> int {
@@ -266,8 +267,9 @@ int launch_stages(drx_ctx* ctx, const KArgs& a, const std::vector<Stage>& stages
     }
     return DRX_OK;
   };
-  int rc = mf_window(0);
-  if (rc) return rc;
+  int rc = DRX_OK;
+  for (int c = 0; c < (mf_first ? nwin : 1); ++c)
+    if ((rc = mf_window(c))) return rc;
   bool persist = false;
   for (const Stage& g : stages) persist = persist || g.persist > 0;
   if (persist) {
@@ -293,10 +295,10 @@ int launch_stages(drx_ctx* ctx, const KArgs& a, const std::vector<Stage>& stages
   }
   if (timed) HIPCHK(ctx, hipEventRecord(ev[1], s1));
   for (int c = 0; c < nwin; ++c) {
-    if (c + 1 < nwin && (rc = mf_window(c + 1))) return rc;
+    if (!mf_first && c + 1 < nwin && (rc = mf_window(c + 1))) return rc;
     if ((rc = shadow_window(c))) return rc;
   }
-  // stream2 ends before stream3's last wait, so joining stream3 joins both
+  // the M_f stream ends before stream3's last wait, so joining stream3 joins both
   HIPCHK(ctx, hipEventRecord(ev[kEvJoin], s3));
   for (const Stage& g : stages) {
     KArgs ca = a;
@@ -520,6 +522,7 @@ struct drx_plan {
   DBuf d_fout, d_rout, d_pout, d_pfull, d_limbs, d_neg, d_topk, d_topk_ws, d_rowk;
   DBuf d_queue;  // persistent mode: one task counter per stage
   DBuf d_gtask;  // walks per workgroup: the stages' task groups
+  bool mf_first = false;  // the M_f pass before the walk (see launch_stages)
   KArgs args{};
   GarnerArgs gargs{};   // rows of at most 256 primes: the wavefront Garner kernel
   GarnerArgs wargs{};   // wider rows outside the matrix path: the wide Garner kernel
@@ -737,7 +740,8 @@ int launch_sequence(drx_plan* pl, bool timed) {
   if (pl->vec) return launch_vec_sequence(pl, timed);
   drx_ctx* ctx = pl->ctx;
   hipStream_t s = ctx->stream;
-  int rc = launch_stages(ctx, pl->args, pl->stages, pl->d_seeds.as<DevTask>(), 1, timed, pl->win, pl->nwin);
+  int rc = launch_stages(ctx, pl->args, pl->stages, pl->d_seeds.as<DevTask>(), 1, timed, pl->win, pl->nwin,
+                         pl->mf_first);
   if (rc) return rc;
   // residue outputs and CRT need only the combined residues: they run while
   // the shadow chain is still going; the final P_f copy needs the shadow
@@ -1294,6 +1298,21 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
     shadow_windows(maxd, &pl->win, &pl->nwin);
     order_ranges(pl->stages, hr, pl->win, pl->nwin);
   }
+  {
+    // the M_f pass first when the walk work dwarfs it: lane instructions of
+    // the walk (about 12 per packed residue instruction, 6 per modular
+    // multiply of a forward-difference step) against those of the M_f pass
+    // (about 12 per FP64 instruction, one lane per step)
+    double walk_li = 0, mf_li = 0;
+    for (int i = 0; i < n; ++i) {
+      const drx_walk& w = b.walks[i];
+      const HostProgram& hp = ctx->progs[w.prog];
+      const double per = kind_of(i) == WALK_INTERP ? 12.0 * (double)hp.pk.code.size() : 6.0 * hp.mm_ops;
+      walk_li += (double)w.depth * w.K * per;
+      mf_li += (double)w.depth * 12.0 * (double)hp.pk.fcode.size();
+    }
+    pl->mf_first = pl->cfg.mode != DRX_MODE_PLAIN && walk_li >= 16.0 * mf_li;
+  }
   pl->ntask = (int)ht.size();
   pl->residue_steps = rsteps;
   pl->mm_total = mmt;
@@ -1702,7 +1721,7 @@ int drx_program_upload_mode(drx_ctx* ctx, const uint32_t* ops, int n_ops, int m,
     else if (op == RO_INV) mmo += 60;
     else if (op == RO_NZ1 || op == RO_ZSEL || op == RO_RED) mmo += 1;  // one reduction and a select
     else if (op == RO_MULNZ) mmo += 2;
-    else if (op == RO_LIN) mao += 1;
+    else if (op == RO_ADD || op == RO_LIN) mao += 1;
   }
   // the step product (a vector walk's product is part of its code)
   for (int j = 0; j < m && mode == DRX_WALK_MATRIX; ++j) {

@@ -160,8 +160,13 @@ struct Stage {
 // `timed`, events 0..3 mark walk start, walk end, combine end, join.
 // The FP64 chain runs in `nwin` windows of `win` steps: the M_f pass of
 // window c on stream2, the shadow of window c on stream3 behind it.
+// With `mf_first` the whole M_f pass runs on `stream` before the walk
+// kernels, with the device to itself, and the shadow windows follow it on
+// stream3 while the walk runs: for batches whose walk work dwarfs the M_f
+// pass (interpreted walks), where a pass interleaved with the walk would be
+// starved of workgroup slots and end long after it.
 int launch_stages(drx_ctx* ctx, const KArgs& a, const std::vector<Stage>& stages, const DevTask* seed_tasks,
-                  int finalize, bool timed, int64_t win, int nwin);
+                  int finalize, bool timed, int64_t win, int nwin, bool mf_first = false);
 int join_stages(drx_ctx* ctx, bool timed);
 // Window length (a multiple of kMfRange) and count for walks of up to
 // `max_steps` steps: at most kMaxWin windows of at least two M_f ranges.

@@ -977,9 +977,11 @@ __global__ void __launch_bounds__(256) k_mf(const KArgs a) {
 // P_f <- P_f @ M_f(s) for every step in order (runner.py:388-405), each entry
 // a sequential FMA chain acc = a0*b0; acc = fma(ak, bk, acc); then
 // mx = max|P_f| (NaN propagates) and, if mx > 1e10, P_f /= mx and log_scale
-// grows by ln mx. For 3x3 and 4x4, 16 lanes per walk, lane (i, j) holding
-// entry P_f[i][j]: rows come from the other lanes of the group (ds_bpermute)
-// and the maximum is a butterfly over the group; 2x2 walks take k_shadow2.
+// grows by ln mx. For 3x3 and 4x4, one 16-lane DPP row per walk: quad i holds
+// row i of P_f, lane j of the quad entry (i, j) (3x3 leaves lane 3 of each
+// quad and quad 3 idle). A lane takes its row from its own quad by DPP
+// broadcasts and the maximum (and the NaN test) crosses the row by four DPP
+// steps, so the chain holds no LDS round trip; 2x2 walks take k_shadow2.
 // log_scale is carried as a mantissa product and a binary exponent, both
 // exact in step order, and ln is taken once when it is read.
 template <int CTRL>
@@ -991,24 +993,29 @@ __device__ __forceinline__ double dpp_f64(double x) {
 }
 // quad_perm selector (lane l of a quad reads lane sel[l]): the partner at xor 1
 constexpr int kQuadX1 = 0xB1;  // [1, 0, 3, 2]
+constexpr int kQuadX2 = 0x4E;  // [2, 3, 0, 1]
+constexpr int kRowHalfMirror = 0x141;  // lane l of each 8 reads lane 7 - l
+constexpr int kRowMirror = 0x140;      // lane l of the 16 reads lane 15 - l
 
-template <int M>
-__device__ __forceinline__ double group_row(double v, int gbase, int i, int q) {
-  return __shfl(v, gbase + i * M + q, 64);
+// Entry q of the lane's row: lane q of its quad, by a DPP broadcast.
+template <int Q>
+__device__ __forceinline__ double quad_lane(double v) {
+  return dpp_f64<Q * 0x55>(v);
 }
 
-template <int M>
-__device__ __forceinline__ double group_max(double v) {
-#pragma unroll
-  for (int d = 1; d < 16; d <<= 1) v = fmax(v, __shfl_xor(v, d, 64));
-  return v;
+// The maximum over the 16-lane row (every lane ends with it).
+__device__ __forceinline__ double row16_max(double v) {
+  v = fmax(v, dpp_f64<kQuadX1>(v));
+  v = fmax(v, dpp_f64<kQuadX2>(v));
+  v = fmax(v, dpp_f64<kRowHalfMirror>(v));
+  return fmax(v, dpp_f64<kRowMirror>(v));
 }
 
-template <int M>
-__device__ __forceinline__ int group_or(int v) {
-#pragma unroll
-  for (int d = 1; d < 16; d <<= 1) v |= __shfl_xor(v, d, 64);
-  return v;
+__device__ __forceinline__ int row16_or(int v) {
+  v |= __builtin_amdgcn_mov_dpp(v, kQuadX1, 0xF, 0xF, false);
+  v |= __builtin_amdgcn_mov_dpp(v, kQuadX2, 0xF, 0xF, false);
+  v |= __builtin_amdgcn_mov_dpp(v, kRowHalfMirror, 0xF, 0xF, false);
+  return v | __builtin_amdgcn_mov_dpp(v, kRowMirror, 0xF, 0xF, false);
 }
 
 // Shadow state of one lane: its entry of P_f, and log_scale as the mantissa
@@ -1020,13 +1027,17 @@ struct ShadowLane {
 };
 
 template <int M, bool MASK>
-__device__ __forceinline__ void shadow_step(ShadowLane& L, const double (&col)[M], int gbase, int ei, bool ent,
-                                            bool in) {
-  double acc = group_row<M>(L.Pm, gbase, ei, 0) * col[0];
+__device__ __forceinline__ void shadow_step(ShadowLane& L, const double (&col)[M], bool ent, bool in) {
+  double row[M];
+  row[0] = quad_lane<0>(L.Pm);
+  row[1] = quad_lane<1>(L.Pm);
+  row[2] = quad_lane<2>(L.Pm);
+  if constexpr (M == 4) row[3] = quad_lane<3>(L.Pm);
+  double acc = row[0] * col[0];
 #pragma unroll
-  for (int q = 1; q < M; ++q) acc = __builtin_fma(group_row<M>(L.Pm, gbase, ei, q), col[q], acc);
-  const double mx = group_max<M>(ent ? __builtin_fabs(acc) : 0.0);
-  const int anynan = group_or<M>((ent && acc != acc) ? 1 : 0);
+  for (int q = 1; q < M; ++q) acc = __builtin_fma(row[q], col[q], acc);
+  const double mx = row16_max(ent ? __builtin_fabs(acc) : 0.0);
+  const int anynan = row16_or((ent && acc != acc) ? 1 : 0);
   const bool resc = anynan == 0 && mx > 1e10;
   const double np = ent ? (resc ? acc / mx : acc) : 0.0;
   int e1 = 0;
@@ -1054,13 +1065,13 @@ __device__ __forceinline__ void shadow_tile(double (&t)[kShadowTile][M], const d
 
 template <int M, bool MASK>
 __device__ __forceinline__ void shadow_run_tile(ShadowLane& L, const double (&t)[kShadowTile][M], int64_t s0,
-                                                int64_t nsteps, int gbase, int ei, bool ent) {
+                                                int64_t nsteps, bool ent) {
 #pragma unroll
   for (int u = 0; u < kShadowTile; ++u) {
     double col[M];
 #pragma unroll
     for (int q = 0; q < M; ++q) col[q] = t[u][q];
-    shadow_step<M, MASK>(L, col, gbase, ei, ent, s0 + u < nsteps);
+    shadow_step<M, MASK>(L, col, ent, s0 + u < nsteps);
   }
 }
 
@@ -1077,12 +1088,14 @@ __global__ void __launch_bounds__(256) k_shadow(const KArgs a, int nspan, int fi
   const bool live = gid < nspan;
   const DevSpan sp = a.spans[live ? gid : nspan - 1];
   const DevWalk w = a.walks[sp.walk];
-  const bool ent = lg < M * M;
-  const int ei = ent ? lg / M : 0, ej = ent ? lg % M : 0;
+  const int qi = lg >> 2, qj = lg & 3;  // quad = row, lane of the quad = column
+  const bool ent = qi < M && qj < M;
+  const int ej = ent ? qj : 0;
+  const int e = ent ? qi * M + qj : 0;  // the entry's index in the stored state
   const int gbase = (int)(threadIdx.x & 63) - lg;
   const double* st_in = (w0 == 0 ? a.fst_in : a.fst) + w.fst_off;
   ShadowLane L;
-  L.Pm = ent ? st_in[lg] : 0.0;
+  L.Pm = ent ? st_in[e] : 0.0;
   {
     const double m0 = st_in[M * M];
     int e0 = 0;
@@ -1107,31 +1120,32 @@ __global__ void __launch_bounds__(256) k_shadow(const KArgs a, int nspan, int fi
   int64_t s = 0;
   for (; s + 2 * T <= nsteps; s += 2 * T) {
     shadow_tile<M>(tb, mf, s + T, last, ej);
-    shadow_run_tile<M, false>(L, ta, s, nsteps, gbase, ei, ent);
+    shadow_run_tile<M, false>(L, ta, s, nsteps, ent);
     shadow_tile<M>(ta, mf, s + 2 * T, last, ej);
-    shadow_run_tile<M, false>(L, tb, s + T, nsteps, gbase, ei, ent);
+    shadow_run_tile<M, false>(L, tb, s + T, nsteps, ent);
     int e2 = 0;
     L.mant = __builtin_frexp(L.mant, &e2);
     L.ex += e2;
   }
   if (s < nsteps) {  // last partial pair of tiles, step by step masked
     shadow_tile<M>(tb, mf, s + T, last, ej);
-    shadow_run_tile<M, true>(L, ta, s, nsteps, gbase, ei, ent);
-    shadow_run_tile<M, true>(L, tb, s + T, nsteps, gbase, ei, ent);
+    shadow_run_tile<M, true>(L, ta, s, nsteps, ent);
+    shadow_run_tile<M, true>(L, tb, s + T, nsteps, ent);
   }
   int e3 = 0;
   L.mant = __builtin_frexp(L.mant, &e3);
   L.ex += e3;
   const double Pm = L.Pm, mant = L.mant, ex = (double)L.ex;
   double* st = a.fst + w.fst_off;
-  if (live && ent) st[lg] = Pm;
+  if (live && ent) st[e] = Pm;
   if (live && lg == 0) {
     st[M * M] = mant;
     st[M * M + 1] = ex;
   }
   if (finalize) {
-    // delta_float (runner.py:86-100) on lane 0 of the group
-    const double pf = __shfl(Pm, gbase + M - 1, 64), qf = __shfl(Pm, gbase + M * M - 1, 64);
+    // delta_float (runner.py:86-100) on lane 0 of the group: p = P_f[0][m-1]
+    // (quad 0, lane m-1), q = P_f[m-1][m-1] (quad m-1, lane m-1)
+    const double pf = __shfl(Pm, gbase + M - 1, 64), qf = __shfl(Pm, gbase + 4 * (M - 1) + M - 1, 64);
     if (live && lg == 0) {
       const double ls = log(mant) + ex * 0.69314718055994530942;
       double delta, logq;
