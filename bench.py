@@ -206,7 +206,7 @@ def cpu_baseline(walks, depth, K, n_one, n_all, cores):
 
 
 # ------------------------------------------------------------------ served
-def served_sweep(rates, seconds, mem_budget, seed_base=1005):
+def served_sweep(rates, seconds, mem_budget, seed_base=1005, threads=0, round_steps=1000):
     """The served stream: Poisson arrivals of serve5 requests (m in
     {2, 3, 4}, depth log-U[1000, 100000], K in {32, 64, 128}) at each offered
     rate for `seconds`, then the queue drained. Per rate: p50/p99 latency
@@ -217,7 +217,7 @@ def served_sweep(rates, seconds, mem_budget, seed_base=1005):
 
     pool = serve5.program_pool(16)
     per_rate = []
-    with WalkServer(round_steps=1000, mem_budget=mem_budget) as srv:
+    with WalkServer(round_steps=round_steps, threads=threads, mem_budget=mem_budget) as srv:
         slots = srv.capacity()
         warm = [serve5.request_spec(dict(r, depth=1000, K=128, L=0.0)) for m in pool for r in pool[m]]
         serve5.run_stream(srv, warm, np.linspace(0.0, 0.05, len(warm)), timeout=120)  # uploads every program
@@ -236,7 +236,8 @@ def served_sweep(rates, seconds, mem_budget, seed_base=1005):
     return {
         "workload": "serve5: Poisson arrivals, m in {2,3,4} (pcf2-shape PCFs, 3x3/4x4 CMFs of 10-400 ops), depth "
                     "log-U[1000,100000], K in {32,64,128}; programs from a seeded pool of 48 generator draws",
-        "seconds_per_rate": seconds, "slots": slots, "rates": per_rate,
+        "seconds_per_rate": seconds, "slots": slots, "threads": threads or 256, "round_steps": round_steps,
+        "rates": per_rate,
         "saturation_req_s": max((s["rate_req_s"] for s in keep), default=None),
         "capacity_residue_steps_per_s": cap,
         "capacity_req_s": cap / top["mean_residue_steps_per_request"],
@@ -487,6 +488,9 @@ def main():
     ap.add_argument("--serve-rates", default="100,1000,3000,10000")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-budget-gib", type=float, default=4.0)
+    ap.add_argument("--serve-threads", type=int, default=0, help="server walk workgroup size (0: 256)")
+    ap.add_argument("--serve-round-steps", type=int, default=1000)
+    ap.add_argument("--serve-only", action="store_true", help="only the served stream (no batch lines)")
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--mode", choices=("batched", "persistent"), default="batched")
@@ -523,6 +527,11 @@ def main():
             ops = 6 if op == 3 else 1
             print(json.dumps({"op": name, "per_s": r, "lane_ops_per_s": r * ops, "peak_lane_ops_per_s": pk,
                               "frac_of_lane_peak": r * ops / pk, "lane_ops_each": ops}))
+        return
+    if args.serve_only:
+        rates = [float(r) for r in args.serve_rates.split(",") if r]
+        print(json.dumps(served_sweep(rates, args.serve_seconds, int(args.serve_budget_gib * (1 << 30)),
+                                      threads=args.serve_threads, round_steps=args.serve_round_steps)))
         return
     if args.workload != "pcf2" and not args.profile:
         return workload_line(args)
@@ -603,7 +612,8 @@ def main():
     served = None
     if not args.no_serve:
         rates = [float(r) for r in args.serve_rates.split(",") if r]
-        served = served_sweep(rates, args.serve_seconds, int(args.serve_budget_gib * (1 << 30)))
+        served = served_sweep(rates, args.serve_seconds, int(args.serve_budget_gib * (1 << 30)),
+                              threads=args.serve_threads, round_steps=args.serve_round_steps)
 
     extra_lines = {}
     if not args.no_extra:
