@@ -710,7 +710,8 @@ int setup_crt(drx_ctx* ctx, drx_plan* pl) {
 }
 
 // Vector walks: the residue kernel on the library's stream and the FP64
-// shadow on stream2 side by side; CRT after the residues, top-k after both.
+// shadow on stream2 side by side; CRT after the residues, top-k behind the
+// shadow on stream2, then the join.
 int launch_vec_sequence(drx_plan* pl, bool timed) {
   drx_ctx* ctx = pl->ctx;
   hipStream_t s = ctx->stream, s2 = ctx->stream2;
@@ -719,6 +720,9 @@ int launch_vec_sequence(drx_plan* pl, bool timed) {
   HIPCHK(ctx, hipEventRecord(ev[kEvFork], s));
   HIPCHK(ctx, hipStreamWaitEvent(s2, ev[kEvFork], 0));
   HIPCHK(ctx, launch_vec_fp(pl->args, pl->nfg, pl->fthreads, pl->flds, s2));
+  if (pl->b.top_k > 0)
+    HIPCHK(ctx, launch_topk(pl->d_fout.as<double>() + 3, 6, pl->n, pl->b.top_k, pl->d_topk.as<int32_t>(),
+                            pl->d_topk_ws.p, s2));
   HIPCHK(ctx, hipEventRecord(ev[kEvJoin], s2));
   if (timed) HIPCHK(ctx, hipEventRecord(ev[0], s));
   HIPCHK(ctx, launch_vec_walk(pl->args, pl->nvg, pl->vthreads, pl->vlds, s));
@@ -728,11 +732,10 @@ int launch_vec_sequence(drx_plan* pl, bool timed) {
   }
   if (pl->b.crt_limbs && (rc = launch_crt(pl, s))) return rc;
   HIPCHK(ctx, hipStreamWaitEvent(s, ev[kEvJoin], 0));
-  if (timed) HIPCHK(ctx, hipEventRecord(ev[3], s));
-  if (pl->b.top_k > 0)
-    HIPCHK(ctx, launch_topk(pl->d_fout.as<double>() + 3, 6, pl->n, pl->b.top_k, pl->d_topk.as<int32_t>(),
-                            pl->d_topk_ws.p, s));
-  if (timed) HIPCHK(ctx, hipEventRecord(ev[4], s));
+  if (timed) {
+    HIPCHK(ctx, hipEventRecord(ev[3], s));
+    HIPCHK(ctx, hipEventRecord(ev[4], s));
+  }
   return DRX_OK;
 }
 
@@ -743,6 +746,11 @@ int launch_sequence(drx_plan* pl, bool timed) {
   int rc = launch_stages(ctx, pl->args, pl->stages, pl->d_seeds.as<DevTask>(), 1, timed, pl->win, pl->nwin,
                          pl->mf_first);
   if (rc) return rc;
+  if (pl->b.top_k > 0) {  // on the shadow's stream, right behind the delta_float it ranks
+    HIPCHK(ctx, launch_topk(pl->d_fout.as<double>() + 3, 6, pl->n, pl->b.top_k, pl->d_topk.as<int32_t>(),
+                            pl->d_topk_ws.p, ctx->stream3));
+    HIPCHK(ctx, hipEventRecord(ctx->ev[kEvJoin], ctx->stream3));
+  }
   // residue outputs and CRT need only the combined residues: they run while
   // the shadow chain is still going; the final P_f copy needs the shadow
   const bool ffull = pl->args.ffull != nullptr;
@@ -751,9 +759,6 @@ int launch_sequence(drx_plan* pl, bool timed) {
   if ((rc = join_stages(ctx, timed))) return rc;
   if (ffull) HIPCHK(ctx, launch_finalize(pl->args, (int)pl->walk_order.size(), pl->b.full != nullptr, s));
   if (ffull && pl->b.crt_limbs && (rc = launch_crt(pl, s))) return rc;
-  if (pl->b.top_k > 0)
-    HIPCHK(ctx, launch_topk(pl->d_fout.as<double>() + 3, 6, pl->n, pl->b.top_k, pl->d_topk.as<int32_t>(),
-                            pl->d_topk_ws.p, s));
   if (timed) HIPCHK(ctx, hipEventRecord(ctx->ev[4], s));
   return DRX_OK;
 }

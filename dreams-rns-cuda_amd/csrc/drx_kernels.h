@@ -124,7 +124,7 @@ hipError_t launch_fd_seed(int m, const KArgs& a, int nseed, int kp, int threads,
 // 4x4 seeds of 8256 walks at K = 1000).
 inline int fd_seed_threads(int m, int kp) {
   const int pts = fd_dmax(m) + 1;
-  int j = 256 / kp;
+  int j = (kp <= 64 ? 64 : 256) / kp;
   if (j < 1) j = 1;
   if (j > pts) j = pts;
   return j * kp;

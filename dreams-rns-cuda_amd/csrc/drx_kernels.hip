@@ -1048,7 +1048,11 @@ __device__ __forceinline__ void shadow_step(ShadowLane& L, const double (&col)[M
   L.ex += take ? e1 : 0;
 }
 
-constexpr int kShadowTile = 16;  // M_f steps per register tile
+// M_f steps per register tile. Two tiles of 4 steps keep the kernel near 80
+// VGPRs, so its waves find room beside the walk kernel's while it runs;
+// tiles of 16 (about 240 VGPRs) had to wait for walk waves to retire, which
+// left the shadow chain running for milliseconds after the walk.
+constexpr int kShadowTile = 4;
 
 // M_f column of lane column j for steps s0 .. s0 + T (clamped to `last`)
 template <int M>
@@ -1176,7 +1180,9 @@ __global__ void __launch_bounds__(256) k_shadow(const KArgs a, int nspan, int fi
 // k_shadow (acc = P[i][0] * M[0][j]; acc = fma(P[i][1], M[1][j], acc); the
 // same maximum, test, division and log_scale mantissa), so the results are
 // bit for bit the same; the chain per step is about half as long.
-constexpr int kShadow2Tile = 16;  // M_f steps per register tile (4 doubles each)
+// M_f steps per register tile (4 doubles each); two tiles of 8 keep the
+// kernel small enough to sit beside the walk kernel's waves.
+constexpr int kShadow2Tile = 8;
 
 __device__ __forceinline__ void shadow2_tile(double4 (&t)[kShadow2Tile], const double* __restrict__ mf, int64_t s0,
                                              int64_t last) {
@@ -1902,6 +1908,99 @@ __global__ void __launch_bounds__(256) k_topk_round(const double* __restrict__ d
   }
 }
 
+// Top-k for k <= kTopSmall without a sorting network: a candidate's place
+// is the number of candidates before it, and it is written there when the
+// place is below k. The first round places each value among the kTopR values
+// of its block (one sorted list of k per block); a merge round places each
+// candidate of up to kTopLists lists by summing, over the lists, how many
+// of a list's entries precede it, a binary search in a sorted list (its own
+// list gives its position). Padding past n carries key 0 and the index
+// 0x80000000 | position, after every real value, so the order stays strict
+// and each place below k has exactly one writer.
+constexpr int kTopSmall = 64;
+constexpr int kTopR = 256;   // values per first-round block
+constexpr int kTopLists = 32;                  // lists per merge block
+constexpr int kTopM = kTopLists * kTopSmall;   // candidates per merge block, at most
+constexpr int kTopMT = 1024;                   // merge block threads
+
+__device__ __forceinline__ bool cand_before(uint64_t ka, uint32_t ia, uint64_t kb, uint32_t ib) {
+  return ka > kb || (ka == kb && ia < ib);
+}
+
+__global__ void __launch_bounds__(kTopR) k_topk_place(const double* __restrict__ d, int stride, int n, int k,
+                                                      TopCand* __restrict__ cout, int32_t* __restrict__ out) {
+  __shared__ uint64_t sk[kTopR];
+  __shared__ uint32_t si[kTopR];
+  const int t = threadIdx.x;
+  const int i = blockIdx.x * kTopR + t;
+  const uint64_t key = i < n ? order_key(d[(uint64_t)i * stride]) : 0ull;
+  const uint32_t idx = i < n ? (uint32_t)i : (0x80000000u | (uint32_t)i);
+  sk[t] = key;
+  si[t] = idx;
+  __syncthreads();
+  int r = 0;
+#pragma unroll 8
+  for (int j = 0; j < kTopR; ++j) r += cand_before(sk[j], si[j], key, idx) ? 1 : 0;
+  if (r < k) {
+    if (out) {
+      out[r] = (int32_t)idx;  // one block: n <= kTopR, so the first k are real
+    } else {
+      TopCand c;
+      c.key = key;
+      c.idx = idx;
+      c.pad = 0u;
+      cout[(uint64_t)blockIdx.x * k + r] = c;
+    }
+  }
+}
+
+// Entries of the sorted list at `base` (len <= kTopSmall) before (key, idx):
+// they form a prefix, found by halving steps from 64 down.
+__device__ __forceinline__ int count_before(const uint64_t* sk, const uint32_t* si, int base, int len,
+                                            uint64_t key, uint32_t idx) {
+  int pos = 0;
+#pragma unroll
+  for (int step = kTopSmall; step > 0; step >>= 1) {
+    const int q = pos + step;
+    const int a = base + min(q, len) - 1;  // stays in the list
+    if (q <= len && cand_before(sk[a], si[a], key, idx)) pos = q;
+  }
+  return pos;
+}
+
+__global__ void __launch_bounds__(kTopMT) k_topk_merge(const TopCand* __restrict__ cin, int nlist, int k, int lpb,
+                                                       TopCand* __restrict__ cout, int32_t* __restrict__ out) {
+  __shared__ uint64_t sk[kTopM];
+  __shared__ uint32_t si[kTopM];
+  const int l0 = blockIdx.x * lpb;
+  const int nl = min(lpb, nlist - l0);
+  const int nc = nl * k;
+  for (int c = threadIdx.x; c < nc; c += kTopMT) {
+    const TopCand x = cin[(uint64_t)l0 * k + c];
+    sk[c] = x.key;
+    si[c] = x.idx;
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < nc; c += kTopMT) {
+    const uint64_t key = sk[c];
+    const uint32_t idx = si[c];
+    int r = 0;
+#pragma unroll 4
+    for (int m = 0; m < nl; ++m) r += count_before(sk, si, m * k, k, key, idx);
+    if (r < k) {
+      if (out) {
+        out[r] = (int32_t)idx;
+      } else {
+        TopCand y;
+        y.key = key;
+        y.idx = idx;
+        y.pad = 0u;
+        cout[(uint64_t)blockIdx.x * k + r] = y;
+      }
+    }
+  }
+}
+
 // k above the tournament's reach: rank(i) = #{j before i}, element i written
 // at out[rank] when rank < k. LDS-tiled O(n^2 / T).
 __global__ void k_topk_rank(const double* __restrict__ d, int stride, int n, int k,
@@ -2163,6 +2262,7 @@ static int topk_block(int k) {
 }
 
 size_t topk_workspace_bytes(int n, int k) {
+  if (k <= kTopSmall) return n <= kTopR ? 0 : 2 * (((size_t)n + kTopR - 1) / kTopR) * (size_t)k * sizeof(TopCand);
   if (n <= kTopB || k > kTopB / 2) return 0;
   const size_t nb = ((size_t)n + topk_block(k) - 1) / topk_block(k);
   return 2 * nb * (size_t)k * sizeof(TopCand);
@@ -2192,6 +2292,26 @@ static int topk_final_block(int cur) {
 hipError_t launch_topk(const double* d, int stride, int n, int k, int32_t* out, void* ws, hipStream_t s) {
   if (n <= 0 || k <= 0) return hipSuccess;
   if (k > n) return hipErrorInvalidValue;
+  if (k <= kTopSmall) {
+    int nl = (n + kTopR - 1) / kTopR;
+    if (nl == 1) {
+      hipLaunchKernelGGL(k_topk_place, dim3(1), dim3(kTopR), 0, s, d, stride, n, k, nullptr, out);
+      return hipGetLastError();
+    }
+    if (!ws) return hipErrorInvalidValue;
+    TopCand* buf[2] = {(TopCand*)ws, (TopCand*)ws + (size_t)nl * k};
+    hipLaunchKernelGGL(k_topk_place, dim3(nl), dim3(kTopR), 0, s, d, stride, n, k, buf[0], nullptr);
+    const int lpb = kTopLists;
+    int src = 0;
+    while (nl > lpb) {
+      const int nb = (nl + lpb - 1) / lpb;
+      hipLaunchKernelGGL(k_topk_merge, dim3(nb), dim3(kTopMT), 0, s, buf[src], nl, k, lpb, buf[1 - src], nullptr);
+      nl = nb;
+      src = 1 - src;
+    }
+    hipLaunchKernelGGL(k_topk_merge, dim3(1), dim3(kTopMT), 0, s, buf[src], nl, k, lpb, nullptr, out);
+    return hipGetLastError();
+  }
   if (k > kTopB / 2) {
     hipLaunchKernelGGL(k_topk_rank, dim3((n + 255) / 256), dim3(256), 0, s, d, stride, n, k, out);
     return hipGetLastError();

@@ -294,11 +294,13 @@ def _topk_ref(v, k):
 
 
 def test_topk_tournament_and_rank_paths(gpu_ctx):
-    # several tournament rounds, NaN, +-0 and ties; k above the tournament's
-    # reach goes through the rank kernel
+    # k <= 64: placement and merge rounds (one to three); larger k: several
+    # tournament rounds, and above the tournament's reach the rank kernel;
+    # NaN, +-0 and ties throughout
     rng = np.random.default_rng(21)
-    for n, k in ((1, 1), (5, 5), (2048, 64), (2049, 1), (4096, 64), (4096, 1024), (70000, 1024),
-                 (300000, 64), (300000, 128), (5000, 1025), (3000, 3000)):
+    for n, k in ((1, 1), (5, 5), (256, 64), (257, 64), (300, 7), (2048, 64), (2049, 1), (4096, 64), (5000, 17),
+                 (20000, 33), (100000, 1), (4096, 65), (4096, 1024), (70000, 1024), (300000, 64), (300000, 128),
+                 (5000, 1025), (3000, 3000)):
         v = rng.normal(size=n).round(2)          # many exact ties
         v[rng.integers(0, n, size=max(1, n // 50))] = np.nan
         v[rng.integers(0, n, size=max(1, n // 100))] = -0.0
