@@ -1334,7 +1334,7 @@ int plan_build(drx_ctx* ctx, drx_plan* pl) {
   HIPCHK(ctx, pl->d_res.ensure(std::max<size_t>(1, res0.size()) * 8));
   HIPCHK(ctx, pl->d_fst.ensure(std::max<size_t>(1, fst0.size()) * 8));
   HIPCHK(ctx, pl->d_cp.ensure(std::max<uint64_t>(1, cp_total) * 8));
-  HIPCHK(ctx, pl->d_mf.ensure(std::max<uint64_t>(16, mf_total) * 8));
+  HIPCHK(ctx, pl->d_mf.ensure((std::max<uint64_t>(16, mf_total) + kMfSlack) * 8));
   HIPCHK(ctx, pl->d_fdt.ensure(std::max<uint64_t>(1, fdt_total) * 8));
   HIPCHK(ctx, pl->d_fout.ensure((size_t)std::max(1, n) * 6 * 8));
   HIPCHK(ctx, pl->d_rout.ensure((size_t)std::max(1, n) * 2 * pl->ld * 8));

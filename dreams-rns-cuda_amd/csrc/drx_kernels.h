@@ -65,6 +65,11 @@ enum WalkKind { WALK_INTERP = 0, WALK_FD = 1, WALK_FD_PCF = 2, WALK_FD_COMP = 3 
 // covers kMfRange consecutive steps of one task.
 constexpr int kMfBlock = 256;
 constexpr int kMfRange = 2 * kMfBlock;  // each lane evaluates 2 steps
+// Doubles of slack after the M_f pool: the shadow kernels load their
+// register tiles whole, up to two tiles (2 x 8 steps of 4 entries, or 2 x 4
+// steps of 16) past a walk's last step, into the next walk's M_f or this
+// slack; nothing loaded there is used.
+constexpr int kMfSlack = 256;
 
 struct GarnerArgs {
   const int64_t* res;     // rows of canonical residues, row stride res_stride

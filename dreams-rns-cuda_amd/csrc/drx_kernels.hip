@@ -1054,14 +1054,18 @@ __device__ __forceinline__ void shadow_step(ShadowLane& L, const double (&col)[M
 // left the shadow chain running for milliseconds after the walk.
 constexpr int kShadowTile = 4;
 
-// M_f column of lane column j for steps s0 .. s0 + T (clamped to `last`)
+static_assert(2 * kShadowTile * 16 <= kMfSlack, "M_f slack below two 4x4 tiles");
+
+// M_f column of lane column j for steps s0 .. s0 + T; steps past the walk's
+// last land in the pool's slack or the next walk's M_f and are never used,
+// so the addresses need no per-lane clamp (one base, immediate offsets).
 template <int M>
 __device__ __forceinline__ void shadow_tile(double (&t)[kShadowTile][M], const double* __restrict__ mf, int64_t s0,
-                                            int64_t last, int ej) {
+                                            int ej) {
+  const double* b = mf + (uint64_t)s0 * M * M + (uint64_t)ej * M;
 #pragma unroll
   for (int u = 0; u < kShadowTile; ++u) {
-    const int64_t sl = s0 + u < last ? s0 + u : last;
-    const double* c = mf + (uint64_t)sl * M * M + (uint64_t)ej * M;
+    const double* c = b + u * M * M;
 #pragma unroll
     for (int q = 0; q < M; ++q) t[u][q] = c[q];
   }
@@ -1117,22 +1121,21 @@ __global__ void __launch_bounds__(256) k_shadow(const KArgs a, int nspan, int fi
       nsteps = end - w0;
     }
   }
-  const int64_t last = nsteps > 0 ? nsteps - 1 : 0;
   // two register tiles: the loads of one are in flight while the other is used
   double ta[T][M], tb[T][M];
-  shadow_tile<M>(ta, mf, 0, last, ej);
+  shadow_tile<M>(ta, mf, 0, ej);
   int64_t s = 0;
   for (; s + 2 * T <= nsteps; s += 2 * T) {
-    shadow_tile<M>(tb, mf, s + T, last, ej);
+    shadow_tile<M>(tb, mf, s + T, ej);
     shadow_run_tile<M, false>(L, ta, s, nsteps, ent);
-    shadow_tile<M>(ta, mf, s + 2 * T, last, ej);
+    shadow_tile<M>(ta, mf, s + 2 * T, ej);
     shadow_run_tile<M, false>(L, tb, s + T, nsteps, ent);
     int e2 = 0;
     L.mant = __builtin_frexp(L.mant, &e2);
     L.ex += e2;
   }
   if (s < nsteps) {  // last partial pair of tiles, step by step masked
-    shadow_tile<M>(tb, mf, s + T, last, ej);
+    shadow_tile<M>(tb, mf, s + T, ej);
     shadow_run_tile<M, true>(L, ta, s, nsteps, ent);
     shadow_run_tile<M, true>(L, tb, s + T, nsteps, ent);
   }
@@ -1184,12 +1187,15 @@ __global__ void __launch_bounds__(256) k_shadow(const KArgs a, int nspan, int fi
 // kernel small enough to sit beside the walk kernel's waves.
 constexpr int kShadow2Tile = 8;
 
-__device__ __forceinline__ void shadow2_tile(double4 (&t)[kShadow2Tile], const double* __restrict__ mf, int64_t s0,
-                                             int64_t last) {
+static_assert(2 * kShadow2Tile * 4 <= kMfSlack, "M_f slack below two 2x2 tiles");
+
+// Steps s0 .. s0 + T of M_f; as in shadow_tile, steps past the walk's last
+// are loaded unclamped and never used.
+__device__ __forceinline__ void shadow2_tile(double4 (&t)[kShadow2Tile], const double* __restrict__ mf, int64_t s0) {
+  const double2* b = reinterpret_cast<const double2*>(mf + (uint64_t)s0 * 4);  // 16-byte aligned
 #pragma unroll
   for (int u = 0; u < kShadow2Tile; ++u) {
-    const int64_t sl = s0 + u < last ? s0 + u : last;
-    const double2* c = reinterpret_cast<const double2*>(mf + (uint64_t)sl * 4);  // 16-byte aligned
+    const double2* c = b + 2 * u;
     const double2 lo = c[0], hi = c[1];  // column-major: M00, M10 | M01, M11
     t[u] = make_double4(lo.x, lo.y, hi.x, hi.y);
   }
@@ -1248,21 +1254,20 @@ __global__ void __launch_bounds__(64) k_shadow2(const KArgs a, int nspan, int fi
       nsteps = end - w0;
     }
   }
-  const int64_t last = nsteps > 0 ? nsteps - 1 : 0;
   double4 ta[T], tb[T];
-  shadow2_tile(ta, mf, 0, last);
+  shadow2_tile(ta, mf, 0);
   int64_t s = 0;
   for (; s + 2 * T <= nsteps; s += 2 * T) {
-    shadow2_tile(tb, mf, s + T, last);
+    shadow2_tile(tb, mf, s + T);
     shadow2_run_tile<false>(x0, x1, mant, ex, ta, s, nsteps);
-    shadow2_tile(ta, mf, s + 2 * T, last);
+    shadow2_tile(ta, mf, s + 2 * T);
     shadow2_run_tile<false>(x0, x1, mant, ex, tb, s + T, nsteps);
     int e2 = 0;
     mant = __builtin_frexp(mant, &e2);
     ex += e2;
   }
   if (s < nsteps) {
-    shadow2_tile(tb, mf, s + T, last);
+    shadow2_tile(tb, mf, s + T);
     shadow2_run_tile<true>(x0, x1, mant, ex, ta, s, nsteps);
     shadow2_run_tile<true>(x0, x1, mant, ex, tb, s + T, nsteps);
   }

@@ -311,7 +311,7 @@ int round_launch(drx_server* s, const std::vector<SrvReq>& adm, const std::vecto
   DRX_HIPCHK(ctx, s->d_sspans.ensure(std::max<size_t>(1, s->hseed.size()) * sizeof(DevTask)));
   DRX_HIPCHK(ctx, s->d_ranges.ensure(std::max<size_t>(1, s->hr.size()) * sizeof(DevRange)));
   DRX_HIPCHK(ctx, s->d_cp.ensure(nt * slot_res(s) * 8));
-  DRX_HIPCHK(ctx, s->d_mf.ensure(std::max<uint64_t>(16, mf_total) * 8));
+  DRX_HIPCHK(ctx, s->d_mf.ensure((std::max<uint64_t>(16, mf_total) + kMfSlack) * 8));
   DRX_HIPCHK(ctx, hipMemcpyAsync(s->d_walks.p, s->h_walks.data(), s->h_walks.size() * sizeof(DevWalk), hipMemcpyHostToDevice, st));
   DRX_HIPCHK(ctx, hipMemcpyAsync(s->d_tasks.p, s->ht.data(), nt * sizeof(DevTask), hipMemcpyHostToDevice, st));
   DRX_HIPCHK(ctx, hipMemcpyAsync(s->d_spans.p, s->hs.data(), nh * sizeof(DevSpan), hipMemcpyHostToDevice, st));
