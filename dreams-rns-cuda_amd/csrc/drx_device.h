@@ -134,16 +134,35 @@ __device__ __forceinline__ double canon(double x, double p, double pinv) {
   return r;
 }
 
-// a^(p-2) mod p by left-to-right square-and-multiply over the 31 bits of
-// e = p - 2 (bit 30 is always set for p in [2^30, 2^31)). Fermat gives
-// 0 for a == 0 mod p, the reference's INV(0) = 0.
-__device__ __forceinline__ double inv_fermat(double a, double p, double pinv, uint32_t e) {
+// a^(p-2) mod p, left to right over the 31 bits of e = p - 2 (bit 30 is
+// always set for p in [2^30, 2^31)). Fermat gives 0 for a == 0 mod p, the
+// reference's INV(0) = 0. Bit by bit: 60 modular multiplies, few registers.
+__device__ __forceinline__ double inv_fermat_bits(double a, double p, double pinv, uint32_t e) {
   double r = a;  // bit 30
 #pragma unroll 1
   for (int bit = 29; bit >= 0; --bit) {
     r = mm(r, r, p, pinv);
     const double t = mm(r, a, p, pinv);
     r = ((e >> bit) & 1u) ? t : r;
+  }
+  return r;
+}
+// The same power in 2-bit windows: two squarings, then one multiply by a,
+// a^2 or a^3 as the window's digit says (47 modular multiplies; every lane
+// has its own e, so the window schedule is the same for all). The residue
+// class is the same as bit by bit, the balanced representative may differ.
+__device__ __forceinline__ double inv_fermat(double a, double p, double pinv, uint32_t e) {
+  const double a2 = mm(a, a, p, pinv);
+  const double a3 = mm(a2, a, p, pinv);
+  double r = a;  // bit 30
+#pragma unroll 1
+  for (int bit = 28; bit >= 0; bit -= 2) {
+    r = mm(r, r, p, pinv);
+    r = mm(r, r, p, pinv);
+    const uint32_t d = (e >> bit) & 3u;
+    const double f = d == 1u ? a : (d == 2u ? a2 : a3);
+    const double t = mm(r, f, p, pinv);
+    r = d ? t : r;
   }
   return r;
 }

@@ -63,6 +63,7 @@ __device__ __forceinline__ CodePtr code_ptr(const uint32_t* p) { return (CodePtr
 // `le` their entry rows and `lv` their state rows ([rows][T], by thread).
 // MUL and ADD, most of any program, take the first two branches; STM, STE
 // and NOP carry dst = x, so every instruction ends in one write.
+template <bool WIN_INV>
 __device__ __forceinline__ void exec_res(uint32_t wx, uint32_t wy, uint32_t wd, uint32_t wo, RBank& B,
                                          const double* __restrict__ lc, uint32_t KP, double* __restrict__ lm,
                                          double* __restrict__ le, const double* __restrict__ lv, const Lane& ln) {
@@ -80,7 +81,9 @@ __device__ __forceinline__ void exec_res(uint32_t wx, uint32_t wy, uint32_t wd, 
       case RO_LIN: r = __builtin_fma(y, lin_sign(row, 2), x * lin_sign(row, 0)); break;
       case RO_MULNZ: r = mm(x, (red(y, ln.p, ln.pinv) == 0.0) ? 1.0 : y, ln.p, ln.pinv); break;
       case RO_ZSEL: r = (red(y, ln.p, ln.pinv) == 0.0) ? 0.0 : x; break;
-      case RO_INV: r = inv_fermat(x, ln.p, ln.pinv, ln.pe); break;
+      case RO_INV:
+        r = WIN_INV ? inv_fermat(x, ln.p, ln.pinv, ln.pe) : inv_fermat_bits(x, ln.p, ln.pinv, ln.pe);
+        break;
       case RO_RED: r = red(x, ln.p, ln.pinv); break;
       case RO_NZ1: r = (red(x, ln.p, ln.pinv) == 0.0) ? 1.0 : x; break;
       case RO_LDM: r = lm[row * ln.T + ln.tid]; break;
@@ -102,7 +105,9 @@ __device__ __forceinline__ void exec_res(uint32_t wx, uint32_t wy, uint32_t wd, 
 
 // The residue program over one prime per lane, kResGroup instructions (their
 // kResWords words each) per group of scalar loads; n, the instruction count,
-// is a multiple of kResGroup.
+// is a multiple of kResGroup. WIN_INV picks the windowed Fermat inversion
+// (fewer multiplies, a few more registers).
+template <bool WIN_INV = true>
 __device__ __forceinline__ void run_program(const uint32_t* __restrict__ code_g, uint32_t n, RBank& B,
                                             const double* __restrict__ lc, uint32_t KP, double* __restrict__ lm,
                                             double* __restrict__ le, const double* __restrict__ lv,
@@ -115,7 +120,7 @@ __device__ __forceinline__ void run_program(const uint32_t* __restrict__ code_g,
     for (int j = 0; j < kResGroup * kResWords; ++j) w[j] = code[pc * kResWords + j];
 #pragma unroll
     for (int j = 0; j < kResGroup; ++j)
-      exec_res(w[4 * j], w[4 * j + 1], w[4 * j + 2], w[4 * j + 3], B, lc, KP, lm, le, lv, ln);
+      exec_res<WIN_INV>(w[4 * j], w[4 * j + 1], w[4 * j + 2], w[4 * j + 3], B, lc, KP, lm, le, lv, ln);
   }
 }
 
@@ -1350,7 +1355,7 @@ __global__ void __launch_bounds__(256) k_walk_vec(const KArgs a) {
   double* nxt = sb;
 #pragma unroll 1
   for (int64_t s = 0; s < vg.depth; ++s) {
-    run_program(code, pg.ncode, B, L.lc, KP, L.lm, nxt, cur, ln);
+    run_program<false>(code, pg.ncode, B, L.lc, KP, L.lm, nxt, cur, ln);  // 96 VGPRs: 5 waves per SIMD
     double* t = cur;
     cur = nxt;
     nxt = t;
