@@ -396,9 +396,11 @@ int ensure_crt_table(drx_ctx* ctx, int K, const CrtTable** out) {
   t->L = (int)M.size();
   t->Kp = (K + 31) / 32 * 32;
   t->T = 4 * t->L + 1;
-  t->Tp = (t->T + 127) / 128 * 128;
+  t->Tp = (t->T + 3 + 127) / 128 * 128;  // the plane shifts reach column T + 2
   const int L = t->L, Kp = t->Kp;
-  std::vector<int8_t> bt((size_t)t->Tp * Kp, 0);
+  // kBtLead zero rows before row 0 (the GEMM reads rows t - b down to -3)
+  std::vector<int8_t> bt((size_t)(kBtLead + t->Tp) * Kp, 0);
+  int8_t* bt0 = bt.data() + (size_t)kBtLead * Kp;
   std::vector<double> w(K);
   std::vector<uint32_t> q(L);
   for (int i = 0; i < K; ++i) {
@@ -417,9 +419,9 @@ int ensure_crt_table(drx_ctx* ctx, int K, const CrtTable** out) {
     for (int d = 0; d < 4 * L; ++d) {
       int v = (int)((q[d / 4] >> (8 * (d % 4))) & 255u) + carry;
       carry = v >= 128 ? 1 : 0;
-      bt[(size_t)d * Kp + i] = (int8_t)(v - 256 * carry);
+      bt0[(size_t)d * Kp + i] = (int8_t)(v - 256 * carry);
     }
-    bt[(size_t)(4 * L) * Kp + i] = (int8_t)carry;
+    bt0[(size_t)(4 * L) * Kp + i] = (int8_t)carry;
   }
   std::vector<uint32_t> m(L + 2, 0u), h(L + 2, 0u);
   for (int l = 0; l < L; ++l) m[l] = M[l];
@@ -602,7 +604,7 @@ static void crt_matrix_args(drx_ctx* ctx, const CrtTable* t, int n, int npad, in
   c.prime_p = ctx->d_prime_p.as<double>();
   c.prime_pinv = ctx->d_prime_pinv.as<double>();
   c.w = t->d_w.as<double>();
-  c.bt = t->d_bt.as<int8_t>();
+  c.bt = t->d_bt.as<int8_t>() + (size_t)kBtLead * t->Kp;
   c.m = t->d_m.as<uint32_t>();
   c.h = t->d_h.as<uint32_t>();
   c.A = A;
@@ -686,7 +688,7 @@ int setup_crt(drx_ctx* ctx, drx_plan* pl) {
     if ((size_t)t->L + 2 > (size_t)pl->ld) return fail(ctx, DRX_EINVAL, "internal: CRT rows narrower than M");
     const size_t npad = (e.second.size() + 127) / 128 * 128;
     a_bytes = std::max(a_bytes, (size_t)4 * npad * t->Kp);
-    c_bytes = std::max(c_bytes, (size_t)4 * t->Tp * npad * 4);
+    c_bytes = std::max(c_bytes, (size_t)t->Tp * npad * 4);
     k_bytes = std::max(k_bytes, npad * 12);
     tabs.push_back(t);
   }
@@ -2239,7 +2241,7 @@ int drx_crt(drx_ctx* ctx, const int64_t* residues, int n, int K, int stride, uin
     HIPCHK(ctx, ctx->d_crt_rows.ensure((size_t)n * 4));
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_crt_rows.p, ids.data(), (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, ctx->d_crt_a.ensure((size_t)4 * npad * t->Kp));
-    HIPCHK(ctx, ctx->d_crt_c.ensure((size_t)4 * t->Tp * npad * 4));
+    HIPCHK(ctx, ctx->d_crt_c.ensure((size_t)t->Tp * npad * 4));
     HIPCHK(ctx, ctx->d_crt_k.ensure((size_t)npad * 12));
     CrtArgs c{};
     crt_matrix_args(ctx, t, n, npad, ctx->d_crt_rows.as<int32_t>(), ctx->d_tmp0.as<int64_t>(), (uint64_t)stride,

@@ -7,9 +7,10 @@
 // up to one multiple of M that an exact comparison with (M - 1) / 2 settles.
 // S is an integer matrix product: the c_i of a row in four balanced base-256
 // digits (planes b) times the M_i in balanced base-256 digits (columns t),
-// int8 x int8 -> int32 on the MFMA units (|sum| < K 2^14 < 2^26), and
-// S = sum_{b,t} C[b][t] 256^(b + t), folded into 32-bit limbs by one lane per
-// row. Work per row: 4 K T multiply-adds with T ~ 4 * 31 K / 32, against
+// int8 x int8 -> int32 on the MFMA units. The planes are one reduction:
+// D[t] = sum_b sum_i A[b][i] B[t - b][i] is the coefficient of 256^t in
+// S = sum_t D[t] 256^t (|D| < 4 K 2^14 <= 2^26), folded into 32-bit limbs by
+// one lane per row. Work per row: 4 K T multiply-adds with T ~ 4 * 31 K / 32, against
 // Garner's K^2 / 2 dependent modular steps; the matrix units make it the
 // faster one for wide K.
 #include "drx_crt.h"
@@ -66,35 +67,39 @@ __global__ void __launch_bounds__(256) k_crt_digits(const CrtArgs g) {
   }
 }
 
-// C[b][t][j] = sum_i A[b][j][i] BT[t][i]: a block of 4 wavefronts computes
-// a 128 x 128 tile (each wave 64 rows x 64 columns, 2 x 2 MFMA 32x32x32 i8
-// tiles); rows, columns and the reduction are padded, so there are no edges.
+// D[t][j] = sum_b sum_i A[b][j][i] BT[t - b][i] (BT has zero rows -3 .. -1
+// and T .. Tp - 1): a block of 4 wavefronts computes a 128 x 128 tile (each
+// wave 64 rows x 64 columns, 2 x 2 MFMA 32x32x32 i8 tiles) over the four
+// planes in turn; rows, columns and the reduction are padded, so there are
+// no edges. One int32 plane out instead of four.
 // MFMA operands: lane l supplies row (or column) l % 32 and 16 reduction
 // elements, the same 16 for the A and the B operand (here 16 (l / 32) ..
 // 16 (l / 32) + 15 of the 32); accumulator element e of lane l is row
 // 8 (e / 4) + 4 (l / 32) + e % 4, column l % 32.
 __global__ void __launch_bounds__(256) k_crt_gemm(const CrtArgs g) {
-  const int b = blockIdx.z, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int row0 = blockIdx.y * 128 + (wave >> 1) * 64;
   const int col0 = blockIdx.x * 128 + (wave & 1) * 64;
-  const int8_t* A = g.A + (size_t)b * g.npad * g.Kp;
   const int kl = (lane >> 5) * 16, rl = lane & 31;
-  const int8_t* a0p = A + (size_t)(row0 + rl) * g.Kp + kl;
-  const int8_t* a1p = A + (size_t)(row0 + 32 + rl) * g.Kp + kl;
-  const int8_t* b0p = g.bt + (size_t)(col0 + rl) * g.Kp + kl;
-  const int8_t* b1p = g.bt + (size_t)(col0 + 32 + rl) * g.Kp + kl;
   v16i c00 = {0}, c01 = {0}, c10 = {0}, c11 = {0};
-  for (int k0 = 0; k0 < g.Kp; k0 += 32) {
-    const v4i a0 = *reinterpret_cast<const v4i*>(a0p + k0);
-    const v4i a1 = *reinterpret_cast<const v4i*>(a1p + k0);
-    const v4i b0 = *reinterpret_cast<const v4i*>(b0p + k0);
-    const v4i b1 = *reinterpret_cast<const v4i*>(b1p + k0);
-    c00 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b0, c00, 0, 0, 0);
-    c01 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b1, c01, 0, 0, 0);
-    c10 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b0, c10, 0, 0, 0);
-    c11 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b1, c11, 0, 0, 0);
+  for (int b = 0; b < 4; ++b) {
+    const int8_t* A = g.A + (size_t)b * g.npad * g.Kp;
+    const int8_t* a0p = A + (size_t)(row0 + rl) * g.Kp + kl;
+    const int8_t* a1p = A + (size_t)(row0 + 32 + rl) * g.Kp + kl;
+    const int8_t* b0p = g.bt + (ptrdiff_t)(col0 + rl - b) * g.Kp + kl;  // row -b .. : zero rows below 0
+    const int8_t* b1p = g.bt + (ptrdiff_t)(col0 + 32 + rl - b) * g.Kp + kl;
+    for (int k0 = 0; k0 < g.Kp; k0 += 32) {
+      const v4i a0 = *reinterpret_cast<const v4i*>(a0p + k0);
+      const v4i a1 = *reinterpret_cast<const v4i*>(a1p + k0);
+      const v4i b0 = *reinterpret_cast<const v4i*>(b0p + k0);
+      const v4i b1 = *reinterpret_cast<const v4i*>(b1p + k0);
+      c00 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b0, c00, 0, 0, 0);
+      c01 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b1, c01, 0, 0, 0);
+      c10 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b0, c10, 0, 0, 0);
+      c11 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b1, c11, 0, 0, 0);
+    }
   }
-  int32_t* C = g.C + (size_t)b * g.Tp * g.npad;
+  int32_t* C = g.C;
   const int rq = (lane >> 5) * 4;
 #pragma unroll
   for (int e = 0; e < 16; ++e) {
@@ -106,32 +111,31 @@ __global__ void __launch_bounds__(256) k_crt_gemm(const CrtArgs g) {
   }
 }
 
-// s (S - k M) into NL = L + 2 limbs of one row, four limbs per round so
-// that the 64 loads of a round are in flight together; returns the final
-// carry (the sign word: 0 or -1).
+// s (S - k M) into NL = L + 2 limbs of one row (limb l from D[4l .. 4l+3]),
+// eight limbs per round so that the 32 loads of a round are in flight
+// together; returns the final carry (the sign word: 0 or -1).
 __device__ __forceinline__ int64_t fold_limbs(const CrtArgs& g, int j, int64_t kq, int64_t s, uint32_t* out,
                                               int NL) {
-  const size_t pl = (size_t)g.Tp * g.npad;
   int64_t carry = 0;
-  for (int l0 = 0; l0 < NL; l0 += 4) {
-    int32_t cv[4][4][4];
+  for (int l0 = 0; l0 < NL; l0 += 8) {
+    int32_t cv[8][4];
 #pragma unroll
-    for (int lq = 0; lq < 4; ++lq)
+    for (int lq = 0; lq < 8; ++lq)
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
+      for (int q = 0; q < 4; ++q) {
+        // every load issued (a clamped row, then a select), so the 32 are in
+        // flight together; a load under the bound's branch waited for each
+        const int t = 4 * (l0 + lq) + q;
+        const int32_t x = g.C[(size_t)(t < g.Tp ? t : g.Tp - 1) * g.npad + j];
+        cv[lq][q] = t < g.Tp ? x : 0;
+      }
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          const int t = 4 * (l0 + lq) + q - b;
-          cv[lq][q][b] = (t >= 0 && t < g.T) ? g.C[b * pl + (size_t)t * g.npad + j] : 0;
-        }
-#pragma unroll
-    for (int lq = 0; lq < 4; ++lq) {
+    for (int lq = 0; lq < 8; ++lq) {
       const int l = l0 + lq;
       if (l >= NL) break;
       int64_t v = 0;
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-        v += (int64_t)(cv[lq][q][0] + cv[lq][q][1] + cv[lq][q][2] + cv[lq][q][3]) * ((int64_t)1 << (8 * q));
+      for (int q = 0; q < 4; ++q) v += (int64_t)cv[lq][q] * ((int64_t)1 << (8 * q));
       v = s * (v - kq * (int64_t)g.m[l]) + carry;  // |v| < 2^53
       const uint32_t limb = (uint32_t)v;
       carry = (v - (int64_t)limb) >> 32;
@@ -202,10 +206,10 @@ __global__ void __launch_bounds__(256) k_crt_fold(const CrtArgs g) {
 hipError_t launch_crt_matrix(const CrtArgs& g, hipStream_t s) {
   if (g.n <= 0) return hipSuccess;
   if (g.npad % 128 || g.Tp % 128 || g.Kp % 32 || g.npad < g.n || g.Kp < g.K || g.Tp < g.T ||
-      (size_t)g.L + 2 > g.res_stride)
+      (size_t)g.L + 2 > g.res_stride || g.Tp < g.T + 3)
     return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_crt_digits, dim3(g.npad), dim3(256), 0, s, g);
-  hipLaunchKernelGGL(k_crt_gemm, dim3(g.Tp / 128, g.npad / 128, 4), dim3(256), 0, s, g);
+  hipLaunchKernelGGL(k_crt_gemm, dim3(g.Tp / 128, g.npad / 128), dim3(256), 0, s, g);
   hipLaunchKernelGGL(k_crt_fold, dim3((g.n + 255) / 256), dim3(256), 0, s, g);
   return hipGetLastError();
 }

@@ -985,8 +985,9 @@ __global__ void __launch_bounds__(256) k_mf(const KArgs a) {
 // grows by ln mx. For 3x3 and 4x4, one 16-lane DPP row per walk: quad i holds
 // row i of P_f, lane j of the quad entry (i, j) (3x3 leaves lane 3 of each
 // quad and quad 3 idle). A lane takes its row from its own quad by DPP
-// broadcasts and the maximum (and the NaN test) crosses the row by four DPP
-// steps, so the chain holds no LDS round trip; 2x2 walks take k_shadow2.
+// broadcasts, the maximum crosses the row by four DPP steps and the NaN test
+// by one wave ballot, so the chain holds no LDS round trip; 2x2 walks take
+// k_shadow2.
 // log_scale is carried as a mantissa product and a binary exponent, both
 // exact in step order, and ln is taken once when it is read.
 template <int CTRL>
@@ -1016,11 +1017,13 @@ __device__ __forceinline__ double row16_max(double v) {
   return fmax(v, dpp_f64<kRowMirror>(v));
 }
 
-__device__ __forceinline__ int row16_or(int v) {
-  v |= __builtin_amdgcn_mov_dpp(v, kQuadX1, 0xF, 0xF, false);
-  v |= __builtin_amdgcn_mov_dpp(v, kQuadX2, 0xF, 0xF, false);
-  v |= __builtin_amdgcn_mov_dpp(v, kRowHalfMirror, 0xF, 0xF, false);
-  return v | __builtin_amdgcn_mov_dpp(v, kRowMirror, 0xF, 0xF, false);
+// Whether the flag is set on any lane of the lane's 16-lane row: the row's
+// 16 bits of one wave ballot (the compare that makes the flag writes it), in
+// place of four dependent DPP steps. The lanes of a row are one walk's, so
+// they are active or inactive together.
+__device__ __forceinline__ bool row16_any(bool f) {
+  const uint64_t b = __ballot(f);
+  return ((b >> (threadIdx.x & 48u)) & 0xFFFFull) != 0ull;
 }
 
 // Shadow state of one lane: its entry of P_f, and log_scale as the mantissa
@@ -1042,8 +1045,8 @@ __device__ __forceinline__ void shadow_step(ShadowLane& L, const double (&col)[M
 #pragma unroll
   for (int q = 1; q < M; ++q) acc = __builtin_fma(row[q], col[q], acc);
   const double mx = row16_max(ent ? __builtin_fabs(acc) : 0.0);
-  const int anynan = row16_or((ent && acc != acc) ? 1 : 0);
-  const bool resc = anynan == 0 && mx > 1e10;
+  const bool anynan = row16_any(ent && acc != acc);
+  const bool resc = !anynan && mx > 1e10;
   const double np = ent ? (resc ? acc / mx : acc) : 0.0;
   int e1 = 0;
   const double m1 = __builtin_frexp(mx, &e1);

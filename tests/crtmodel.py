@@ -9,8 +9,10 @@ w_i = M_i^-1 mod p_i):
   k   = rint(sum_i c_i / p_i) in FP64, so S - k M is the centred x up to one
         multiple of M, fixed by an exact comparison with (M - 1) / 2.
 S is a matrix product: c_i in 4 balanced base-256 digits (planes b) times
-M_i in balanced base-256 digits (columns t), int8 x int8 -> int32, and
-S = sum_{b,t} C[b][t] 256^(b+t).
+M_i in balanced base-256 digits (columns t), int8 x int8 -> int32. The
+planes are one reduction, D[t] = sum_b sum_i A[b][i] B[t - b][i], the
+coefficient of 256^t in S = sum_t D[t] 256^t, and the fold builds 32-bit
+limb l from D[4l .. 4l + 3].
 """
 
 from __future__ import annotations
@@ -67,15 +69,23 @@ def crt_rows(res: np.ndarray, primes) -> List[int]:
             A[:, j, i] = balanced_digits(c, 4)
             f += c * (1.0 / p)
         kq.append(int(np.rint(f)))
-    C = np.einsum("bji,ti->btj", A, bt.astype(np.int64))  # [b][t][j], each |.| < 2^26
-    assert np.abs(C).max() < 2 ** 26
+    T = bt.shape[0]
+    C = np.einsum("bji,ti->btj", A, bt.astype(np.int64))  # [b][t][j]
+    D = np.zeros((T + 3, n), dtype=np.int64)  # D[t] = sum_b C[b][t - b]
+    for b in range(4):
+        D[b:b + T] += C[b]
+    assert np.abs(D).max() < 2 ** 26
+    NL = L + 2
+    Dp = np.zeros((4 * NL, n), dtype=np.int64)
+    Dp[:min(4 * NL, T + 3)] = D[:4 * NL]
     H = (M - 1) // 2
     out = []
     for j in range(n):
         S = 0
-        for b in range(4):
-            for t in range(C.shape[1]):
-                S += int(C[b, t, j]) << (8 * (b + t))
+        for l in range(NL):  # limb l from D[4l .. 4l + 3]
+            v = sum(int(Dp[4 * l + q, j]) << (8 * q) for q in range(4))
+            assert abs(v) < 2 ** 53
+            S += v << (32 * l)
         x = S - kq[j] * M
         if x > H:
             x -= M
