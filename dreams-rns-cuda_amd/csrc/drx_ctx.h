@@ -159,7 +159,8 @@ struct Stage {
 // stream2 (work on stream before it overlaps the shadow chain). With
 // `timed`, events 0..3 mark walk start, walk end, combine end, join.
 // The FP64 chain runs in `nwin` windows of `win` steps: the M_f pass of
-// window c on stream2, the shadow of window c on stream3 behind it.
+// window c on stream2 (with several windows, the first alone on stream ahead
+// of the seeds), the shadow of window c on stream3 behind it.
 // With `mf_first` the whole M_f pass runs on `stream` before the walk
 // kernels, with the device to itself, and the shadow windows follow it on
 // stream3 while the walk runs: for batches whose walk work dwarfs the M_f
