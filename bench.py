@@ -342,9 +342,11 @@ def with_copy(rsteps: float, copy_ms: list) -> dict:
                     "to pinned host memory"}
 
 
-def batch_rate(name, steps=2, warmup=1, threads=256, chunk=512):
+def batch_rate(name, steps=2, warmup=1, threads=256, chunk=256):
     """One batch workload timed in the default line: residue-steps/s with
-    the outputs in HBM (plan built before the timed region)."""
+    the outputs in HBM (plan built before the timed region), at the launch
+    shape the autotuner picks for cmf3 and mix4 (256-lane workgroups,
+    256-step tasks)."""
     from dreams_rns.engine import BatchPlan
 
     t0 = time.perf_counter()

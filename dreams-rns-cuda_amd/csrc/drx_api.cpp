@@ -2117,8 +2117,8 @@ int drx_autotune(drx_ctx* ctx, const drx_batch* b, const drx_launch_cfg* base, i
   cfg0.mode = DRX_MODE_BATCHED;
   cfg0.use_graph = 0;
   const int thr_c[] = {64, 128, 256};
-  const int chunk_c[] = {512, 1024, 2048, 4096, 8192, 16384};
-  const int reps = 2, ncand = 18;
+  const int chunk_c[] = {128, 256, 512, 1024, 2048, 4096, 8192, 16384};
+  const int reps = 2, ncand = 24;
   const double t_start = now_s();
   // sample size: the whole batch (up to max_walks) unless one run of the
   // default shape takes more than its share of the budget, then halve

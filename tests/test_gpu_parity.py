@@ -384,7 +384,7 @@ def test_autotune_choice_is_valid_and_results_unchanged(gpu_ctx):
     specs = [pcf_spec(D.compile_pcf(w["a"], w["b"]), D.pcf_initial_values(w["a"]), 1500, 32, 1, target=w["L"])
              for w in cfg["walks"]]
     t = autotune(specs, max_walks=128, budget_s=10.0)
-    assert t["threads"] in (64, 128, 256) and t["chunk_steps"] in (512, 1024, 2048, 4096, 8192, 16384)
+    assert t["threads"] in (64, 128, 256) and t["chunk_steps"] in (128, 256, 512, 1024, 2048, 4096, 8192, 16384)
     assert t["walks_per_wg"] in (1, 2, 4, 8) and t["walks_per_wg"] * 32 <= t["threads"]
     assert t["slice"] == 256  # 32-prime walks have nothing to slice
     assert t["sample_walk_ms"] > 0
