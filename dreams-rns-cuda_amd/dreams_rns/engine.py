@@ -298,6 +298,7 @@ class BatchPlan:
             threads, chunk_steps = t["threads"], t["chunk_steps"]
             walks_per_wg, slice = t["walks_per_wg"], t["slice"]
         self.threads, self.chunk_steps = int(threads), int(chunk_steps)
+        self.use_graph = bool(use_graph)
         self.walks_per_wg, self.slice = int(walks_per_wg), int(slice)
         # outputs live in page-locked memory: the copies back run at link speed
         z = N.pinned_zeros

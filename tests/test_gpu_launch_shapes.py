@@ -120,3 +120,26 @@ def test_bad_launch_shapes_are_refused(gpu_ctx):
     for kw in ({"slice": 48}, {"slice": 512}, {"walks_per_wg": 3}):
         with pytest.raises(Exception):
             BatchPlan(specs, **kw)
+
+
+def test_graph_replays_equal_stream_launch(gpu_ctx):
+    # the captured launch sequence (three M_f / shadow windows, the first
+    # ahead of the seeds, top-k on the shadow's stream) replayed four times,
+    # with stream-launched work between the replays and both the copying and
+    # the resident run: every replay gives the stream launch's outputs
+    cfg = synth.pcf2(n_walks=256, seed=131)
+    specs = [pcf_spec(D.compile_pcf(w["a"], w["b"]), D.pcf_initial_values(w["a"]), 1200, 32, 1, target=w["L"])
+             for w in cfg["walks"]]
+    kw = {"crt": True, "top_k": 16, "threads": 64, "chunk_steps": 4096}
+    with BatchPlan(specs, **kw) as ps:
+        ref = _copy(ps.run())
+        with BatchPlan(specs, use_graph=True, **kw) as pg:
+            for rep in range(4):
+                print("graph replay", rep, flush=True)
+                if rep % 2:
+                    pg.run(fetch=False)
+                    got = _copy(pg.fetch())
+                else:
+                    got = _copy(pg.run())
+                _same(ref, got, rep)
+                ps.run()
