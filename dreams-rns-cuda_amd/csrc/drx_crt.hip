@@ -77,28 +77,57 @@ __global__ void __launch_bounds__(256) k_crt_digits(const CrtArgs g) {
 // 16 (l / 32) + 15 of the 32); accumulator element e of lane l is row
 // 8 (e / 4) + 4 (l / 32) + e % 4, column l % 32.
 __global__ void __launch_bounds__(256) k_crt_gemm(const CrtArgs g) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int row0 = blockIdx.y * 128 + (wave >> 1) * 64;
-  const int col0 = blockIdx.x * 128 + (wave & 1) * 64;
+  // The block's 128 rows and 128 columns of each 32-wide reduction step are
+  // staged once in LDS (each thread copies 16 bytes of each), double
+  // buffered, and the four waves read their MFMA operands from there: half
+  // the L2 traffic of every wave loading its own. Rows are LS bytes apart in
+  // LDS (32 bytes of data, 16 of padding) to spread the 16-byte reads over
+  // the banks.
+  constexpr int LS = 48;
+  __shared__ __attribute__((aligned(16))) int8_t la[2][128 * LS];
+  __shared__ __attribute__((aligned(16))) int8_t lb[2][128 * LS];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int brow = blockIdx.y * 128, bcol = blockIdx.x * 128;
+  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;  // the wave's 64 x 64 of the block's tile
   const int kl = (lane >> 5) * 16, rl = lane & 31;
+  const int lr = tid >> 1, lh = (tid & 1) * 16;  // this thread's 16 staged bytes: row lr, bytes lh ..
+  const int nk = g.Kp / 32, nsteps = 4 * nk;   // (plane, 32-wide slice) steps
+  // row lr of the block in plane b (A) and digit row bcol + lr - b of the table (B; rows -3 .. -1 are zero)
+  auto ga = [&](int st) -> const v4i* {
+    const int b = st / nk, k0 = (st - b * nk) * 32;
+    return reinterpret_cast<const v4i*>(g.A + (size_t)b * g.npad * g.Kp + (size_t)(brow + lr) * g.Kp + k0 + lh);
+  };
+  auto gb = [&](int st) -> const v4i* {
+    const int b = st / nk, k0 = (st - b * nk) * 32;
+    return reinterpret_cast<const v4i*>(g.bt + (ptrdiff_t)(bcol + lr - b) * g.Kp + k0 + lh);
+  };
+  v4i ra = *ga(0), rb = *gb(0);
+  *reinterpret_cast<v4i*>(&la[0][lr * LS + lh]) = ra;
+  *reinterpret_cast<v4i*>(&lb[0][lr * LS + lh]) = rb;
+  __syncthreads();
   v16i c00 = {0}, c01 = {0}, c10 = {0}, c11 = {0};
-  for (int b = 0; b < 4; ++b) {
-    const int8_t* A = g.A + (size_t)b * g.npad * g.Kp;
-    const int8_t* a0p = A + (size_t)(row0 + rl) * g.Kp + kl;
-    const int8_t* a1p = A + (size_t)(row0 + 32 + rl) * g.Kp + kl;
-    const int8_t* b0p = g.bt + (ptrdiff_t)(col0 + rl - b) * g.Kp + kl;  // row -b .. : zero rows below 0
-    const int8_t* b1p = g.bt + (ptrdiff_t)(col0 + 32 + rl - b) * g.Kp + kl;
-    for (int k0 = 0; k0 < g.Kp; k0 += 32) {
-      const v4i a0 = *reinterpret_cast<const v4i*>(a0p + k0);
-      const v4i a1 = *reinterpret_cast<const v4i*>(a1p + k0);
-      const v4i b0 = *reinterpret_cast<const v4i*>(b0p + k0);
-      const v4i b1 = *reinterpret_cast<const v4i*>(b1p + k0);
-      c00 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b0, c00, 0, 0, 0);
-      c01 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b1, c01, 0, 0, 0);
-      c10 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b0, c10, 0, 0, 0);
-      c11 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b1, c11, 0, 0, 0);
+  for (int st = 0; st < nsteps; ++st) {
+    const int cur = st & 1;
+    const bool next = st + 1 < nsteps;
+    if (next) {  // the next step's global loads are in flight during this step's MFMAs
+      ra = *ga(st + 1);
+      rb = *gb(st + 1);
     }
+    const v4i a0 = *reinterpret_cast<const v4i*>(&la[cur][(wr + rl) * LS + kl]);
+    const v4i a1 = *reinterpret_cast<const v4i*>(&la[cur][(wr + 32 + rl) * LS + kl]);
+    const v4i b0 = *reinterpret_cast<const v4i*>(&lb[cur][(wc + rl) * LS + kl]);
+    const v4i b1 = *reinterpret_cast<const v4i*>(&lb[cur][(wc + 32 + rl) * LS + kl]);
+    c00 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b0, c00, 0, 0, 0);
+    c01 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a0, b1, c01, 0, 0, 0);
+    c10 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b0, c10, 0, 0, 0);
+    c11 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b1, c11, 0, 0, 0);
+    if (next) {  // the other buffer was last read in the previous step, before its barrier
+      *reinterpret_cast<v4i*>(&la[cur ^ 1][lr * LS + lh]) = ra;
+      *reinterpret_cast<v4i*>(&lb[cur ^ 1][lr * LS + lh]) = rb;
+    }
+    __syncthreads();
   }
+  const int row0 = brow + wr, col0 = bcol + wc;
   int32_t* C = g.C;
   const int rq = (lane >> 5) * 4;
 #pragma unroll
