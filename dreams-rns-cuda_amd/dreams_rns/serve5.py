@@ -1,4 +1,4 @@
-"""The served-stream workload and its driver: Poisson arrivals of walk
+"""The served-stream workload and its load generator: Poisson arrivals of walk
 requests into :class:`~dreams_rns.server.WalkServer`.
 
 Requests follow the serve5 generator (:func:`dreams_rns.synth.serve5_request`):

@@ -150,7 +150,7 @@ class CmfSweep:
             for ti, traj in enumerate(self.trajectories):
                 try:
                     prog = compile_cmf_spec(spec, trajectory=list(traj))
-                except Exception as e:  # the driver skips the pair and goes on
+                except Exception as e:  # a100_sweep.py skips the pair and goes on
                     self.skipped.append((si, ti, f"{type(e).__name__}: {e}"))
                     continue
                 if prog is None:

@@ -1,9 +1,9 @@
 """Walks and CRT over the reference's whole prime range: K above 1024 primes
-(the Euler2AI driver's automatic K reaches 2000-4300 at depth 2000,
+(euler2ai_full_verify.py's automatic K reaches 2000-4300 at depth 2000,
 euler2ai_full_verify.py:40-92, 331-333), the wide Garner kernel against the
 textbook CRT and round trips, crt_reconstruct on any odd primes with the
 reference's argument rules (runner.py:57-76), and the Euler2AI report at
-the driver's defaults without ERROR rows."""
+that script's defaults without ERROR rows."""
 
 import math
 
