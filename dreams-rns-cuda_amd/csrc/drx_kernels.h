@@ -127,6 +127,9 @@ hipError_t launch_fd_seed(int m, const KArgs& a, int nseed, int kp, int threads,
 // blocks of 256 lanes overlap each other's latency where one block of 512
 // would leave the CU idle between blocks; 3.3 ms against 5.5 ms for the
 // 4x4 seeds of 8256 walks at K = 1000).
+// Seed launches of fewer lanes than this split a task's points over
+// workgroups (a small batch's tasks alone leave most of the device idle).
+constexpr size_t kSeedSplitLanes = (size_t)1 << 20;
 inline int fd_seed_threads(int m, int kp) {
   const int pts = fd_dmax(m) + 1;
   int j = (kp <= 64 ? 64 : 256) / kp;

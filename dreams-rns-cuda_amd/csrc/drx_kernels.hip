@@ -516,12 +516,39 @@ __global__ void __launch_bounds__(256, chunk_waves(M, PERSIST)) k_walk_chunk(con
 // with Delta^j f(sb) = sum_t C(sb, t) N_{j+t} and runs its steps with no
 // interpreter at all.
 
-// One workgroup per prime slice of a walk (a seed task: walk, k0, kn) and
-// group of J = blockDim / KP points (blockIdx.y): lane (g, k) interprets the
-// program at step blockIdx.y J + g for prime k0 + k (KP = `kp` lanes per
-// point) and stores the raw entry values in the walk's table; k_fd_diff then
-// turns each prime's D + 1 values into Newton forward differences in place.
+// Prime k's D + 1 point values of every general entry turned into Newton
+// forward differences in place: all D + 1 points, so the differences above
+// an entry's degree come out as exact zeros, which the full-depth kernel
+// relies on. Only general entries have tables; the walk kernels read no
+// others.
 template <int M>
+__device__ __forceinline__ void fd_differences(double* __restrict__ tab, uint32_t kinds, uint32_t Kw, uint32_t k,
+                                               double p, double pinv) {
+  constexpr int D = fd_dmax(M);
+#pragma unroll
+  for (int e = 0; e < M * M; ++e) {
+    if (((kinds >> (2 * e)) & 3u) != E_GEN) continue;  // uniform
+    double N[D + 1];
+#pragma unroll
+    for (int j = 0; j <= D; ++j) N[j] = tab[((uint64_t)e * (D + 1) + j) * Kw + k];
+#pragma unroll
+    for (int i = 1; i <= D; ++i)
+#pragma unroll
+      for (int j = D; j >= i; --j) N[j] = N[j] - N[j - 1];
+#pragma unroll
+    for (int j = 0; j <= D; ++j) tab[((uint64_t)e * (D + 1) + j) * Kw + k] = red(N[j], p, pinv);
+  }
+}
+
+// One workgroup per prime slice of a walk (a seed task: walk, k0, kn):
+// lane (g, k) = point group g, prime k0 + k, with KP = `kp` lanes per group
+// and J = blockDim / KP groups. It interprets the program at steps g, g + J,
+// ... and stores the raw entry values in the walk's table, then turns each
+// prime's D + 1 values into Newton forward differences in place. With
+// SPLIT, workgroup (task, blockIdx.y) takes the one point group blockIdx.y
+// (steps blockIdx.y J + g) and k_fd_diff takes the differences: for small
+// batches, whose tasks alone would leave most of the device idle.
+template <int M, bool SPLIT>
 __global__ void __launch_bounds__(512) k_fd_seed(const KArgs a, uint32_t kp) {
   constexpr int D = fd_dmax(M);
   extern __shared__ double lds[];
@@ -543,11 +570,11 @@ __global__ void __launch_bounds__(512) k_fd_seed(const KArgs a, uint32_t kp) {
   __syncthreads();
   const uint32_t* code = a.code + pg.code_off;
   double* tab = a.fdt + w.fdt_off + k0;  // column k of the slice is prime k0 + k
-  const uint32_t j = blockIdx.y * J + g;  // this lane group's point
-  // a loop of at most one trip rather than an if: the compiler keeps the
-  // interpreter in 109 VGPRs inside a loop, 178 in straight-line code
+  // (SPLIT: one trip; a loop rather than an if, which the compiler keeps in
+  // 109 VGPRs against 178 in straight-line code)
+  const uint32_t jstep = SPLIT ? (uint32_t)D + 1u : J;
 #pragma unroll 1
-  for (uint32_t once = 0; once < 1u && j <= (uint32_t)D; ++once) {
+  for (uint32_t j = blockIdx.y * J + g; j <= (uint32_t)D; j += jstep) {
     RBank B;
     B.lo = 0u;
     B.hi = 0u;
@@ -563,37 +590,22 @@ __global__ void __launch_bounds__(512) k_fd_seed(const KArgs a, uint32_t kp) {
         if (((pg.kinds >> (2 * e)) & 3u) == E_GEN) tab[((uint64_t)e * (D + 1) + j) * Kw + k] = E[e];
     }
   }
+  if constexpr (SPLIT) return;  // k_fd_diff follows
+  __syncthreads();
+  if (g == 0 && active) fd_differences<M>(tab, pg.kinds, Kw, k, ln.p, ln.pinv);
 }
 
 // The Newton forward differences of each seed task's D + 1 point values, in
-// place, one lane per prime of the slice: all D + 1 points, so the
-// differences above an entry's degree come out as exact zeros, which the
-// full-depth kernel relies on. Only general entries have tables; the walk
-// kernels read no others.
+// place, one lane per prime of the slice (after a split seed launch).
 template <int M>
 __global__ void __launch_bounds__(256) k_fd_diff(const KArgs a) {
-  constexpr int D = fd_dmax(M);
   const DevTask st = a.tasks[blockIdx.x];
   const DevWalk w = a.walks[st.walk];
   const uint32_t kinds = a.progs[w.prog].kinds;
   const uint32_t Kw = (uint32_t)w.K, k0 = (uint32_t)st.k0, K = (uint32_t)st.kn;
   double* tab = a.fdt + w.fdt_off + k0;
-  for (uint32_t k = threadIdx.x; k < K; k += blockDim.x) {
-    const double p = a.prime_p[k0 + k], pinv = a.prime_pinv[k0 + k];
-#pragma unroll
-    for (int e = 0; e < M * M; ++e) {
-      if (((kinds >> (2 * e)) & 3u) != E_GEN) continue;  // uniform
-      double N[D + 1];
-#pragma unroll
-      for (int j = 0; j <= D; ++j) N[j] = tab[((uint64_t)e * (D + 1) + j) * Kw + k];
-#pragma unroll
-      for (int i = 1; i <= D; ++i)
-#pragma unroll
-        for (int j = D; j >= i; --j) N[j] = N[j] - N[j - 1];
-#pragma unroll
-      for (int j = 0; j <= D; ++j) tab[((uint64_t)e * (D + 1) + j) * Kw + k] = red(N[j], p, pinv);
-    }
-  }
+  for (uint32_t k = threadIdx.x; k < K; k += blockDim.x)
+    fd_differences<M>(tab, kinds, Kw, k, a.prime_p[k0 + k], a.prime_pinv[k0 + k]);
 }
 
 // Newton table at step 0 -> at step s0: G_j = sum_{t <= D-j} C(s0, t) N_{j+t},
@@ -2142,20 +2154,27 @@ hipError_t launch_fill_u32(uint32_t* p, int n, uint32_t v, hipStream_t s) {
 hipError_t launch_fd_seed(int m, const KArgs& a, int nseed, int kp, int threads, size_t lds, hipStream_t s) {
   if (nseed <= 0) return hipSuccess;
   if (kp <= 0 || kp > kSliceK || threads % kp != 0 || threads > 512 || m < 2 || m > 4) return hipErrorInvalidValue;
-  // one workgroup per (task, group of threads / kp points): the points of a
-  // task are interpreted side by side rather than in turn; then the
-  // differences, once every point is in
+  // small batches: one workgroup per (task, group of threads / kp points),
+  // the points of a task interpreted side by side rather than in turn, then
+  // the differences once every point is in (pcf2's 4096 seeds: 26 + 7 us
+  // against 44); large ones keep one workgroup per task (the sweep's 33024
+  // seeds: 1.6 ms against 1.7 + 0.4 split)
   const int pts = fd_dmax(m) + 1, per = threads / kp;
-  const dim3 grid(nseed, (pts + per - 1) / per), dblock(kp < 64 ? 64 : kp);
-  if (m == 2) {
-    hipLaunchKernelGGL(k_fd_seed<2>, grid, dim3(threads), lds, s, a, (uint32_t)kp);
-    hipLaunchKernelGGL(k_fd_diff<2>, dim3(nseed), dblock, 0, s, a);
-  } else if (m == 3) {
-    hipLaunchKernelGGL(k_fd_seed<3>, grid, dim3(threads), lds, s, a, (uint32_t)kp);
-    hipLaunchKernelGGL(k_fd_diff<3>, dim3(nseed), dblock, 0, s, a);
+  const bool split = (size_t)nseed * threads < kSeedSplitLanes && per < pts;
+  const dim3 block(threads), dblock(kp < 64 ? 64 : kp);
+  const uint32_t ukp = (uint32_t)kp;
+  if (split) {
+    const dim3 grid(nseed, (pts + per - 1) / per);
+    if (m == 2) hipLaunchKernelGGL((k_fd_seed<2, true>), grid, block, lds, s, a, ukp);
+    else if (m == 3) hipLaunchKernelGGL((k_fd_seed<3, true>), grid, block, lds, s, a, ukp);
+    else hipLaunchKernelGGL((k_fd_seed<4, true>), grid, block, lds, s, a, ukp);
+    if (m == 2) hipLaunchKernelGGL(k_fd_diff<2>, dim3(nseed), dblock, 0, s, a);
+    else if (m == 3) hipLaunchKernelGGL(k_fd_diff<3>, dim3(nseed), dblock, 0, s, a);
+    else hipLaunchKernelGGL(k_fd_diff<4>, dim3(nseed), dblock, 0, s, a);
   } else {
-    hipLaunchKernelGGL(k_fd_seed<4>, grid, dim3(threads), lds, s, a, (uint32_t)kp);
-    hipLaunchKernelGGL(k_fd_diff<4>, dim3(nseed), dblock, 0, s, a);
+    if (m == 2) hipLaunchKernelGGL((k_fd_seed<2, false>), dim3(nseed), block, lds, s, a, ukp);
+    else if (m == 3) hipLaunchKernelGGL((k_fd_seed<3, false>), dim3(nseed), block, lds, s, a, ukp);
+    else hipLaunchKernelGGL((k_fd_seed<4, false>), dim3(nseed), block, lds, s, a, ukp);
   }
   return hipGetLastError();
 }

@@ -279,3 +279,20 @@ def test_served_companion_walks_equal_batched(gpu_ctx, sweep_fix):
         assert r["q_residues"].tolist() == ref[i]["q_residues"].tolist(), i
         for key in ("p_float", "q_float", "log_scale"):
             assert float(r[key]).hex() == float(ref[i][key]).hex(), (i, key)
+
+
+def test_seed_launch_shapes_equal_interpreter(gpu_ctx):
+    # Newton-table seeds split over point groups (small batches) and one
+    # workgroup per seed task (launches of 2^20 lanes and more: 4200 walks x
+    # 256 primes) give the tables the program's steps imply: residues and
+    # CRT bit-exact against the interpreter
+    spec = {"p": 3, "q": 2, "a_params": ["-1/4", "-1/6", "1/6"], "b_params": ["-1/3", "1/4"]}
+    rng = np.random.default_rng(33)
+    progs = [compile_cmf_spec(spec, [int(v) for v in rng.integers(-2, 3, 5)]) for _ in range(8)]
+    for n, K in ((64, 32), (4200, 256)):
+        specs = [cmf_spec(progs[i % 8], 40, K, [int(v) for v in rng.integers(1, 5, 5)]) for i in range(n)]
+        print("seed shapes", n, K, flush=True)
+        a = BatchPlan(specs, crt=True).run()
+        b = BatchPlan(specs, crt=True, interp_only=True).run()
+        assert np.array_equal(a.residues, b.residues), (n, K)
+        assert np.array_equal(a.limbs, b.limbs) and np.array_equal(a.neg, b.neg), (n, K)
