@@ -342,7 +342,8 @@ def with_copy(rsteps: float, copy_ms: list) -> dict:
     ms = float(np.mean(copy_ms))
     return {"ms_per_step": ms, "value": rsteps / (ms * 1e-3), "unit": "residue-steps/s",
             "what": "the same step plus the copy of every output (residues, CRT limbs, FP64 results, top-k) "
-                    "to pinned host memory"}
+                    "to pinned host memory; such a run launches on streams, the residues and limbs copied "
+                    "while the shadow chain still runs"}
 
 
 def batch_rate(name, steps=2, warmup=1, threads=256, chunk=256):

@@ -132,9 +132,10 @@ struct drx_ctx {
 namespace drx {
 
 // Events of the context: [0..4] timing marks of the last timed launch
-// sequence, [6] fork, [7] join back, [10..11] copy calibration, [12..] one per
-// M_f window (the shadow of window c waits for it).
-enum { kEvFork = 6, kEvJoin = 7, kEvWin = 12, kMaxWin = 8 };
+// sequence, [5] early output copies done, [6] fork, [7] join back, [8] combined
+// residues final, [9] CRT limbs final, [10..11] copy calibration, [12..] one
+// per M_f window (the shadow of window c waits for it).
+enum { kEvCopied = 5, kEvFork = 6, kEvJoin = 7, kEvOut = 8, kEvLimbs = 9, kEvWin = 12, kMaxWin = 8 };
 
 // One group of a launch: walks of one matrix order and walk-kernel family.
 struct Stage {
