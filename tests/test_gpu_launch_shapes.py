@@ -143,3 +143,24 @@ def test_graph_replays_equal_stream_launch(gpu_ctx):
                     got = _copy(pg.run())
                 _same(ref, got, rep)
                 ps.run()
+
+
+def test_copy_back_into_pageable_sinks(gpu_ctx, monkeypatch):
+    # page-locked sinks take the outputs as soon as each is final (residues
+    # and CRT limbs beside the shadow chain); pageable ones get every copy
+    # after the join: both give the same outputs, with and without a graph,
+    # also when the sinks are reused run after run
+    from dreams_rns import _native as N
+    cfg = synth.pcf2(n_walks=192, seed=137)
+    specs = [pcf_spec(D.compile_pcf(w["a"], w["b"]), D.pcf_initial_values(w["a"]), 900, 32, 1, target=w["L"])
+             for w in cfg["walks"]]
+    kw = {"crt": True, "top_k": 16, "threads": 64, "chunk_steps": 4096}
+    with BatchPlan(specs, **kw) as pin:
+        ref = _copy(pin.run())
+        monkeypatch.setattr(N, "pinned_zeros", lambda shape, dtype: np.zeros(shape, dtype))
+        for graph in (False, True):
+            print("pageable sinks, graph", graph, flush=True)
+            with BatchPlan(specs, use_graph=graph, **kw) as pg:
+                for rep in range(2):
+                    _same(ref, _copy(pg.run()), ("pageable", graph, rep))
+                    _same(ref, _copy(pin.run()), ("page-locked", graph, rep))
