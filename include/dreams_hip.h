@@ -189,7 +189,10 @@ int drx_walk_batch(drx_ctx* ctx, const drx_batch* b, const drx_launch_cfg* cfg);
  * (inputs become resident in HBM) and, with use_graph, captures the launch
  * sequence; drx_plan_run executes it (re-runnable: every run starts from the
  * walks' initial matrices) and copies the outputs into the drx_batch buffers
- * given at creation, which must stay valid. */
+ * given at creation, which must stay valid. When every such buffer is
+ * page-locked (drx_host_alloc), the residues and CRT limbs are copied as soon
+ * as they are final, while the FP64 shadow still runs, and the run launches on
+ * streams even for a captured plan; the results are the same either way. */
 typedef struct drx_plan drx_plan;
 int drx_plan_create(drx_ctx* ctx, const drx_batch* b, const drx_launch_cfg* cfg, drx_plan** out);
 int drx_plan_run(drx_plan* plan);
