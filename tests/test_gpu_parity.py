@@ -298,7 +298,8 @@ def test_topk_tournament_and_rank_paths(gpu_ctx):
     # tournament rounds, and above the tournament's reach the rank kernel;
     # NaN, +-0 and ties throughout
     rng = np.random.default_rng(21)
-    for n, k in ((1, 1), (5, 5), (256, 64), (257, 64), (300, 7), (2048, 64), (2049, 1), (4096, 64), (5000, 17),
+    for n, k in ((1, 1), (5, 5), (256, 64), (257, 64), (300, 7), (777, 64), (2048, 64), (2049, 1), (4095, 33),
+                 (4096, 64), (4097, 64), (5000, 17),
                  (20000, 33), (100000, 1), (4096, 65), (4096, 1024), (70000, 1024), (300000, 64), (300000, 128),
                  (5000, 1025), (3000, 3000)):
         v = rng.normal(size=n).round(2)          # many exact ties
